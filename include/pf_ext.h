@@ -1,0 +1,30 @@
+/*
+ * pf_ext.h -- entry points of libpixelforge_amd.so with no reference
+ * counterpart, beyond pfFinish/pfFlush/pfSubmitBatch in pixelforge.h.
+ */
+#ifndef PF_EXT_H
+#define PF_EXT_H
+
+#include "pixelforge.h"
+#include "pf_frame.h"
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* Runs a recorded frame; returns 0 on success. */
+typedef int (*PFXexecutor)(const PFFframe* frame, void* user);
+
+/* Replaces the frame executor.  The default (NULL) is the GPU executor,
+ * pfdExecuteFrame.  A test harness installs its own to inspect the
+ * recorded frames; the library never installs one by itself. */
+PF_API void pfxSetExecutor(PFXexecutor fn, void* user);
+
+/* Number of frames the current context has executed so far. */
+PF_API PFuint pfxFrameCount(void);
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif /* PF_EXT_H */

@@ -1,0 +1,114 @@
+/*
+ * pf_frame.h -- the recorded frame: what the host library hands to the GPU.
+ *
+ * The reference rasterises each triangle synchronously inside pfVertex*
+ * (context.c:1671-1674 -> internal/context/context.c:94-244).  This library
+ * instead records, per render target, a frame made of
+ *   - triangle streams: three vertices per recorded triangle (object-space
+ *     position, RGBA8 colour, texture coordinate after the texture matrix,
+ *     normal after PF_NORMALIZE), plus one word per triangle naming its
+ *     state snapshot and which culling passes it runs (FRONT, BACK);
+ *   - state snapshots: everything that pfiProcessRasterize_TRIANGLE_IMPL and
+ *     Rasterize_Triangle read from the context (triangles.c:62-116, 287-558);
+ *   - textures referenced by the snapshots;
+ *   - an ordered command list of CLEAR and DRAW(range) commands.
+ * All types are plain C structs; every pointer is a host pointer.
+ */
+#ifndef PF_FRAME_H
+#define PF_FRAME_H
+
+#include <stdint.h>
+
+#define PFF_MAX_LIGHTS 8            /* config.h:37-39 */
+
+/* Bits of PFFstate.flags; same values as PFstate in pixelforge.h. */
+#define PFF_TEXTURE_2D   0x0001u
+#define PFF_BLEND        0x0004u
+#define PFF_DEPTH_TEST   0x0008u
+#define PFF_LIGHTING     0x0040u    /* lighting enabled and >= 1 active light */
+
+/* Bits of the per-triangle word (PFFtriangles.info). */
+#define PFF_STATE_MASK   0x00FFFFFFu
+#define PFF_PASS_FRONT   0x01000000u /* run the FRONT pass (triangles.c:303-305) */
+#define PFF_PASS_BACK    0x02000000u /* run the BACK pass; after FRONT if both  */
+
+/* Texture pixel formats accepted by the pipeline. */
+#define PFF_TEX_RGBA8    0u         /* pixel.h:696-700, 2909-2913 */
+#define PFF_TEX_RGBAF32  1u         /* pixel.h:735-746, 2983-3008 */
+
+typedef struct {
+    float    position[3];
+    float    direction[3];
+    float    innerCutOff, outerCutOff;      /* cosines; >= pi disables the spot */
+    float    attConstant, attLinear, attQuadratic;
+    uint32_t ambient, diffuse, specular;    /* packed RGBA8, r in the low byte */
+    uint32_t pad[3];
+} PFFlight;                                 /* 80 bytes; context.h:216-229 */
+
+typedef struct {
+    uint32_t ambient, diffuse, specular, emission; /* packed RGBA8 */
+    float    shininess;
+    uint32_t pad[3];
+} PFFmaterial;                              /* 32 bytes; context.h:234-240 */
+
+typedef struct {
+    float    mvp[16];          /* column-major, as pfmVec4Transform reads it  */
+    float    matNormal[16];    /* transpose(inverse(model)) or identity       */
+    float    viewRow2[4];      /* matView[8..10]: Gouraud N.V test (triangles.c:96) */
+    float    viewPos[4];       /* translation of inverse(matView) (triangles.c:84-86) */
+    int32_t  vpPos[2];         /* context.h:340 */
+    uint32_t vpDim[2];         /* viewport size minus one */
+    int32_t  vpMin[2];
+    int32_t  vpMax[2];
+    uint32_t flags;            /* PFF_* state bits */
+    uint32_t depthFunc;        /* PFdepthmode */
+    uint32_t blendMode;        /* PFblendmode */
+    uint32_t shadeModel;       /* PFshademode */
+    uint32_t lightMode;        /* PFlightmode */
+    int32_t  texture;          /* index into PFFframe.textures, -1 for none   */
+    uint32_t numLights;        /* active lights in enable order               */
+    uint32_t pad0;
+    PFFmaterial material[2];   /* [PF_FRONT], [PF_BACK] */
+    PFFlight    lights[PFF_MAX_LIGHTS];
+} PFFstate;
+
+typedef struct {
+    const void* pixels;        /* host memory, read at execution time         */
+    uint32_t    width, height;
+    uint32_t    format;        /* PFF_TEX_* */
+    uint32_t    wrap;          /* PFtexturewrap */
+    uint32_t    filter;        /* PFtexturefilter */
+    uint32_t    pad;
+} PFFtexture;
+
+#define PFF_CMD_CLEAR 1u
+#define PFF_CMD_DRAW  2u
+
+typedef struct {
+    uint32_t kind;             /* PFF_CMD_* */
+    uint32_t first, last;      /* DRAW: triangle range [first, last)          */
+    uint32_t clearColor;       /* CLEAR: packed RGBA8                         */
+    float    clearDepth;       /* CLEAR */
+    uint32_t pad[3];
+} PFFcmd;
+
+typedef struct {
+    uint32_t        count;
+    const float*    position;  /* [count*3][4] object space, w included      */
+    const uint32_t* color;     /* [count*3] packed RGBA8                      */
+    const float*    texcoord;  /* [count*3][2]; NULL when no state textures   */
+    const float*    normal;    /* [count*3][3]; NULL when no state lights     */
+    const uint32_t* info;      /* [count] state index | PFF_PASS_* bits       */
+} PFFtriangles;
+
+typedef struct {
+    uint32_t           width, height;    /* render target, RGBA8 + float32 depth */
+    uint32_t*          color;            /* host buffers of the render target    */
+    float*             depth;
+    PFFtriangles       tris;
+    const PFFstate*    states;   uint32_t numStates;
+    const PFFtexture*  textures; uint32_t numTextures;
+    const PFFcmd*      cmds;     uint32_t numCmds;
+} PFFframe;
+
+#endif /* PF_FRAME_H */

@@ -1,0 +1,840 @@
+/*
+ * pf_oracle.c -- CPU oracle for the PixelForge triangle fragment pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
+ * compared against; it is never linked into, loaded by or called from the
+ * product library.  Only tests/, __graft_entry__.smoke() and the CPU
+ * baseline leg of bench.py may use it.
+ *
+ * It restates, pixel by pixel and in plain scalar C, the algorithm of the
+ * reference's SIMD (AVX2) build -- the build whose numerics define parity --
+ * and renders a recorded frame (include/pf_frame.h) into its host buffers.
+ * Every function cites the reference lines it follows.  Paths below are
+ * relative to the reference's src/ directory; tri.c = internal/primitives/
+ * triangles.c, ictx.c = internal/context/context.c.
+ *
+ * Deliberate, documented differences from the reference binary:
+ *   D1  rcpps / rsqrtps (simd.h:1217-1245) are CPU-model specific ~12-bit
+ *       approximations; they are reproduced bit for bit as measured on the
+ *       Intel Xeon that generated the golden frames (a reference run on
+ *       another CPU model may differ in depth and in the texels picked).
+ *   D2  BILINEAR filtering follows the intended colour lerp of
+ *       color.h:137-144 (the reference reads an uninitialised vector there,
+ *       so its output is undefined).
+ *   D3  Reads of a texel offset outside [0, w*h) -- possible in the
+ *       reference for u or v at exactly 1.0 in MIRRORED/CLAMP modes with an
+ *       even texture size -- are clamped to the buffer instead of reading
+ *       past it.
+ *   D4  The glibc powf of the Gouraud path (lighting.c:115) is evaluated as
+ *       (float)pow((double)x, (double)y), i.e. the correctly rounded powf
+ *       in all but vanishingly rare cases.
+ *   D5  Pixels outside the render target (possible in the reference with a
+ *       viewport larger than, or offset from, the framebuffer in 3D, where
+ *       the bounding box is not clamped) are skipped instead of written out
+ *       of bounds.
+ * Parity of this oracle against the compiled reference is pinned by
+ * tests/test_oracle_golden.py (golden frames) and tests/test_oracle_units.py
+ * (closed forms and invariants).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#include "pf_frame.h"
+#include "pf_x86_approx.h"
+
+#define PF_PI_D 3.14159265358979323846   /* PFM_PI, pfm.h */
+#define CLIP_EPS 1e-5f                    /* config.h:50 */
+#define MAX_POLY 12                       /* config.h:42 */
+
+/* ------------------------------------------------------------------------ */
+/* x86 numeric semantics                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* C (int) cast of a float as gcc emits it on x86-64 (cvttss2si): NaN and
+ * out-of-range values give INT32_MIN. */
+static int32_t cvtt_i32(float f)
+{
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+
+/* _mm256_cvtps_epi32 under the default MXCSR (round to nearest even);
+ * NaN and out-of-range give INT32_MIN (simd.h:921-929). */
+static int32_t cvt_rne_i32(float f)
+{
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)nearbyintf(f);
+}
+
+/* (PFubyte)(float): gcc converts through a 32-bit signed int and keeps the
+ * low byte. */
+static uint8_t f2u8(float f) { return (uint8_t)(uint32_t)cvtt_i32(f); }
+
+/* minps / maxps return the second operand when either is NaN. */
+static float mn_ps(float a, float b) { return a < b ? a : b; }
+static float mx_ps(float a, float b) { return a > b ? a : b; }
+static float clamp_ps(float x, float lo, float hi) { return mn_ps(mx_ps(x, lo), hi); } /* simd.h:1086-1093 */
+
+static uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* _mm256_rcp_ps and _mm256_rsqrt_ps (simd.h:1217-1245): the CPU's table
+ * approximations, reproduced from tables measured on Intel Xeon
+ * (include/pf_x86_approx.h; model in pixelforge_amd/csrc/gen_x86_approx.c). */
+static float rcp_(float xf)
+{
+    uint32_t x = fbits(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+    if (E == 0xffu) return bitsf(m ? (x | 0x00400000u) : s);
+    if (E == 0) return bitsf(s | 0x7f800000u);
+    int oe = 253 - (int)E;
+    if (oe <= 0) return bitsf(s);
+    return bitsf(s | ((uint32_t)oe << 23) | ((uint32_t)pf_rcp_table[m >> 12] << 11));
+}
+
+static float rsqrt_(float xf)
+{
+    uint32_t x = fbits(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+    if (E == 0xffu) return bitsf(m ? (x | 0x00400000u) : (s ? 0xffc00000u : 0u));
+    if (E == 0) return bitsf(s | 0x7f800000u);
+    if (s) return bitsf(0xffc00000u);
+    int odd = (int)(E & 1u);
+    int oe = odd ? 126 - ((int)E - 127) / 2 : 127 - ((int)E - 126) / 2;
+    return bitsf(((uint32_t)oe << 23) | ((uint32_t)pf_rsqrt_table[odd][m >> 13] << 11));
+}
+
+static int ch(uint32_t c, int i) { return (int)((c >> (8 * i)) & 0xFFu); }
+
+/* PF_MIN_255 (pixelforge.h:96-98) on an int, returning the byte. */
+static uint8_t min255(int n) { return (uint8_t)(n | ((255 - n) >> 31)); }
+
+/* ------------------------------------------------------------------------ */
+/* Cephes single-precision log/exp (S. Moshier's Cephes library, as          */
+/* vectorised in J. Pommier's sse_mathfun); the reference's pow is           */
+/* exp(log(x)*e) built on them (simd.h:183-304, 1157-1169).                  */
+/* ------------------------------------------------------------------------ */
+
+static float cephes_logf(float x)
+{
+    const float one = 1.0f;
+    uint32_t invalid = (x <= 0.0f) ? 0xFFFFFFFFu : 0u;      /* NaN -> not invalid */
+    x = mx_ps(x, bitsf(0x00800000u));                        /* smallest normal */
+    int32_t e_i = (int32_t)(fbits(x) >> 23);
+    x = bitsf((fbits(x) & ~0x7f800000u) | fbits(0.5f));
+    e_i -= 0x7f;
+    float e = (float)e_i;
+    e = e + one;
+    int lt = x < 0.707106781186547524f;
+    float tmp = lt ? x : 0.0f;
+    x = x - one;
+    e = e - (lt ? one : 0.0f);
+    x = x + tmp;
+    float z = x * x;
+    float y = 7.0376836292E-2f;
+    y = y * x; y = y + -1.1514610310E-1f;
+    y = y * x; y = y + 1.1676998740E-1f;
+    y = y * x; y = y + -1.2420140846E-1f;
+    y = y * x; y = y + 1.4249322787E-1f;
+    y = y * x; y = y + -1.6668057665E-1f;
+    y = y * x; y = y + 2.0000714765E-1f;
+    y = y * x; y = y + -2.4999993993E-1f;
+    y = y * x; y = y + 3.3333331174E-1f;
+    y = y * x;
+    y = y * z;
+    tmp = e * -2.12194440e-4f;
+    y = y + tmp;
+    tmp = z * 0.5f;
+    y = y - tmp;
+    tmp = e * 0.693359375f;
+    x = x + y;
+    x = x + tmp;
+    return bitsf(fbits(x) | invalid);
+}
+
+static float cephes_expf(float x)
+{
+    const float one = 1.0f;
+    x = mn_ps(x, 88.3762626647949f);
+    x = mx_ps(x, -88.3762626647949f);
+    float fx = x * 1.44269504088896341f;
+    fx = fx + 0.5f;
+    float tmp = floorf(fx);
+    float mask = (tmp > fx) ? one : 0.0f;
+    fx = tmp - mask;
+    tmp = fx * 0.693359375f;
+    float z = fx * -2.12194440e-4f;
+    x = x - tmp;
+    x = x - z;
+    z = x * x;
+    float y = 1.9875691500E-4f;
+    y = y * x; y = y + 1.3981999507E-3f;
+    y = y * x; y = y + 8.3334519073E-3f;
+    y = y * x; y = y + 4.1665795894E-2f;
+    y = y * x; y = y + 1.6666665459E-1f;
+    y = y * x; y = y + 5.0000001201E-1f;
+    y = y * z;
+    y = y + x;
+    y = y + one;
+    int32_t n = cvtt_i32(fx) + 0x7f;
+    float pow2n = bitsf((uint32_t)n << 23);
+    return y * pow2n;
+}
+
+/* pfiSimdPow_F32 (simd.h:1157-1169): pow(0, e) is a large finite number. */
+float pforacle_pow(float base, float exponent)
+{
+    return cephes_expf(cephes_logf(base) * exponent);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Colour helpers                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* pfiColorSIMDFromVecI_simd (color.h:98-108): OR of channel << 8i, so a
+ * channel above 255 carries into the next one. */
+static uint32_t pack_i(const int32_t* v, int n)
+{
+    uint32_t p = 0;
+    for (int i = 0; i < n; i++) p |= (uint32_t)v[i] << (8 * i);
+    return p;
+}
+
+/* pfiColorSIMDFromVecF_simd (color.h:110-121). */
+static uint32_t pack_f(const float* v, int n)
+{
+    uint32_t p = 0;
+    for (int i = 0; i < n; i++) {
+        float c = clamp_ps(v[i], 0.0f, 1.0f) * 255.0f;
+        p |= (uint32_t)cvt_rne_i32(c) << (8 * i);
+    }
+    return p;
+}
+
+/* pfiColorBarySmooth_simd (color.h:153-181). */
+uint32_t pforacle_bary_smooth(uint32_t c1, uint32_t c2, uint32_t c3,
+                              float b1, float b2, float b3)
+{
+    int32_t u1 = cvt_rne_i32(b1 * 255.0f);
+    int32_t u2 = cvt_rne_i32(b2 * 255.0f);
+    int32_t u3 = cvt_rne_i32(b3 * 255.0f);
+    int32_t r[4];
+    for (int i = 0; i < 4; i++) {
+        uint32_t s = (uint32_t)u1 * (uint32_t)ch(c1, i)
+                   + (uint32_t)u2 * (uint32_t)ch(c2, i);
+        s = s + (uint32_t)u3 * (uint32_t)ch(c3, i);
+        r[i] = (int32_t)((s * 257u) >> 16);
+    }
+    return pack_i(r, 4);
+}
+
+/* pfiColorBaryFlat_simd (color.h:183-203). */
+static uint32_t bary_flat(uint32_t c1, uint32_t c2, uint32_t c3, float b1, float b2, float b3)
+{
+    float m = mx_ps(b1, mx_ps(b2, b3));
+    return ((m == b1) ? c1 : 0u) | (((m == b2) ? c2 : 0u) | ((m == b3) ? c3 : 0u));
+}
+
+/* pfiBlendMultiplicative (blend.h:76-85), the scalar form used on vertices. */
+static uint32_t mul_scalar(uint32_t s, uint32_t d)
+{
+    int32_t r[4];
+    for (int i = 0; i < 4; i++) r[i] = (ch(s, i) * ch(d, i)) / 255;
+    return pack_i(r, 4);
+}
+
+/* GC_blendFuncs_simd (blend.h:137-274), on 0..255 channel values. */
+uint32_t pforacle_blend(uint32_t mode, uint32_t s, uint32_t d)
+{
+    int32_t o[4];
+    int sa = ch(s, 3) + 1;
+    for (int i = 0; i < 4; i++) {
+        int sc = ch(s, i), dc = ch(d, i);
+        switch (mode) {
+        case 0: o[i] = (sc + dc) >> 1; break;                              /* AVERAGE */
+        case 1: o[i] = ((i == 3 ? 255 : sc) * sa + dc * (256 - sa)) >> 8;   /* ALPHA   */
+                break;
+        case 2: o[i] = (sc + dc) < 255 ? (sc + dc) : 255; break;           /* ADD     */
+        case 3: o[i] = (sc + dc) > 0 ? (sc + dc) : 0; break;               /* SUB: adds (blend.h:191-194) */
+        case 4: o[i] = (sc * dc) >> 8; break;                              /* MUL     */
+        case 5: { int t = ((dc * (255 - sc)) >> 8) + sc;                   /* SCREEN  */
+                  o[i] = t < 255 ? t : 255; } break;
+        case 6: o[i] = sc > dc ? sc : dc; break;                           /* LIGHTEN */
+        default: o[i] = sc < dc ? sc : dc; break;                          /* DARKEN  */
+        }
+    }
+    return pack_i(o, 4);
+}
+
+/* pfiBlendMultiplicative_simd (blend.h:199-212): texel modulation. */
+static uint32_t modulate(uint32_t texel, uint32_t frag) { return pforacle_blend(4, texel, frag); }
+
+/* ------------------------------------------------------------------------ */
+/* Texture sampling (sampler.h:202-410)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* RGBA8 gather (pixel.h:2909-2913) and RGBA float gather with cvt_rne(f*255)
+ * and no clamp (pixel.h:2983-3008). */
+static uint32_t texel_at(const PFFtexture* t, int32_t off)
+{
+    int64_t n = (int64_t)t->width * t->height;
+    int64_t o = off;
+    if (o < 0) o = 0;                       /* D3 */
+    if (o >= n) o = n - 1;
+    if (t->format == PFF_TEX_RGBA8) return ((const uint32_t*)t->pixels)[o];
+    const float* p = (const float*)t->pixels + 4 * o;
+    int32_t r = cvt_rne_i32(p[0] * 255.0f), g = cvt_rne_i32(p[1] * 255.0f);
+    int32_t b = cvt_rne_i32(p[2] * 255.0f), a = cvt_rne_i32(p[3] * 255.0f);
+    return ((uint32_t)a << 24) | ((uint32_t)b << 16) | ((uint32_t)g << 8) | (uint32_t)r;
+}
+
+/* Wrap maps: REPEAT sampler.h:202-218, MIRRORED sampler.h:220-240,
+ * CLAMP_TO_EDGE sampler.h:242-255. */
+static int32_t wrap_coord(uint32_t wrap, float u, uint32_t size)
+{
+    float tw = (float)(size - 1);
+    if (wrap == 0) {
+        float f = (u - truncf(u)) * tw;
+        int32_t x = cvt_rne_i32(f);
+        return x < 0 ? (int32_t)(0u - (uint32_t)x) : x;   /* _mm256_abs_epi32 */
+    }
+    if (wrap == 1) {
+        float a = fabsf(u);
+        float q = a / 2.0f;                                  /* pfiSimdMod_F32, simd.h:1182 */
+        float m = a - floorf(q) * 2.0f;
+        float mirror = 1.0f - (m - 1.0f);
+        if (m > 1.0f) m = mirror;
+        return cvt_rne_i32(m * tw + 0.5f);
+    }
+    float c = clamp_ps(u, 0.0f, 1.0f);
+    return cvt_rne_i32(c * tw + 0.5f);
+}
+
+/* pfiColorLerpSmooth_simd as intended (color.h:137-144; D2). */
+static uint32_t lerp_color(uint32_t a, uint32_t b, float t)
+{
+    const float inv255 = 1.0f / 255.0f;
+    float r[4];
+    for (int i = 0; i < 4; i++) {
+        float fa = (float)ch(a, i) * inv255;
+        float fb = (float)ch(b, i) * inv255;
+        r[i] = fa + t * (fb - fa);                     /* pfiVec4LerpR_simd, simd.h:3147 */
+    }
+    return pack_f(r, 4);
+}
+
+uint32_t pforacle_sample(const PFFtexture* t, float u, float v)
+{
+    int32_t w = (int32_t)t->width;
+    if (t->filter == 0) {                                       /* sampler.h:259-296 */
+        int32_t x = wrap_coord(t->wrap, u, t->width);
+        int32_t y = wrap_coord(t->wrap, v, t->height);
+        return texel_at(t, (int32_t)((uint32_t)y * (uint32_t)w + (uint32_t)x));
+    }
+    /* sampler.h:298-334 (and the two twins that follow it) */
+    float tx = 1.0f / (float)t->width, ty = 1.0f / (float)t->height;  /* texture.c:52-53 */
+    int32_t x0 = wrap_coord(t->wrap, u, t->width);
+    int32_t y0 = wrap_coord(t->wrap, v, t->height);
+    int32_t x1 = wrap_coord(t->wrap, u + tx, t->width);
+    int32_t y1 = wrap_coord(t->wrap, v + ty, t->height);
+    float fx = u * (float)t->width - (float)x0;
+    float fy = v * (float)t->height - (float)y0;
+    fx = clamp_ps(fx, 0.0f, 1.0f);
+    fy = clamp_ps(fy, 0.0f, 1.0f);
+    uint32_t c00 = texel_at(t, (int32_t)((uint32_t)y0 * (uint32_t)w + (uint32_t)x0));
+    uint32_t c10 = texel_at(t, (int32_t)((uint32_t)y0 * (uint32_t)w + (uint32_t)x1));
+    uint32_t c01 = texel_at(t, (int32_t)((uint32_t)y1 * (uint32_t)w + (uint32_t)x0));
+    uint32_t c11 = texel_at(t, (int32_t)((uint32_t)y1 * (uint32_t)w + (uint32_t)x1));
+    uint32_t c0 = lerp_color(c00, c10, fx);
+    uint32_t c1 = lerp_color(c01, c11, fx);
+    return lerp_color(c0, c1, fy);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Lighting                                                                  */
+/* ------------------------------------------------------------------------ */
+
+static float dot3(const float* a, const float* b)        /* pfmVec3Dot, pfm.h:1227 */
+{
+    float d = 0.0f;
+    d += a[0] * b[0];
+    d += a[1] * b[1];
+    d += a[2] * b[2];
+    return d;
+}
+
+/* pfiLightingProcess (lighting.c:23-144): scalar Blinn-Phong per vertex. */
+static uint32_t light_vertex(const PFFstate* st, const PFFmaterial* mat, uint32_t diffuse,
+                             const float* viewPos, const float* fragPos, const float* N)
+{
+    uint8_t R = (uint8_t)ch(mat->emission, 0);
+    uint8_t G = (uint8_t)ch(mat->emission, 1);
+    uint8_t B = (uint8_t)ch(mat->emission, 2);
+    uint8_t aC[3];
+    for (int i = 0; i < 3; i++) aC[i] = (uint8_t)((ch(mat->ambient, i) * ch(diffuse, i)) / 255);
+
+    float V[3];                                            /* pfmVec3DirectionR, pfm.h:1322 */
+    float lsq = 0.0f;
+    for (int i = 0; i < 3; i++) { V[i] = viewPos[i] - fragPos[i]; lsq += V[i] * V[i]; }
+    float inv = 1.0f / sqrtf(lsq);
+    for (int i = 0; i < 3; i++) V[i] = V[i] * inv;
+
+    for (uint32_t li = 0; li < st->numLights; li++) {
+        const PFFlight* L_ = &st->lights[li];
+        uint8_t lC[3] = {0, 0, 0};
+        float L[3];
+        for (int i = 0; i < 3; i++) L[i] = L_->position[i] - fragPos[i];
+        float distSq = L[0] * L[0] + L[1] * L[1] + L[2] * L[2];
+        float dist = 0.0f;
+        if (distSq != 0.0f) {
+            dist = sqrtf(distSq);
+            float invMag = 1.0f / dist;
+            for (int i = 0; i < 3; i++) L[i] *= invMag;
+        }
+        uint8_t intensity = 255;
+        if ((double)L_->innerCutOff < PF_PI_D) {
+            float nd[3] = { -L_->direction[0], -L_->direction[1], -L_->direction[2] };
+            float theta = dot3(L, nd);
+            float eps = L_->innerCutOff - L_->outerCutOff;
+            int32_t iv = cvtt_i32(255.0f * (theta - L_->outerCutOff) / eps);
+            intensity = (uint8_t)(iv < 0 ? 0 : (iv > 255 ? 255 : iv));
+            if (intensity == 0) goto apply;
+        }
+        uint8_t att = 255;
+        if (L_->attLinear != 0.0f || L_->attQuadratic != 0.0f) {
+            att = f2u8(255.0f / (L_->attConstant + L_->attLinear * dist + L_->attQuadratic * distSq));
+            if (att == 0) goto apply;
+        }
+        {
+            uint8_t factor = (uint8_t)((intensity * att) / 255);
+            int32_t dv = cvtt_i32(255.0f * dot3(N, L));
+            uint8_t diff = (uint8_t)(dv > 0 ? dv : 0);
+            for (int i = 0; i < 3; i++)
+                lC[i] = min255(lC[i] + (ch(diffuse, i) * ch(L_->diffuse, i) * diff) / (255 * 255));
+            float H[3];
+            for (int i = 0; i < 3; i++) H[i] = L[i] + V[i];
+            float hs = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];   /* pfmVec3Normalize, pfm.h:1183 */
+            if (hs != 0.0f) {
+                float hi = 1.0f / sqrtf(hs);
+                for (int i = 0; i < 3; i++) H[i] = H[i] * hi;
+            }
+            float nh = fmaxf(dot3(N, H), 0.0f);
+            uint8_t spec = f2u8(255.0f * (float)pow((double)nh, (double)mat->shininess)); /* D4 */
+            for (int i = 0; i < 3; i++)
+                lC[i] = min255(lC[i] + (ch(mat->specular, i) * ch(L_->specular, i) * spec) / (255 * 255));
+            for (int i = 0; i < 3; i++) lC[i] = (uint8_t)((lC[i] * factor) / 255);
+        }
+    apply:
+        R = min255(R + lC[0] + (aC[0] * ch(L_->ambient, 0)) / 255);
+        G = min255(G + lC[1] + (aC[1] * ch(L_->ambient, 1)) / 255);
+        B = min255(B + lC[2] + (aC[2] * ch(L_->ambient, 2)) / 255);
+    }
+    return (uint32_t)R | ((uint32_t)G << 8) | ((uint32_t)B << 16) | (diffuse & 0xFF000000u);
+}
+
+/* pfiVec3DirectionR_simd (simd.h:2469-2490): epsilon-guarded normalisation. */
+static void direction_simd(float* d, const float* a, const float* b)
+{
+    for (int i = 0; i < 3; i++) d[i] = a[i] - b[i];
+    float l = d[0] * d[0] + d[1] * d[1];
+    l = l + d[2] * d[2];
+    l = mx_ps(l, 1e-5f);
+    float inv = rsqrt_(l);
+    for (int i = 0; i < 3; i++) d[i] = d[i] * inv;
+}
+
+static float dot3_simd(const float* a, const float* b)  /* simd.h:2356-2363 */
+{
+    float d = a[0] * b[0] + a[1] * b[1];
+    return d + a[2] * b[2];
+}
+
+/* pfiSimdLightingProcess (lighting.c:148-258): per-fragment Blinn-Phong.
+ * Alpha of the result is 0 and emission is ignored, as in the reference. */
+static uint32_t light_fragment(const PFFstate* st, const PFFmaterial* mat, uint32_t frag,
+                               const float* P, const float* N)
+{
+    const float inv255 = 1.0f / 255.0f;
+    float D[3], A[3], S[3];
+    for (int i = 0; i < 3; i++) {
+        D[i] = (float)ch(frag, i) * inv255;
+        A[i] = (float)ch(mat->ambient, i) * inv255;
+        S[i] = (float)ch(mat->specular, i) * inv255;
+        A[i] = A[i] * D[i];
+    }
+    float V[3];
+    direction_simd(V, st->viewPos, P);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t li = 0; li < st->numLights; li++) {
+        const PFFlight* Lt = &st->lights[li];
+        float la[3], ld[3], ls[3];
+        for (int i = 0; i < 3; i++) {
+            la[i] = (float)ch(Lt->ambient, i) * inv255;
+            ld[i] = (float)ch(Lt->diffuse, i) * inv255;
+            ls[i] = (float)ch(Lt->specular, i) * inv255;
+        }
+        float L[3];
+        direction_simd(L, Lt->position, P);
+        float amb[3], dif[3], spc[3];
+        for (int i = 0; i < 3; i++) amb[i] = la[i] * A[i];
+        float diff = mx_ps(dot3_simd(N, L), 0.0f);
+        for (int i = 0; i < 3; i++) { dif[i] = ld[i] * diff; dif[i] = dif[i] * D[i]; }
+        float H[3];
+        for (int i = 0; i < 3; i++) H[i] = L[i] + V[i];
+        float hs = H[0] * H[0] + H[1] * H[1];                 /* pfiVec3Normalize_simd, simd.h:2292 */
+        hs = hs + H[2] * H[2];
+        float hinv = rsqrt_(hs);
+        for (int i = 0; i < 3; i++) H[i] = H[i] * hinv;
+        float spec = mx_ps(dot3_simd(N, H), 0.0f);
+        spec = pforacle_pow(spec, mat->shininess);
+        for (int i = 0; i < 3; i++) { spc[i] = ls[i] * spec; spc[i] = spc[i] * S[i]; }
+        if ((double)Lt->innerCutOff < PF_PI_D) {
+            float nd[3] = { 0.0f - Lt->direction[0], 0.0f - Lt->direction[1], 0.0f - Lt->direction[2] };
+            float theta = dot3_simd(L, nd);
+            float eps = Lt->innerCutOff - Lt->outerCutOff;
+            float I = (theta - Lt->outerCutOff) / eps;
+            I = clamp_ps(I, 0.0f, 1.0f);
+            for (int i = 0; i < 3; i++) { dif[i] = dif[i] * I; spc[i] = spc[i] * I; }
+        }
+        if (Lt->attLinear != 0.0f || Lt->attQuadratic != 0.0f) {
+            /* pfiVec3DistanceSq_simd (simd.h:2424-2442) uses dy twice and no dz */
+            float d0 = Lt->position[0] - P[0];
+            float d1 = Lt->position[1] - P[1];
+            float d2 = Lt->position[1] - P[1];
+            float dsq = d0 * d0 + (d1 * d1 + d2 * d2);
+            float dist = sqrtf(dsq);
+            float att = rcp_(Lt->attConstant + (Lt->attLinear * dist + Lt->attQuadratic * dsq));
+            for (int i = 0; i < 3; i++) { amb[i] *= att; dif[i] *= att; spc[i] *= att; }
+        }
+        for (int i = 0; i < 3; i++) {
+            acc[i] = acc[i] + amb[i];
+            acc[i] = acc[i] + dif[i];
+            acc[i] = acc[i] + spc[i];
+        }
+    }
+    return pack_f(acc, 3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Vertex processing                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float hom[4];
+    float screen[2];
+    float pos[4];
+    float normal[3];
+    float uv[2];
+    uint32_t color;
+} Vtx;                                                     /* PFIvertex, context.h:203-210 */
+
+/* pfiLerpVertex (ictx.c:67-90). */
+static Vtx lerp_vtx(const Vtx* s, const Vtx* e, float t)
+{
+    Vtx r;
+    memset(&r, 0, sizeof r);
+    uint8_t uT = f2u8(255.0f * t);
+    int32_t col[4];
+    for (int i = 0; i < 4; i++) {
+        r.hom[i] = s->hom[i] + t * (e->hom[i] - s->hom[i]);
+        r.pos[i] = s->pos[i] + t * (e->pos[i] - s->pos[i]);
+        col[i] = (uint8_t)(ch(s->color, i) + (uT * (ch(e->color, i) - ch(s->color, i))) / 255);
+        if (i < 2) r.uv[i] = s->uv[i] + t * (e->uv[i] - s->uv[i]);
+        if (i < 3) r.normal[i] = s->normal[i] + t * (e->normal[i] - s->normal[i]);
+    }
+    r.color = pack_i(col, 4);
+    return r;
+}
+
+/* pfiHomogeneousToScreen (ictx.c:51-65). */
+static void to_screen(const PFFstate* st, Vtx* v)
+{
+    v->screen[0] = ((float)st->vpPos[0] + ((v->hom[0] + 1.0f) * 0.5f) * (float)st->vpDim[0]) + 0.5f;
+    v->screen[1] = ((float)st->vpPos[1] + ((1.0f - v->hom[1]) * 0.5f) * (float)st->vpDim[1]) + 0.5f;
+}
+
+/* Process_ClipPolygonW (tri.c:157-182). */
+static int clip_w(Vtx* poly, int* n)
+{
+    Vtx in[MAX_POLY];
+    memcpy(in, poly, (size_t)*n * sizeof(Vtx));
+    int cnt = *n;
+    *n = 0;
+    const Vtx* prev = &in[cnt - 1];
+    int pd = (prev->hom[3] < CLIP_EPS) ? -1 : 1;
+    for (int i = 0; i < cnt; i++) {
+        int cd = (in[i].hom[3] < CLIP_EPS) ? -1 : 1;
+        if (pd * cd < 0)
+            poly[(*n)++] = lerp_vtx(prev, &in[i], (CLIP_EPS - prev->hom[3]) / (in[i].hom[3] - prev->hom[3]));
+        if (cd > 0) poly[(*n)++] = in[i];
+        pd = cd;
+        prev = &in[i];
+    }
+    return *n > 0;
+}
+
+/* Process_ClipPolygonXYZ (tri.c:184-244). */
+static int clip_xyz(Vtx* poly, int* n)
+{
+    for (int ax = 0; ax < 3; ax++) {
+        for (int side = 0; side < 2; side++) {
+            if (*n == 0) return 0;
+            Vtx in[MAX_POLY];
+            memcpy(in, poly, (size_t)*n * sizeof(Vtx));
+            int cnt = *n;
+            *n = 0;
+            const Vtx* prev = &in[cnt - 1];
+            float pa = side ? -prev->hom[ax] : prev->hom[ax];
+            int pd = (pa <= prev->hom[3]) ? 1 : -1;
+            for (int i = 0; i < cnt; i++) {
+                float ca = side ? -in[i].hom[ax] : in[i].hom[ax];
+                int cd = (ca <= in[i].hom[3]) ? 1 : -1;
+                if (pd * cd <= 0) {
+                    float num, den;
+                    if (!side) {
+                        num = prev->hom[3] - prev->hom[ax];
+                        den = num - (in[i].hom[3] - in[i].hom[ax]);
+                    } else {
+                        num = prev->hom[3] + prev->hom[ax];
+                        den = num - (in[i].hom[3] + in[i].hom[ax]);
+                    }
+                    poly[(*n)++] = lerp_vtx(prev, &in[i], num / den);
+                }
+                if (cd > 0) poly[(*n)++] = in[i];
+                pd = cd;
+                prev = &in[i];
+            }
+        }
+    }
+    return *n > 0;
+}
+
+/* Process_ProjectAndClipTriangle (tri.c:246-280); returns is3D. */
+static int project_clip(const PFFstate* st, Vtx* poly, int* n)
+{
+    const float* m = st->mvp;
+    float wsum = 0.0f;
+    for (int i = 0; i < *n; i++) {
+        const float* p = poly[i].pos;                          /* pfmVec4Transform, pfm.h:1996 */
+        float h0 = m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12] * p[3];
+        float h1 = m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13] * p[3];
+        float h2 = m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14] * p[3];
+        float h3 = m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15] * p[3];
+        poly[i].hom[0] = h0; poly[i].hom[1] = h1; poly[i].hom[2] = h2; poly[i].hom[3] = h3;
+        wsum += h3;
+    }
+    if (fabsf(wsum - 3.0f) < CLIP_EPS) {
+        for (int i = 0; i < *n; i++) to_screen(st, &poly[i]);
+        return 0;
+    }
+    if (clip_w(poly, n) && clip_xyz(poly, n)) {
+        for (int i = 0; i < *n; i++) {
+            Vtx* v = &poly[i];
+            v->hom[2] = 1.0f / v->hom[2];
+            v->uv[0] = v->uv[0] * v->hom[2];
+            v->uv[1] = v->uv[1] * v->hom[2];
+            float invW = 1.0f / v->hom[3];
+            v->hom[0] *= invW;
+            v->hom[1] *= invW;
+            to_screen(st, v);
+        }
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rasteriser                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const PFFframe* f;
+    uint64_t fragments;      /* covered samples, before the depth test */
+} RCtx;
+
+static int depth_pass(uint32_t fn, float z, float d)     /* depth.h:80-124 */
+{
+    switch (fn) {
+    case 0: case 1: return z == d;                       /* NOTEQUAL tests EQUAL */
+    case 2: return z < d;
+    case 3: return z <= d;
+    case 4: return z > d;
+    default: return z >= d;
+    }
+}
+
+/* Rasterize_Triangle, barycentric SIMD build (tri.c:287-558). */
+static void raster_tri(RCtx* rc, const PFFstate* st, int face, int is3D,
+                       const Vtx* v1, const Vtx* v2, const Vtx* v3)
+{
+    const PFFframe* f = rc->f;
+    int32_t x1 = cvtt_i32(v1->screen[0]), y1 = cvtt_i32(v1->screen[1]);
+    int32_t x2 = cvtt_i32(v2->screen[0]), y2 = cvtt_i32(v2->screen[1]);
+    int32_t x3 = cvtt_i32(v3->screen[0]), y3 = cvtt_i32(v3->screen[1]);
+
+    uint32_t ux1 = (uint32_t)x1, uy1 = (uint32_t)y1, ux2 = (uint32_t)x2, uy2 = (uint32_t)y2;
+    uint32_t ux3 = (uint32_t)x3, uy3 = (uint32_t)y3;
+    float area = (float)(int32_t)((ux2 - ux1) * (uy3 - uy1) - (ux3 - ux1) * (uy2 - uy1));
+    if ((face == 0 && area >= 0.0f) || (face == 1 && area <= 0.0f)) return;
+
+    int32_t xMin = x1 < x2 ? (x1 < x3 ? x1 : x3) : (x2 < x3 ? x2 : x3);
+    int32_t yMin = y1 < y2 ? (y1 < y3 ? y1 : y3) : (y2 < y3 ? y2 : y3);
+    int32_t xMax = x1 > x2 ? (x1 > x3 ? x1 : x3) : (x2 > x3 ? x2 : x3);
+    int32_t yMax = y1 > y2 ? (y1 > y3 ? y1 : y3) : (y2 > y3 ? y2 : y3);
+    if (!is3D) {
+#define CLAMPI(v, lo, hi) ((v) < (lo) ? (lo) : ((v) > (hi) ? (hi) : (v)))
+        xMin = CLAMPI(xMin, st->vpMin[0], st->vpMax[0]);
+        yMin = CLAMPI(yMin, st->vpMin[1], st->vpMax[1]);
+        xMax = CLAMPI(xMax, st->vpMin[0], st->vpMax[0]);
+        yMax = CLAMPI(yMax, st->vpMin[1], st->vpMax[1]);
+#undef CLAMPI
+    }
+    /* The row and column loops run on unsigned PFsizei (tri.c:404, 409):
+     * a negative minimum against a non-negative maximum runs no iteration. */
+    if ((uint32_t)xMin > (uint32_t)xMax || (uint32_t)yMin > (uint32_t)yMax) return;
+
+    uint32_t w1x = uy3 - uy2, w1y = ux2 - ux3;
+    uint32_t w2x = uy1 - uy3, w2y = ux3 - ux1;
+    uint32_t w3x = uy2 - uy1, w3y = ux1 - ux2;
+    if (face == 1) {
+        w1x = 0u - w1x; w1y = 0u - w1y;
+        w2x = 0u - w2x; w2y = 0u - w2y;
+        w3x = 0u - w3x; w3y = 0u - w3y;
+    }
+    uint32_t uxm = (uint32_t)xMin, uym = (uint32_t)yMin;
+    uint32_t w1Row = (uxm - ux2) * w1x + w1y * (uym - uy2);
+    uint32_t w2Row = (uxm - ux3) * w2x + w2y * (uym - uy3);
+    uint32_t w3Row = (uxm - ux1) * w3x + w3y * (uym - uy1);
+    float wInvSum = 1.0f / (float)(int32_t)(w1Row + w2Row + w3Row);
+
+    int texOn = (st->flags & PFF_TEXTURE_2D) && st->texture >= 0 && (uint32_t)st->texture < f->numTextures;
+    const PFFtexture* tex = texOn ? &f->textures[st->texture] : NULL;
+    int phong = (st->flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
+    int blend = (st->flags & PFF_BLEND) != 0;
+    int dtest = (st->flags & PFF_DEPTH_TEST) != 0;
+    const PFFmaterial* mat = &st->material[face];
+    const int W = (int)f->width, H = (int)f->height;
+
+    /* Rows yMin..yMax inclusive, columns xMin..xMax-1 (tri.c:404-420). */
+    for (int32_t y = yMin; y <= yMax; y++) {
+        for (int32_t x = xMin; x < xMax; x++) {
+            uint32_t dx = (uint32_t)x - uxm, dy = (uint32_t)y - uym;
+            int32_t w1 = (int32_t)(w1Row + dx * w1x + dy * w1y);
+            int32_t w2 = (int32_t)(w2Row + dx * w2x + dy * w2y);
+            int32_t w3 = (int32_t)(w3Row + dx * w3x + dy * w3y);
+            if ((w1 | w2 | w3) <= 0) continue;
+            if (x < 0 || y < 0 || x >= W || y >= H) continue;   /* D5 */
+            rc->fragments++;
+            size_t off = (size_t)y * (size_t)W + (size_t)x;
+            float b1 = (float)w1 * wInvSum, b2 = (float)w2 * wInvSum, b3 = (float)w3 * wInvSum;
+            float zq = rcp_((v1->hom[2] * b1 + v2->hom[2] * b2) + v3->hom[2] * b3);
+            if (dtest && !depth_pass(st->depthFunc, zq, f->depth[off])) continue;
+
+            uint32_t frag = st->shadeModel == 1
+                ? pforacle_bary_smooth(v1->color, v2->color, v3->color, b1, b2, b3)
+                : bary_flat(v1->color, v2->color, v3->color, b1, b2, b3);
+            if (tex) {                                              /* tri.c:504-513 */
+                float u = (v1->uv[0] * b1 + v2->uv[0] * b2) + v3->uv[0] * b3;
+                float v = (v1->uv[1] * b1 + v2->uv[1] * b2) + v3->uv[1] * b3;
+                if (is3D) { u = u * zq; v = v * zq; }
+                frag = modulate(pforacle_sample(tex, u, v), frag);
+            }
+            if (phong) {                                            /* tri.c:515-521 */
+                float N[3], P[3];
+                for (int i = 0; i < 3; i++) {
+                    N[i] = (v1->normal[i] * b1 + v2->normal[i] * b2) + v3->normal[i] * b3;
+                    P[i] = (v1->pos[i] * b1 + v2->pos[i] * b2) + v3->pos[i] * b3;
+                }
+                frag = light_fragment(st, mat, frag, P, N);
+            }
+            if (blend) frag = pforacle_blend(st->blendMode, frag, f->color[off]);
+            f->color[off] = frag;
+            f->depth[off] = zq;
+        }
+    }
+}
+
+/* pfiProcessRasterize_TRIANGLE_IMPL (tri.c:62-116) for one culling pass. */
+static void process_pass(RCtx* rc, const PFFstate* st, int face, const Vtx* in3)
+{
+    Vtx poly[MAX_POLY];
+    memcpy(poly, in3, 3 * sizeof(Vtx));
+    int n = 3;
+    if (st->flags & PFF_LIGHTING) {
+        for (int i = 0; i < 3; i++) {
+            Vtx* v = &poly[i];
+            const float* m = st->matNormal;                         /* pfmVec3Transform, pfm.h:1507 */
+            float nx = m[0] * v->normal[0] + m[4] * v->normal[1] + m[8] * v->normal[2] + m[12];
+            float ny = m[1] * v->normal[0] + m[5] * v->normal[1] + m[9] * v->normal[2] + m[13];
+            float nz = m[2] * v->normal[0] + m[6] * v->normal[1] + m[10] * v->normal[2] + m[14];
+            v->normal[0] = nx; v->normal[1] = ny; v->normal[2] = nz;
+            float sq = nx * nx + ny * ny + nz * nz;                 /* pfmVec3Normalize, pfm.h:1183 */
+            if (sq != 0.0f) {
+                float inv = 1.0f / sqrtf(sq);
+                for (int k = 0; k < 3; k++) v->normal[k] = v->normal[k] * inv;
+            }
+            v->color = mul_scalar(v->color, st->material[face].diffuse);
+            if (st->lightMode == 0) {
+                float ndv = dot3(v->normal, st->viewRow2);
+                v->color = light_vertex(st, &st->material[ndv < 0.0f ? 0 : 1], v->color,
+                                        st->viewPos, v->pos, v->normal);
+            }
+        }
+    }
+    int is3D = project_clip(st, poly, &n);
+    if (n < 3) return;
+    for (int i = 0; i < n - 2; i++)                                 /* tri.c:113-115 */
+        raster_tri(rc, st, face, is3D, &poly[0], &poly[i + 1], &poly[i + 2]);
+}
+
+/* pfClear, SIMD build (context.c:680-716): pixels 0..7 keep their value and
+ * the tail past the last multiple of 8 copies pixel 0. */
+static void clear(const PFFframe* f, const PFFcmd* c)
+{
+    size_t size = (size_t)f->width * f->height;
+    size_t aligned = size - (size % 8);
+    for (size_t i = 8; i < aligned; i++) { f->color[i] = c->clearColor; f->depth[i] = c->clearDepth; }
+    for (size_t i = aligned; i < size; i++) { f->color[i] = f->color[0]; f->depth[i] = f->depth[0]; }
+}
+
+/* Renders a recorded frame into frame->color / frame->depth.  Returns 0 on
+ * success; *fragments receives the number of covered samples (before the
+ * depth test) summed over rasterised triangles. */
+int pforacle_render(const PFFframe* f, uint64_t* fragments)
+{
+    RCtx rc = { f, 0 };
+    for (uint32_t c = 0; c < f->numCmds; c++) {
+        const PFFcmd* cmd = &f->cmds[c];
+        if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
+        if (cmd->kind != PFF_CMD_DRAW) return -1;
+        for (uint32_t t = cmd->first; t < cmd->last; t++) {
+            uint32_t info = f->tris.info[t];
+            uint32_t si = info & PFF_STATE_MASK;
+            if (si >= f->numStates) return -2;
+            const PFFstate* st = &f->states[si];
+            Vtx v[3];
+            memset(v, 0, sizeof v);
+            for (int k = 0; k < 3; k++) {
+                size_t vi = (size_t)t * 3 + k;
+                memcpy(v[k].pos, f->tris.position + 4 * vi, 16);
+                v[k].color = f->tris.color[vi];
+                if (f->tris.texcoord) memcpy(v[k].uv, f->tris.texcoord + 2 * vi, 8);
+                if (f->tris.normal) memcpy(v[k].normal, f->tris.normal + 3 * vi, 12);
+            }
+            if (info & PFF_PASS_FRONT) process_pass(&rc, st, 0, v);
+            if (info & PFF_PASS_BACK)  process_pass(&rc, st, 1, v);
+        }
+    }
+    if (fragments) *fragments = rc.fragments;
+    return 0;
+}
+
+/* Frame executor with the signature of PFXexecutor (pf_ext.h), so that a
+ * test harness can run the host library's recorded frames through this
+ * oracle.  `user`, when not NULL, points to a uint64_t that accumulates the
+ * fragment count. */
+int pforacle_executor(const PFFframe* f, void* user)
+{
+    uint64_t frags = 0;
+    int rc = pforacle_render(f, &frags);
+    if (user) *(uint64_t*)user += frags;
+    return rc;
+}

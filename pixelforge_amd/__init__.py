@@ -1,0 +1,177 @@
+"""pixelforge_amd -- MI355X-native PixelForge triangle fragment pipeline.
+
+The product is the C library libpixelforge_amd.so (include/pixelforge.h is
+its drop-in API, include/pf_device.h the C ABI of its HIP kernels).  This
+module only loads the built libraries with ctypes and mirrors the structs a
+Python caller needs; it has no rendering code of its own and no fallback:
+`lib()` raises if the native library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpixelforge_amd.so")
+SCENES_PATH = os.path.join(HERE, "libpf_scenes.so")
+
+# Scene kinds (csrc/pf_scenes.c)
+RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D = 1, 2, 3, 4
+
+# Enum values of include/pixelforge.h used by callers
+PF_FRONT, PF_BACK = 0, 1
+PF_EQUAL, PF_NOTEQUAL, PF_LESS, PF_LEQUAL, PF_GREATER, PF_GEQUAL = range(6)
+(PF_BLEND_AVERAGE, PF_BLEND_ALPHA, PF_BLEND_ADD, PF_BLEND_SUB, PF_BLEND_MUL,
+ PF_BLEND_SCREEN, PF_BLEND_LIGHTEN, PF_BLEND_DARKEN) = range(8)
+PF_REPEAT, PF_MIRRORED_REPEAT, PF_CLAMP_TO_EDGE = range(3)
+PF_NEAREST, PF_BILINEAR = range(2)
+PF_FLAT, PF_SMOOTH = range(2)
+
+
+class SceneParams(C.Structure):
+    """PFSparams of csrc/pf_scenes.c."""
+    _fields_ = [
+        ("kind", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("ntris", C.c_uint32), ("seed", C.c_uint64),
+        ("cull", C.c_int32), ("depthTest", C.c_int32), ("depthFunc", C.c_int32),
+        ("blend", C.c_int32), ("blendMode", C.c_int32),
+        ("texture", C.c_int32), ("texW", C.c_int32), ("texH", C.c_int32),
+        ("wrap", C.c_int32), ("filter", C.c_int32), ("shade", C.c_int32),
+        ("lighting", C.c_int32), ("numLights", C.c_int32), ("spot", C.c_int32),
+        ("attenuation", C.c_int32), ("blendFraction", C.c_float),
+        ("gridX", C.c_int32), ("gridZ", C.c_int32), ("spheres", C.c_int32),
+        ("sphereLat", C.c_int32), ("sphereLon", C.c_int32),
+        ("clearFirst", C.c_int32), ("colorMaterial", C.c_int32),
+        ("triScale", C.c_float), ("flushEvery", C.c_int32), ("useQuads", C.c_int32),
+    ]
+
+
+def scene(**kw):
+    """SceneParams with the defaults of a smooth-shaded, depth-tested scene."""
+    p = SceneParams()
+    p.cull = -1
+    p.depthTest = 1
+    p.depthFunc = PF_LESS
+    p.shade = PF_SMOOTH
+    p.blendMode = PF_BLEND_ALPHA
+    p.clearFirst = 1
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+class PFDstats(C.Structure):
+    _fields_ = [("rasterTriangles", C.c_uint64), ("tilePairs", C.c_uint64),
+                ("fragments", C.c_uint64), ("inputTriangles", C.c_uint64),
+                ("numTiles", C.c_uint32), ("tileShift", C.c_uint32),
+                ("deviceBytes", C.c_uint64)]
+
+
+class PFDtimes(C.Structure):
+    _fields_ = [("setupMs", C.c_float), ("binMs", C.c_float),
+                ("rasterMs", C.c_float), ("totalMs", C.c_float)]
+
+
+class PFDconfig(C.Structure):
+    _fields_ = [("tileShift", C.c_uint32), ("binBatch", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+
+_lib = None
+_scenes = None
+
+
+def lib():
+    """The native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python -m pixelforge_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.pfdInit.restype = C.c_int
+        L.pfdLastError.restype = C.c_char_p
+        L.pfdJobCreate.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]
+        for name in ("pfdJobUploadTargets", "pfdJobLaunch", "pfdJobSync", "pfdJobDownload"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.pfdJobRunTimed.argtypes = [C.c_void_p, C.POINTER(PFDtimes)]
+        L.pfdJobTimeReplays.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_float)]
+        L.pfdJobStats.argtypes = [C.c_void_p, C.POINTER(PFDstats)]
+        L.pfdJobSetConfig.argtypes = [C.c_void_p, C.POINTER(PFDconfig)]
+        L.pfdJobAutotune.argtypes = [C.c_void_p, C.POINTER(PFDconfig)]
+        L.pfdJobDestroy.argtypes = [C.c_void_p]
+        L.pfdJobDestroy.restype = None
+        L.pfdTakeCapturedJob.restype = C.c_void_p
+        L.pfxSetExecutor.argtypes = [C.c_void_p, C.c_void_p]
+        L.pfxSetExecutor.restype = None
+        L.pfCreateContext.restype = C.c_void_p
+        L.pfCreateContext.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int]
+        L.pfMakeCurrent.argtypes = [C.c_void_p]
+        L.pfDeleteContext.argtypes = [C.c_void_p]
+        L.pfGetError.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def scenes():
+    global _scenes
+    if _scenes is None:
+        if not os.path.exists(SCENES_PATH):
+            raise RuntimeError(f"{SCENES_PATH} is not built: run python -m pixelforge_amd.build")
+        S = C.CDLL(SCENES_PATH)
+        S.pfs_bind.restype = C.c_void_p
+        S.pfs_bind.argtypes = [C.c_char_p]
+        S.pfs_dlerror.restype = C.c_char_p
+        S.pfs_render.argtypes = [C.c_void_p, C.POINTER(SceneParams), C.c_void_p, C.c_void_p]
+        S.pfs_render.restype = C.c_int
+        S.pfs_render_batch.argtypes = [C.c_void_p, C.POINTER(SceneParams), C.c_uint32, C.POINTER(C.c_void_p)]
+        S.pfs_render_batch.restype = C.c_int
+        S.pfs_record.argtypes = [C.c_void_p, C.POINTER(SceneParams), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+        S.pfs_record.restype = C.c_int
+        S.pfs_free_texture.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        S.pfs_free_texture.restype = None
+        _scenes = S
+    return _scenes
+
+
+_apis = {}
+
+
+def api(path=LIB_PATH):
+    """Scene-driver binding (PFSapi*) of a library exporting the API."""
+    path = os.path.abspath(path)
+    if path not in _apis:
+        a = scenes().pfs_bind(path.encode())
+        if not a:
+            raise RuntimeError(f"cannot bind PixelForge API of {path}: {scenes().pfs_dlerror()}")
+        _apis[path] = a
+    return _apis[path]
+
+
+def render(params, path=LIB_PATH):
+    """Renders one scene through the library at `path`; returns
+    (color uint32[H, W], depth float32[H, W], error code)."""
+    h, w = params.height, params.width
+    color = np.zeros((h * w + 16,), np.uint32)
+    depth = np.zeros((h * w + 16,), np.float32)
+    err = scenes().pfs_render(api(path), C.byref(params), color.ctypes.data, depth.ctypes.data)
+    return color[: h * w].reshape(h, w), depth[: h * w].reshape(h, w), err
+
+
+def render_batch(params_list, path=LIB_PATH):
+    """Renders several scenes, one context each, submitted as one batch;
+    returns (list of color uint32[H, W], error code)."""
+    n = len(params_list)
+    arr = (SceneParams * n)(*params_list)
+    bufs = [np.zeros((p.height * p.width + 16,), np.uint32) for p in params_list]
+    ptrs = (C.c_void_p * n)(*[b.ctypes.data for b in bufs])
+    err = scenes().pfs_render_batch(api(path), arr, n, ptrs)
+    return [b[: p.height * p.width].reshape(p.height, p.width) for b, p in zip(bufs, params_list)], err
+
+
+def unpack(color):
+    """uint32 RGBA8 image -> uint8 [H, W, 4] (r, g, b, a)."""
+    return color.view(np.uint8).reshape(color.shape + (4,))

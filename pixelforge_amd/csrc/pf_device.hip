@@ -1,0 +1,1810 @@
+// pf_device.hip -- HIP kernels for gfx950 (MI355X) and the C ABI of
+// include/pf_device.h.
+//
+// Pipeline for one pass of a job (a job holds one or more recorded frames;
+// a pass holds at most one clear+draw segment per frame):
+//   k_setup<false>   one thread per recorded triangle: lighting (Gouraud),
+//                    projection, clipping, fan split, cull; counts the raster
+//                    triangles it emits                   (triangles.c:62-116, 157-349)
+//   scan             exclusive scan of the counts -> output slots (submission order)
+//   k_setup<true>    same work, writes the raster-triangle records and the
+//                    number of screen tiles each one overlaps
+//   scan             -> pair slots
+//   k_pair_emit      (tile, raster triangle) pairs in submission order
+//   radix sort       stable LSD sort of the pairs by tile, 8 bits per pass,
+//                    so each tile's list stays in submission order
+//   k_tile_bounds    per-tile [begin, end) of the sorted pairs
+//   k_raster         one workgroup per tile; the tile's RGBA8 colour and f32
+//                    depth live in VGPRs (one pixel per lane per 8x8 block),
+//                    initialised by the fused clear or loaded once, updated
+//                    triangle by triangle in list order, written back once
+//                                                          (triangles.c:287-558)
+// All arithmetic that decides coverage, depth and colour follows the
+// reference's AVX2 path operation by operation; the file is compiled with
+// -ffp-contract=off and uses correctly rounded division and square root, so
+// it is bit-identical to the CPU oracle's restatement.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <algorithm>
+
+#include "pf_frame.h"
+#include "pf_device.h"
+#define PF_X86_TABLE_QUAL static __constant__ const
+#include "pf_x86_approx.h"
+
+#define PF_PI_D 3.14159265358979323846
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Device-side tables
+// ---------------------------------------------------------------------------
+
+struct DTex {
+    const void* pixels;     // device copy
+    uint32_t    w, h, format, wrap, filter, pad;
+};
+
+struct DFrame {
+    uint32_t* color;        // device render target
+    float*    depth;
+    uint32_t  W, H;
+    uint32_t  tilesX, tilesY, tileBase, pad;
+};
+
+struct DSeg {               // the part of one frame a pass renders
+    uint32_t frame;
+    uint32_t triFirst, triLast;   // global recorded-triangle range
+    uint32_t passTriBase;         // index of triFirst within the pass
+    uint32_t clear;               // 1: a clear precedes the draws
+    uint32_t clearColor;
+    float    clearDepth;
+    uint32_t pad;
+};
+
+// Raster triangle: what the tile kernel needs for coverage, depth and
+// attribute interpolation.  w_k(x, y) = A_k*x + B_k*y + C_k (mod 2^32)
+// equals the reference's incremental edge functions (triangles.c:324-349).
+struct __attribute__((aligned(16))) RTri {
+    int32_t  A[3], B[3], C[3];
+    int32_t  xMin, yMin, xMax, yMax;
+    float    wInvSum;
+    float    z[3];
+    uint32_t c[3];
+    uint32_t meta;          // see META_* below
+    uint32_t state;         // global state index
+    float    uv[6];
+    uint32_t pad[3];
+};
+static_assert(sizeof(RTri) == 128, "RTri layout");
+
+struct __attribute__((aligned(16))) RTriL {   // per-fragment lighting inputs
+    float n[9];
+    float p[9];
+    float pad[2];
+};
+static_assert(sizeof(RTriL) == 80, "RTriL layout");
+
+enum : uint32_t {
+    META_BACK      = 1u << 0,
+    META_3D        = 1u << 1,
+    META_SMOOTH    = 1u << 2,
+    META_DEPTH     = 1u << 3,
+    META_BLEND     = 1u << 4,
+    META_TEX       = 1u << 5,
+    META_PHONG     = 1u << 6,
+    META_DFUNC_SH  = 8,     // 3 bits
+    META_BMODE_SH  = 12,    // 3 bits
+};
+
+// ---------------------------------------------------------------------------
+// Numeric helpers (x86 semantics of the reference build)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t cvtt_i32(float f) {
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+__device__ __forceinline__ int32_t cvt_rne_i32(float f) {
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)__builtin_rintf(f);
+}
+__device__ __forceinline__ uint8_t f2u8(float f) { return (uint8_t)(uint32_t)cvtt_i32(f); }
+__device__ __forceinline__ float mn_ps(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float mx_ps(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float clamp_ps(float x, float lo, float hi) { return mn_ps(mx_ps(x, lo), hi); }
+__device__ __forceinline__ float sqrt_cr(float x) { return __fsqrt_rn(x); }
+
+// _mm256_rcp_ps / _mm256_rsqrt_ps of the reference build (simd.h:1217-1245):
+// table approximations measured on Intel Xeon (include/pf_x86_approx.h).
+__device__ __forceinline__ float rcp_x86(float xf) {
+    uint32_t x = __float_as_uint(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+    if (E == 0xffu) return __uint_as_float(m ? (x | 0x00400000u) : s);
+    if (E == 0) return __uint_as_float(s | 0x7f800000u);
+    int oe = 253 - (int)E;
+    if (oe <= 0) return __uint_as_float(s);
+    return __uint_as_float(s | ((uint32_t)oe << 23) | ((uint32_t)pf_rcp_table[m >> 12] << 11));
+}
+__device__ __forceinline__ float rsqrt_x86(float xf) {
+    uint32_t x = __float_as_uint(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+    if (E == 0xffu) return __uint_as_float(m ? (x | 0x00400000u) : (s ? 0xffc00000u : 0u));
+    if (E == 0) return __uint_as_float(s | 0x7f800000u);
+    if (s) return __uint_as_float(0xffc00000u);
+    int odd = (int)(E & 1u);
+    int oe = odd ? 126 - ((int)E - 127) / 2 : 127 - ((int)E - 126) / 2;
+    return __uint_as_float(((uint32_t)oe << 23) | ((uint32_t)pf_rsqrt_table[odd][m >> 13] << 11));
+}
+__device__ __forceinline__ int ch(uint32_t c, int i) { return (int)((c >> (8 * i)) & 0xFFu); }
+__device__ __forceinline__ uint8_t min255(int n) { return (uint8_t)(n | ((255 - n) >> 31)); }
+
+// Cephes single-precision logf / expf as vectorised in Pommier's
+// sse_mathfun; pow(x, e) = exp(log(x) * e) (simd.h:183-304, 1157-1169).
+__device__ float cephes_logf(float x) {
+    const float one = 1.0f;
+    uint32_t invalid = (x <= 0.0f) ? 0xFFFFFFFFu : 0u;
+    x = mx_ps(x, __uint_as_float(0x00800000u));
+    int32_t e_i = (int32_t)(__float_as_uint(x) >> 23);
+    x = __uint_as_float((__float_as_uint(x) & ~0x7f800000u) | __float_as_uint(0.5f));
+    e_i -= 0x7f;
+    float e = (float)e_i;
+    e = e + one;
+    bool lt = x < 0.707106781186547524f;
+    float tmp = lt ? x : 0.0f;
+    x = x - one;
+    e = e - (lt ? one : 0.0f);
+    x = x + tmp;
+    float z = x * x;
+    float y = 7.0376836292E-2f;
+    y = y * x; y = y + -1.1514610310E-1f;
+    y = y * x; y = y + 1.1676998740E-1f;
+    y = y * x; y = y + -1.2420140846E-1f;
+    y = y * x; y = y + 1.4249322787E-1f;
+    y = y * x; y = y + -1.6668057665E-1f;
+    y = y * x; y = y + 2.0000714765E-1f;
+    y = y * x; y = y + -2.4999993993E-1f;
+    y = y * x; y = y + 3.3333331174E-1f;
+    y = y * x;
+    y = y * z;
+    tmp = e * -2.12194440e-4f;
+    y = y + tmp;
+    tmp = z * 0.5f;
+    y = y - tmp;
+    tmp = e * 0.693359375f;
+    x = x + y;
+    x = x + tmp;
+    return __uint_as_float(__float_as_uint(x) | invalid);
+}
+
+__device__ float cephes_expf(float x) {
+    const float one = 1.0f;
+    x = mn_ps(x, 88.3762626647949f);
+    x = mx_ps(x, -88.3762626647949f);
+    float fx = x * 1.44269504088896341f;
+    fx = fx + 0.5f;
+    float tmp = floorf(fx);
+    float mask = (tmp > fx) ? one : 0.0f;
+    fx = tmp - mask;
+    tmp = fx * 0.693359375f;
+    float z = fx * -2.12194440e-4f;
+    x = x - tmp;
+    x = x - z;
+    z = x * x;
+    float y = 1.9875691500E-4f;
+    y = y * x; y = y + 1.3981999507E-3f;
+    y = y * x; y = y + 8.3334519073E-3f;
+    y = y * x; y = y + 4.1665795894E-2f;
+    y = y * x; y = y + 1.6666665459E-1f;
+    y = y * x; y = y + 5.0000001201E-1f;
+    y = y * z;
+    y = y + x;
+    y = y + one;
+    int32_t n = cvtt_i32(fx) + 0x7f;
+    float pow2n = __uint_as_float((uint32_t)n << 23);
+    return y * pow2n;
+}
+
+__device__ __forceinline__ float pow_cephes(float b, float e) { return cephes_expf(cephes_logf(b) * e); }
+
+// ---------------------------------------------------------------------------
+// Colour operations (color.h, blend.h)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pack_f3(float r, float g, float b) {      // color.h:110-121, vecSize 3
+    uint32_t p = (uint32_t)cvt_rne_i32(clamp_ps(r, 0.0f, 1.0f) * 255.0f);
+    p |= (uint32_t)cvt_rne_i32(clamp_ps(g, 0.0f, 1.0f) * 255.0f) << 8;
+    p |= (uint32_t)cvt_rne_i32(clamp_ps(b, 0.0f, 1.0f) * 255.0f) << 16;
+    return p;
+}
+
+__device__ __forceinline__ uint32_t bary_smooth(uint32_t c1, uint32_t c2, uint32_t c3,
+                                                float b1, float b2, float b3) {  // color.h:153-181
+    uint32_t u1 = (uint32_t)cvt_rne_i32(b1 * 255.0f);
+    uint32_t u2 = (uint32_t)cvt_rne_i32(b2 * 255.0f);
+    uint32_t u3 = (uint32_t)cvt_rne_i32(b3 * 255.0f);
+    uint32_t p = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        uint32_t s = u1 * (uint32_t)ch(c1, i) + u2 * (uint32_t)ch(c2, i);
+        s = s + u3 * (uint32_t)ch(c3, i);
+        p |= ((s * 257u) >> 16) << (8 * i);
+    }
+    return p;
+}
+
+__device__ __forceinline__ uint32_t bary_flat(uint32_t c1, uint32_t c2, uint32_t c3,
+                                              float b1, float b2, float b3) {    // color.h:183-203
+    float m = mx_ps(b1, mx_ps(b2, b3));
+    return ((m == b1) ? c1 : 0u) | (((m == b2) ? c2 : 0u) | ((m == b3) ? c3 : 0u));
+}
+
+__device__ __forceinline__ uint32_t blend_op(uint32_t mode, uint32_t s, uint32_t d) {  // blend.h:137-274
+    uint32_t p = 0;
+    int sa = ch(s, 3) + 1;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int sc = ch(s, i), dc = ch(d, i), o;
+        switch (mode) {
+        case 0: o = (sc + dc) >> 1; break;
+        case 1: o = ((i == 3 ? 255 : sc) * sa + dc * (256 - sa)) >> 8; break;
+        case 2: o = min(sc + dc, 255); break;
+        case 3: o = max(sc + dc, 0); break;
+        case 4: o = (sc * dc) >> 8; break;
+        case 5: o = min(((dc * (255 - sc)) >> 8) + sc, 255); break;
+        case 6: o = max(sc, dc); break;
+        default: o = min(sc, dc); break;
+        }
+        p |= (uint32_t)o << (8 * i);
+    }
+    return p;
+}
+
+__device__ __forceinline__ uint32_t modulate(uint32_t t, uint32_t f) {       // blend.h:199-212
+    uint32_t p = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) p |= (uint32_t)((ch(t, i) * ch(f, i)) >> 8) << (8 * i);
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Texture sampling (sampler.h:202-410; pixel.h:2909-3008)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t texel_at(const DTex& t, int32_t off) {
+    int64_t n = (int64_t)t.w * t.h;
+    int64_t o = off;
+    o = o < 0 ? 0 : (o >= n ? n - 1 : o);
+    if (t.format == PFF_TEX_RGBA8) return ((const uint32_t*)t.pixels)[o];
+    const float4 p = ((const float4*)t.pixels)[o];
+    int32_t r = cvt_rne_i32(p.x * 255.0f), g = cvt_rne_i32(p.y * 255.0f);
+    int32_t b = cvt_rne_i32(p.z * 255.0f), a = cvt_rne_i32(p.w * 255.0f);
+    return ((uint32_t)a << 24) | ((uint32_t)b << 16) | ((uint32_t)g << 8) | (uint32_t)r;
+}
+
+__device__ __forceinline__ int32_t wrap_coord(uint32_t wrap, float u, uint32_t size) {
+    float tw = (float)(size - 1);
+    if (wrap == 0) {
+        int32_t x = cvt_rne_i32((u - truncf(u)) * tw);
+        return x < 0 ? (int32_t)(0u - (uint32_t)x) : x;
+    }
+    if (wrap == 1) {
+        float a = fabsf(u);
+        float m = a - floorf(__fdiv_rn(a, 2.0f)) * 2.0f;
+        float mirror = 1.0f - (m - 1.0f);
+        if (m > 1.0f) m = mirror;
+        return cvt_rne_i32(m * tw + 0.5f);
+    }
+    return cvt_rne_i32(clamp_ps(u, 0.0f, 1.0f) * tw + 0.5f);
+}
+
+__device__ __forceinline__ uint32_t lerp_color(uint32_t a, uint32_t b, float t) {   // color.h:137-144
+    const float inv255 = 1.0f / 255.0f;
+    uint32_t p = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float fa = (float)ch(a, i) * inv255;
+        float fb = (float)ch(b, i) * inv255;
+        float r = fa + t * (fb - fa);
+        p |= (uint32_t)cvt_rne_i32(clamp_ps(r, 0.0f, 1.0f) * 255.0f) << (8 * i);
+    }
+    return p;
+}
+
+__device__ uint32_t sample_tex(const DTex& t, float u, float v) {
+    uint32_t w = t.w;
+    if (t.filter == 0) {
+        int32_t x = wrap_coord(t.wrap, u, t.w);
+        int32_t y = wrap_coord(t.wrap, v, t.h);
+        return texel_at(t, (int32_t)((uint32_t)y * w + (uint32_t)x));
+    }
+    float tx = __fdiv_rn(1.0f, (float)t.w), ty = __fdiv_rn(1.0f, (float)t.h);
+    int32_t x0 = wrap_coord(t.wrap, u, t.w);
+    int32_t y0 = wrap_coord(t.wrap, v, t.h);
+    int32_t x1 = wrap_coord(t.wrap, u + tx, t.w);
+    int32_t y1 = wrap_coord(t.wrap, v + ty, t.h);
+    float fx = clamp_ps(u * (float)t.w - (float)x0, 0.0f, 1.0f);
+    float fy = clamp_ps(v * (float)t.h - (float)y0, 0.0f, 1.0f);
+    uint32_t c00 = texel_at(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x0));
+    uint32_t c10 = texel_at(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x1));
+    uint32_t c01 = texel_at(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x0));
+    uint32_t c11 = texel_at(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x1));
+    uint32_t c0 = lerp_color(c00, c10, fx);
+    uint32_t c1 = lerp_color(c01, c11, fx);
+    return lerp_color(c0, c1, fy);
+}
+
+// ---------------------------------------------------------------------------
+// Lighting
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float dot3(const float* a, const float* b) {      // pfm.h:1227
+    float d = 0.0f;
+    d += a[0] * b[0];
+    d += a[1] * b[1];
+    d += a[2] * b[2];
+    return d;
+}
+
+// pfiLightingProcess (lighting.c:23-144), per vertex.
+__device__ uint32_t light_vertex(const PFFstate& st, const PFFmaterial& mat, uint32_t diffuse,
+                                 const float* viewPos, const float* fragPos, const float* N) {
+    uint8_t R = (uint8_t)ch(mat.emission, 0), G = (uint8_t)ch(mat.emission, 1), B = (uint8_t)ch(mat.emission, 2);
+    uint8_t aC[3];
+    for (int i = 0; i < 3; i++) aC[i] = (uint8_t)((ch(mat.ambient, i) * ch(diffuse, i)) / 255);
+    float V[3];
+    float lsq = 0.0f;
+    for (int i = 0; i < 3; i++) { V[i] = viewPos[i] - fragPos[i]; lsq += V[i] * V[i]; }
+    float inv = __fdiv_rn(1.0f, sqrt_cr(lsq));
+    for (int i = 0; i < 3; i++) V[i] = V[i] * inv;
+    for (uint32_t li = 0; li < st.numLights; li++) {
+        const PFFlight& Lt = st.lights[li];
+        uint8_t lC[3] = {0, 0, 0};
+        float L[3];
+        for (int i = 0; i < 3; i++) L[i] = Lt.position[i] - fragPos[i];
+        float distSq = L[0] * L[0] + L[1] * L[1] + L[2] * L[2];
+        float dist = 0.0f;
+        uint8_t intensity = 255, att = 255;
+        if (distSq != 0.0f) {
+            dist = sqrt_cr(distSq);
+            float invMag = __fdiv_rn(1.0f, dist);
+            for (int i = 0; i < 3; i++) L[i] *= invMag;
+        }
+        bool skip = false;
+        if ((double)Lt.innerCutOff < PF_PI_D) {
+            float nd[3] = { -Lt.direction[0], -Lt.direction[1], -Lt.direction[2] };
+            float theta = dot3(L, nd);
+            float eps = Lt.innerCutOff - Lt.outerCutOff;
+            int32_t iv = cvtt_i32(__fdiv_rn(255.0f * (theta - Lt.outerCutOff), eps));
+            intensity = (uint8_t)(iv < 0 ? 0 : (iv > 255 ? 255 : iv));
+            if (intensity == 0) skip = true;
+        }
+        if (!skip && (Lt.attLinear != 0.0f || Lt.attQuadratic != 0.0f)) {
+            att = f2u8(__fdiv_rn(255.0f, Lt.attConstant + Lt.attLinear * dist + Lt.attQuadratic * distSq));
+            if (att == 0) skip = true;
+        }
+        if (!skip) {
+            uint8_t factor = (uint8_t)((intensity * att) / 255);
+            int32_t dv = cvtt_i32(255.0f * dot3(N, L));
+            uint8_t diff = (uint8_t)(dv > 0 ? dv : 0);
+            for (int i = 0; i < 3; i++)
+                lC[i] = min255(lC[i] + (ch(diffuse, i) * ch(Lt.diffuse, i) * diff) / (255 * 255));
+            float H[3];
+            for (int i = 0; i < 3; i++) H[i] = L[i] + V[i];
+            float hs = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+            if (hs != 0.0f) {
+                float hi = __fdiv_rn(1.0f, sqrt_cr(hs));
+                for (int i = 0; i < 3; i++) H[i] = H[i] * hi;
+            }
+            float nh = fmaxf(dot3(N, H), 0.0f);
+            uint8_t spec = f2u8(255.0f * (float)pow((double)nh, (double)mat.shininess));
+            for (int i = 0; i < 3; i++)
+                lC[i] = min255(lC[i] + (ch(mat.specular, i) * ch(Lt.specular, i) * spec) / (255 * 255));
+            for (int i = 0; i < 3; i++) lC[i] = (uint8_t)((lC[i] * factor) / 255);
+        }
+        R = min255(R + lC[0] + (aC[0] * ch(Lt.ambient, 0)) / 255);
+        G = min255(G + lC[1] + (aC[1] * ch(Lt.ambient, 1)) / 255);
+        B = min255(B + lC[2] + (aC[2] * ch(Lt.ambient, 2)) / 255);
+    }
+    return (uint32_t)R | ((uint32_t)G << 8) | ((uint32_t)B << 16) | (diffuse & 0xFF000000u);
+}
+
+__device__ __forceinline__ void direction_simd(float* d, const float* a, const float* b) {  // simd.h:2469-2490
+    for (int i = 0; i < 3; i++) d[i] = a[i] - b[i];
+    float l = d[0] * d[0] + d[1] * d[1];
+    l = l + d[2] * d[2];
+    l = mx_ps(l, 1e-5f);
+    float inv = rsqrt_x86(l);
+    for (int i = 0; i < 3; i++) d[i] = d[i] * inv;
+}
+
+__device__ __forceinline__ float dot3_simd(const float* a, const float* b) {  // simd.h:2356-2363
+    float d = a[0] * b[0] + a[1] * b[1];
+    return d + a[2] * b[2];
+}
+
+// pfiSimdLightingProcess (lighting.c:148-258), per fragment.
+__device__ uint32_t light_fragment(const PFFstate* __restrict__ st, const PFFmaterial& mat, uint32_t frag,
+                                   const float* P, const float* N) {
+    const float inv255 = 1.0f / 255.0f;
+    float D[3], A[3], S[3];
+    for (int i = 0; i < 3; i++) {
+        D[i] = (float)ch(frag, i) * inv255;
+        A[i] = (float)ch(mat.ambient, i) * inv255;
+        S[i] = (float)ch(mat.specular, i) * inv255;
+        A[i] = A[i] * D[i];
+    }
+    float V[3];
+    direction_simd(V, st->viewPos, P);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    const uint32_t nl = st->numLights;
+    for (uint32_t li = 0; li < nl; li++) {
+        const PFFlight& Lt = st->lights[li];
+        float la[3], ld[3], ls[3];
+        for (int i = 0; i < 3; i++) {
+            la[i] = (float)ch(Lt.ambient, i) * inv255;
+            ld[i] = (float)ch(Lt.diffuse, i) * inv255;
+            ls[i] = (float)ch(Lt.specular, i) * inv255;
+        }
+        float L[3];
+        direction_simd(L, Lt.position, P);
+        float amb[3], dif[3], spc[3];
+        for (int i = 0; i < 3; i++) amb[i] = la[i] * A[i];
+        float diff = mx_ps(dot3_simd(N, L), 0.0f);
+        for (int i = 0; i < 3; i++) { dif[i] = ld[i] * diff; dif[i] = dif[i] * D[i]; }
+        float H[3];
+        for (int i = 0; i < 3; i++) H[i] = L[i] + V[i];
+        float hs = H[0] * H[0] + H[1] * H[1];
+        hs = hs + H[2] * H[2];
+        float hinv = rsqrt_x86(hs);
+        for (int i = 0; i < 3; i++) H[i] = H[i] * hinv;
+        float spec = mx_ps(dot3_simd(N, H), 0.0f);
+        spec = pow_cephes(spec, mat.shininess);
+        for (int i = 0; i < 3; i++) { spc[i] = ls[i] * spec; spc[i] = spc[i] * S[i]; }
+        if ((double)Lt.innerCutOff < PF_PI_D) {
+            float nd[3] = { 0.0f - Lt.direction[0], 0.0f - Lt.direction[1], 0.0f - Lt.direction[2] };
+            float theta = dot3_simd(L, nd);
+            float eps = Lt.innerCutOff - Lt.outerCutOff;
+            float I = clamp_ps(__fdiv_rn(theta - Lt.outerCutOff, eps), 0.0f, 1.0f);
+            for (int i = 0; i < 3; i++) { dif[i] = dif[i] * I; spc[i] = spc[i] * I; }
+        }
+        if (Lt.attLinear != 0.0f || Lt.attQuadratic != 0.0f) {
+            float d0 = Lt.position[0] - P[0];
+            float d1 = Lt.position[1] - P[1];
+            float d2 = Lt.position[1] - P[1];          // simd.h:2429 reads y twice
+            float dsq = d0 * d0 + (d1 * d1 + d2 * d2);
+            float dist = sqrt_cr(dsq);
+            float att = rcp_x86(Lt.attConstant + (Lt.attLinear * dist + Lt.attQuadratic * dsq));
+            for (int i = 0; i < 3; i++) { amb[i] *= att; dif[i] *= att; spc[i] *= att; }
+        }
+        for (int i = 0; i < 3; i++) {
+            acc[i] = acc[i] + amb[i];
+            acc[i] = acc[i] + dif[i];
+            acc[i] = acc[i] + spc[i];
+        }
+    }
+    return pack_f3(acc[0], acc[1], acc[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Setup: projection, clipping, fan split, cull (triangles.c:62-349)
+// ---------------------------------------------------------------------------
+
+struct Vtx {
+    float hom[4];
+    float scr[2];
+    float pos[4];
+    float nrm[3];
+    float uv[2];
+    uint32_t col;
+};
+
+__device__ __forceinline__ Vtx lerp_vtx(const Vtx& s, const Vtx& e, float t) {   // ictx.c:67-90
+    Vtx r;
+    uint8_t uT = f2u8(255.0f * t);
+    uint32_t col = 0;
+    for (int i = 0; i < 4; i++) {
+        r.hom[i] = s.hom[i] + t * (e.hom[i] - s.hom[i]);
+        r.pos[i] = s.pos[i] + t * (e.pos[i] - s.pos[i]);
+        int cs = ch(s.col, i), ce = ch(e.col, i);
+        col |= (uint32_t)(uint8_t)(cs + ((int)uT * (ce - cs)) / 255) << (8 * i);
+    }
+    for (int i = 0; i < 2; i++) r.uv[i] = s.uv[i] + t * (e.uv[i] - s.uv[i]);
+    for (int i = 0; i < 3; i++) r.nrm[i] = s.nrm[i] + t * (e.nrm[i] - s.nrm[i]);
+    r.scr[0] = r.scr[1] = 0.0f;
+    r.col = col;
+    return r;
+}
+
+__device__ __forceinline__ void to_screen(const PFFstate& st, Vtx& v) {        // ictx.c:51-65
+    v.scr[0] = ((float)st.vpPos[0] + ((v.hom[0] + 1.0f) * 0.5f) * (float)st.vpDim[0]) + 0.5f;
+    v.scr[1] = ((float)st.vpPos[1] + ((1.0f - v.hom[1]) * 0.5f) * (float)st.vpDim[1]) + 0.5f;
+}
+
+// Sutherland-Hodgman against w >= eps then +-x, +-y, +-z <= w
+// (triangles.c:157-244).  poly holds up to 12 vertices.
+__device__ __noinline__ int clip_polygon(Vtx* poly, int n) {
+    Vtx in[12];
+    // W plane
+    for (int i = 0; i < n; i++) in[i] = poly[i];
+    int cnt = n;
+    n = 0;
+    {
+        int prev = cnt - 1;
+        int pd = (in[prev].hom[3] < 1e-5f) ? -1 : 1;
+        for (int i = 0; i < cnt; i++) {
+            int cd = (in[i].hom[3] < 1e-5f) ? -1 : 1;
+            if (pd * cd < 0 && n < 12)
+                poly[n++] = lerp_vtx(in[prev], in[i], __fdiv_rn(1e-5f - in[prev].hom[3], in[i].hom[3] - in[prev].hom[3]));
+            if (cd > 0 && n < 12) poly[n++] = in[i];
+            pd = cd;
+            prev = i;
+        }
+    }
+    if (n == 0) return 0;
+    for (int ax = 0; ax < 3; ax++) {
+        for (int side = 0; side < 2; side++) {
+            if (n == 0) return 0;
+            for (int i = 0; i < n; i++) in[i] = poly[i];
+            cnt = n;
+            n = 0;
+            int prev = cnt - 1;
+            float pa = side ? -in[prev].hom[ax] : in[prev].hom[ax];
+            int pd = (pa <= in[prev].hom[3]) ? 1 : -1;
+            for (int i = 0; i < cnt; i++) {
+                float ca = side ? -in[i].hom[ax] : in[i].hom[ax];
+                int cd = (ca <= in[i].hom[3]) ? 1 : -1;
+                if (pd * cd <= 0 && n < 12) {
+                    float num, den;
+                    if (!side) { num = in[prev].hom[3] - in[prev].hom[ax]; den = num - (in[i].hom[3] - in[i].hom[ax]); }
+                    else       { num = in[prev].hom[3] + in[prev].hom[ax]; den = num - (in[i].hom[3] + in[i].hom[ax]); }
+                    poly[n++] = lerp_vtx(in[prev], in[i], __fdiv_rn(num, den));
+                }
+                if (cd > 0 && n < 12) poly[n++] = in[i];
+                pd = cd;
+                prev = i;
+            }
+        }
+    }
+    return n;
+}
+
+struct SetupArgs {
+    const float4*   pos;
+    const uint32_t* col;
+    const float2*   uv;
+    const float*    nrm;
+    const uint32_t* info;
+    const PFFstate* states;
+    const DSeg*     segs;
+    const DFrame*   frames;
+    uint32_t        numSegs;
+    uint32_t        numPassTris;
+    uint32_t        tileShift;
+    uint32_t*       count;          // [numPassTris] raster triangles per recorded triangle
+    const uint32_t* offset;         // write pass: output slot of the first one
+    RTri*           rts;
+    RTriL*          rtl;
+    uint32_t*       pairCount;      // write pass: tiles overlapped per raster triangle
+};
+
+// Builds (or only counts) the raster triangle of screen vertices a, b, c for
+// one culling pass (Rasterize_Triangle setup, triangles.c:287-349).
+template <bool WRITE>
+__device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& st, const DFrame& F,
+                                            uint32_t stateIdx, int face, bool is3D,
+                                            const Vtx& a, const Vtx& b, const Vtx& c, uint32_t slot) {
+    int32_t x1 = cvtt_i32(a.scr[0]), y1 = cvtt_i32(a.scr[1]);
+    int32_t x2 = cvtt_i32(b.scr[0]), y2 = cvtt_i32(b.scr[1]);
+    int32_t x3 = cvtt_i32(c.scr[0]), y3 = cvtt_i32(c.scr[1]);
+    uint32_t ux1 = x1, uy1 = y1, ux2 = x2, uy2 = y2, ux3 = x3, uy3 = y3;
+    float area = (float)(int32_t)((ux2 - ux1) * (uy3 - uy1) - (ux3 - ux1) * (uy2 - uy1));
+    if ((face == 0 && area >= 0.0f) || (face == 1 && area <= 0.0f)) return false;
+    int32_t xMin = min(x1, min(x2, x3)), yMin = min(y1, min(y2, y3));
+    int32_t xMax = max(x1, max(x2, x3)), yMax = max(y1, max(y2, y3));
+    if (!is3D) {
+        xMin = min(max(xMin, st.vpMin[0]), st.vpMax[0]);
+        yMin = min(max(yMin, st.vpMin[1]), st.vpMax[1]);
+        xMax = min(max(xMax, st.vpMin[0]), st.vpMax[0]);
+        yMax = min(max(yMax, st.vpMin[1]), st.vpMax[1]);
+    }
+    // Unsigned loops of the reference (triangles.c:404, 409) run nothing
+    // when the minimum is negative and the maximum is not.
+    if ((uint32_t)xMin > (uint32_t)xMax || (uint32_t)yMin > (uint32_t)yMax) return false;
+    // Pixels outside the render target are never written: the covered
+    // columns are [xMin, xMax) and rows [yMin, yMax].
+    int32_t cx0 = max(xMin, 0), cy0 = max(yMin, 0);
+    int32_t cx1 = min(xMax - 1, (int32_t)F.W - 1), cy1 = min(yMax, (int32_t)F.H - 1);
+    if (cx0 > cx1 || cy0 > cy1) return false;
+    if (!WRITE) return true;
+
+    uint32_t Ax[3] = { uy3 - uy2, uy1 - uy3, uy2 - uy1 };
+    uint32_t By[3] = { ux2 - ux3, ux3 - ux1, ux1 - ux2 };
+    if (face == 1) for (int k = 0; k < 3; k++) { Ax[k] = 0u - Ax[k]; By[k] = 0u - By[k]; }
+    uint32_t ox[3] = { ux2, ux3, ux1 }, oy[3] = { uy2, uy3, uy1 };
+    RTri r;
+    uint32_t sum = 0;
+    for (int k = 0; k < 3; k++) {
+        uint32_t Ck = 0u - (ox[k] * Ax[k] + oy[k] * By[k]);
+        r.A[k] = (int32_t)Ax[k];
+        r.B[k] = (int32_t)By[k];
+        r.C[k] = (int32_t)Ck;
+        sum += Ax[k] * (uint32_t)xMin + By[k] * (uint32_t)yMin + Ck;
+    }
+    r.xMin = xMin; r.yMin = yMin; r.xMax = xMax; r.yMax = yMax;
+    r.wInvSum = __fdiv_rn(1.0f, (float)(int32_t)sum);
+    r.z[0] = a.hom[2]; r.z[1] = b.hom[2]; r.z[2] = c.hom[2];
+    r.c[0] = a.col; r.c[1] = b.col; r.c[2] = c.col;
+    uint32_t meta = (face ? META_BACK : 0u) | (is3D ? META_3D : 0u);
+    if (st.shadeModel == 1) meta |= META_SMOOTH;
+    if (st.flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st.depthFunc & 7u) << META_DFUNC_SH);
+    if (st.flags & PFF_BLEND) meta |= META_BLEND | ((st.blendMode & 7u) << META_BMODE_SH);
+    if ((st.flags & PFF_TEXTURE_2D) && st.texture >= 0) meta |= META_TEX;
+    bool phong = (st.flags & PFF_LIGHTING) && st.lightMode == 1 && st.numLights > 0;
+    if (phong) meta |= META_PHONG;
+    r.meta = meta;
+    r.state = stateIdx;
+    r.uv[0] = a.uv[0]; r.uv[1] = a.uv[1]; r.uv[2] = b.uv[0]; r.uv[3] = b.uv[1]; r.uv[4] = c.uv[0]; r.uv[5] = c.uv[1];
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    A.rts[slot] = r;
+    if (phong) {
+        RTriL l;
+        for (int i = 0; i < 3; i++) {
+            l.n[i] = a.nrm[i]; l.n[3 + i] = b.nrm[i]; l.n[6 + i] = c.nrm[i];
+            l.p[i] = a.pos[i]; l.p[3 + i] = b.pos[i]; l.p[6 + i] = c.pos[i];
+        }
+        l.pad[0] = l.pad[1] = 0.0f;
+        A.rtl[slot] = l;
+    }
+    uint32_t ts = A.tileShift;
+    uint32_t ntx = ((uint32_t)cx1 >> ts) - ((uint32_t)cx0 >> ts) + 1;
+    uint32_t nty = ((uint32_t)cy1 >> ts) - ((uint32_t)cy0 >> ts) + 1;
+    A.pairCount[slot] = ntx * nty;
+    return true;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= A.numPassTris) return;
+    // Segment of this thread: the last one whose passTriBase <= gid.
+    uint32_t lo = 0, hi = A.numSegs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (A.segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
+    }
+    const DSeg sg = A.segs[lo];
+    const uint32_t t = sg.triFirst + (gid - sg.passTriBase);
+    const DFrame F = A.frames[sg.frame];
+    const uint32_t info = A.info[t];
+    const uint32_t si = info & PFF_STATE_MASK;
+    const PFFstate& st = A.states[si];
+
+    Vtx base[3];
+    for (int k = 0; k < 3; k++) {
+        size_t vi = (size_t)t * 3 + k;
+        float4 p = A.pos[vi];
+        base[k].pos[0] = p.x; base[k].pos[1] = p.y; base[k].pos[2] = p.z; base[k].pos[3] = p.w;
+        base[k].col = A.col[vi];
+        if (A.uv) { float2 q = A.uv[vi]; base[k].uv[0] = q.x; base[k].uv[1] = q.y; }
+        else { base[k].uv[0] = base[k].uv[1] = 0.0f; }
+        if (A.nrm) { base[k].nrm[0] = A.nrm[3 * vi]; base[k].nrm[1] = A.nrm[3 * vi + 1]; base[k].nrm[2] = A.nrm[3 * vi + 2]; }
+        else { base[k].nrm[0] = base[k].nrm[1] = base[k].nrm[2] = 0.0f; }
+        base[k].scr[0] = base[k].scr[1] = 0.0f;
+    }
+
+    uint32_t emitted = 0;
+    const uint32_t slot0 = WRITE ? A.offset[gid] : 0u;
+    for (int face = 0; face < 2; face++) {
+        if (!(info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT))) continue;
+        Vtx v[3];
+        for (int k = 0; k < 3; k++) v[k] = base[k];
+        if (st.flags & PFF_LIGHTING) {                                  // triangles.c:78-104
+            for (int k = 0; k < 3; k++) {
+                const float* m = st.matNormal;
+                float nx = m[0] * v[k].nrm[0] + m[4] * v[k].nrm[1] + m[8] * v[k].nrm[2] + m[12];
+                float ny = m[1] * v[k].nrm[0] + m[5] * v[k].nrm[1] + m[9] * v[k].nrm[2] + m[13];
+                float nz = m[2] * v[k].nrm[0] + m[6] * v[k].nrm[1] + m[10] * v[k].nrm[2] + m[14];
+                v[k].nrm[0] = nx; v[k].nrm[1] = ny; v[k].nrm[2] = nz;
+                float sq = nx * nx + ny * ny + nz * nz;
+                if (sq != 0.0f) {
+                    float inv = __fdiv_rn(1.0f, sqrt_cr(sq));
+                    for (int i = 0; i < 3; i++) v[k].nrm[i] = v[k].nrm[i] * inv;
+                }
+                uint32_t dcol = st.material[face].diffuse, cc = 0;
+                for (int i = 0; i < 4; i++) cc |= (uint32_t)((ch(v[k].col, i) * ch(dcol, i)) / 255) << (8 * i);
+                v[k].col = cc;
+                if (st.lightMode == 0) {
+                    float ndv = dot3(v[k].nrm, st.viewRow2);
+                    v[k].col = light_vertex(st, st.material[ndv < 0.0f ? 0 : 1], v[k].col,
+                                            st.viewPos, v[k].pos, v[k].nrm);
+                }
+            }
+        }
+        // Process_ProjectAndClipTriangle (triangles.c:246-280)
+        const float* m = st.mvp;
+        float wsum = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            const float* p = v[k].pos;
+            v[k].hom[0] = m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12] * p[3];
+            v[k].hom[1] = m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13] * p[3];
+            v[k].hom[2] = m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14] * p[3];
+            v[k].hom[3] = m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15] * p[3];
+            wsum += v[k].hom[3];
+        }
+        if (fabsf(wsum - 3.0f) < 1e-5f) {                                 // "2D" path
+            for (int k = 0; k < 3; k++) to_screen(st, v[k]);
+            if (emit_raster<WRITE>(A, st, F, si, face, false, v[0], v[1], v[2], slot0 + emitted)) emitted++;
+            continue;
+        }
+        bool inside = true;                   // clipping returns the polygon unchanged
+        for (int k = 0; k < 3; k++) {
+            const float w = v[k].hom[3];
+            inside = inside && !(w < 1e-5f);
+            for (int ax = 0; ax < 3; ax++)
+                inside = inside && (v[k].hom[ax] <= w) && (-v[k].hom[ax] <= w);
+        }
+        if (inside) {
+            for (int k = 0; k < 3; k++) {
+                v[k].hom[2] = __fdiv_rn(1.0f, v[k].hom[2]);
+                v[k].uv[0] = v[k].uv[0] * v[k].hom[2];
+                v[k].uv[1] = v[k].uv[1] * v[k].hom[2];
+                float invW = __fdiv_rn(1.0f, v[k].hom[3]);
+                v[k].hom[0] *= invW;
+                v[k].hom[1] *= invW;
+                to_screen(st, v[k]);
+            }
+            if (emit_raster<WRITE>(A, st, F, si, face, true, v[0], v[1], v[2], slot0 + emitted)) emitted++;
+            continue;
+        }
+        Vtx poly[12];
+        for (int k = 0; k < 3; k++) poly[k] = v[k];
+        int n = clip_polygon(poly, 3);
+        if (n < 3) continue;
+        for (int k = 0; k < n; k++) {
+            poly[k].hom[2] = __fdiv_rn(1.0f, poly[k].hom[2]);
+            poly[k].uv[0] = poly[k].uv[0] * poly[k].hom[2];
+            poly[k].uv[1] = poly[k].uv[1] * poly[k].hom[2];
+            float invW = __fdiv_rn(1.0f, poly[k].hom[3]);
+            poly[k].hom[0] *= invW;
+            poly[k].hom[1] *= invW;
+            to_screen(st, poly[k]);
+        }
+        for (int i = 0; i < n - 2; i++)
+            if (emit_raster<WRITE>(A, st, F, si, face, true, poly[0], poly[i + 1], poly[i + 2], slot0 + emitted)) emitted++;
+    }
+    if (!WRITE) A.count[gid] = emitted;
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of uint32 (reduce, top, down)
+// ---------------------------------------------------------------------------
+
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Exclusive block scan of one value per thread; returns the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t s = lane < nw ? lds[lane] : 0u;
+        uint32_t si = wave_incl_scan(s);
+        if (lane < nw) lds[lane] = si - s;
+        if (lane == nw - 1) lds[32] = si;
+    }
+    __syncthreads();
+    uint32_t r = inc - v + lds[wave];
+    total = lds[32];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                              uint32_t* __restrict__ blockSums) {
+    __shared__ uint32_t lds[64];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) if (base + i < n) s += in[base + i];
+    uint32_t total;
+    block_excl_scan(s, lds, total);
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
+}
+
+// One block scans all block sums in place (exclusive) and writes the total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* __restrict__ blockSums, uint32_t nb,
+                                                           uint32_t* __restrict__ totalOut) {
+    __shared__ uint32_t lds[64];
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_TILE) {
+        uint32_t base = c0 + threadIdx.x * SCAN_ITEMS;
+        uint32_t v[SCAN_ITEMS], s = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) { v[i] = (base + i < nb) ? blockSums[base + i] : 0u; s += v[i]; }
+        uint32_t total;
+        uint32_t ex = block_excl_scan(s, lds, total) + carry;
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) { if (base + i < nb) blockSums[base + i] = ex; ex += v[i]; }
+        carry += total;
+    }
+    if (threadIdx.x == 0) *totalOut = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* __restrict__ in, uint32_t n,
+                                                            const uint32_t* __restrict__ blockSums,
+                                                            uint32_t* __restrict__ out) {
+    __shared__ uint32_t lds[64];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) { v[i] = (base + i < n) ? in[base + i] : 0u; s += v[i]; }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(s, lds, total) + blockSums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) { if (base + i < n) out[base + i] = ex; ex += v[i]; }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Tile pairs and the stable radix sort
+// ---------------------------------------------------------------------------
+
+
+__global__ __launch_bounds__(256) void k_pair_emit(const RTri* __restrict__ rts, const uint32_t* __restrict__ rtFrame,
+                                                   const DFrame* __restrict__ frames, const uint32_t* __restrict__ pairOffset,
+                                                   uint32_t numRT, uint32_t ts,
+                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= numRT) return;
+    const RTri& t = rts[r];
+    const DFrame F = frames[rtFrame[r]];
+    int32_t cx0 = max(t.xMin, 0), cy0 = max(t.yMin, 0);
+    int32_t cx1 = min(t.xMax - 1, (int32_t)F.W - 1), cy1 = min(t.yMax, (int32_t)F.H - 1);
+    uint32_t tx0 = (uint32_t)cx0 >> ts, tx1 = (uint32_t)cx1 >> ts;
+    uint32_t ty0 = (uint32_t)cy0 >> ts, ty1 = (uint32_t)cy1 >> ts;
+    uint32_t p = pairOffset[r];
+    for (uint32_t ty = ty0; ty <= ty1; ty++)
+        for (uint32_t tx = tx0; tx <= tx1; tx++) {
+            keys[p] = F.tileBase + ty * F.tilesX + tx;
+            vals[p] = r;
+            p++;
+        }
+}
+
+constexpr int RADIX_THREADS = 256;
+constexpr int RADIX_ROUNDS = 16;
+constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ROUNDS;
+
+__global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                              uint32_t nb, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ROUNDS; r++) {
+        size_t i = base + (size_t)r * RADIX_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items of a block are taken in index order, 256 per round;
+// an item's rank among equal digits is its rank within its wave (ballot
+// match on the 8 digit bits) plus the counts of lower waves and of earlier
+// rounds.
+__global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+                                                                 uint32_t n, uint32_t shift, uint32_t nb,
+                                                                 const uint32_t* __restrict__ histScan,
+                                                                 uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t cnt[RADIX_THREADS / 64][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    base[threadIdx.x] = histScan[(size_t)threadIdx.x * nb + blockIdx.x];
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    size_t blockBase = (size_t)blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ROUNDS; r++) {
+        for (int w = 0; w < RADIX_THREADS / 64; w++) cnt[w][threadIdx.x] = 0;
+        __syncthreads();
+        size_t i = blockBase + (size_t)r * RADIX_THREADS + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? keysIn[i] : 0u;
+        uint32_t val = valid ? valsIn[i] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t bb = __ballot((d >> b) & 1u);
+            match &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        uint32_t wrank = __popcll(match & ltMask);
+        if (valid && wrank == 0) cnt[wave][d] = __popcll(match);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + wrank;
+            for (int w = 0; w < wave; w++) pos += cnt[w][d];
+            keysOut[pos] = key;
+            valsOut[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < RADIX_THREADS / 64; w++) add += cnt[w][threadIdx.x];
+        base[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+__global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
+                              uint32_t* __restrict__ tileBegin, uint32_t* __restrict__ tileEnd) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k) tileBegin[k] = (uint32_t)i;
+        if (i == n - 1 || keys[i + 1] != k) tileEnd[k] = (uint32_t)i + 1;
+    }
+}
+
+// Raster triangle -> frame index, for pair emission.
+__global__ void k_rt_frame(const uint32_t* __restrict__ offset, const uint32_t* __restrict__ count, const DSeg* __restrict__ segs,
+                           uint32_t numSegs, uint32_t numPassTris, uint32_t* __restrict__ rtFrame) {
+    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= numPassTris) return;
+    uint32_t n = count[gid];
+    if (!n) return;
+    uint32_t lo = 0, hi = numSegs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
+    }
+    uint32_t f = segs[lo].frame, o = offset[gid];
+    for (uint32_t k = 0; k < n; k++) rtFrame[o + k] = f;
+}
+
+// ---------------------------------------------------------------------------
+// Tile raster
+// ---------------------------------------------------------------------------
+
+struct RasterArgs {
+    const RTri*     rts;
+    const RTriL*    rtl;
+    const PFFstate* states;
+    const DTex*     texs;
+    const DFrame*   frames;
+    const int32_t*  frameSeg;       // per frame: index into segs, -1 if idle this pass
+    const DSeg*     segs;
+    const uint32_t* tileBegin;
+    const uint32_t* tileEnd;
+    const uint32_t* vals;           // sorted pair values (raster triangle ids)
+    uint32_t        numFrames;
+    uint32_t        numTiles;
+    unsigned long long* fragCounter;
+};
+
+// Shades one covered, depth-passing set of lanes of an 8x8 block.
+__device__ __forceinline__ void shade_block(const RasterArgs& A, const RTri& t, uint32_t meta, uint32_t rtIdx,
+                                            int32_t w1, int32_t w2, int32_t w3, bool cov,
+                                            uint32_t& col, float& dep) {
+    float b1 = (float)w1 * t.wInvSum, b2 = (float)w2 * t.wInvSum, b3 = (float)w3 * t.wInvSum;
+    float zq = rcp_x86((t.z[0] * b1 + t.z[1] * b2) + t.z[2] * b3);
+    bool pass = cov;
+    if (meta & META_DEPTH) {
+        uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
+        bool ok;
+        switch (fn) {
+        case 0: case 1: ok = zq == dep; break;
+        case 2: ok = zq < dep; break;
+        case 3: ok = zq <= dep; break;
+        case 4: ok = zq > dep; break;
+        default: ok = zq >= dep; break;
+        }
+        pass = pass && ok;
+    }
+    if (!__builtin_amdgcn_ballot_w64(pass)) return;
+    if (!pass) return;
+    uint32_t frag = (meta & META_SMOOTH) ? bary_smooth(t.c[0], t.c[1], t.c[2], b1, b2, b3)
+                                         : bary_flat(t.c[0], t.c[1], t.c[2], b1, b2, b3);
+    if (meta & META_TEX) {
+        const PFFstate* st = &A.states[t.state];
+        const DTex tex = A.texs[st->texture];
+        float u = (t.uv[0] * b1 + t.uv[2] * b2) + t.uv[4] * b3;
+        float v = (t.uv[1] * b1 + t.uv[3] * b2) + t.uv[5] * b3;
+        if (meta & META_3D) { u = u * zq; v = v * zq; }
+        frag = modulate(sample_tex(tex, u, v), frag);
+    }
+    if (meta & META_PHONG) {
+        const PFFstate* st = &A.states[t.state];
+        const RTriL& l = A.rtl[rtIdx];
+        float N[3], P[3];
+        for (int i = 0; i < 3; i++) {
+            N[i] = (l.n[i] * b1 + l.n[3 + i] * b2) + l.n[6 + i] * b3;
+            P[i] = (l.p[i] * b1 + l.p[3 + i] * b2) + l.p[6 + i] * b3;
+        }
+        frag = light_fragment(st, st->material[(meta & META_BACK) ? 1 : 0], frag, P, N);
+    }
+    if (meta & META_BLEND) frag = blend_op((meta >> META_BMODE_SH) & 7u, frag, col);
+    col = frag;
+    dep = zq;
+}
+
+// One workgroup per tile of TS x TS pixels; NW waves split the tile's 8x8
+// blocks, each lane owning one pixel of each of its blocks.
+template <int TS, int NW>
+__global__ __launch_bounds__(64 * NW) void k_raster(RasterArgs A) {
+    constexpr int BPR = TS / 8;                       // 8x8 blocks per tile row
+    constexpr int PW = (BPR * BPR) / NW;              // blocks per wave
+    const uint32_t tile = blockIdx.x;
+    if (tile >= A.numTiles) return;
+    // Frame of this tile: last frame whose tileBase <= tile.
+    uint32_t lo = 0, hi = A.numFrames;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (A.frames[mid].tileBase <= tile) lo = mid; else hi = mid;
+    }
+    const DFrame F = A.frames[lo];
+    const int32_t segIdx = A.frameSeg[lo];
+    if (segIdx < 0) return;
+    const DSeg sg = A.segs[segIdx];
+    const uint32_t e0 = A.tileBegin[tile], e1 = A.tileEnd[tile];
+    if (e0 == e1 && !sg.clear) return;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t local = tile - F.tileBase;
+    const int tx = (int)(local % F.tilesX), ty = (int)(local / F.tilesX);
+    const size_t npix = (size_t)F.W * F.H;
+    const size_t aligned = npix - (npix % 8);
+
+    int px[PW], py[PW];
+    uint32_t col[PW];
+    float dep[PW];
+    bool in[PW];
+#pragma unroll
+    for (int p = 0; p < PW; p++) {
+        const int blk = wave * PW + p;
+        px[p] = tx * TS + (blk % BPR) * 8 + (lane & 7);
+        py[p] = ty * TS + (blk / BPR) * 8 + (lane >> 3);
+        in[p] = px[p] < (int)F.W && py[p] < (int)F.H;
+        col[p] = 0u;
+        dep[p] = 0.0f;
+        if (in[p]) {
+            size_t i = (size_t)py[p] * F.W + px[p];
+            if (sg.clear && i >= 8 && i < aligned) {         // context.c:693-713
+                col[p] = sg.clearColor;
+                dep[p] = sg.clearDepth;
+            } else {
+                size_t src = (sg.clear && i >= aligned) ? 0 : i;
+                col[p] = F.color[src];
+                dep[p] = F.depth[src];
+            }
+        }
+    }
+
+    uint32_t frags = 0;
+    for (uint32_t e = e0; e < e1; e++) {
+        const uint32_t r = __builtin_amdgcn_readfirstlane(A.vals[e]);
+        const RTri& t = A.rts[r];
+        const uint32_t meta = t.meta;
+#pragma unroll
+        for (int p = 0; p < PW; p++) {
+            const int bx0 = tx * TS + ((wave * PW + p) % BPR) * 8;
+            const int by0 = ty * TS + ((wave * PW + p) / BPR) * 8;
+            if (t.xMax <= bx0 || t.xMin > bx0 + 7 || t.yMax < by0 || t.yMin > by0 + 7) continue;
+            const uint32_t ux = (uint32_t)px[p], uy = (uint32_t)py[p];
+            int32_t w1 = (int32_t)((uint32_t)t.A[0] * ux + (uint32_t)t.B[0] * uy + (uint32_t)t.C[0]);
+            int32_t w2 = (int32_t)((uint32_t)t.A[1] * ux + (uint32_t)t.B[1] * uy + (uint32_t)t.C[1]);
+            int32_t w3 = (int32_t)((uint32_t)t.A[2] * ux + (uint32_t)t.B[2] * uy + (uint32_t)t.C[2]);
+            bool cov = in[p] && ((w1 | w2 | w3) > 0) && px[p] >= t.xMin && px[p] < t.xMax
+                       && py[p] >= t.yMin && py[p] <= t.yMax;
+            uint64_t bal = __builtin_amdgcn_ballot_w64(cov);
+            if (!bal) continue;
+            frags += (uint32_t)__popcll(bal);
+            shade_block(A, t, meta, r, w1, w2, w3, cov, col[p], dep[p]);
+        }
+    }
+
+#pragma unroll
+    for (int p = 0; p < PW; p++) {
+        if (in[p]) {
+            size_t i = (size_t)py[p] * F.W + px[p];
+            F.color[i] = col[p];
+            F.depth[i] = dep[p];
+        }
+    }
+    if (lane == 0 && frags) atomicAdd(A.fragCounter, (unsigned long long)frags);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+thread_local char g_err[512] = "";
+
+void set_error(const char* what, hipError_t e) {
+    snprintf(g_err, sizeof g_err, "%s: %s", what, e == hipSuccess ? "error" : hipGetErrorString(e));
+}
+
+#define HIP_TRY(call, code)                                          \
+    do {                                                             \
+        hipError_t e_ = (call);                                      \
+        if (e_ != hipSuccess) { set_error(#call, e_); return code; } \
+    } while (0)
+
+int g_initState = 0;   // 0 unknown, 1 ok, -1 failed
+
+struct DevBuf {
+    void*  p = nullptr;
+    size_t bytes = 0;
+};
+
+int ensure(DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return 0;
+    if (b.p) {
+        // Kernels already queued may still read the old block.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) { set_error("hipMalloc", e); b.p = nullptr; return PFD_E_NOMEM; }
+    b.bytes = want;
+    return 0;
+}
+
+void release(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+struct PassInfo {
+    std::vector<DSeg> segs;
+    std::vector<int32_t> frameSeg;
+    uint32_t numPassTris = 0;
+    uint32_t numRT = 0;        // known after the first run
+    uint32_t numPairs = 0;
+    bool     sized = false;
+    DevBuf   dSegs, dFrameSeg;
+};
+
+}  // namespace
+
+struct PFDjob {
+    hipStream_t stream = nullptr;
+    std::vector<PFFframe> src;            // host frame descriptors (targets)
+    std::vector<DFrame>   frames;
+    DevBuf dFrames;
+    std::vector<DevBuf> targets;          // colour + depth per frame
+    DevBuf pos, col, uv, nrm, info, states, texTable;
+    std::vector<DevBuf> texData;
+    uint32_t numIn = 0;
+    bool hasUV = false, hasNrm = false;
+    std::vector<PassInfo> passes;
+    PFDconfig cfg{};
+    uint32_t tileShift = 3;
+    uint32_t numTiles = 0;
+    // scratch
+    DevBuf cnt, off, rts, rtl, rtFrame, pcnt, poff, keysA, valsA, keysB, valsB;
+    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, fragCounter;
+    uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    // stats
+    uint64_t lastFragments = 0, totalRT = 0, totalPairs = 0;
+};
+
+namespace {
+
+int scan_u32(PFDjob* J, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* totalDev) {
+    uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) nb = 1;
+    int rc = ensure(J->scanBlocks, (size_t)nb * 4);
+    if (rc) return rc;
+    uint32_t* bs = (uint32_t*)J->scanBlocks.p;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, bs);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, bs, nb, totalDev);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, (const uint32_t*)bs, out);
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    return 0;
+}
+
+uint32_t read_total(PFDjob* J, int* rc) {
+    uint32_t v = 0;
+    hipError_t e = hipMemcpyAsync(&v, J->scanTotal.p, 4, hipMemcpyDeviceToHost, J->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+    if (e != hipSuccess) { set_error("read_total", e); *rc = PFD_E_LAUNCH; }
+    return v;
+}
+
+uint32_t grid_for(uint64_t n, uint32_t block) {
+    uint64_t g = (n + block - 1) / block;
+    return g == 0 ? 1u : (uint32_t)g;
+}
+
+template <int TS, int NW>
+void launch_raster(const RasterArgs& ra, uint32_t numTiles, hipStream_t s) {
+    hipLaunchKernelGGL((k_raster<TS, NW>), dim3(numTiles), dim3(64 * NW), 0, s, ra);
+}
+
+// Issues one pass.  With discover set, the raster-triangle and pair counts
+// are read back (synchronising) and scratch buffers are sized; otherwise
+// the sizes of the previous run are used and nothing synchronises.
+int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
+    int rc = 0;
+    if (P.numPassTris == 0 && P.segs.empty()) return 0;
+    const uint32_t n = P.numPassTris;
+    if ((rc = ensure(J->cnt, (size_t)n * 4 + 4))) return rc;
+    if ((rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
+    if ((rc = ensure(J->scanTotal, 16))) return rc;
+    if ((rc = ensure(J->fragCounter, 16))) return rc;
+
+    SetupArgs sa;
+    sa.pos = (const float4*)J->pos.p;
+    sa.col = (const uint32_t*)J->col.p;
+    sa.uv = J->hasUV ? (const float2*)J->uv.p : nullptr;
+    sa.nrm = J->hasNrm ? (const float*)J->nrm.p : nullptr;
+    sa.info = (const uint32_t*)J->info.p;
+    sa.states = (const PFFstate*)J->states.p;
+    sa.segs = (const DSeg*)P.dSegs.p;
+    sa.frames = (const DFrame*)J->dFrames.p;
+    sa.numSegs = (uint32_t)P.segs.size();
+    sa.numPassTris = n;
+    sa.tileShift = J->tileShift;
+    sa.count = (uint32_t*)J->cnt.p;
+    sa.offset = (const uint32_t*)J->off.p;
+    sa.rts = nullptr;
+    sa.rtl = nullptr;
+    sa.pairCount = nullptr;
+
+    if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
+    uint32_t numRT = 0;
+    if (n) {
+        hipLaunchKernelGGL((k_setup<false>), dim3(grid_for(n, 256)), dim3(256), 0, J->stream, sa);
+        if ((rc = scan_u32(J, (const uint32_t*)J->cnt.p, (uint32_t*)J->off.p, n, (uint32_t*)J->scanTotal.p))) return rc;
+        if (discover) { numRT = read_total(J, &rc); if (rc) return rc; P.numRT = numRT; }
+        else numRT = P.numRT;
+    }
+    if ((rc = ensure(J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
+    if ((rc = ensure(J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
+    if ((rc = ensure(J->rtFrame, (size_t)numRT * 4 + 4))) return rc;
+    if ((rc = ensure(J->pcnt, (size_t)numRT * 4 + 4))) return rc;
+    if ((rc = ensure(J->poff, (size_t)numRT * 4 + 4))) return rc;
+    sa.rts = (RTri*)J->rts.p;
+    sa.rtl = (RTriL*)J->rtl.p;
+    sa.pairCount = (uint32_t*)J->pcnt.p;
+    uint32_t numPairs = 0;
+    if (numRT) {
+        hipLaunchKernelGGL((k_setup<true>), dim3(grid_for(n, 256)), dim3(256), 0, J->stream, sa);
+        hipLaunchKernelGGL(k_rt_frame, dim3(grid_for(n, 256)), dim3(256), 0, J->stream,
+                           (const uint32_t*)J->off.p, (const uint32_t*)J->cnt.p, (const DSeg*)P.dSegs.p,
+                           (uint32_t)P.segs.size(), n, (uint32_t*)J->rtFrame.p);
+        HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+        if ((rc = scan_u32(J, (const uint32_t*)J->pcnt.p, (uint32_t*)J->poff.p, numRT, (uint32_t*)J->scanTotal.p))) return rc;
+        if (discover) { numPairs = read_total(J, &rc); if (rc) return rc; P.numPairs = numPairs; }
+        else numPairs = P.numPairs;
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
+
+    // Pairs, sort, tile bounds.
+    const size_t pb = (size_t)numPairs * 4 + 4;
+    if ((rc = ensure(J->keysA, pb)) || (rc = ensure(J->valsA, pb)) || (rc = ensure(J->keysB, pb)) || (rc = ensure(J->valsB, pb)))
+        return rc;
+    if ((rc = ensure(J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J->tileEnd, (size_t)J->numTiles * 4 + 4)))
+        return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J->numTiles, 256 * 4)), dim3(256), 0, J->stream,
+                       (uint32_t*)J->tileBegin.p, J->numTiles, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J->numTiles, 256 * 4)), dim3(256), 0, J->stream,
+                       (uint32_t*)J->tileEnd.p, J->numTiles, 0u);
+    uint32_t* keys = (uint32_t*)J->keysA.p;
+    uint32_t* vals = (uint32_t*)J->valsA.p;
+    if (numPairs) {
+        hipLaunchKernelGGL(k_pair_emit, dim3(grid_for(numRT, 256)), dim3(256), 0, J->stream,
+                           (const RTri*)J->rts.p, (const uint32_t*)J->rtFrame.p, (const DFrame*)J->dFrames.p,
+                           (const uint32_t*)J->poff.p, numRT, J->tileShift, keys, vals);
+        uint32_t bits = 1;
+        while (bits < 32 && (1u << bits) < J->numTiles) bits++;
+        uint32_t nb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;
+        if ((rc = ensure(J->hist, (size_t)nb * 256 * 4)) || (rc = ensure(J->histScan, (size_t)nb * 256 * 4 + 4))) return rc;
+        uint32_t* kIn = keys; uint32_t* vIn = vals;
+        uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
+        for (uint32_t shift = 0; shift < bits; shift += 8) {
+            hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, nb, (uint32_t*)J->hist.p);
+            if ((rc = scan_u32(J, (const uint32_t*)J->hist.p, (uint32_t*)J->histScan.p, nb * 256, (uint32_t*)J->scanTotal.p))) return rc;
+            hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift, nb,
+                               (const uint32_t*)J->histScan.p, kOut, vOut);
+            std::swap(kIn, kOut);
+            std::swap(vIn, vOut);
+        }
+        keys = kIn;
+        vals = vIn;
+        hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(numPairs, 256 * 4)), dim3(256), 0, J->stream,
+                           keys, numPairs, (uint32_t*)J->tileBegin.p, (uint32_t*)J->tileEnd.p);
+    }
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    if (ev) HIP_TRY(hipEventRecord(ev[2], J->stream), PFD_E_LAUNCH);
+
+    RasterArgs ra;
+    ra.rts = (const RTri*)J->rts.p;
+    ra.rtl = (const RTriL*)J->rtl.p;
+    ra.states = (const PFFstate*)J->states.p;
+    ra.texs = (const DTex*)J->texTable.p;
+    ra.frames = (const DFrame*)J->dFrames.p;
+    ra.frameSeg = (const int32_t*)P.dFrameSeg.p;
+    ra.segs = (const DSeg*)P.dSegs.p;
+    ra.tileBegin = (const uint32_t*)J->tileBegin.p;
+    ra.tileEnd = (const uint32_t*)J->tileEnd.p;
+    ra.vals = vals;
+    ra.numFrames = (uint32_t)J->frames.size();
+    ra.numTiles = J->numTiles;
+    ra.fragCounter = (unsigned long long*)J->fragCounter.p;
+    switch (J->tileShift) {
+    case 3: launch_raster<8, 1>(ra, J->numTiles, J->stream); break;
+    case 4: launch_raster<16, 1>(ra, J->numTiles, J->stream); break;
+    case 5: launch_raster<32, 4>(ra, J->numTiles, J->stream); break;
+    default: launch_raster<64, 4>(ra, J->numTiles, J->stream); break;
+    }
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
+    P.sized = true;
+    return 0;
+}
+
+int reset_counters(PFDjob* J) {
+    int rc = ensure(J->fragCounter, 16);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->fragCounter.p, 4u, 0u);
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    return 0;
+}
+
+int run_all(PFDjob* J, bool discover) {
+    int rc = reset_counters(J);
+    if (rc) return rc;
+    for (auto& P : J->passes)
+        if ((rc = run_pass(J, P, discover || !P.sized, nullptr))) return rc;
+    return 0;
+}
+
+void layout_tiles(PFDjob* J) {
+    uint32_t ts = J->tileShift, base = 0;
+    for (auto& F : J->frames) {
+        F.tilesX = (F.W + (1u << ts) - 1) >> ts;
+        F.tilesY = (F.H + (1u << ts) - 1) >> ts;
+        F.tileBase = base;
+        base += F.tilesX * F.tilesY;
+    }
+    J->numTiles = base;
+}
+
+int upload_frames_table(PFDjob* J) {
+    int rc = ensure(J->dFrames, J->frames.size() * sizeof(DFrame));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(J->dFrames.p, J->frames.data(), J->frames.size() * sizeof(DFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
+    return 0;
+}
+
+void drop_graph(PFDjob* J) {
+    if (J->graphExec) { (void)hipGraphExecDestroy(J->graphExec); J->graphExec = nullptr; }
+    if (J->graph) { (void)hipGraphDestroy(J->graph); J->graph = nullptr; }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int pfdInit(void) {
+    if (g_initState == 1) return PFD_OK;
+    if (g_initState == -1) return PFD_E_NODEVICE;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        set_error("hipGetDeviceCount (no GPU visible)", e);
+        g_initState = -1;
+        return PFD_E_NODEVICE;
+    }
+    e = hipSetDevice(0);
+    if (e != hipSuccess) { set_error("hipSetDevice", e); g_initState = -1; return PFD_E_NODEVICE; }
+    g_initState = 1;
+    return PFD_OK;
+}
+
+const char* pfdLastError(void) { return g_err; }
+
+int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out) {
+    if (!out || !frames || count == 0) { snprintf(g_err, sizeof g_err, "pfdJobCreate: bad arguments"); return PFD_E_INVALID; }
+    *out = nullptr;
+    int rc = pfdInit();
+    if (rc) return rc;
+    PFDjob* J = new PFDjob();
+    if (cfg) J->cfg = *cfg;
+    J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
+    hipError_t e = hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { set_error("hipStreamCreate", e); delete J; return PFD_E_NODEVICE; }
+
+    // Concatenate triangle streams, states and textures of all frames.
+    uint32_t totalTris = 0, totalStates = 0;
+    bool anyUV = false, anyNrm = false;
+    for (uint32_t f = 0; f < count; f++) {
+        const PFFframe& fr = frames[f];
+        if (!fr.color || !fr.depth || fr.width == 0 || fr.height == 0) {
+            snprintf(g_err, sizeof g_err, "frame %u has no render target", f);
+            pfdJobDestroy(J);
+            return PFD_E_INVALID;
+        }
+        totalTris += fr.tris.count;
+        totalStates += fr.numStates;
+        anyUV |= fr.tris.texcoord != nullptr;
+        anyNrm |= fr.tris.normal != nullptr;
+    }
+    J->numIn = totalTris;
+    J->hasUV = anyUV;
+    J->hasNrm = anyNrm;
+    std::vector<float> hpos((size_t)totalTris * 12);
+    std::vector<uint32_t> hcol((size_t)totalTris * 3), hinfo(totalTris);
+    std::vector<float> huv(anyUV ? (size_t)totalTris * 6 : 0), hnrm(anyNrm ? (size_t)totalTris * 9 : 0);
+    std::vector<PFFstate> hstates(totalStates ? totalStates : 1);
+    std::vector<DTex> htex;
+    uint32_t triBase = 0, stateBase = 0;
+    J->src.assign(frames, frames + count);
+    J->frames.resize(count);
+    J->targets.resize(2 * count);
+    for (uint32_t f = 0; f < count; f++) {
+        const PFFframe& fr = frames[f];
+        size_t nt = fr.tris.count;
+        if (nt) {
+            memcpy(&hpos[(size_t)triBase * 12], fr.tris.position, nt * 12 * sizeof(float));
+            memcpy(&hcol[(size_t)triBase * 3], fr.tris.color, nt * 3 * sizeof(uint32_t));
+            if (anyUV && fr.tris.texcoord) memcpy(&huv[(size_t)triBase * 6], fr.tris.texcoord, nt * 6 * sizeof(float));
+            if (anyNrm && fr.tris.normal) memcpy(&hnrm[(size_t)triBase * 9], fr.tris.normal, nt * 9 * sizeof(float));
+            for (size_t t = 0; t < nt; t++) {
+                uint32_t si = fr.tris.info[t] & PFF_STATE_MASK;
+                if (si >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad state index"); pfdJobDestroy(J); return PFD_E_INVALID; }
+                hinfo[triBase + t] = (fr.tris.info[t] & ~PFF_STATE_MASK) | (si + stateBase);
+            }
+        }
+        uint32_t texBase = (uint32_t)htex.size();
+        for (uint32_t k = 0; k < fr.numTextures; k++) {
+            const PFFtexture& T = fr.textures[k];
+            DTex d;
+            memset(&d, 0, sizeof d);
+            d.w = T.width; d.h = T.height; d.format = T.format; d.wrap = T.wrap; d.filter = T.filter;
+            size_t bytes = (size_t)T.width * T.height * (T.format == PFF_TEX_RGBA8 ? 4 : 16);
+            J->texData.emplace_back();
+            if ((rc = ensure(J->texData.back(), bytes ? bytes : 4))) { pfdJobDestroy(J); return rc; }
+            if (bytes) {
+                e = hipMemcpy(J->texData.back().p, T.pixels, bytes, hipMemcpyHostToDevice);
+                if (e != hipSuccess) { set_error("texture upload", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+            }
+            d.pixels = J->texData.back().p;
+            htex.push_back(d);
+        }
+        for (uint32_t s = 0; s < fr.numStates; s++) {
+            PFFstate st = fr.states[s];
+            if (st.texture >= 0) {
+                if ((uint32_t)st.texture >= fr.numTextures) st.texture = -1;
+                else st.texture += (int32_t)texBase;
+            }
+            hstates[stateBase + s] = st;
+        }
+        DFrame& D = J->frames[f];
+        memset(&D, 0, sizeof D);
+        D.W = fr.width;
+        D.H = fr.height;
+        size_t npx = (size_t)fr.width * fr.height;
+        if ((rc = ensure(J->targets[2 * f], npx * 4)) || (rc = ensure(J->targets[2 * f + 1], npx * 4))) { pfdJobDestroy(J); return rc; }
+        D.color = (uint32_t*)J->targets[2 * f].p;
+        D.depth = (float*)J->targets[2 * f + 1].p;
+
+        // Commands -> segments; the k-th segment of a frame runs in pass k.
+        uint32_t segNo = 0;
+        DSeg cur;
+        memset(&cur, 0, sizeof cur);
+        cur.frame = f;
+        bool open = false, hasDraw = false;
+        auto flush_seg = [&](void) {
+            if (!open) return;
+            if (J->passes.size() <= segNo) J->passes.emplace_back();
+            J->passes[segNo].segs.push_back(cur);
+            segNo++;
+            memset(&cur, 0, sizeof cur);
+            cur.frame = f;
+            open = false;
+            hasDraw = false;
+        };
+        for (uint32_t c = 0; c < fr.numCmds; c++) {
+            const PFFcmd& cmd = fr.cmds[c];
+            if (cmd.kind == PFF_CMD_CLEAR) {
+                if (hasDraw) flush_seg();
+                cur.clear = 1;
+                cur.clearColor = cmd.clearColor;
+                cur.clearDepth = cmd.clearDepth;
+                open = true;
+            } else if (cmd.kind == PFF_CMD_DRAW) {
+                if (cmd.first > cmd.last || cmd.last > fr.tris.count) {
+                    snprintf(g_err, sizeof g_err, "bad draw range");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                if (hasDraw && cur.triLast != triBase + cmd.first) flush_seg();
+                if (!hasDraw) { cur.triFirst = triBase + cmd.first; cur.triLast = triBase + cmd.first; }
+                cur.triLast = triBase + cmd.last;
+                hasDraw = true;
+                open = true;
+            }
+        }
+        flush_seg();
+        triBase += (uint32_t)nt;
+        stateBase += fr.numStates;
+    }
+    // Device copies of the inputs.
+    auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
+        int r = ensure(b, bytes ? bytes : 4);
+        if (r) return r;
+        if (bytes) {
+            hipError_t ee = hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice);
+            if (ee != hipSuccess) { set_error("upload", ee); return PFD_E_LAUNCH; }
+        }
+        return 0;
+    };
+    if ((rc = up(J->pos, hpos.data(), hpos.size() * 4)) || (rc = up(J->col, hcol.data(), hcol.size() * 4))
+        || (rc = up(J->info, hinfo.data(), hinfo.size() * 4)) || (rc = up(J->uv, huv.data(), huv.size() * 4))
+        || (rc = up(J->nrm, hnrm.data(), hnrm.size() * 4))
+        || (rc = up(J->states, hstates.data(), hstates.size() * sizeof(PFFstate)))
+        || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex)))) {
+        pfdJobDestroy(J);
+        return rc;
+    }
+    // Per-pass tables.
+    for (auto& P : J->passes) {
+        uint32_t acc = 0;
+        P.frameSeg.assign(count, -1);
+        for (size_t s = 0; s < P.segs.size(); s++) {
+            P.segs[s].passTriBase = acc;
+            acc += P.segs[s].triLast - P.segs[s].triFirst;
+            P.frameSeg[P.segs[s].frame] = (int32_t)s;
+        }
+        P.numPassTris = acc;
+        if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
+            || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
+            pfdJobDestroy(J);
+            return rc;
+        }
+    }
+    layout_tiles(J);
+    if ((rc = upload_frames_table(J))) { pfdJobDestroy(J); return rc; }
+    *out = J;
+    return PFD_OK;
+}
+
+int pfdJobSetConfig(PFDjob* J, const PFDconfig* cfg) {
+    if (!J || !cfg) return PFD_E_INVALID;
+    J->cfg = *cfg;
+    uint32_t ts = (cfg->tileShift >= 3 && cfg->tileShift <= 6) ? cfg->tileShift : 3;
+    if (ts != J->tileShift) {
+        J->tileShift = ts;
+        layout_tiles(J);
+        int rc = upload_frames_table(J);
+        if (rc) return rc;
+        for (auto& P : J->passes) P.sized = false;
+        drop_graph(J);
+    }
+    return PFD_OK;
+}
+
+// Host render targets -> device.  A frame whose first command is a clear
+// needs only pixels 0..7 (all other pixels are overwritten or copy pixel 0).
+int pfdJobUploadTargets(PFDjob* J) {
+    if (!J) return PFD_E_INVALID;
+    for (size_t f = 0; f < J->frames.size(); f++) {
+        const PFFframe& fr = J->src[f];
+        size_t npx = (size_t)fr.width * fr.height;
+        bool leadClear = fr.numCmds > 0 && fr.cmds[0].kind == PFF_CMD_CLEAR;
+        size_t n = leadClear ? (npx < 8 ? npx : 8) : npx;
+        HIP_TRY(hipMemcpyAsync(J->frames[f].color, fr.color, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpyAsync(J->frames[f].depth, fr.depth, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+    }
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+int pfdJobLaunch(PFDjob* J) {
+    if (!J) return PFD_E_INVALID;
+    bool allSized = true;
+    for (auto& P : J->passes) allSized = allSized && P.sized;
+    if (!allSized) return run_all(J, true);
+    if (!J->graphExec) {
+        HIP_TRY(hipStreamBeginCapture(J->stream, hipStreamCaptureModeThreadLocal), PFD_E_LAUNCH);
+        int rc = run_all(J, false);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(J->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) { set_error("hipStreamEndCapture", e); return PFD_E_LAUNCH; }
+        J->graph = g;
+        e = hipGraphInstantiate(&J->graphExec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { set_error("hipGraphInstantiate", e); drop_graph(J); return PFD_E_LAUNCH; }
+    }
+    HIP_TRY(hipGraphLaunch(J->graphExec, J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+int pfdJobSync(PFDjob* J) {
+    if (!J) return PFD_E_INVALID;
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+int pfdJobDownload(PFDjob* J) {
+    if (!J) return PFD_E_INVALID;
+    for (size_t f = 0; f < J->frames.size(); f++) {
+        const PFFframe& fr = J->src[f];
+        size_t npx = (size_t)fr.width * fr.height;
+        HIP_TRY(hipMemcpyAsync(fr.color, J->frames[f].color, npx * 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpyAsync(fr.depth, J->frames[f].depth, npx * 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+    }
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+int pfdJobRunTimed(PFDjob* J, PFDtimes* t) {
+    if (!J || !t) return PFD_E_INVALID;
+    memset(t, 0, sizeof *t);
+    hipEvent_t ev[4];
+    for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&ev[i]), PFD_E_LAUNCH);
+    int rc = reset_counters(J);
+    for (auto& P : J->passes) {
+        if (rc) break;
+        rc = run_pass(J, P, !P.sized, ev);
+        if (rc) break;
+        HIP_TRY(hipEventSynchronize(ev[3]), PFD_E_LAUNCH);
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+        t->setupMs += a; t->binMs += b; t->rasterMs += c;
+    }
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+    t->totalMs = t->setupMs + t->binMs + t->rasterMs;
+    return rc;
+}
+
+int pfdJobTimeReplays(PFDjob* J, uint32_t iters, float* msTotal) {
+    if (!J || !msTotal) return PFD_E_INVALID;
+    int rc = pfdJobLaunch(J);              // sizes (first call) or warm graph
+    if (rc) return rc;
+    if ((rc = pfdJobLaunch(J))) return rc; // instantiate the graph if needed
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), PFD_E_LAUNCH);
+    HIP_TRY(hipEventCreate(&b), PFD_E_LAUNCH);
+    HIP_TRY(hipEventRecord(a, J->stream), PFD_E_LAUNCH);
+    for (uint32_t i = 0; i < iters; i++) HIP_TRY(hipGraphLaunch(J->graphExec, J->stream), PFD_E_LAUNCH);
+    HIP_TRY(hipEventRecord(b, J->stream), PFD_E_LAUNCH);
+    HIP_TRY(hipEventSynchronize(b), PFD_E_LAUNCH);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *msTotal = ms;
+    return PFD_OK;
+}
+
+int pfdJobStats(PFDjob* J, PFDstats* st) {
+    if (!J || !st) return PFD_E_INVALID;
+    memset(st, 0, sizeof *st);
+    unsigned long long frags = 0;
+    if (J->fragCounter.p) {
+        HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpy(&frags, J->fragCounter.p, 8, hipMemcpyDeviceToHost), PFD_E_LAUNCH);
+    }
+    st->fragments = frags;
+    for (auto& P : J->passes) { st->rasterTriangles += P.numRT; st->tilePairs += P.numPairs; }
+    st->inputTriangles = J->numIn;
+    st->numTiles = J->numTiles;
+    st->tileShift = J->tileShift;
+    size_t bytes = 0;
+    DevBuf* bufs[] = { &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off, &J->rts,
+                       &J->rtl, &J->rtFrame, &J->pcnt, &J->poff, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
+                       &J->scanBlocks, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd };
+    for (DevBuf* b : bufs) bytes += b->bytes;
+    for (auto& b : J->targets) bytes += b.bytes;
+    for (auto& b : J->texData) bytes += b.bytes;
+    st->deviceBytes = bytes;
+    return PFD_OK;
+}
+
+// Times every tile size on the job (one sizing run plus a few graph
+// replays each) and keeps the fastest.  The target buffers are restored
+// from the host before each candidate so the job's output is unchanged.
+int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
+    if (!J) return PFD_E_INVALID;
+    float best = 1e30f;
+    uint32_t bestShift = J->tileShift;
+    for (uint32_t ts = 3; ts <= 6; ts++) {
+        PFDconfig c = J->cfg;
+        c.tileShift = ts;
+        int rc = pfdJobSetConfig(J, &c);
+        if (rc) return rc;
+        if ((rc = pfdJobUploadTargets(J))) return rc;
+        float ms = 0;
+        if ((rc = pfdJobTimeReplays(J, 3, &ms))) return rc;
+        if (ms < best) { best = ms; bestShift = ts; }
+    }
+    PFDconfig c = J->cfg;
+    c.tileShift = bestShift;
+    int rc = pfdJobSetConfig(J, &c);
+    if (rc) return rc;
+    if ((rc = pfdJobUploadTargets(J))) return rc;
+    if (chosen) *chosen = J->cfg;
+    return PFD_OK;
+}
+
+void pfdJobDestroy(PFDjob* J) {
+    if (!J) return;
+    if (J->stream) (void)hipStreamSynchronize(J->stream);
+    drop_graph(J);
+    DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
+                       &J->rts, &J->rtl, &J->rtFrame, &J->pcnt, &J->poff, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
+                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->fragCounter };
+    for (DevBuf* b : bufs) release(*b);
+    for (auto& b : J->targets) release(b);
+    for (auto& b : J->texData) release(b);
+    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); }
+    if (J->stream) (void)hipStreamDestroy(J->stream);
+    delete J;
+}
+
+int pfdExecuteFrames(const PFFframe* frames, uint32_t count) {
+    PFDjob* J = nullptr;
+    int rc = pfdJobCreate(frames, count, nullptr, &J);
+    if (rc) return rc;
+    if (!(rc = pfdJobUploadTargets(J)) && !(rc = pfdJobLaunch(J)) && !(rc = pfdJobSync(J)))
+        rc = pfdJobDownload(J);
+    pfdJobDestroy(J);
+    return rc;
+}
+
+int pfdExecuteFrame(const PFFframe* frame) { return pfdExecuteFrames(frame, 1); }
+
+static PFDjob* g_captured = nullptr;
+
+int pfdCaptureExecutor(const PFFframe* frame, void* user) {
+    (void)user;
+    PFDjob* J = nullptr;
+    int rc = pfdJobCreate(frame, 1, nullptr, &J);
+    if (rc) return rc;
+    if (g_captured) pfdJobDestroy(g_captured);
+    g_captured = J;
+    return PFD_OK;
+}
+
+PFDjob* pfdTakeCapturedJob(void) {
+    PFDjob* J = g_captured;
+    g_captured = nullptr;
+    return J;
+}
+
+}  // extern "C"

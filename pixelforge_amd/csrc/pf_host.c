@@ -1,0 +1,1422 @@
+/*
+ * pf_host.c -- host side of the drop-in: the PixelForge C API state machine.
+ *
+ * Mirrors the reference context (context.c, texture.c, framebuffer.c,
+ * internal/context/context.c) call for call, except that a completed
+ * triangle is appended to the context's recorded frame (pf_frame.h) instead
+ * of being rasterised on the spot.  Frames are executed by the frame
+ * executor -- the GPU (pfdExecuteFrame) unless a harness installed another
+ * -- at the synchronisation points listed in pixelforge.h.
+ *
+ * Floating-point evaluation order follows the reference operation by
+ * operation (no contraction: built with -ffp-contract=off) so that matrices,
+ * texture coordinates and normals are bit-identical to the reference's.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pixelforge.h"
+#include "pf_frame.h"
+#include "pf_ext.h"
+#include "pf_device.h"
+
+#define PF_PI_D        3.14159265358979323846      /* PFM_PI, pfm.h:59 */
+#define PF_DEG2RAD_D   (PF_PI_D / 180.0)           /* PFM_DEG2RAD, pfm.h:62 */
+#define MAX_PROJ_STACK 2                           /* config.h:25-27 */
+#define MAX_MV_STACK   8                           /* config.h:29-31 */
+#define MAX_TEX_STACK  4                           /* config.h:33-35 */
+#define MAX_LIGHTS     8                           /* config.h:37-39 */
+
+/* ------------------------------------------------------------------------ */
+/* Objects                                                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {                       /* struct PFItex, context.h:160-176 */
+    void*           pixels;
+    uint32_t        w, h;
+    PFpixelformat   format;
+    PFdatatype      type;
+    PFtexturewrap   wrap;
+    PFtexturefilter filter;
+} Tex;
+
+typedef struct {                       /* PFIlight, context.h:216-229 */
+    float   position[3], direction[3];
+    float   innerCutOff, outerCutOff;
+    float   attConstant, attLinear, attQuadratic;
+    PFcolor ambient, diffuse, specular;
+    int     next;                      /* index of the next active light, -1 */
+} Light;
+
+typedef struct {                       /* PFImaterial, context.h:234-240 */
+    PFcolor ambient, diffuse, specular, emission;
+    float   shininess;
+} Material;
+
+typedef struct {                       /* PFIvertex, context.h:203-210 */
+    float   position[4];
+    float   normal[3];
+    float   texcoord[2];
+    PFcolor color;
+} Vert;
+
+typedef struct {                       /* the frame being recorded */
+    uint32_t*  targetColor;
+    float*     targetDepth;
+    uint32_t   targetW, targetH;
+    float*     pos;  uint32_t* col;  float* uv;  float* nrm;  uint32_t* info;
+    size_t     ntri, captri;
+    PFFstate*  states;   size_t nstates, capstates;
+    PFFtexture* texs;    const Tex** texKeys; size_t ntex, captex;
+    PFFcmd*    cmds;     size_t ncmd, capcmd;
+    int        anyTex, anyLight;
+} Rec;
+
+typedef struct {
+    PFframebuffer  mainFB;
+    PFframebuffer* boundFB;
+    PFframebuffer* curFB;
+    void*          auxBuffer;
+
+    int32_t  vpPos[2];  uint32_t vpDim[2];  int32_t vpMin[2], vpMax[2];
+
+    uint32_t       state;
+    PFdepthmode    depthFunc;
+    PFblendmode    blendMode;
+    PFshademode    shadeModel;
+    PFlightmode    lightMode;
+    PFface         cullFace;
+    PFpolygonmode  polyMode[2];
+    PFcolor        clearColor;
+    float          clearDepth;
+
+    float matProj[16], matTex[16], matModel[16], matView[16], matMVP[16], matNormal[16];
+    float stackProj[MAX_PROJ_STACK][16], stackMV[MAX_MV_STACK][16], stackTex[MAX_TEX_STACK][16];
+    uint32_t nProj, nMV, nTex;
+    PFmatrixmode matMode;
+    float*   curMat;
+    int      modelUsed;
+
+    Light    lights[MAX_LIGHTS];
+    int      activeHead;               /* -1: no active light */
+    Material mat[2];
+    PFface   followFace;
+    PFenum   followMode;
+
+    float    curNormal[3], curUV[2];
+    PFcolor  curColor;
+    PFdrawmode drawMode;
+    Vert     vb[6];
+    uint32_t vcount;
+    Tex*     curTex;
+
+    PFerrcode err;
+    Rec      rec;
+    int      stateDirty;
+    uint32_t texGen;
+    uint32_t frames;
+} Ctx;
+
+static __thread Ctx* G = NULL;        /* current context, one per thread */
+static PFXexecutor g_exec = NULL;
+static void* g_execUser = NULL;
+static uint32_t g_texGen = 1;          /* bumped by pfTextureParameter */
+
+/* ------------------------------------------------------------------------ */
+/* Matrix helpers (pfm.h); same evaluation order as the reference.           */
+/* ------------------------------------------------------------------------ */
+
+static void m4_identity(float* d)
+{
+    memset(d, 0, 16 * sizeof(float));
+    d[0] = d[5] = d[10] = d[15] = 1.0f;
+}
+
+/* pfmMat4Mul (pfm.h:2176-2198): dst[i*4+j] = sum_k L[i*4+k] * R[k*4+j]. */
+static void m4_mul(float* dst, const float* L, const float* R)
+{
+    float r[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float s = 0.0f;
+            for (int k = 0; k < 4; k++) s += L[i * 4 + k] * R[k * 4 + j];
+            r[i * 4 + j] = s;
+        }
+    memcpy(dst, r, sizeof r);
+}
+
+static void m4_transpose(float* d, const float* s)     /* pfm.h:2053-2066 */
+{
+    float r[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) r[i * 4 + j] = s[j * 4 + i];
+    memcpy(d, r, sizeof r);
+}
+
+/* pfmMat4Invert (pfm.h:2082-2122): cofactor expansion through the twelve
+ * 2x2 minors of the upper and lower row pairs. */
+static void m4_invert(float* d, const float* s)
+{
+    float a00 = s[0],  a01 = s[1],  a02 = s[2],  a03 = s[3];
+    float a10 = s[4],  a11 = s[5],  a12 = s[6],  a13 = s[7];
+    float a20 = s[8],  a21 = s[9],  a22 = s[10], a23 = s[11];
+    float a30 = s[12], a31 = s[13], a32 = s[14], a33 = s[15];
+    float b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10;
+    float b02 = a00 * a13 - a03 * a10, b03 = a01 * a12 - a02 * a11;
+    float b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    float b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30;
+    float b08 = a20 * a33 - a23 * a30, b09 = a21 * a32 - a22 * a31;
+    float b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    float inv = 1.0f / (b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06);
+    d[0]  = (a11 * b11 - a12 * b10 + a13 * b09) * inv;
+    d[1]  = (-a01 * b11 + a02 * b10 - a03 * b09) * inv;
+    d[2]  = (a31 * b05 - a32 * b04 + a33 * b03) * inv;
+    d[3]  = (-a21 * b05 + a22 * b04 - a23 * b03) * inv;
+    d[4]  = (-a10 * b11 + a12 * b08 - a13 * b07) * inv;
+    d[5]  = (a00 * b11 - a02 * b08 + a03 * b07) * inv;
+    d[6]  = (-a30 * b05 + a32 * b02 - a33 * b01) * inv;
+    d[7]  = (a20 * b05 - a22 * b02 + a23 * b01) * inv;
+    d[8]  = (a10 * b10 - a11 * b08 + a13 * b06) * inv;
+    d[9]  = (-a00 * b10 + a01 * b08 - a03 * b06) * inv;
+    d[10] = (a30 * b04 - a31 * b02 + a33 * b00) * inv;
+    d[11] = (-a20 * b04 + a21 * b02 - a23 * b00) * inv;
+    d[12] = (-a10 * b09 + a11 * b07 - a12 * b06) * inv;
+    d[13] = (a00 * b09 - a01 * b07 + a02 * b06) * inv;
+    d[14] = (-a30 * b03 + a31 * b01 - a32 * b00) * inv;
+    d[15] = (a20 * b03 - a21 * b01 + a22 * b00) * inv;
+}
+
+static void m4_ortho(float* d, float l, float r, float b, float t, float n, float f) /* pfm.h:2419-2440 */
+{
+    memset(d, 0, 16 * sizeof(float));
+    float rl = r - l, tb = t - b, fn = f - n;
+    d[0] = 2.0f / rl;
+    d[5] = 2.0f / tb;
+    d[10] = -2.0f / fn;
+    d[11] = 0.0f;
+    d[12] = -(l + r) / rl;
+    d[13] = -(t + b) / tb;
+    d[14] = -(f + n) / fn;
+    d[15] = 1.0f;
+}
+
+static void m4_frustum(float* d, float l, float r, float b, float t, float n, float f) /* pfm.h:2375-2392 */
+{
+    memset(d, 0, 16 * sizeof(float));
+    float rl = r - l, tb = t - b, fn = f - n;
+    d[0] = (n * 2.0f) / rl;
+    d[5] = (n * 2.0f) / tb;
+    d[8] = (r + l) / rl;
+    d[9] = (t + b) / tb;
+    d[10] = -(f + n) / fn;
+    d[11] = -1.0f;
+    d[14] = -(f * n * 2.0f) / fn;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Errors and small helpers                                                  */
+/* ------------------------------------------------------------------------ */
+
+static void set_err(PFerrcode e) { if (G) G->err = e; }
+
+static uint32_t pack(PFcolor c)
+{
+    return (uint32_t)c.r | ((uint32_t)c.g << 8) | ((uint32_t)c.b << 16) | ((uint32_t)c.a << 24);
+}
+
+static PFubyte f2u8(float f)           /* (PFubyte)(float) as gcc emits it on x86-64 */
+{
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return 0;
+    return (PFubyte)(uint32_t)(int32_t)f;
+}
+
+static PFcolor rgb_from_f(const float* v)     /* context.c:951-975: alpha forced to 255 */
+{
+    PFcolor c = { f2u8(v[0] * 255.0f), f2u8(v[1] * 255.0f), f2u8(v[2] * 255.0f), 255 };
+    return c;
+}
+
+static int grow(void** p, size_t* cap, size_t need, size_t elem)
+{
+    if (need <= *cap) return 1;
+    size_t nc = *cap ? *cap : 256;
+    while (nc < need) nc *= 2;
+    void* q = realloc(*p, nc * elem);
+    if (!q) return 0;
+    *p = q;
+    *cap = nc;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Recording                                                                 */
+/* ------------------------------------------------------------------------ */
+
+static void rec_reset(Rec* r)
+{
+    r->ntri = r->nstates = r->ntex = r->ncmd = 0;
+    r->anyTex = r->anyLight = 0;
+    r->targetColor = NULL;
+    r->targetDepth = NULL;
+}
+
+static void rec_free(Rec* r)
+{
+    free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info);
+    free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
+    memset(r, 0, sizeof *r);
+}
+
+static Tex* fb_tex(const PFframebuffer* fb) { return (Tex*)fb->texture; }
+
+static int exec_frame(const PFFframe* f)
+{
+    if (g_exec) return g_exec(f, g_execUser);
+    return pfdExecuteFrame(f);
+}
+
+static void build_frame(Ctx* c, PFFframe* f)
+{
+    Rec* r = &c->rec;
+    memset(f, 0, sizeof *f);
+    f->width = r->targetW;
+    f->height = r->targetH;
+    f->color = r->targetColor;
+    f->depth = r->targetDepth;
+    f->tris.count = (uint32_t)r->ntri;
+    f->tris.position = r->pos;
+    f->tris.color = r->col;
+    f->tris.texcoord = r->anyTex ? r->uv : NULL;
+    f->tris.normal = r->anyLight ? r->nrm : NULL;
+    f->tris.info = r->info;
+    f->states = r->states;
+    f->numStates = (uint32_t)r->nstates;
+    f->textures = r->texs;
+    f->numTextures = (uint32_t)r->ntex;
+    f->cmds = r->cmds;
+    f->numCmds = (uint32_t)r->ncmd;
+}
+
+/* Executes the recorded frame, if any, and starts an empty one. */
+static void submit(Ctx* c)
+{
+    if (!c || c->rec.ncmd == 0) { if (c) rec_reset(&c->rec); return; }
+    PFFframe f;
+    build_frame(c, &f);
+    int rc = exec_frame(&f);
+    if (rc != 0) {
+        c->err = (rc == PFD_E_NOMEM) ? PF_ERROR_OUT_OF_MEMORY : PF_INVALID_OPERATION;
+        if (!g_exec) fprintf(stderr, "pixelforge_amd: frame execution failed (%d): %s\n", rc, pfdLastError());
+    }
+    c->frames++;
+    rec_reset(&c->rec);
+    c->stateDirty = 1;
+}
+
+/* Makes sure the recorded frame targets the current framebuffer; a frame
+ * for another target is executed first.  Returns 0 if there is no valid
+ * render target. */
+static int ensure_target(Ctx* c)
+{
+    Tex* t = fb_tex(c->curFB);
+    if (!t || !t->pixels || !c->curFB->zbuffer) return 0;
+    Rec* r = &c->rec;
+    if (r->ncmd && (r->targetColor != (uint32_t*)t->pixels || r->targetDepth != c->curFB->zbuffer
+                    || r->targetW != t->w || r->targetH != t->h))
+        submit(c);
+    if (r->ncmd == 0) {
+        r->targetColor = (uint32_t*)t->pixels;
+        r->targetDepth = c->curFB->zbuffer;
+        r->targetW = t->w;
+        r->targetH = t->h;
+    }
+    return 1;
+}
+
+static int rec_texture(Ctx* c, const Tex* t)
+{
+    Rec* r = &c->rec;
+    for (size_t i = 0; i < r->ntex; i++)
+        if (r->texKeys[i] == t && r->texs[i].wrap == (uint32_t)t->wrap
+            && r->texs[i].filter == (uint32_t)t->filter && r->texs[i].pixels == t->pixels)
+            return (int)i;
+    size_t cap = r->captex;
+    if (!grow((void**)&r->texs, &cap, r->ntex + 1, sizeof(PFFtexture))) return -1;
+    cap = r->captex;
+    if (!grow((void**)&r->texKeys, &cap, r->ntex + 1, sizeof(Tex*))) return -1;
+    r->captex = cap;
+    PFFtexture* e = &r->texs[r->ntex];
+    memset(e, 0, sizeof *e);
+    e->pixels = t->pixels;
+    e->width = t->w;
+    e->height = t->h;
+    e->format = (t->type == PF_FLOAT) ? PFF_TEX_RGBAF32 : PFF_TEX_RGBA8;
+    e->wrap = t->wrap;
+    e->filter = t->filter;
+    r->texKeys[r->ntex] = t;
+    return (int)r->ntex++;
+}
+
+/* Snapshot of everything pfiProcessRasterize_TRIANGLE_IMPL and
+ * Rasterize_Triangle read from the context (triangles.c:62-116, 287-396). */
+static int current_state(Ctx* c)
+{
+    Rec* r = &c->rec;
+    if (!c->stateDirty && r->nstates && c->texGen == g_texGen) return (int)r->nstates - 1;
+    PFFstate s;
+    memset(&s, 0, sizeof s);
+    memcpy(s.mvp, c->matMVP, sizeof s.mvp);
+    memcpy(s.matNormal, c->matNormal, sizeof s.matNormal);
+    memcpy(s.viewRow2, c->matView + 8, 3 * sizeof(float));
+    int lit = (c->state & PF_LIGHTING) && c->activeHead >= 0;
+    if (lit) {                                           /* triangles.c:82-86 */
+        float inv[16];
+        m4_invert(inv, c->matView);
+        memcpy(s.viewPos, inv + 12, 3 * sizeof(float));
+    }
+    memcpy(s.vpPos, c->vpPos, sizeof s.vpPos);
+    memcpy(s.vpDim, c->vpDim, sizeof s.vpDim);
+    memcpy(s.vpMin, c->vpMin, sizeof s.vpMin);
+    memcpy(s.vpMax, c->vpMax, sizeof s.vpMax);
+    s.flags = c->state & (PFF_TEXTURE_2D | PFF_BLEND | PFF_DEPTH_TEST);
+    if (lit) s.flags |= PFF_LIGHTING;
+    s.depthFunc = c->depthFunc;
+    s.blendMode = c->blendMode;
+    s.shadeModel = c->shadeModel;
+    s.lightMode = c->lightMode;
+    s.texture = -1;
+    if ((c->state & PF_TEXTURE_2D) && c->curTex) {
+        s.texture = rec_texture(c, c->curTex);
+        if (s.texture < 0) return -1;
+        r->anyTex = 1;
+    }
+    for (int f = 0; f < 2; f++) {
+        s.material[f].ambient = pack(c->mat[f].ambient);
+        s.material[f].diffuse = pack(c->mat[f].diffuse);
+        s.material[f].specular = pack(c->mat[f].specular);
+        s.material[f].emission = pack(c->mat[f].emission);
+        s.material[f].shininess = c->mat[f].shininess;
+    }
+    if (lit) {
+        r->anyLight = 1;
+        for (int li = c->activeHead; li >= 0 && s.numLights < MAX_LIGHTS; li = c->lights[li].next) {
+            const Light* L = &c->lights[li];
+            PFFlight* d = &s.lights[s.numLights++];
+            memcpy(d->position, L->position, sizeof d->position);
+            memcpy(d->direction, L->direction, sizeof d->direction);
+            d->innerCutOff = L->innerCutOff;
+            d->outerCutOff = L->outerCutOff;
+            d->attConstant = L->attConstant;
+            d->attLinear = L->attLinear;
+            d->attQuadratic = L->attQuadratic;
+            d->ambient = pack(L->ambient);
+            d->diffuse = pack(L->diffuse);
+            d->specular = pack(L->specular);
+        }
+    }
+    if (r->nstates && memcmp(&r->states[r->nstates - 1], &s, sizeof s) == 0) {
+        c->stateDirty = 0;
+        c->texGen = g_texGen;
+        return (int)r->nstates - 1;
+    }
+    if (!grow((void**)&r->states, &r->capstates, r->nstates + 1, sizeof(PFFstate))) return -1;
+    r->states[r->nstates] = s;
+    c->stateDirty = 0;
+    c->texGen = g_texGen;
+    return (int)r->nstates++;
+}
+
+static void push_cmd(Ctx* c, const PFFcmd* cmd)
+{
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->cmds, &r->capcmd, r->ncmd + 1, sizeof(PFFcmd))) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    r->cmds[r->ncmd++] = *cmd;
+}
+
+/* Appends one triangle with the given culling passes (PFF_PASS_*). */
+static void emit_tri(Ctx* c, const Vert* a, const Vert* b, const Vert* v3, uint32_t passes)
+{
+    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); return; }
+    int si = current_state(c);
+    if (si < 0) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    Rec* r = &c->rec;
+    if (r->ntri + 1 > r->captri) {
+        size_t cap = r->captri;
+        size_t need = r->ntri + 1;
+        size_t nc = cap ? cap : 1024;
+        while (nc < need) nc *= 2;
+        float* pos = realloc(r->pos, nc * 12 * sizeof(float));
+        if (pos) r->pos = pos;
+        uint32_t* col = realloc(r->col, nc * 3 * sizeof(uint32_t));
+        if (col) r->col = col;
+        float* uv = realloc(r->uv, nc * 6 * sizeof(float));
+        if (uv) r->uv = uv;
+        float* nrm = realloc(r->nrm, nc * 9 * sizeof(float));
+        if (nrm) r->nrm = nrm;
+        uint32_t* info = realloc(r->info, nc * sizeof(uint32_t));
+        if (info) r->info = info;
+        if (!pos || !col || !uv || !nrm || !info) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+        r->captri = nc;
+    }
+    size_t t = r->ntri;
+    const Vert* v[3] = { a, b, v3 };
+    for (int k = 0; k < 3; k++) {
+        size_t vi = t * 3 + k;
+        memcpy(r->pos + 4 * vi, v[k]->position, 16);
+        r->col[vi] = pack(v[k]->color);
+        memcpy(r->uv + 2 * vi, v[k]->texcoord, 8);
+        memcpy(r->nrm + 3 * vi, v[k]->normal, 12);
+    }
+    r->info[t] = (uint32_t)si | passes;
+    r->ntri++;
+    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_DRAW && r->cmds[r->ncmd - 1].last == t) {
+        r->cmds[r->ncmd - 1].last = (uint32_t)r->ntri;
+    } else {
+        PFFcmd cmd;
+        memset(&cmd, 0, sizeof cmd);
+        cmd.kind = PFF_CMD_DRAW;
+        cmd.first = (uint32_t)t;
+        cmd.last = (uint32_t)r->ntri;
+        push_cmd(c, &cmd);
+    }
+}
+
+/* pfiProcessRasterize_TRIANGLE_FAN / _STRIP (triangles.c:127-152) for one
+ * face: records the triangles with that face's pass only. */
+static void emit_fan(Ctx* c, uint32_t passes, int n)
+{
+    for (int i = 0; i < n; i++) emit_tri(c, &c->vb[0], &c->vb[i + 1], &c->vb[i + 2], passes);
+}
+
+static void emit_strip(Ctx* c, uint32_t passes, int n)
+{
+    for (int i = 0; i < n; i++) {
+        if (i % 2 == 0) emit_tri(c, &c->vb[i], &c->vb[i + 1], &c->vb[i + 2], passes);
+        else            emit_tri(c, &c->vb[i + 2], &c->vb[i + 1], &c->vb[i], passes);
+    }
+}
+
+static uint32_t face_pass(int face) { return face == PF_FRONT ? PFF_PASS_FRONT : PFF_PASS_BACK; }
+
+/* pfiProcessAndRasterize (internal/context/context.c:94-244), triangle
+ * modes.  Point and line polygon modes are not part of this library. */
+static void process_primitive(Ctx* c)
+{
+    int culled = (c->state & PF_CULL_FACE) != 0;
+    int face = culled ? !c->cullFace : PF_FRONT_AND_BACK;
+    switch (c->drawMode) {
+    case PF_TRIANGLES:
+        if (face == PF_FRONT_AND_BACK) {
+            uint32_t passes = 0;
+            if (c->polyMode[0] == PF_FILL) passes |= PFF_PASS_FRONT;
+            if (c->polyMode[1] == PF_FILL) passes |= PFF_PASS_BACK;
+            if (passes != (PFF_PASS_FRONT | PFF_PASS_BACK)) set_err(PF_INVALID_OPERATION);
+            if (passes) emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], passes);
+        } else if (c->polyMode[face] == PF_FILL) {
+            emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], face_pass(face));
+        } else set_err(PF_INVALID_OPERATION);
+        break;
+    case PF_TRIANGLE_FAN:
+    case PF_QUAD_FAN:
+    case PF_TRIANGLE_STRIP:
+    case PF_QUAD_STRIP: {
+        int n = (c->drawMode == PF_TRIANGLE_FAN || c->drawMode == PF_TRIANGLE_STRIP) ? 2 : 4;
+        int fan = (c->drawMode == PF_TRIANGLE_FAN || c->drawMode == PF_QUAD_FAN);
+        for (int f = 0; f < 2; f++) {
+            if (face != PF_FRONT_AND_BACK && face != f) continue;
+            if (fan) emit_fan(c, face_pass(f), n); else emit_strip(c, face_pass(f), n);
+        }
+        break;
+    }
+    case PF_QUADS:
+        for (int f = 0; f < 2; f++) {
+            if (face != PF_FRONT_AND_BACK && face != f) continue;
+            if (c->polyMode[f] == PF_FILL) emit_fan(c, face_pass(f), 2);
+            else set_err(PF_INVALID_OPERATION);
+        }
+        break;
+    default:                                 /* PF_POINTS, PF_LINES */
+        set_err(PF_INVALID_OPERATION);
+        break;
+    }
+}
+
+static uint32_t mode_vertex_count(PFdrawmode m)   /* context.c:61-74 */
+{
+    switch (m) {
+    case PF_POINTS: return 1;
+    case PF_LINES: return 2;
+    case PF_TRIANGLES: return 3;
+    case PF_TRIANGLE_FAN: case PF_TRIANGLE_STRIP: return 4;
+    case PF_QUADS: return 4;
+    case PF_QUAD_FAN: case PF_QUAD_STRIP: return 6;
+    }
+    return 0;
+}
+
+static void reset_vertex_buffer(Ctx* c)            /* context.c:41-59 */
+{
+    switch (c->drawMode) {
+    case PF_TRIANGLE_FAN:
+    case PF_TRIANGLE_STRIP:
+        c->vcount = 1;
+        c->vb[0] = c->vb[3];
+        break;
+    case PF_QUAD_FAN:
+    case PF_QUAD_STRIP:
+        c->vcount = 2;
+        c->vb[0] = c->vb[4];
+        c->vb[1] = c->vb[5];
+        break;
+    default:
+        c->vcount = 0;
+        break;
+    }
+}
+
+/* pfiUpdateMatrices (context.c:96-111). */
+static void update_matrices(Ctx* c, int withNormal)
+{
+    if (c->modelUsed) {
+        m4_mul(c->matMVP, c->matModel, c->matView);
+        m4_mul(c->matMVP, c->matMVP, c->matProj);
+        if (withNormal && (c->state & PF_LIGHTING)) {
+            m4_invert(c->matNormal, c->matModel);
+            m4_transpose(c->matNormal, c->matNormal);
+        }
+    } else {
+        m4_mul(c->matMVP, c->matView, c->matProj);
+        if (withNormal && (c->state & PF_LIGHTING)) m4_identity(c->matNormal);
+    }
+    c->stateDirty = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Textures (texture.c:30-123)                                               */
+/* ------------------------------------------------------------------------ */
+
+static int format_ok(PFpixelformat f, PFdatatype t)
+{
+    return f == PF_RGBA && (t == PF_UNSIGNED_BYTE || t == PF_FLOAT);
+}
+
+PFtexture pfGenTexture(void* pixels, PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
+{
+    if (!format_ok(format, type)) { set_err(PF_INVALID_ENUM); return NULL; }
+    Tex* t = (Tex*)calloc(1, sizeof(Tex));
+    if (!t) { set_err(PF_ERROR_OUT_OF_MEMORY); return NULL; }
+    t->pixels = pixels;
+    t->w = width;
+    t->h = height;
+    t->format = format;
+    t->type = type;
+    t->wrap = PF_REPEAT;                 /* texture.c:59-60: NEAREST / REPEAT */
+    t->filter = PF_NEAREST;
+    return t;
+}
+
+void pfDeleteTexture(PFtexture* texture, PFboolean freeBuffer)
+{
+    Tex* t = texture ? (Tex*)*texture : NULL;
+    if (!t) return;
+    if (G) {
+        if (G->curTex == t) G->curTex = NULL;
+        if (G->rec.ncmd) submit(G);      /* recorded draws may read it */
+    }
+    if (freeBuffer && t->pixels) free(t->pixels);
+    free(t);
+    *texture = NULL;
+}
+
+PFboolean pfIsValidTexture(const PFtexture texture)
+{
+    const Tex* t = (const Tex*)texture;
+    return t && t->pixels && t->w > 0 && t->h > 0;
+}
+
+void pfTextureParameter(PFtexture texture, PFtexturewrap wrapMode, PFtexturefilter filterMode)
+{
+    Tex* t = (Tex*)texture;
+    if (!(wrapMode <= PF_CLAMP_TO_EDGE && filterMode <= PF_BILINEAR)) { set_err(PF_INVALID_ENUM); return; }
+    if (!t) return;
+    t->wrap = wrapMode;
+    t->filter = filterMode;
+    g_texGen++;
+}
+
+void* pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFsizei* height,
+                         PFpixelformat* format, PFdatatype* type)
+{
+    const Tex* t = (const Tex*)texture;
+    if (!t) return NULL;
+    if (width) *width = t->w;
+    if (height) *height = t->h;
+    if (format) *format = t->format;
+    if (type) *type = t->type;
+    return t->pixels;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Context (context.c:115-385)                                               */
+/* ------------------------------------------------------------------------ */
+
+PFcontext pfCreateContext(void* targetBuffer, PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
+{
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { set_err(PF_INVALID_ENUM); return NULL; }
+    Ctx* c = (Ctx*)calloc(1, sizeof(Ctx));
+    if (!c) return NULL;
+    size_t n = (size_t)width * height;
+    c->mainFB.texture = pfGenTexture(targetBuffer, width, height, format, type);
+    c->mainFB.zbuffer = (float*)malloc((n ? n : 1) * sizeof(float));
+    if (!c->mainFB.texture || !c->mainFB.zbuffer) {
+        free(c->mainFB.texture); free(c->mainFB.zbuffer); free(c);
+        return NULL;
+    }
+    for (size_t i = 0; i < n; i++) c->mainFB.zbuffer[i] = FLT_MAX;    /* context.c:136-138 */
+    c->curFB = &c->mainFB;
+
+    c->vpPos[0] = c->vpMin[0] = 0;                                     /* context.c:146-149 */
+    c->vpPos[1] = c->vpMin[1] = 0;
+    c->vpDim[0] = width - 1;  c->vpMax[0] = (int32_t)(width - 1);
+    c->vpDim[1] = height - 1; c->vpMax[1] = (int32_t)(height - 1);
+
+    c->blendMode = PF_BLEND_ALPHA;                                     /* context.c:155-161 */
+    c->depthFunc = PF_LESS;
+    c->clearColor = (PFcolor){ 0, 0, 0, 255 };
+    c->clearDepth = FLT_MAX;
+    c->polyMode[0] = c->polyMode[1] = PF_FILL;
+    c->curColor = (PFcolor){ 255, 255, 255, 255 };
+
+    for (int i = 0; i < MAX_LIGHTS; i++) {                             /* context.c:184-200 */
+        Light* L = &c->lights[i];
+        memset(L, 0, sizeof *L);
+        L->innerCutOff = (float)PF_PI_D;
+        L->outerCutOff = (float)PF_PI_D;
+        L->attConstant = 1.0f;
+        L->ambient = (PFcolor){ 51, 51, 51, 255 };
+        L->diffuse = (PFcolor){ 255, 255, 255, 255 };
+        L->specular = (PFcolor){ 255, 255, 255, 255 };
+        L->next = -1;
+    }
+    c->activeHead = -1;
+    for (int f = 0; f < 2; f++) {                                      /* context.c:213-223 */
+        c->mat[f].ambient = (PFcolor){ 255, 255, 255, 255 };
+        c->mat[f].diffuse = (PFcolor){ 255, 255, 255, 255 };
+        c->mat[f].specular = (PFcolor){ 255, 255, 255, 255 };
+        c->mat[f].emission = (PFcolor){ 0, 0, 0, 255 };
+        c->mat[f].shininess = 64.0f;
+    }
+    c->followFace = PF_FRONT_AND_BACK;
+    c->followMode = PF_AMBIENT_AND_DIFFUSE;
+
+    c->matMode = PF_MODELVIEW;                                         /* context.c:232-239 */
+    c->curMat = c->matView;
+    m4_ortho(c->matProj, -1.0f, 1.0f, -1.0f, 1.0f, -1.0f, 1.0f);
+    m4_identity(c->matTex);
+    m4_identity(c->matNormal);
+    m4_identity(c->matModel);
+    m4_identity(c->matView);
+
+    c->state |= PF_CULL_FACE;                                          /* context.c:243-247 */
+    c->shadeModel = PF_SMOOTH;
+    c->lightMode = PF_GOURAUD;
+    c->cullFace = PF_BACK;
+    c->err = PF_NO_ERROR;
+    c->stateDirty = 1;
+    return c;
+}
+
+void pfDeleteContext(PFcontext ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return;
+    submit(c);
+    free(c->mainFB.zbuffer);
+    free(c->mainFB.texture);
+    rec_free(&c->rec);
+    if (G == c) G = NULL;
+    free(c);
+}
+
+void pfSetMainBuffer(void* targetBuffer, PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (targetBuffer == NULL || width == 0 || height == 0) { c->err = PF_INVALID_VALUE; return; }
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { c->err = PF_INVALID_ENUM; return; }
+    submit(c);
+    Tex* old = fb_tex(&c->mainFB);
+    uint32_t oldW = old->w, oldH = old->h;
+    if (oldW != width || oldH != height) {
+        /* context.c:279-312, with the depth buffer sized in floats (the
+         * reference reallocates it to width*height bytes). */
+        float* z = (float*)malloc((size_t)width * height * sizeof(float));
+        if (!z) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+        for (uint32_t y = 0; y < height; y++)
+            for (uint32_t x = 0; x < width; x++) {
+                size_t i = (size_t)y * width + x;
+                size_t oi = (size_t)y * width + x;     /* same linear index as the realloc kept */
+                z[i] = (oi < (size_t)oldW * oldH) ? c->mainFB.zbuffer[oi] : c->clearDepth;
+                if (x >= oldW || y >= oldH) z[i] = c->clearDepth;
+            }
+        free(c->mainFB.zbuffer);
+        c->mainFB.zbuffer = z;
+    }
+    old->pixels = targetBuffer;
+    old->w = width;
+    old->h = height;
+    c->auxBuffer = NULL;
+}
+
+void pfSetAuxBuffer(void* auxFramebuffer) { if (G) G->auxBuffer = auxFramebuffer; }
+
+void pfSwapBuffers(void)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!c->auxBuffer) { c->err = PF_INVALID_OPERATION; return; }
+    submit(c);
+    Tex* t = fb_tex(c->curFB);
+    void* tmp = t->pixels;
+    t->pixels = c->auxBuffer;
+    c->auxBuffer = tmp;
+}
+
+PFcontext pfGetCurrentContext(void) { return G; }
+void pfMakeCurrent(PFcontext ctx) { G = (Ctx*)ctx; }
+
+PFboolean pfIsEnabled(PFstate state) { return G && (G->state & state) != 0; }
+
+void pfEnable(PFstate state)
+{
+    Ctx* c = G;
+    if (!c) return;
+    c->state |= state;
+    if (state & PF_FRAMEBUFFER) c->curFB = c->boundFB ? c->boundFB : &c->mainFB;
+    c->stateDirty = 1;
+}
+
+void pfDisable(PFstate state)
+{
+    Ctx* c = G;
+    if (!c) return;
+    c->state &= ~(uint32_t)state;
+    if (state & PF_FRAMEBUFFER) c->curFB = &c->mainFB;
+    c->stateDirty = 1;
+}
+
+PFerrcode pfGetError(void)
+{
+    if (!G) return PF_NO_ERROR;
+    PFerrcode e = G->err;
+    G->err = PF_NO_ERROR;
+    return e;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Matrices (context.c:395-548)                                              */
+/* ------------------------------------------------------------------------ */
+
+void pfMatrixMode(PFmatrixmode mode)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (mode) {
+    case PF_PROJECTION: c->curMat = c->matProj; break;
+    case PF_MODELVIEW:  c->curMat = c->modelUsed ? c->matModel : c->matView; break;
+    case PF_TEXTURE:    c->curMat = c->matTex; break;
+    default: c->err = PF_INVALID_ENUM; return;
+    }
+    c->matMode = mode;
+}
+
+void pfPushMatrix(void)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (c->matMode) {
+    case PF_PROJECTION:
+        if (c->nProj >= MAX_PROJ_STACK) { c->err = PF_STACK_OVERFLOW; return; }
+        memcpy(c->stackProj[c->nProj++], c->matProj, 64);
+        break;
+    case PF_MODELVIEW:
+        if (c->nMV >= MAX_MV_STACK) { c->err = PF_STACK_OVERFLOW; return; }
+        if (c->modelUsed) memcpy(c->stackMV[c->nMV++], c->matModel, 64);
+        else { c->curMat = c->matModel; c->modelUsed = 1; }
+        break;
+    case PF_TEXTURE:
+        if (c->nTex >= MAX_TEX_STACK) { c->err = PF_STACK_OVERFLOW; return; }
+        memcpy(c->stackTex[c->nTex++], c->matTex, 64);
+        break;
+    }
+}
+
+void pfPopMatrix(void)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (c->matMode) {
+    case PF_PROJECTION:
+        if (c->nProj == 0) { c->err = PF_STACK_UNDERFLOW; return; }
+        memcpy(c->matProj, c->stackProj[--c->nProj], 64);
+        break;
+    case PF_MODELVIEW:
+        if (c->nMV == 0) {
+            if (!c->modelUsed) { c->err = PF_STACK_UNDERFLOW; return; }
+            m4_identity(c->matModel);
+            c->modelUsed = 0;
+            c->curMat = c->matView;
+        } else {
+            memcpy(c->matModel, c->stackMV[--c->nMV], 64);
+        }
+        break;
+    case PF_TEXTURE:
+        if (c->nTex == 0) { c->err = PF_STACK_UNDERFLOW; return; }
+        memcpy(c->matTex, c->stackTex[--c->nTex], 64);
+        break;
+    }
+}
+
+void pfLoadIdentity(void) { if (G) m4_identity(G->curMat); }
+
+void pfTranslatef(PFfloat x, PFfloat y, PFfloat z)         /* pfmMat4Translate, pfm.h:2218 */
+{
+    if (!G) return;
+    float t[16];
+    m4_identity(t);
+    t[12] = x; t[13] = y; t[14] = z;
+    m4_mul(G->curMat, t, G->curMat);
+}
+
+void pfRotatef(PFfloat angle, PFfloat x, PFfloat y, PFfloat z)  /* pfmMat4Rotate, pfm.h:2227-2263 */
+{
+    if (!G) return;
+    float a = (float)(angle * PF_DEG2RAD_D);
+    float d[16];
+    m4_identity(d);
+    float lsq = x * x + y * y + z * z;
+    if (lsq != 1.0f && lsq != 0.0f) {
+        float il = 1.0f / sqrtf(lsq);
+        x *= il; y *= il; z *= il;
+    }
+    float sn = sinf(a), cs = cosf(a), t = 1.0f - cs;
+    d[0] = x * x * t + cs;  d[1] = y * x * t + z * sn;  d[2] = z * x * t - y * sn;
+    d[4] = x * y * t - z * sn;  d[5] = y * y * t + cs;  d[6] = z * y * t + x * sn;
+    d[8] = x * z * t + y * sn;  d[9] = y * z * t - x * sn;  d[10] = z * z * t + cs;
+    m4_mul(G->curMat, d, G->curMat);
+}
+
+void pfScalef(PFfloat x, PFfloat y, PFfloat z)             /* pfmMat4Scale, pfm.h:2368 */
+{
+    if (!G) return;
+    float s[16];
+    memset(s, 0, sizeof s);
+    s[0] = x; s[5] = y; s[10] = z; s[15] = 1.0f;
+    m4_mul(G->curMat, s, G->curMat);
+}
+
+void pfMultMatrixf(const PFfloat* mat) { if (G) m4_mul(G->curMat, G->curMat, mat); }
+
+void pfFrustum(PFfloat l, PFfloat r, PFfloat b, PFfloat t, PFfloat n, PFfloat f)
+{
+    if (!G) return;
+    float m[16];
+    m4_frustum(m, l, r, b, t, n, f);
+    m4_mul(G->curMat, G->curMat, m);
+}
+
+void pfOrtho(PFfloat l, PFfloat r, PFfloat b, PFfloat t, PFfloat n, PFfloat f)
+{
+    if (!G) return;
+    float m[16];
+    m4_ortho(m, l, r, b, t, n, f);
+    m4_mul(G->curMat, G->curMat, m);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Render state (context.c:553-797)                                          */
+/* ------------------------------------------------------------------------ */
+
+void pfViewport(PFint x, PFint y, PFsizei width, PFsizei height)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (x <= -(PFint)width || y <= -(PFint)height) { c->err = PF_INVALID_OPERATION; return; }
+    Tex* m = fb_tex(&c->mainFB);
+    c->vpPos[0] = x;
+    c->vpPos[1] = y;
+    c->vpDim[0] = width - 1;
+    c->vpDim[1] = height - 1;
+    c->vpMin[0] = x > 0 ? x : 0;
+    c->vpMin[1] = y > 0 ? y : 0;
+    int64_t mx = (int64_t)x + width, my = (int64_t)y + height;
+    c->vpMax[0] = (int32_t)(mx < (int64_t)m->w - 1 ? mx : (int64_t)m->w - 1);
+    c->vpMax[1] = (int32_t)(my < (int64_t)m->h - 1 ? my : (int64_t)m->h - 1);
+    c->stateDirty = 1;
+}
+
+void pfPolygonMode(PFface face, PFpolygonmode mode)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!(mode == PF_POINT || mode == PF_LINE || mode == PF_FILL)) { c->err = PF_INVALID_ENUM; return; }
+    switch (face) {
+    case PF_FRONT: c->polyMode[0] = mode; break;
+    case PF_BACK: c->polyMode[1] = mode; break;
+    case PF_FRONT_AND_BACK: c->polyMode[0] = c->polyMode[1] = mode; break;
+    default: c->err = PF_INVALID_ENUM; break;
+    }
+}
+
+void pfShadeModel(PFshademode mode) { if (G) { G->shadeModel = mode; G->stateDirty = 1; } }
+void pfLightModel(PFlightmode mode) { if (G) { G->lightMode = mode; G->stateDirty = 1; } }
+
+void pfCullFace(PFface face)
+{
+    if (!G) return;
+    if (face > PF_BACK) { G->err = PF_INVALID_ENUM; return; }
+    G->cullFace = face;
+}
+
+void pfBlendFunc(PFblendmode mode)
+{
+    if (!G) return;
+    if (mode > PF_BLEND_DARKEN) { G->err = PF_INVALID_ENUM; return; }
+    G->blendMode = mode;
+    G->stateDirty = 1;
+}
+
+void pfDepthFunc(PFdepthmode mode)
+{
+    if (!G) return;
+    if (mode > PF_GEQUAL) { G->err = PF_INVALID_ENUM; return; }
+    G->depthFunc = mode;
+    G->stateDirty = 1;
+}
+
+void pfBindFramebuffer(PFframebuffer* framebuffer)
+{
+    Ctx* c = G;
+    if (!c) return;
+    c->boundFB = framebuffer;
+    if (c->state & PF_FRAMEBUFFER) c->curFB = framebuffer ? framebuffer : &c->mainFB;
+}
+
+void pfBindTexture(PFtexture texture)
+{
+    if (!G) return;
+    G->curTex = (Tex*)texture;
+    G->stateDirty = 1;
+}
+
+void pfClear(PFclearflag flag)
+{
+    Ctx* c = G;
+    if (!c || !flag) return;
+    if (!ensure_target(c)) { c->err = PF_INVALID_OPERATION; return; }
+    /* context.c:696: either bit clears colour and depth. */
+    if (!(flag & (PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT))) return;
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_CLEAR;
+    cmd.clearColor = pack(c->clearColor);
+    cmd.clearDepth = c->clearDepth;
+    push_cmd(c, &cmd);
+}
+
+void pfClearDepth(PFfloat depth) { if (G) G->clearDepth = depth; }
+void pfClearColor(PFubyte r, PFubyte g, PFubyte b, PFubyte a) { if (G) G->clearColor = (PFcolor){ r, g, b, a }; }
+
+/* ------------------------------------------------------------------------ */
+/* Lights and materials (context.c:802-1156)                                 */
+/* ------------------------------------------------------------------------ */
+
+void pfEnableLight(PFsizei light)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (light >= MAX_LIGHTS) { c->err = PF_INVALID_VALUE; return; }
+    int* link = &c->activeHead;
+    while (*link >= 0) {
+        if (*link == (int)light) { c->err = PF_INVALID_OPERATION; return; }
+        link = &c->lights[*link].next;
+    }
+    c->lights[light].next = -1;
+    *link = (int)light;
+    c->stateDirty = 1;
+}
+
+void pfDisableLight(PFsizei light)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (light >= MAX_LIGHTS) { c->err = PF_INVALID_VALUE; return; }
+    int* link = &c->activeHead;
+    while (*link >= 0) {
+        if (*link == (int)light) {
+            *link = c->lights[light].next;
+            c->lights[light].next = -1;
+            c->stateDirty = 1;
+            return;
+        }
+        link = &c->lights[*link].next;
+    }
+    c->err = PF_INVALID_OPERATION;
+}
+
+PFboolean pfIsEnabledLight(PFsizei light)
+{
+    Ctx* c = G;
+    if (!c) return PF_FALSE;
+    if (light >= MAX_LIGHTS) { c->err = PF_INVALID_VALUE; return PF_FALSE; }
+    for (int i = c->activeHead; i >= 0; i = c->lights[i].next)
+        if (i == (int)light) return PF_TRUE;
+    return PF_FALSE;
+}
+
+static int cutoff_ok(float v) { return (v >= 0 && v <= 90) || v == 180; }
+
+void pfLightf(PFsizei light, PFenum param, PFfloat value)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (light >= MAX_LIGHTS) { c->err = PF_STACK_OVERFLOW; return; }
+    Light* L = &c->lights[light];
+    switch (param) {
+    case PF_SPOT_INNER_CUTOFF:
+        if (cutoff_ok(value)) L->innerCutOff = cosf((float)(value * PF_DEG2RAD_D)); else c->err = PF_INVALID_VALUE;
+        break;
+    case PF_SPOT_OUTER_CUTOFF:
+        if (cutoff_ok(value)) L->outerCutOff = cosf((float)(value * PF_DEG2RAD_D)); else c->err = PF_INVALID_VALUE;
+        break;
+    case PF_CONSTANT_ATTENUATION: L->attConstant = value; break;
+    case PF_LINEAR_ATTENUATION: L->attLinear = value; break;
+    case PF_QUADRATIC_ATTENUATION: L->attQuadratic = value; break;
+    default: c->err = PF_INVALID_ENUM; return;
+    }
+    c->stateDirty = 1;
+}
+
+void pfLightfv(PFsizei light, PFenum param, const void* value)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (light >= MAX_LIGHTS) { c->err = PF_STACK_OVERFLOW; return; }
+    Light* L = &c->lights[light];
+    const float* v = (const float*)value;
+    switch (param) {
+    case PF_POSITION: memcpy(L->position, v, 12); break;
+    case PF_SPOT_DIRECTION: memcpy(L->direction, v, 12); break;
+    case PF_SPOT_INNER_CUTOFF:
+    case PF_SPOT_OUTER_CUTOFF:
+        pfLightf(light, param, v[0]);
+        return;
+    case PF_CONSTANT_ATTENUATION: L->attConstant = v[0]; break;
+    case PF_LINEAR_ATTENUATION: L->attLinear = v[0]; break;
+    case PF_QUADRATIC_ATTENUATION: L->attQuadratic = v[0]; break;
+    case PF_AMBIENT: L->ambient = rgb_from_f(v); break;
+    case PF_DIFFUSE: L->diffuse = rgb_from_f(v); break;
+    case PF_SPECULAR: L->specular = rgb_from_f(v); break;
+    default: c->err = PF_INVALID_ENUM; return;
+    }
+    c->stateDirty = 1;
+}
+
+static int material_targets(Ctx* c, PFface face, Material** m0, Material** m1)
+{
+    switch (face) {
+    case PF_FRONT: *m0 = *m1 = &c->mat[0]; return 1;
+    case PF_BACK: *m0 = *m1 = &c->mat[1]; return 1;
+    case PF_FRONT_AND_BACK: *m0 = &c->mat[0]; *m1 = &c->mat[1]; return 1;
+    default: c->err = PF_INVALID_ENUM; return 0;
+    }
+}
+
+static void material_set(Ctx* c, PFface face, PFenum param, const float* v3, float scalar, int vec)
+{
+    Material *m0, *m1;
+    if (!material_targets(c, face, &m0, &m1)) return;
+    float s3[3] = { scalar, scalar, scalar };
+    const float* v = vec ? v3 : s3;
+    switch (param) {
+    case PF_AMBIENT: m0->ambient = m1->ambient = rgb_from_f(v); break;
+    case PF_DIFFUSE: m0->diffuse = m1->diffuse = rgb_from_f(v); break;
+    case PF_SPECULAR: m0->specular = m1->specular = rgb_from_f(v); break;
+    case PF_EMISSION: m0->emission = m1->emission = rgb_from_f(v); break;
+    case PF_SHININESS: m0->shininess = m1->shininess = vec ? v3[0] : scalar; break;
+    case PF_AMBIENT_AND_DIFFUSE:
+        m0->ambient = m1->ambient = rgb_from_f(v);
+        m0->diffuse = m1->diffuse = rgb_from_f(v);
+        break;
+    default: c->err = PF_INVALID_ENUM; return;
+    }
+    c->stateDirty = 1;
+}
+
+void pfMaterialf(PFface face, PFenum param, PFfloat value) { if (G) material_set(G, face, param, NULL, value, 0); }
+void pfMaterialfv(PFface face, PFenum param, const void* value) { if (G) material_set(G, face, param, (const float*)value, 0.0f, 1); }
+
+void pfColorMaterial(PFface face, PFenum mode)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (face > PF_FRONT_AND_BACK) { c->err = PF_INVALID_ENUM; return; }
+    if (mode < PF_AMBIENT_AND_DIFFUSE || mode > PF_EMISSION) { c->err = PF_INVALID_ENUM; return; }
+    c->followFace = face;
+    c->followMode = mode;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Immediate mode (context.c:1580-1913)                                      */
+/* ------------------------------------------------------------------------ */
+
+void pfBegin(PFdrawmode mode)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (mode > PF_QUAD_STRIP) { c->err = PF_INVALID_ENUM; return; }
+    update_matrices(c, !(mode == PF_POINTS || mode == PF_LINES));
+    c->drawMode = mode;
+    c->vcount = 0;
+}
+
+void pfEnd(void) { if (G) G->vcount = 0; }
+
+void pfVertex4fv(const PFfloat* v)
+{
+    Ctx* c = G;
+    if (!c) return;
+    Vert* d = &c->vb[c->vcount++];
+    memcpy(d->position, v, 16);
+    memcpy(d->normal, c->curNormal, 12);
+    memcpy(d->texcoord, c->curUV, 8);
+    d->color = c->curColor;
+    if (c->vcount == mode_vertex_count(c->drawMode)) {
+        process_primitive(c);
+        reset_vertex_buffer(c);
+    }
+}
+
+void pfVertex2i(PFint x, PFint y) { float v[4] = { (float)x, (float)y, 0.0f, 1.0f }; pfVertex4fv(v); }
+void pfVertex2f(PFfloat x, PFfloat y) { float v[4] = { x, y, 0.0f, 1.0f }; pfVertex4fv(v); }
+void pfVertex2fv(const PFfloat* p) { float v[4] = { p[0], p[1], 0.0f, 1.0f }; pfVertex4fv(v); }
+void pfVertex3i(PFint x, PFint y, PFint z) { float v[4] = { (float)x, (float)y, (float)z, 1.0f }; pfVertex4fv(v); }
+void pfVertex3f(PFfloat x, PFfloat y, PFfloat z) { float v[4] = { x, y, z, 1.0f }; pfVertex4fv(v); }
+void pfVertex3fv(const PFfloat* p) { float v[4] = { p[0], p[1], p[2], 1.0f }; pfVertex4fv(v); }
+void pfVertex4i(PFint x, PFint y, PFint z, PFint w) { float v[4] = { (float)x, (float)y, (float)z, (float)w }; pfVertex4fv(v); }
+void pfVertex4f(PFfloat x, PFfloat y, PFfloat z, PFfloat w) { float v[4] = { x, y, z, w }; pfVertex4fv(v); }
+
+void pfColor(PFcolor color)                                  /* context.c:1687-1713 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (c->state & PF_COLOR_MATERIAL) {
+        Material* m1 = &c->mat[0];
+        Material* m2 = &c->mat[1];
+        if (c->followFace == PF_FRONT) m2 = m1;
+        else if (c->followFace == PF_BACK) m1 = m2;
+        switch (c->followMode) {
+        case PF_AMBIENT_AND_DIFFUSE: m1->ambient = m2->ambient = color; m1->diffuse = m2->diffuse = color; break;
+        case PF_AMBIENT: m1->ambient = m2->ambient = color; break;
+        case PF_DIFFUSE: m1->diffuse = m2->diffuse = color; break;
+        case PF_SPECULAR: m1->specular = m2->specular = color; break;
+        case PF_EMISSION: m1->emission = m2->emission = color; break;
+        }
+        c->stateDirty = 1;
+    } else {
+        c->curColor = color;
+    }
+}
+
+void pfColor1ui(PFuint color)
+{
+    PFcolor c;
+    memcpy(&c, &color, 4);
+    pfColor(c);
+}
+void pfColor3ub(PFubyte r, PFubyte g, PFubyte b) { pfColor((PFcolor){ r, g, b, 255 }); }
+void pfColor3ubv(const PFubyte* v) { pfColor((PFcolor){ v[0], v[1], v[2], 255 }); }
+void pfColor3us(PFushort r, PFushort g, PFushort b) { pfColor((PFcolor){ (PFubyte)(r >> 8), (PFubyte)(g >> 8), (PFubyte)(b >> 8), 255 }); }
+void pfColor3usv(const PFushort* v) { pfColor3us(v[0], v[1], v[2]); }
+void pfColor3ui(PFuint r, PFuint g, PFuint b) { pfColor((PFcolor){ (PFubyte)(r >> 24), (PFubyte)(g >> 24), (PFubyte)(b >> 24), 255 }); }
+void pfColor3uiv(const PFuint* v) { pfColor3ui(v[0], v[1], v[2]); }
+void pfColor3f(PFfloat r, PFfloat g, PFfloat b) { pfColor((PFcolor){ f2u8(r * 255), f2u8(g * 255), f2u8(b * 255), 255 }); }
+void pfColor3fv(const PFfloat* v) { pfColor3f(v[0], v[1], v[2]); }
+void pfColor4ub(PFubyte r, PFubyte g, PFubyte b, PFubyte a) { pfColor((PFcolor){ r, g, b, a }); }
+void pfColor4ubv(const PFubyte* v) { pfColor((PFcolor){ v[0], v[1], v[2], v[3] }); }
+void pfColor4us(PFushort r, PFushort g, PFushort b, PFushort a) { pfColor((PFcolor){ (PFubyte)(r >> 8), (PFubyte)(g >> 8), (PFubyte)(b >> 8), (PFubyte)(a >> 8) }); }
+void pfColor4usv(const PFushort* v) { pfColor4us(v[0], v[1], v[2], v[3]); }
+void pfColor4ui(PFuint r, PFuint g, PFuint b, PFuint a) { pfColor((PFcolor){ (PFubyte)(r >> 24), (PFubyte)(g >> 24), (PFubyte)(b >> 24), (PFubyte)(a >> 24) }); }
+void pfColor4uiv(const PFuint* v) { pfColor4ui(v[0], v[1], v[2], v[3]); }
+void pfColor4f(PFfloat r, PFfloat g, PFfloat b, PFfloat a) { pfColor((PFcolor){ f2u8(r * 255), f2u8(g * 255), f2u8(b * 255), f2u8(a * 255) }); }
+void pfColor4fv(const PFfloat* v) { pfColor4f(v[0], v[1], v[2], v[3]); }
+
+void pfTexCoord2f(PFfloat u, PFfloat v)                      /* pfmVec2Transform, pfm.h:954-962 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    const float* m = c->matTex;
+    float tu = m[0] * u + m[4] * v + m[12];
+    float tv = m[1] * u + m[5] * v + m[13];
+    c->curUV[0] = tu;
+    c->curUV[1] = tv;
+}
+
+void pfTexCoordfv(const PFfloat* v) { pfTexCoord2f(v[0], v[1]); }
+
+void pfNormal3f(PFfloat x, PFfloat y, PFfloat z)
+{
+    Ctx* c = G;
+    if (!c) return;
+    c->curNormal[0] = x; c->curNormal[1] = y; c->curNormal[2] = z;
+    if (c->state & PF_NORMALIZE) {                           /* pfmVec3Normalize, pfm.h:1183 */
+        float sq = x * x + y * y + z * z;
+        if (sq == 0.0f) return;
+        float inv = 1.0f / sqrtf(sq);
+        for (int i = 0; i < 3; i++) c->curNormal[i] = c->curNormal[i] * inv;
+    }
+}
+
+void pfNormal3fv(const PFfloat* v) { pfNormal3f(v[0], v[1], v[2]); }
+
+/* ------------------------------------------------------------------------ */
+/* Readback and framebuffers                                                 */
+/* ------------------------------------------------------------------------ */
+
+static void finish_target(const void* colorPixels)
+{
+    if (G && G->rec.ncmd && G->rec.targetColor == colorPixels) submit(G);
+}
+
+void pfReadPixels(PFint x, PFint y, PFsizei width, PFsizei height,
+                  PFpixelformat format, PFdatatype type, void* pixels)   /* context.c:2349-2395 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { c->err = PF_INVALID_ENUM; return; }
+    submit(c);
+    const Tex* t = fb_tex(c->curFB);
+    int32_t xMin = x < 0 ? 0 : (x > (int32_t)t->w - 1 ? (int32_t)t->w - 1 : x);
+    int32_t yMin = y < 0 ? 0 : (y > (int32_t)t->h - 1 ? (int32_t)t->h - 1 : y);
+    int64_t xe = (int64_t)x + (int32_t)width, ye = (int64_t)y + (int32_t)height;
+    int32_t xMax = (int32_t)(xe < 0 ? 0 : (xe > t->w ? t->w : xe));
+    int32_t yMax = (int32_t)(ye < 0 ? 0 : (ye > t->h ? t->h : ye));
+    const uint32_t* src = (const uint32_t*)t->pixels;
+    uint32_t* dst = (uint32_t*)pixels;
+    for (int32_t ys = yMin, yd = 0; ys < yMax; ys++, yd++)
+        for (int32_t xs = xMin, xd = 0; xs < xMax; xs++, xd++)
+            dst[(size_t)yd * width + xd] = src[(size_t)ys * t->w + xs];
+}
+
+PFframebuffer pfGenFramebuffer(PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
+{
+    PFframebuffer fb = { NULL, NULL };
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { set_err(PF_INVALID_ENUM); return fb; }
+    size_t n = (size_t)width * height;
+    void* px = calloc(n ? n : 1, 4);
+    if (!px) return fb;
+    PFtexture t = pfGenTexture(px, width, height, format, type);
+    if (!t) { free(px); set_err(PF_INVALID_ENUM); return fb; }
+    float* z = (float*)malloc((n ? n : 1) * sizeof(float));
+    if (!z) { set_err(PF_ERROR_OUT_OF_MEMORY); pfDeleteTexture(&t, PF_TRUE); return fb; }
+    for (size_t i = 0; i < n; i++) z[i] = FLT_MAX;
+    fb.texture = t;
+    fb.zbuffer = z;
+    return fb;
+}
+
+void pfDeleteFramebuffer(PFframebuffer* fb)
+{
+    if (!fb) return;
+    Tex* t = fb_tex(fb);
+    if (t) finish_target(t->pixels);
+    if (G && G->boundFB == fb) G->boundFB = NULL;
+    if (G && G->curFB == fb) G->curFB = &G->mainFB;
+    pfDeleteTexture(&fb->texture, PF_TRUE);
+    free(fb->zbuffer);
+    fb->zbuffer = NULL;
+    fb->texture = NULL;
+}
+
+PFboolean pfIsValidFramebuffer(PFframebuffer* fb)
+{
+    Tex* t = fb ? fb_tex(fb) : NULL;
+    return t && fb->zbuffer && t->w > 0 && t->h > 0 && pfIsValidTexture(fb->texture);
+}
+
+void pfClearFramebuffer(PFframebuffer* fb, PFcolor color, PFfloat depth)   /* framebuffer.c:86-100 */
+{
+    Tex* t = fb_tex(fb);
+    if (!t) return;
+    finish_target(t->pixels);
+    size_t n = (size_t)t->w * t->h;
+    uint32_t c = pack(color);
+    for (size_t i = 0; i < n; i++) { ((uint32_t*)t->pixels)[i] = c; fb->zbuffer[i] = depth; }
+}
+
+PFcolor pfGetFramebufferPixel(const PFframebuffer* fb, PFsizei x, PFsizei y)
+{
+    const Tex* t = fb_tex(fb);
+    finish_target(t->pixels);
+    PFcolor c;
+    memcpy(&c, (const uint32_t*)t->pixels + (size_t)y * t->w + x, 4);
+    return c;
+}
+
+PFfloat pfGetFramebufferDepth(const PFframebuffer* fb, PFsizei x, PFsizei y)
+{
+    const Tex* t = fb_tex(fb);
+    finish_target(t->pixels);
+    return fb->zbuffer[(size_t)y * t->w + x];
+}
+
+/* ------------------------------------------------------------------------ */
+/* New entry points                                                          */
+/* ------------------------------------------------------------------------ */
+
+void pfFinish(void) { if (G) submit(G); }
+void pfFlush(void) { if (G) submit(G); }
+
+void pfSubmitBatch(const PFcontext* contexts, PFsizei count)
+{
+    if (!contexts || count == 0) return;
+    PFFframe* frames = (PFFframe*)calloc(count, sizeof(PFFframe));
+    Ctx** owners = (Ctx**)calloc(count, sizeof(Ctx*));
+    if (!frames || !owners) { free(frames); free(owners); set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    uint32_t n = 0;
+    for (PFsizei i = 0; i < count; i++) {
+        Ctx* c = (Ctx*)contexts[i];
+        if (!c || c->rec.ncmd == 0) continue;
+        build_frame(c, &frames[n]);
+        owners[n++] = c;
+    }
+    int rc = 0;
+    if (n) {
+        if (g_exec) {
+            for (uint32_t i = 0; i < n && rc == 0; i++) rc = g_exec(&frames[i], g_execUser);
+        } else {
+            rc = pfdExecuteFrames(frames, n);
+            if (rc != 0) fprintf(stderr, "pixelforge_amd: batch execution failed (%d): %s\n", rc, pfdLastError());
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        Ctx* c = owners[i];
+        if (rc != 0) c->err = (rc == PFD_E_NOMEM) ? PF_ERROR_OUT_OF_MEMORY : PF_INVALID_OPERATION;
+        c->frames++;
+        rec_reset(&c->rec);
+        c->stateDirty = 1;
+    }
+    free(frames);
+    free(owners);
+}
+
+void pfxSetExecutor(PFXexecutor fn, void* user)
+{
+    g_exec = fn;
+    g_execUser = user;
+}
+
+PFuint pfxFrameCount(void) { return G ? G->frames : 0; }

@@ -1,0 +1,512 @@
+/*
+ * pf_scenes.c -- seeded synthetic scenes issued through the PixelForge C API.
+ *
+ * The same scene code drives any library that exports the PixelForge API:
+ * libpixelforge_amd.so (GPU) or a CPU build of the reference.  The API is
+ * bound at run time with dlsym, so one scene function renders identical call
+ * sequences on every backend.  Randomness comes from SplitMix64 (Steele,
+ * Lea, Flood 2014) seeded per scene; floats are (u >> 40) * 2^-24.
+ *
+ * Scene kinds:
+ *   PFS_RANDOM_NDC  triangles with vertices uniform in NDC [-1,1]^3 and the
+ *                   default matrices (the reference's 2D path), random RGBA8
+ *                   vertex colours; optional texture, blending, depth func.
+ *   PFS_HEIGHTFIELD jittered height-field grid seen through a perspective
+ *                   camera, textured, with a fraction of blended triangles
+ *                   drawn after the opaque ones.
+ *   PFS_SPHERES     cluster of tessellated spheres with normals, lit per
+ *                   fragment (Phong) or per vertex (Gouraud).
+ *   PFS_RANDOM_3D   random triangles in a perspective frustum, including
+ *                   vertices behind the camera and outside the side planes,
+ *                   so that clipping and both culling passes run.
+ */
+#include <dlfcn.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pixelforge.h"
+
+#define PFS_RANDOM_NDC   1
+#define PFS_HEIGHTFIELD  2
+#define PFS_SPHERES      3
+#define PFS_RANDOM_3D    4
+
+/* Scene description; zero fields take the defaults noted. */
+typedef struct {
+    uint32_t kind;
+    uint32_t width, height;
+    uint32_t ntris;          /* random kinds: triangle count                        */
+    uint64_t seed;
+    int32_t  cull;           /* -1: disabled, PF_FRONT/PF_BACK: cull that face       */
+    int32_t  depthTest;      /* 0/1 */
+    int32_t  depthFunc;      /* PFdepthmode */
+    int32_t  blend;          /* 0/1 */
+    int32_t  blendMode;      /* PFblendmode */
+    int32_t  texture;        /* 0 none, 1 RGBA8, 2 RGBA float */
+    int32_t  texW, texH;
+    int32_t  wrap, filter;
+    int32_t  shade;          /* PFshademode */
+    int32_t  lighting;       /* 0 none, 1 Gouraud, 2 Phong */
+    int32_t  numLights;
+    int32_t  spot;           /* 0/1: spot cut-offs on the lights */
+    int32_t  attenuation;    /* 0/1: linear/quadratic attenuation */
+    float    blendFraction;  /* heightfield: fraction of triangles drawn blended     */
+    int32_t  gridX, gridZ;   /* heightfield cells                                    */
+    int32_t  spheres;        /* spheres: count                                       */
+    int32_t  sphereLat, sphereLon;
+    int32_t  clearFirst;     /* 1: pfClear before drawing                            */
+    int32_t  colorMaterial;  /* 1: pfColor drives the material                      */
+    float    triScale;       /* random kinds: triangle size relative to the screen   */
+    int32_t  flushEvery;     /* >0: pfFinish every N triangles (multi-frame)         */
+    int32_t  useQuads;       /* 1: PF_QUADS for grids                                */
+} PFSparams;
+
+typedef struct {
+    void* h;
+    PFcontext (*CreateContext)(void*, PFsizei, PFsizei, PFpixelformat, PFdatatype);
+    void (*DeleteContext)(PFcontext);
+    void (*MakeCurrent)(PFcontext);
+    void (*Enable)(PFstate);
+    void (*Disable)(PFstate);
+    void (*MatrixMode)(PFmatrixmode);
+    void (*LoadIdentity)(void);
+    void (*Translatef)(PFfloat, PFfloat, PFfloat);
+    void (*Rotatef)(PFfloat, PFfloat, PFfloat, PFfloat);
+    void (*Frustum)(PFfloat, PFfloat, PFfloat, PFfloat, PFfloat, PFfloat);
+    void (*CullFace)(PFface);
+    void (*BlendFunc)(PFblendmode);
+    void (*DepthFunc)(PFdepthmode);
+    void (*ShadeModel)(PFshademode);
+    void (*LightModel)(PFlightmode);
+    void (*BindTexture)(PFtexture);
+    void (*BindFramebuffer)(PFframebuffer*);
+    void (*Clear)(PFclearflag);
+    void (*ClearColor)(PFubyte, PFubyte, PFubyte, PFubyte);
+    void (*EnableLight)(PFsizei);
+    void (*Lightf)(PFsizei, PFenum, PFfloat);
+    void (*Lightfv)(PFsizei, PFenum, const void*);
+    void (*Materialf)(PFface, PFenum, PFfloat);
+    void (*Materialfv)(PFface, PFenum, const void*);
+    void (*ColorMaterial)(PFface, PFenum);
+    void (*Begin)(PFdrawmode);
+    void (*End)(void);
+    void (*Vertex3f)(PFfloat, PFfloat, PFfloat);
+    void (*Color4ub)(PFubyte, PFubyte, PFubyte, PFubyte);
+    void (*TexCoord2f)(PFfloat, PFfloat);
+    void (*Normal3f)(PFfloat, PFfloat, PFfloat);
+    PFframebuffer (*GenFramebuffer)(PFsizei, PFsizei, PFpixelformat, PFdatatype);
+    void (*DeleteFramebuffer)(PFframebuffer*);
+    PFtexture (*GenTexture)(void*, PFsizei, PFsizei, PFpixelformat, PFdatatype);
+    void (*DeleteTexture)(PFtexture*, PFboolean);
+    void (*TextureParameter)(PFtexture, PFtexturewrap, PFtexturefilter);
+    void* (*GetTexturePixels)(const PFtexture, PFsizei*, PFsizei*, PFpixelformat*, PFdatatype*);
+    PFerrcode (*GetError)(void);
+    void (*Finish)(void);                    /* absent from the reference */
+    void (*SubmitBatch)(const PFcontext*, PFsizei);   /* absent from the reference */
+} PFSapi;
+
+#define BIND(name) do { a->name = (void*)dlsym(h, "pf" #name); if (!a->name) goto fail; } while (0)
+
+PFSapi* pfs_bind(const char* path)
+{
+    void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return NULL;
+    PFSapi* a = (PFSapi*)calloc(1, sizeof(PFSapi));
+    if (!a) return NULL;
+    a->h = h;
+    BIND(CreateContext); BIND(DeleteContext); BIND(MakeCurrent); BIND(Enable); BIND(Disable);
+    BIND(MatrixMode); BIND(LoadIdentity); BIND(Translatef); BIND(Rotatef); BIND(Frustum);
+    BIND(CullFace); BIND(BlendFunc); BIND(DepthFunc); BIND(ShadeModel); BIND(LightModel);
+    BIND(BindTexture); BIND(BindFramebuffer); BIND(Clear); BIND(ClearColor); BIND(EnableLight);
+    BIND(Lightf); BIND(Lightfv); BIND(Materialf); BIND(Materialfv); BIND(ColorMaterial);
+    BIND(Begin); BIND(End); BIND(Vertex3f); BIND(Color4ub); BIND(TexCoord2f); BIND(Normal3f);
+    BIND(GenFramebuffer); BIND(DeleteFramebuffer); BIND(GenTexture); BIND(DeleteTexture);
+    BIND(TextureParameter); BIND(GetTexturePixels); BIND(GetError);
+    a->Finish = (void (*)(void))dlsym(h, "pfFinish");
+    a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
+    return a;
+fail:
+    free(a);
+    return NULL;
+}
+
+const char* pfs_dlerror(void) { return dlerror(); }
+
+/* ------------------------------------------------------------------------ */
+/* SplitMix64                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint64_t s; } Rng;
+
+static uint64_t rnext(Rng* r)
+{
+    uint64_t z = (r->s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float runit(Rng* r) { return (float)(rnext(r) >> 40) * (1.0f / 16777216.0f); }     /* [0,1) */
+static float rrange(Rng* r, float lo, float hi) { return lo + (hi - lo) * runit(r); }
+static uint8_t rbyte(Rng* r) { return (uint8_t)(rnext(r) >> 56); }
+
+/* Exported so that harnesses can reproduce the stream. */
+uint64_t pfs_splitmix64(uint64_t* state)
+{
+    Rng r = { *state };
+    uint64_t v = rnext(&r);
+    *state = r.s;
+    return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scene pieces                                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    PFtexture tex;
+    void*     pixels;
+} SceneTex;
+
+static SceneTex make_texture(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    SceneTex st = { NULL, NULL };
+    if (!p->texture) return st;
+    uint32_t w = p->texW ? (uint32_t)p->texW : 256, h = p->texH ? (uint32_t)p->texH : 256;
+    size_t n = (size_t)w * h;
+    if (p->texture == 2) {
+        float* px = (float*)malloc(n * 4 * sizeof(float) + 64);
+        for (size_t i = 0; i < 4 * n; i++) px[i] = runit(r);
+        st.pixels = px;
+        st.tex = a->GenTexture(px, w, h, PF_RGBA, PF_FLOAT);
+    } else {
+        uint32_t* px = (uint32_t*)malloc(n * 4 + 64);
+        for (size_t i = 0; i < n; i++) px[i] = (uint32_t)(rnext(r) >> 32);
+        st.pixels = px;
+        st.tex = a->GenTexture(px, w, h, PF_RGBA, PF_UNSIGNED_BYTE);
+    }
+    a->TextureParameter(st.tex, (PFtexturewrap)p->wrap, (PFtexturefilter)p->filter);
+    return st;
+}
+
+static void free_texture(const PFSapi* a, SceneTex* st)
+{
+    if (st->tex) a->DeleteTexture(&st->tex, 0);
+    free(st->pixels);
+    st->pixels = NULL;
+}
+
+static void common_state(const PFSapi* a, const PFSparams* p)
+{
+    if (p->cull < 0) a->Disable(PF_CULL_FACE);
+    else { a->Enable(PF_CULL_FACE); a->CullFace((PFface)p->cull); }
+    if (p->depthTest) { a->Enable(PF_DEPTH_TEST); a->DepthFunc((PFdepthmode)p->depthFunc); }
+    a->ShadeModel((PFshademode)p->shade);
+    if (p->blend) a->BlendFunc((PFblendmode)p->blendMode);
+}
+
+static void setup_lights(const PFSapi* a, Rng* r, const PFSparams* p, float cx, float cy, float cz, float spread)
+{
+    if (!p->lighting) return;
+    a->Enable(PF_LIGHTING);
+    a->LightModel(p->lighting == 2 ? PF_PHONG : PF_GOURAUD);
+    int n = p->numLights > 0 ? p->numLights : 1;
+    for (int i = 0; i < n && i < 8; i++) {
+        float pos[3] = { cx + rrange(r, -spread, spread), cy + rrange(r, -spread, spread), cz + rrange(r, -spread, spread) };
+        float dif[3] = { runit(r), runit(r), runit(r) };
+        float spc[3] = { runit(r), runit(r), runit(r) };
+        float amb[3] = { 0.2f * runit(r), 0.2f * runit(r), 0.2f * runit(r) };
+        a->Lightfv((PFsizei)i, PF_POSITION, pos);
+        a->Lightfv((PFsizei)i, PF_DIFFUSE, dif);
+        a->Lightfv((PFsizei)i, PF_SPECULAR, spc);
+        a->Lightfv((PFsizei)i, PF_AMBIENT, amb);
+        if (p->spot) {
+            float dir[3] = { cx - pos[0], cy - pos[1], cz - pos[2] };
+            a->Lightfv((PFsizei)i, PF_SPOT_DIRECTION, dir);
+            a->Lightf((PFsizei)i, PF_SPOT_INNER_CUTOFF, 20.0f);
+            a->Lightf((PFsizei)i, PF_SPOT_OUTER_CUTOFF, 35.0f);
+        }
+        if (p->attenuation) {
+            a->Lightf((PFsizei)i, PF_LINEAR_ATTENUATION, 0.05f + 0.2f * runit(r));
+            a->Lightf((PFsizei)i, PF_QUADRATIC_ATTENUATION, 0.02f * runit(r));
+        }
+        a->EnableLight((PFsizei)i);
+    }
+    float mspec[3] = { 0.9f, 0.9f, 0.9f };
+    a->Materialfv(PF_FRONT_AND_BACK, PF_SPECULAR, mspec);
+    a->Materialf(PF_FRONT_AND_BACK, PF_SHININESS, 32.0f);
+    if (p->colorMaterial) {
+        a->ColorMaterial(PF_FRONT_AND_BACK, PF_AMBIENT_AND_DIFFUSE);
+        a->Enable(PF_COLOR_MATERIAL);
+    }
+}
+
+static void perspective(const PFSapi* a, const PFSparams* p, float fovScale, float n, float f)
+{
+    float aspect = (float)p->width / (float)p->height;
+    a->MatrixMode(PF_PROJECTION);
+    a->LoadIdentity();
+    a->Frustum(-aspect * fovScale * n, aspect * fovScale * n, -fovScale * n, fovScale * n, n, f);
+    a->MatrixMode(PF_MODELVIEW);
+    a->LoadIdentity();
+}
+
+static void maybe_flush(const PFSapi* a, const PFSparams* p, uint32_t drawn)
+{
+    if (p->flushEvery > 0 && a->Finish && drawn % (uint32_t)p->flushEvery == 0) a->Finish();
+}
+
+/* Random triangles with vertices uniform in NDC [-1,1]^3 (2D path). */
+static void scene_random_ndc(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    float s = p->triScale > 0 ? p->triScale : 1.0f;
+    a->Begin(PF_TRIANGLES);
+    for (uint32_t t = 0; t < p->ntris; t++) {
+        float cx = rrange(r, -1, 1), cy = rrange(r, -1, 1), cz = rrange(r, -1, 1);
+        for (int k = 0; k < 3; k++) {
+            float x = rrange(r, -1, 1), y = rrange(r, -1, 1), z = rrange(r, -1, 1);
+            if (s != 1.0f) { x = cx + s * x; y = cy + s * y; z = cz + s * z; }
+            a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
+            if (p->texture) a->TexCoord2f(rrange(r, -2, 2), rrange(r, -2, 2));
+            a->Vertex3f(x, y, z);
+        }
+        maybe_flush(a, p, t + 1);
+    }
+    a->End();
+}
+
+/* Random triangles inside and around a perspective frustum (3D path). */
+static void scene_random_3d(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    perspective(a, p, 0.6f, 0.5f, 20.0f);
+    float s = p->triScale > 0 ? p->triScale : 1.0f;
+    a->Begin(PF_TRIANGLES);
+    for (uint32_t t = 0; t < p->ntris; t++) {
+        float cz = rrange(r, -12.0f, -0.2f);
+        float cx = rrange(r, -1.2f, 1.2f) * -cz * 0.8f, cy = rrange(r, -1.2f, 1.2f) * -cz * 0.6f;
+        for (int k = 0; k < 3; k++) {
+            float x = cx + s * rrange(r, -2, 2), y = cy + s * rrange(r, -2, 2), z = cz + s * rrange(r, -2, 2);
+            a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
+            if (p->texture) a->TexCoord2f(rrange(r, -2, 2), rrange(r, -2, 2));
+            if (p->lighting) a->Normal3f(rrange(r, -1, 1), rrange(r, -1, 1), rrange(r, -1, 1));
+            a->Vertex3f(x, y, z);
+        }
+        maybe_flush(a, p, t + 1);
+    }
+    a->End();
+}
+
+static float hf_height(float x, float z, const float* ph)
+{
+    return 0.35f * sinf(1.3f * x + ph[0]) * cosf(0.9f * z + ph[1])
+         + 0.15f * sinf(3.1f * x + ph[2]) + 0.1f * cosf(2.7f * z + ph[3]);
+}
+
+/* Jittered height-field grid under a perspective camera. */
+static void scene_heightfield(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    int NX = p->gridX > 0 ? p->gridX : 1000, NZ = p->gridZ > 0 ? p->gridZ : 100;
+    perspective(a, p, 0.42f, 0.25f, 60.0f);
+    a->Rotatef(28.0f, 1.0f, 0.0f, 0.0f);
+    a->Translatef(0.0f, -2.2f, -1.0f);
+    float ph[4] = { rrange(r, 0, 6.28f), rrange(r, 0, 6.28f), rrange(r, 0, 6.28f), rrange(r, 0, 6.28f) };
+    size_t nv = (size_t)(NX + 1) * (NZ + 1);
+    float* P = (float*)malloc(nv * 3 * sizeof(float));
+    uint32_t* C = (uint32_t*)malloc(nv * sizeof(uint32_t));
+    float SX = 7.0f, Z0 = 0.4f, SZ = 9.0f;
+    for (int j = 0; j <= NZ; j++)
+        for (int i = 0; i <= NX; i++) {
+            size_t v = (size_t)j * (NX + 1) + i;
+            float jx = (i > 0 && i < NX) ? rrange(r, -0.3f, 0.3f) : 0.0f;
+            float jz = (j > 0 && j < NZ) ? rrange(r, -0.3f, 0.3f) : 0.0f;
+            float x = ((float)i + jx) / NX * SX - 0.5f * SX;
+            float z = -(Z0 + ((float)j + jz) / NZ * SZ);
+            P[3 * v] = x;
+            P[3 * v + 1] = hf_height(x, z, ph);
+            P[3 * v + 2] = z;
+            C[v] = (uint32_t)(rnext(r) >> 32);
+        }
+    size_t ntri = (size_t)NX * NZ * 2;
+    uint8_t* blended = (uint8_t*)malloc(ntri);
+    for (size_t t = 0; t < ntri; t++) blended[t] = runit(r) < p->blendFraction;
+    float ru = 40.0f, rv = 8.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            if (p->blendFraction <= 0.0f) break;
+            a->Enable(PF_BLEND);
+            a->BlendFunc((PFblendmode)(p->blend ? p->blendMode : PF_BLEND_ALPHA));
+        }
+        a->Begin(PF_TRIANGLES);
+        uint32_t drawn = 0;
+        for (int j = 0; j < NZ; j++)
+            for (int i = 0; i < NX; i++)
+                for (int h = 0; h < 2; h++) {
+                    size_t t = ((size_t)j * NX + i) * 2 + h;
+                    if (blended[t] != pass) continue;
+                    int ci[3][2];
+                    if (h == 0) { ci[0][0] = i; ci[0][1] = j; ci[1][0] = i + 1; ci[1][1] = j; ci[2][0] = i; ci[2][1] = j + 1; }
+                    else        { ci[0][0] = i + 1; ci[0][1] = j; ci[1][0] = i + 1; ci[1][1] = j + 1; ci[2][0] = i; ci[2][1] = j + 1; }
+                    for (int k = 0; k < 3; k++) {
+                        size_t v = (size_t)ci[k][1] * (NX + 1) + ci[k][0];
+                        uint32_t c = C[v];
+                        a->Color4ub((PFubyte)c, (PFubyte)(c >> 8), (PFubyte)(c >> 16), pass ? 128 : 255);
+                        a->TexCoord2f((float)ci[k][0] / NX * ru, (float)ci[k][1] / NZ * rv);
+                        a->Vertex3f(P[3 * v], P[3 * v + 1], P[3 * v + 2]);
+                    }
+                    maybe_flush(a, p, ++drawn);
+                }
+        a->End();
+    }
+    free(P); free(C); free(blended);
+}
+
+/* Cluster of tessellated spheres with per-vertex normals. */
+static void scene_spheres(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    int NS = p->spheres > 0 ? p->spheres : 256;
+    int NLAT = p->sphereLat > 0 ? p->sphereLat : 31, NLON = p->sphereLon > 0 ? p->sphereLon : 63;
+    perspective(a, p, 0.5f, 0.5f, 30.0f);
+    a->Translatef(0.0f, 0.0f, -4.5f);
+    setup_lights(a, r, p, 0.0f, 0.0f, 0.0f, 2.5f);
+    a->Begin(PF_TRIANGLES);
+    uint32_t drawn = 0;
+    for (int s = 0; s < NS; s++) {
+        float cx = rrange(r, -1.6f, 1.6f), cy = rrange(r, -1.0f, 1.0f), cz = rrange(r, -1.2f, 1.2f);
+        float rad = rrange(r, 0.05f, 0.3f);
+        uint32_t col = (uint32_t)(rnext(r) >> 32) | 0xFF000000u;
+        for (int la = 0; la < NLAT; la++)
+            for (int lo = 0; lo < NLON; lo++)
+                for (int h = 0; h < 2; h++) {
+                    int cc[3][2];
+                    if (h == 0) { cc[0][0] = la; cc[0][1] = lo; cc[1][0] = la + 1; cc[1][1] = lo; cc[2][0] = la + 1; cc[2][1] = lo + 1; }
+                    else        { cc[0][0] = la; cc[0][1] = lo; cc[1][0] = la + 1; cc[1][1] = lo + 1; cc[2][0] = la; cc[2][1] = lo + 1; }
+                    for (int k = 0; k < 3; k++) {
+                        float th = (float)cc[k][0] / NLAT * 3.14159265f;
+                        float phi = (float)cc[k][1] / NLON * 6.28318531f;
+                        float nx = sinf(th) * cosf(phi), ny = cosf(th), nz = sinf(th) * sinf(phi);
+                        a->Color4ub((PFubyte)col, (PFubyte)(col >> 8), (PFubyte)(col >> 16), 255);
+                        a->Normal3f(nx, ny, nz);
+                        if (p->texture) a->TexCoord2f((float)cc[k][1] / NLON, (float)cc[k][0] / NLAT);
+                        a->Vertex3f(cx + rad * nx, cy + rad * ny, cz + rad * nz);
+                    }
+                    maybe_flush(a, p, ++drawn);
+                }
+    }
+    a->End();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Entry points                                                              */
+/* ------------------------------------------------------------------------ */
+
+static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* tex)
+{
+    common_state(a, p);
+    *tex = make_texture(a, r, p);
+    if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
+    if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
+    if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
+    switch (p->kind) {
+    case PFS_RANDOM_NDC: scene_random_ndc(a, r, p); break;
+    case PFS_HEIGHTFIELD: scene_heightfield(a, r, p); break;
+    case PFS_SPHERES: scene_spheres(a, r, p); break;
+    case PFS_RANDOM_3D:
+        setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
+        scene_random_3d(a, r, p);
+        break;
+    default: break;
+    }
+}
+
+/* Renders one scene into an offscreen framebuffer of the bound library and
+ * copies its colour (RGBA8) and depth into the caller's arrays.  The
+ * context's own buffer is `mainColor` (W*H+16 pixels, may be NULL).
+ * Returns the library's last error code, or -1 on setup failure. */
+int pfs_render(const PFSapi* a, const PFSparams* p, uint32_t* colorOut, float* depthOut)
+{
+    if (!a || !p || !p->width || !p->height) return -1;
+    size_t n = (size_t)p->width * p->height;
+    uint32_t* mainColor = (uint32_t*)calloc(n + 16, 4);
+    if (!mainColor) return -1;
+    PFcontext ctx = a->CreateContext(mainColor, p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
+    if (!ctx) { free(mainColor); return -1; }
+    a->MakeCurrent(ctx);
+    PFframebuffer fb = a->GenFramebuffer(p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
+    a->BindFramebuffer(&fb);
+    a->Enable(PF_FRAMEBUFFER);
+    Rng r = { p->seed };
+    SceneTex tex;
+    draw_scene(a, &r, p, &tex);
+    if (a->Finish) a->Finish();
+    int err = (int)a->GetError();
+    uint32_t* px = (uint32_t*)a->GetTexturePixels(fb.texture, NULL, NULL, NULL, NULL);
+    if (colorOut) memcpy(colorOut, px, n * 4);
+    if (depthOut) memcpy(depthOut, fb.zbuffer, n * 4);
+    a->Disable(PF_FRAMEBUFFER);
+    a->BindFramebuffer(NULL);
+    a->DeleteFramebuffer(&fb);
+    free_texture(a, &tex);
+    a->DeleteContext(ctx);
+    a->MakeCurrent(NULL);
+    free(mainColor);
+    return err;
+}
+
+/* Renders `count` scenes, each into its own context and main buffer, and
+ * submits them together with pfSubmitBatch when the library has it (one
+ * after another otherwise).  colors[i] must hold W_i*H_i+16 pixels. */
+int pfs_render_batch(const PFSapi* a, const PFSparams* ps, uint32_t count, uint32_t** colors)
+{
+    PFcontext* ctxs = (PFcontext*)calloc(count, sizeof(PFcontext));
+    SceneTex* texs = (SceneTex*)calloc(count, sizeof(SceneTex));
+    if (!ctxs || !texs) { free(ctxs); free(texs); return -1; }
+    int err = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const PFSparams* p = &ps[i];
+        ctxs[i] = a->CreateContext(colors[i], p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
+        if (!ctxs[i]) { err = -1; break; }
+        a->MakeCurrent(ctxs[i]);
+        Rng r = { p->seed };
+        draw_scene(a, &r, p, &texs[i]);
+        if (!a->SubmitBatch) { int e = (int)a->GetError(); if (e) err = e; }
+    }
+    if (err == 0 && a->SubmitBatch) {
+        a->SubmitBatch(ctxs, count);
+        for (uint32_t i = 0; i < count; i++) {
+            a->MakeCurrent(ctxs[i]);
+            int e = (int)a->GetError();
+            if (e) err = e;
+        }
+    }
+    for (uint32_t i = 0; i < count; i++) {
+        if (!ctxs[i]) continue;
+        a->MakeCurrent(ctxs[i]);
+        free_texture(a, &texs[i]);
+        a->DeleteContext(ctxs[i]);
+    }
+    a->MakeCurrent(NULL);
+    free(ctxs);
+    free(texs);
+    return err;
+}
+
+/* Records a scene into the current context of the bound library without
+ * creating or deleting anything; the caller owns context and target.  The
+ * texture is returned through *texOut / *texPixels for the caller to free
+ * with pfs_free_texture after the frame has run. */
+int pfs_record(const PFSapi* a, const PFSparams* p, PFtexture* texOut, void** texPixels)
+{
+    Rng r = { p->seed };
+    SceneTex tex;
+    draw_scene(a, &r, p, &tex);
+    if (texOut) *texOut = tex.tex;
+    if (texPixels) *texPixels = tex.pixels;
+    return (int)a->GetError();
+}
+
+void pfs_free_texture(const PFSapi* a, PFtexture tex, void* pixels)
+{
+    if (tex) a->DeleteTexture(&tex, 0);
+    free(pixels);
+}
