@@ -589,6 +589,7 @@ struct SetupArgs {
     RTri*           rts;
     RTriL*          rtl;
     uint32_t*       pairCount;      // write pass: tiles overlapped per raster triangle
+    uint32_t        capRT;          // write pass: raster triangles the buffers hold
 };
 
 // Builds (or only counts) the raster triangle of screen vertices a, b, c for
@@ -620,6 +621,7 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& 
     int32_t cx1 = min(xMax - 1, (int32_t)F.W - 1), cy1 = min(yMax, (int32_t)F.H - 1);
     if (cx0 > cx1 || cy0 > cy1) return false;
     if (!WRITE) return true;
+    if (slot >= A.capRT) return false;
 
     uint32_t Ax[3] = { uy3 - uy2, uy1 - uy3, uy2 - uy1 };
     uint32_t By[3] = { ux2 - ux3, ux3 - ux1, ux1 - ux2 };
@@ -872,7 +874,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 
 __global__ __launch_bounds__(256) void k_pair_emit(const RTri* __restrict__ rts, const uint32_t* __restrict__ rtFrame,
                                                    const DFrame* __restrict__ frames, const uint32_t* __restrict__ pairOffset,
-                                                   uint32_t numRT, uint32_t ts,
+                                                   uint32_t numRT, uint32_t ts, uint32_t capPairs,
                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= numRT) return;
@@ -885,6 +887,7 @@ __global__ __launch_bounds__(256) void k_pair_emit(const RTri* __restrict__ rts,
     uint32_t p = pairOffset[r];
     for (uint32_t ty = ty0; ty <= ty1; ty++)
         for (uint32_t tx = tx0; tx <= tx1; tx++) {
+            if (p >= capPairs) return;
             keys[p] = F.tileBase + ty * F.tilesX + tx;
             vals[p] = r;
             p++;
@@ -965,7 +968,7 @@ __global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
 
 // Raster triangle -> frame index, for pair emission.
 __global__ void k_rt_frame(const uint32_t* __restrict__ offset, const uint32_t* __restrict__ count, const DSeg* __restrict__ segs,
-                           uint32_t numSegs, uint32_t numPassTris, uint32_t* __restrict__ rtFrame) {
+                           uint32_t numSegs, uint32_t numPassTris, uint32_t capRT, uint32_t* __restrict__ rtFrame) {
     uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= numPassTris) return;
     uint32_t n = count[gid];
@@ -976,7 +979,7 @@ __global__ void k_rt_frame(const uint32_t* __restrict__ offset, const uint32_t* 
         if (segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
     }
     uint32_t f = segs[lo].frame, o = offset[gid];
-    for (uint32_t k = 0; k < n; k++) rtFrame[o + k] = f;
+    for (uint32_t k = 0; k < n && o + k < capRT; k++) rtFrame[o + k] = f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1275,6 +1278,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.rts = nullptr;
     sa.rtl = nullptr;
     sa.pairCount = nullptr;
+    sa.capRT = 0;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
     uint32_t numRT = 0;
@@ -1292,12 +1296,13 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.rts = (RTri*)J->rts.p;
     sa.rtl = (RTriL*)J->rtl.p;
     sa.pairCount = (uint32_t*)J->pcnt.p;
+    sa.capRT = numRT;
     uint32_t numPairs = 0;
     if (numRT) {
         hipLaunchKernelGGL((k_setup<true>), dim3(grid_for(n, 256)), dim3(256), 0, J->stream, sa);
         hipLaunchKernelGGL(k_rt_frame, dim3(grid_for(n, 256)), dim3(256), 0, J->stream,
                            (const uint32_t*)J->off.p, (const uint32_t*)J->cnt.p, (const DSeg*)P.dSegs.p,
-                           (uint32_t)P.segs.size(), n, (uint32_t*)J->rtFrame.p);
+                           (uint32_t)P.segs.size(), n, numRT, (uint32_t*)J->rtFrame.p);
         HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
         if ((rc = scan_u32(J, (const uint32_t*)J->pcnt.p, (uint32_t*)J->poff.p, numRT, (uint32_t*)J->scanTotal.p))) return rc;
         if (discover) { numPairs = read_total(J, &rc); if (rc) return rc; P.numPairs = numPairs; }
@@ -1320,7 +1325,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (numPairs) {
         hipLaunchKernelGGL(k_pair_emit, dim3(grid_for(numRT, 256)), dim3(256), 0, J->stream,
                            (const RTri*)J->rts.p, (const uint32_t*)J->rtFrame.p, (const DFrame*)J->dFrames.p,
-                           (const uint32_t*)J->poff.p, numRT, J->tileShift, keys, vals);
+                           (const uint32_t*)J->poff.p, numRT, J->tileShift, numPairs, keys, vals);
         uint32_t bits = 1;
         while (bits < 32 && (1u << bits) < J->numTiles) bits++;
         uint32_t nb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;

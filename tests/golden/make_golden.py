@@ -1,0 +1,43 @@
+"""Writes the golden frames of tests/scenes.py GOLDEN_SCENES.
+
+Renders each scene through a CPU build of the reference (oracle/_ref,
+built by `make -C oracle ref` from the reference sources) with one OpenMP
+thread -- the reference's row-parallel loop races on shared 8-pixel stores
+(triangles.c:402, 434) -- and stores colour (packed RGBA8) and depth in one
+compressed .npz per scene together with the scene parameters.
+
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+os.environ["OMP_NUM_THREADS"] = "1"
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+import pixelforge_amd as pf  # noqa: E402
+import scenes  # noqa: E402
+
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
+
+
+def main():
+    if not os.path.exists(REF_SO):
+        sys.exit(f"{REF_SO} missing: run make -C oracle ref")
+    for name, kw in scenes.GOLDEN_SCENES:
+        p = scenes.params(kw)
+        color, depth, err = pf.render(p, REF_SO)
+        if err:
+            sys.exit(f"{name}: reference reported error {err}")
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), color=color, depth=depth,
+                            params=json.dumps(kw))
+        print(name, color.shape)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,30 @@
+"""The CPU oracle against golden frames of the compiled reference.
+
+Every golden (tests/golden/*.npz, written by tests/golden/make_golden.py
+from a CPU build of the reference with one OpenMP thread) must be
+reproduced bit for bit -- colour and depth -- by the host library's
+recorded frame rendered through the oracle.  This pins the oracle and the
+host library's recording of the API together.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import scenes
+from conftest import GOLDEN, assert_same, render_oracle
+
+NAMES = [n for n, _ in scenes.GOLDEN_SCENES]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_matches_golden(name):
+    path = os.path.join(GOLDEN, f"{name}.npz")
+    g = np.load(path, allow_pickle=False)
+    kw = dict(scenes.GOLDEN_SCENES)[name]
+    assert json.loads(str(g["params"])) == kw, "golden was made from other scene parameters"
+    color, depth, err, frags = render_oracle(scenes.params(kw))
+    assert err == 0
+    assert frags > 0
+    assert_same(g["color"], g["depth"], color, depth, name)
