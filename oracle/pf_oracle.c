@@ -838,3 +838,7 @@ int pforacle_executor(const PFFframe* f, void* user)
     if (user) *(uint64_t*)user += frags;
     return rc;
 }
+
+/* Exported for the unit tests of the oracle's own arithmetic. */
+float pforacle_rcp(float x) { return rcp_(x); }
+float pforacle_rsqrt(float x) { return rsqrt_(x); }
