@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-shift", type=int, default=0, help="3..6 fixes the tile size; 0 autotunes")
+    ap.add_argument("--raster-mode", type=int, default=0, help="with --tile-shift: 0 packed, 1 in-place")
     ap.add_argument("--profile-runs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -115,6 +116,7 @@ def main():
     cfg = pf.PFDconfig()
     if args.tile_shift:
         cfg.tileShift = args.tile_shift
+        cfg.rasterMode = args.raster_mode
         check(lib.pfdJobSetConfig(job, C.byref(cfg)), "config")
     else:
         check(lib.pfdJobAutotune(job, C.byref(cfg)), "autotune")
@@ -166,10 +168,13 @@ def main():
                                "1024^2 RGBA8 noise texture bilinear REPEAT, LEQUAL, 30% alpha-blended",
                    "fragments_per_frame": frags, "raster_triangles": int(st.rasterTriangles),
                    "tile_pairs": int(st.tilePairs), "tile": 1 << int(st.tileShift),
+                   "max_tile_list": int(st.maxTileList), "busy_tiles": int(st.busyTiles),
+                   "raster_mode": ["packed", "in-place"][cfg.rasterMode],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_raster", "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
+                     "kernel": "k_raster_packed" if cfg.rasterMode == 0 else "k_raster",
+                     "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
         "phases_ms": {"setup": statistics.mean(setup), "binning": statistics.mean(binning), "raster": raster_ms},
         "record_s": rec_s,
     }

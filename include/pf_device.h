@@ -33,11 +33,14 @@ extern "C" {
 typedef struct PFDjob PFDjob;
 
 /* Tile configuration of a job.  tileShift: log2 of the square screen tile
- * handled by one wavefront group (3..6 -> 8x8 .. 64x64).  binBatch: raster
- * triangles per pair-emit work item.  Zero fields take defaults. */
+ * one workgroup rasterises (3..6 -> 8x8 .. 64x64) -- also the binning
+ * granularity.  rasterMode: 0 packs covered fragments of several triangles
+ * into full wavefronts before shading; 1 shades each triangle in place on
+ * the lanes it covers (better when triangles cover whole 8x8 blocks).
+ * Zero fields take defaults. */
 typedef struct {
     uint32_t tileShift;
-    uint32_t binBatch;
+    uint32_t rasterMode;
     uint32_t reserved[6];
 } PFDconfig;
 
@@ -50,6 +53,8 @@ typedef struct {
     uint32_t numTiles;
     uint32_t tileShift;
     uint64_t deviceBytes;      /* device memory held by the job               */
+    uint32_t maxTileList;      /* longest per-tile pair list (last pass)      */
+    uint32_t busyTiles;        /* tiles with a non-empty list (last pass)     */
 } PFDstats;
 
 /* Per-kernel timing of one run, in milliseconds (HIP events on the job's
@@ -64,6 +69,10 @@ typedef struct {
 /* Device initialisation; idempotent.  Returns PFD_E_NODEVICE without a GPU. */
 int         pfdInit(void);
 const char* pfdLastError(void);
+
+/* Configuration of jobs created without one (pfdExecuteFrame(s), the host
+ * library's default executor). */
+int pfdSetDefaultConfig(const PFDconfig* cfg);
 
 /* One-shot execution used by the host library's default executor: uploads
  * the frame and the render target, runs it and downloads the target. */

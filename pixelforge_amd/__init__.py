@@ -66,7 +66,7 @@ class PFDstats(C.Structure):
     _fields_ = [("rasterTriangles", C.c_uint64), ("tilePairs", C.c_uint64),
                 ("fragments", C.c_uint64), ("inputTriangles", C.c_uint64),
                 ("numTiles", C.c_uint32), ("tileShift", C.c_uint32),
-                ("deviceBytes", C.c_uint64)]
+                ("deviceBytes", C.c_uint64), ("maxTileList", C.c_uint32), ("busyTiles", C.c_uint32)]
 
 
 class PFDtimes(C.Structure):
@@ -75,7 +75,7 @@ class PFDtimes(C.Structure):
 
 
 class PFDconfig(C.Structure):
-    _fields_ = [("tileShift", C.c_uint32), ("binBatch", C.c_uint32),
+    _fields_ = [("tileShift", C.c_uint32), ("rasterMode", C.c_uint32),
                 ("reserved", C.c_uint32 * 6)]
 
 
@@ -102,6 +102,7 @@ def lib():
         L.pfdJobStats.argtypes = [C.c_void_p, C.POINTER(PFDstats)]
         L.pfdJobSetConfig.argtypes = [C.c_void_p, C.POINTER(PFDconfig)]
         L.pfdJobAutotune.argtypes = [C.c_void_p, C.POINTER(PFDconfig)]
+        L.pfdSetDefaultConfig.argtypes = [C.POINTER(PFDconfig)]
         L.pfdJobDestroy.argtypes = [C.c_void_p]
         L.pfdJobDestroy.restype = None
         L.pfdTakeCapturedJob.restype = C.c_void_p
@@ -170,6 +171,17 @@ def render_batch(params_list, path=LIB_PATH):
     ptrs = (C.c_void_p * n)(*[b.ctypes.data for b in bufs])
     err = scenes().pfs_render_batch(api(path), arr, n, ptrs)
     return [b[: p.height * p.width].reshape(p.height, p.width) for b, p in zip(bufs, params_list)], err
+
+
+def set_default_config(tile_shift=3, raster_mode=0):
+    """Tile size (2**tile_shift) and raster mode of frames run by the
+    default executor."""
+    c = PFDconfig()
+    c.tileShift = tile_shift
+    c.rasterMode = raster_mode
+    rc = lib().pfdSetDefaultConfig(C.byref(c))
+    if rc:
+        raise RuntimeError(f"pfdSetDefaultConfig failed ({rc})")
 
 
 def unpack(color):

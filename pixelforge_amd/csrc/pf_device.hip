@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stddef.h>
 #include <math.h>
 #include <vector>
 #include <algorithm>
@@ -82,6 +83,8 @@ struct __attribute__((aligned(16))) RTri {
     uint32_t pad[3];
 };
 static_assert(sizeof(RTri) == 128, "RTri layout");
+static_assert(offsetof(RTri, xMin) == 36 && offsetof(RTri, yMax) == 48, "bbox read as dwords 9..12");
+static_assert(offsetof(RTri, meta) == 80, "meta read as dword 20");
 
 struct __attribute__((aligned(16))) RTriL {   // per-fragment lighting inputs
     float n[9];
@@ -98,6 +101,7 @@ enum : uint32_t {
     META_BLEND     = 1u << 4,
     META_TEX       = 1u << 5,
     META_PHONG     = 1u << 6,
+    META_SMALL     = 1u << 7,   // every A_k, B_k fits in 24 signed bits
     META_DFUNC_SH  = 8,     // 3 bits
     META_BMODE_SH  = 12,    // 3 bits
 };
@@ -519,9 +523,10 @@ __device__ __forceinline__ Vtx lerp_vtx(const Vtx& s, const Vtx& e, float t) {  
     return r;
 }
 
-__device__ __forceinline__ void to_screen(const PFFstate& st, Vtx& v) {        // ictx.c:51-65
-    v.scr[0] = ((float)st.vpPos[0] + ((v.hom[0] + 1.0f) * 0.5f) * (float)st.vpDim[0]) + 0.5f;
-    v.scr[1] = ((float)st.vpPos[1] + ((1.0f - v.hom[1]) * 0.5f) * (float)st.vpDim[1]) + 0.5f;
+// pfiHomogeneousToScreen (ictx.c:51-65); vp = {vpPos.x, vpPos.y, vpDim.x, vpDim.y} as floats.
+__device__ __forceinline__ void to_screen(const float* vp, Vtx& v) {
+    v.scr[0] = (vp[0] + ((v.hom[0] + 1.0f) * 0.5f) * vp[2]) + 0.5f;
+    v.scr[1] = (vp[1] + ((1.0f - v.hom[1]) * 0.5f) * vp[3]) + 0.5f;
 }
 
 // Sutherland-Hodgman against w >= eps then +-x, +-y, +-z <= w
@@ -572,6 +577,14 @@ __device__ __noinline__ int clip_polygon(Vtx* poly, int n) {
     return n;
 }
 
+// Kernel-argument pointers are read through the constant address space
+// when the index is wavefront-uniform, so that the loads become scalar.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
+typedef const __attribute__((address_space(4))) PFFstate CState;
+
 struct SetupArgs {
     const float4*   pos;
     const uint32_t* col;
@@ -584,20 +597,33 @@ struct SetupArgs {
     uint32_t        numSegs;
     uint32_t        numPassTris;
     uint32_t        tileShift;
-    uint32_t*       count;          // [numPassTris] raster triangles per recorded triangle
-    const uint32_t* offset;         // write pass: output slot of the first one
+    uint32_t        numTiles;
+    uint32_t*       cntRT;          // [numPassTris] raster triangles per recorded triangle
+    uint32_t*       cntPairs;       // [numPassTris] tile pairs per recorded triangle
+    uint32_t*       blockRT;        // per block: sum (count pass), then exclusive scan
+    uint32_t*       blockPairs;
+    uint32_t*       tileBegin;      // zeroed by the count pass
+    uint32_t*       tileEnd;
     RTri*           rts;
     RTriL*          rtl;
-    uint32_t*       pairCount;      // write pass: tiles overlapped per raster triangle
-    uint32_t        capRT;          // write pass: raster triangles the buffers hold
+    uint32_t*       keys;           // tile of each pair
+    uint32_t*       vals;           // raster triangle of each pair
+    uint32_t        capRT;
+    uint32_t        capPairs;
 };
 
+// Tiles [tx0..tx1] x [ty0..ty1] overlapped by the covered pixels of a
+// triangle: columns [xMin, xMax) and rows [yMin, yMax] clipped to the target.
+struct TileSpan { uint32_t tx0, tx1, ty0, ty1; };
+
 // Builds (or only counts) the raster triangle of screen vertices a, b, c for
-// one culling pass (Rasterize_Triangle setup, triangles.c:287-349).
-template <bool WRITE>
-__device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& st, const DFrame& F,
+// one culling pass (Rasterize_Triangle setup, triangles.c:287-349) and its
+// tile pairs.  Returns false when it is culled or covers no pixel.
+template <int PHASE>
+__device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, const DFrame& F,
                                             uint32_t stateIdx, int face, bool is3D,
-                                            const Vtx& a, const Vtx& b, const Vtx& c, uint32_t slot) {
+                                            const Vtx& a, const Vtx& b, const Vtx& c,
+                                            uint32_t rtSlot, uint32_t pairSlot, uint32_t& pairsOut) {
     int32_t x1 = cvtt_i32(a.scr[0]), y1 = cvtt_i32(a.scr[1]);
     int32_t x2 = cvtt_i32(b.scr[0]), y2 = cvtt_i32(b.scr[1]);
     int32_t x3 = cvtt_i32(c.scr[0]), y3 = cvtt_i32(c.scr[1]);
@@ -607,21 +633,24 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& 
     int32_t xMin = min(x1, min(x2, x3)), yMin = min(y1, min(y2, y3));
     int32_t xMax = max(x1, max(x2, x3)), yMax = max(y1, max(y2, y3));
     if (!is3D) {
-        xMin = min(max(xMin, st.vpMin[0]), st.vpMax[0]);
-        yMin = min(max(yMin, st.vpMin[1]), st.vpMax[1]);
-        xMax = min(max(xMax, st.vpMin[0]), st.vpMax[0]);
-        yMax = min(max(yMax, st.vpMin[1]), st.vpMax[1]);
+        xMin = min(max(xMin, st->vpMin[0]), st->vpMax[0]);
+        yMin = min(max(yMin, st->vpMin[1]), st->vpMax[1]);
+        xMax = min(max(xMax, st->vpMin[0]), st->vpMax[0]);
+        yMax = min(max(yMax, st->vpMin[1]), st->vpMax[1]);
     }
     // Unsigned loops of the reference (triangles.c:404, 409) run nothing
     // when the minimum is negative and the maximum is not.
     if ((uint32_t)xMin > (uint32_t)xMax || (uint32_t)yMin > (uint32_t)yMax) return false;
-    // Pixels outside the render target are never written: the covered
-    // columns are [xMin, xMax) and rows [yMin, yMax].
+    // Pixels outside the render target are never written.
     int32_t cx0 = max(xMin, 0), cy0 = max(yMin, 0);
     int32_t cx1 = min(xMax - 1, (int32_t)F.W - 1), cy1 = min(yMax, (int32_t)F.H - 1);
     if (cx0 > cx1 || cy0 > cy1) return false;
-    if (!WRITE) return true;
-    if (slot >= A.capRT) return false;
+    const uint32_t ts = A.tileShift;
+    const uint32_t tx0 = (uint32_t)cx0 >> ts, tx1 = (uint32_t)cx1 >> ts;
+    const uint32_t ty0 = (uint32_t)cy0 >> ts, ty1 = (uint32_t)cy1 >> ts;
+    pairsOut = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    if (PHASE == 0) return true;
+    if (rtSlot >= A.capRT) return false;
 
     uint32_t Ax[3] = { uy3 - uy2, uy1 - uy3, uy2 - uy1 };
     uint32_t By[3] = { ux2 - ux3, ux3 - ux1, ux1 - ux2 };
@@ -641,17 +670,24 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& 
     r.z[0] = a.hom[2]; r.z[1] = b.hom[2]; r.z[2] = c.hom[2];
     r.c[0] = a.col; r.c[1] = b.col; r.c[2] = c.col;
     uint32_t meta = (face ? META_BACK : 0u) | (is3D ? META_3D : 0u);
-    if (st.shadeModel == 1) meta |= META_SMOOTH;
-    if (st.flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st.depthFunc & 7u) << META_DFUNC_SH);
-    if (st.flags & PFF_BLEND) meta |= META_BLEND | ((st.blendMode & 7u) << META_BMODE_SH);
-    if ((st.flags & PFF_TEXTURE_2D) && st.texture >= 0) meta |= META_TEX;
-    bool phong = (st.flags & PFF_LIGHTING) && st.lightMode == 1 && st.numLights > 0;
+    bool small = true;
+    for (int k = 0; k < 3; k++) {
+        int32_t va = (int32_t)Ax[k], vb = (int32_t)By[k];
+        small = small && va > -(1 << 23) && va < (1 << 23) && vb > -(1 << 23) && vb < (1 << 23);
+    }
+    if (small) meta |= META_SMALL;
+    const uint32_t flags = st->flags;
+    if (st->shadeModel == 1) meta |= META_SMOOTH;
+    if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st->depthFunc & 7u) << META_DFUNC_SH);
+    if (flags & PFF_BLEND) meta |= META_BLEND | ((st->blendMode & 7u) << META_BMODE_SH);
+    if ((flags & PFF_TEXTURE_2D) && st->texture >= 0) meta |= META_TEX;
+    const bool phong = (flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
     if (phong) meta |= META_PHONG;
     r.meta = meta;
     r.state = stateIdx;
     r.uv[0] = a.uv[0]; r.uv[1] = a.uv[1]; r.uv[2] = b.uv[0]; r.uv[3] = b.uv[1]; r.uv[4] = c.uv[0]; r.uv[5] = c.uv[1];
     r.pad[0] = r.pad[1] = r.pad[2] = 0;
-    A.rts[slot] = r;
+    A.rts[rtSlot] = r;
     if (phong) {
         RTriL l;
         for (int i = 0; i < 3; i++) {
@@ -659,75 +695,74 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, const PFFstate& 
             l.p[i] = a.pos[i]; l.p[3 + i] = b.pos[i]; l.p[6 + i] = c.pos[i];
         }
         l.pad[0] = l.pad[1] = 0.0f;
-        A.rtl[slot] = l;
+        A.rtl[rtSlot] = l;
     }
-    uint32_t ts = A.tileShift;
-    uint32_t ntx = ((uint32_t)cx1 >> ts) - ((uint32_t)cx0 >> ts) + 1;
-    uint32_t nty = ((uint32_t)cy1 >> ts) - ((uint32_t)cy0 >> ts) + 1;
-    A.pairCount[slot] = ntx * nty;
+    // Tile pairs of this raster triangle, in submission order.
+    uint32_t p = pairSlot;
+    for (uint32_t ty = ty0; ty <= ty1; ty++)
+        for (uint32_t tx = tx0; tx <= tx1; tx++) {
+            if (p >= A.capPairs) return true;
+            A.keys[p] = F.tileBase + ty * F.tilesX + tx;
+            A.vals[p] = rtSlot;
+            p++;
+        }
     return true;
 }
 
-template <bool WRITE>
-__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= A.numPassTris) return;
-    // Segment of this thread: the last one whose passTriBase <= gid.
-    uint32_t lo = 0, hi = A.numSegs;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (A.segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
-    }
-    const DSeg sg = A.segs[lo];
-    const uint32_t t = sg.triFirst + (gid - sg.passTriBase);
-    const DFrame F = A.frames[sg.frame];
-    const uint32_t info = A.info[t];
-    const uint32_t si = info & PFF_STATE_MASK;
-    const PFFstate& st = A.states[si];
-
+// All culling passes of recorded triangle t under state st.  PHASE 0 counts
+// raster triangles and tile pairs (geometry only); PHASE 1 also lights the
+// vertices and writes records and pairs from the given slots.
+template <int PHASE, bool LIT>
+__device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
+                                               uint32_t t, uint32_t info, uint32_t rtBase, uint32_t pairBase,
+                                               uint32_t& nRT, uint32_t& nPairs) {
     Vtx base[3];
     for (int k = 0; k < 3; k++) {
         size_t vi = (size_t)t * 3 + k;
         float4 p = A.pos[vi];
         base[k].pos[0] = p.x; base[k].pos[1] = p.y; base[k].pos[2] = p.z; base[k].pos[3] = p.w;
-        base[k].col = A.col[vi];
-        if (A.uv) { float2 q = A.uv[vi]; base[k].uv[0] = q.x; base[k].uv[1] = q.y; }
+        base[k].col = (PHASE == 1) ? A.col[vi] : 0u;
+        if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; base[k].uv[0] = q.x; base[k].uv[1] = q.y; }
         else { base[k].uv[0] = base[k].uv[1] = 0.0f; }
-        if (A.nrm) { base[k].nrm[0] = A.nrm[3 * vi]; base[k].nrm[1] = A.nrm[3 * vi + 1]; base[k].nrm[2] = A.nrm[3 * vi + 2]; }
+        if (PHASE == 1 && A.nrm) { base[k].nrm[0] = A.nrm[3 * vi]; base[k].nrm[1] = A.nrm[3 * vi + 1]; base[k].nrm[2] = A.nrm[3 * vi + 2]; }
         else { base[k].nrm[0] = base[k].nrm[1] = base[k].nrm[2] = 0.0f; }
         base[k].scr[0] = base[k].scr[1] = 0.0f;
     }
-
-    uint32_t emitted = 0;
-    const uint32_t slot0 = WRITE ? A.offset[gid] : 0u;
+    const uint32_t flags = st->flags;
     for (int face = 0; face < 2; face++) {
         if (!(info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT))) continue;
         Vtx v[3];
         for (int k = 0; k < 3; k++) v[k] = base[k];
-        if (st.flags & PFF_LIGHTING) {                                  // triangles.c:78-104
+        if (PHASE == 1 && LIT && (flags & PFF_LIGHTING)) {              // triangles.c:78-104
+            const PFFstate& stg = A.states[si];
             for (int k = 0; k < 3; k++) {
-                const float* m = st.matNormal;
-                float nx = m[0] * v[k].nrm[0] + m[4] * v[k].nrm[1] + m[8] * v[k].nrm[2] + m[12];
-                float ny = m[1] * v[k].nrm[0] + m[5] * v[k].nrm[1] + m[9] * v[k].nrm[2] + m[13];
-                float nz = m[2] * v[k].nrm[0] + m[6] * v[k].nrm[1] + m[10] * v[k].nrm[2] + m[14];
+                const float m0 = st->matNormal[0], m1 = st->matNormal[1], m2 = st->matNormal[2];
+                const float m4 = st->matNormal[4], m5 = st->matNormal[5], m6 = st->matNormal[6];
+                const float m8 = st->matNormal[8], m9 = st->matNormal[9], m10 = st->matNormal[10];
+                const float m12 = st->matNormal[12], m13 = st->matNormal[13], m14 = st->matNormal[14];
+                float nx = m0 * v[k].nrm[0] + m4 * v[k].nrm[1] + m8 * v[k].nrm[2] + m12;
+                float ny = m1 * v[k].nrm[0] + m5 * v[k].nrm[1] + m9 * v[k].nrm[2] + m13;
+                float nz = m2 * v[k].nrm[0] + m6 * v[k].nrm[1] + m10 * v[k].nrm[2] + m14;
                 v[k].nrm[0] = nx; v[k].nrm[1] = ny; v[k].nrm[2] = nz;
                 float sq = nx * nx + ny * ny + nz * nz;
                 if (sq != 0.0f) {
                     float inv = __fdiv_rn(1.0f, sqrt_cr(sq));
                     for (int i = 0; i < 3; i++) v[k].nrm[i] = v[k].nrm[i] * inv;
                 }
-                uint32_t dcol = st.material[face].diffuse, cc = 0;
+                uint32_t dcol = st->material[face].diffuse, cc = 0;
                 for (int i = 0; i < 4; i++) cc |= (uint32_t)((ch(v[k].col, i) * ch(dcol, i)) / 255) << (8 * i);
                 v[k].col = cc;
-                if (st.lightMode == 0) {
-                    float ndv = dot3(v[k].nrm, st.viewRow2);
-                    v[k].col = light_vertex(st, st.material[ndv < 0.0f ? 0 : 1], v[k].col,
-                                            st.viewPos, v[k].pos, v[k].nrm);
+                if (st->lightMode == 0) {
+                    float vr[3] = { st->viewRow2[0], st->viewRow2[1], st->viewRow2[2] };
+                    float ndv = dot3(v[k].nrm, vr);
+                    v[k].col = light_vertex(stg, stg.material[ndv < 0.0f ? 0 : 1], v[k].col,
+                                            stg.viewPos, v[k].pos, v[k].nrm);
                 }
             }
         }
         // Process_ProjectAndClipTriangle (triangles.c:246-280)
-        const float* m = st.mvp;
+        float m[16];
+        for (int i = 0; i < 16; i++) m[i] = st->mvp[i];
         float wsum = 0.0f;
         for (int k = 0; k < 3; k++) {
             const float* p = v[k].pos;
@@ -737,9 +772,14 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
             v[k].hom[3] = m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15] * p[3];
             wsum += v[k].hom[3];
         }
+        const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
+        uint32_t np = 0;
         if (fabsf(wsum - 3.0f) < 1e-5f) {                                 // "2D" path
-            for (int k = 0; k < 3; k++) to_screen(st, v[k]);
-            if (emit_raster<WRITE>(A, st, F, si, face, false, v[0], v[1], v[2], slot0 + emitted)) emitted++;
+            for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
+            if (emit_raster<PHASE>(A, st, F, si, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+                nRT++;
+                nPairs += np;
+            }
             continue;
         }
         bool inside = true;                   // clipping returns the polygon unchanged
@@ -757,9 +797,12 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
                 float invW = __fdiv_rn(1.0f, v[k].hom[3]);
                 v[k].hom[0] *= invW;
                 v[k].hom[1] *= invW;
-                to_screen(st, v[k]);
+                to_screen(stl, v[k]);
             }
-            if (emit_raster<WRITE>(A, st, F, si, face, true, v[0], v[1], v[2], slot0 + emitted)) emitted++;
+            if (emit_raster<PHASE>(A, st, F, si, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+                nRT++;
+                nPairs += np;
+            }
             continue;
         }
         Vtx poly[12];
@@ -773,21 +816,16 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
             float invW = __fdiv_rn(1.0f, poly[k].hom[3]);
             poly[k].hom[0] *= invW;
             poly[k].hom[1] *= invW;
-            to_screen(st, poly[k]);
+            to_screen(stl, poly[k]);
         }
         for (int i = 0; i < n - 2; i++)
-            if (emit_raster<WRITE>(A, st, F, si, face, true, poly[0], poly[i + 1], poly[i + 2], slot0 + emitted)) emitted++;
+            if (emit_raster<PHASE>(A, st, F, si, face, true, poly[0], poly[i + 1], poly[i + 2], rtBase + nRT,
+                                   pairBase + nPairs, np)) {
+                nRT++;
+                nPairs += np;
+            }
     }
-    if (!WRITE) A.count[gid] = emitted;
 }
-
-// ---------------------------------------------------------------------------
-// Device-wide exclusive scan of uint32 (reduce, top, down)
-// ---------------------------------------------------------------------------
-
-constexpr int SCAN_THREADS = 1024;
-constexpr int SCAN_ITEMS = 4;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -818,6 +856,69 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
     return r;
 }
 
+// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
+// per block; PHASE 1 turns the block sums (scanned in between) and a
+// block-local scan into output slots and writes records and pairs.
+template <int PHASE, bool LIT>
+__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    if (PHASE == 0) {
+        // Reset the tile ranges of this pass (the raster reads [begin, end)).
+        for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
+    }
+    uint32_t rtBase = 0, pairBase = 0;
+    if (PHASE == 1) {
+        uint32_t tot;
+        uint32_t cR = valid ? A.cntRT[gid] : 0u, cP = valid ? A.cntPairs[gid] : 0u;
+        rtBase = block_excl_scan(cR, lds, tot) + A.blockRT[blockIdx.x];
+        pairBase = block_excl_scan(cP, lds, tot) + A.blockPairs[blockIdx.x];
+    }
+    uint32_t nRT = 0, nPairs = 0;
+    uint32_t t = 0, info = 0, si = 0;
+    DFrame F;
+    if (valid) {
+        // Segment of this thread: the last one whose passTriBase <= gid.
+        uint32_t lo = 0, hi = A.numSegs;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (A.segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
+        }
+        const DSeg sg = A.segs[lo];
+        t = sg.triFirst + (gid - sg.passTriBase);
+        F = A.frames[sg.frame];
+        info = A.info[t];
+        si = info & PFF_STATE_MASK;
+    }
+    // One iteration per distinct state in the wavefront: the state is then
+    // uniform and read with scalar loads.
+    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
+        const bool mine = valid && si == s0;
+        todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        if (mine)
+            setup_triangle<PHASE, LIT>(A, as_const(A.states + s0), s0, F, t, info, rtBase, pairBase, nRT, nPairs);
+    }
+    if (PHASE == 0) {
+        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
+        uint32_t totR, totP;
+        block_excl_scan(nRT, lds, totR);
+        block_excl_scan(nPairs, lds, totP);
+        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of uint32 (reduce, top, down)
+// ---------------------------------------------------------------------------
+
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
                                                               uint32_t* __restrict__ blockSums) {
     __shared__ uint32_t lds[64];
@@ -830,23 +931,28 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __
     if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
 }
 
-// One block scans all block sums in place (exclusive) and writes the total.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* __restrict__ blockSums, uint32_t nb,
+// One block scans an array in place (exclusive) and writes its total;
+// with a second array (b != nullptr) it scans that one too.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t nb,
                                                            uint32_t* __restrict__ totalOut) {
     __shared__ uint32_t lds[64];
-    uint32_t carry = 0;
-    for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_TILE) {
-        uint32_t base = c0 + threadIdx.x * SCAN_ITEMS;
-        uint32_t v[SCAN_ITEMS], s = 0;
+    for (int which = 0; which < 2; which++) {
+        uint32_t* arr = which ? b : a;
+        if (!arr) break;
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < nb; c0 += SCAN_TILE) {
+            uint32_t base = c0 + threadIdx.x * SCAN_ITEMS;
+            uint32_t v[SCAN_ITEMS], s = 0;
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; i++) { v[i] = (base + i < nb) ? blockSums[base + i] : 0u; s += v[i]; }
-        uint32_t total;
-        uint32_t ex = block_excl_scan(s, lds, total) + carry;
+            for (int i = 0; i < SCAN_ITEMS; i++) { v[i] = (base + i < nb) ? arr[base + i] : 0u; s += v[i]; }
+            uint32_t total;
+            uint32_t ex = block_excl_scan(s, lds, total) + carry;
 #pragma unroll
-        for (int i = 0; i < SCAN_ITEMS; i++) { if (base + i < nb) blockSums[base + i] = ex; ex += v[i]; }
-        carry += total;
+            for (int i = 0; i < SCAN_ITEMS; i++) { if (base + i < nb) arr[base + i] = ex; ex += v[i]; }
+            carry += total;
+        }
+        if (threadIdx.x == 0) totalOut[which] = carry;
     }
-    if (threadIdx.x == 0) *totalOut = carry;
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* __restrict__ in, uint32_t n,
@@ -871,28 +977,6 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 // Tile pairs and the stable radix sort
 // ---------------------------------------------------------------------------
 
-
-__global__ __launch_bounds__(256) void k_pair_emit(const RTri* __restrict__ rts, const uint32_t* __restrict__ rtFrame,
-                                                   const DFrame* __restrict__ frames, const uint32_t* __restrict__ pairOffset,
-                                                   uint32_t numRT, uint32_t ts, uint32_t capPairs,
-                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= numRT) return;
-    const RTri& t = rts[r];
-    const DFrame F = frames[rtFrame[r]];
-    int32_t cx0 = max(t.xMin, 0), cy0 = max(t.yMin, 0);
-    int32_t cx1 = min(t.xMax - 1, (int32_t)F.W - 1), cy1 = min(t.yMax, (int32_t)F.H - 1);
-    uint32_t tx0 = (uint32_t)cx0 >> ts, tx1 = (uint32_t)cx1 >> ts;
-    uint32_t ty0 = (uint32_t)cy0 >> ts, ty1 = (uint32_t)cy1 >> ts;
-    uint32_t p = pairOffset[r];
-    for (uint32_t ty = ty0; ty <= ty1; ty++)
-        for (uint32_t tx = tx0; tx <= tx1; tx++) {
-            if (p >= capPairs) return;
-            keys[p] = F.tileBase + ty * F.tilesX + tx;
-            vals[p] = r;
-            p++;
-        }
-}
 
 constexpr int RADIX_THREADS = 256;
 constexpr int RADIX_ROUNDS = 16;
@@ -966,22 +1050,6 @@ __global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
     }
 }
 
-// Raster triangle -> frame index, for pair emission.
-__global__ void k_rt_frame(const uint32_t* __restrict__ offset, const uint32_t* __restrict__ count, const DSeg* __restrict__ segs,
-                           uint32_t numSegs, uint32_t numPassTris, uint32_t capRT, uint32_t* __restrict__ rtFrame) {
-    uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= numPassTris) return;
-    uint32_t n = count[gid];
-    if (!n) return;
-    uint32_t lo = 0, hi = numSegs;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
-    }
-    uint32_t f = segs[lo].frame, o = offset[gid];
-    for (uint32_t k = 0; k < n && o + k < capRT; k++) rtFrame[o + k] = f;
-}
-
 // ---------------------------------------------------------------------------
 // Tile raster
 // ---------------------------------------------------------------------------
@@ -1002,27 +1070,72 @@ struct RasterArgs {
     unsigned long long* fragCounter;
 };
 
-// Shades one covered, depth-passing set of lanes of an 8x8 block.
-__device__ __forceinline__ void shade_block(const RasterArgs& A, const RTri& t, uint32_t meta, uint32_t rtIdx,
-                                            int32_t w1, int32_t w2, int32_t w3, bool cov,
-                                            uint32_t& col, float& dep) {
-    float b1 = (float)w1 * t.wInvSum, b2 = (float)w2 * t.wInvSum, b3 = (float)w3 * t.wInvSum;
-    float zq = rcp_x86((t.z[0] * b1 + t.z[1] * b2) + t.z[2] * b3);
-    bool pass = cov;
-    if (meta & META_DEPTH) {
-        uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
-        bool ok;
-        switch (fn) {
-        case 0: case 1: ok = zq == dep; break;
-        case 2: ok = zq < dep; break;
-        case 3: ok = zq <= dep; break;
-        case 4: ok = zq > dep; break;
-        default: ok = zq >= dep; break;
-        }
-        pass = pass && ok;
+// ---------------------------------------------------------------------------
+// Tile raster
+//
+// Each wave owns one 8x8 block of a tile (one pixel per lane; colour and
+// depth stay in VGPRs from the fused clear / load to the single write-back)
+// and walks the tile's pair list in batches of 32: the 32 lanes fetch 32
+// list entries and their raster-triangle records in one round trip, keep
+// the ones whose bbox meets the block and stage them, compacted and still in
+// list order, in the wave's LDS.  Each staged triangle is then tested
+// against the block (edge functions per lane, ballot).  Two ways to shade:
+//   packed   (mode 0): covered (triangle, pixel) fragments are queued in LDS
+//            with a per-pixel chain in submission order; when 64 are queued
+//            lane i shades fragment i, then every lane walks its own pixel's
+//            chain doing the depth test and blend.  Shading does not read
+//            the framebuffer, so shading before testing gives the same
+//            pixels as the reference's order.
+//   in place (mode 1): the lanes a triangle covers shade and update their
+//            pixel immediately (best when triangles cover whole blocks).
+// ---------------------------------------------------------------------------
+
+constexpr int BATCH = 32;
+
+template <bool PHONG>
+struct WaveQueueT {
+    uint32_t rec[BATCH][32];                // staged RTri records
+    uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
+    uint32_t fcol[64];
+    float    fz[64];
+    uint8_t  ftri[64];
+    uint8_t  fpix[64];
+    uint8_t  fnext[64];
+    uint8_t  pad[64];
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void edge_values(const RTri& t, int px, int py, int32_t& w1, int32_t& w2, int32_t& w3) {
+    if (t.meta & META_SMALL) {
+        // 24x24-bit multiplies keep the low 32 bits of the product: the same
+        // value mod 2^32 as the full multiply when |A|, |B| < 2^23.
+        w1 = __mul24(t.A[0], px) + __mul24(t.B[0], py) + t.C[0];
+        w2 = __mul24(t.A[1], px) + __mul24(t.B[1], py) + t.C[1];
+        w3 = __mul24(t.A[2], px) + __mul24(t.B[2], py) + t.C[2];
+        return;
     }
-    if (!__builtin_amdgcn_ballot_w64(pass)) return;
-    if (!pass) return;
+    const uint32_t ux = (uint32_t)px, uy = (uint32_t)py;
+    w1 = (int32_t)((uint32_t)t.A[0] * ux + (uint32_t)t.B[0] * uy + (uint32_t)t.C[0]);
+    w2 = (int32_t)((uint32_t)t.A[1] * ux + (uint32_t)t.B[1] * uy + (uint32_t)t.C[1]);
+    w3 = (int32_t)((uint32_t)t.A[2] * ux + (uint32_t)t.B[2] * uy + (uint32_t)t.C[2]);
+}
+
+// Colour and depth of one fragment of raster triangle t at pixel (px, py)
+// (triangles.c:422-432, 500-521).
+template <bool PHONG>
+__device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& t, const float* lrec,
+                                               int px, int py, uint32_t& outCol, float& outZ) {
+    int32_t w1, w2, w3;
+    edge_values(t, px, py, w1, w2, w3);
+    const float inv = t.wInvSum;
+    float b1 = (float)w1 * inv, b2 = (float)w2 * inv, b3 = (float)w3 * inv;
+    float zq = rcp_x86((t.z[0] * b1 + t.z[1] * b2) + t.z[2] * b3);
+    const uint32_t meta = t.meta;
     uint32_t frag = (meta & META_SMOOTH) ? bary_smooth(t.c[0], t.c[1], t.c[2], b1, b2, b3)
                                          : bary_flat(t.c[0], t.c[1], t.c[2], b1, b2, b3);
     if (meta & META_TEX) {
@@ -1033,30 +1146,178 @@ __device__ __forceinline__ void shade_block(const RasterArgs& A, const RTri& t, 
         if (meta & META_3D) { u = u * zq; v = v * zq; }
         frag = modulate(sample_tex(tex, u, v), frag);
     }
-    if (meta & META_PHONG) {
+    if (PHONG && (meta & META_PHONG)) {
         const PFFstate* st = &A.states[t.state];
-        const RTriL& l = A.rtl[rtIdx];
         float N[3], P[3];
         for (int i = 0; i < 3; i++) {
-            N[i] = (l.n[i] * b1 + l.n[3 + i] * b2) + l.n[6 + i] * b3;
-            P[i] = (l.p[i] * b1 + l.p[3 + i] * b2) + l.p[6 + i] * b3;
+            N[i] = (lrec[i] * b1 + lrec[3 + i] * b2) + lrec[6 + i] * b3;
+            P[i] = (lrec[9 + i] * b1 + lrec[12 + i] * b2) + lrec[15 + i] * b3;
         }
         frag = light_fragment(st, st->material[(meta & META_BACK) ? 1 : 0], frag, P, N);
     }
-    if (meta & META_BLEND) frag = blend_op((meta >> META_BMODE_SH) & 7u, frag, col);
-    col = frag;
-    dep = zq;
+    outCol = frag;
+    outZ = zq;
 }
 
-// One workgroup per tile of TS x TS pixels; NW waves split the tile's 8x8
-// blocks, each lane owning one pixel of each of its blocks.
-template <int TS, int NW>
-__global__ __launch_bounds__(64 * NW) void k_raster(RasterArgs A) {
-    constexpr int BPR = TS / 8;                       // 8x8 blocks per tile row
-    constexpr int PW = (BPR * BPR) / NW;              // blocks per wave
-    const uint32_t tile = blockIdx.x;
+// Depth test and blend of one fragment against the lane's pixel
+// (depth.h:80-124, blend.h:137-274, triangles.c:434-438, 523-529).
+__device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t& col, float& dep) {
+    if (meta & META_DEPTH) {
+        uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
+        bool ok;
+        switch (fn) {
+        case 0: case 1: ok = z == dep; break;
+        case 2: ok = z < dep; break;
+        case 3: ok = z <= dep; break;
+        case 4: ok = z > dep; break;
+        default: ok = z >= dep; break;
+        }
+        if (!ok) return;
+    }
+    if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
+    col = c;
+    dep = z;
+}
+
+// Shades the queued fragments and applies them in order (packed mode).
+template <bool PHONG>
+__device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
+                                            int nf, int head, uint32_t& col, float& dep) {
+    wave_sync();
+    if (lane < nf) {
+        const int j = Q.ftri[lane];
+        const int p = Q.fpix[lane];
+        const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+        uint32_t c;
+        float z;
+        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p & 7), by0 + (p >> 3), c, z);
+        Q.fcol[lane] = c;
+        Q.fz[lane] = z;
+    }
+    wave_sync();
+    for (int s = head; s >= 0;) {
+        const int j = Q.ftri[s];
+        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], col, dep);
+        const int nx = Q.fnext[s];
+        s = (nx == 0xFF) ? -1 : nx;
+    }
+    wave_sync();
+}
+
+// Renders the 8x8 block at (bx0, by0) of a tile whose pair list is
+// [e0, e1); returns the covered samples it saw.
+template <int MODE, bool PHONG>
+__device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DSeg& sg,
+                                                 int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueT<PHONG>& Q) {
+    const int lane = threadIdx.x & 63;
+    const int px = bx0 + (lane & 7), py = by0 + (lane >> 3);
+    const bool in = px < (int)F.W && py < (int)F.H;
+    const size_t npix = (size_t)F.W * F.H;
+    const size_t aligned = npix - (npix % 8);
+    uint32_t col = 0u;
+    float dep = 0.0f;
+    const size_t pi = in ? (size_t)py * F.W + px : 0;
+    if (in) {
+        if (sg.clear && pi >= 8 && pi < aligned) {                 // context.c:693-713
+            col = sg.clearColor;
+            dep = sg.clearDepth;
+        } else {
+            const size_t src = (sg.clear && pi >= aligned) ? 0 : pi;
+            col = F.color[src];
+            dep = F.depth[src];
+        }
+    }
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t frags = 0;
+    for (uint32_t e = e0; e < e1; e += BATCH) {
+        // Fetch up to 32 list entries and their records; keep the ones
+        // that meet this block, compacted in list order.
+        const uint32_t nb = min((uint32_t)BATCH, e1 - e);
+        bool ov = false;
+        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+        uint4 q0 = z4, q1 = z4, q2 = z4, q3 = z4, q4 = z4, q5 = z4, q6 = z4, q7 = z4;
+        uint32_t r = 0;
+        if ((uint32_t)lane < nb) {
+            r = A.vals[e + lane];
+            const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
+            q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
+            q4 = src[4]; q5 = src[5]; q6 = src[6]; q7 = src[7];
+            const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
+            const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
+            ov = !(xMax <= bx0 || xMin > bx0 + 7 || yMax < by0 || yMin > by0 + 7);
+        }
+        const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
+        const int nt = __popcll(OV);
+        if (ov) {
+            const int slot = (int)__popcll(OV & ltMask);
+            uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
+            dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
+            dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
+            if (PHONG && (q5.x & META_PHONG)) {
+                const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
+                uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
+#pragma unroll
+                for (int k = 0; k < 5; k++) ld[k] = ls[k];
+            }
+        }
+        wave_sync();
+        int nf = 0, head = -1, tail = -1;
+        for (int j = 0; j < nt; j++) {
+            const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+            int32_t w1, w2, w3;
+            edge_values(t, px, py, w1, w2, w3);
+            const bool cov = in && ((w1 | w2 | w3) > 0) && px >= t.xMin && px < t.xMax
+                             && py >= t.yMin && py <= t.yMax;
+            const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
+            if (!M) continue;
+            const int c = __popcll(M);
+            frags += (uint32_t)c;
+            if (MODE == 1) {
+                if (cov) {
+                    uint32_t fc;
+                    float fz;
+                    shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), px, py, fc, fz);
+                    rop(t.meta, fc, fz, col, dep);
+                }
+                continue;
+            }
+            if (nf + c > 64) {
+                flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep);
+                nf = 0;
+                head = tail = -1;
+            }
+            if (cov) {
+                const int slot = nf + (int)__popcll(M & ltMask);
+                Q.ftri[slot] = (uint8_t)j;
+                Q.fpix[slot] = (uint8_t)lane;
+                Q.fnext[slot] = 0xFF;
+                if (tail >= 0) Q.fnext[tail] = (uint8_t)slot; else head = slot;
+                tail = slot;
+            }
+            nf += c;
+        }
+        if (MODE == 0 && nf > 0) flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep);
+        wave_sync();
+    }
+    if (in) {
+        F.color[pi] = col;
+        F.depth[pi] = dep;
+    }
+    return frags;
+}
+
+// Four waves per workgroup.  TS = 8: four tiles per workgroup, one per
+// wave; TS >= 16: one tile per workgroup, its 8x8 blocks dealt round-robin
+// to the four waves.
+template <int TS, int MODE, bool PHONG>
+__global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
+    __shared__ WaveQueueT<PHONG> Qs[4];
+    constexpr int BPR = TS / 8;
+    constexpr int NB = BPR * BPR;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    WaveQueueT<PHONG>& Q = Qs[wave];
+    const uint32_t tile = (NB == 1) ? blockIdx.x * 4 + wave : blockIdx.x;
     if (tile >= A.numTiles) return;
-    // Frame of this tile: last frame whose tileBase <= tile.
     uint32_t lo = 0, hi = A.numFrames;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -1068,70 +1329,15 @@ __global__ __launch_bounds__(64 * NW) void k_raster(RasterArgs A) {
     const DSeg sg = A.segs[segIdx];
     const uint32_t e0 = A.tileBegin[tile], e1 = A.tileEnd[tile];
     if (e0 == e1 && !sg.clear) return;
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t local = tile - F.tileBase;
     const int tx = (int)(local % F.tilesX), ty = (int)(local / F.tilesX);
-    const size_t npix = (size_t)F.W * F.H;
-    const size_t aligned = npix - (npix % 8);
-
-    int px[PW], py[PW];
-    uint32_t col[PW];
-    float dep[PW];
-    bool in[PW];
-#pragma unroll
-    for (int p = 0; p < PW; p++) {
-        const int blk = wave * PW + p;
-        px[p] = tx * TS + (blk % BPR) * 8 + (lane & 7);
-        py[p] = ty * TS + (blk / BPR) * 8 + (lane >> 3);
-        in[p] = px[p] < (int)F.W && py[p] < (int)F.H;
-        col[p] = 0u;
-        dep[p] = 0.0f;
-        if (in[p]) {
-            size_t i = (size_t)py[p] * F.W + px[p];
-            if (sg.clear && i >= 8 && i < aligned) {         // context.c:693-713
-                col[p] = sg.clearColor;
-                dep[p] = sg.clearDepth;
-            } else {
-                size_t src = (sg.clear && i >= aligned) ? 0 : i;
-                col[p] = F.color[src];
-                dep[p] = F.depth[src];
-            }
-        }
-    }
-
     uint32_t frags = 0;
-    for (uint32_t e = e0; e < e1; e++) {
-        const uint32_t r = __builtin_amdgcn_readfirstlane(A.vals[e]);
-        const RTri& t = A.rts[r];
-        const uint32_t meta = t.meta;
-#pragma unroll
-        for (int p = 0; p < PW; p++) {
-            const int bx0 = tx * TS + ((wave * PW + p) % BPR) * 8;
-            const int by0 = ty * TS + ((wave * PW + p) / BPR) * 8;
-            if (t.xMax <= bx0 || t.xMin > bx0 + 7 || t.yMax < by0 || t.yMin > by0 + 7) continue;
-            const uint32_t ux = (uint32_t)px[p], uy = (uint32_t)py[p];
-            int32_t w1 = (int32_t)((uint32_t)t.A[0] * ux + (uint32_t)t.B[0] * uy + (uint32_t)t.C[0]);
-            int32_t w2 = (int32_t)((uint32_t)t.A[1] * ux + (uint32_t)t.B[1] * uy + (uint32_t)t.C[1]);
-            int32_t w3 = (int32_t)((uint32_t)t.A[2] * ux + (uint32_t)t.B[2] * uy + (uint32_t)t.C[2]);
-            bool cov = in[p] && ((w1 | w2 | w3) > 0) && px[p] >= t.xMin && px[p] < t.xMax
-                       && py[p] >= t.yMin && py[p] <= t.yMax;
-            uint64_t bal = __builtin_amdgcn_ballot_w64(cov);
-            if (!bal) continue;
-            frags += (uint32_t)__popcll(bal);
-            shade_block(A, t, meta, r, w1, w2, w3, cov, col[p], dep[p]);
-        }
+    for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
+        const int bx0 = tx * TS + (b % BPR) * 8, by0 = ty * TS + (b / BPR) * 8;
+        if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
+        frags += raster_block<MODE, PHONG>(A, F, sg, bx0, by0, e0, e1, Q);
     }
-
-#pragma unroll
-    for (int p = 0; p < PW; p++) {
-        if (in[p]) {
-            size_t i = (size_t)py[p] * F.W + px[p];
-            F.color[i] = col[p];
-            F.depth[i] = dep[p];
-        }
-    }
-    if (lane == 0 && frags) atomicAdd(A.fragCounter, (unsigned long long)frags);
+    if ((threadIdx.x & 63) == 0 && frags) atomicAdd(A.fragCounter, (unsigned long long)frags);
 }
 
 // ---------------------------------------------------------------------------
@@ -1200,13 +1406,13 @@ struct PFDjob {
     DevBuf pos, col, uv, nrm, info, states, texTable;
     std::vector<DevBuf> texData;
     uint32_t numIn = 0;
-    bool hasUV = false, hasNrm = false;
+    bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
     std::vector<PassInfo> passes;
     PFDconfig cfg{};
     uint32_t tileShift = 3;
     uint32_t numTiles = 0;
     // scratch
-    DevBuf cnt, off, rts, rtl, rtFrame, pcnt, poff, keysA, valsA, keysB, valsB;
+    DevBuf cnt, off, blockRT, blockPairs, rts, rtl, keysA, valsA, keysB, valsB;
     DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, fragCounter;
     uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
     // graph
@@ -1225,7 +1431,7 @@ int scan_u32(PFDjob* J, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t*
     if (rc) return rc;
     uint32_t* bs = (uint32_t*)J->scanBlocks.p;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, bs);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, bs, nb, totalDev);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, bs, (uint32_t*)nullptr, nb, totalDev);
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, (const uint32_t*)bs, out);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     return 0;
@@ -1244,24 +1450,32 @@ uint32_t grid_for(uint64_t n, uint32_t block) {
     return g == 0 ? 1u : (uint32_t)g;
 }
 
-template <int TS, int NW>
-void launch_raster(const RasterArgs& ra, uint32_t numTiles, hipStream_t s) {
-    hipLaunchKernelGGL((k_raster<TS, NW>), dim3(numTiles), dim3(64 * NW), 0, s, ra);
+template <int TS, int MODE>
+void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, hipStream_t s) {
+    uint32_t grid = (TS == 8) ? (numTiles + 3) / 4 : numTiles;
+    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true>), dim3(grid), dim3(256), 0, s, ra);
+    else       hipLaunchKernelGGL((k_raster<TS, MODE, false>), dim3(grid), dim3(256), 0, s, ra);
 }
 
 // Issues one pass.  With discover set, the raster-triangle and pair counts
 // are read back (synchronising) and scratch buffers are sized; otherwise
-// the sizes of the previous run are used and nothing synchronises.
+// the sizes of the previous run are used and nothing synchronises, so the
+// pass can be captured in a graph.
 int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     int rc = 0;
     if (P.numPassTris == 0 && P.segs.empty()) return 0;
     const uint32_t n = P.numPassTris;
-    if ((rc = ensure(J->cnt, (size_t)n * 4 + 4))) return rc;
-    if ((rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
-    if ((rc = ensure(J->scanTotal, 16))) return rc;
-    if ((rc = ensure(J->fragCounter, 16))) return rc;
-
+    const uint32_t nb = grid_for(n, 256);
+    if ((rc = ensure(J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
+    if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
+    if ((rc = ensure(J->scanTotal, 16)) || (rc = ensure(J->fragCounter, 16))) return rc;
+    if ((rc = ensure(J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J->tileEnd, (size_t)J->numTiles * 4 + 4)))
+        return rc;
+    if (!discover) {
+        // sizes known from the sizing run; buffers already hold them
+    }
     SetupArgs sa;
+    memset(&sa, 0, sizeof sa);
     sa.pos = (const float4*)J->pos.p;
     sa.col = (const uint32_t*)J->col.p;
     sa.uv = J->hasUV ? (const float2*)J->uv.p : nullptr;
@@ -1273,70 +1487,70 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.numSegs = (uint32_t)P.segs.size();
     sa.numPassTris = n;
     sa.tileShift = J->tileShift;
-    sa.count = (uint32_t*)J->cnt.p;
-    sa.offset = (const uint32_t*)J->off.p;
-    sa.rts = nullptr;
-    sa.rtl = nullptr;
-    sa.pairCount = nullptr;
-    sa.capRT = 0;
+    sa.numTiles = J->numTiles;
+    sa.cntRT = (uint32_t*)J->cnt.p;
+    sa.cntPairs = (uint32_t*)J->off.p;
+    sa.blockRT = (uint32_t*)J->blockRT.p;
+    sa.blockPairs = (uint32_t*)J->blockPairs.p;
+    sa.tileBegin = (uint32_t*)J->tileBegin.p;
+    sa.tileEnd = (uint32_t*)J->tileEnd.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
-    uint32_t numRT = 0;
-    if (n) {
-        hipLaunchKernelGGL((k_setup<false>), dim3(grid_for(n, 256)), dim3(256), 0, J->stream, sa);
-        if ((rc = scan_u32(J, (const uint32_t*)J->cnt.p, (uint32_t*)J->off.p, n, (uint32_t*)J->scanTotal.p))) return rc;
-        if (discover) { numRT = read_total(J, &rc); if (rc) return rc; P.numRT = numRT; }
-        else numRT = P.numRT;
+    hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
+                       (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    if (discover) {
+        uint32_t tot[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(tot, J->scanTotal.p, 8, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+        P.numRT = tot[0];
+        P.numPairs = tot[1];
     }
+    const uint32_t numRT = P.numRT, numPairs = P.numPairs;
     if ((rc = ensure(J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
-    if ((rc = ensure(J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
-    if ((rc = ensure(J->rtFrame, (size_t)numRT * 4 + 4))) return rc;
-    if ((rc = ensure(J->pcnt, (size_t)numRT * 4 + 4))) return rc;
-    if ((rc = ensure(J->poff, (size_t)numRT * 4 + 4))) return rc;
-    sa.rts = (RTri*)J->rts.p;
-    sa.rtl = (RTriL*)J->rtl.p;
-    sa.pairCount = (uint32_t*)J->pcnt.p;
-    sa.capRT = numRT;
-    uint32_t numPairs = 0;
-    if (numRT) {
-        hipLaunchKernelGGL((k_setup<true>), dim3(grid_for(n, 256)), dim3(256), 0, J->stream, sa);
-        hipLaunchKernelGGL(k_rt_frame, dim3(grid_for(n, 256)), dim3(256), 0, J->stream,
-                           (const uint32_t*)J->off.p, (const uint32_t*)J->cnt.p, (const DSeg*)P.dSegs.p,
-                           (uint32_t)P.segs.size(), n, numRT, (uint32_t*)J->rtFrame.p);
-        HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
-        if ((rc = scan_u32(J, (const uint32_t*)J->pcnt.p, (uint32_t*)J->poff.p, numRT, (uint32_t*)J->scanTotal.p))) return rc;
-        if (discover) { numPairs = read_total(J, &rc); if (rc) return rc; P.numPairs = numPairs; }
-        else numPairs = P.numPairs;
-    }
-    if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
-
-    // Pairs, sort, tile bounds.
+    if (J->anyPhong && (rc = ensure(J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
     const size_t pb = (size_t)numPairs * 4 + 4;
     if ((rc = ensure(J->keysA, pb)) || (rc = ensure(J->valsA, pb)) || (rc = ensure(J->keysB, pb)) || (rc = ensure(J->valsB, pb)))
         return rc;
-    if ((rc = ensure(J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J->tileEnd, (size_t)J->numTiles * 4 + 4)))
-        return rc;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J->numTiles, 256 * 4)), dim3(256), 0, J->stream,
-                       (uint32_t*)J->tileBegin.p, J->numTiles, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(J->numTiles, 256 * 4)), dim3(256), 0, J->stream,
-                       (uint32_t*)J->tileEnd.p, J->numTiles, 0u);
+    sa.rts = (RTri*)J->rts.p;
+    sa.rtl = (RTriL*)J->rtl.p;
+    sa.keys = (uint32_t*)J->keysA.p;
+    sa.vals = (uint32_t*)J->valsA.p;
+    sa.capRT = numRT;
+    sa.capPairs = numPairs;
+    if (numRT) {
+        if (J->anyLit) hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
+        else           hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+    }
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
+
+    // Stable sort of the pairs by tile, then per-tile ranges.
     uint32_t* keys = (uint32_t*)J->keysA.p;
     uint32_t* vals = (uint32_t*)J->valsA.p;
     if (numPairs) {
-        hipLaunchKernelGGL(k_pair_emit, dim3(grid_for(numRT, 256)), dim3(256), 0, J->stream,
-                           (const RTri*)J->rts.p, (const uint32_t*)J->rtFrame.p, (const DFrame*)J->dFrames.p,
-                           (const uint32_t*)J->poff.p, numRT, J->tileShift, numPairs, keys, vals);
         uint32_t bits = 1;
         while (bits < 32 && (1u << bits) < J->numTiles) bits++;
-        uint32_t nb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;
-        if ((rc = ensure(J->hist, (size_t)nb * 256 * 4)) || (rc = ensure(J->histScan, (size_t)nb * 256 * 4 + 4))) return rc;
+        const uint32_t rb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;
+        const uint32_t nh = rb * 256;
+        if ((rc = ensure(J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J->histScan, (size_t)nh * 4 + 4))) return rc;
         uint32_t* kIn = keys; uint32_t* vIn = vals;
         uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
         for (uint32_t shift = 0; shift < bits; shift += 8) {
-            hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, nb, (uint32_t*)J->hist.p);
-            if ((rc = scan_u32(J, (const uint32_t*)J->hist.p, (uint32_t*)J->histScan.p, nb * 256, (uint32_t*)J->scanTotal.p))) return rc;
-            hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift, nb,
-                               (const uint32_t*)J->histScan.p, kOut, vOut);
+            hipLaunchKernelGGL(k_radix_hist, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb, (uint32_t*)J->hist.p);
+            const uint32_t* scanned;
+            if (nh <= 65536) {
+                hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, (uint32_t*)J->hist.p,
+                                   (uint32_t*)nullptr, nh, (uint32_t*)J->scanTotal.p + 2);
+                scanned = (const uint32_t*)J->hist.p;
+            } else {
+                if ((rc = scan_u32(J, (const uint32_t*)J->hist.p, (uint32_t*)J->histScan.p, nh, (uint32_t*)J->scanTotal.p + 2)))
+                    return rc;
+                scanned = (const uint32_t*)J->histScan.p;
+            }
+            hipLaunchKernelGGL(k_radix_scatter, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift, rb,
+                               scanned, kOut, vOut);
             std::swap(kIn, kOut);
             std::swap(vIn, vOut);
         }
@@ -1362,11 +1576,20 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.numFrames = (uint32_t)J->frames.size();
     ra.numTiles = J->numTiles;
     ra.fragCounter = (unsigned long long*)J->fragCounter.p;
-    switch (J->tileShift) {
-    case 3: launch_raster<8, 1>(ra, J->numTiles, J->stream); break;
-    case 4: launch_raster<16, 1>(ra, J->numTiles, J->stream); break;
-    case 5: launch_raster<32, 4>(ra, J->numTiles, J->stream); break;
-    default: launch_raster<64, 4>(ra, J->numTiles, J->stream); break;
+    if (J->cfg.rasterMode == 1) {
+        switch (J->tileShift) {
+        case 3: launch_raster<8, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        case 4: launch_raster<16, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        case 5: launch_raster<32, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        default: launch_raster<64, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        }
+    } else {
+        switch (J->tileShift) {
+        case 3: launch_raster<8, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        case 4: launch_raster<16, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        case 5: launch_raster<32, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        default: launch_raster<64, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
+        }
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
@@ -1439,13 +1662,21 @@ int pfdInit(void) {
 
 const char* pfdLastError(void) { return g_err; }
 
+static PFDconfig g_defaultCfg = {3u, 0u, {0, 0, 0, 0, 0, 0}};
+
+int pfdSetDefaultConfig(const PFDconfig* cfg) {
+    if (!cfg) return PFD_E_INVALID;
+    g_defaultCfg = *cfg;
+    return PFD_OK;
+}
+
 int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out) {
     if (!out || !frames || count == 0) { snprintf(g_err, sizeof g_err, "pfdJobCreate: bad arguments"); return PFD_E_INVALID; }
     *out = nullptr;
     int rc = pfdInit();
     if (rc) return rc;
     PFDjob* J = new PFDjob();
-    if (cfg) J->cfg = *cfg;
+    J->cfg = cfg ? *cfg : g_defaultCfg;
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
     hipError_t e = hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { set_error("hipStreamCreate", e); delete J; return PFD_E_NODEVICE; }
@@ -1509,6 +1740,8 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         }
         for (uint32_t s = 0; s < fr.numStates; s++) {
             PFFstate st = fr.states[s];
+            if ((st.flags & PFF_LIGHTING) && st.lightMode == 1 && st.numLights > 0) J->anyPhong = true;
+            if (st.flags & PFF_LIGHTING) J->anyLit = true;
             if (st.texture >= 0) {
                 if ((uint32_t)st.texture >= fr.numTextures) st.texture = -1;
                 else st.texture += (int32_t)texBase;
@@ -1607,6 +1840,7 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
 
 int pfdJobSetConfig(PFDjob* J, const PFDconfig* cfg) {
     if (!J || !cfg) return PFD_E_INVALID;
+    if (cfg->rasterMode != J->cfg.rasterMode) drop_graph(J);
     J->cfg = *cfg;
     uint32_t ts = (cfg->tileShift >= 3 && cfg->tileShift <= 6) ? cfg->tileShift : 3;
     if (ts != J->tileShift) {
@@ -1732,34 +1966,45 @@ int pfdJobStats(PFDjob* J, PFDstats* st) {
     st->tileShift = J->tileShift;
     size_t bytes = 0;
     DevBuf* bufs[] = { &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off, &J->rts,
-                       &J->rtl, &J->rtFrame, &J->pcnt, &J->poff, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
+                       &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
                        &J->scanBlocks, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd };
     for (DevBuf* b : bufs) bytes += b->bytes;
     for (auto& b : J->targets) bytes += b.bytes;
     for (auto& b : J->texData) bytes += b.bytes;
     st->deviceBytes = bytes;
+    if (J->tileBegin.p && J->numTiles) {
+        std::vector<uint32_t> b(J->numTiles), e(J->numTiles);
+        HIP_TRY(hipMemcpy(b.data(), J->tileBegin.p, (size_t)J->numTiles * 4, hipMemcpyDeviceToHost), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpy(e.data(), J->tileEnd.p, (size_t)J->numTiles * 4, hipMemcpyDeviceToHost), PFD_E_LAUNCH);
+        for (uint32_t i = 0; i < J->numTiles; i++) {
+            uint32_t n = e[i] - b[i];
+            if (n) st->busyTiles++;
+            if (n > st->maxTileList) st->maxTileList = n;
+        }
+    }
     return PFD_OK;
 }
 
-// Times every tile size on the job (one sizing run plus a few graph
-// replays each) and keeps the fastest.  The target buffers are restored
-// from the host before each candidate so the job's output is unchanged.
+// Times every tile size and raster mode on the job (one sizing run plus a
+// few graph replays each) and keeps the fastest.  The target buffers are
+// restored from the host afterwards so the job's output is unchanged.
 int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if (!J) return PFD_E_INVALID;
     float best = 1e30f;
-    uint32_t bestShift = J->tileShift;
-    for (uint32_t ts = 3; ts <= 6; ts++) {
-        PFDconfig c = J->cfg;
-        c.tileShift = ts;
-        int rc = pfdJobSetConfig(J, &c);
-        if (rc) return rc;
-        if ((rc = pfdJobUploadTargets(J))) return rc;
-        float ms = 0;
-        if ((rc = pfdJobTimeReplays(J, 3, &ms))) return rc;
-        if (ms < best) { best = ms; bestShift = ts; }
-    }
-    PFDconfig c = J->cfg;
-    c.tileShift = bestShift;
+    PFDconfig bestCfg = J->cfg;
+    for (uint32_t mode = 0; mode <= 1; mode++)
+        for (uint32_t ts = 3; ts <= 6; ts++) {
+            PFDconfig c = J->cfg;
+            c.tileShift = ts;
+            c.rasterMode = mode;
+            int rc = pfdJobSetConfig(J, &c);
+            if (rc) return rc;
+            if ((rc = pfdJobUploadTargets(J))) return rc;
+            float ms = 0;
+            if ((rc = pfdJobTimeReplays(J, 3, &ms))) return rc;
+            if (ms < best) { best = ms; bestCfg = c; }
+        }
+    PFDconfig c = bestCfg;
     int rc = pfdJobSetConfig(J, &c);
     if (rc) return rc;
     if ((rc = pfdJobUploadTargets(J))) return rc;
@@ -1772,7 +2017,7 @@ void pfdJobDestroy(PFDjob* J) {
     if (J->stream) (void)hipStreamSynchronize(J->stream);
     drop_graph(J);
     DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
-                       &J->rts, &J->rtl, &J->rtFrame, &J->pcnt, &J->poff, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
+                       &J->rts, &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
                        &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->fragCounter };
     for (DevBuf* b : bufs) release(*b);
     for (auto& b : J->targets) release(b);

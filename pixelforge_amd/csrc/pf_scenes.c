@@ -330,7 +330,7 @@ static void scene_heightfield(const PFSapi* a, Rng* r, const PFSparams* p)
     size_t ntri = (size_t)NX * NZ * 2;
     uint8_t* blended = (uint8_t*)malloc(ntri);
     for (size_t t = 0; t < ntri; t++) blended[t] = runit(r) < p->blendFraction;
-    float ru = 40.0f, rv = 8.0f;
+    float ru = 4.0f, rv = 2.0f;      /* texture repeats across the grid: about one texel per pixel up front */
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1) {
             if (p->blendFraction <= 0.0f) break;

@@ -29,3 +29,26 @@ def test_gpu_matches_oracle(name, kw):
     if os.path.exists(path):
         g = np.load(path, allow_pickle=False)
         assert_same(g["color"], g["depth"], gc, gd, name + " vs golden")
+
+
+CONFIG_SWEEP = ["c1_ndc_less", "c2_heightfield_nearest", "c3_spheres_phong", "blend_1", "clip_3d",
+                "odd_size_clear", "multi_frame", "bilinear_0"]
+
+
+@pytest.mark.parametrize("tile_shift", [3, 4, 5, 6])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_every_tile_size_and_raster_mode(tile_shift, mode):
+    """Tile size and raster mode are performance knobs only: every
+    configuration renders the oracle's frame bit for bit."""
+    table = dict(scenes.ALL_SCENES)
+    try:
+        pf.set_default_config(tile_shift, mode)
+        for name in CONFIG_SWEEP:
+            print(f"config {tile_shift}/{mode} scene {name}", flush=True)
+            p = scenes.params(table[name])
+            oc, od, _, _ = render_oracle(p)
+            gc, gd, ge = render_gpu(p)
+            assert ge == 0
+            assert_same(oc, od, gc, gd, f"{name} tile {1 << tile_shift} mode {mode}")
+    finally:
+        pf.set_default_config(3, 0)
