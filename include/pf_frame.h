@@ -32,6 +32,8 @@
 #define PFF_PASS_FRONT   0x01000000u /* run the FRONT pass (triangles.c:303-305) */
 #define PFF_PASS_BACK    0x02000000u /* run the BACK pass; after FRONT if both  */
 
+#define PFF_NO_VERTEX    0xFFFFFFFFu /* corner without a colour-array index */
+
 /* Texture pixel formats accepted by the pipeline. */
 #define PFF_TEX_RGBA8    0u         /* pixel.h:696-700, 2909-2913 */
 #define PFF_TEX_RGBAF32  1u         /* pixel.h:735-746, 2983-3008 */
@@ -42,7 +44,8 @@ typedef struct {
     float    innerCutOff, outerCutOff;      /* cosines; >= pi disables the spot */
     float    attConstant, attLinear, attQuadratic;
     uint32_t ambient, diffuse, specular;    /* packed RGBA8, r in the low byte */
-    uint32_t pad[3];
+    uint32_t id;                            /* light number (pfEnableLight)    */
+    uint32_t pad[2];
 } PFFlight;                                 /* 80 bytes; context.h:216-229 */
 
 typedef struct {
@@ -99,6 +102,10 @@ typedef struct {
     const float*    texcoord;  /* [count*3][2]; NULL when no state textures   */
     const float*    normal;    /* [count*3][3]; NULL when no state lights     */
     const uint32_t* info;      /* [count] state index | PFF_PASS_* bits       */
+    const uint32_t* vertexId;  /* [count*3] colour-array index of each corner
+                                  (pfDrawArrays / pfDrawElements with a colour
+                                  array), PFF_NO_VERTEX otherwise; NULL when
+                                  no corner has one                          */
 } PFFtriangles;
 
 typedef struct {

@@ -197,6 +197,48 @@ typedef enum {
     PF_QUADRATIC_ATTENUATION    = 14
 } PFlightparam;
 
+/* Queries of pfGet*v (pixelforge.h:185-247). */
+typedef enum {
+    PF_VIEWPORT                         = 10000,
+    PF_COLOR_CLEAR_VALUE,
+    PF_DEPTH_CLEAR_VALUE,
+    PF_CULL_FACE_MODE,
+    PF_CURRENT_COLOR,
+    PF_CURRENT_NORMAL,
+    PF_CURRENT_TEXTURE_COORDS,
+    PF_CURRENT_RASTER_POSITION,
+    PF_BLEND_FUNC,
+    PF_DEPTH_FUNC,
+    PF_POLYGON_MODE,
+    PF_POINT_SIZE,
+    PF_LINE_WIDTH,
+    PF_MATRIX_MODE,
+    PF_PROJECTION_MATRIX,
+    PF_MODELVIEW_MATRIX,
+    PF_TEXTURE_MATRIX,
+    PF_MAX_PROJECTION_STACK_DEPTH,
+    PF_MAX_MODELVIEW_STACK_DEPTH,
+    PF_MAX_TEXTURE_STACK_DEPTH,
+    PF_SHADE_MODEL,
+    PF_MAX_LIGHTS,
+    PF_VERTEX_ARRAY_SIZE,
+    PF_VERTEX_ARRAY_STRIDE,
+    PF_VERTEX_ARRAY_TYPE,
+    PF_NORMAL_ARRAY_STRIDE,
+    PF_NORMAL_ARRAY_TYPE,
+    PF_TEXTURE_COORD_ARRAY_STRIDE,
+    PF_TEXTURE_COORD_ARRAY_TYPE,
+    PF_COLOR_ARRAY_SIZE,
+    PF_COLOR_ARRAY_STRIDE,
+    PF_COLOR_ARRAY_TYPE,
+    PF_ZOOM_X,
+    PF_ZOOM_Y
+} PFgettable;
+
+/* pixelforge.h:385-397 */
+typedef enum { PF_FOG_MODE, PF_FOG_DENSITY, PF_FOG_START, PF_FOG_END, PF_FOG_COLOR } PFfogparam;
+typedef enum { PF_LINEAR, PF_EXP, PF_EXP2 } PFfogmode;
+
 /* pixelforge.h:398-400: bytes r,g,b,a in memory order. */
 typedef struct { PFubyte r, g, b, a; } PFcolor;
 
@@ -206,6 +248,9 @@ typedef enum {
     PF_LUMINANCE, PF_LUMINANCE_ALPHA,
     PF_RGB, PF_RGBA, PF_BGR, PF_BGRA
 } PFpixelformat;
+
+/* pixelforge.h:403 */
+typedef PFcolor (*PFpostprocessfunc)(PFint, PFint, PFfloat, PFcolor);
 
 /* pixelforge.h:428-431: callers may dereference zbuffer directly. */
 typedef struct {
@@ -234,6 +279,11 @@ PF_API PFboolean pfIsEnabled(PFstate state);                               /* pi
 PF_API void      pfEnable(PFstate state);                                  /* pixelforge.h:549 */
 PF_API void      pfDisable(PFstate state);                                 /* pixelforge.h:559 */
 PF_API PFerrcode pfGetError(void);                                         /* pixelforge.h:627 */
+PF_API void      pfGetBooleanv(PFenum pname, PFboolean* params);           /* pixelforge.h:574 */
+PF_API void      pfGetIntegerv(PFenum pname, PFint* params);               /* pixelforge.h:585 */
+PF_API void      pfGetFloatv(PFenum pname, PFfloat* params);               /* pixelforge.h:596 */
+PF_API void      pfGetDoublev(PFenum pname, PFdouble* params);             /* pixelforge.h:607 */
+PF_API void      pfGetPointerv(PFenum pname, const void** params);         /* pixelforge.h:618 */
 
 /* ------------------------------------------------------------------------ */
 /* Matrices (pixelforge.h:640-743; context.c:395-548)                        */
@@ -321,6 +371,18 @@ PF_API void pfNormal3f(PFfloat x, PFfloat y, PFfloat z);                   /* pi
 PF_API void pfNormal3fv(const PFfloat* v);                                 /* pixelforge.h:1460 */
 
 /* ------------------------------------------------------------------------ */
+/* Vertex arrays (pixelforge.h:1031-1095; context.c:1161-1575).  As in the   */
+/* reference, `stride` is stored but every array is read tightly packed.     */
+/* ------------------------------------------------------------------------ */
+
+PF_API void pfVertexPointer(PFint size, PFenum type, PFsizei stride, const void* pointer); /* pixelforge.h:1031 */
+PF_API void pfNormalPointer(PFenum type, PFsizei stride, const void* pointer);             /* pixelforge.h:1044 */
+PF_API void pfTexCoordPointer(PFenum type, PFsizei stride, const void* pointer);           /* pixelforge.h:1056 */
+PF_API void pfColorPointer(PFint size, PFenum type, PFsizei stride, const void* pointer);  /* pixelforge.h:1069 */
+PF_API void pfDrawElements(PFdrawmode mode, PFsizei count, PFdatatype type, const void* indices); /* pixelforge.h:1082 */
+PF_API void pfDrawArrays(PFdrawmode mode, PFint first, PFsizei count);                     /* pixelforge.h:1095 */
+
+/* ------------------------------------------------------------------------ */
 /* Readback (pixelforge.h:1717; context.c:2349-2395)                         */
 /* ------------------------------------------------------------------------ */
 
@@ -338,6 +400,11 @@ PF_API PFboolean pfIsValidFramebuffer(PFframebuffer* framebuffer);         /* pi
 PF_API void      pfClearFramebuffer(PFframebuffer* framebuffer, PFcolor color, PFfloat depth); /* pixelforge.h:1923 */
 PF_API PFcolor   pfGetFramebufferPixel(const PFframebuffer* framebuffer, PFsizei x, PFsizei y); /* pixelforge.h:1937 */
 PF_API PFfloat   pfGetFramebufferDepth(const PFframebuffer* framebuffer, PFsizei x, PFsizei y); /* pixelforge.h:1951 */
+PF_API void      pfSetFramebufferPixelDepthTest(PFframebuffer* framebuffer, PFsizei x, PFsizei y, PFfloat z,
+                                                PFcolor color, PFdepthmode depthMode); /* pixelforge.h:1969 */
+PF_API void      pfSetFramebufferPixelDepth(PFframebuffer* framebuffer, PFsizei x, PFsizei y, PFfloat z,
+                                            PFcolor color);                /* pixelforge.h:1986 */
+PF_API void      pfSetFramebufferPixel(PFframebuffer* framebuffer, PFsizei x, PFsizei y, PFcolor color); /* pixelforge.h:2003 */
 
 /* ------------------------------------------------------------------------ */
 /* Textures (pixelforge.h:2024-2085; texture.c:30-123)                       */
@@ -351,6 +418,43 @@ PF_API void      pfTextureParameter(PFtexture texture, PFtexturewrap wrapMode,
                                     PFtexturefilter filterMode);           /* pixelforge.h:2066 */
 PF_API void*     pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFsizei* height,
                                     PFpixelformat* format, PFdatatype* type); /* pixelforge.h:2085 */
+
+/* ------------------------------------------------------------------------ */
+/* Outside the GPU triangle path.  Points, lines, rectangles, pixel          */
+/* transfers, fog, post-processing and render lists keep their signatures so */
+/* that callers link; their state setters store what the getters return, and */
+/* the drawing calls set PF_INVALID_OPERATION without drawing.               */
+/* ------------------------------------------------------------------------ */
+
+PF_API void pfLineWidth(PFfloat width);                                    /* pixelforge.h:823 */
+PF_API void pfPointSize(PFfloat size);                                     /* pixelforge.h:833 */
+PF_API void pfRects(PFshort x1, PFshort y1, PFshort x2, PFshort y2);       /* pixelforge.h:1477 */
+PF_API void pfRectsv(const PFshort* v1, const PFshort* v2);                /* pixelforge.h:1488 */
+PF_API void pfRectf(PFfloat x1, PFfloat y1, PFfloat x2, PFfloat y2);       /* pixelforge.h:1501 */
+PF_API void pfRectfv(const PFfloat* v1, const PFfloat* v2);                /* pixelforge.h:1511 */
+PF_API void pfDrawPixels(PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type,
+                         const void* pixels);                              /* pixelforge.h:1527 */
+PF_API void pfPixelZoom(PFfloat xfactor, PFfloat yfactor);                 /* pixelforge.h:1540 */
+PF_API void pfRasterPos2i(PFint x, PFint y);                               /* pixelforge.h:1551 */
+PF_API void pfRasterPos2f(PFfloat x, PFfloat y);                           /* pixelforge.h:1562 */
+PF_API void pfRasterPos2fv(const PFfloat* v);                              /* pixelforge.h:1572 */
+PF_API void pfRasterPos3i(PFint x, PFint y, PFint z);                      /* pixelforge.h:1584 */
+PF_API void pfRasterPos3f(PFfloat x, PFfloat y, PFfloat z);                /* pixelforge.h:1596 */
+PF_API void pfRasterPos3fv(const PFfloat* v);                              /* pixelforge.h:1606 */
+PF_API void pfRasterPos4i(PFint x, PFint y, PFint z, PFint w);             /* pixelforge.h:1619 */
+PF_API void pfRasterPos4f(PFfloat x, PFfloat y, PFfloat z, PFfloat w);     /* pixelforge.h:1632 */
+PF_API void pfRasterPos4fv(const PFfloat* v);                              /* pixelforge.h:1642 */
+PF_API void pfFogi(PFfogparam pname, PFint param);                         /* pixelforge.h:1656 */
+PF_API void pfFogf(PFfogparam pname, PFfloat param);                       /* pixelforge.h:1667 */
+PF_API void pfFogiv(PFfogparam pname, PFint* param);                       /* pixelforge.h:1678 */
+PF_API void pfFogfv(PFfogparam pname, PFfloat* param);                     /* pixelforge.h:1689 */
+PF_API void pfFogProcess(void);                                            /* pixelforge.h:1697 */
+PF_API void pfPostProcess(PFpostprocessfunc postProcessFunction);          /* pixelforge.h:1735 */
+PF_API PFrenderlist pfGenList(void);                                       /* pixelforge.h:1768 */
+PF_API void pfDeleteList(PFrenderlist* renderList);                        /* pixelforge.h:1793 */
+PF_API void pfNewList(PFrenderlist renderList);                            /* pixelforge.h:1817 */
+PF_API void pfEndList(void);                                               /* pixelforge.h:1838 */
+PF_API void pfCallList(const PFrenderlist renderList);                     /* pixelforge.h:1867 */
 
 /* ------------------------------------------------------------------------ */
 /* New entry points (no reference counterpart)                               */

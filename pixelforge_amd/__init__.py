@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpixelforge_amd.so")
 SCENES_PATH = os.path.join(HERE, "libpf_scenes.so")
 
 # Scene kinds (csrc/pf_scenes.c)
-RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D = 1, 2, 3, 4
+RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D, MESH = 1, 2, 3, 4, 5
 
 # Enum values of include/pixelforge.h used by callers
 PF_FRONT, PF_BACK = 0, 1
@@ -43,6 +43,8 @@ class SceneParams(C.Structure):
         ("sphereLat", C.c_int32), ("sphereLon", C.c_int32),
         ("clearFirst", C.c_int32), ("colorMaterial", C.c_int32),
         ("triScale", C.c_float), ("flushEvery", C.c_int32), ("useQuads", C.c_int32),
+        ("arrays", C.c_int32), ("indexType", C.c_int32), ("posType", C.c_int32),
+        ("colType", C.c_int32), ("colSize", C.c_int32),
     ]
 
 

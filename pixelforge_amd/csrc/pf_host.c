@@ -61,18 +61,26 @@ typedef struct {                       /* PFIvertex, context.h:203-210 */
     float   normal[3];
     float   texcoord[2];
     PFcolor color;
+    uint32_t vid;                      /* colour-array index, PFF_NO_VERTEX */
 } Vert;
+
+typedef struct {                       /* PFIvertexattribbuffer, context.h:242-247 */
+    const void* buffer;
+    PFsizei     stride;                /* stored, never used (context.c:1285) */
+    PFint       size;
+    PFenum      type;
+} Attrib;
 
 typedef struct {                       /* the frame being recorded */
     uint32_t*  targetColor;
     float*     targetDepth;
     uint32_t   targetW, targetH;
-    float*     pos;  uint32_t* col;  float* uv;  float* nrm;  uint32_t* info;
+    float*     pos;  uint32_t* col;  float* uv;  float* nrm;  uint32_t* info;  uint32_t* vid;
     size_t     ntri, captri;
     PFFstate*  states;   size_t nstates, capstates;
     PFFtexture* texs;    const Tex** texKeys; size_t ntex, captex;
     PFFcmd*    cmds;     size_t ncmd, capcmd;
-    int        anyTex, anyLight;
+    int        anyTex, anyLight, anyVid;
 } Rec;
 
 typedef struct {
@@ -112,6 +120,10 @@ typedef struct {
     Vert     vb[6];
     uint32_t vcount;
     Tex*     curTex;
+    Attrib   aPos, aNrm, aUV, aCol;    /* vertex arrays (context.c:1161-1229) */
+
+    float    pointSize, lineWidth;     /* out-of-path state, kept for the getters */
+    float    pixelZoom[2], rasterPos[4];
 
     PFerrcode err;
     Rec      rec;
@@ -258,14 +270,14 @@ static int grow(void** p, size_t* cap, size_t need, size_t elem)
 static void rec_reset(Rec* r)
 {
     r->ntri = r->nstates = r->ntex = r->ncmd = 0;
-    r->anyTex = r->anyLight = 0;
+    r->anyTex = r->anyLight = r->anyVid = 0;
     r->targetColor = NULL;
     r->targetDepth = NULL;
 }
 
 static void rec_free(Rec* r)
 {
-    free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info);
+    free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info); free(r->vid);
     free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
     memset(r, 0, sizeof *r);
 }
@@ -292,6 +304,7 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->tris.texcoord = r->anyTex ? r->uv : NULL;
     f->tris.normal = r->anyLight ? r->nrm : NULL;
     f->tris.info = r->info;
+    f->tris.vertexId = r->anyVid ? r->vid : NULL;
     f->states = r->states;
     f->numStates = (uint32_t)r->nstates;
     f->textures = r->texs;
@@ -415,6 +428,7 @@ static int current_state(Ctx* c)
             d->ambient = pack(L->ambient);
             d->diffuse = pack(L->diffuse);
             d->specular = pack(L->specular);
+            d->id = (uint32_t)li;
         }
     }
     if (r->nstates && memcmp(&r->states[r->nstates - 1], &s, sizeof s) == 0) {
@@ -458,7 +472,9 @@ static void emit_tri(Ctx* c, const Vert* a, const Vert* b, const Vert* v3, uint3
         if (nrm) r->nrm = nrm;
         uint32_t* info = realloc(r->info, nc * sizeof(uint32_t));
         if (info) r->info = info;
-        if (!pos || !col || !uv || !nrm || !info) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+        uint32_t* vid = realloc(r->vid, nc * 3 * sizeof(uint32_t));
+        if (vid) r->vid = vid;
+        if (!pos || !col || !uv || !nrm || !info || !vid) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
         r->captri = nc;
     }
     size_t t = r->ntri;
@@ -469,6 +485,8 @@ static void emit_tri(Ctx* c, const Vert* a, const Vert* b, const Vert* v3, uint3
         r->col[vi] = pack(v[k]->color);
         memcpy(r->uv + 2 * vi, v[k]->texcoord, 8);
         memcpy(r->nrm + 3 * vi, v[k]->normal, 12);
+        r->vid[vi] = v[k]->vid;
+        if (v[k]->vid != PFF_NO_VERTEX) r->anyVid = 1;
     }
     r->info[t] = (uint32_t)si | passes;
     r->ntri++;
@@ -711,6 +729,9 @@ PFcontext pfCreateContext(void* targetBuffer, PFsizei width, PFsizei height, PFp
     }
     c->followFace = PF_FRONT_AND_BACK;
     c->followMode = PF_AMBIENT_AND_DIFFUSE;
+    c->pointSize = c->lineWidth = 1.0f;                                 /* context.c:166-180 */
+    c->pixelZoom[0] = c->pixelZoom[1] = 1.0f;
+    c->rasterPos[3] = 1.0f;
 
     c->matMode = PF_MODELVIEW;                                         /* context.c:232-239 */
     c->curMat = c->matView;
@@ -1194,6 +1215,7 @@ void pfVertex4fv(const PFfloat* v)
     memcpy(d->normal, c->curNormal, 12);
     memcpy(d->texcoord, c->curUV, 8);
     d->color = c->curColor;
+    d->vid = PFF_NO_VERTEX;
     if (c->vcount == mode_vertex_count(c->drawMode)) {
         process_primitive(c);
         reset_vertex_buffer(c);
@@ -1281,6 +1303,360 @@ void pfNormal3f(PFfloat x, PFfloat y, PFfloat z)
 }
 
 void pfNormal3fv(const PFfloat* v) { pfNormal3f(v[0], v[1], v[2]); }
+
+/* ------------------------------------------------------------------------ */
+/* Vertex arrays (context.c:1161-1575)                                       */
+/* ------------------------------------------------------------------------ */
+
+void pfVertexPointer(PFint size, PFenum type, PFsizei stride, const void* pointer)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (size < 2 || size > 4) { c->err = PF_INVALID_VALUE; return; }
+    if (!(type == PF_SHORT || type == PF_INT || type == PF_FLOAT || type == PF_DOUBLE)) { c->err = PF_INVALID_ENUM; return; }
+    c->aPos = (Attrib){ pointer, stride, size, type };
+}
+
+void pfNormalPointer(PFenum type, PFsizei stride, const void* pointer)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!(type == PF_FLOAT || type == PF_DOUBLE)) { c->err = PF_INVALID_ENUM; return; }
+    c->aNrm = (Attrib){ pointer, stride, 3, type };
+}
+
+void pfTexCoordPointer(PFenum type, PFsizei stride, const void* pointer)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!(type == PF_FLOAT || type == PF_DOUBLE)) { c->err = PF_INVALID_ENUM; return; }
+    c->aUV = (Attrib){ pointer, stride, 2, type };
+}
+
+void pfColorPointer(PFint size, PFenum type, PFsizei stride, const void* pointer)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (size < 3 || size > 4) { c->err = PF_INVALID_VALUE; return; }
+    if (!(type == PF_UNSIGNED_BYTE || type == PF_UNSIGNED_SHORT || type == PF_UNSIGNED_INT ||
+          type == PF_FLOAT || type == PF_DOUBLE)) { c->err = PF_INVALID_ENUM; return; }
+    c->aCol = (Attrib){ pointer, stride, size, type };
+}
+
+static PFubyte d2u8(double d)          /* (PFubyte)(double) as gcc emits it on x86-64 */
+{
+    if (!(d > -2147483649.0 && d < 2147483648.0)) return 0;
+    return (PFubyte)(uint32_t)(int32_t)d;
+}
+
+/* Component k of element j of a tightly packed array of `size`-wide
+ * elements (the reference ignores the stride: context.c:1285, 1318). */
+static float attr_f(const Attrib* a, size_t j, int k)
+{
+    size_t i = j * (size_t)a->size + (size_t)k;
+    switch (a->type) {
+    case PF_SHORT:  return (float)((const PFshort*)a->buffer)[i];
+    case PF_INT:    return (float)((const PFint*)a->buffer)[i];
+    case PF_FLOAT:  return ((const PFfloat*)a->buffer)[i];
+    case PF_DOUBLE: return (float)((const PFdouble*)a->buffer)[i];
+    }
+    return 0.0f;
+}
+
+static PFubyte attr_u8(const Attrib* a, size_t j, int k)   /* context.c:1356-1392 */
+{
+    size_t i = j * (size_t)a->size + (size_t)k;
+    switch (a->type) {
+    case PF_UNSIGNED_BYTE:  return ((const PFubyte*)a->buffer)[i];
+    case PF_UNSIGNED_SHORT: return (PFubyte)(((const PFushort*)a->buffer)[i] >> 8);
+    case PF_UNSIGNED_INT:   return (PFubyte)(((const PFuint*)a->buffer)[i] >> 24);
+    case PF_FLOAT:          return f2u8(((const PFfloat*)a->buffer)[i] * 255);
+    case PF_DOUBLE:         return d2u8(((const PFdouble*)a->buffer)[i] * 255);
+    }
+    return 0;
+}
+
+/* The vertex loop shared by pfDrawElements and pfDrawArrays: element j of
+ * the enabled arrays becomes the next vertex of the primitive being built
+ * (context.c:1245-1404, 1427-1575).  Attributes without an enabled array
+ * keep the values the slots were reset to: zero normal and texcoord and the
+ * current colour (context.c:1254-1257); array texcoords skip the texture
+ * matrix and array normals skip PF_NORMALIZE. */
+static void draw_array_begin(Ctx* c, PFdrawmode mode)
+{
+    uint32_t n = mode_vertex_count(mode);
+    for (uint32_t i = 0; i < n && i < 6; i++) {
+        memset(&c->vb[i], 0, sizeof c->vb[i]);
+        c->vb[i].color = c->curColor;
+        c->vb[i].vid = PFF_NO_VERTEX;
+    }
+    pfBegin(mode);
+}
+
+static void draw_array_vertex(Ctx* c, size_t j)
+{
+    const int useUV = (c->state & PF_TEXTURE_COORD_ARRAY) && c->aUV.buffer;
+    const int useN = (c->state & PF_NORMAL_ARRAY) && c->aNrm.buffer;
+    const int useC = (c->state & PF_COLOR_ARRAY) && c->aCol.buffer;
+    Vert* d = &c->vb[c->vcount++];
+    d->position[0] = d->position[1] = d->position[2] = 0.0f;
+    d->position[3] = 1.0f;
+    for (int k = 0; k < c->aPos.size; k++) d->position[k] = attr_f(&c->aPos, j, k);
+    if (useN) for (int k = 0; k < 3; k++) d->normal[k] = attr_f(&c->aNrm, j, k);
+    if (useUV) for (int k = 0; k < 2; k++) d->texcoord[k] = attr_f(&c->aUV, j, k);
+    if (useC) {
+        PFubyte ch8[4] = { 255, 255, 255, 255 };
+        for (int k = 0; k < c->aCol.size; k++) ch8[k] = attr_u8(&c->aCol, j, k);
+        d->color = (PFcolor){ ch8[0], ch8[1], ch8[2], ch8[3] };
+        d->vid = j <= 0xFFFFFFFEu ? (uint32_t)j : PFF_NO_VERTEX;
+    }
+    if (c->vcount == mode_vertex_count(c->drawMode)) {
+        process_primitive(c);
+        reset_vertex_buffer(c);
+    }
+}
+
+static int draw_array_ready(Ctx* c, PFdrawmode mode)
+{
+    if (!(c->state & PF_VERTEX_ARRAY)) { c->err = PF_INVALID_OPERATION; return 0; }
+    if (mode > PF_QUAD_STRIP) { c->err = PF_INVALID_ENUM; return 0; }
+    /* The reference reads a missing position array through NULL. */
+    if (!c->aPos.buffer) { c->err = PF_INVALID_OPERATION; return 0; }
+    return 1;
+}
+
+void pfDrawElements(PFdrawmode mode, PFsizei count, PFdatatype type, const void* indices)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!(type == PF_UNSIGNED_BYTE || type == PF_UNSIGNED_SHORT || type == PF_UNSIGNED_INT)) {
+        c->err = PF_INVALID_ENUM;
+        return;
+    }
+    if (!draw_array_ready(c, mode)) return;
+    draw_array_begin(c, mode);
+    for (PFsizei i = 0; i < count; i++) {
+        size_t j = type == PF_UNSIGNED_BYTE ? ((const PFubyte*)indices)[i]
+                 : type == PF_UNSIGNED_SHORT ? ((const PFushort*)indices)[i]
+                 : ((const PFuint*)indices)[i];
+        draw_array_vertex(c, j);
+    }
+    pfEnd();
+}
+
+void pfDrawArrays(PFdrawmode mode, PFint first, PFsizei count)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!draw_array_ready(c, mode)) return;
+    draw_array_begin(c, mode);
+    for (PFsizei i = 0; i < count; i++) draw_array_vertex(c, (size_t)((int64_t)first + i));
+    pfEnd();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Getters (getter.c)                                                        */
+/* ------------------------------------------------------------------------ */
+
+void pfGetBooleanv(PFenum pname, PFboolean* params)          /* getter.c:24-83 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_TEXTURE_2D: case PF_FRAMEBUFFER: case PF_BLEND: case PF_DEPTH_TEST:
+    case PF_CULL_FACE: case PF_NORMALIZE: case PF_LIGHTING: case PF_COLOR_MATERIAL:
+    case PF_VERTEX_ARRAY: case PF_NORMAL_ARRAY: case PF_COLOR_ARRAY: case PF_TEXTURE_COORD_ARRAY:
+        *params = (c->state & pname) != 0;
+        break;
+    default:
+        c->err = PF_INVALID_ENUM;
+        break;
+    }
+}
+
+void pfGetIntegerv(PFenum pname, PFint* params)              /* getter.c:85-190 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_VIEWPORT:
+        params[0] = c->vpPos[0]; params[1] = c->vpPos[1];
+        params[2] = (PFint)(c->vpDim[0] + 1); params[3] = (PFint)(c->vpDim[1] + 1);
+        break;
+    case PF_COLOR_CLEAR_VALUE:
+        params[0] = c->clearColor.r; params[1] = c->clearColor.g;
+        params[2] = c->clearColor.b; params[3] = c->clearColor.a;
+        break;
+    case PF_CULL_FACE_MODE: *params = (PFint)c->cullFace; break;
+    case PF_CURRENT_COLOR:
+        params[0] = c->curColor.r; params[1] = c->curColor.g;
+        params[2] = c->curColor.b; params[3] = c->curColor.a;
+        break;
+    case PF_CURRENT_RASTER_POSITION:
+        params[0] = (PFint)c->rasterPos[0]; params[1] = (PFint)c->rasterPos[1];
+        break;
+    case PF_POLYGON_MODE: params[0] = (PFint)c->polyMode[0]; params[1] = (PFint)c->polyMode[1]; break;
+    case PF_MATRIX_MODE: *params = (PFint)c->matMode; break;
+    case PF_MAX_PROJECTION_STACK_DEPTH: *params = MAX_PROJ_STACK; break;
+    case PF_MAX_MODELVIEW_STACK_DEPTH: *params = MAX_MV_STACK; break;
+    case PF_MAX_TEXTURE_STACK_DEPTH: *params = MAX_TEX_STACK; break;
+    case PF_SHADE_MODEL: *params = (PFint)c->shadeModel; break;
+    case PF_MAX_LIGHTS: *params = MAX_LIGHTS; break;
+    case PF_VERTEX_ARRAY_SIZE: *params = c->aPos.size; break;
+    case PF_VERTEX_ARRAY_STRIDE: *params = (PFint)c->aPos.stride; break;
+    case PF_VERTEX_ARRAY_TYPE: *params = (PFint)c->aPos.type; break;
+    case PF_NORMAL_ARRAY_STRIDE: *params = (PFint)c->aNrm.stride; break;
+    case PF_NORMAL_ARRAY_TYPE: *params = (PFint)c->aNrm.type; break;
+    case PF_TEXTURE_COORD_ARRAY_STRIDE: *params = (PFint)c->aUV.stride; break;
+    case PF_TEXTURE_COORD_ARRAY_TYPE: *params = (PFint)c->aUV.type; break;
+    case PF_COLOR_ARRAY_SIZE: *params = c->aCol.size; break;
+    case PF_COLOR_ARRAY_STRIDE: *params = (PFint)c->aCol.stride; break;
+    case PF_COLOR_ARRAY_TYPE: *params = (PFint)c->aCol.type; break;
+    default:
+        c->err = PF_INVALID_ENUM;
+        break;
+    }
+}
+
+#define PF_INV_255_D (1.0 / 255.0)                           /* PF_INV_255, context.h */
+
+static void modelview(const Ctx* c, float* out) { m4_mul(out, c->matModel, c->matView); }
+
+void pfGetFloatv(PFenum pname, PFfloat* params)              /* getter.c:192-352 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    const float inv = (float)PF_INV_255_D;
+    switch (pname) {
+    case PF_COLOR_CLEAR_VALUE:
+        params[0] = c->clearColor.r * inv; params[1] = c->clearColor.g * inv;
+        params[2] = c->clearColor.b * inv; params[3] = c->clearColor.a * inv;
+        break;
+    case PF_DEPTH_CLEAR_VALUE: *params = c->clearDepth; break;
+    case PF_CURRENT_COLOR:
+        params[0] = c->curColor.r * inv; params[1] = c->curColor.g * inv;
+        params[2] = c->curColor.b * inv; params[3] = c->curColor.a * inv;
+        break;
+    case PF_CURRENT_NORMAL: memcpy(params, c->curNormal, 12); break;
+    case PF_CURRENT_TEXTURE_COORDS: memcpy(params, c->curUV, 8); break;
+    case PF_CURRENT_RASTER_POSITION: params[0] = c->rasterPos[0]; params[1] = c->rasterPos[1]; break;
+    case PF_POINT_SIZE: *params = c->pointSize; break;
+    case PF_LINE_WIDTH: *params = c->lineWidth; break;
+    case PF_PROJECTION_MATRIX: memcpy(params, c->matProj, 64); break;
+    case PF_MODELVIEW_MATRIX: modelview(c, params); break;
+    case PF_TEXTURE_MATRIX: memcpy(params, c->matTex, 64); break;
+    case PF_ZOOM_X: *params = c->pixelZoom[0]; break;
+    case PF_ZOOM_Y: *params = c->pixelZoom[1]; break;
+    default:
+        c->err = PF_INVALID_ENUM;
+        break;
+    }
+}
+
+void pfGetDoublev(PFenum pname, PFdouble* params)            /* getter.c:354-476 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    float m[16];
+    switch (pname) {
+    case PF_COLOR_CLEAR_VALUE:
+        params[0] = c->clearColor.r * PF_INV_255_D; params[1] = c->clearColor.g * PF_INV_255_D;
+        params[2] = c->clearColor.b * PF_INV_255_D; params[3] = c->clearColor.a * PF_INV_255_D;
+        break;
+    case PF_DEPTH_CLEAR_VALUE: *params = c->clearDepth; break;
+    case PF_CURRENT_COLOR:
+        params[0] = c->curColor.r * PF_INV_255_D; params[1] = c->curColor.g * PF_INV_255_D;
+        params[2] = c->curColor.b * PF_INV_255_D; params[3] = c->curColor.a * PF_INV_255_D;
+        break;
+    case PF_CURRENT_NORMAL: for (int i = 0; i < 3; i++) params[i] = c->curNormal[i]; break;
+    case PF_CURRENT_TEXTURE_COORDS: params[0] = c->curUV[0]; params[1] = c->curUV[1]; break;
+    case PF_CURRENT_RASTER_POSITION: params[0] = c->rasterPos[0]; params[1] = c->rasterPos[1]; break;
+    case PF_POINT_SIZE: *params = c->pointSize; break;
+    case PF_LINE_WIDTH: *params = c->lineWidth; break;
+    case PF_PROJECTION_MATRIX: for (int i = 0; i < 16; i++) params[i] = c->matProj[i]; break;
+    case PF_MODELVIEW_MATRIX: modelview(c, m); for (int i = 0; i < 16; i++) params[i] = m[i]; break;
+    case PF_TEXTURE_MATRIX: for (int i = 0; i < 16; i++) params[i] = c->matTex[i]; break;
+    case PF_ZOOM_X: *params = c->pixelZoom[0]; break;
+    case PF_ZOOM_Y: *params = c->pixelZoom[1]; break;
+    default:
+        c->err = PF_INVALID_ENUM;
+        break;
+    }
+}
+
+void pfGetPointerv(PFenum pname, const void** params)        /* getter.c:478-521 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_TEXTURE_2D: *params = c->curTex; break;
+    case PF_FRAMEBUFFER: *params = c->boundFB; break;
+    case PF_BLEND_FUNC: *params = &c->blendMode; break;
+    case PF_DEPTH_FUNC: *params = &c->depthFunc; break;
+    default:
+        c->err = PF_INVALID_ENUM;
+        break;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Outside the GPU triangle path                                             */
+/* ------------------------------------------------------------------------ */
+
+void pfLineWidth(PFfloat width)                              /* context.c:611-618 */
+{
+    if (!G) return;
+    if (width <= 0.0f) { G->err = PF_INVALID_VALUE; return; }
+    G->lineWidth = width;
+}
+
+void pfPointSize(PFfloat size)                               /* context.c:620-627 */
+{
+    if (!G) return;
+    if (size <= 0.0f) { G->err = PF_INVALID_VALUE; return; }
+    G->pointSize = size;
+}
+
+void pfPixelZoom(PFfloat xfactor, PFfloat yfactor)           /* context.c:2086-2090 */
+{
+    if (!G) return;
+    G->pixelZoom[0] = xfactor;
+    G->pixelZoom[1] = yfactor;
+}
+
+static void raster_pos(float x, float y, float z, float w)   /* context.c:2091-2155 */
+{
+    if (!G) return;
+    G->rasterPos[0] = x; G->rasterPos[1] = y; G->rasterPos[2] = z; G->rasterPos[3] = w;
+}
+void pfRasterPos2i(PFint x, PFint y) { raster_pos((float)x, (float)y, 0.0f, 1.0f); }
+void pfRasterPos2f(PFfloat x, PFfloat y) { raster_pos(x, y, 0.0f, 1.0f); }
+void pfRasterPos2fv(const PFfloat* v) { raster_pos(v[0], v[1], 0.0f, 1.0f); }
+void pfRasterPos3i(PFint x, PFint y, PFint z) { raster_pos((float)x, (float)y, (float)z, 1.0f); }
+void pfRasterPos3f(PFfloat x, PFfloat y, PFfloat z) { raster_pos(x, y, z, 1.0f); }
+void pfRasterPos3fv(const PFfloat* v) { raster_pos(v[0], v[1], v[2], 1.0f); }
+void pfRasterPos4i(PFint x, PFint y, PFint z, PFint w) { raster_pos((float)x, (float)y, (float)z, (float)w); }
+void pfRasterPos4f(PFfloat x, PFfloat y, PFfloat z, PFfloat w) { raster_pos(x, y, z, w); }
+void pfRasterPos4fv(const PFfloat* v) { raster_pos(v[0], v[1], v[2], v[3]); }
+
+static void not_in_path(void) { if (G) G->err = PF_INVALID_OPERATION; }
+void pfRects(PFshort x1, PFshort y1, PFshort x2, PFshort y2) { (void)x1; (void)y1; (void)x2; (void)y2; not_in_path(); }
+void pfRectsv(const PFshort* v1, const PFshort* v2) { (void)v1; (void)v2; not_in_path(); }
+void pfRectf(PFfloat x1, PFfloat y1, PFfloat x2, PFfloat y2) { (void)x1; (void)y1; (void)x2; (void)y2; not_in_path(); }
+void pfRectfv(const PFfloat* v1, const PFfloat* v2) { (void)v1; (void)v2; not_in_path(); }
+void pfDrawPixels(PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type, const void* pixels)
+{ (void)width; (void)height; (void)format; (void)type; (void)pixels; not_in_path(); }
+void pfFogi(PFfogparam pname, PFint param) { (void)pname; (void)param; not_in_path(); }
+void pfFogf(PFfogparam pname, PFfloat param) { (void)pname; (void)param; not_in_path(); }
+void pfFogiv(PFfogparam pname, PFint* param) { (void)pname; (void)param; not_in_path(); }
+void pfFogfv(PFfogparam pname, PFfloat* param) { (void)pname; (void)param; not_in_path(); }
+void pfFogProcess(void) { not_in_path(); }
+void pfPostProcess(PFpostprocessfunc fn) { (void)fn; not_in_path(); }
+PFrenderlist pfGenList(void) { not_in_path(); return NULL; }
+void pfDeleteList(PFrenderlist* list) { (void)list; not_in_path(); }
+void pfNewList(PFrenderlist list) { (void)list; not_in_path(); }
+void pfEndList(void) { not_in_path(); }
+void pfCallList(const PFrenderlist list) { (void)list; not_in_path(); }
 
 /* ------------------------------------------------------------------------ */
 /* Readback and framebuffers                                                 */
@@ -1371,6 +1747,47 @@ PFfloat pfGetFramebufferDepth(const PFframebuffer* fb, PFsizei x, PFsizei y)
     const Tex* t = fb_tex(fb);
     finish_target(t->pixels);
     return fb->zbuffer[(size_t)y * t->w + x];
+}
+
+/* framebuffer.c:115-148.  The scalar depth test: NOTEQUAL is a real !=
+ * here, unlike the triangle path (depth.h:35-71). */
+void pfSetFramebufferPixelDepthTest(PFframebuffer* fb, PFsizei x, PFsizei y, PFfloat z, PFcolor color,
+                                    PFdepthmode depthMode)
+{
+    if (depthMode > PF_GEQUAL) { set_err(PF_INVALID_ENUM); return; }
+    Tex* t = fb_tex(fb);
+    finish_target(t->pixels);
+    size_t off = (size_t)y * t->w + x;
+    float d = fb->zbuffer[off];
+    int pass = 0;
+    switch (depthMode) {
+    case PF_EQUAL: pass = z == d; break;
+    case PF_NOTEQUAL: pass = z != d; break;
+    case PF_LESS: pass = z < d; break;
+    case PF_LEQUAL: pass = z <= d; break;
+    case PF_GREATER: pass = z > d; break;
+    case PF_GEQUAL: pass = z >= d; break;
+    }
+    if (pass) {
+        ((uint32_t*)t->pixels)[off] = pack(color);
+        fb->zbuffer[off] = z;
+    }
+}
+
+void pfSetFramebufferPixelDepth(PFframebuffer* fb, PFsizei x, PFsizei y, PFfloat z, PFcolor color)
+{
+    Tex* t = fb_tex(fb);
+    finish_target(t->pixels);
+    size_t off = (size_t)y * t->w + x;
+    ((uint32_t*)t->pixels)[off] = pack(color);
+    fb->zbuffer[off] = z;
+}
+
+void pfSetFramebufferPixel(PFframebuffer* fb, PFsizei x, PFsizei y, PFcolor color)
+{
+    Tex* t = fb_tex(fb);
+    finish_target(t->pixels);
+    ((uint32_t*)t->pixels)[(size_t)y * t->w + x] = pack(color);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -19,6 +19,10 @@
  *   PFS_RANDOM_3D   random triangles in a perspective frustum, including
  *                   vertices behind the camera and outside the side planes,
  *                   so that clipping and both culling passes run.
+ *   PFS_MESH        indexed grid mesh facing a perspective camera, with
+ *                   per-vertex normals, texture coordinates and float
+ *                   colours, drawn through pfDrawElements, pfDrawArrays or
+ *                   the equivalent immediate-mode calls.
  */
 #include <dlfcn.h>
 #include <math.h>
@@ -32,6 +36,7 @@
 #define PFS_HEIGHTFIELD  2
 #define PFS_SPHERES      3
 #define PFS_RANDOM_3D    4
+#define PFS_MESH         5
 
 /* Scene description; zero fields take the defaults noted. */
 typedef struct {
@@ -61,6 +66,11 @@ typedef struct {
     float    triScale;       /* random kinds: triangle size relative to the screen   */
     int32_t  flushEvery;     /* >0: pfFinish every N triangles (multi-frame)         */
     int32_t  useQuads;       /* 1: PF_QUADS for grids                                */
+    int32_t  arrays;         /* mesh: 0 immediate mode, 1 pfDrawElements, 2 pfDrawArrays */
+    int32_t  indexType;      /* mesh, pfDrawElements: 0 uint, 1 ubyte, 2 ushort      */
+    int32_t  posType;        /* mesh arrays: 0 float, 1 double positions             */
+    int32_t  colType;        /* mesh arrays: 0 float, 1 ubyte, 2 ushort, 3 uint, 4 double */
+    int32_t  colSize;        /* mesh arrays: 3 or 4 colour components (0: 4)         */
 } PFSparams;
 
 typedef struct {
@@ -103,6 +113,13 @@ typedef struct {
     void (*TextureParameter)(PFtexture, PFtexturewrap, PFtexturefilter);
     void* (*GetTexturePixels)(const PFtexture, PFsizei*, PFsizei*, PFpixelformat*, PFdatatype*);
     PFerrcode (*GetError)(void);
+    void (*Color4f)(PFfloat, PFfloat, PFfloat, PFfloat);
+    void (*VertexPointer)(PFint, PFenum, PFsizei, const void*);
+    void (*NormalPointer)(PFenum, PFsizei, const void*);
+    void (*TexCoordPointer)(PFenum, PFsizei, const void*);
+    void (*ColorPointer)(PFint, PFenum, PFsizei, const void*);
+    void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
+    void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
     void (*Finish)(void);                    /* absent from the reference */
     void (*SubmitBatch)(const PFcontext*, PFsizei);   /* absent from the reference */
 } PFSapi;
@@ -123,7 +140,9 @@ PFSapi* pfs_bind(const char* path)
     BIND(Lightf); BIND(Lightfv); BIND(Materialf); BIND(Materialfv); BIND(ColorMaterial);
     BIND(Begin); BIND(End); BIND(Vertex3f); BIND(Color4ub); BIND(TexCoord2f); BIND(Normal3f);
     BIND(GenFramebuffer); BIND(DeleteFramebuffer); BIND(GenTexture); BIND(DeleteTexture);
-    BIND(TextureParameter); BIND(GetTexturePixels); BIND(GetError);
+    BIND(TextureParameter); BIND(GetTexturePixels); BIND(GetError); BIND(Color4f);
+    BIND(VertexPointer); BIND(NormalPointer); BIND(TexCoordPointer); BIND(ColorPointer);
+    BIND(DrawElements); BIND(DrawArrays);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
     return a;
@@ -361,6 +380,138 @@ static void scene_heightfield(const PFSapi* a, Rng* r, const PFSparams* p)
     free(P); free(C); free(blended);
 }
 
+/* Indexed grid mesh (gridX x gridZ cells, two triangles each) spanning the
+ * view of a perspective camera: a gently bumped sheet at z ~ -2.4 with
+ * analytic normals, texture coordinates [0,1]^2 across the sheet and random
+ * float colours in [0.2, 1).  The same vertices go through
+ * pfDrawElements (arrays = 1), pfDrawArrays over the expanded corners,
+ * offset by `first` (arrays = 2), or pfColor4f / pfTexCoord2f /
+ * pfNormal3f / pfVertex3f (arrays = 0).  Values are chosen so that every
+ * array type represents them exactly. */
+static void store_f(void* arr, int type, size_t i, double v)
+{
+    if (type == PF_DOUBLE) ((double*)arr)[i] = v; else ((float*)arr)[i] = (float)v;
+}
+
+static void store_col(void* arr, int type, size_t i, float v)
+{
+    switch (type) {
+    case PF_UNSIGNED_BYTE:  ((uint8_t*)arr)[i] = (uint8_t)(v * 255.0f); break;
+    case PF_UNSIGNED_SHORT: ((uint16_t*)arr)[i] = (uint16_t)((uint32_t)(v * 255.0f) << 8); break;
+    case PF_UNSIGNED_INT:   ((uint32_t*)arr)[i] = (uint32_t)(v * 255.0f) << 24; break;
+    case PF_DOUBLE:         ((double*)arr)[i] = (double)v; break;
+    default:                ((float*)arr)[i] = v; break;
+    }
+}
+
+static size_t type_size(int type)
+{
+    switch (type) {
+    case PF_UNSIGNED_BYTE: return 1;
+    case PF_UNSIGNED_SHORT: return 2;
+    case PF_DOUBLE: return 8;
+    default: return 4;
+    }
+}
+
+static void scene_mesh(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    int NX = p->gridX > 0 ? p->gridX : 100, NZ = p->gridZ > 0 ? p->gridZ : 100;
+    perspective(a, p, 0.5f, 0.5f, 10.0f);
+    a->Rotatef(-12.0f, 1.0f, 0.0f, 0.0f);
+    float aspect = (float)p->width / (float)p->height;
+    size_t nv = (size_t)(NX + 1) * (NZ + 1);
+    static const int colTypes[5] = { PF_FLOAT, PF_UNSIGNED_BYTE, PF_UNSIGNED_SHORT, PF_UNSIGNED_INT, PF_DOUBLE };
+    static const int idxTypes[3] = { PF_UNSIGNED_INT, PF_UNSIGNED_BYTE, PF_UNSIGNED_SHORT };
+    int posType = p->posType == 1 ? PF_DOUBLE : PF_FLOAT;
+    int colType = colTypes[(p->colType >= 0 && p->colType < 5) ? p->colType : 0];
+    int colSize = p->colSize ? p->colSize : 4;
+    float* P = (float*)malloc(nv * 3 * sizeof(float));
+    float* N = (float*)malloc(nv * 3 * sizeof(float));
+    float* T = (float*)malloc(nv * 2 * sizeof(float));
+    float* Cf = (float*)malloc(nv * 4 * sizeof(float));
+    float ph0 = rrange(r, 0, 6.28f), ph1 = rrange(r, 0, 6.28f);
+    for (int j = 0; j <= NZ; j++)
+        for (int i = 0; i <= NX; i++) {
+            size_t v = (size_t)j * (NX + 1) + i;
+            float x = (2.0f * i / NX - 1.0f) * 1.25f * aspect, y = (2.0f * j / NZ - 1.0f) * 1.25f;
+            float h = 0.12f * sinf(2.1f * x + ph0) * cosf(1.7f * y + ph1);
+            float dx = 0.12f * 2.1f * cosf(2.1f * x + ph0) * cosf(1.7f * y + ph1);
+            float dy = -0.12f * 1.7f * sinf(2.1f * x + ph0) * sinf(1.7f * y + ph1);
+            P[3 * v] = x; P[3 * v + 1] = y; P[3 * v + 2] = -2.4f + h;
+            float nl = sqrtf(dx * dx + dy * dy + 1.0f);
+            N[3 * v] = -dx / nl; N[3 * v + 1] = -dy / nl; N[3 * v + 2] = 1.0f / nl;
+            T[2 * v] = (float)i / NX; T[2 * v + 1] = (float)j / NZ;
+            for (int k = 0; k < 4; k++) {
+                float c = (k == 3) ? 1.0f : 0.2f + 0.8f * runit(r);
+                /* Multiples of 1/256: c*255 is exact in float and double, so
+                 * every colour array type truncates to the same byte. */
+                Cf[4 * v + k] = (float)((int)(c * 256.0f)) / 256.0f;
+            }
+        }
+    size_t ni = (size_t)NX * NZ * 6;
+    uint32_t* I = (uint32_t*)malloc(ni * sizeof(uint32_t));
+    size_t q = 0;
+    for (int j = 0; j < NZ; j++)
+        for (int i = 0; i < NX; i++) {
+            uint32_t v00 = (uint32_t)(j * (NX + 1) + i), v10 = v00 + 1;
+            uint32_t v01 = v00 + (uint32_t)(NX + 1), v11 = v01 + 1;
+            I[q++] = v00; I[q++] = v10; I[q++] = v11;
+            I[q++] = v00; I[q++] = v11; I[q++] = v01;
+        }
+    if (p->arrays == 0) {
+        a->Begin(PF_TRIANGLES);
+        for (size_t k = 0; k < ni; k++) {
+            uint32_t v = I[k];
+            const float* c = &Cf[4 * v];
+            a->Color4f(c[0], c[1], c[2], colSize == 4 ? c[3] : 1.0f);
+            a->TexCoord2f(T[2 * v], T[2 * v + 1]);
+            a->Normal3f(N[3 * v], N[3 * v + 1], N[3 * v + 2]);
+            a->Vertex3f(P[3 * v], P[3 * v + 1], P[3 * v + 2]);
+            maybe_flush(a, p, (uint32_t)(k + 1) / 3);
+        }
+        a->End();
+    } else {
+        int expand = p->arrays == 2;
+        size_t first = expand ? 5 : 0;                  /* DrawArrays from a non-zero first */
+        size_t n = expand ? first + ni : nv;
+        void* pa = calloc(n * 3, type_size(posType));
+        void* na = calloc(n * 3, sizeof(float));
+        void* ta = calloc(n * 2, sizeof(float));
+        void* ca = calloc(n * (size_t)colSize, type_size(colType));
+        for (size_t e = 0; e < n; e++) {
+            size_t v = expand ? (e >= first ? I[e - first] : 0) : e;
+            for (int k = 0; k < 3; k++) store_f(pa, posType, 3 * e + k, P[3 * v + k]);
+            for (int k = 0; k < 3; k++) ((float*)na)[3 * e + k] = N[3 * v + k];
+            for (int k = 0; k < 2; k++) ((float*)ta)[2 * e + k] = T[2 * v + k];
+            for (int k = 0; k < colSize; k++) store_col(ca, colType, (size_t)colSize * e + k, Cf[4 * v + k]);
+        }
+        a->Enable(PF_VERTEX_ARRAY | PF_NORMAL_ARRAY | PF_TEXTURE_COORD_ARRAY | PF_COLOR_ARRAY);
+        a->VertexPointer(3, (PFenum)posType, 0, pa);
+        a->NormalPointer(PF_FLOAT, 0, na);
+        a->TexCoordPointer(PF_FLOAT, 0, ta);
+        a->ColorPointer(colSize, (PFenum)colType, 0, ca);
+        if (expand) {
+            a->DrawArrays(PF_TRIANGLES, (PFint)first, (PFsizei)ni);
+        } else {
+            int it = idxTypes[(p->indexType >= 0 && p->indexType < 3) ? p->indexType : 0];
+            void* ix = malloc(ni * type_size(it));
+            for (size_t k = 0; k < ni; k++) {
+                if (it == PF_UNSIGNED_BYTE) ((uint8_t*)ix)[k] = (uint8_t)I[k];
+                else if (it == PF_UNSIGNED_SHORT) ((uint16_t*)ix)[k] = (uint16_t)I[k];
+                else ((uint32_t*)ix)[k] = I[k];
+            }
+            a->DrawElements(PF_TRIANGLES, (PFsizei)ni, (PFdatatype)it, ix);
+            free(ix);
+        }
+        /* The reference reads the arrays inside the draw call; this library
+         * copies them into the recorded frame, so both may be freed here. */
+        a->Disable(PF_VERTEX_ARRAY | PF_NORMAL_ARRAY | PF_TEXTURE_COORD_ARRAY | PF_COLOR_ARRAY);
+        free(pa); free(na); free(ta); free(ca);
+    }
+    free(P); free(N); free(T); free(Cf); free(I);
+}
+
 /* Cluster of tessellated spheres with per-vertex normals. */
 static void scene_spheres(const PFSapi* a, Rng* r, const PFSparams* p)
 {
@@ -414,6 +565,10 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     case PFS_RANDOM_3D:
         setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
         scene_random_3d(a, r, p);
+        break;
+    case PFS_MESH:
+        setup_lights(a, r, p, 0.0f, 0.0f, -1.2f, 1.0f);
+        scene_mesh(a, r, p);
         break;
     default: break;
     }

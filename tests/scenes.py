@@ -85,7 +85,26 @@ BILINEAR_SCENES = [
                                      depthFunc=pf.PF_LEQUAL, blendFraction=0.3, cull=-1)),
 ]
 
-GOLDEN_SCENES = CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES
+# Vertex arrays (pfDrawElements / pfDrawArrays) against the same mesh drawn
+# in immediate mode; every index, position and colour type.
+_MESH = dict(kind=pf.MESH, width=W, height=H, gridX=12, gridZ=10, texture=1, texW=33, texH=33,
+             filter=pf.PF_NEAREST, lighting=2, numLights=2, cull=-1)
+ARRAY_SCENES = [
+    ("mesh_immediate", dict(_MESH, seed=80, arrays=0)),
+    ("mesh_elements_uint", dict(_MESH, seed=80, arrays=1)),
+    ("mesh_elements_ubyte", dict(_MESH, seed=80, arrays=1, indexType=1)),
+    ("mesh_elements_ushort", dict(_MESH, seed=80, arrays=1, indexType=2)),
+    ("mesh_draw_arrays", dict(_MESH, seed=80, arrays=2)),
+    ("mesh_double_positions", dict(_MESH, seed=80, arrays=1, posType=1)),
+    ("mesh_colors_ubyte", dict(_MESH, seed=81, arrays=1, colType=1)),
+    ("mesh_colors_ushort", dict(_MESH, seed=81, arrays=1, colType=2)),
+    ("mesh_colors_uint", dict(_MESH, seed=81, arrays=1, colType=3)),
+    ("mesh_colors_double", dict(_MESH, seed=81, arrays=1, colType=4)),
+    ("mesh_colors_rgb", dict(_MESH, seed=82, arrays=1, colSize=3)),
+    ("mesh_gouraud_cull", dict(_MESH, seed=83, arrays=1, lighting=1, cull=pf.PF_BACK)),
+]
+
+GOLDEN_SCENES = CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES
 
 

@@ -87,14 +87,17 @@ class FrameLog:
         f = C.cast(fp, C.POINTER(Frame)).contents
         cmds = [(f.cmds[i].kind, f.cmds[i].first, f.cmds[i].last) for i in range(f.numCmds)]
         info = [f.tris.info[i] for i in range(f.tris.count)]
+        vid = [f.tris.vertexId[i] for i in range(3 * f.tris.count)] if f.tris.vertexId else None
         self.frames.append(dict(w=f.width, h=f.height, ntri=f.tris.count, nstates=f.numStates,
-                                cmds=cmds, info=info, tex=bool(f.tris.texcoord), nrm=bool(f.tris.normal)))
+                                cmds=cmds, info=info, tex=bool(f.tris.texcoord), nrm=bool(f.tris.normal),
+                                vid=vid))
         return 0
 
 
 class Tris(C.Structure):
     _fields_ = [("count", C.c_uint32), ("position", C.c_void_p), ("color", C.c_void_p),
-                ("texcoord", C.c_void_p), ("normal", C.c_void_p), ("info", C.POINTER(C.c_uint32))]
+                ("texcoord", C.c_void_p), ("normal", C.c_void_p), ("info", C.POINTER(C.c_uint32)),
+                ("vertexId", C.POINTER(C.c_uint32))]
 
 
 class Cmd(C.Structure):
@@ -135,6 +138,7 @@ def test_recorded_frame_structure():
     both, front = 0x03000000, 0x01000000
     assert [i & 0xFF000000 for i in f["info"]] == [both, both, front, front]
     assert not f["tex"] and not f["nrm"]
+    assert f["vid"] is None                            # immediate mode: no colour-array indices
 
 
 def test_target_change_splits_frames():
