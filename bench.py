@@ -1,18 +1,29 @@
 """Benchmark of the fragment pipeline on one MI355X.
 
-Workload (one "step" = one full frame through the hot path):
-  1920x1080 RGBA8 + float32 depth, 200,000 triangles of a seeded jittered
-  height-field grid (1000 x 100 cells, two triangles each) seen through a
-  perspective camera; 1024x1024 RGBA8 uniform-noise texture, bilinear,
-  REPEAT; depth test LEQUAL; 30% of the triangles (seeded choice) drawn
-  after the opaque ones with alpha 128 and PF_BLEND_ALPHA; back faces
-  culled; the frame starts with pfClear.
+Three measurements, one JSON line (the headline is the first):
 
-The scene is recorded once through the PixelForge C API into a frame that
-stays resident on the GPU (inputs in HBM); each step replays the whole
-kernel chain (setup, binning, tile raster) as one HIP graph.  The metric is
-covered fragments (before the depth test, counted by the raster kernel)
-per second of wall time over the timed steps.
+1. Fragments/s at 1080p (`value`).  One step = one full frame:
+   1920x1080 RGBA8 + float32 depth, 200,000 triangles of a seeded jittered
+   height-field grid (1000 x 100 cells, two triangles each) seen through a
+   perspective camera; 1024x1024 RGBA8 uniform-noise texture, bilinear,
+   REPEAT; depth test LEQUAL; 30% of the triangles (seeded choice) drawn
+   after the opaque ones with alpha 128 and PF_BLEND_ALPHA; back faces
+   culled; the frame starts with pfClear.  Fragments are covered samples
+   (before the depth test) counted by the raster kernel.
+2. Batched frames/s (`batched`): 256 independent contexts of uneven size
+   (sides uniform in [64, 1024]), triangle counts log-uniform in
+   [100, 50000], RGBA8 textures with sides in [64, 512], every second
+   context additively blended; all contexts render in the same launches
+   (pfSubmitBatch), tile size autotuned over 8x8 .. 64x64.
+3. Differentiable (`differentiable`): 512x512, 20,000-triangle grid mesh,
+   256x256x4 float texels, per-vertex float colours and two lights'
+   diffuse/specular as parameters; L2 loss against a target rendered from
+   the texture perturbed by N(0, 0.1); device time of forward and adjoint,
+   and the gradients' relative error against the FP64 oracle.
+
+Scenes are recorded once through the PixelForge C API into jobs that stay
+resident on the GPU (inputs in HBM); each step replays the whole kernel
+chain (setup, binning, tile raster) as one HIP graph.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
 """
@@ -34,10 +45,12 @@ os.environ.setdefault("OMP_NUM_THREADS", str(min(_CPU_THREADS, 16)))
 import numpy as np  # noqa: E402
 
 import pixelforge_amd as pf  # noqa: E402
+from pixelforge_amd.diffmesh import Scene, params_of  # noqa: E402
 
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libpforacle.so")
 PEAK_HBM_GBS = 8000.0
-PF_RGBA, PF_UNSIGNED_BYTE = 7, 0
+SETUP_BYTES_PER_TRI = 88          # position 3x16 + colour 3x4 + texcoord 3x8 + info 4
 
 
 def workload(width=1920, height=1080, seed=2):
@@ -46,48 +59,234 @@ def workload(width=1920, height=1080, seed=2):
                     depthFunc=pf.PF_LEQUAL, blendFraction=0.3, cull=pf.PF_BACK)
 
 
-def record_job(params):
-    """Records the scene through the API and returns a resident GPU job."""
-    lib = pf.lib()
-    n = params.width * params.height
-    color = np.zeros(n + 16, np.uint32)
-    ctx = lib.pfCreateContext(color.ctypes.data, params.width, params.height, PF_RGBA, PF_UNSIGNED_BYTE)
-    if not ctx:
-        raise RuntimeError("pfCreateContext failed")
-    lib.pfMakeCurrent(ctx)
-    tex, texpx = C.c_void_p(), C.c_void_p()
-    err = pf.scenes().pfs_record(pf.api(), C.byref(params), C.byref(tex), C.byref(texpx))
-    if err:
-        raise RuntimeError(f"scene recording failed: {err}")
-    lib.pfxSetExecutor(C.cast(lib.pfdCaptureExecutor, C.c_void_p), None)
-    lib.pfFinish()
-    lib.pfxSetExecutor(None, None)
-    job = C.c_void_p(lib.pfdTakeCapturedJob())
-    if not job.value:
-        raise RuntimeError(f"frame capture failed: {lib.pfdLastError().decode()}")
-    return job, (ctx, color, tex, texpx)
-
-
 def check(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed ({rc}): {pf.lib().pfdLastError().decode()}")
 
 
-def cpu_baseline(params, frags, frames=5):
-    """The reference's own CPU path (oracle/_ref) on the same scene."""
-    if not os.path.exists(REF_SO):
-        return None
-    pf.render(params, REF_SO)                       # warm-up
+def configure(job, tile_shift, raster_mode):
+    lib = pf.lib()
+    cfg = pf.PFDconfig()
+    if tile_shift:
+        cfg.tileShift = tile_shift
+        cfg.rasterMode = raster_mode
+        check(lib.pfdJobSetConfig(job, C.byref(cfg)), "config")
+    else:
+        check(lib.pfdJobAutotune(job, C.byref(cfg)), "autotune")
+    return cfg
+
+
+def time_replays(job, steps, warmup):
+    lib = pf.lib()
+    for _ in range(max(warmup, 2)):
+        check(lib.pfdJobLaunch(job), "launch")
+    check(lib.pfdJobSync(job), "sync")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        check(lib.pfdJobLaunch(job), "launch")
+    check(lib.pfdJobSync(job), "sync")
+    return time.perf_counter() - t0
+
+
+def phase_times(job, runs):
+    lib = pf.lib()
+    raster, setup, binning = [], [], []
+    for _ in range(runs):
+        t = pf.PFDtimes()
+        check(lib.pfdJobRunTimed(job, C.byref(t)), "timed run")
+        raster.append(t.rasterMs)
+        setup.append(t.setupMs)
+        binning.append(t.binMs)
+    return statistics.mean(setup), statistics.mean(binning), statistics.mean(raster)
+
+
+def stats(job):
+    st = pf.PFDstats()
+    check(pf.lib().pfdJobStats(job, C.byref(st)), "stats")
+    return st
+
+
+def cpu_frame_rate(params_list, frames=3):
+    """The reference's own CPU path (oracle/_ref) on the given scenes:
+    median seconds per pass over all of them (API calls included)."""
+    for p in params_list:
+        pf.render(p, REF_SO)                        # warm-up
     times = []
     for _ in range(frames):
         t0 = time.perf_counter()
-        pf.render(params, REF_SO)
+        for p in params_list:
+            pf.render(p, REF_SO)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": frags / med, "unit": "fragments/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+    return statistics.median(times)
+
+
+# ---------------------------------------------------------------------------
+# 1. 1080p frame
+# ---------------------------------------------------------------------------
+
+def bench_frame(args):
+    params = workload()
+    t0 = time.perf_counter()
+    cap = pf.capture(params)
+    rec_s = time.perf_counter() - t0
+    job = cap.job
+    lib = pf.lib()
+    check(lib.pfdJobUploadTargets(job), "upload")
+    cfg = configure(job, args.tile_shift, args.raster_mode)
+    elapsed = time_replays(job, args.steps, args.warmup)
+    st = stats(job)
+    frags = int(st.fragments)
+    setup_ms, bin_ms, raster_ms = phase_times(job, args.profile_runs)
+    W, H = params.width, params.height
+    raster_bytes = W * H * 8 + 1024 * 1024 * 4      # colour + depth written once, texels read once
+    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
+    out = {
+        "value": frags * args.steps / elapsed,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "config": {"workload": "1920x1080 RGBA8+f32 depth, 200k-triangle jittered height-field grid, "
+                               "1024^2 RGBA8 noise texture bilinear REPEAT, LEQUAL, 30% alpha-blended",
+                   "fragments_per_frame": frags, "raster_triangles": int(st.rasterTriangles),
+                   "tile_pairs": int(st.tilePairs), "tile": 1 << int(st.tileShift),
+                   "max_tile_list": int(st.maxTileList), "busy_tiles": int(st.busyTiles),
+                   "raster_mode": ["packed", "in-place"][cfg.rasterMode], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_raster", "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
+        "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
+        "record_s": rec_s,
+    }
+    if not args.no_cpu and os.path.exists(REF_SO):
+        med = cpu_frame_rate([params], frames=5)
+        out["cpu_baseline"] = {
+            "value": frags / med, "unit": "fragments/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": "reference",
-            "sample": f"1 warm-up + median of {frames} full frames of the same scene (API calls incl. scene "
-                      f"generation), reference built -O3 -mavx2 -fopenmp; median {med * 1e3:.1f} ms/frame"}
+            "sample": f"1 warm-up + median of 5 full frames of the same scene (API calls incl. scene generation), "
+                      f"reference built -O3 -mavx2 -fopenmp; median {med * 1e3:.1f} ms/frame"}
+    else:
+        out["cpu_baseline"] = None
+    cap.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 2. batched contexts
+# ---------------------------------------------------------------------------
+
+def bench_batch(args):
+    params = pf.batch_workload(args.contexts)
+    cap = pf.capture_batch(params)
+    job = cap.job
+    lib = pf.lib()
+    check(lib.pfdJobUploadTargets(job), "upload")
+    cfg = configure(job, 0, 0)
+    elapsed = time_replays(job, args.batch_steps, 3)
+    st = stats(job)
+    setup_ms, bin_ms, raster_ms = phase_times(job, max(3, args.profile_runs // 4))
+    ms = elapsed * 1e3 / args.batch_steps
+    px = sum(p.width * p.height for p in params)
+    tex = sum(p.texW * p.texH * 4 for p in params)
+    tris = int(st.inputTriangles)
+    algo = px * 8 + tex + tris * SETUP_BYTES_PER_TRI
+    out = {
+        "value": len(params) * args.batch_steps / elapsed, "unit": "frames/s", "contexts": len(params),
+        "ms_per_batch": ms, "fragments": int(st.fragments), "input_triangles": tris,
+        "pixels": px, "tile": 1 << int(st.tileShift), "raster_mode": ["packed", "in-place"][cfg.rasterMode],
+        "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
+        "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": algo / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None, "scope": "whole batch step",
+                     "algorithmic_bytes": algo,
+                     "raster_kernel": {"achieved": (px * 8 + tex) / (raster_ms * 1e-3) / 1e9, "kernel_ms": raster_ms}},
+    }
+    if not args.no_cpu and os.path.exists(REF_SO):
+        sample = params[: args.cpu_contexts]
+        med = cpu_frame_rate(sample, frames=3)
+        out["cpu_baseline"] = {
+            "value": len(sample) / med, "unit": "frames/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "reference",
+            "sample": f"first {len(sample)} of the {len(params)} contexts, one after another (the reference has no "
+                      f"batched path), 1 warm-up + median of 3; {med * 1e3:.0f} ms per pass"}
+    else:
+        out["cpu_baseline"] = None
+    cap.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 3. differentiable path
+# ---------------------------------------------------------------------------
+
+class _OracleParams(C.Structure):        # PFOdiffParams of the FP64 oracle
+    _fields_ = [("texture", C.c_int32), ("texels", C.c_void_p), ("numVertices", C.c_uint32),
+                ("vertexColors", C.c_void_p), ("lights", C.c_void_p)]
+
+
+def _fp64_gradients(scene, p64, target):
+    """cpu_baseline leg: the FP64 oracle's forward + adjoint of the same
+    frame, timed, and its gradients (to measure the GPU's error)."""
+    o = C.CDLL(ORACLE_SO)
+    o.pforacle_diff_prepare.restype = C.c_void_p
+    o.pforacle_diff_prepare.argtypes = [C.c_void_p]
+    o.pforacle_diff_eval.argtypes = [C.c_void_p, C.POINTER(_OracleParams)] + [C.c_void_p] * 5
+    o.pforacle_diff_free.argtypes = [C.c_void_p]
+    handles = []
+    cb = pf.EXECUTOR(lambda fp, user: (handles.append(o.pforacle_diff_prepare(fp)), 0)[1])
+    scene.record()
+    L = pf.lib()
+    L.pfxSetExecutor(C.cast(cb, C.c_void_p), None)
+    try:
+        L.pfFinish()
+    finally:
+        L.pfxSetExecutor(None, None)
+    h = handles[0]
+    t, c, d, s = (np.ascontiguousarray(a, np.float64) for a in p64)
+    lights = np.concatenate([d.ravel(), s.ravel()])
+    prm = _OracleParams(0, t.ctypes.data, scene.nv, c.ctypes.data, lights.ctypes.data)
+    t0 = time.perf_counter()
+    img = np.zeros((scene.H, scene.W, 4))
+    o.pforacle_diff_eval(h, C.byref(prm), img.ctypes.data, None, None, None, None)
+    dimg = 2.0 * (img - target)
+    g = [np.zeros_like(t), np.zeros_like(c), np.zeros(48)]
+    o.pforacle_diff_eval(h, C.byref(prm), None, dimg.ctypes.data, g[0].ctypes.data, g[1].ctypes.data, g[2].ctypes.data)
+    dt = time.perf_counter() - t0
+    o.pforacle_diff_free(h)
+    return img, (g[0], g[1], g[2][:24].reshape(8, 3), g[2][24:].reshape(8, 3)), dt
+
+
+def bench_diff(args):
+    s = Scene(width=512, height=512, nx=100, nz=100, tex=(256, 256), lights=2, seed=5)
+    p64 = params_of(s)
+    p32 = [a.astype(np.float32) for a in p64]
+    rng = np.random.default_rng(99)
+    target_tex = (p64[0] + rng.normal(0.0, 0.1, p64[0].shape)).astype(np.float32)
+    s.record()
+    target = s.gpu_forward(target_tex, *p32[1:]).astype(np.float64)
+    img = s.gpu_forward(*p32)
+    grads = s.gpu_backward(*p32, (2.0 * (img - target)).astype(np.float32))
+    raster_ms, fwd_ms, bwd_ms = s.gpu_time(*p32, iters=args.diff_iters)
+    out = {
+        "config": {"workload": "512x512, 20k-triangle grid mesh (pfDrawElements), 256x256x4 f32 texels, "
+                               "10201 RGBA f32 vertex colours, 2 Phong lights' diffuse+specular; L2 loss vs a "
+                               "target from the texture perturbed by N(0, 0.1)"},
+        "integer_pass_ms": raster_ms, "forward_ms": fwd_ms, "adjoint_ms": bwd_ms,
+        "fwd_bwd_ms": raster_ms + fwd_ms + bwd_ms,
+        "fwd_bwd_over_fwd": (raster_ms + fwd_ms + bwd_ms) / (raster_ms + fwd_ms),
+        "loss": float(((img - target) ** 2).sum()),
+    }
+    s.close()
+    if not args.no_cpu and os.path.exists(ORACLE_SO):
+        s2 = Scene(width=512, height=512, nx=100, nz=100, tex=(256, 256), lights=2, seed=5)
+        img64, g64, dt = _fp64_gradients(s2, p64, target)
+        s2.close()
+        rel = {}
+        for name, a, b in zip(("texels", "vertex_colors", "light_diffuse", "light_specular"), grads, g64):
+            n = np.linalg.norm(b)
+            rel[name] = float(np.linalg.norm(np.asarray(a, np.float64) - b) / n) if n else 0.0
+        out["grad_rel_err_vs_fp64"] = rel
+        out["image_max_abs_err_vs_fp64"] = float(np.abs(img - img64).max())
+        out["cpu_baseline"] = {"value": dt * 1e3, "unit": "ms (forward + adjoint, FP64)", "cores": 1, "kind": "port",
+                               "sample": "the FP64 oracle's float-domain forward and adjoint of the same frame "
+                                         "(coverage from its integer pass, not timed)"}
+    return out
 
 
 def main():
@@ -99,90 +298,46 @@ def main():
     ap.add_argument("--tile-shift", type=int, default=0, help="3..6 fixes the tile size; 0 autotunes")
     ap.add_argument("--raster-mode", type=int, default=0, help="with --tile-shift: 0 packed, 1 in-place")
     ap.add_argument("--profile-runs", type=int, default=20)
+    ap.add_argument("--contexts", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--cpu-contexts", type=int, default=16)
+    ap.add_argument("--diff-iters", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    lib = pf.lib()
-    check(lib.pfdInit(), "pfdInit")
-    params = workload()
-    t0 = time.perf_counter()
-    job, keep = record_job(params)
-    rec_s = time.perf_counter() - t0
-    check(lib.pfdJobUploadTargets(job), "upload")
-
-    cfg = pf.PFDconfig()
-    if args.tile_shift:
-        cfg.tileShift = args.tile_shift
-        cfg.rasterMode = args.raster_mode
-        check(lib.pfdJobSetConfig(job, C.byref(cfg)), "config")
-    else:
-        check(lib.pfdJobAutotune(job, C.byref(cfg)), "autotune")
-
-    for _ in range(max(args.warmup, 2)):
-        check(lib.pfdJobLaunch(job), "launch")
-    check(lib.pfdJobSync(job), "sync")
-
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        check(lib.pfdJobLaunch(job), "launch")
-    check(lib.pfdJobSync(job), "sync")
-    elapsed = time.perf_counter() - t0
-
-    st = pf.PFDstats()
-    check(lib.pfdJobStats(job, C.byref(st)), "stats")
-    frags = int(st.fragments)
-    value = frags * args.steps / elapsed
-    ms_per_step = elapsed * 1e3 / args.steps
-
-    # Per-kernel durations (HIP events on the job's stream).
-    raster, setup, binning = [], [], []
-    for _ in range(args.profile_runs):
-        t = pf.PFDtimes()
-        check(lib.pfdJobRunTimed(job, C.byref(t)), "timed run")
-        raster.append(t.rasterMs)
-        setup.append(t.setupMs)
-        binning.append(t.binMs)
-    raster_ms = statistics.mean(raster)
-    W, H = params.width, params.height
-    tex_bytes = 1024 * 1024 * 4
-    raster_bytes = W * H * 8 + tex_bytes            # colour + depth written once, texels read once
-    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
-
+    check(pf.lib().pfdInit(), "pfdInit")
+    frame = bench_frame(args)
+    print(f"1080p: {frame['value']:.4g} fragments/s, {frame['ms_per_step']:.3f} ms/frame", file=sys.stderr, flush=True)
     out = {
-        "metric": "shaded fragments/s at 1080p (1 GPU)",
-        "value": value,
+        "metric": "shaded fragments/s at 1080p (1 GPU) + batched frames/s; fwd+bwd time at 1e-3 grad err",
+        "value": frame["value"],
         "unit": "fragments/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": frame["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "1920x1080 RGBA8+f32 depth, 200k-triangle jittered height-field grid, "
-                               "1024^2 RGBA8 noise texture bilinear REPEAT, LEQUAL, 30% alpha-blended",
-                   "fragments_per_frame": frags, "raster_triangles": int(st.rasterTriangles),
-                   "tile_pairs": int(st.tilePairs), "tile": 1 << int(st.tileShift),
-                   "max_tile_list": int(st.maxTileList), "busy_tiles": int(st.busyTiles),
-                   "raster_mode": ["packed", "in-place"][cfg.rasterMode],
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_raster_packed" if cfg.rasterMode == 0 else "k_raster",
-                     "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
-        "phases_ms": {"setup": statistics.mean(setup), "binning": statistics.mean(binning), "raster": raster_ms},
-        "record_s": rec_s,
+        "config": frame["config"],
+        "roofline": frame["roofline"],
+        "cpu_baseline": frame["cpu_baseline"],
+        "phases_ms": frame["phases_ms"],
+        "record_s": frame["record_s"],
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(params, frags)
-    else:
-        out["cpu_baseline"] = None
-    lib.pfdJobDestroy(job)
+    if not args.no_batch:
+        out["batched"] = bench_batch(args)
+        print(f"batched: {out['batched']['value']:.4g} frames/s", file=sys.stderr, flush=True)
+    if not args.no_diff:
+        out["differentiable"] = bench_diff(args)
+        print(f"differentiable: {out['differentiable']['fwd_bwd_ms']:.3f} ms fwd+bwd", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -5,7 +5,8 @@
  * No reference counterpart: the reference rasterises on the CPU inside
  * pfVertex* (context.c:1671-1674).  Here a recorded frame (pf_frame.h) is
  * uploaded into a "job" resident in GPU memory and run by the kernel chain
- *   setup -> pair count/scan -> pair emit -> tile sort -> tile raster
+ *   setup count -> block scan -> setup write (records + tile pairs) ->
+ *   stable radix sort by tile -> tile ranges -> tile raster
  * captured once as a HIP graph.  A job holds one or more frames (one per
  * render target); several frames in one job render in the same launches
  * (the batched path of pfSubmitBatch).
@@ -94,9 +95,63 @@ int  pfdJobStats(PFDjob* job, PFDstats* st);
 int  pfdJobAutotune(PFDjob* job, PFDconfig* chosen);
 void pfdJobDestroy(PFDjob* job);
 
+/* ------------------------------------------------------------------------
+ * Differentiable rendering (no reference counterpart).
+ *
+ * A differentiable job renders one frame through the integer pipeline
+ * once, keeping for every pixel the raster triangle whose fragment was
+ * written last (coverage and depth winners are then fixed: edges stay
+ * hard), and shades those fragments again in floating point without
+ * quantisation:
+ *   colour  = sum_k b_k * corner colour_k (corner colours are parameters
+ *             where the corner came from a colour array; times material
+ *             diffuse / 255 when lit)
+ *   texture = texel taps of the reference's index maps, bilinear weights
+ *             (1-fx)(1-fy) .. fx fy, times the colour per channel
+ *   Phong   = clamp(sum over lights of ambient + kd * diffuse * D +
+ *             ks * specular * S, 0, 1), alpha 0, with exact sqrt and pow
+ * and returns gradients of a scalar loss with respect to the texels of one
+ * texture, the vertex colours and the lights' diffuse and specular colours
+ * given dL/dimage.  Frames with blending, flat shading, Gouraud lighting
+ * or a clear after the first command are rejected (PFD_E_INVALID).
+ * ------------------------------------------------------------------------ */
+
+typedef struct {
+    int32_t      texture;        /* index into PFFframe.textures whose texels are parameters, -1: none */
+    const float* texels;         /* [h][w][4] RGBA; NULL: that texture's own RGBA-float pixels */
+    uint32_t     numVertices;    /* colour parameters, addressed by PFFtriangles.vertexId */
+    const float* vertexColors;   /* [numVertices][4]; NULL: recorded RGBA8 / 255, no gradient */
+    const float* lightDiffuse;   /* [8][3] by light number; NULL: recorded colours / 255 */
+    const float* lightSpecular;  /* [8][3] */
+} PFDdiffParams;
+
+typedef struct {                 /* outputs; NULL members are not computed */
+    float* texels;               /* [h][w][4] */
+    float* vertexColors;         /* [numVertices][4] */
+    float* lightDiffuse;         /* [8][3] */
+    float* lightSpecular;        /* [8][3] */
+} PFDdiffGrads;
+
+typedef struct {
+    float rasterMs;              /* integer pass (setup, binning, raster with winners) */
+    float forwardMs;             /* float-domain forward */
+    float backwardMs;            /* adjoint (gradient clears, scatter, light sums)     */
+} PFDdiffTimes;
+
+int pfdDiffCreate(const PFFframe* frame, PFDjob** out);
+/* Float image [H][W][4] of the current parameters (host memory). */
+int pfdDiffForward(PFDjob* job, const PFDdiffParams* params, float* image);
+/* Gradients for dL/dimage [H][W][4] (host memory); outputs are overwritten. */
+int pfdDiffBackward(PFDjob* job, const PFDdiffParams* params, const float* dImage, PFDdiffGrads* grads);
+/* Mean device times of `iters` integer passes, forwards and adjoints on
+ * resident data (HIP events on the job's stream). */
+int pfdDiffTime(PFDjob* job, const PFDdiffParams* params, uint32_t iters, PFDdiffTimes* times);
+
 /* Frame executor that, instead of running the frame, keeps it as a
  * resident job; pfdTakeCapturedJob() hands the last one over. */
 int     pfdCaptureExecutor(const PFFframe* frame, void* user);
+/* Same for the frames of one pfSubmitBatch call (one job, all frames). */
+int     pfdCaptureBatchExecutor(const PFFframe* frames, uint32_t count, void* user);
 PFDjob* pfdTakeCapturedJob(void);
 
 #if defined(__cplusplus)

@@ -23,6 +23,20 @@ PF_API void pfxSetExecutor(PFXexecutor fn, void* user);
 /* Number of frames the current context has executed so far. */
 PF_API PFuint pfxFrameCount(void);
 
+/* Runs the frames of one pfSubmitBatch call together; returns 0 on success. */
+typedef int (*PFXbatchExecutor)(const PFFframe* frames, uint32_t count, void* user);
+
+/* Replaces the executor of pfSubmitBatch.  The default (NULL) runs the
+ * frames in one GPU job (pfdExecuteFrames), or one by one through the frame
+ * executor when one is installed. */
+PF_API void pfxSetBatchExecutor(PFXbatchExecutor fn, void* user);
+
+/* Mean device times (ms) over `iters` runs of the current context's
+ * differentiable frame (the last pfDiffRender): integer pass, float
+ * forward, adjoint.  Returns 0 on success. */
+PF_API int pfxDiffTime(const PFdiffparams* params, PFuint iters, float* rasterMs, float* forwardMs,
+                       float* backwardMs);
+
 #if defined(__cplusplus)
 }
 #endif

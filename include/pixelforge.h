@@ -473,6 +473,37 @@ PF_API void pfFlush(void);
  * waits for it; each context's buffers hold its result on return. */
 PF_API void pfSubmitBatch(const PFcontext* contexts, PFsizei count);
 
+/* Differentiable rendering.  pfDiffRender runs the current context's
+ * recorded frame once through the GPU pipeline (the target receives the
+ * usual RGBA8 image) and keeps, for every pixel, the fragment that was
+ * written last; it then writes a float image [H][W][4] of those fragments
+ * shaded without quantisation from the given parameters.  Called again with
+ * nothing recorded in between, it re-shades the same fragments with new
+ * parameter values (coverage stays fixed).  pfDiffBackward returns the
+ * gradients of a loss for dL/dimage [H][W][4].  Vertex colour parameter i
+ * is element i of the colour array of pfDrawElements / pfDrawArrays; light
+ * parameters are RGB per light number.  Frames with blending, flat shading,
+ * Gouraud lighting or a clear after the first command set
+ * PF_INVALID_OPERATION. */
+typedef struct {
+    PFtexture      texture;        /* texture whose texels are parameters, NULL: none        */
+    const PFfloat* texels;         /* [h][w][4] RGBA; NULL: that texture's RGBA-float pixels */
+    PFsizei        numVertices;
+    const PFfloat* vertexColors;   /* [numVertices][4]; NULL: recorded colours, no gradient  */
+    const PFfloat* lightDiffuse;   /* [8][3]; NULL: the lights' own colours                  */
+    const PFfloat* lightSpecular;  /* [8][3] */
+} PFdiffparams;
+
+typedef struct {                   /* outputs; NULL members are skipped */
+    PFfloat* texels;
+    PFfloat* vertexColors;
+    PFfloat* lightDiffuse;
+    PFfloat* lightSpecular;
+} PFdiffgrads;
+
+PF_API void pfDiffRender(const PFdiffparams* params, PFfloat* image);
+PF_API void pfDiffBackward(const PFdiffparams* params, const PFfloat* dImage, PFdiffgrads* grads);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -35,6 +35,13 @@
  * Parity of this oracle against the compiled reference is pinned by
  * tests/test_oracle_golden.py (golden frames) and tests/test_oracle_units.py
  * (closed forms and invariants).
+ *
+ * The second part (pforacle_diff_*) is the FP64 reference of the
+ * differentiable path, which has no counterpart in the reference: the same
+ * integer rendering fixes each pixel's winning fragment, which is then
+ * shaded in double precision in the float domain (no quantisation, exact
+ * sqrt/division/pow) and differentiated by hand.  It is pinned by FP64
+ * central finite differences of its own forward (tests/test_diff_oracle.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -525,6 +532,7 @@ typedef struct {
     float normal[3];
     float uv[2];
     uint32_t color;
+    float bw[3];             /* weights of the recorded corners (float domain only) */
 } Vtx;                                                     /* PFIvertex, context.h:203-210 */
 
 /* pfiLerpVertex (ictx.c:67-90). */
@@ -540,6 +548,7 @@ static Vtx lerp_vtx(const Vtx* s, const Vtx* e, float t)
         col[i] = (uint8_t)(ch(s->color, i) + (uT * (ch(e->color, i) - ch(s->color, i))) / 255);
         if (i < 2) r.uv[i] = s->uv[i] + t * (e->uv[i] - s->uv[i]);
         if (i < 3) r.normal[i] = s->normal[i] + t * (e->normal[i] - s->normal[i]);
+        if (i < 3) r.bw[i] = s->bw[i] + t * (e->bw[i] - s->bw[i]);
     }
     r.color = pack_i(col, 4);
     return r;
@@ -645,9 +654,26 @@ static int project_clip(const PFFstate* st, Vtx* poly, int* n)
 /* Rasteriser                                                                */
 /* ------------------------------------------------------------------------ */
 
+/* Raster triangle kept for the float-domain shading of its winning pixels. */
+typedef struct {
+    const PFFstate* st;
+    int      face, is3D;
+    uint32_t src;            /* recorded triangle */
+    float    wInvSum;
+    float    z[3], uv[3][2], nrm[3][3], pos[3][3], bw[3][3];
+} DRec;
+
 typedef struct {
     const PFFframe* f;
     uint64_t fragments;      /* covered samples, before the depth test */
+    /* winner tracking (differentiable path only) */
+    int32_t* win;            /* per pixel: DRec index of the last fragment written, -1 */
+    int32_t* ww;             /* per pixel: its w1, w2, w3 */
+    DRec*    recs;
+    size_t   nrecs, caprecs;
+    int32_t  cur;            /* DRec of the triangle being rasterised, -1: none yet */
+    uint32_t src;
+    int      oom;
 } RCtx;
 
 static int depth_pass(uint32_t fn, float z, float d)     /* depth.h:80-124 */
@@ -712,6 +738,7 @@ static void raster_tri(RCtx* rc, const PFFstate* st, int face, int is3D,
     int dtest = (st->flags & PFF_DEPTH_TEST) != 0;
     const PFFmaterial* mat = &st->material[face];
     const int W = (int)f->width, H = (int)f->height;
+    rc->cur = -1;
 
     /* Rows yMin..yMax inclusive, columns xMin..xMax-1 (tri.c:404-420). */
     for (int32_t y = yMin; y <= yMax; y++) {
@@ -748,6 +775,32 @@ static void raster_tri(RCtx* rc, const PFFstate* st, int face, int is3D,
             if (blend) frag = pforacle_blend(st->blendMode, frag, f->color[off]);
             f->color[off] = frag;
             f->depth[off] = zq;
+            if (rc->win) {
+                if (rc->cur < 0) {
+                    if (rc->nrecs == rc->caprecs) {
+                        size_t nc = rc->caprecs ? 2 * rc->caprecs : 1024;
+                        DRec* q = (DRec*)realloc(rc->recs, nc * sizeof(DRec));
+                        if (!q) { rc->oom = 1; return; }
+                        rc->recs = q;
+                        rc->caprecs = nc;
+                    }
+                    DRec* d = &rc->recs[rc->nrecs];
+                    const Vtx* vv[3] = { v1, v2, v3 };
+                    d->st = st; d->face = face; d->is3D = is3D; d->src = rc->src; d->wInvSum = wInvSum;
+                    for (int k = 0; k < 3; k++) {
+                        d->z[k] = vv[k]->hom[2];
+                        for (int i = 0; i < 2; i++) d->uv[k][i] = vv[k]->uv[i];
+                        for (int i = 0; i < 3; i++) {
+                            d->nrm[k][i] = vv[k]->normal[i];
+                            d->pos[k][i] = vv[k]->pos[i];
+                            d->bw[k][i] = vv[k]->bw[i];
+                        }
+                    }
+                    rc->cur = (int32_t)rc->nrecs++;
+                }
+                rc->win[off] = rc->cur;
+                rc->ww[3 * off] = w1; rc->ww[3 * off + 1] = w2; rc->ww[3 * off + 2] = w3;
+            }
         }
     }
 }
@@ -798,9 +851,22 @@ static void clear(const PFFframe* f, const PFFcmd* c)
 /* Renders a recorded frame into frame->color / frame->depth.  Returns 0 on
  * success; *fragments receives the number of covered samples (before the
  * depth test) summed over rasterised triangles. */
+static int render_frame(RCtx* rc);
+
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
 {
-    RCtx rc = { f, 0 };
+    RCtx rc;
+    memset(&rc, 0, sizeof rc);
+    rc.f = f;
+    int r = render_frame(&rc);
+    if (fragments) *fragments = rc.fragments;
+    return r;
+}
+
+static int render_frame(RCtx* rcp)
+{
+    RCtx rc = *rcp;
+    const PFFframe* f = rc.f;
     for (uint32_t c = 0; c < f->numCmds; c++) {
         const PFFcmd* cmd = &f->cmds[c];
         if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
@@ -812,8 +878,10 @@ int pforacle_render(const PFFframe* f, uint64_t* fragments)
             const PFFstate* st = &f->states[si];
             Vtx v[3];
             memset(v, 0, sizeof v);
+            rc.src = t;
             for (int k = 0; k < 3; k++) {
                 size_t vi = (size_t)t * 3 + k;
+                v[k].bw[k] = 1.0f;
                 memcpy(v[k].pos, f->tris.position + 4 * vi, 16);
                 v[k].color = f->tris.color[vi];
                 if (f->tris.texcoord) memcpy(v[k].uv, f->tris.texcoord + 2 * vi, 8);
@@ -823,8 +891,8 @@ int pforacle_render(const PFFframe* f, uint64_t* fragments)
             if (info & PFF_PASS_BACK)  process_pass(&rc, st, 1, v);
         }
     }
-    if (fragments) *fragments = rc.fragments;
-    return 0;
+    *rcp = rc;
+    return rc.oom ? -3 : 0;
 }
 
 /* Frame executor with the signature of PFXexecutor (pf_ext.h), so that a
@@ -842,3 +910,308 @@ int pforacle_executor(const PFFframe* f, void* user)
 /* Exported for the unit tests of the oracle's own arithmetic. */
 float pforacle_rcp(float x) { return rcp_(x); }
 float pforacle_rsqrt(float x) { return rsqrt_(x); }
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable path, FP64 (no reference counterpart)                      */
+/*                                                                           */
+/* Float-domain shading of each pixel's winning fragment, as defined for    */
+/* the GPU adjoint: colour = sum_k b_k sum_j M[k][j] C_j (times material    */
+/* diffuse when lit), texture taps of the reference index maps with         */
+/* bilinear weights, Phong with exact sqrt/division/pow, clamp to [0, 1],   */
+/* alpha 0.  Barycentrics and texture taps are computed in float exactly as */
+/* the integer path does (they fix the geometry); everything that depends   */
+/* on a parameter is evaluated in double.                                   */
+/* ------------------------------------------------------------------------ */
+
+#define DIFF_LIGHT_VALS (2 * PFF_MAX_LIGHTS * 3)
+
+typedef struct {
+    int32_t       texture;        /* frame texture whose texels are parameters, -1 */
+    const double* texels;         /* [h][w][4] */
+    uint32_t      numVertices;
+    const double* vertexColors;   /* [numVertices][4], or NULL */
+    const double* lights;         /* [DIFF_LIGHT_VALS]: diffuse then specular by light number, or NULL */
+} PFOdiffParams;
+
+typedef struct {
+    PFFframe  f;                  /* the rendered frame; states, textures and the
+                                     colour / vertex-id streams are private copies,
+                                     texel memory stays the caller's */
+    uint32_t* color;              /* RGBA8 after the integer pass */
+    float*    depth;
+    PFFstate* states;
+    PFFtexture* texs;
+    uint32_t* tcol;
+    uint32_t* tvid;
+    RCtx      rc;
+} DiffJob;
+
+static void diff_job_free(DiffJob* J)
+{
+    free(J->color); free(J->depth); free(J->rc.win); free(J->rc.ww); free(J->rc.recs);
+    free(J->states); free(J->texs); free(J->tcol); free(J->tvid); free(J);
+}
+
+/* Renders the frame with winner tracking into private buffers.  Returns a
+ * handle, or NULL. */
+void* pforacle_diff_prepare(const PFFframe* f)
+{
+    DiffJob* J = (DiffJob*)calloc(1, sizeof(DiffJob));
+    if (!J) return NULL;
+    size_t n = (size_t)f->width * f->height;
+    J->f = *f;
+    size_t nt = (size_t)f->tris.count * 3;
+    J->color = (uint32_t*)malloc((n + 8) * 4);
+    J->depth = (float*)malloc((n + 8) * 4);
+    J->rc.win = (int32_t*)malloc((n + 1) * 4);
+    J->rc.ww = (int32_t*)malloc((3 * n + 3) * 4);
+    J->states = (PFFstate*)malloc((f->numStates + 1) * sizeof(PFFstate));
+    J->texs = (PFFtexture*)malloc((f->numTextures + 1) * sizeof(PFFtexture));
+    J->tcol = (uint32_t*)malloc((nt + 1) * 4);
+    J->tvid = (uint32_t*)malloc((nt + 1) * 4);
+    if (!J->color || !J->depth || !J->rc.win || !J->rc.ww || !J->states || !J->texs || !J->tcol || !J->tvid) goto fail;
+    memcpy(J->states, f->states, f->numStates * sizeof(PFFstate));
+    memcpy(J->texs, f->textures, f->numTextures * sizeof(PFFtexture));
+    memcpy(J->tcol, f->tris.color, nt * 4);
+    if (f->tris.vertexId) memcpy(J->tvid, f->tris.vertexId, nt * 4);
+    J->f.states = J->states;
+    J->f.textures = J->texs;
+    memcpy(J->color, f->color, n * 4);
+    memcpy(J->depth, f->depth, n * 4);
+    for (size_t i = 0; i < n; i++) J->rc.win[i] = -1;
+    J->f.color = J->color;
+    J->f.depth = J->depth;
+    J->rc.f = &J->f;
+    if (render_frame(&J->rc) != 0) goto fail;
+    /* Only these streams are read after the render. */
+    J->f.tris.color = J->tcol;
+    J->f.tris.vertexId = f->tris.vertexId ? J->tvid : NULL;
+    J->f.tris.position = NULL;
+    J->f.tris.texcoord = NULL;
+    J->f.tris.normal = NULL;
+    J->f.tris.info = NULL;
+    J->f.cmds = NULL;
+    return J;
+fail:
+    diff_job_free(J);
+    return NULL;
+}
+
+void pforacle_diff_free(void* h)
+{
+    if (h) diff_job_free((DiffJob*)h);
+}
+
+/* Winner of pixel i (index into the raster records, -1) and the integer
+ * RGBA8 image, for the tests. */
+const int32_t* pforacle_diff_winners(void* h) { return ((DiffJob*)h)->rc.win; }
+const uint32_t* pforacle_diff_color(void* h) { return ((DiffJob*)h)->color; }
+
+typedef struct {              /* texel taps of one lookup */
+    int32_t off[4];
+    double  w[4];
+    int     n;
+} DTaps;
+
+static int32_t clamp_off(const PFFtexture* t, int64_t o)
+{
+    int64_t n = (int64_t)t->width * t->height;
+    return (int32_t)(o < 0 ? 0 : (o >= n ? n - 1 : o));
+}
+
+/* Index maps of pforacle_sample (sampler.h:202-334) in float; weights in
+ * double from the float fractions. */
+static void diff_taps(const PFFtexture* t, float u, float v, DTaps* tp)
+{
+    uint32_t w = t->width;
+    if (t->filter == 0) {
+        int32_t x = wrap_coord(t->wrap, u, t->width), y = wrap_coord(t->wrap, v, t->height);
+        tp->n = 1;
+        tp->off[0] = clamp_off(t, (int32_t)((uint32_t)y * w + (uint32_t)x));
+        tp->w[0] = 1.0;
+        return;
+    }
+    float tx = 1.0f / (float)t->width, ty = 1.0f / (float)t->height;
+    int32_t x0 = wrap_coord(t->wrap, u, t->width), y0 = wrap_coord(t->wrap, v, t->height);
+    int32_t x1 = wrap_coord(t->wrap, u + tx, t->width), y1 = wrap_coord(t->wrap, v + ty, t->height);
+    double fx = clamp_ps(u * (float)t->width - (float)x0, 0.0f, 1.0f);
+    double fy = clamp_ps(v * (float)t->height - (float)y0, 0.0f, 1.0f);
+    tp->n = 4;
+    tp->off[0] = clamp_off(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x0));
+    tp->off[1] = clamp_off(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x1));
+    tp->off[2] = clamp_off(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x0));
+    tp->off[3] = clamp_off(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x1));
+    tp->w[0] = (1.0 - fx) * (1.0 - fy);
+    tp->w[1] = fx * (1.0 - fy);
+    tp->w[2] = (1.0 - fx) * fy;
+    tp->w[3] = fx * fy;
+}
+
+static void texel_d(const PFFtexture* t, const PFOdiffParams* p, int param, int32_t off, double* o)
+{
+    if (param) { for (int c = 0; c < 4; c++) o[c] = p->texels[(size_t)off * 4 + c]; return; }
+    if (t->format == PFF_TEX_RGBA8) {
+        uint32_t c = ((const uint32_t*)t->pixels)[off];
+        for (int i = 0; i < 4; i++) o[i] = ch(c, i) / 255.0;
+        return;
+    }
+    for (int c = 0; c < 4; c++) o[c] = ((const float*)t->pixels)[(size_t)off * 4 + c];
+}
+
+static double light_param(const PFOdiffParams* p, const PFFlight* L, int which, int c)
+{
+    if (p->lights) return p->lights[which * PFF_MAX_LIGHTS * 3 + (L->id & 7u) * 3 + c];
+    return ch(which ? L->specular : L->diffuse, c) / 255.0;
+}
+
+typedef struct { double amb[3], kd, ks; } DLight;
+
+/* Phong factors of one light (lighting.c:148-258 without approximations). */
+static DLight light_terms(const PFFlight* L, const PFFmaterial* mat, const double* P, const double* N, const double* V)
+{
+    DLight r;
+    double Ld[3], d2 = 0.0;
+    for (int i = 0; i < 3; i++) { Ld[i] = L->position[i] - P[i]; d2 += Ld[i] * Ld[i]; }
+    double il = 1.0 / sqrt(d2 > 1e-5 ? d2 : 1e-5);
+    for (int i = 0; i < 3; i++) Ld[i] *= il;
+    double nl = N[0] * Ld[0] + N[1] * Ld[1] + N[2] * Ld[2];
+    if (nl < 0.0) nl = 0.0;
+    double H[3] = { Ld[0] + V[0], Ld[1] + V[1], Ld[2] + V[2] };
+    double hs = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+    double nh = hs > 0.0 ? (N[0] * H[0] + N[1] * H[1] + N[2] * H[2]) / sqrt(hs) : 0.0;
+    if (nh < 0.0) nh = 0.0;
+    double spec = pow(nh, (double)mat->shininess);
+    double I = 1.0, att = 1.0;
+    if ((double)L->innerCutOff < PF_PI_D) {
+        double theta = -(Ld[0] * L->direction[0] + Ld[1] * L->direction[1] + Ld[2] * L->direction[2]);
+        I = (theta - L->outerCutOff) / ((double)L->innerCutOff - L->outerCutOff);
+        I = I < 0.0 ? 0.0 : (I > 1.0 ? 1.0 : I);
+    }
+    if (L->attLinear != 0.0f || L->attQuadratic != 0.0f) {
+        double d0 = L->position[0] - P[0], d1 = L->position[1] - P[1];
+        double dsq = d0 * d0 + 2.0 * d1 * d1;                 /* simd.h:2427-2429 reads y twice */
+        att = 1.0 / (L->attConstant + L->attLinear * sqrt(dsq) + L->attQuadratic * dsq);
+    }
+    for (int c = 0; c < 3; c++) r.amb[c] = att * (ch(L->ambient, c) / 255.0) * (ch(mat->ambient, c) / 255.0);
+    r.kd = att * I * nl;
+    r.ks = att * I * spec;
+    return r;
+}
+
+/* Forward (image != NULL) and/or adjoint (dImage != NULL) over all pixels.
+ * Gradient outputs are overwritten; any may be NULL. */
+int pforacle_diff_eval(void* h, const PFOdiffParams* p, double* image, const double* dImage,
+                       double* gTex, double* gCol, double* gLight)
+{
+    DiffJob* J = (DiffJob*)h;
+    const PFFframe* f = &J->f;
+    const size_t npx = (size_t)f->width * f->height;
+    if (gTex && p->texture >= 0 && (uint32_t)p->texture < f->numTextures) {
+        const PFFtexture* T = &f->textures[p->texture];
+        memset(gTex, 0, (size_t)T->width * T->height * 4 * sizeof(double));
+    }
+    if (gCol) memset(gCol, 0, (size_t)p->numVertices * 4 * sizeof(double));
+    if (gLight) memset(gLight, 0, DIFF_LIGHT_VALS * sizeof(double));
+    for (size_t pix = 0; pix < npx; pix++) {
+        int32_t wi = J->rc.win[pix];
+        if (wi < 0) {
+            if (image) for (int c = 0; c < 4; c++) image[pix * 4 + c] = ch(J->color[pix], c) / 255.0;
+            continue;
+        }
+        const DRec* d = &J->rc.recs[wi];
+        const PFFstate* st = d->st;
+        const PFFmaterial* mat = &st->material[d->face];
+        const int32_t* w = &J->rc.ww[3 * pix];
+        const float bf[3] = { (float)w[0] * d->wInvSum, (float)w[1] * d->wInvSum, (float)w[2] * d->wInvSum };
+        double beta[3], m[4] = { 1, 1, 1, 1 }, cf[4] = { 0, 0, 0, 0 }, tex[4] = { 1, 1, 1, 1 };
+        for (int j = 0; j < 3; j++) beta[j] = (double)bf[0] * d->bw[0][j] + (double)bf[1] * d->bw[1][j] + (double)bf[2] * d->bw[2][j];
+        if (st->flags & PFF_LIGHTING) for (int c = 0; c < 4; c++) m[c] = ch(mat->diffuse, c) / 255.0;
+        uint32_t vids[3];
+        for (int j = 0; j < 3; j++) {
+            size_t ci = (size_t)d->src * 3 + j;
+            vids[j] = f->tris.vertexId ? f->tris.vertexId[ci] : PFF_NO_VERTEX;
+            double C[4];
+            if (p->vertexColors && vids[j] < p->numVertices)
+                for (int c = 0; c < 4; c++) C[c] = p->vertexColors[(size_t)vids[j] * 4 + c];
+            else
+                for (int c = 0; c < 4; c++) C[c] = ch(f->tris.color[ci], c) / 255.0;
+            for (int c = 0; c < 4; c++) cf[c] += beta[j] * (C[c] * m[c]);
+        }
+        int texOn = (st->flags & PFF_TEXTURE_2D) && st->texture >= 0 && (uint32_t)st->texture < f->numTextures;
+        int texParam = texOn && st->texture == p->texture;
+        DTaps tp;
+        tp.n = 0;
+        if (texOn) {
+            const PFFtexture* T = &f->textures[st->texture];
+            float u = (d->uv[0][0] * bf[0] + d->uv[1][0] * bf[1]) + d->uv[2][0] * bf[2];
+            float v = (d->uv[0][1] * bf[0] + d->uv[1][1] * bf[1]) + d->uv[2][1] * bf[2];
+            if (d->is3D) {
+                float zq = 1.0f / ((d->z[0] * bf[0] + d->z[1] * bf[1]) + d->z[2] * bf[2]);
+                u = u * zq;
+                v = v * zq;
+            }
+            diff_taps(T, u, v, &tp);
+            for (int c = 0; c < 4; c++) tex[c] = 0.0;
+            for (int k = 0; k < tp.n; k++) {
+                double q[4];
+                texel_d(T, p, texParam, tp.off[k], q);
+                for (int c = 0; c < 4; c++) tex[c] += tp.w[k] * q[c];
+            }
+        }
+        double frag[4], out[4], dfrag[4] = { 0, 0, 0, 0 }, g[4] = { 0, 0, 0, 0 };
+        if (dImage) for (int c = 0; c < 4; c++) g[c] = dImage[pix * 4 + c];
+        for (int c = 0; c < 4; c++) { frag[c] = cf[c] * tex[c]; dfrag[c] = g[c]; }
+        int phong = (st->flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
+        if (phong) {
+            double N[3], P[3], V[3], S[3], acc[3] = { 0, 0, 0 }, vl = 0.0;
+            for (int i = 0; i < 3; i++) {
+                N[i] = d->nrm[0][i] * (double)bf[0] + d->nrm[1][i] * (double)bf[1] + d->nrm[2][i] * (double)bf[2];
+                P[i] = d->pos[0][i] * (double)bf[0] + d->pos[1][i] * (double)bf[1] + d->pos[2][i] * (double)bf[2];
+                V[i] = st->viewPos[i] - P[i];
+                vl += V[i] * V[i];
+                S[i] = ch(mat->specular, i) / 255.0;
+            }
+            vl = 1.0 / sqrt(vl > 1e-5 ? vl : 1e-5);
+            for (int i = 0; i < 3; i++) V[i] *= vl;
+            DLight lt[PFF_MAX_LIGHTS];
+            for (uint32_t l = 0; l < st->numLights; l++) {
+                const PFFlight* L = &st->lights[l];
+                lt[l] = light_terms(L, mat, P, N, V);
+                for (int c = 0; c < 3; c++)
+                    acc[c] += lt[l].amb[c] * frag[c] + lt[l].kd * light_param(p, L, 0, c) * frag[c]
+                            + lt[l].ks * light_param(p, L, 1, c) * S[c];
+            }
+            for (int c = 0; c < 3; c++) out[c] = acc[c] < 0.0 ? 0.0 : (acc[c] > 1.0 ? 1.0 : acc[c]);
+            out[3] = 0.0;
+            if (dImage) {
+                double gc[3];
+                for (int c = 0; c < 3; c++) gc[c] = (acc[c] > 0.0 && acc[c] < 1.0) ? g[c] : 0.0;
+                for (int c = 0; c < 3; c++) dfrag[c] = 0.0;
+                dfrag[3] = 0.0;
+                for (uint32_t l = 0; l < st->numLights; l++) {
+                    const PFFlight* L = &st->lights[l];
+                    for (int c = 0; c < 3; c++) {
+                        dfrag[c] += gc[c] * (lt[l].amb[c] + lt[l].kd * light_param(p, L, 0, c));
+                        if (gLight) {
+                            gLight[(L->id & 7u) * 3 + c] += gc[c] * lt[l].kd * frag[c];
+                            gLight[PFF_MAX_LIGHTS * 3 + (L->id & 7u) * 3 + c] += gc[c] * lt[l].ks * S[c];
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int c = 0; c < 4; c++) out[c] = frag[c];
+        }
+        if (image) for (int c = 0; c < 4; c++) image[pix * 4 + c] = out[c];
+        if (!dImage) continue;
+        if (texParam && gTex)
+            for (int k = 0; k < tp.n; k++)
+                for (int c = 0; c < 4; c++) gTex[(size_t)tp.off[k] * 4 + c] += tp.w[k] * dfrag[c] * cf[c];
+        if (gCol && p->vertexColors)
+            for (int j = 0; j < 3; j++) {
+                if (vids[j] >= p->numVertices) continue;
+                for (int c = 0; c < 4; c++) gCol[(size_t)vids[j] * 4 + c] += beta[j] * m[c] * dfrag[c] * tex[c];
+            }
+    }
+    return 0;
+}

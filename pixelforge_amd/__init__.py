@@ -82,6 +82,7 @@ class PFDconfig(C.Structure):
 
 
 EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+BATCH_EXECUTOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.c_void_p)
 
 _lib = None
 _scenes = None
@@ -110,6 +111,10 @@ def lib():
         L.pfdTakeCapturedJob.restype = C.c_void_p
         L.pfxSetExecutor.argtypes = [C.c_void_p, C.c_void_p]
         L.pfxSetExecutor.restype = None
+        L.pfxSetBatchExecutor.argtypes = [C.c_void_p, C.c_void_p]
+        L.pfxSetBatchExecutor.restype = None
+        L.pfSubmitBatch.argtypes = [C.c_void_p, C.c_uint32]
+        L.pfSubmitBatch.restype = None
         L.pfCreateContext.restype = C.c_void_p
         L.pfCreateContext.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int]
         L.pfMakeCurrent.argtypes = [C.c_void_p]
@@ -189,3 +194,97 @@ def set_default_config(tile_shift=3, raster_mode=0):
 def unpack(color):
     """uint32 RGBA8 image -> uint8 [H, W, 4] (r, g, b, a)."""
     return color.view(np.uint8).reshape(color.shape + (4,))
+
+
+PF_RGBA, PF_UNSIGNED_BYTE = 7, 0
+
+
+class Captured:
+    """A recorded frame (or batch of frames) kept resident on the GPU as a
+    PFDjob, with the host objects it refers to (contexts, target buffers,
+    textures) kept alive until close()."""
+
+    def __init__(self, job, ctxs, buffers, textures):
+        self.job = C.c_void_p(job)
+        self.ctxs, self.buffers, self.textures = ctxs, buffers, textures
+
+    def close(self):
+        L = lib()
+        if self.job.value:
+            L.pfdJobDestroy(self.job)
+            self.job = C.c_void_p(0)
+        for ctx, (tex, px) in zip(self.ctxs, self.textures):
+            L.pfMakeCurrent(ctx)
+            scenes().pfs_free_texture(api(), tex, px)
+            L.pfDeleteContext(ctx)
+        L.pfMakeCurrent(None)
+        self.ctxs, self.textures = [], []
+
+
+def _record(params):
+    L = lib()
+    color = np.zeros(params.width * params.height + 16, np.uint32)
+    ctx = L.pfCreateContext(color.ctypes.data, params.width, params.height, PF_RGBA, PF_UNSIGNED_BYTE)
+    if not ctx:
+        raise RuntimeError("pfCreateContext failed")
+    L.pfMakeCurrent(ctx)
+    tex, texpx = C.c_void_p(), C.c_void_p()
+    err = scenes().pfs_record(api(), C.byref(params), C.byref(tex), C.byref(texpx))
+    if err:
+        raise RuntimeError(f"scene recording failed: {err}")
+    return ctx, color, (tex, texpx)
+
+
+def capture(params):
+    """Records one scene through the API and returns it as a resident job."""
+    L = lib()
+    ctx, color, tex = _record(params)
+    L.pfxSetExecutor(C.cast(L.pfdCaptureExecutor, C.c_void_p), None)
+    try:
+        L.pfFinish()
+    finally:
+        L.pfxSetExecutor(None, None)
+    job = L.pfdTakeCapturedJob()
+    if not job:
+        raise RuntimeError(f"frame capture failed: {L.pfdLastError().decode()}")
+    return Captured(job, [ctx], [color], [tex])
+
+
+def capture_batch(params_list):
+    """Records one context per scene and captures them, as pfSubmitBatch
+    hands them over, into one resident job."""
+    L = lib()
+    recs = [_record(p) for p in params_list]
+    ctxs = (C.c_void_p * len(recs))(*[r[0] for r in recs])
+    L.pfxSetBatchExecutor(C.cast(L.pfdCaptureBatchExecutor, C.c_void_p), None)
+    try:
+        L.pfSubmitBatch(ctxs, len(recs))
+    finally:
+        L.pfxSetBatchExecutor(None, None)
+    job = L.pfdTakeCapturedJob()
+    if not job:
+        raise RuntimeError(f"batch capture failed: {L.pfdLastError().decode()}")
+    return Captured(job, [r[0] for r in recs], [r[1] for r in recs], [r[2] for r in recs])
+
+
+def batch_workload(n=256, seed=4, max_side=1024, max_tris=50000, max_tex=512):
+    """The batched workload: n independent contexts, framebuffer sides
+    uniform in [64, max_side], triangle counts log-uniform in
+    [100, max_tris] (jittered height-field grids about 10 cells wide per
+    cell deep), RGBA8 noise textures with sides uniform in [64, max_tex]
+    (bilinear, REPEAT), depth LEQUAL, back faces culled; every second
+    context draws all its triangles with additive blending."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        w, h = (int(v) for v in rng.integers(64, max_side + 1, 2))
+        tris = float(np.exp(rng.uniform(np.log(100.0), np.log(max_tris))))
+        gz = max(1, int(round((tris / 20.0) ** 0.5)))
+        gx = max(1, int(round(tris / (2.0 * gz))))
+        tw, th = (int(v) for v in rng.integers(64, max_tex + 1, 2))
+        add = i % 2 == 1
+        out.append(scene(kind=HEIGHTFIELD, width=w, height=h, seed=1000 + i, gridX=gx, gridZ=gz,
+                         texture=1, texW=tw, texH=th, wrap=PF_REPEAT, filter=PF_BILINEAR,
+                         depthFunc=PF_LEQUAL, cull=PF_BACK, blend=1 if add else 0,
+                         blendMode=PF_BLEND_ADD, blendFraction=1.0 if add else 0.0))
+    return out

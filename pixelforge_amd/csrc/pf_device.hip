@@ -93,6 +93,17 @@ struct __attribute__((aligned(16))) RTriL {   // per-fragment lighting inputs
 };
 static_assert(sizeof(RTriL) == 80, "RTriL layout");
 
+// Float-domain companion of a raster triangle (differentiable jobs only):
+// corner k of the raster triangle is sum_j M[3k+j] * corner j of recorded
+// triangle `src` (the identity unless clipping made new vertices).
+struct __attribute__((aligned(16))) RTriD {
+    float    M[9];
+    uint32_t src;
+    uint32_t face;
+    uint32_t pad;
+};
+static_assert(sizeof(RTriD) == 48, "RTriD layout");
+
 enum : uint32_t {
     META_BACK      = 1u << 0,
     META_3D        = 1u << 1,
@@ -504,6 +515,7 @@ struct Vtx {
     float nrm[3];
     float uv[2];
     uint32_t col;
+    float bw[3];      // weights of the recorded triangle's corners (float domain)
 };
 
 __device__ __forceinline__ Vtx lerp_vtx(const Vtx& s, const Vtx& e, float t) {   // ictx.c:67-90
@@ -518,6 +530,7 @@ __device__ __forceinline__ Vtx lerp_vtx(const Vtx& s, const Vtx& e, float t) {  
     }
     for (int i = 0; i < 2; i++) r.uv[i] = s.uv[i] + t * (e.uv[i] - s.uv[i]);
     for (int i = 0; i < 3; i++) r.nrm[i] = s.nrm[i] + t * (e.nrm[i] - s.nrm[i]);
+    for (int i = 0; i < 3; i++) r.bw[i] = s.bw[i] + t * (e.bw[i] - s.bw[i]);
     r.scr[0] = r.scr[1] = 0.0f;
     r.col = col;
     return r;
@@ -606,6 +619,7 @@ struct SetupArgs {
     uint32_t*       tileEnd;
     RTri*           rts;
     RTriL*          rtl;
+    RTriD*          rtd;            // float-domain records, or null
     uint32_t*       keys;           // tile of each pair
     uint32_t*       vals;           // raster triangle of each pair
     uint32_t        capRT;
@@ -621,7 +635,7 @@ struct TileSpan { uint32_t tx0, tx1, ty0, ty1; };
 // tile pairs.  Returns false when it is culled or covers no pixel.
 template <int PHASE>
 __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, const DFrame& F,
-                                            uint32_t stateIdx, int face, bool is3D,
+                                            uint32_t stateIdx, uint32_t src, int face, bool is3D,
                                             const Vtx& a, const Vtx& b, const Vtx& c,
                                             uint32_t rtSlot, uint32_t pairSlot, uint32_t& pairsOut) {
     int32_t x1 = cvtt_i32(a.scr[0]), y1 = cvtt_i32(a.scr[1]);
@@ -697,6 +711,14 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         l.pad[0] = l.pad[1] = 0.0f;
         A.rtl[rtSlot] = l;
     }
+    if (A.rtd) {
+        RTriD d;
+        for (int j = 0; j < 3; j++) { d.M[j] = a.bw[j]; d.M[3 + j] = b.bw[j]; d.M[6 + j] = c.bw[j]; }
+        d.src = src;
+        d.face = (uint32_t)face;
+        d.pad = 0;
+        A.rtd[rtSlot] = d;
+    }
     // Tile pairs of this raster triangle, in submission order.
     uint32_t p = pairSlot;
     for (uint32_t ty = ty0; ty <= ty1; ty++)
@@ -727,6 +749,9 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
         if (PHASE == 1 && A.nrm) { base[k].nrm[0] = A.nrm[3 * vi]; base[k].nrm[1] = A.nrm[3 * vi + 1]; base[k].nrm[2] = A.nrm[3 * vi + 2]; }
         else { base[k].nrm[0] = base[k].nrm[1] = base[k].nrm[2] = 0.0f; }
         base[k].scr[0] = base[k].scr[1] = 0.0f;
+        base[k].bw[0] = k == 0 ? 1.0f : 0.0f;
+        base[k].bw[1] = k == 1 ? 1.0f : 0.0f;
+        base[k].bw[2] = k == 2 ? 1.0f : 0.0f;
     }
     const uint32_t flags = st->flags;
     for (int face = 0; face < 2; face++) {
@@ -776,7 +801,7 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
         uint32_t np = 0;
         if (fabsf(wsum - 3.0f) < 1e-5f) {                                 // "2D" path
             for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
-            if (emit_raster<PHASE>(A, st, F, si, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
             }
@@ -799,7 +824,7 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
                 v[k].hom[1] *= invW;
                 to_screen(stl, v[k]);
             }
-            if (emit_raster<PHASE>(A, st, F, si, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE>(A, st, F, si, t, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
             }
@@ -819,7 +844,7 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
             to_screen(stl, poly[k]);
         }
         for (int i = 0; i < n - 2; i++)
-            if (emit_raster<PHASE>(A, st, F, si, face, true, poly[0], poly[i + 1], poly[i + 2], rtBase + nRT,
+            if (emit_raster<PHASE>(A, st, F, si, t, face, true, poly[0], poly[i + 1], poly[i + 2], rtBase + nRT,
                                    pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
@@ -1067,7 +1092,11 @@ struct RasterArgs {
     const uint32_t* vals;           // sorted pair values (raster triangle ids)
     uint32_t        numFrames;
     uint32_t        numTiles;
-    unsigned long long* fragCounter;
+    uint32_t*       tileFrags;      // [numTiles][4]: covered samples per tile and wave (plain
+                                    // stores: one counter shared by all waves costs more than
+                                    // the raster itself)
+    uint32_t*       winner;         // per pixel of frame 0: 1 + raster triangle of the last
+                                    // fragment written, 0 for none; null unless differentiable
 };
 
 // ---------------------------------------------------------------------------
@@ -1161,7 +1190,8 @@ __device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& 
 
 // Depth test and blend of one fragment against the lane's pixel
 // (depth.h:80-124, blend.h:137-274, triangles.c:434-438, 523-529).
-__device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t& col, float& dep) {
+__device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t& col, float& dep,
+                                    uint32_t& win) {
     if (meta & META_DEPTH) {
         uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
         bool ok;
@@ -1177,12 +1207,17 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
     if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
     col = c;
     dep = z;
+    win = id + 1u;
 }
+
+// Dword of a staged record that holds its raster-triangle index (the
+// first pad word of RTri, filled when the record is staged).
+constexpr int REC_ID = offsetof(RTri, pad) / 4;
 
 // Shades the queued fragments and applies them in order (packed mode).
 template <bool PHONG>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
-                                            int nf, int head, uint32_t& col, float& dep) {
+                                            int nf, int head, uint32_t& col, float& dep, uint32_t& win) {
     wave_sync();
     if (lane < nf) {
         const int j = Q.ftri[lane];
@@ -1197,7 +1232,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
     wave_sync();
     for (int s = head; s >= 0;) {
         const int j = Q.ftri[s];
-        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], col, dep);
+        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.rec[j][REC_ID], col, dep, win);
         const int nx = Q.fnext[s];
         s = (nx == 0xFF) ? -1 : nx;
     }
@@ -1214,7 +1249,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
     const bool in = px < (int)F.W && py < (int)F.H;
     const size_t npix = (size_t)F.W * F.H;
     const size_t aligned = npix - (npix % 8);
-    uint32_t col = 0u;
+    uint32_t col = 0u, win = 0u;
     float dep = 0.0f;
     const size_t pi = in ? (size_t)py * F.W + px : 0;
     if (in) {
@@ -1242,6 +1277,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
             q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
             q4 = src[4]; q5 = src[5]; q6 = src[6]; q7 = src[7];
+            q7.x = r;                                    // REC_ID
             const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
             const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
             ov = !(xMax <= bx0 || xMin > bx0 + 7 || yMax < by0 || yMin > by0 + 7);
@@ -1277,12 +1313,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     uint32_t fc;
                     float fz;
                     shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), px, py, fc, fz);
-                    rop(t.meta, fc, fz, col, dep);
+                    rop(t.meta, fc, fz, Q.rec[j][REC_ID], col, dep, win);
                 }
                 continue;
             }
             if (nf + c > 64) {
-                flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep);
+                flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
                 nf = 0;
                 head = tail = -1;
             }
@@ -1296,12 +1332,13 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             }
             nf += c;
         }
-        if (MODE == 0 && nf > 0) flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep);
+        if (MODE == 0 && nf > 0) flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
         wave_sync();
     }
     if (in) {
         F.color[pi] = col;
         F.depth[pi] = dep;
+        if (A.winner) A.winner[pi] = win;
     }
     return frags;
 }
@@ -1318,6 +1355,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     WaveQueueT<PHONG>& Q = Qs[wave];
     const uint32_t tile = (NB == 1) ? blockIdx.x * 4 + wave : blockIdx.x;
     if (tile >= A.numTiles) return;
+    uint32_t* fragOut = A.tileFrags + (size_t)tile * 4 + (NB == 1 ? 0 : wave);
+    const bool lead = (threadIdx.x & 63) == 0;
     uint32_t lo = 0, hi = A.numFrames;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) >> 1;
@@ -1325,10 +1364,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     }
     const DFrame F = A.frames[lo];
     const int32_t segIdx = A.frameSeg[lo];
-    if (segIdx < 0) return;
+    if (segIdx < 0) { if (lead) *fragOut = 0u; return; }
     const DSeg sg = A.segs[segIdx];
     const uint32_t e0 = A.tileBegin[tile], e1 = A.tileEnd[tile];
-    if (e0 == e1 && !sg.clear) return;
+    if (e0 == e1 && !sg.clear) { if (lead) *fragOut = 0u; return; }
     const uint32_t local = tile - F.tileBase;
     const int tx = (int)(local % F.tilesX), ty = (int)(local / F.tilesX);
     uint32_t frags = 0;
@@ -1337,7 +1376,364 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
         frags += raster_block<MODE, PHONG>(A, F, sg, bx0, by0, e0, e1, Q);
     }
-    if ((threadIdx.x & 63) == 0 && frags) atomicAdd(A.fragCounter, (unsigned long long)frags);
+    if (lead) *fragOut = frags;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable shading: the float-domain forward and its adjoint
+//
+// Coverage and the depth-test winner of every pixel come from the integer
+// pipeline above (the raster kernel records 1 + the raster triangle of the
+// last fragment written).  Each pixel's winning fragment is then shaded
+// again without quantisation:
+//   b_k      = w_k * wInvSum (the raster's own barycentrics)
+//   c_k      = sum_j M[k][j] * C_j  (recorded-triangle corner colours C_j:
+//              parameters where the corner has a colour-array index, the
+//              recorded RGBA8 / 255 otherwise), times material diffuse / 255
+//              when lighting is on (triangles.c:88-91)
+//   frag     = sum_k b_k c_k
+//   texture  = sum of texel taps of the reference index maps with weights
+//              (1-fx)(1-fy), fx(1-fy), (1-fx)fy, fx fy; frag *= texel
+//   Phong    = lighting.c:148-258 with exact sqrt, division and pow:
+//              out.rgb = clamp(sum_l amb_l + kd_l ld_l D + ks_l ls_l S, 0, 1),
+//              out.a = 0; amb/kd/ks hold attenuation and spot factors
+// Background pixels take the target's RGBA8 / 255.  The adjoint recomputes
+// the same quantities and scatters dL/dimage into the texel, vertex-colour
+// and light parameters (coverage fixed: edges stay hard).
+// ---------------------------------------------------------------------------
+
+constexpr int DIFF_LIGHT_VALS = 2 * PFF_MAX_LIGHTS * 3;   // diffuse then specular, by light number
+
+struct DiffArgs {
+    const RTri*     rts;
+    const RTriL*    rtl;
+    const RTriD*    rtd;
+    const uint32_t* winner;
+    const uint32_t* color;          // RGBA8 target after the integer pass
+    const uint32_t* srcCol;         // recorded corner colours
+    const uint32_t* vid;            // corner colour-array indices, or null
+    const PFFstate* states;
+    const DTex*     texs;
+    int32_t         paramTex;       // texture whose texels are parameters, -1 for none
+    const float*    texParams;      // [h][w][4]
+    const float*    colParams;      // [numVertices][4], or null
+    uint32_t        numVertices;
+    const float*    lightParams;    // [DIFF_LIGHT_VALS], or null: recorded colours / 255
+    uint32_t        W, H;
+    float*          image;          // forward output [H][W][4]
+    const float*    dImage;         // adjoint input [H][W][4]
+    float*          gTex;           // adjoint outputs, accumulated; each may be null
+    float*          gCol;
+    float*          gLightPart;     // [blocks][DIFF_LIGHT_VALS]
+};
+
+struct Taps {
+    int32_t off[4];
+    float   w[4];
+    int     n;
+};
+
+__device__ __forceinline__ int32_t clamp_off(const DTex& t, int32_t off) {
+    const int64_t n = (int64_t)t.w * t.h;
+    int64_t o = off;
+    return (int32_t)(o < 0 ? 0 : (o >= n ? n - 1 : o));
+}
+
+// Texel offsets and weights of one lookup, with the index maps of
+// sample_tex (sampler.h:202-334) and the same out-of-texture clamp.
+__device__ __forceinline__ void tex_taps(const DTex& t, float u, float v, Taps& tp) {
+    const uint32_t w = t.w;
+    if (t.filter == 0) {
+        int32_t x = wrap_coord(t.wrap, u, t.w), y = wrap_coord(t.wrap, v, t.h);
+        tp.n = 1;
+        tp.off[0] = clamp_off(t, (int32_t)((uint32_t)y * w + (uint32_t)x));
+        tp.w[0] = 1.0f;
+        return;
+    }
+    float tx = __fdiv_rn(1.0f, (float)t.w), ty = __fdiv_rn(1.0f, (float)t.h);
+    int32_t x0 = wrap_coord(t.wrap, u, t.w), y0 = wrap_coord(t.wrap, v, t.h);
+    int32_t x1 = wrap_coord(t.wrap, u + tx, t.w), y1 = wrap_coord(t.wrap, v + ty, t.h);
+    float fx = clamp_ps(u * (float)t.w - (float)x0, 0.0f, 1.0f);
+    float fy = clamp_ps(v * (float)t.h - (float)y0, 0.0f, 1.0f);
+    tp.n = 4;
+    tp.off[0] = clamp_off(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x0));
+    tp.off[1] = clamp_off(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x1));
+    tp.off[2] = clamp_off(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x0));
+    tp.off[3] = clamp_off(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x1));
+    tp.w[0] = (1.0f - fx) * (1.0f - fy);
+    tp.w[1] = fx * (1.0f - fy);
+    tp.w[2] = (1.0f - fx) * fy;
+    tp.w[3] = fx * fy;
+}
+
+__device__ __forceinline__ float4 texel_f(const DiffArgs& A, const DTex& t, bool param, int32_t off) {
+    if (param) return reinterpret_cast<const float4*>(A.texParams)[off];
+    if (t.format == PFF_TEX_RGBA8) {
+        const uint32_t c = ((const uint32_t*)t.pixels)[off];
+        const float k = 1.0f / 255.0f;
+        return make_float4((float)ch(c, 0) * k, (float)ch(c, 1) * k, (float)ch(c, 2) * k, (float)ch(c, 3) * k);
+    }
+    return ((const float4*)t.pixels)[off];
+}
+
+__device__ __forceinline__ float light_param(const DiffArgs& A, const PFFlight& Lt, int which, int c) {
+    if (A.lightParams) return A.lightParams[which * PFF_MAX_LIGHTS * 3 + (Lt.id & 7u) * 3 + c];
+    return (float)ch(which ? Lt.specular : Lt.diffuse, c) * (1.0f / 255.0f);
+}
+
+// Per-light factors at one fragment: amb[c] = att * la/255 * ambient/255 (to
+// multiply D), kd = att * I * max(N.L, 0), ks = att * I * max(N.H, 0)^shininess.
+struct LightTerms {
+    float amb[3];
+    float kd, ks;
+};
+
+__device__ LightTerms light_terms(const PFFlight& Lt, const PFFmaterial& mat, const float* P, const float* N,
+                                  const float* V) {
+    LightTerms r;
+    float L[3], d2 = 0.0f;
+    for (int i = 0; i < 3; i++) { L[i] = Lt.position[i] - P[i]; d2 += L[i] * L[i]; }
+    const float il = 1.0f / sqrtf(fmaxf(d2, 1e-5f));
+    for (int i = 0; i < 3; i++) L[i] *= il;
+    float nl = fmaxf(N[0] * L[0] + N[1] * L[1] + N[2] * L[2], 0.0f);
+    float H[3] = { L[0] + V[0], L[1] + V[1], L[2] + V[2] };
+    float hs = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+    float ih = hs > 0.0f ? 1.0f / sqrtf(hs) : 0.0f;
+    float nh = fmaxf((N[0] * H[0] + N[1] * H[1] + N[2] * H[2]) * ih, 0.0f);
+    float spec = powf(nh, mat.shininess);
+    float I = 1.0f, att = 1.0f;
+    if ((double)Lt.innerCutOff < PF_PI_D) {
+        float theta = -(L[0] * Lt.direction[0] + L[1] * Lt.direction[1] + L[2] * Lt.direction[2]);
+        I = fminf(fmaxf((theta - Lt.outerCutOff) / (Lt.innerCutOff - Lt.outerCutOff), 0.0f), 1.0f);
+    }
+    if (Lt.attLinear != 0.0f || Lt.attQuadratic != 0.0f) {
+        float d0 = Lt.position[0] - P[0], d1 = Lt.position[1] - P[1];
+        float dsq = d0 * d0 + 2.0f * d1 * d1;                 // simd.h:2427-2429 reads y twice
+        att = 1.0f / (Lt.attConstant + Lt.attLinear * sqrtf(dsq) + Lt.attQuadratic * dsq);
+    }
+    for (int c = 0; c < 3; c++)
+        r.amb[c] = att * ((float)ch(Lt.ambient, c) / 255.0f) * ((float)ch(mat.ambient, c) / 255.0f);
+    r.kd = att * I * nl;
+    r.ks = att * I * spec;
+    return r;
+}
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+// One thread per pixel, 16x16 pixels per block.  BWD = false writes the
+// image; BWD = true accumulates the parameter gradients.
+template <bool BWD>
+__global__ __launch_bounds__(256) void k_diff(DiffArgs A) {
+    __shared__ float lg[DIFF_LIGHT_VALS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (BWD) {
+        for (int i = tid; i < DIFF_LIGHT_VALS; i += 256) lg[i] = 0.0f;
+        __syncthreads();
+    }
+    const uint32_t px = blockIdx.x * 16 + (tid & 15), py = blockIdx.y * 16 + (tid >> 4);
+    const bool in = px < A.W && py < A.H;
+    const size_t pix = in ? (size_t)py * A.W + px : 0;
+    const uint32_t win = in ? A.winner[pix] : 0u;
+    const bool cov = win != 0u;
+
+    float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (BWD && cov) {
+        const float4 d = reinterpret_cast<const float4*>(A.dImage)[pix];
+        g[0] = d.x; g[1] = d.y; g[2] = d.z; g[3] = d.w;
+    }
+    // Fragment inputs
+    const PFFstate* st = nullptr;
+    const PFFmaterial* mat = nullptr;
+    float beta[3] = {0.0f, 0.0f, 0.0f}, m[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float cf[4] = {0.0f, 0.0f, 0.0f, 0.0f}, tex[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float N[3] = {0.0f, 0.0f, 0.0f}, P[3] = {0.0f, 0.0f, 0.0f}, V[3] = {0.0f, 0.0f, 0.0f};
+    Taps tp;
+    tp.n = 0;
+    bool texOn = false, texParam = false, phong = false;
+    uint32_t src = 0, nl = 0;
+    DTex T;
+    if (cov) {
+        const uint32_t r = win - 1u;
+        const RTri& t = A.rts[r];
+        const RTriD& d = A.rtd[r];
+        int32_t w1, w2, w3;
+        edge_values(t, (int)px, (int)py, w1, w2, w3);
+        const float b[3] = { (float)w1 * t.wInvSum, (float)w2 * t.wInvSum, (float)w3 * t.wInvSum };
+        st = &A.states[t.state];
+        mat = &st->material[d.face & 1u];
+        src = d.src;
+        for (int j = 0; j < 3; j++) beta[j] = b[0] * d.M[j] + b[1] * d.M[3 + j] + b[2] * d.M[6 + j];
+        if (st->flags & PFF_LIGHTING)
+            for (int c = 0; c < 4; c++) m[c] = (float)ch(mat->diffuse, c) / 255.0f;
+        for (int j = 0; j < 3; j++) {
+            const size_t ci = (size_t)src * 3 + j;
+            const uint32_t vi = A.vid ? A.vid[ci] : PFF_NO_VERTEX;
+            float C[4];
+            if (A.colParams && vi < A.numVertices) {
+                const float4 q = reinterpret_cast<const float4*>(A.colParams)[vi];
+                C[0] = q.x; C[1] = q.y; C[2] = q.z; C[3] = q.w;
+            } else {
+                const uint32_t cc = A.srcCol[ci];
+                for (int c = 0; c < 4; c++) C[c] = (float)ch(cc, c) / 255.0f;
+            }
+            for (int c = 0; c < 4; c++) cf[c] += beta[j] * (C[c] * m[c]);
+        }
+        if (t.meta & META_TEX) {
+            texOn = true;
+            T = A.texs[st->texture];
+            texParam = st->texture == A.paramTex;
+            float u = (t.uv[0] * b[0] + t.uv[2] * b[1]) + t.uv[4] * b[2];
+            float v = (t.uv[1] * b[0] + t.uv[3] * b[1]) + t.uv[5] * b[2];
+            if (t.meta & META_3D) {
+                const float zq = __fdiv_rn(1.0f, (t.z[0] * b[0] + t.z[1] * b[1]) + t.z[2] * b[2]);
+                u = u * zq;
+                v = v * zq;
+            }
+            tex_taps(T, u, v, tp);
+            for (int c = 0; c < 4; c++) tex[c] = 0.0f;
+            for (int k = 0; k < tp.n; k++) {
+                const float4 q = texel_f(A, T, texParam, tp.off[k]);
+                tex[0] += tp.w[k] * q.x; tex[1] += tp.w[k] * q.y; tex[2] += tp.w[k] * q.z; tex[3] += tp.w[k] * q.w;
+            }
+        }
+        if (t.meta & META_PHONG) {
+            phong = true;
+            nl = st->numLights;
+            const RTriL& l = A.rtl[r];
+            for (int i = 0; i < 3; i++) {
+                N[i] = l.n[i] * b[0] + l.n[3 + i] * b[1] + l.n[6 + i] * b[2];
+                P[i] = l.p[i] * b[0] + l.p[3 + i] * b[1] + l.p[6 + i] * b[2];
+                V[i] = st->viewPos[i] - P[i];
+            }
+            const float iv = 1.0f / sqrtf(fmaxf(V[0] * V[0] + V[1] * V[1] + V[2] * V[2], 1e-5f));
+            for (int i = 0; i < 3; i++) V[i] *= iv;
+        }
+    }
+    float frag[4];
+    for (int c = 0; c < 4; c++) frag[c] = cf[c] * tex[c];
+    float dfrag[4] = { g[0], g[1], g[2], g[3] };
+    float gcs[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
+    if (phong) {
+        float D[3], acc[3] = {0.0f, 0.0f, 0.0f};
+        for (int c = 0; c < 3; c++) { D[c] = frag[c]; S[c] = (float)ch(mat->specular, c) / 255.0f; }
+        for (uint32_t li = 0; li < nl; li++) {
+            const PFFlight& Lt = st->lights[li];
+            const LightTerms lt = light_terms(Lt, *mat, P, N, V);
+            for (int c = 0; c < 3; c++)
+                acc[c] += lt.amb[c] * D[c] + lt.kd * light_param(A, Lt, 0, c) * D[c] + lt.ks * light_param(A, Lt, 1, c) * S[c];
+        }
+        for (int c = 0; c < 3; c++) out[c] = fminf(fmaxf(acc[c], 0.0f), 1.0f);
+        out[3] = 0.0f;
+        if (BWD) {
+            float dD[3] = {0.0f, 0.0f, 0.0f};
+            for (int c = 0; c < 3; c++) gcs[c] = (acc[c] > 0.0f && acc[c] < 1.0f) ? g[c] : 0.0f;
+            for (uint32_t li = 0; li < nl; li++) {
+                const PFFlight& Lt = st->lights[li];
+                const LightTerms lt = light_terms(Lt, *mat, P, N, V);
+                for (int c = 0; c < 3; c++) dD[c] += gcs[c] * (lt.amb[c] + lt.kd * light_param(A, Lt, 0, c));
+            }
+            for (int c = 0; c < 3; c++) dfrag[c] = dD[c];
+            dfrag[3] = 0.0f;
+        }
+    } else {
+        for (int c = 0; c < 4; c++) out[c] = frag[c];
+    }
+    if (!BWD) {
+        if (in) {
+            if (!cov) {
+                const uint32_t cc = A.color[pix];
+                for (int c = 0; c < 4; c++) out[c] = (float)ch(cc, c) / 255.0f;
+            }
+            reinterpret_cast<float4*>(A.image)[pix] = make_float4(out[0], out[1], out[2], out[3]);
+        }
+        return;
+    }
+    // Light parameters: per light slot, a wave sum per light number, then
+    // one LDS add per wave and value.
+    const bool lit = phong && A.gLightPart;
+    uint32_t maxL = lit ? nl : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) maxL = max(maxL, (uint32_t)__shfl_xor((int)maxL, d, 64));
+    maxL = __builtin_amdgcn_readfirstlane(maxL);
+    for (uint32_t li = 0; li < maxL; li++) {
+        const bool has = lit && li < nl;
+        uint32_t id = 0;
+        float v[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (has) {
+            const PFFlight& Lt = st->lights[li];
+            id = Lt.id & 7u;
+            const LightTerms lt = light_terms(Lt, *mat, P, N, V);
+            for (int c = 0; c < 3; c++) {
+                v[c] = gcs[c] * lt.kd * frag[c];
+                v[3 + c] = gcs[c] * lt.ks * S[c];
+            }
+        }
+        uint64_t pend = __builtin_amdgcn_ballot_w64(has);
+        while (pend) {
+            const uint32_t lead = __builtin_amdgcn_readlane(id, __builtin_ctzll(pend));
+            const bool mine = has && id == lead;
+            for (int c = 0; c < 6; c++) {
+                const float x = wave_sum(mine ? v[c] : 0.0f);
+                if (lane == 0) atomicAdd(&lg[(c < 3 ? 0 : PFF_MAX_LIGHTS * 3) + lead * 3 + (c % 3)], x);
+            }
+            pend &= ~__builtin_amdgcn_ballot_w64(mine);
+        }
+    }
+    // Texels and vertex colours
+    if (cov) {
+        float dcf[4];
+        for (int c = 0; c < 4; c++) dcf[c] = dfrag[c] * tex[c];
+        if (texOn && texParam && A.gTex) {
+            for (int k = 0; k < tp.n; k++) {
+                float* gp = A.gTex + (size_t)tp.off[k] * 4;
+                for (int c = 0; c < 4; c++) {
+                    const float x = tp.w[k] * dfrag[c] * cf[c];
+                    if (x != 0.0f) unsafeAtomicAdd(gp + c, x);
+                }
+            }
+        }
+        if (A.colParams && A.gCol && A.vid) {
+            for (int j = 0; j < 3; j++) {
+                const uint32_t vi = A.vid[(size_t)src * 3 + j];
+                if (vi >= A.numVertices) continue;
+                float* gp = A.gCol + (size_t)vi * 4;
+                for (int c = 0; c < 4; c++) {
+                    const float x = beta[j] * m[c] * dcf[c];
+                    if (x != 0.0f) unsafeAtomicAdd(gp + c, x);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (A.gLightPart) {
+        const uint32_t blk = blockIdx.y * gridDim.x + blockIdx.x;
+        for (int i = tid; i < DIFF_LIGHT_VALS; i += 256) A.gLightPart[(size_t)blk * DIFF_LIGHT_VALS + i] = lg[i];
+    }
+}
+
+// Sums the per-block light partials in a fixed order.
+__global__ __launch_bounds__(256) void k_diff_light_sum(const float* __restrict__ part, uint32_t blocks,
+                                                        float* __restrict__ out) {
+    __shared__ float red[256];
+    const uint32_t i = blockIdx.x;          // one block per value
+    float acc = 0.0f;
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256) acc += part[(size_t)b * DIFF_LIGHT_VALS + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1399,7 +1795,11 @@ struct PassInfo {
 
 struct PFDjob {
     hipStream_t stream = nullptr;
-    std::vector<PFFframe> src;            // host frame descriptors (targets)
+    std::vector<PFFframe> src;            // host frame descriptors: only the target fields are
+                                          // used after creation (the rest may be reused by the caller)
+    std::vector<uint8_t>  leadClear;      // per frame: the first command is a clear
+    std::vector<PFFtexture> srcTex;       // frame 0's textures and states (differentiable jobs)
+    std::vector<PFFstate>   srcStates;
     std::vector<DFrame>   frames;
     DevBuf dFrames;
     std::vector<DevBuf> targets;          // colour + depth per frame
@@ -1413,13 +1813,18 @@ struct PFDjob {
     uint32_t numTiles = 0;
     // scratch
     DevBuf cnt, off, blockRT, blockPairs, rts, rtl, keysA, valsA, keysB, valsB;
-    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, fragCounter;
+    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags;
     uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     // stats
     uint64_t lastFragments = 0, totalRT = 0, totalPairs = 0;
+    // differentiable jobs
+    bool diff = false, diffRan = false;
+    DevBuf rtd, winner, vid, texParams, colParams, lightParams, image, dImage, gTex, gCol, gLightPart, gLight;
+    uint32_t numVertices = 0;
+    int32_t paramTex = -1;
 };
 
 namespace {
@@ -1468,7 +1873,15 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     const uint32_t nb = grid_for(n, 256);
     if ((rc = ensure(J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
     if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
-    if ((rc = ensure(J->scanTotal, 16)) || (rc = ensure(J->fragCounter, 16))) return rc;
+    if ((rc = ensure(J->scanTotal, 16))) return rc;
+    const size_t fragSlots = (size_t)J->numTiles * 4;
+    const size_t passIdx = (size_t)(&P - J->passes.data());
+    if (discover) {
+        // Slots no wave writes (TS = 8 uses one of four) must read as zero.
+        if ((rc = ensure(J->tileFrags, fragSlots * J->passes.size() * 4 + 4))) return rc;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(fragSlots, 256 * 8)), dim3(256), 0, J->stream,
+                           (uint32_t*)J->tileFrags.p + passIdx * fragSlots, (uint32_t)fragSlots, 0u);
+    }
     if ((rc = ensure(J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J->tileEnd, (size_t)J->numTiles * 4 + 4)))
         return rc;
     if (!discover) {
@@ -1510,11 +1923,13 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     const uint32_t numRT = P.numRT, numPairs = P.numPairs;
     if ((rc = ensure(J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
     if (J->anyPhong && (rc = ensure(J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
+    if (J->diff && (rc = ensure(J->rtd, (size_t)numRT * sizeof(RTriD) + sizeof(RTriD)))) return rc;
     const size_t pb = (size_t)numPairs * 4 + 4;
     if ((rc = ensure(J->keysA, pb)) || (rc = ensure(J->valsA, pb)) || (rc = ensure(J->keysB, pb)) || (rc = ensure(J->valsB, pb)))
         return rc;
     sa.rts = (RTri*)J->rts.p;
     sa.rtl = (RTriL*)J->rtl.p;
+    sa.rtd = J->diff ? (RTriD*)J->rtd.p : nullptr;
     sa.keys = (uint32_t*)J->keysA.p;
     sa.vals = (uint32_t*)J->valsA.p;
     sa.capRT = numRT;
@@ -1575,7 +1990,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.vals = vals;
     ra.numFrames = (uint32_t)J->frames.size();
     ra.numTiles = J->numTiles;
-    ra.fragCounter = (unsigned long long*)J->fragCounter.p;
+    ra.tileFrags = (uint32_t*)J->tileFrags.p + passIdx * fragSlots;
+    ra.winner = J->diff ? (uint32_t*)J->winner.p : nullptr;
     if (J->cfg.rasterMode == 1) {
         switch (J->tileShift) {
         case 3: launch_raster<8, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
@@ -1597,17 +2013,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     return 0;
 }
 
-int reset_counters(PFDjob* J) {
-    int rc = ensure(J->fragCounter, 16);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->fragCounter.p, 4u, 0u);
-    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
-    return 0;
-}
-
 int run_all(PFDjob* J, bool discover) {
-    int rc = reset_counters(J);
-    if (rc) return rc;
+    int rc = 0;
     for (auto& P : J->passes)
         if ((rc = run_pass(J, P, discover || !P.sized, nullptr))) return rc;
     return 0;
@@ -1706,6 +2113,11 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     std::vector<DTex> htex;
     uint32_t triBase = 0, stateBase = 0;
     J->src.assign(frames, frames + count);
+    J->leadClear.resize(count);
+    for (uint32_t f = 0; f < count; f++)
+        J->leadClear[f] = frames[f].numCmds > 0 && frames[f].cmds[0].kind == PFF_CMD_CLEAR;
+    J->srcTex.assign(frames[0].textures, frames[0].textures + frames[0].numTextures);
+    J->srcStates.assign(frames[0].states, frames[0].states + frames[0].numStates);
     J->frames.resize(count);
     J->targets.resize(2 * count);
     for (uint32_t f = 0; f < count; f++) {
@@ -1861,7 +2273,7 @@ int pfdJobUploadTargets(PFDjob* J) {
     for (size_t f = 0; f < J->frames.size(); f++) {
         const PFFframe& fr = J->src[f];
         size_t npx = (size_t)fr.width * fr.height;
-        bool leadClear = fr.numCmds > 0 && fr.cmds[0].kind == PFF_CMD_CLEAR;
+        bool leadClear = J->leadClear[f] != 0;
         size_t n = leadClear ? (npx < 8 ? npx : 8) : npx;
         HIP_TRY(hipMemcpyAsync(J->frames[f].color, fr.color, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipMemcpyAsync(J->frames[f].depth, fr.depth, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
@@ -1913,9 +2325,8 @@ int pfdJobRunTimed(PFDjob* J, PFDtimes* t) {
     memset(t, 0, sizeof *t);
     hipEvent_t ev[4];
     for (int i = 0; i < 4; i++) HIP_TRY(hipEventCreate(&ev[i]), PFD_E_LAUNCH);
-    int rc = reset_counters(J);
+    int rc = 0;
     for (auto& P : J->passes) {
-        if (rc) break;
         rc = run_pass(J, P, !P.sized, ev);
         if (rc) break;
         HIP_TRY(hipEventSynchronize(ev[3]), PFD_E_LAUNCH);
@@ -1955,9 +2366,12 @@ int pfdJobStats(PFDjob* J, PFDstats* st) {
     if (!J || !st) return PFD_E_INVALID;
     memset(st, 0, sizeof *st);
     unsigned long long frags = 0;
-    if (J->fragCounter.p) {
+    if (J->tileFrags.p) {
+        const size_t n = (size_t)J->numTiles * 4 * J->passes.size();
+        std::vector<uint32_t> tf(n);
         HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
-        HIP_TRY(hipMemcpy(&frags, J->fragCounter.p, 8, hipMemcpyDeviceToHost), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpy(tf.data(), J->tileFrags.p, n * 4, hipMemcpyDeviceToHost), PFD_E_LAUNCH);
+        for (uint32_t v : tf) frags += v;
     }
     st->fragments = frags;
     for (auto& P : J->passes) { st->rasterTriangles += P.numRT; st->tilePairs += P.numPairs; }
@@ -2018,7 +2432,9 @@ void pfdJobDestroy(PFDjob* J) {
     drop_graph(J);
     DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
                        &J->rts, &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
-                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->fragCounter };
+                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags,
+                       &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams, &J->lightParams, &J->image,
+                       &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
     for (DevBuf* b : bufs) release(*b);
     for (auto& b : J->targets) release(b);
     for (auto& b : J->texData) release(b);
@@ -2051,10 +2467,252 @@ int pfdCaptureExecutor(const PFFframe* frame, void* user) {
     return PFD_OK;
 }
 
+int pfdCaptureBatchExecutor(const PFFframe* frames, uint32_t count, void* user) {
+    (void)user;
+    PFDjob* J = nullptr;
+    int rc = pfdJobCreate(frames, count, nullptr, &J);
+    if (rc) return rc;
+    if (g_captured) pfdJobDestroy(g_captured);
+    g_captured = J;
+    return PFD_OK;
+}
+
 PFDjob* pfdTakeCapturedJob(void) {
     PFDjob* J = g_captured;
     g_captured = nullptr;
     return J;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable jobs
+// ---------------------------------------------------------------------------
+
+int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
+    if (!frame || !out) return PFD_E_INVALID;
+    *out = nullptr;
+    // One pass: at most one clear, before every draw.
+    for (uint32_t c = 0; c < frame->numCmds; c++)
+        if (frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) {
+            snprintf(g_err, sizeof g_err, "differentiable frame: a clear after the first command");
+            return PFD_E_INVALID;
+        }
+    for (uint32_t i = 0; i < frame->numStates; i++) {
+        const PFFstate& st = frame->states[i];
+        const char* why = nullptr;
+        if (st.flags & PFF_BLEND) why = "blending";
+        else if (st.shadeModel != 1) why = "flat shading";
+        else if ((st.flags & PFF_LIGHTING) && st.lightMode != 1) why = "Gouraud lighting";
+        if (why) {
+            snprintf(g_err, sizeof g_err, "differentiable frame: %s is not differentiable here", why);
+            return PFD_E_INVALID;
+        }
+    }
+    PFDjob* J = nullptr;
+    int rc = pfdJobCreate(frame, 1, nullptr, &J);
+    if (rc) return rc;
+    J->diff = true;
+    const size_t npx = (size_t)frame->width * frame->height;
+    if ((rc = ensure(J->winner, npx * 4)) || (rc = ensure(J->image, npx * 16)) || (rc = ensure(J->dImage, npx * 16))) {
+        pfdJobDestroy(J);
+        return rc;
+    }
+    if (frame->tris.vertexId && frame->tris.count) {
+        const size_t bytes = (size_t)frame->tris.count * 12;
+        if ((rc = ensure(J->vid, bytes))) { pfdJobDestroy(J); return rc; }
+        hipError_t e = hipMemcpy(J->vid.p, frame->tris.vertexId, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { set_error("vertex ids", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+    }
+    *out = J;
+    return PFD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Uploads the parameters (each call: they change between optimisation
+// steps) and fills the kernel arguments.
+int diff_args(PFDjob* J, const PFDdiffParams* p, DiffArgs& a) {
+    if (!J || !J->diff || !p) { snprintf(g_err, sizeof g_err, "not a differentiable job"); return PFD_E_INVALID; }
+    int rc;
+    memset(&a, 0, sizeof a);
+    a.paramTex = -1;
+    if (p->texture >= 0) {
+        if ((uint32_t)p->texture >= J->srcTex.size()) { snprintf(g_err, sizeof g_err, "bad parameter texture"); return PFD_E_INVALID; }
+        const PFFtexture& T = J->srcTex[p->texture];
+        const float* texels = p->texels;
+        if (!texels) {
+            if (T.format != PFF_TEX_RGBAF32) {
+                snprintf(g_err, sizeof g_err, "parameter texture must be RGBA float or come with texels");
+                return PFD_E_INVALID;
+            }
+            texels = (const float*)T.pixels;
+        }
+        const size_t bytes = (size_t)T.width * T.height * 16;
+        if ((rc = ensure(J->texParams, bytes))) return rc;
+        HIP_TRY(hipMemcpyAsync(J->texParams.p, texels, bytes, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        a.paramTex = p->texture;
+        a.texParams = (const float*)J->texParams.p;
+    }
+    if (p->vertexColors && p->numVertices) {
+        const size_t bytes = (size_t)p->numVertices * 16;
+        if ((rc = ensure(J->colParams, bytes))) return rc;
+        HIP_TRY(hipMemcpyAsync(J->colParams.p, p->vertexColors, bytes, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        a.colParams = (const float*)J->colParams.p;
+        a.numVertices = p->numVertices;
+    }
+    J->numVertices = a.numVertices;
+    J->paramTex = a.paramTex;
+    if (p->lightDiffuse || p->lightSpecular) {
+        // Recorded colours / 255 for whichever of the two is not given.
+        float lp[DIFF_LIGHT_VALS];
+        for (int id = 0; id < PFF_MAX_LIGHTS; id++)
+            for (int c = 0; c < 3; c++) { lp[id * 3 + c] = 0.0f; lp[PFF_MAX_LIGHTS * 3 + id * 3 + c] = 0.0f; }
+        for (const PFFstate& st : J->srcStates)
+            for (uint32_t l = 0; l < st.numLights; l++) {
+                const PFFlight& L = st.lights[l];
+                for (int c = 0; c < 3; c++) {
+                    lp[(L.id & 7u) * 3 + c] = (float)((L.diffuse >> (8 * c)) & 0xFFu) / 255.0f;
+                    lp[PFF_MAX_LIGHTS * 3 + (L.id & 7u) * 3 + c] = (float)((L.specular >> (8 * c)) & 0xFFu) / 255.0f;
+                }
+            }
+        if (p->lightDiffuse) memcpy(lp, p->lightDiffuse, PFF_MAX_LIGHTS * 3 * sizeof(float));
+        if (p->lightSpecular) memcpy(lp + PFF_MAX_LIGHTS * 3, p->lightSpecular, PFF_MAX_LIGHTS * 3 * sizeof(float));
+        if ((rc = ensure(J->lightParams, sizeof lp))) return rc;
+        HIP_TRY(hipMemcpyAsync(J->lightParams.p, lp, sizeof lp, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);   // lp is on this stack frame
+        a.lightParams = (const float*)J->lightParams.p;
+    }
+    a.rts = (const RTri*)J->rts.p;
+    a.rtl = (const RTriL*)J->rtl.p;
+    a.rtd = (const RTriD*)J->rtd.p;
+    a.winner = (const uint32_t*)J->winner.p;
+    a.color = J->frames[0].color;
+    a.srcCol = (const uint32_t*)J->col.p;
+    a.vid = J->vid.p ? (const uint32_t*)J->vid.p : nullptr;
+    a.states = (const PFFstate*)J->states.p;
+    a.texs = (const DTex*)J->texTable.p;
+    a.W = J->frames[0].W;
+    a.H = J->frames[0].H;
+    a.image = (float*)J->image.p;
+    a.dImage = (const float*)J->dImage.p;
+    return PFD_OK;
+}
+
+dim3 diff_grid(const DiffArgs& a) { return dim3((a.W + 15) / 16, (a.H + 15) / 16); }
+
+// The integer pass that fixes coverage and winners; runs once per job.
+int diff_raster(PFDjob* J) {
+    if (J->diffRan) return PFD_OK;
+    int rc;
+    if ((rc = pfdJobUploadTargets(J)) || (rc = run_all(J, true))) return rc;
+    if ((rc = pfdJobDownload(J))) return rc;
+    J->diffRan = true;
+    return PFD_OK;
+}
+
+int diff_backward_launch(PFDjob* J, DiffArgs& a, bool wantTex, bool wantCol, bool wantLight) {
+    int rc;
+    const dim3 grid = diff_grid(a);
+    const uint32_t blocks = grid.x * grid.y;
+    if (wantTex && a.paramTex >= 0) {
+        const PFFtexture& T = J->srcTex[a.paramTex];
+        const size_t n = (size_t)T.width * T.height * 4;
+        if ((rc = ensure(J->gTex, n * 4))) return rc;
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256 * 8)), dim3(256), 0, J->stream, (float*)J->gTex.p, n);
+        a.gTex = (float*)J->gTex.p;
+    }
+    if (wantCol && a.colParams) {
+        const size_t n = (size_t)a.numVertices * 4;
+        if ((rc = ensure(J->gCol, n * 4))) return rc;
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256 * 8)), dim3(256), 0, J->stream, (float*)J->gCol.p, n);
+        a.gCol = (float*)J->gCol.p;
+    }
+    if (wantLight) {
+        if ((rc = ensure(J->gLightPart, (size_t)blocks * DIFF_LIGHT_VALS * 4)) || (rc = ensure(J->gLight, DIFF_LIGHT_VALS * 4)))
+            return rc;
+        a.gLightPart = (float*)J->gLightPart.p;
+    }
+    hipLaunchKernelGGL(k_diff<true>, grid, dim3(256), 0, J->stream, a);
+    if (wantLight)
+        hipLaunchKernelGGL(k_diff_light_sum, dim3(DIFF_LIGHT_VALS), dim3(256), 0, J->stream,
+                           (const float*)J->gLightPart.p, blocks, (float*)J->gLight.p);
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfdDiffForward(PFDjob* J, const PFDdiffParams* p, float* image) {
+    DiffArgs a;
+    int rc;
+    if ((rc = diff_raster(J)) || (rc = diff_args(J, p, a))) return rc;
+    hipLaunchKernelGGL(k_diff<false>, diff_grid(a), dim3(256), 0, J->stream, a);
+    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+    if (image)
+        HIP_TRY(hipMemcpyAsync(image, J->image.p, (size_t)a.W * a.H * 16, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
+}
+
+int pfdDiffBackward(PFDjob* J, const PFDdiffParams* p, const float* dImage, PFDdiffGrads* g) {
+    if (!dImage || !g) return PFD_E_INVALID;
+    DiffArgs a;
+    int rc;
+    if ((rc = diff_raster(J)) || (rc = diff_args(J, p, a))) return rc;
+    HIP_TRY(hipMemcpyAsync(J->dImage.p, dImage, (size_t)a.W * a.H * 16, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+    const bool wantLight = g->lightDiffuse || g->lightSpecular;
+    if ((rc = diff_backward_launch(J, a, g->texels != nullptr, g->vertexColors != nullptr, wantLight))) return rc;
+    if (g->texels && a.gTex) {
+        const PFFtexture& T = J->srcTex[a.paramTex];
+        HIP_TRY(hipMemcpyAsync(g->texels, a.gTex, (size_t)T.width * T.height * 16, hipMemcpyDeviceToHost, J->stream),
+                PFD_E_LAUNCH);
+    }
+    if (g->vertexColors && a.gCol)
+        HIP_TRY(hipMemcpyAsync(g->vertexColors, a.gCol, (size_t)a.numVertices * 16, hipMemcpyDeviceToHost, J->stream),
+                PFD_E_LAUNCH);
+    float lg[DIFF_LIGHT_VALS];
+    if (wantLight)
+        HIP_TRY(hipMemcpyAsync(lg, J->gLight.p, sizeof lg, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    if (g->lightDiffuse) memcpy(g->lightDiffuse, lg, PFF_MAX_LIGHTS * 3 * sizeof(float));
+    if (g->lightSpecular) memcpy(g->lightSpecular, lg + PFF_MAX_LIGHTS * 3, PFF_MAX_LIGHTS * 3 * sizeof(float));
+    return PFD_OK;
+}
+
+int pfdDiffTime(PFDjob* J, const PFDdiffParams* p, uint32_t iters, PFDdiffTimes* t) {
+    if (!t || iters == 0) return PFD_E_INVALID;
+    DiffArgs a;
+    int rc;
+    if ((rc = diff_raster(J)) || (rc = diff_args(J, p, a))) return rc;
+    // dL/dimage: leave whatever the device buffer holds (timing only).
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e), PFD_E_LAUNCH);
+    float raster = 0.0f, fwd = 0.0f, bwd = 0.0f;
+    for (uint32_t i = 0; i < iters && rc == 0; i++) {
+        (void)hipEventRecord(ev[0], J->stream);
+        rc = pfdJobLaunch(J);
+        (void)hipEventRecord(ev[1], J->stream);
+        hipLaunchKernelGGL(k_diff<false>, diff_grid(a), dim3(256), 0, J->stream, a);
+        (void)hipEventRecord(ev[2], J->stream);
+        DiffArgs b = a;
+        if (!rc) rc = diff_backward_launch(J, b, a.paramTex >= 0, a.colParams != nullptr, true);
+        (void)hipEventRecord(ev[3], J->stream);
+        if (hipEventSynchronize(ev[3]) != hipSuccess) rc = PFD_E_LAUNCH;
+        float m0 = 0, m1 = 0, m2 = 0;
+        (void)hipEventElapsedTime(&m0, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&m1, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&m2, ev[2], ev[3]);
+        raster += m0; fwd += m1; bwd += m2;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    t->rasterMs = raster / iters;
+    t->forwardMs = fwd / iters;
+    t->backwardMs = bwd / iters;
+    return PFD_OK;
 }
 
 }  // extern "C"

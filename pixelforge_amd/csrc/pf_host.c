@@ -130,11 +130,17 @@ typedef struct {
     int      stateDirty;
     uint32_t texGen;
     uint32_t frames;
+
+    PFDjob*    diffJob;                /* last frame rendered by pfDiffRender */
+    const Tex** diffTex;               /* its textures, by frame texture index */
+    uint32_t   diffNumTex;
 } Ctx;
 
 static __thread Ctx* G = NULL;        /* current context, one per thread */
 static PFXexecutor g_exec = NULL;
 static void* g_execUser = NULL;
+static PFXbatchExecutor g_batchExec = NULL;
+static void* g_batchUser = NULL;
 static uint32_t g_texGen = 1;          /* bumped by pfTextureParameter */
 
 /* ------------------------------------------------------------------------ */
@@ -755,6 +761,8 @@ void pfDeleteContext(PFcontext ctx)
     Ctx* c = (Ctx*)ctx;
     if (!c) return;
     submit(c);
+    pfdJobDestroy(c->diffJob);
+    free(c->diffTex);
     free(c->mainFB.zbuffer);
     free(c->mainFB.texture);
     rec_free(&c->rec);
@@ -1812,7 +1820,9 @@ void pfSubmitBatch(const PFcontext* contexts, PFsizei count)
     }
     int rc = 0;
     if (n) {
-        if (g_exec) {
+        if (g_batchExec) {
+            rc = g_batchExec(frames, n, g_batchUser);
+        } else if (g_exec) {
             for (uint32_t i = 0; i < n && rc == 0; i++) rc = g_exec(&frames[i], g_execUser);
         } else {
             rc = pfdExecuteFrames(frames, n);
@@ -1836,4 +1846,98 @@ void pfxSetExecutor(PFXexecutor fn, void* user)
     g_execUser = user;
 }
 
+void pfxSetBatchExecutor(PFXbatchExecutor fn, void* user)
+{
+    g_batchExec = fn;
+    g_batchUser = user;
+}
+
 PFuint pfxFrameCount(void) { return G ? G->frames : 0; }
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable rendering                                                  */
+/* ------------------------------------------------------------------------ */
+
+static int diff_params(Ctx* c, const PFdiffparams* p, PFDdiffParams* d)
+{
+    memset(d, 0, sizeof *d);
+    d->texture = -1;
+    if (!p) return 1;
+    if (p->texture) {
+        for (uint32_t i = 0; i < c->diffNumTex; i++)
+            if (c->diffTex[i] == (const Tex*)p->texture) d->texture = (int32_t)i;
+        if (d->texture < 0) return 0;          /* not used by the frame */
+    }
+    d->texels = p->texels;
+    d->numVertices = p->numVertices;
+    d->vertexColors = p->vertexColors;
+    d->lightDiffuse = p->lightDiffuse;
+    d->lightSpecular = p->lightSpecular;
+    return 1;
+}
+
+static void diff_fail(Ctx* c, int rc, const char* what)
+{
+    c->err = (rc == PFD_E_NOMEM) ? PF_ERROR_OUT_OF_MEMORY : PF_INVALID_OPERATION;
+    fprintf(stderr, "pixelforge_amd: %s failed (%d): %s\n", what, rc, pfdLastError());
+}
+
+void pfDiffRender(const PFdiffparams* params, PFfloat* image)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (c->rec.ncmd) {
+        PFFframe f;
+        build_frame(c, &f);
+        pfdJobDestroy(c->diffJob);
+        c->diffJob = NULL;
+        free(c->diffTex);
+        c->diffTex = NULL;
+        c->diffNumTex = 0;
+        int rc = pfdDiffCreate(&f, &c->diffJob);
+        if (rc == 0) {
+            c->diffTex = (const Tex**)malloc((c->rec.ntex ? c->rec.ntex : 1) * sizeof(Tex*));
+            if (c->diffTex) {
+                memcpy(c->diffTex, c->rec.texKeys, c->rec.ntex * sizeof(Tex*));
+                c->diffNumTex = (uint32_t)c->rec.ntex;
+            }
+        }
+        c->frames++;
+        rec_reset(&c->rec);
+        c->stateDirty = 1;
+        if (rc) { diff_fail(c, rc, "pfDiffRender"); return; }
+    }
+    if (!c->diffJob) { c->err = PF_INVALID_OPERATION; return; }
+    PFDdiffParams d;
+    if (!diff_params(c, params, &d)) { c->err = PF_INVALID_VALUE; return; }
+    int rc = pfdDiffForward(c->diffJob, &d, image);
+    if (rc) diff_fail(c, rc, "pfDiffRender");
+}
+
+void pfDiffBackward(const PFdiffparams* params, const PFfloat* dImage, PFdiffgrads* grads)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!c->diffJob || !dImage || !grads) { c->err = PF_INVALID_OPERATION; return; }
+    PFDdiffParams d;
+    if (!diff_params(c, params, &d)) { c->err = PF_INVALID_VALUE; return; }
+    PFDdiffGrads g = { grads->texels, grads->vertexColors, grads->lightDiffuse, grads->lightSpecular };
+    int rc = pfdDiffBackward(c->diffJob, &d, dImage, &g);
+    if (rc) diff_fail(c, rc, "pfDiffBackward");
+}
+
+/* Device-resident timing of the last pfDiffRender frame (for benchmarks). */
+int pfxDiffTime(const PFdiffparams* params, PFuint iters, float* rasterMs, float* forwardMs, float* backwardMs)
+{
+    Ctx* c = G;
+    if (!c || !c->diffJob) return PFD_E_INVALID;
+    PFDdiffParams d;
+    if (!diff_params(c, params, &d)) return PFD_E_INVALID;
+    PFDdiffTimes t;
+    int rc = pfdDiffTime(c->diffJob, &d, iters, &t);
+    if (rc) return rc;
+    *rasterMs = t.rasterMs;
+    *forwardMs = t.forwardMs;
+    *backwardMs = t.backwardMs;
+    return 0;
+}

@@ -1,0 +1,51 @@
+"""Batched contexts on the GPU: several contexts of uneven size, triangle
+count and texture, half of them additively blended, rendered in the same
+launches (pfSubmitBatch), must each equal their single-context render and
+the oracle bit for bit, at every tile size and raster mode."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pixelforge_amd as pf
+from conftest import render_gpu, render_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_equals_single_context_renders():
+    params = pf.batch_workload(24, seed=7, max_side=256, max_tris=3000, max_tex=128)
+    imgs, err = pf.render_batch(params)
+    assert err == 0
+    for i, (p, img) in enumerate(zip(params, imgs)):
+        single, _, e = render_gpu(p)
+        assert e == 0
+        assert np.array_equal(img, single), f"context {i} ({p.width}x{p.height}) differs from its single render"
+        ref, _, e, frags = render_oracle(p)
+        assert frags > 0
+        assert np.array_equal(img, ref), f"context {i} differs from the oracle"
+
+
+@pytest.mark.parametrize("tile_shift", [3, 4, 5, 6])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_job_every_tile_size(tile_shift, mode):
+    params = pf.batch_workload(10, seed=8, max_side=200, max_tris=2000, max_tex=96)
+    ref = [render_oracle(p)[0] for p in params]
+    cap = pf.capture_batch(params)
+    lib = pf.lib()
+    try:
+        cfg = pf.PFDconfig()
+        cfg.tileShift, cfg.rasterMode = tile_shift, mode
+        assert lib.pfdJobSetConfig(cap.job, C.byref(cfg)) == 0
+        for b in cap.buffers:
+            b[:] = 0
+        assert lib.pfdJobUploadTargets(cap.job) == 0
+        for _ in range(2):                              # sizing run, then the graph
+            assert lib.pfdJobLaunch(cap.job) == 0
+        assert lib.pfdJobSync(cap.job) == 0
+        assert lib.pfdJobDownload(cap.job) == 0
+        for i, (p, r) in enumerate(zip(params, ref)):
+            got = cap.buffers[i][: p.width * p.height].reshape(p.height, p.width)
+            assert np.array_equal(got, r), f"context {i} at tile {1 << tile_shift}, mode {mode}"
+    finally:
+        cap.close()
