@@ -34,10 +34,11 @@ extern "C" {
 typedef struct PFDjob PFDjob;
 
 /* Tile configuration of a job.  tileShift: log2 of the square screen tile
- * one workgroup rasterises (3..6 -> 8x8 .. 64x64) -- also the binning
- * granularity.  rasterMode: 0 packs covered fragments of several triangles
- * into full wavefronts before shading; 1 shades each triangle in place on
- * the lanes it covers (better when triangles cover whole 8x8 blocks).
+ * that is binned and rasterised by one workgroup (3..6 -> 8x8 .. 64x64).
+ * rasterMode: 0 packs the covered fragments of several triangles into full
+ * wavefronts before shading; 1 shades each triangle in place on the lanes
+ * it covers; 2 and 3 are 0 and 1 with four pixels per lane (a wave renders
+ * 16x16 pixels per walk of the tile list; tiles of 8x8 fall back to 0/1).
  * Zero fields take defaults. */
 typedef struct {
     uint32_t tileShift;

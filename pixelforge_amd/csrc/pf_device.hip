@@ -58,6 +58,17 @@ struct DFrame {
     uint32_t  tilesX, tilesY, tileBase, pad;
 };
 
+// What the raster kernel needs of a tile's frame in one pass: the frame
+// and its segment's clear, in one record (one scalar load per wave).
+struct DTileFrame {
+    DFrame   F;
+    uint32_t active;        // the frame has a segment in this pass
+    uint32_t clear;
+    uint32_t clearColor;
+    float    clearDepth;
+    uint32_t pad[2];
+};
+
 struct DSeg {               // the part of one frame a pass renders
     uint32_t frame;
     uint32_t triFirst, triLast;   // global recorded-triangle range
@@ -80,7 +91,8 @@ struct __attribute__((aligned(16))) RTri {
     uint32_t meta;          // see META_* below
     uint32_t state;         // global state index
     float    uv[6];
-    uint32_t pad[3];
+    uint32_t tex[3];        // texture of META_TEX triangles: address bits 0-31; bits 32-47 |
+                            // format << 16 | wrap << 17 | filter << 19; width | height << 16
 };
 static_assert(sizeof(RTri) == 128, "RTri layout");
 static_assert(offsetof(RTri, xMin) == 36 && offsetof(RTri, yMax) == 48, "bbox read as dwords 9..12");
@@ -543,50 +555,56 @@ __device__ __forceinline__ void to_screen(const float* vp, Vtx& v) {
 }
 
 // Sutherland-Hodgman against w >= eps then +-x, +-y, +-z <= w
-// (triangles.c:157-244).  poly holds up to 12 vertices.
-__device__ __noinline__ int clip_polygon(Vtx* poly, int n) {
-    Vtx in[12];
-    // W plane
-    for (int i = 0; i < n; i++) in[i] = poly[i];
-    int cnt = n;
-    n = 0;
-    {
-        int prev = cnt - 1;
-        int pd = (in[prev].hom[3] < 1e-5f) ? -1 : 1;
-        for (int i = 0; i < cnt; i++) {
-            int cd = (in[i].hom[3] < 1e-5f) ? -1 : 1;
-            if (pd * cd < 0 && n < 12)
-                poly[n++] = lerp_vtx(in[prev], in[i], __fdiv_rn(1e-5f - in[prev].hom[3], in[i].hom[3] - in[prev].hom[3]));
-            if (cd > 0 && n < 12) poly[n++] = in[i];
-            pd = cd;
-            prev = i;
+// (triangles.c:157-244), in the caller's two LDS buffers of 12 vertices
+// (ping-pong; the result is returned through *out).  A plane with every
+// vertex inside leaves the polygon as it is, so it is skipped: the result
+// is the same, without the copy the reference makes for it.
+__device__ __forceinline__ int clip_polygon(Vtx* buf0, Vtx* buf1, int n, Vtx** out) {
+    Vtx* in = buf0;
+    Vtx* o = buf1;
+    for (int pl = 0; pl < 7; pl++) {
+        const int ax = (pl - 1) >> 1, side = (pl - 1) & 1;
+        bool all = true;
+        for (int i = 0; i < n; i++) {
+            const float w = in[i].hom[3];
+            const bool ins = (pl == 0) ? !(w < 1e-5f) : ((side ? -in[i].hom[ax] : in[i].hom[ax]) <= w);
+            all = all && ins;
         }
-    }
-    if (n == 0) return 0;
-    for (int ax = 0; ax < 3; ax++) {
-        for (int side = 0; side < 2; side++) {
-            if (n == 0) return 0;
-            for (int i = 0; i < n; i++) in[i] = poly[i];
-            cnt = n;
-            n = 0;
-            int prev = cnt - 1;
+        if (all) continue;
+        int m = 0;
+        int prev = n - 1;
+        if (pl == 0) {
+            int pd = (in[prev].hom[3] < 1e-5f) ? -1 : 1;
+            for (int i = 0; i < n; i++) {
+                int cd = (in[i].hom[3] < 1e-5f) ? -1 : 1;
+                if (pd * cd < 0 && m < 12)
+                    o[m++] = lerp_vtx(in[prev], in[i], __fdiv_rn(1e-5f - in[prev].hom[3], in[i].hom[3] - in[prev].hom[3]));
+                if (cd > 0 && m < 12) o[m++] = in[i];
+                pd = cd;
+                prev = i;
+            }
+        } else {
             float pa = side ? -in[prev].hom[ax] : in[prev].hom[ax];
             int pd = (pa <= in[prev].hom[3]) ? 1 : -1;
-            for (int i = 0; i < cnt; i++) {
+            for (int i = 0; i < n; i++) {
                 float ca = side ? -in[i].hom[ax] : in[i].hom[ax];
                 int cd = (ca <= in[i].hom[3]) ? 1 : -1;
-                if (pd * cd <= 0 && n < 12) {
+                if (pd * cd <= 0 && m < 12) {
                     float num, den;
                     if (!side) { num = in[prev].hom[3] - in[prev].hom[ax]; den = num - (in[i].hom[3] - in[i].hom[ax]); }
                     else       { num = in[prev].hom[3] + in[prev].hom[ax]; den = num - (in[i].hom[3] + in[i].hom[ax]); }
-                    poly[n++] = lerp_vtx(in[prev], in[i], __fdiv_rn(num, den));
+                    o[m++] = lerp_vtx(in[prev], in[i], __fdiv_rn(num, den));
                 }
-                if (cd > 0 && n < 12) poly[n++] = in[i];
+                if (cd > 0 && m < 12) o[m++] = in[i];
                 pd = cd;
                 prev = i;
             }
         }
+        n = m;
+        Vtx* t = in; in = o; o = t;
+        if (n == 0) break;
     }
+    *out = in;
     return n;
 }
 
@@ -605,9 +623,11 @@ struct SetupArgs {
     const float*    nrm;
     const uint32_t* info;
     const PFFstate* states;
+    const DTex*     texs;
     const DSeg*     segs;
     const DFrame*   frames;
     uint32_t        numSegs;
+    const uint32_t* blockSeg;       // per 256-thread block: segment of its first thread
     uint32_t        numPassTris;
     uint32_t        tileShift;
     uint32_t        numTiles;
@@ -624,6 +644,8 @@ struct SetupArgs {
     uint32_t*       vals;           // raster triangle of each pair
     uint32_t        capRT;
     uint32_t        capPairs;
+    uint32_t*       clipList;       // pass triangles that need clipping (PHASE 0 appends)
+    uint32_t*       clipCount;
 };
 
 // Tiles [tx0..tx1] x [ty0..ty1] overlapped by the covered pixels of a
@@ -700,7 +722,14 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     r.meta = meta;
     r.state = stateIdx;
     r.uv[0] = a.uv[0]; r.uv[1] = a.uv[1]; r.uv[2] = b.uv[0]; r.uv[3] = b.uv[1]; r.uv[4] = c.uv[0]; r.uv[5] = c.uv[1];
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    r.tex[0] = r.tex[1] = r.tex[2] = 0;
+    if (meta & META_TEX) {
+        const DTex T = A.texs[st->texture];
+        const uint64_t addr = (uint64_t)(uintptr_t)T.pixels;
+        r.tex[0] = (uint32_t)addr;
+        r.tex[1] = ((uint32_t)(addr >> 32) & 0xFFFFu) | ((T.format & 1u) << 16) | ((T.wrap & 3u) << 17) | ((T.filter & 1u) << 19);
+        r.tex[2] = (T.w & 0xFFFFu) | (T.h << 16);
+    }
     A.rts[rtSlot] = r;
     if (phong) {
         RTriL l;
@@ -731,33 +760,80 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     return true;
 }
 
+// Clip-space position (pfmVec4Transform, pfm.h:1996; no contraction).
+__device__ __forceinline__ void transform_pos(CState* st, float4 p, float* h) {
+    h[0] = st->mvp[0] * p.x + st->mvp[4] * p.y + st->mvp[8] * p.z + st->mvp[12] * p.w;
+    h[1] = st->mvp[1] * p.x + st->mvp[5] * p.y + st->mvp[9] * p.z + st->mvp[13] * p.w;
+    h[2] = st->mvp[2] * p.x + st->mvp[6] * p.y + st->mvp[10] * p.z + st->mvp[14] * p.w;
+    h[3] = st->mvp[3] * p.x + st->mvp[7] * p.y + st->mvp[11] * p.z + st->mvp[15] * p.w;
+}
+
 // All culling passes of recorded triangle t under state st.  PHASE 0 counts
 // raster triangles and tile pairs (geometry only); PHASE 1 also lights the
-// vertices and writes records and pairs from the given slots.
-template <int PHASE, bool LIT>
-__device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
+// vertices and writes records and pairs from the given slots.  With CLIP
+// false a triangle that needs clipping is left alone and true is returned:
+// the caller runs it again with CLIP true, one lane at a time, clipping in
+// the wave's LDS buffers (no per-lane private arrays, so no scratch).
+template <int PHASE, bool LIT, bool CLIP>
+__device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
                                                uint32_t t, uint32_t info, uint32_t rtBase, uint32_t pairBase,
-                                               uint32_t& nRT, uint32_t& nPairs) {
-    Vtx base[3];
-    for (int k = 0; k < 3; k++) {
-        size_t vi = (size_t)t * 3 + k;
-        float4 p = A.pos[vi];
-        base[k].pos[0] = p.x; base[k].pos[1] = p.y; base[k].pos[2] = p.z; base[k].pos[3] = p.w;
-        base[k].col = (PHASE == 1) ? A.col[vi] : 0u;
-        if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; base[k].uv[0] = q.x; base[k].uv[1] = q.y; }
-        else { base[k].uv[0] = base[k].uv[1] = 0.0f; }
-        if (PHASE == 1 && A.nrm) { base[k].nrm[0] = A.nrm[3 * vi]; base[k].nrm[1] = A.nrm[3 * vi + 1]; base[k].nrm[2] = A.nrm[3 * vi + 2]; }
-        else { base[k].nrm[0] = base[k].nrm[1] = base[k].nrm[2] = 0.0f; }
-        base[k].scr[0] = base[k].scr[1] = 0.0f;
-        base[k].bw[0] = k == 0 ? 1.0f : 0.0f;
-        base[k].bw[1] = k == 1 ? 1.0f : 0.0f;
-        base[k].bw[2] = k == 2 ? 1.0f : 0.0f;
+                                               uint32_t& nRT, uint32_t& nPairs, Vtx* cpoly, Vtx* ctmp) {
+    // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
+    // tests from the positions alone.  The clip planes are taken in the
+    // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
+    // all three vertices outside while every earlier plane had all of them
+    // inside (so the polygon reaching it is still the triangle) clips it to
+    // nothing, exactly as the clipper would.
+    bool is2D, inside, reject = false;
+    {
+        float h[3][4];
+        float wsum = 0.0f;
+        for (int k = 0; k < 3; k++) {
+            transform_pos(st, A.pos[(size_t)t * 3 + k], h[k]);
+            wsum += h[k][3];
+        }
+        is2D = fabsf(wsum - 3.0f) < 1e-5f;
+        bool unchanged = true;
+        for (int pl = 0; pl < 7 && unchanged; pl++) {
+            int nout = 0;
+            for (int k = 0; k < 3; k++) {
+                bool out;
+                if (pl == 0) out = h[k][3] < 1e-5f;
+                else {
+                    const int ax = (pl - 1) >> 1;
+                    const float a = ((pl - 1) & 1) ? -h[k][ax] : h[k][ax];
+                    out = !(a <= h[k][3]);
+                }
+                nout += out ? 1 : 0;
+            }
+            if (nout == 3) reject = true;
+            if (nout > 0) unchanged = false;
+        }
+        inside = unchanged;
     }
+    if (!is2D && reject) return false;
+    if (!CLIP && !is2D && !inside) return true;
     const uint32_t flags = st->flags;
+    const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
     for (int face = 0; face < 2; face++) {
         if (!(info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT))) continue;
+        // Vertices are (re)loaded per pass rather than kept live across both.
         Vtx v[3];
-        for (int k = 0; k < 3; k++) v[k] = base[k];
+        for (int k = 0; k < 3; k++) {
+            size_t vi = (size_t)t * 3 + k;
+            float4 p = A.pos[vi];
+            v[k].pos[0] = p.x; v[k].pos[1] = p.y; v[k].pos[2] = p.z; v[k].pos[3] = p.w;
+            transform_pos(st, p, v[k].hom);
+            v[k].col = (PHASE == 1) ? A.col[vi] : 0u;
+            if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; v[k].uv[0] = q.x; v[k].uv[1] = q.y; }
+            else { v[k].uv[0] = v[k].uv[1] = 0.0f; }
+            if (PHASE == 1 && A.nrm) { v[k].nrm[0] = A.nrm[3 * vi]; v[k].nrm[1] = A.nrm[3 * vi + 1]; v[k].nrm[2] = A.nrm[3 * vi + 2]; }
+            else { v[k].nrm[0] = v[k].nrm[1] = v[k].nrm[2] = 0.0f; }
+            v[k].scr[0] = v[k].scr[1] = 0.0f;
+            v[k].bw[0] = k == 0 ? 1.0f : 0.0f;
+            v[k].bw[1] = k == 1 ? 1.0f : 0.0f;
+            v[k].bw[2] = k == 2 ? 1.0f : 0.0f;
+        }
         if (PHASE == 1 && LIT && (flags & PFF_LIGHTING)) {              // triangles.c:78-104
             const PFFstate& stg = A.states[si];
             for (int k = 0; k < 3; k++) {
@@ -785,34 +861,14 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
                 }
             }
         }
-        // Process_ProjectAndClipTriangle (triangles.c:246-280)
-        float m[16];
-        for (int i = 0; i < 16; i++) m[i] = st->mvp[i];
-        float wsum = 0.0f;
-        for (int k = 0; k < 3; k++) {
-            const float* p = v[k].pos;
-            v[k].hom[0] = m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12] * p[3];
-            v[k].hom[1] = m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13] * p[3];
-            v[k].hom[2] = m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14] * p[3];
-            v[k].hom[3] = m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15] * p[3];
-            wsum += v[k].hom[3];
-        }
-        const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
         uint32_t np = 0;
-        if (fabsf(wsum - 3.0f) < 1e-5f) {                                 // "2D" path
+        if (is2D) {                                                       // "2D" path
             for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
             if (emit_raster<PHASE>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
             }
             continue;
-        }
-        bool inside = true;                   // clipping returns the polygon unchanged
-        for (int k = 0; k < 3; k++) {
-            const float w = v[k].hom[3];
-            inside = inside && !(w < 1e-5f);
-            for (int ax = 0; ax < 3; ax++)
-                inside = inside && (v[k].hom[ax] <= w) && (-v[k].hom[ax] <= w);
         }
         if (inside) {
             for (int k = 0; k < 3; k++) {
@@ -830,26 +886,32 @@ __device__ __forceinline__ void setup_triangle(const SetupArgs& A, CState* st, u
             }
             continue;
         }
-        Vtx poly[12];
-        for (int k = 0; k < 3; k++) poly[k] = v[k];
-        int n = clip_polygon(poly, 3);
-        if (n < 3) continue;
-        for (int k = 0; k < n; k++) {
-            poly[k].hom[2] = __fdiv_rn(1.0f, poly[k].hom[2]);
-            poly[k].uv[0] = poly[k].uv[0] * poly[k].hom[2];
-            poly[k].uv[1] = poly[k].uv[1] * poly[k].hom[2];
-            float invW = __fdiv_rn(1.0f, poly[k].hom[3]);
-            poly[k].hom[0] *= invW;
-            poly[k].hom[1] *= invW;
-            to_screen(stl, poly[k]);
-        }
-        for (int i = 0; i < n - 2; i++)
-            if (emit_raster<PHASE>(A, st, F, si, t, face, true, poly[0], poly[i + 1], poly[i + 2], rtBase + nRT,
-                                   pairBase + nPairs, np)) {
-                nRT++;
-                nPairs += np;
+        if (CLIP) {
+            for (int k = 0; k < 3; k++) cpoly[k] = v[k];
+            Vtx* poly = cpoly;                   // the buffer holding the result (either one)
+            int n = clip_polygon(cpoly, ctmp, 3, &poly);
+            if (n < 3) continue;
+            for (int k = 0; k < n; k++) {
+                Vtx q = poly[k];
+                q.hom[2] = __fdiv_rn(1.0f, q.hom[2]);
+                q.uv[0] = q.uv[0] * q.hom[2];
+                q.uv[1] = q.uv[1] * q.hom[2];
+                float invW = __fdiv_rn(1.0f, q.hom[3]);
+                q.hom[0] *= invW;
+                q.hom[1] *= invW;
+                to_screen(stl, q);
+                poly[k] = q;
             }
+            for (int i = 0; i < n - 2; i++) {                             // triangles.c:113-115
+                const Vtx a = poly[0], b = poly[i + 1], c = poly[i + 2];
+                if (emit_raster<PHASE>(A, st, F, si, t, face, true, a, b, c, rtBase + nRT, pairBase + nPairs, np)) {
+                    nRT++;
+                    nPairs += np;
+                }
+            }
+        }
     }
+    return false;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -884,6 +946,24 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
 // One thread per recorded triangle of the pass.  PHASE 0 counts and sums
 // per block; PHASE 1 turns the block sums (scanned in between) and a
 // block-local scan into output slots and writes records and pairs.
+// Recorded triangle, state and frame of pass-triangle gid: the last
+// segment whose passTriBase <= gid, found forward from its block's first.
+__device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, uint32_t& t, uint32_t& info,
+                                              uint32_t& si, DFrame& F) {
+    uint32_t lo = A.blockSeg[gid / 256];
+    while (lo + 1 < A.numSegs && A.segs[lo + 1].passTriBase <= gid) lo++;
+    const DSeg sg = A.segs[lo];
+    t = sg.triFirst + (gid - sg.passTriBase);
+    F = A.frames[sg.frame];
+    info = A.info[t];
+    si = info & PFF_STATE_MASK;
+}
+
+// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
+// per block; PHASE 1 turns the block sums (scanned in between) and a
+// block scan of the counts into output slots and writes.  Triangles that
+// need clipping are only listed here (PHASE 0) and handed their slots
+// (PHASE 1, through cntRT/cntPairs); k_clip does their work.
 template <int PHASE, bool LIT>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     __shared__ uint32_t lds[64];
@@ -903,36 +983,72 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     uint32_t nRT = 0, nPairs = 0;
     uint32_t t = 0, info = 0, si = 0;
     DFrame F;
-    if (valid) {
-        // Segment of this thread: the last one whose passTriBase <= gid.
-        uint32_t lo = 0, hi = A.numSegs;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (A.segs[mid].passTriBase <= gid) lo = mid; else hi = mid;
-        }
-        const DSeg sg = A.segs[lo];
-        t = sg.triFirst + (gid - sg.passTriBase);
-        F = A.frames[sg.frame];
-        info = A.info[t];
-        si = info & PFF_STATE_MASK;
-    }
+    if (valid) pass_triangle(A, gid, t, info, si, F);
     // One iteration per distinct state in the wavefront: the state is then
     // uniform and read with scalar loads.
     uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+    bool deferred = false;
     while (todo) {
         const int leader = __builtin_ctzll(todo);
         const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
         const bool mine = valid && si == s0;
         todo &= ~__builtin_amdgcn_ballot_w64(mine);
         if (mine)
-            setup_triangle<PHASE, LIT>(A, as_const(A.states + s0), s0, F, t, info, rtBase, pairBase, nRT, nPairs);
+            deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, rtBase, pairBase,
+                                                         nRT, nPairs, nullptr, nullptr);
     }
     if (PHASE == 0) {
+        // List the triangles to clip: one counter update per wave.
+        const int lane = threadIdx.x & 63;
+        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
+        if (D) {
+            uint32_t base = 0;
+            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
+            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
+            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            if (deferred) A.clipList[base + __popcll(D & ltMask)] = gid;
+        }
         if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
         uint32_t totR, totP;
         block_excl_scan(nRT, lds, totR);
         block_excl_scan(nPairs, lds, totP);
         if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
+    } else if (deferred) {
+        A.cntRT[gid] = rtBase;            // k_clip<1> writes from these slots
+        A.cntPairs[gid] = pairBase;
+    }
+}
+
+// The listed triangles that need clipping, one per lane, each lane
+// clipping in its own LDS area (so no per-lane private arrays: no
+// scratch).  One wave per workgroup, one workgroup per CU; the list is
+// walked with a grid stride.  PHASE 0 stores the counts and adds them to
+// the block sums; PHASE 1 writes from the slots k_setup<1> handed over.
+constexpr int CLIP_WORDS = 2 * 12 * (int)(sizeof(Vtx) / 4) + 1;   // odd: lanes fall in distinct banks
+constexpr int CLIP_GRID = 256;
+
+template <int PHASE, bool LIT>
+__global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
+    __shared__ uint32_t mem[64 * CLIP_WORDS];
+    const int lane = threadIdx.x;
+    Vtx* poly = reinterpret_cast<Vtx*>(&mem[lane * CLIP_WORDS]);
+    Vtx* tmp = poly + 12;
+    const uint32_t n = *A.clipCount;
+    for (uint32_t i = blockIdx.x * 64 + lane; i < n; i += CLIP_GRID * 64) {
+        const uint32_t gid = A.clipList[i];
+        uint32_t t, info, si;
+        DFrame F;
+        pass_triangle(A, gid, t, info, si, F);
+        uint32_t nRT = 0, nPairs = 0;
+        const uint32_t rtBase = PHASE == 1 ? A.cntRT[gid] : 0u, pairBase = PHASE == 1 ? A.cntPairs[gid] : 0u;
+        setup_triangle<PHASE, LIT, true>(A, as_const(A.states + si), si, F, t, info, rtBase, pairBase, nRT, nPairs,
+                                         poly, tmp);
+        if (PHASE == 0) {
+            A.cntRT[gid] = nRT;
+            A.cntPairs[gid] = nPairs;
+            if (nRT) atomicAdd(&A.blockRT[gid / 256], nRT);
+            if (nPairs) atomicAdd(&A.blockPairs[gid / 256], nPairs);
+        }
     }
 }
 
@@ -1083,14 +1199,11 @@ struct RasterArgs {
     const RTri*     rts;
     const RTriL*    rtl;
     const PFFstate* states;
-    const DTex*     texs;
-    const DFrame*   frames;
-    const int32_t*  frameSeg;       // per frame: index into segs, -1 if idle this pass
-    const DSeg*     segs;
+    const uint32_t* tileFrame;      // per tile: its frame
+    const DTileFrame* tileDesc;     // per frame: frame and clear of this pass
     const uint32_t* tileBegin;
     const uint32_t* tileEnd;
     const uint32_t* vals;           // sorted pair values (raster triangle ids)
-    uint32_t        numFrames;
     uint32_t        numTiles;
     uint32_t*       tileFrags;      // [numTiles][4]: covered samples per tile and wave (plain
                                     // stores: one counter shared by all waves costs more than
@@ -1125,6 +1238,7 @@ template <bool PHONG>
 struct WaveQueueT {
     uint32_t rec[BATCH][32];                // staged RTri records
     uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
+    uint32_t recId[BATCH];                  // raster-triangle index of each staged record
     uint32_t fcol[64];
     float    fz[64];
     uint8_t  ftri[64];
@@ -1168,8 +1282,14 @@ __device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& 
     uint32_t frag = (meta & META_SMOOTH) ? bary_smooth(t.c[0], t.c[1], t.c[2], b1, b2, b3)
                                          : bary_flat(t.c[0], t.c[1], t.c[2], b1, b2, b3);
     if (meta & META_TEX) {
-        const PFFstate* st = &A.states[t.state];
-        const DTex tex = A.texs[st->texture];
+        DTex tex;
+        tex.pixels = (const void*)(uintptr_t)(((uint64_t)(t.tex[1] & 0xFFFFu) << 32) | t.tex[0]);
+        tex.format = (t.tex[1] >> 16) & 1u;
+        tex.wrap = (t.tex[1] >> 17) & 3u;
+        tex.filter = (t.tex[1] >> 19) & 1u;
+        tex.w = t.tex[2] & 0xFFFFu;
+        tex.h = t.tex[2] >> 16;
+        tex.pad = 0;
         float u = (t.uv[0] * b1 + t.uv[2] * b2) + t.uv[4] * b3;
         float v = (t.uv[1] * b1 + t.uv[3] * b2) + t.uv[5] * b3;
         if (meta & META_3D) { u = u * zq; v = v * zq; }
@@ -1210,14 +1330,15 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
     win = id + 1u;
 }
 
-// Dword of a staged record that holds its raster-triangle index (the
-// first pad word of RTri, filled when the record is staged).
-constexpr int REC_ID = offsetof(RTri, pad) / 4;
+
 
 // Shades the queued fragments and applies them in order (packed mode).
-template <bool PHONG>
+// Queued pixel p of a TW-wide block is (bx0 + p % TW, by0 + p / TW); lane
+// l owns pixels l + 64k, k < K, with chains head[k].
+template <bool PHONG, int K>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
-                                            int nf, int head, uint32_t& col, float& dep, uint32_t& win) {
+                                            int nf, const int* head, uint32_t* col, float* dep, uint32_t* win) {
+    constexpr int TW = (K == 1) ? 8 : 16;
     wave_sync();
     if (lane < nf) {
         const int j = Q.ftri[lane];
@@ -1225,41 +1346,58 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
         const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
         uint32_t c;
         float z;
-        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p & 7), by0 + (p >> 3), c, z);
+        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p % TW),
+                              by0 + (p / TW), c, z);
         Q.fcol[lane] = c;
         Q.fz[lane] = z;
     }
     wave_sync();
-    for (int s = head; s >= 0;) {
-        const int j = Q.ftri[s];
-        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.rec[j][REC_ID], col, dep, win);
-        const int nx = Q.fnext[s];
-        s = (nx == 0xFF) ? -1 : nx;
-    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        for (int s = head[k]; s >= 0;) {
+            const int j = Q.ftri[s];
+            rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.recId[j], col[k], dep[k], win[k]);
+            const int nx = Q.fnext[s];
+            s = (nx == 0xFF) ? -1 : nx;
+        }
     wave_sync();
 }
 
-// Renders the 8x8 block at (bx0, by0) of a tile whose pair list is
-// [e0, e1); returns the covered samples it saw.
-template <int MODE, bool PHONG>
-__device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DSeg& sg,
+// Renders the block at (bx0, by0) of a tile whose pair list is [e0, e1):
+// 8x8 with one pixel per lane (K = 1) or 16x16 with four pixels per lane
+// (K = 4: lane l owns column l % 16 of rows l / 16 + 4k), so that one walk
+// of the list serves 256 pixels.  Returns the covered samples it saw.
+template <int MODE, bool PHONG, int K>
+__device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
                                                  int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueT<PHONG>& Q) {
+    constexpr int TW = (K == 1) ? 8 : 16;
+    constexpr int RS = 64 / TW;                    // rows between a lane's pixels
     const int lane = threadIdx.x & 63;
-    const int px = bx0 + (lane & 7), py = by0 + (lane >> 3);
-    const bool in = px < (int)F.W && py < (int)F.H;
+    const int px = bx0 + (lane % TW);
+    int py[K];
+    bool in[K];
+    size_t pi[K];
+    uint32_t col[K], win[K];
+    float dep[K];
     const size_t npix = (size_t)F.W * F.H;
     const size_t aligned = npix - (npix % 8);
-    uint32_t col = 0u, win = 0u;
-    float dep = 0.0f;
-    const size_t pi = in ? (size_t)py * F.W + px : 0;
-    if (in) {
-        if (sg.clear && pi >= 8 && pi < aligned) {                 // context.c:693-713
-            col = sg.clearColor;
-            dep = sg.clearDepth;
-        } else {
-            const size_t src = (sg.clear && pi >= aligned) ? 0 : pi;
-            col = F.color[src];
-            dep = F.depth[src];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        py[k] = by0 + lane / TW + RS * k;
+        in[k] = px < (int)F.W && py[k] < (int)F.H;
+        pi[k] = in[k] ? (size_t)py[k] * F.W + px : 0;
+        col[k] = 0u;
+        win[k] = 0u;
+        dep[k] = 0.0f;
+        if (in[k]) {
+            if (sg.clear && pi[k] >= 8 && pi[k] < aligned) {       // context.c:693-713
+                col[k] = sg.clearColor;
+                dep[k] = sg.clearDepth;
+            } else {
+                const size_t src = (sg.clear && pi[k] >= aligned) ? 0 : pi[k];
+                col[k] = F.color[src];
+                dep[k] = F.depth[src];
+            }
         }
     }
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1277,10 +1415,9 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
             q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
             q4 = src[4]; q5 = src[5]; q6 = src[6]; q7 = src[7];
-            q7.x = r;                                    // REC_ID
             const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
             const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
-            ov = !(xMax <= bx0 || xMin > bx0 + 7 || yMax < by0 || yMin > by0 + 7);
+            ov = !(xMax <= bx0 || xMin > bx0 + TW - 1 || yMax < by0 || yMin > by0 + TW - 1);
         }
         const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
         const int nt = __popcll(OV);
@@ -1289,6 +1426,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
             dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
             dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
+            Q.recId[slot] = r;
             if (PHONG && (q5.x & META_PHONG)) {
                 const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
                 uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
@@ -1297,59 +1435,71 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             }
         }
         wave_sync();
-        int nf = 0, head = -1, tail = -1;
+        int nf = 0, head[K], tail[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) head[k] = tail[k] = -1;
         for (int j = 0; j < nt; j++) {
             const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
-            int32_t w1, w2, w3;
-            edge_values(t, px, py, w1, w2, w3);
-            const bool cov = in && ((w1 | w2 | w3) > 0) && px >= t.xMin && px < t.xMax
-                             && py >= t.yMin && py <= t.yMax;
-            const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
-            if (!M) continue;
-            const int c = __popcll(M);
-            frags += (uint32_t)c;
-            if (MODE == 1) {
-                if (cov) {
-                    uint32_t fc;
-                    float fz;
-                    shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), px, py, fc, fz);
-                    rop(t.meta, fc, fz, Q.rec[j][REC_ID], col, dep, win);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                int32_t w1, w2, w3;
+                edge_values(t, px, py[k], w1, w2, w3);
+                const bool cov = in[k] && ((w1 | w2 | w3) > 0) && px >= t.xMin && px < t.xMax
+                                 && py[k] >= t.yMin && py[k] <= t.yMax;
+                const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
+                if (!M) continue;
+                const int c = __popcll(M);
+                frags += (uint32_t)c;
+                if (MODE == 1) {
+                    if (cov) {
+                        uint32_t fc;
+                        float fz;
+                        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), px, py[k],
+                                              fc, fz);
+                        rop(t.meta, fc, fz, Q.recId[j], col[k], dep[k], win[k]);
+                    }
+                    continue;
                 }
-                continue;
+                if (nf + c > 64) {
+                    flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
+                    nf = 0;
+#pragma unroll
+                    for (int q = 0; q < K; q++) head[q] = tail[q] = -1;
+                }
+                if (cov) {
+                    const int slot = nf + (int)__popcll(M & ltMask);
+                    Q.ftri[slot] = (uint8_t)j;
+                    Q.fpix[slot] = (uint8_t)(lane + 64 * k);
+                    Q.fnext[slot] = 0xFF;
+                    if (tail[k] >= 0) Q.fnext[tail[k]] = (uint8_t)slot; else head[k] = slot;
+                    tail[k] = slot;
+                }
+                nf += c;
             }
-            if (nf + c > 64) {
-                flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
-                nf = 0;
-                head = tail = -1;
-            }
-            if (cov) {
-                const int slot = nf + (int)__popcll(M & ltMask);
-                Q.ftri[slot] = (uint8_t)j;
-                Q.fpix[slot] = (uint8_t)lane;
-                Q.fnext[slot] = 0xFF;
-                if (tail >= 0) Q.fnext[tail] = (uint8_t)slot; else head = slot;
-                tail = slot;
-            }
-            nf += c;
         }
-        if (MODE == 0 && nf > 0) flush_queue<PHONG>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
+        if (MODE == 0 && nf > 0) flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
         wave_sync();
     }
-    if (in) {
-        F.color[pi] = col;
-        F.depth[pi] = dep;
-        if (A.winner) A.winner[pi] = win;
-    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (in[k]) {
+            F.color[pi[k]] = col[k];
+            F.depth[pi[k]] = dep[k];
+            if (A.winner) A.winner[pi[k]] = win[k];
+        }
     return frags;
 }
 
-// Four waves per workgroup.  TS = 8: four tiles per workgroup, one per
-// wave; TS >= 16: one tile per workgroup, its 8x8 blocks dealt round-robin
-// to the four waves.
-template <int TS, int MODE, bool PHONG>
+// Four waves per workgroup.  A wave renders blocks of BW x BW pixels
+// (BW = 8 for K = 1, 16 for K = 4).  When a tile is one block, each wave
+// takes its own tile (four tiles per workgroup); otherwise one tile per
+// workgroup, its blocks dealt round-robin to the four waves.
+template <int TS, int MODE, bool PHONG, int K>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     __shared__ WaveQueueT<PHONG> Qs[4];
-    constexpr int BPR = TS / 8;
+    constexpr int BW = (K == 1) ? 8 : 16;
+    static_assert(TS >= BW, "tile smaller than a block");
+    constexpr int BPR = TS / BW;
     constexpr int NB = BPR * BPR;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveQueueT<PHONG>& Q = Qs[wave];
@@ -1357,24 +1507,18 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     if (tile >= A.numTiles) return;
     uint32_t* fragOut = A.tileFrags + (size_t)tile * 4 + (NB == 1 ? 0 : wave);
     const bool lead = (threadIdx.x & 63) == 0;
-    uint32_t lo = 0, hi = A.numFrames;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (A.frames[mid].tileBase <= tile) lo = mid; else hi = mid;
-    }
-    const DFrame F = A.frames[lo];
-    const int32_t segIdx = A.frameSeg[lo];
-    if (segIdx < 0) { if (lead) *fragOut = 0u; return; }
-    const DSeg sg = A.segs[segIdx];
+    const DTileFrame sg = A.tileDesc[A.tileFrame[tile]];
+    if (!sg.active) { if (lead) *fragOut = 0u; return; }
+    const DFrame& F = sg.F;
     const uint32_t e0 = A.tileBegin[tile], e1 = A.tileEnd[tile];
     if (e0 == e1 && !sg.clear) { if (lead) *fragOut = 0u; return; }
     const uint32_t local = tile - F.tileBase;
     const int tx = (int)(local % F.tilesX), ty = (int)(local / F.tilesX);
     uint32_t frags = 0;
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
-        const int bx0 = tx * TS + (b % BPR) * 8, by0 = ty * TS + (b / BPR) * 8;
+        const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        frags += raster_block<MODE, PHONG>(A, F, sg, bx0, by0, e0, e1, Q);
+        frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -1788,7 +1932,7 @@ struct PassInfo {
     uint32_t numRT = 0;        // known after the first run
     uint32_t numPairs = 0;
     bool     sized = false;
-    DevBuf   dSegs, dFrameSeg;
+    DevBuf   dSegs, dFrameSeg, dBlockSeg, dTileDesc;
 };
 
 }  // namespace
@@ -1813,7 +1957,7 @@ struct PFDjob {
     uint32_t numTiles = 0;
     // scratch
     DevBuf cnt, off, blockRT, blockPairs, rts, rtl, keysA, valsA, keysB, valsB;
-    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags;
+    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags, dTileFrame, clipList, clipCount;
     uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
     // graph
     hipGraph_t graph = nullptr;
@@ -1855,11 +1999,22 @@ uint32_t grid_for(uint64_t n, uint32_t block) {
     return g == 0 ? 1u : (uint32_t)g;
 }
 
-template <int TS, int MODE>
+template <int TS, int MODE, int K>
 void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, hipStream_t s) {
-    uint32_t grid = (TS == 8) ? (numTiles + 3) / 4 : numTiles;
-    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true>), dim3(grid), dim3(256), 0, s, ra);
-    else       hipLaunchKernelGGL((k_raster<TS, MODE, false>), dim3(grid), dim3(256), 0, s, ra);
+    constexpr int BW = (K == 1) ? 8 : 16;
+    uint32_t grid = (TS == BW) ? (numTiles + 3) / 4 : numTiles;
+    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, K>), dim3(grid), dim3(256), 0, s, ra);
+    else       hipLaunchKernelGGL((k_raster<TS, MODE, false, K>), dim3(grid), dim3(256), 0, s, ra);
+}
+
+template <int MODE, int K>
+void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, bool phong, hipStream_t s) {
+    switch (shift) {
+    case 3: launch_raster<8, MODE, 1>(ra, numTiles, phong, s); break;   // a 8x8 tile is one K = 1 block
+    case 4: launch_raster<16, MODE, K>(ra, numTiles, phong, s); break;
+    case 5: launch_raster<32, MODE, K>(ra, numTiles, phong, s); break;
+    default: launch_raster<64, MODE, K>(ra, numTiles, phong, s); break;
+    }
 }
 
 // Issues one pass.  With discover set, the raster-triangle and pair counts
@@ -1874,6 +2029,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if ((rc = ensure(J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
     if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
     if ((rc = ensure(J->scanTotal, 16))) return rc;
+    if ((rc = ensure(J->clipList, (size_t)n * 4 + 4)) || (rc = ensure(J->clipCount, 16))) return rc;
     const size_t fragSlots = (size_t)J->numTiles * 4;
     const size_t passIdx = (size_t)(&P - J->passes.data());
     if (discover) {
@@ -1895,9 +2051,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.nrm = J->hasNrm ? (const float*)J->nrm.p : nullptr;
     sa.info = (const uint32_t*)J->info.p;
     sa.states = (const PFFstate*)J->states.p;
+    sa.texs = (const DTex*)J->texTable.p;
     sa.segs = (const DSeg*)P.dSegs.p;
     sa.frames = (const DFrame*)J->dFrames.p;
     sa.numSegs = (uint32_t)P.segs.size();
+    sa.blockSeg = (const uint32_t*)P.dBlockSeg.p;
     sa.numPassTris = n;
     sa.tileShift = J->tileShift;
     sa.numTiles = J->numTiles;
@@ -1907,9 +2065,13 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.blockPairs = (uint32_t*)J->blockPairs.p;
     sa.tileBegin = (uint32_t*)J->tileBegin.p;
     sa.tileEnd = (uint32_t*)J->tileEnd.p;
+    sa.clipList = (uint32_t*)J->clipList.p;
+    sa.clipCount = (uint32_t*)J->clipCount.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
     hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+    hipLaunchKernelGGL((k_clip<0, false>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
                        (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
@@ -1935,8 +2097,13 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.capRT = numRT;
     sa.capPairs = numPairs;
     if (numRT) {
-        if (J->anyLit) hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
-        else           hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+        if (J->anyLit) {
+            hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
+            hipLaunchKernelGGL((k_clip<1, true>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+        } else {
+            hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+            hipLaunchKernelGGL((k_clip<1, false>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+        }
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
@@ -1981,31 +2148,19 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.rts = (const RTri*)J->rts.p;
     ra.rtl = (const RTriL*)J->rtl.p;
     ra.states = (const PFFstate*)J->states.p;
-    ra.texs = (const DTex*)J->texTable.p;
-    ra.frames = (const DFrame*)J->dFrames.p;
-    ra.frameSeg = (const int32_t*)P.dFrameSeg.p;
-    ra.segs = (const DSeg*)P.dSegs.p;
+    ra.tileFrame = (const uint32_t*)J->dTileFrame.p;
+    ra.tileDesc = (const DTileFrame*)P.dTileDesc.p;
     ra.tileBegin = (const uint32_t*)J->tileBegin.p;
     ra.tileEnd = (const uint32_t*)J->tileEnd.p;
     ra.vals = vals;
-    ra.numFrames = (uint32_t)J->frames.size();
     ra.numTiles = J->numTiles;
     ra.tileFrags = (uint32_t*)J->tileFrags.p + passIdx * fragSlots;
     ra.winner = J->diff ? (uint32_t*)J->winner.p : nullptr;
-    if (J->cfg.rasterMode == 1) {
-        switch (J->tileShift) {
-        case 3: launch_raster<8, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        case 4: launch_raster<16, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        case 5: launch_raster<32, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        default: launch_raster<64, 1>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        }
-    } else {
-        switch (J->tileShift) {
-        case 3: launch_raster<8, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        case 4: launch_raster<16, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        case 5: launch_raster<32, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        default: launch_raster<64, 0>(ra, J->numTiles, J->anyPhong, J->stream); break;
-        }
+    switch (J->cfg.rasterMode) {
+    case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
+    case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
+    case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
+    default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
@@ -2035,6 +2190,29 @@ int upload_frames_table(PFDjob* J) {
     int rc = ensure(J->dFrames, J->frames.size() * sizeof(DFrame));
     if (rc) return rc;
     HIP_TRY(hipMemcpy(J->dFrames.p, J->frames.data(), J->frames.size() * sizeof(DFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
+    std::vector<uint32_t> tf(J->numTiles ? J->numTiles : 1, 0u);
+    for (uint32_t f = 0; f < J->frames.size(); f++) {
+        const DFrame& F = J->frames[f];
+        for (uint32_t t = 0; t < F.tilesX * F.tilesY; t++) tf[F.tileBase + t] = f;
+    }
+    if ((rc = ensure(J->dTileFrame, tf.size() * 4))) return rc;
+    HIP_TRY(hipMemcpy(J->dTileFrame.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice), PFD_E_LAUNCH);
+    for (auto& P : J->passes) {
+        std::vector<DTileFrame> d(J->frames.size());
+        for (size_t f = 0; f < d.size(); f++) {
+            memset(&d[f], 0, sizeof d[f]);
+            d[f].F = J->frames[f];
+            const int32_t si = f < P.frameSeg.size() ? P.frameSeg[f] : -1;
+            if (si < 0) continue;
+            const DSeg& sg = P.segs[si];
+            d[f].active = 1;
+            d[f].clear = sg.clear;
+            d[f].clearColor = sg.clearColor;
+            d[f].clearDepth = sg.clearDepth;
+        }
+        if ((rc = ensure(P.dTileDesc, d.size() * sizeof(DTileFrame)))) return rc;
+        HIP_TRY(hipMemcpy(P.dTileDesc.p, d.data(), d.size() * sizeof(DTileFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
+    }
     return 0;
 }
 
@@ -2137,6 +2315,11 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         uint32_t texBase = (uint32_t)htex.size();
         for (uint32_t k = 0; k < fr.numTextures; k++) {
             const PFFtexture& T = fr.textures[k];
+            if (T.width > 0xFFFFu || T.height > 0xFFFFu) {       // raster records keep 16-bit sides
+                snprintf(g_err, sizeof g_err, "texture %ux%u: sides above 65535 are not supported", T.width, T.height);
+                pfdJobDestroy(J);
+                return PFD_E_INVALID;
+            }
             DTex d;
             memset(&d, 0, sizeof d);
             d.w = T.width; d.h = T.height; d.format = T.format; d.wrap = T.wrap; d.filter = T.filter;
@@ -2238,6 +2421,12 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             P.frameSeg[P.segs[s].frame] = (int32_t)s;
         }
         P.numPassTris = acc;
+        std::vector<uint32_t> bs((acc + 255) / 256 + 1, 0u);
+        for (size_t b = 0, sgi = 0; b < bs.size(); b++) {
+            while (sgi + 1 < P.segs.size() && P.segs[sgi + 1].passTriBase <= b * 256) sgi++;
+            bs[b] = (uint32_t)sgi;
+        }
+        if ((rc = up(P.dBlockSeg, bs.data(), bs.size() * 4))) { pfdJobDestroy(J); return rc; }
         if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
             || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
             pfdJobDestroy(J);
@@ -2406,8 +2595,8 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if (!J) return PFD_E_INVALID;
     float best = 1e30f;
     PFDconfig bestCfg = J->cfg;
-    for (uint32_t mode = 0; mode <= 1; mode++)
-        for (uint32_t ts = 3; ts <= 6; ts++) {
+    for (uint32_t mode = 0; mode <= 3; mode++)
+        for (uint32_t ts = (mode >= 2 ? 4 : 3); ts <= 6; ts++) {   // four pixels per lane needs 16x16 tiles
             PFDconfig c = J->cfg;
             c.tileShift = ts;
             c.rasterMode = mode;
@@ -2432,13 +2621,13 @@ void pfdJobDestroy(PFDjob* J) {
     drop_graph(J);
     DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
                        &J->rts, &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
-                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags,
+                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList, &J->clipCount,
                        &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams, &J->lightParams, &J->image,
                        &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
     for (DevBuf* b : bufs) release(*b);
     for (auto& b : J->targets) release(b);
     for (auto& b : J->texData) release(b);
-    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); }
+    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); }
     if (J->stream) (void)hipStreamDestroy(J->stream);
     delete J;
 }
