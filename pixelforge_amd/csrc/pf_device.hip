@@ -127,7 +127,19 @@ enum : uint32_t {
     META_SMALL     = 1u << 7,   // every A_k, B_k fits in 24 signed bits
     META_DFUNC_SH  = 8,     // 3 bits
     META_BMODE_SH  = 12,    // 3 bits
+    META_NOWRAP    = 1u << 16,  // no int32 wrap of any w_k within 64 px of the bbox
+    META_EXACT     = 1u << 17,  // NOWRAP and an unclamped (3D) bbox: covered pixels lie
+                                // inside the bbox, so only x < xMax needs testing
 };
+
+// Smallest and largest true (64-bit) value of w = A*x + B*y + C over the
+// pixels [x0, x1] x [y0, y1] (a linear function: extremes at corners).
+__device__ __forceinline__ void edge_range(int32_t A, int32_t B, int32_t C, int32_t x0, int32_t x1, int32_t y0,
+                                           int32_t y1, int64_t& lo, int64_t& hi) {
+    const int64_t ax0 = (int64_t)A * x0, ax1 = (int64_t)A * x1, by0 = (int64_t)B * y0, by1 = (int64_t)B * y1;
+    lo = (ax0 < ax1 ? ax0 : ax1) + (by0 < by1 ? by0 : by1) + C;
+    hi = (ax0 < ax1 ? ax1 : ax0) + (by0 < by1 ? by1 : by0) + C;
+}
 
 // ---------------------------------------------------------------------------
 // Numeric helpers (x86 semantics of the reference build)
@@ -712,6 +724,15 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         small = small && va > -(1 << 23) && va < (1 << 23) && vb > -(1 << 23) && vb < (1 << 23);
     }
     if (small) meta |= META_SMALL;
+    {
+        bool nowrap = true;
+        for (int k = 0; k < 3; k++) {
+            int64_t lo, hi;
+            edge_range(r.A[k], r.B[k], r.C[k], xMin - 64, xMax + 64, yMin - 64, yMax + 64, lo, hi);
+            nowrap = nowrap && lo >= INT32_MIN && hi <= INT32_MAX;
+        }
+        if (nowrap) meta |= META_NOWRAP | (is3D ? META_EXACT : 0u);
+    }
     const uint32_t flags = st->flags;
     if (st->shadeModel == 1) meta |= META_SMOOTH;
     if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st->depthFunc & 7u) << META_DFUNC_SH);
@@ -1234,9 +1255,15 @@ struct RasterArgs {
 
 constexpr int BATCH = 32;
 
+// Words between staged records.  36 (16-byte aligned, off the 32-bank
+// period) spreads lanes reading one field of different records over the
+// banks, but the larger queue lowers the workgroups per CU from 8 to 7 and
+// measured slower on the batched workload; 32 is kept.
+constexpr int REC_STRIDE = 32;
+
 template <bool PHONG>
 struct WaveQueueT {
-    uint32_t rec[BATCH][32];                // staged RTri records
+    uint32_t rec[BATCH][REC_STRIDE];        // staged RTri records
     uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
     uint32_t recId[BATCH];                  // raster-triangle index of each staged record
     uint32_t fcol[64];
@@ -1418,6 +1445,20 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
             const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
             ov = !(xMax <= bx0 || xMin > bx0 + TW - 1 || yMax < by0 || yMin > by0 + TW - 1);
+            // An edge negative over the whole block leaves it uncovered.  With
+            // NOWRAP the block (inside a tile that meets the bbox, so within 64
+            // px of it) sees no int32 wrap: its largest w is at a corner.
+            if (ov && (q5.x & META_NOWRAP)) {
+                const int32_t xs[2] = { bx0, bx0 + TW - 1 }, ys[2] = { by0, by0 + TW - 1 };
+                const int32_t eA[3] = { (int32_t)q0.x, (int32_t)q0.y, (int32_t)q0.z };
+                const int32_t eB[3] = { (int32_t)q0.w, (int32_t)q1.x, (int32_t)q1.y };
+                const int32_t eC[3] = { (int32_t)q1.z, (int32_t)q1.w, (int32_t)q2.x };
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int32_t hi = eA[k] * xs[eA[k] > 0] + eB[k] * ys[eB[k] > 0] + eC[k];
+                    ov = ov && hi >= 0;
+                }
+            }
         }
         const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
         const int nt = __popcll(OV);
@@ -1440,12 +1481,13 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         for (int k = 0; k < K; k++) head[k] = tail[k] = -1;
         for (int j = 0; j < nt; j++) {
             const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+            const bool exact = (t.meta & META_EXACT) != 0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 int32_t w1, w2, w3;
                 edge_values(t, px, py[k], w1, w2, w3);
-                const bool cov = in[k] && ((w1 | w2 | w3) > 0) && px >= t.xMin && px < t.xMax
-                                 && py[k] >= t.yMin && py[k] <= t.yMax;
+                const bool cov = in[k] && ((w1 | w2 | w3) > 0) && px < t.xMax
+                                 && (exact || (px >= t.xMin && py[k] >= t.yMin && py[k] <= t.yMax));
                 const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
                 if (!M) continue;
                 const int c = __popcll(M);
@@ -2604,7 +2646,7 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
             if (rc) return rc;
             if ((rc = pfdJobUploadTargets(J))) return rc;
             float ms = 0;
-            if ((rc = pfdJobTimeReplays(J, 3, &ms))) return rc;
+            if ((rc = pfdJobTimeReplays(J, 5, &ms))) return rc;
             if (ms < best) { best = ms; bestCfg = c; }
         }
     PFDconfig c = bestCfg;
