@@ -48,15 +48,30 @@ import pixelforge_amd as pf  # noqa: E402
 from pixelforge_amd.diffmesh import Scene, params_of  # noqa: E402
 
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libpforacle.so")
 PEAK_HBM_GBS = 8000.0
 SETUP_BYTES_PER_TRI = 88          # position 3x16 + colour 3x4 + texcoord 3x8 + info 4
+RASTER_MODES = ["packed", "in-place", "packed, 4 px/lane", "in-place, 4 px/lane"]
 
 
 def workload(width=1920, height=1080, seed=2):
     return pf.scene(kind=pf.HEIGHTFIELD, width=width, height=height, seed=seed, gridX=1000, gridZ=100,
                     texture=1, texW=1024, texH=1024, wrap=pf.PF_REPEAT, filter=pf.PF_BILINEAR,
                     depthFunc=pf.PF_LEQUAL, blendFraction=0.3, cull=pf.PF_BACK)
+
+
+def measured_traffic(key):
+    """HBM bytes per dispatch of a kernel configuration, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a copy of known
+    size (profiles/traffic.json, written from profiles/*.md); None if that
+    configuration was not profiled."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f).get(key)
+        return None if t is None else t["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def check(rc, what):
@@ -148,9 +163,10 @@ def bench_frame(args):
                    "fragments_per_frame": frags, "raster_triangles": int(st.rasterTriangles),
                    "tile_pairs": int(st.tilePairs), "tile": 1 << int(st.tileShift),
                    "max_tile_list": int(st.maxTileList), "busy_tiles": int(st.busyTiles),
-                   "raster_mode": ["packed", "in-place"][cfg.rasterMode], "parallelism": "single GPU"},
+                   "raster_mode": RASTER_MODES[cfg.rasterMode], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(f"k_raster 1080p tile {1 << int(st.tileShift)} mode {cfg.rasterMode}"),
                      "kernel": "k_raster", "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
         "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
         "record_s": rec_s,
@@ -190,12 +206,15 @@ def bench_batch(args):
     out = {
         "value": len(params) * args.batch_steps / elapsed, "unit": "frames/s", "contexts": len(params),
         "ms_per_batch": ms, "fragments": int(st.fragments), "input_triangles": tris,
-        "pixels": px, "tile": 1 << int(st.tileShift), "raster_mode": ["packed", "in-place"][cfg.rasterMode],
+        "pixels": px, "tile": 1 << int(st.tileShift), "raster_mode": RASTER_MODES[cfg.rasterMode],
         "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
         "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": algo / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None, "scope": "whole batch step",
+                     "frac": algo / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "scope": "whole batch step",
                      "algorithmic_bytes": algo,
-                     "raster_kernel": {"achieved": (px * 8 + tex) / (raster_ms * 1e-3) / 1e9, "kernel_ms": raster_ms}},
+                     "raster_kernel": {"achieved": (px * 8 + tex) / (raster_ms * 1e-3) / 1e9, "kernel_ms": raster_ms,
+                                       "algorithmic_bytes": px * 8 + tex,
+                                       "traffic": measured_traffic(f"k_raster batch{len(params)} tile "
+                                                                   f"{1 << int(st.tileShift)} mode {cfg.rasterMode}")}},
     }
     if not args.no_cpu and os.path.exists(REF_SO):
         sample = params[: args.cpu_contexts]

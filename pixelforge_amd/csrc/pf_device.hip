@@ -1141,7 +1141,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 
 
 constexpr int RADIX_THREADS = 256;
-constexpr int RADIX_ROUNDS = 16;
+constexpr int RADIX_ROUNDS = 8;      // 2048 items per block: more blocks, shorter serial rounds
 constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ROUNDS;
 
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
@@ -1532,6 +1532,16 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
     return frags;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i mod 8), and each XCD has its own L2.  This maps workgroup i to a
+// contiguous chunk of work per XCD, so neighbouring tiles -- which share
+// raster-triangle records -- run on the same XCD and read them through the
+// same L2.  A bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
+    const uint32_t q = n / 8, r = n % 8, x = i % 8, j = i / 8;
+    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
 // Four waves per workgroup.  A wave renders blocks of BW x BW pixels
 // (BW = 8 for K = 1, 16 for K = 4).  When a tile is one block, each wave
 // takes its own tile (four tiles per workgroup); otherwise one tile per
@@ -1545,7 +1555,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     constexpr int NB = BPR * BPR;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveQueueT<PHONG>& Q = Qs[wave];
-    const uint32_t tile = (NB == 1) ? blockIdx.x * 4 + wave : blockIdx.x;
+    const uint32_t wg = xcd_chunked(blockIdx.x, gridDim.x);
+    const uint32_t tile = (NB == 1) ? wg * 4 + wave : wg;
     if (tile >= A.numTiles) return;
     uint32_t* fragOut = A.tileFrags + (size_t)tile * 4 + (NB == 1 ? 0 : wave);
     const bool lead = (threadIdx.x & 63) == 0;
@@ -1870,10 +1881,8 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A) {
             pend &= ~__builtin_amdgcn_ballot_w64(mine);
         }
     }
-    // Texels and vertex colours
+    // Texels
     if (cov) {
-        float dcf[4];
-        for (int c = 0; c < 4; c++) dcf[c] = dfrag[c] * tex[c];
         if (texOn && texParam && A.gTex) {
             for (int k = 0; k < tp.n; k++) {
                 float* gp = A.gTex + (size_t)tp.off[k] * 4;
@@ -1883,15 +1892,28 @@ __global__ __launch_bounds__(256) void k_diff(DiffArgs A) {
                 }
             }
         }
-        if (A.colParams && A.gCol && A.vid) {
-            for (int j = 0; j < 3; j++) {
-                const uint32_t vi = A.vid[(size_t)src * 3 + j];
-                if (vi >= A.numVertices) continue;
-                float* gp = A.gCol + (size_t)vi * 4;
-                for (int c = 0; c < 4; c++) {
-                    const float x = beta[j] * m[c] * dcf[c];
-                    if (x != 0.0f) unsafeAtomicAdd(gp + c, x);
-                }
+    }
+    // Vertex colours: neighbouring pixels share triangles and vertices, so
+    // each wave first sums the contributions per vertex (one atomic per
+    // vertex and channel instead of one per pixel).
+    if (A.colParams && A.gCol && A.vid) {
+        float dcf[4];
+        for (int c = 0; c < 4; c++) dcf[c] = dfrag[c] * tex[c];
+        for (int j = 0; j < 3; j++) {
+            const uint32_t vi = cov ? A.vid[(size_t)src * 3 + j] : PFF_NO_VERTEX;
+            const bool has = vi < A.numVertices;
+            float x[4];
+            for (int c = 0; c < 4; c++) x[c] = has ? beta[j] * m[c] * dcf[c] : 0.0f;
+            uint64_t pend = __builtin_amdgcn_ballot_w64(has);
+            while (pend) {
+                const uint32_t lead = __builtin_amdgcn_readlane(vi, __builtin_ctzll(pend));
+                const bool mine = has && vi == lead;
+                pend &= ~__builtin_amdgcn_ballot_w64(mine);
+                float sum[4];
+                for (int c = 0; c < 4; c++) sum[c] = wave_sum(mine ? x[c] : 0.0f);
+                if (lane == 0)
+                    for (int c = 0; c < 4; c++)
+                        if (sum[c] != 0.0f) unsafeAtomicAdd(A.gCol + (size_t)lead * 4 + c, sum[c]);
             }
         }
     }
