@@ -566,46 +566,79 @@ __device__ __forceinline__ void to_screen(const float* vp, Vtx& v) {
     v.scr[1] = (vp[1] + ((1.0f - v.hom[1]) * 0.5f) * vp[3]) + 0.5f;
 }
 
+// Forms of a clip vertex in LDS, by what the rest of the setup reads:
+// VF_POS the clip position alone (the counting pass), VF_SLIM position, uv
+// and colour (no Phong record, no corner weights), VF_FULL everything.  The
+// clip position comes first in each form.  Attributes a form leaves out
+// load as zero; screen coordinates are recomputed from the position.
+constexpr int VF_POS = 0, VF_SLIM = 1, VF_FULL = 2;
+template <int VF> struct CVtx;
+template <> struct CVtx<VF_POS>  { float hom[4]; };
+template <> struct CVtx<VF_SLIM> { float hom[4]; float uv[2]; uint32_t col; };
+template <> struct CVtx<VF_FULL> { Vtx v; };
+
+__device__ __forceinline__ const float* cv_hom(const CVtx<VF_FULL>& c) { return c.v.hom; }
+template <int VF> __device__ __forceinline__ const float* cv_hom(const CVtx<VF>& c) { return c.hom; }
+
+__device__ __forceinline__ Vtx cv_load(const CVtx<VF_FULL>& c) { return c.v; }
+__device__ __forceinline__ void cv_store(CVtx<VF_FULL>& c, const Vtx& v) { c.v = v; }
+template <int VF>
+__device__ __forceinline__ Vtx cv_load(const CVtx<VF>& c) {
+    Vtx v;
+    for (int i = 0; i < 4; i++) { v.hom[i] = c.hom[i]; v.pos[i] = 0.0f; }
+    for (int i = 0; i < 3; i++) { v.nrm[i] = 0.0f; v.bw[i] = 0.0f; }
+    v.scr[0] = v.scr[1] = 0.0f;
+    if constexpr (VF == VF_SLIM) { v.uv[0] = c.uv[0]; v.uv[1] = c.uv[1]; v.col = c.col; }
+    else { v.uv[0] = v.uv[1] = 0.0f; v.col = 0u; }
+    return v;
+}
+template <int VF>
+__device__ __forceinline__ void cv_store(CVtx<VF>& c, const Vtx& v) {
+    for (int i = 0; i < 4; i++) c.hom[i] = v.hom[i];
+    if constexpr (VF == VF_SLIM) { c.uv[0] = v.uv[0]; c.uv[1] = v.uv[1]; c.col = v.col; }
+}
+
 // Sutherland-Hodgman against w >= eps then +-x, +-y, +-z <= w
 // (triangles.c:157-244), in the caller's two LDS buffers of 12 vertices
 // (ping-pong; the result is returned through *out).  A plane with every
 // vertex inside leaves the polygon as it is, so it is skipped: the result
 // is the same, without the copy the reference makes for it.
-__device__ __forceinline__ int clip_polygon(Vtx* buf0, Vtx* buf1, int n, Vtx** out) {
-    Vtx* in = buf0;
-    Vtx* o = buf1;
+template <int VF>
+__device__ __forceinline__ int clip_polygon(CVtx<VF>* buf0, CVtx<VF>* buf1, int n, CVtx<VF>** out) {
+    CVtx<VF>* in = buf0;
+    CVtx<VF>* o = buf1;
     for (int pl = 0; pl < 7; pl++) {
         const int ax = (pl - 1) >> 1, side = (pl - 1) & 1;
         bool all = true;
         for (int i = 0; i < n; i++) {
-            const float w = in[i].hom[3];
-            const bool ins = (pl == 0) ? !(w < 1e-5f) : ((side ? -in[i].hom[ax] : in[i].hom[ax]) <= w);
+            const float w = cv_hom(in[i])[3];
+            const bool ins = (pl == 0) ? !(w < 1e-5f) : ((side ? -cv_hom(in[i])[ax] : cv_hom(in[i])[ax]) <= w);
             all = all && ins;
         }
         if (all) continue;
         int m = 0;
         int prev = n - 1;
         if (pl == 0) {
-            int pd = (in[prev].hom[3] < 1e-5f) ? -1 : 1;
+            int pd = (cv_hom(in[prev])[3] < 1e-5f) ? -1 : 1;
             for (int i = 0; i < n; i++) {
-                int cd = (in[i].hom[3] < 1e-5f) ? -1 : 1;
+                int cd = (cv_hom(in[i])[3] < 1e-5f) ? -1 : 1;
                 if (pd * cd < 0 && m < 12)
-                    o[m++] = lerp_vtx(in[prev], in[i], __fdiv_rn(1e-5f - in[prev].hom[3], in[i].hom[3] - in[prev].hom[3]));
+                    cv_store(o[m++], lerp_vtx(cv_load(in[prev]), cv_load(in[i]), __fdiv_rn(1e-5f - cv_hom(in[prev])[3], cv_hom(in[i])[3] - cv_hom(in[prev])[3])));
                 if (cd > 0 && m < 12) o[m++] = in[i];
                 pd = cd;
                 prev = i;
             }
         } else {
-            float pa = side ? -in[prev].hom[ax] : in[prev].hom[ax];
-            int pd = (pa <= in[prev].hom[3]) ? 1 : -1;
+            float pa = side ? -cv_hom(in[prev])[ax] : cv_hom(in[prev])[ax];
+            int pd = (pa <= cv_hom(in[prev])[3]) ? 1 : -1;
             for (int i = 0; i < n; i++) {
-                float ca = side ? -in[i].hom[ax] : in[i].hom[ax];
-                int cd = (ca <= in[i].hom[3]) ? 1 : -1;
+                float ca = side ? -cv_hom(in[i])[ax] : cv_hom(in[i])[ax];
+                int cd = (ca <= cv_hom(in[i])[3]) ? 1 : -1;
                 if (pd * cd <= 0 && m < 12) {
                     float num, den;
-                    if (!side) { num = in[prev].hom[3] - in[prev].hom[ax]; den = num - (in[i].hom[3] - in[i].hom[ax]); }
-                    else       { num = in[prev].hom[3] + in[prev].hom[ax]; den = num - (in[i].hom[3] + in[i].hom[ax]); }
-                    o[m++] = lerp_vtx(in[prev], in[i], __fdiv_rn(num, den));
+                    if (!side) { num = cv_hom(in[prev])[3] - cv_hom(in[prev])[ax]; den = num - (cv_hom(in[i])[3] - cv_hom(in[i])[ax]); }
+                    else       { num = cv_hom(in[prev])[3] + cv_hom(in[prev])[ax]; den = num - (cv_hom(in[i])[3] + cv_hom(in[i])[ax]); }
+                    cv_store(o[m++], lerp_vtx(cv_load(in[prev]), cv_load(in[i]), __fdiv_rn(num, den)));
                 }
                 if (cd > 0 && m < 12) o[m++] = in[i];
                 pd = cd;
@@ -613,7 +646,7 @@ __device__ __forceinline__ int clip_polygon(Vtx* buf0, Vtx* buf1, int n, Vtx** o
             }
         }
         n = m;
-        Vtx* t = in; in = o; o = t;
+        CVtx<VF>* t = in; in = o; o = t;
         if (n == 0) break;
     }
     *out = in;
@@ -795,10 +828,10 @@ __device__ __forceinline__ void transform_pos(CState* st, float4 p, float* h) {
 // false a triangle that needs clipping is left alone and true is returned:
 // the caller runs it again with CLIP true, one lane at a time, clipping in
 // the wave's LDS buffers (no per-lane private arrays, so no scratch).
-template <int PHASE, bool LIT, bool CLIP>
+template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL>
 __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
                                                uint32_t t, uint32_t info, uint32_t rtBase, uint32_t pairBase,
-                                               uint32_t& nRT, uint32_t& nPairs, Vtx* cpoly, Vtx* ctmp) {
+                                               uint32_t& nRT, uint32_t& nPairs, CVtx<VF>* cpoly, CVtx<VF>* ctmp) {
     // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
     // tests from the positions alone.  The clip planes are taken in the
     // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
@@ -908,23 +941,25 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
             continue;
         }
         if (CLIP) {
-            for (int k = 0; k < 3; k++) cpoly[k] = v[k];
-            Vtx* poly = cpoly;                   // the buffer holding the result (either one)
-            int n = clip_polygon(cpoly, ctmp, 3, &poly);
+            for (int k = 0; k < 3; k++) cv_store(cpoly[k], v[k]);
+            CVtx<VF>* poly = cpoly;              // the buffer holding the result (either one)
+            int n = clip_polygon<VF>(cpoly, ctmp, 3, &poly);
             if (n < 3) continue;
             for (int k = 0; k < n; k++) {
-                Vtx q = poly[k];
+                Vtx q = cv_load(poly[k]);
                 q.hom[2] = __fdiv_rn(1.0f, q.hom[2]);
                 q.uv[0] = q.uv[0] * q.hom[2];
                 q.uv[1] = q.uv[1] * q.hom[2];
                 float invW = __fdiv_rn(1.0f, q.hom[3]);
                 q.hom[0] *= invW;
                 q.hom[1] *= invW;
-                to_screen(stl, q);
-                poly[k] = q;
+                cv_store(poly[k], q);
             }
             for (int i = 0; i < n - 2; i++) {                             // triangles.c:113-115
-                const Vtx a = poly[0], b = poly[i + 1], c = poly[i + 2];
+                Vtx a = cv_load(poly[0]), b = cv_load(poly[i + 1]), c = cv_load(poly[i + 2]);
+                to_screen(stl, a);
+                to_screen(stl, b);
+                to_screen(stl, c);
                 if (emit_raster<PHASE>(A, st, F, si, t, face, true, a, b, c, rtBase + nRT, pairBase + nPairs, np)) {
                     nRT++;
                     nPairs += np;
@@ -1016,7 +1051,8 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         todo &= ~__builtin_amdgcn_ballot_w64(mine);
         if (mine)
             deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, rtBase, pairBase,
-                                                         nRT, nPairs, nullptr, nullptr);
+                                                         nRT, nPairs, (CVtx<VF_FULL>*)nullptr,
+                                                         (CVtx<VF_FULL>*)nullptr);
     }
     if (PHASE == 0) {
         // List the triangles to clip: one counter update per wave.
@@ -1042,18 +1078,21 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
 
 // The listed triangles that need clipping, one per lane, each lane
 // clipping in its own LDS area (so no per-lane private arrays: no
-// scratch).  One wave per workgroup, one workgroup per CU; the list is
-// walked with a grid stride.  PHASE 0 stores the counts and adds them to
-// the block sums; PHASE 1 writes from the slots k_setup<1> handed over.
-constexpr int CLIP_WORDS = 2 * 12 * (int)(sizeof(Vtx) / 4) + 1;   // odd: lanes fall in distinct banks
-constexpr int CLIP_GRID = 256;
+// scratch), in the smallest vertex form the pass needs.  One wave per
+// workgroup; the list is walked with a grid stride.  PHASE 0 stores the
+// counts and adds them to the block sums; PHASE 1 writes from the slots
+// k_setup<1> handed over.
+template <int VF>
+constexpr int clip_words() { return 2 * 12 * (int)(sizeof(CVtx<VF>) / 4) + 1; }   // odd: lanes in distinct banks
+constexpr int CLIP_GRID = 1024;
 
-template <int PHASE, bool LIT>
+template <int PHASE, bool LIT, int VF>
 __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
+    constexpr int CLIP_WORDS = clip_words<VF>();
     __shared__ uint32_t mem[64 * CLIP_WORDS];
     const int lane = threadIdx.x;
-    Vtx* poly = reinterpret_cast<Vtx*>(&mem[lane * CLIP_WORDS]);
-    Vtx* tmp = poly + 12;
+    CVtx<VF>* poly = reinterpret_cast<CVtx<VF>*>(&mem[lane * CLIP_WORDS]);
+    CVtx<VF>* tmp = poly + 12;
     const uint32_t n = *A.clipCount;
     for (uint32_t i = blockIdx.x * 64 + lane; i < n; i += CLIP_GRID * 64) {
         const uint32_t gid = A.clipList[i];
@@ -1062,8 +1101,8 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
         pass_triangle(A, gid, t, info, si, F);
         uint32_t nRT = 0, nPairs = 0;
         const uint32_t rtBase = PHASE == 1 ? A.cntRT[gid] : 0u, pairBase = PHASE == 1 ? A.cntPairs[gid] : 0u;
-        setup_triangle<PHASE, LIT, true>(A, as_const(A.states + si), si, F, t, info, rtBase, pairBase, nRT, nPairs,
-                                         poly, tmp);
+        setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, rtBase, pairBase, nRT,
+                                             nPairs, poly, tmp);
         if (PHASE == 0) {
             A.cntRT[gid] = nRT;
             A.cntPairs[gid] = nPairs;
@@ -1482,10 +1521,18 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         for (int j = 0; j < nt; j++) {
             const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
             const bool exact = (t.meta & META_EXACT) != 0;
+            // A lane's pixels are RS rows apart: each edge value steps by
+            // RS * B, and stepping mod 2^32 gives the direct value exactly.
+            int32_t w1, w2, w3;
+            edge_values(t, px, py[0], w1, w2, w3);
+            const uint32_t d1 = (uint32_t)t.B[0] * RS, d2 = (uint32_t)t.B[1] * RS, d3 = (uint32_t)t.B[2] * RS;
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                int32_t w1, w2, w3;
-                edge_values(t, px, py[k], w1, w2, w3);
+                if (k > 0) {
+                    w1 = (int32_t)((uint32_t)w1 + d1);
+                    w2 = (int32_t)((uint32_t)w2 + d2);
+                    w3 = (int32_t)((uint32_t)w3 + d3);
+                }
                 const bool cov = in[k] && ((w1 | w2 | w3) > 0) && px < t.xMax
                                  && (exact || (px >= t.xMin && py[k] >= t.yMin && py[k] <= t.yMax));
                 const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
@@ -1533,13 +1580,20 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
-// i mod 8), and each XCD has its own L2.  This maps workgroup i to a
-// contiguous chunk of work per XCD, so neighbouring tiles -- which share
-// raster-triangle records -- run on the same XCD and read them through the
-// same L2.  A bijection of [0, n).
+// i mod 8), and each XCD has its own L2.  This maps the workgroups of each
+// XCD to chunks of XCD_CHUNK consecutive work items, the chunks themselves
+// dealt round-robin: neighbouring tiles -- which share raster-triangle
+// records -- mostly run on one XCD and read them through one L2, while
+// every XCD still gets work from all over the frame (one contiguous band
+// per XCD left some XCDs with most of the dense tiles).  A bijection of
+// [0, n): the last n mod (8 XCD_CHUNK) items keep their index.
+constexpr uint32_t XCD_CHUNK = 16;
+
 __device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
-    const uint32_t q = n / 8, r = n % 8, x = i % 8, j = i / 8;
-    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+    const uint32_t span = 8 * XCD_CHUNK, full = n - n % span;
+    if (i >= full) return i;
+    const uint32_t x = i % 8, j = i / 8;
+    return ((j / XCD_CHUNK) * 8 + x) * XCD_CHUNK + j % XCD_CHUNK;
 }
 
 // Four waves per workgroup.  A wave renders blocks of BW x BW pixels
@@ -2135,7 +2189,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
     hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-    hipLaunchKernelGGL((k_clip<0, false>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+    hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
                        (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
@@ -2163,10 +2217,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (numRT) {
         if (J->anyLit) {
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
-            hipLaunchKernelGGL((k_clip<1, true>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
         } else {
             hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-            hipLaunchKernelGGL((k_clip<1, false>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
         }
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);

@@ -2,6 +2,7 @@
 
     python -m pixelforge_amd.profsum trace DB          kernel-trace db -> per (kernel, grid) durations
     python -m pixelforge_amd.profsum pmc DIR...        --pmc csv dirs -> per (kernel, grid) counter means
+    python -m pixelforge_amd.profsum md DB TITLE CMD [GRID...]   markdown table of a kernel trace
     python -m pixelforge_amd.profsum traffic FETCH_DIR WRITE_DIR CAL_FETCH_DIR CAL_WRITE_DIR BYTES KERNEL GRID
         HBM bytes per dispatch of KERNEL (name substring) at GRID from
         FETCH_SIZE / WRITE_SIZE runs, each scaled by the same counter of the
@@ -72,6 +73,18 @@ def main():
             print(f"{n}  grid {g}")
             for c, v in sorted(d.items()):
                 print(f"    {c:28s} {v:14.6g}")
+    elif sys.argv[1] == "md":
+        # md DB TITLE COMMAND [GRID...]: markdown table of the kernel trace (optionally only some grids)
+        db, title, cmd = sys.argv[2], sys.argv[3], sys.argv[4]
+        grids = {int(g) for g in sys.argv[5:]}
+        print(f"# {title}\n\nCommand (GPU box, repository root; `cd /tmp && export TMPDIR=/tmp` first):\n")
+        print(f"    {cmd}\n")
+        print("Per kernel and grid size (work-items): mean and median duration over the dispatches of the run.\n")
+        print("| kernel | grid | calls | mean µs | median µs |\n|---|---:|---:|---:|---:|")
+        for n, gx, c, mean, med, tot in trace(db):
+            if grids and gx not in grids:
+                continue
+            print(f"| `{n}` | {gx} | {c} | {mean:.2f} | {med:.2f} |")
     elif sys.argv[1] == "traffic":
         a = sys.argv[2:]
         print(json.dumps(traffic(a[0], a[1], a[2], a[3], float(a[4]), a[5], int(a[6])), indent=1))
