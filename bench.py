@@ -52,7 +52,7 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libpforacle.so")
 PEAK_HBM_GBS = 8000.0
 SETUP_BYTES_PER_TRI = 88          # position 3x16 + colour 3x4 + texcoord 3x8 + info 4
-RASTER_MODES = ["packed", "in-place", "packed, 4 px/lane", "in-place, 4 px/lane"]
+RASTER_MODES = ["packed", "in-place", "packed, 4 px/lane", "in-place, 4 px/lane", "packed, bbox candidates"]
 
 
 def workload(width=1920, height=1080, seed=2):
@@ -194,7 +194,7 @@ def bench_batch(args):
     job = cap.job
     lib = pf.lib()
     check(lib.pfdJobUploadTargets(job), "upload")
-    cfg = configure(job, 0, 0)
+    cfg = configure(job, args.batch_tile_shift, args.batch_raster_mode)
     elapsed = time_replays(job, args.batch_steps, 3)
     st = stats(job)
     setup_ms, bin_ms, raster_ms = phase_times(job, max(3, args.profile_runs // 4))
@@ -315,7 +315,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-shift", type=int, default=0, help="3..6 fixes the tile size; 0 autotunes")
-    ap.add_argument("--raster-mode", type=int, default=0, help="with --tile-shift: 0 packed, 1 in-place")
+    ap.add_argument("--raster-mode", type=int, default=0, help="with --tile-shift: raster mode (pf_device.h)")
+    ap.add_argument("--batch-tile-shift", type=int, default=0, help="the same for the batched job")
+    ap.add_argument("--batch-raster-mode", type=int, default=0)
     ap.add_argument("--profile-runs", type=int, default=20)
     ap.add_argument("--contexts", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)

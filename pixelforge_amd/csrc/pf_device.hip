@@ -134,8 +134,8 @@ enum : uint32_t {
 
 // Smallest and largest true (64-bit) value of w = A*x + B*y + C over the
 // pixels [x0, x1] x [y0, y1] (a linear function: extremes at corners).
-__device__ __forceinline__ void edge_range(int32_t A, int32_t B, int32_t C, int32_t x0, int32_t x1, int32_t y0,
-                                           int32_t y1, int64_t& lo, int64_t& hi) {
+__device__ __forceinline__ void edge_range(int32_t A, int32_t B, int32_t C, int64_t x0, int64_t x1, int64_t y0,
+                                           int64_t y1, int64_t& lo, int64_t& hi) {
     const int64_t ax0 = (int64_t)A * x0, ax1 = (int64_t)A * x1, by0 = (int64_t)B * y0, by1 = (int64_t)B * y1;
     lo = (ax0 < ax1 ? ax0 : ax1) + (by0 < by1 ? by0 : by1) + C;
     hi = (ax0 < ax1 ? ax1 : ax0) + (by0 < by1 ? by1 : by0) + C;
@@ -689,7 +689,7 @@ struct SetupArgs {
     uint32_t*       vals;           // raster triangle of each pair
     uint32_t        capRT;
     uint32_t        capPairs;
-    uint32_t*       clipList;       // pass triangles that need clipping (PHASE 0 appends)
+    uint4*          clipList;       // triangles to clip {gid, t, info, frame} (PHASE 0 appends)
     uint32_t*       clipCount;
 };
 
@@ -729,22 +729,36 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     const uint32_t ts = A.tileShift;
     const uint32_t tx0 = (uint32_t)cx0 >> ts, tx1 = (uint32_t)cx1 >> ts;
     const uint32_t ty0 = (uint32_t)cy0 >> ts, ty1 = (uint32_t)cy1 >> ts;
-    pairsOut = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-    if (PHASE == 0) return true;
-    if (rtSlot >= A.capRT) return false;
 
     uint32_t Ax[3] = { uy3 - uy2, uy1 - uy3, uy2 - uy1 };
     uint32_t By[3] = { ux2 - ux3, ux3 - ux1, ux1 - ux2 };
     if (face == 1) for (int k = 0; k < 3; k++) { Ax[k] = 0u - Ax[k]; By[k] = 0u - By[k]; }
     uint32_t ox[3] = { ux2, ux3, ux1 }, oy[3] = { uy2, uy3, uy1 };
+    // Tile pairs: every tile of the clipped bbox (the raster kernel rejects
+    // the blocks an edge leaves uncovered; testing tiles here measured
+    // slower, the per-lane loops over long thin triangles' spans dominate).
+    pairsOut = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    if (PHASE == 0) return true;
+    if (rtSlot >= A.capRT) return false;
+    uint32_t Cx[3];
+    for (int k = 0; k < 3; k++) Cx[k] = 0u - (ox[k] * Ax[k] + oy[k] * By[k]);
+    // No int32 wrap of any edge within 64 px of the bbox (then a block of a
+    // tile of the span has its largest edge value at a corner).
+    bool nowrap = true;
+    for (int k = 0; k < 3; k++) {
+        int64_t lo, hi;
+        edge_range((int32_t)Ax[k], (int32_t)By[k], (int32_t)Cx[k], (int64_t)xMin - 64, (int64_t)xMax + 64,
+                   (int64_t)yMin - 64, (int64_t)yMax + 64, lo, hi);
+        nowrap = nowrap && lo >= INT32_MIN && hi <= INT32_MAX;
+    }
+
     RTri r;
     uint32_t sum = 0;
     for (int k = 0; k < 3; k++) {
-        uint32_t Ck = 0u - (ox[k] * Ax[k] + oy[k] * By[k]);
         r.A[k] = (int32_t)Ax[k];
         r.B[k] = (int32_t)By[k];
-        r.C[k] = (int32_t)Ck;
-        sum += Ax[k] * (uint32_t)xMin + By[k] * (uint32_t)yMin + Ck;
+        r.C[k] = (int32_t)Cx[k];
+        sum += Ax[k] * (uint32_t)xMin + By[k] * (uint32_t)yMin + Cx[k];
     }
     r.xMin = xMin; r.yMin = yMin; r.xMax = xMax; r.yMax = yMax;
     r.wInvSum = __fdiv_rn(1.0f, (float)(int32_t)sum);
@@ -757,15 +771,7 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         small = small && va > -(1 << 23) && va < (1 << 23) && vb > -(1 << 23) && vb < (1 << 23);
     }
     if (small) meta |= META_SMALL;
-    {
-        bool nowrap = true;
-        for (int k = 0; k < 3; k++) {
-            int64_t lo, hi;
-            edge_range(r.A[k], r.B[k], r.C[k], xMin - 64, xMax + 64, yMin - 64, yMax + 64, lo, hi);
-            nowrap = nowrap && lo >= INT32_MIN && hi <= INT32_MAX;
-        }
-        if (nowrap) meta |= META_NOWRAP | (is3D ? META_EXACT : 0u);
-    }
+    if (nowrap) meta |= META_NOWRAP | (is3D ? META_EXACT : 0u);
     const uint32_t flags = st->flags;
     if (st->shadeModel == 1) meta |= META_SMOOTH;
     if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st->depthFunc & 7u) << META_DFUNC_SH);
@@ -1005,11 +1011,12 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
 // Recorded triangle, state and frame of pass-triangle gid: the last
 // segment whose passTriBase <= gid, found forward from its block's first.
 __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, uint32_t& t, uint32_t& info,
-                                              uint32_t& si, DFrame& F) {
+                                              uint32_t& si, DFrame& F, uint32_t& frame) {
     uint32_t lo = A.blockSeg[gid / 256];
     while (lo + 1 < A.numSegs && A.segs[lo + 1].passTriBase <= gid) lo++;
     const DSeg sg = A.segs[lo];
     t = sg.triFirst + (gid - sg.passTriBase);
+    frame = sg.frame;
     F = A.frames[sg.frame];
     info = A.info[t];
     si = info & PFF_STATE_MASK;
@@ -1037,9 +1044,9 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         pairBase = block_excl_scan(cP, lds, tot) + A.blockPairs[blockIdx.x];
     }
     uint32_t nRT = 0, nPairs = 0;
-    uint32_t t = 0, info = 0, si = 0;
+    uint32_t t = 0, info = 0, si = 0, frame = 0;
     DFrame F;
-    if (valid) pass_triangle(A, gid, t, info, si, F);
+    if (valid) pass_triangle(A, gid, t, info, si, F, frame);
     // One iteration per distinct state in the wavefront: the state is then
     // uniform and read with scalar loads.
     uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
@@ -1063,7 +1070,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
             if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
             base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
             const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            if (deferred) A.clipList[base + __popcll(D & ltMask)] = gid;
+            if (deferred) A.clipList[base + __popcll(D & ltMask)] = make_uint4(gid, t, info, frame);
         }
         if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
         uint32_t totR, totP;
@@ -1095,10 +1102,12 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
     CVtx<VF>* tmp = poly + 12;
     const uint32_t n = *A.clipCount;
     for (uint32_t i = blockIdx.x * 64 + lane; i < n; i += CLIP_GRID * 64) {
-        const uint32_t gid = A.clipList[i];
-        uint32_t t, info, si;
-        DFrame F;
-        pass_triangle(A, gid, t, info, si, F);
+        // The listed triangle with its recorded index, info and frame, so
+        // that its inputs are fetched without the segment lookup's chain
+        // of dependent loads.
+        const uint4 cr = A.clipList[i];
+        const uint32_t gid = cr.x, t = cr.y, info = cr.z, si = info & PFF_STATE_MASK;
+        const DFrame F = A.frames[cr.w];
         uint32_t nRT = 0, nPairs = 0;
         const uint32_t rtBase = PHASE == 1 ? A.cntRT[gid] : 0u, pairBase = PHASE == 1 ? A.cntPairs[gid] : 0u;
         setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, rtBase, pairBase, nRT,
@@ -1309,8 +1318,6 @@ struct WaveQueueT {
     float    fz[64];
     uint8_t  ftri[64];
     uint8_t  fpix[64];
-    uint8_t  fnext[64];
-    uint8_t  pad[64];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -1319,7 +1326,8 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void edge_values(const RTri& t, int px, int py, int32_t& w1, int32_t& w2, int32_t& w3) {
+template <typename TRI>
+__device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_t& w1, int32_t& w2, int32_t& w3) {
     if (t.meta & META_SMALL) {
         // 24x24-bit multiplies keep the low 32 bits of the product: the same
         // value mod 2^32 as the full multiply when |A|, |B| < 2^23.
@@ -1400,10 +1408,11 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
 
 // Shades the queued fragments and applies them in order (packed mode).
 // Queued pixel p of a TW-wide block is (bx0 + p % TW, by0 + p / TW); lane
-// l owns pixels l + 64k, k < K, with chains head[k].
+// l owns pixels l + 64k, k < K, whose fragments sit in the queue slots set
+// in pm[k] (slots grow in submission order).
 template <bool PHONG, int K>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
-                                            int nf, const int* head, uint32_t* col, float* dep, uint32_t* win) {
+                                            int nf, uint64_t* pm, uint32_t* col, float* dep, uint32_t* win) {
     constexpr int TW = (K == 1) ? 8 : 16;
     wave_sync();
     if (lane < nf) {
@@ -1419,13 +1428,16 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
     }
     wave_sync();
 #pragma unroll
-    for (int k = 0; k < K; k++)
-        for (int s = head[k]; s >= 0;) {
+    for (int k = 0; k < K; k++) {
+        uint64_t m = pm[k];
+        pm[k] = 0ull;
+        while (m) {
+            const int s = __builtin_ctzll(m);
+            m &= m - 1ull;
             const int j = Q.ftri[s];
             rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.recId[j], col[k], dep[k], win[k]);
-            const int nx = Q.fnext[s];
-            s = (nx == 0xFF) ? -1 : nx;
         }
+    }
     wave_sync();
 }
 
@@ -1515,9 +1527,10 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             }
         }
         wave_sync();
-        int nf = 0, head[K], tail[K];
+        int nf = 0;
+        uint64_t pm[K];                    // per owned pixel: its queue slots
 #pragma unroll
-        for (int k = 0; k < K; k++) head[k] = tail[k] = -1;
+        for (int k = 0; k < K; k++) pm[k] = 0ull;
         for (int j = 0; j < nt; j++) {
             const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
             const bool exact = (t.meta & META_EXACT) != 0;
@@ -1550,23 +1563,19 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     continue;
                 }
                 if (nf + c > 64) {
-                    flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
+                    flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
                     nf = 0;
-#pragma unroll
-                    for (int q = 0; q < K; q++) head[q] = tail[q] = -1;
                 }
                 if (cov) {
                     const int slot = nf + (int)__popcll(M & ltMask);
                     Q.ftri[slot] = (uint8_t)j;
                     Q.fpix[slot] = (uint8_t)(lane + 64 * k);
-                    Q.fnext[slot] = 0xFF;
-                    if (tail[k] >= 0) Q.fnext[tail[k]] = (uint8_t)slot; else head[k] = slot;
-                    tail[k] = slot;
+                    pm[k] |= 1ull << slot;
                 }
                 nf += c;
             }
         }
-        if (MODE == 0 && nf > 0) flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, head, col, dep, win);
+        if (MODE == 0 && nf > 0) flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
         wave_sync();
     }
 #pragma unroll
@@ -1578,6 +1587,190 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         }
     return frags;
 }
+
+// Compacted mode (4): one 8x8 block per wave, as mode 0, but coverage is
+// tested on candidate pixels instead of on every pixel of the block for
+// every staged triangle.  A staged triangle's candidates are the pixels of
+// its bounding box inside the block and the target (the bbox test of the
+// coverage rule, so only the edge test is left); the candidates of a batch
+// are numbered in list order, triangle by triangle, row by row, and tested
+// 64 at a time.  Covered candidates enter the fragment queue in that order,
+// so the fragments of one pixel stay in submission order; each pixel keeps
+// a mask of its queue slots, applied lowest slot first.  Small triangles
+// (a few pixels per block) then cost a few lanes instead of a wavefront.
+template <bool PHONG>
+struct WaveQueueC {
+    uint32_t rec[BATCH][REC_STRIDE];        // staged RTri records
+    uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
+    uint32_t recId[BATCH];                  // raster-triangle index of each staged record
+    uint32_t cinfo[BATCH];                  // first candidate | x0 << 12 | y0 << 15 | (width - 1) << 18
+    uint32_t fcol[64];
+    float    fz[64];
+    unsigned long long pmask[64];           // per pixel: its queue slots
+    uint8_t  ftri[64];
+    uint8_t  fpix[64];
+};
+
+template <bool PHONG>
+__device__ __forceinline__ void flush_queue_c(const RasterArgs& A, WaveQueueC<PHONG>& Q, int bx0, int by0, int lane,
+                                              int nf, uint32_t& col, float& dep, uint32_t& win) {
+    wave_sync();
+    if (lane < nf) {
+        const int j = Q.ftri[lane];
+        const int p = Q.fpix[lane];
+        const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+        uint32_t c;
+        float z;
+        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p & 7),
+                              by0 + (p >> 3), c, z);
+        Q.fcol[lane] = c;
+        Q.fz[lane] = z;
+    }
+    wave_sync();
+    unsigned long long m = Q.pmask[lane];
+    Q.pmask[lane] = 0ull;
+    while (m) {
+        const int s = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const int j = Q.ftri[s];
+        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.recId[j], col, dep, win);
+    }
+    wave_sync();
+}
+
+template <bool PHONG>
+__device__ __forceinline__ uint32_t raster_block_c(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
+                                                   int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueC<PHONG>& Q) {
+    const int lane = threadIdx.x & 63;
+    const int px = bx0 + (lane & 7), py = by0 + (lane >> 3);
+    const bool in = px < (int)F.W && py < (int)F.H;
+    const size_t npix = (size_t)F.W * F.H;
+    const size_t aligned = npix - (npix % 8);
+    const size_t pi = in ? (size_t)py * F.W + px : 0;
+    uint32_t col = 0u, win = 0u;
+    float dep = 0.0f;
+    if (in) {
+        if (sg.clear && pi >= 8 && pi < aligned) {                 // context.c:693-713
+            col = sg.clearColor;
+            dep = sg.clearDepth;
+        } else {
+            const size_t src = (sg.clear && pi >= aligned) ? 0 : pi;
+            col = F.color[src];
+            dep = F.depth[src];
+        }
+    }
+    // Last column and row of the block inside the target.
+    const int xLim = min(bx0 + 7, (int)F.W - 1), yLim = min(by0 + 7, (int)F.H - 1);
+    Q.pmask[lane] = 0ull;
+    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t frags = 0;
+    for (uint32_t e = e0; e < e1; e += BATCH) {
+        const uint32_t nb = min((uint32_t)BATCH, e1 - e);
+        bool ov = false;
+        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
+        uint4 q0 = z4, q1 = z4, q2 = z4, q3 = z4, q4 = z4, q5 = z4, q6 = z4, q7 = z4;
+        uint32_t r = 0, cnt = 0, rect = 0;
+        if ((uint32_t)lane < nb) {
+            r = A.vals[e + lane];
+            const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
+            q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
+            q4 = src[4]; q5 = src[5]; q6 = src[6]; q7 = src[7];
+            const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
+            const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
+            const int32_t rx0 = max(bx0, xMin), rx1 = min(xLim, xMax - 1);
+            const int32_t ry0 = max(by0, yMin), ry1 = min(yLim, yMax);
+            ov = rx0 <= rx1 && ry0 <= ry1;
+            // An edge negative over the whole block leaves it uncovered (see
+            // raster_block: NOWRAP keeps w free of int32 wrap near the bbox).
+            if (ov && (q5.x & META_NOWRAP)) {
+                const int32_t xs[2] = { bx0, bx0 + 7 }, ys[2] = { by0, by0 + 7 };
+                const int32_t eA[3] = { (int32_t)q0.x, (int32_t)q0.y, (int32_t)q0.z };
+                const int32_t eB[3] = { (int32_t)q0.w, (int32_t)q1.x, (int32_t)q1.y };
+                const int32_t eC[3] = { (int32_t)q1.z, (int32_t)q1.w, (int32_t)q2.x };
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int32_t hi = eA[k] * xs[eA[k] > 0] + eB[k] * ys[eB[k] > 0] + eC[k];
+                    ov = ov && hi >= 0;
+                }
+            }
+            if (ov) {
+                const int32_t w = rx1 - rx0 + 1;
+                cnt = (uint32_t)(w * (ry1 - ry0 + 1));
+                rect = ((uint32_t)(rx0 - bx0) << 12) | ((uint32_t)(ry0 - by0) << 15) | ((uint32_t)(w - 1) << 18);
+            }
+        }
+        const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
+        const int nt = __popcll(OV);
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+        if (ov) {
+            const int slot = (int)__popcll(OV & ltMask);
+            uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
+            dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
+            dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
+            Q.recId[slot] = r;
+            Q.cinfo[slot] = (incl - cnt) | rect;
+            if (PHONG && (q5.x & META_PHONG)) {
+                const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
+                uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
+#pragma unroll
+                for (int k = 0; k < 5; k++) ld[k] = ls[k];
+            }
+        }
+        wave_sync();
+        int nf = 0;
+        for (uint32_t base = 0; base < total; base += 64) {
+            const uint32_t s = base + (uint32_t)lane;
+            bool cov = false;
+            int j = 0, lp = 0;
+            if (s < total) {
+                // The staged record holding candidate s: the last one whose
+                // first candidate is <= s (first candidates increase).
+#pragma unroll
+                for (int step = BATCH / 2; step > 0; step >>= 1)
+                    if (j + step < nt && (Q.cinfo[j + step] & 0xFFFu) <= s) j += step;
+                const uint32_t ci = Q.cinfo[j];
+                const int q = (int)(s - (ci & 0xFFFu));
+                const int w = (int)((ci >> 18) & 7u) + 1;
+                // q / w for q < 64, w <= 8: (q + 0.5) / w lies at least 1/16
+                // from an integer, far beyond the reciprocal's error.
+                const int qy = (int)(((float)q + 0.5f) * __builtin_amdgcn_rcpf((float)w));
+                const int lx = (int)((ci >> 12) & 7u) + (q - qy * w), ly = (int)((ci >> 15) & 7u) + qy;
+                const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+                int32_t w1, w2, w3;
+                edge_values(t, bx0 + lx, by0 + ly, w1, w2, w3);
+                cov = (w1 | w2 | w3) > 0;
+                lp = ly * 8 + lx;
+            }
+            const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
+            if (!M) continue;
+            const int c = __popcll(M);
+            frags += (uint32_t)c;
+            if (nf + c > 64) {
+                flush_queue_c<PHONG>(A, Q, bx0, by0, lane, nf, col, dep, win);
+                nf = 0;
+            }
+            if (cov) {
+                const int slot = nf + (int)__popcll(M & ltMask);
+                Q.ftri[slot] = (uint8_t)j;
+                Q.fpix[slot] = (uint8_t)lp;
+                atomicOr(&Q.pmask[lp], 1ull << slot);
+            }
+            nf += c;
+        }
+        if (nf > 0) flush_queue_c<PHONG>(A, Q, bx0, by0, lane, nf, col, dep, win);
+        wave_sync();
+    }
+    if (in) {
+        F.color[pi] = col;
+        F.depth[pi] = dep;
+        if (A.winner) A.winner[pi] = win;
+    }
+    return frags;
+}
+
+template <int MODE, bool PHONG> struct RasterQueue { typedef WaveQueueT<PHONG> type; };
+template <bool PHONG> struct RasterQueue<4, PHONG> { typedef WaveQueueC<PHONG> type; };
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
 // i mod 8), and each XCD has its own L2.  This maps the workgroups of each
@@ -1602,13 +1795,14 @@ __device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
 // workgroup, its blocks dealt round-robin to the four waves.
 template <int TS, int MODE, bool PHONG, int K>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
-    __shared__ WaveQueueT<PHONG> Qs[4];
+    typedef typename RasterQueue<MODE, PHONG>::type Queue;
+    __shared__ Queue Qs[4];
     constexpr int BW = (K == 1) ? 8 : 16;
     static_assert(TS >= BW, "tile smaller than a block");
     constexpr int BPR = TS / BW;
     constexpr int NB = BPR * BPR;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    WaveQueueT<PHONG>& Q = Qs[wave];
+    Queue& Q = Qs[wave];
     const uint32_t wg = xcd_chunked(blockIdx.x, gridDim.x);
     const uint32_t tile = (NB == 1) ? wg * 4 + wave : wg;
     if (tile >= A.numTiles) return;
@@ -1625,7 +1819,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
         const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
+        if constexpr (MODE == 4) frags += raster_block_c<PHONG>(A, F, sg, bx0, by0, e0, e1, Q);
+        else frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -2147,7 +2342,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if ((rc = ensure(J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
     if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
     if ((rc = ensure(J->scanTotal, 16))) return rc;
-    if ((rc = ensure(J->clipList, (size_t)n * 4 + 4)) || (rc = ensure(J->clipCount, 16))) return rc;
+    if ((rc = ensure(J->clipList, (size_t)n * 16 + 16)) || (rc = ensure(J->clipCount, 16))) return rc;
     const size_t fragSlots = (size_t)J->numTiles * 4;
     const size_t passIdx = (size_t)(&P - J->passes.data());
     if (discover) {
@@ -2183,7 +2378,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.blockPairs = (uint32_t*)J->blockPairs.p;
     sa.tileBegin = (uint32_t*)J->tileBegin.p;
     sa.tileEnd = (uint32_t*)J->tileEnd.p;
-    sa.clipList = (uint32_t*)J->clipList.p;
+    sa.clipList = (uint4*)J->clipList.p;
     sa.clipCount = (uint32_t*)J->clipCount.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
@@ -2241,7 +2436,9 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         for (uint32_t shift = 0; shift < bits; shift += 8) {
             hipLaunchKernelGGL(k_radix_hist, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb, (uint32_t*)J->hist.p);
             const uint32_t* scanned;
-            if (nh <= 65536) {
+            // One block scans 4096 values per step: above two steps the
+            // three-kernel scan is faster.
+            if (nh <= 2 * SCAN_TILE) {
                 hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, (uint32_t*)J->hist.p,
                                    (uint32_t*)nullptr, nh, (uint32_t*)J->scanTotal.p + 2);
                 scanned = (const uint32_t*)J->hist.p;
@@ -2279,6 +2476,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
+    case 4: launch_raster_ts<4, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
@@ -2714,16 +2912,26 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if (!J) return PFD_E_INVALID;
     float best = 1e30f;
     PFDconfig bestCfg = J->cfg;
-    for (uint32_t mode = 0; mode <= 3; mode++)
-        for (uint32_t ts = (mode >= 2 ? 4 : 3); ts <= 6; ts++) {   // four pixels per lane needs 16x16 tiles
+    for (uint32_t mode = 0; mode <= 4; mode++)
+        for (uint32_t ts = (mode == 2 || mode == 3 ? 4 : 3); ts <= 6; ts++) {   // four pixels per lane: 16x16 tiles
             PFDconfig c = J->cfg;
             c.tileShift = ts;
             c.rasterMode = mode;
             int rc = pfdJobSetConfig(J, &c);
             if (rc) return rc;
             if ((rc = pfdJobUploadTargets(J))) return rc;
-            float ms = 0;
-            if ((rc = pfdJobTimeReplays(J, 5, &ms))) return rc;
+            // Replays worth about 10 ms, best of three trials: single short
+            // timings mis-ranked configurations a few percent apart.
+            float est = 0;
+            if ((rc = pfdJobTimeReplays(J, 3, &est))) return rc;
+            const float per = est / 3.0f;
+            const uint32_t iters = per > 0.0f ? (uint32_t)fminf(50.0f, fmaxf(3.0f, 10.0f / per)) : 3u;
+            float ms = 1e30f;
+            for (int trial = 0; trial < 3; trial++) {
+                float t = 0;
+                if ((rc = pfdJobTimeReplays(J, iters, &t))) return rc;
+                ms = fminf(ms, t / (float)iters);
+            }
             if (ms < best) { best = ms; bestCfg = c; }
         }
     PFDconfig c = bestCfg;

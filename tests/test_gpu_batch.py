@@ -27,7 +27,7 @@ def test_batch_equals_single_context_renders():
 
 
 @pytest.mark.parametrize("tile_shift", [3, 4, 5, 6])
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_batch_job_every_tile_size(tile_shift, mode):
     params = pf.batch_workload(10, seed=8, max_side=200, max_tris=2000, max_tex=96)
     ref = [render_oracle(p)[0] for p in params]
