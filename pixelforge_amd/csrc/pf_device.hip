@@ -1122,24 +1122,12 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// Device-wide exclusive scan of uint32 (reduce, top, down)
+// Exclusive scan of the per-block sums of the setup (one workgroup)
 // ---------------------------------------------------------------------------
 
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
-                                                              uint32_t* __restrict__ blockSums) {
-    __shared__ uint32_t lds[64];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) if (base + i < n) s += in[base + i];
-    uint32_t total;
-    block_excl_scan(s, lds, total);
-    if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
-}
 
 // One block scans an array in place (exclusive) and writes its total;
 // with a second array (b != nullptr) it scans that one too.
@@ -1163,20 +1151,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* __restrict_
         }
         if (threadIdx.x == 0) totalOut[which] = carry;
     }
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* __restrict__ in, uint32_t n,
-                                                            const uint32_t* __restrict__ blockSums,
-                                                            uint32_t* __restrict__ out) {
-    __shared__ uint32_t lds[64];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS], s = 0;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) { v[i] = (base + i < n) ? in[base + i] : 0u; s += v[i]; }
-    uint32_t total;
-    uint32_t ex = block_excl_scan(s, lds, total) + blockSums[blockIdx.x];
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) { if (base + i < n) out[base + i] = ex; ex += v[i]; }
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
@@ -1206,18 +1180,48 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
+// One workgroup per digit d: the exclusive prefix over radix blocks of
+// hist[d * nb + b], in place, and the digit's total.  With the totals the
+// scatter forms each block's bases itself, so a pass needs no device-wide
+// scan.
+__global__ __launch_bounds__(RADIX_THREADS) void k_radix_colscan(uint32_t* __restrict__ hist, uint32_t nb,
+                                                                 uint32_t* __restrict__ digitTotal) {
+    __shared__ uint32_t lds[64];
+    constexpr int ITEMS = 8;
+    uint32_t* col = hist + (size_t)blockIdx.x * nb;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < nb; c0 += RADIX_THREADS * ITEMS) {
+        const uint32_t base = c0 + threadIdx.x * ITEMS;
+        uint32_t v[ITEMS], sum = 0;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) { v[i] = (base + i < nb) ? col[base + i] : 0u; sum += v[i]; }
+        uint32_t total;
+        uint32_t ex = block_excl_scan(sum, lds, total) + carry;
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) { if (base + i < nb) col[base + i] = ex; ex += v[i]; }
+        carry += total;
+    }
+    if (threadIdx.x == 0) digitTotal[blockIdx.x] = carry;
+}
+
 // Stable scatter: items of a block are taken in index order, 256 per round;
 // an item's rank among equal digits is its rank within its wave (ballot
 // match on the 8 digit bits) plus the counts of lower waves and of earlier
 // rounds.
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                                                                  uint32_t n, uint32_t shift, uint32_t nb,
-                                                                 const uint32_t* __restrict__ histScan,
+                                                                 const uint32_t* __restrict__ colPrefix,
+                                                                 const uint32_t* __restrict__ digitTotal,
                                                                  uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
     __shared__ uint32_t base[256];
     __shared__ uint32_t cnt[RADIX_THREADS / 64][256];
+    __shared__ uint32_t lds[64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    base[threadIdx.x] = histScan[(size_t)threadIdx.x * nb + blockIdx.x];
+    // Items of digit d start at (items of lower digits) + (items of digit d
+    // in earlier blocks).
+    uint32_t all;
+    const uint32_t lower = block_excl_scan(digitTotal[threadIdx.x], lds, all);
+    base[threadIdx.x] = lower + colPrefix[(size_t)threadIdx.x * nb + blockIdx.x];
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     size_t blockBase = (size_t)blockIdx.x * RADIX_TILE;
     for (int r = 0; r < RADIX_ROUNDS; r++) {
@@ -2286,19 +2290,6 @@ struct PFDjob {
 
 namespace {
 
-int scan_u32(PFDjob* J, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* totalDev) {
-    uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nb == 0) nb = 1;
-    int rc = ensure(J->scanBlocks, (size_t)nb * 4);
-    if (rc) return rc;
-    uint32_t* bs = (uint32_t*)J->scanBlocks.p;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, bs);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, bs, (uint32_t*)nullptr, nb, totalDev);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, J->stream, in, n, (const uint32_t*)bs, out);
-    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
-    return 0;
-}
-
 uint32_t read_total(PFDjob* J, int* rc) {
     uint32_t v = 0;
     hipError_t e = hipMemcpyAsync(&v, J->scanTotal.p, 4, hipMemcpyDeviceToHost, J->stream);
@@ -2435,20 +2426,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
         for (uint32_t shift = 0; shift < bits; shift += 8) {
             hipLaunchKernelGGL(k_radix_hist, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb, (uint32_t*)J->hist.p);
-            const uint32_t* scanned;
-            // One block scans 4096 values per step: above two steps the
-            // three-kernel scan is faster.
-            if (nh <= 2 * SCAN_TILE) {
-                hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream, (uint32_t*)J->hist.p,
-                                   (uint32_t*)nullptr, nh, (uint32_t*)J->scanTotal.p + 2);
-                scanned = (const uint32_t*)J->hist.p;
-            } else {
-                if ((rc = scan_u32(J, (const uint32_t*)J->hist.p, (uint32_t*)J->histScan.p, nh, (uint32_t*)J->scanTotal.p + 2)))
-                    return rc;
-                scanned = (const uint32_t*)J->histScan.p;
-            }
+            hipLaunchKernelGGL(k_radix_colscan, dim3(256), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
+                               (uint32_t*)J->histScan.p);
             hipLaunchKernelGGL(k_radix_scatter, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift, rb,
-                               scanned, kOut, vOut);
+                               (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
             std::swap(kIn, kOut);
             std::swap(vIn, vOut);
         }
