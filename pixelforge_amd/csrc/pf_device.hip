@@ -808,7 +808,9 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         d.pad = 0;
         A.rtd[rtSlot] = d;
     }
-    // Tile pairs of this raster triangle, in submission order.
+    // Tile pairs of this raster triangle, in submission order.  (Writing them
+    // cooperatively across the wavefront measured slower: most triangles
+    // have a few pairs.)
     uint32_t p = pairSlot;
     for (uint32_t ty = ty0; ty <= ty1; ty++)
         for (uint32_t tx = tx0; tx <= tx1; tx++) {
@@ -836,8 +838,9 @@ __device__ __forceinline__ void transform_pos(CState* st, float4 p, float* h) {
 // the wave's LDS buffers (no per-lane private arrays, so no scratch).
 template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL>
 __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
-                                               uint32_t t, uint32_t info, uint32_t rtBase, uint32_t pairBase,
-                                               uint32_t& nRT, uint32_t& nPairs, CVtx<VF>* cpoly, CVtx<VF>* ctmp) {
+                                               uint32_t t, uint32_t info, const float4* P, uint32_t rtBase,
+                                               uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs, CVtx<VF>* cpoly,
+                                               CVtx<VF>* ctmp) {
     // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
     // tests from the positions alone.  The clip planes are taken in the
     // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
@@ -849,7 +852,7 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
         float h[3][4];
         float wsum = 0.0f;
         for (int k = 0; k < 3; k++) {
-            transform_pos(st, A.pos[(size_t)t * 3 + k], h[k]);
+            transform_pos(st, P[k], h[k]);
             wsum += h[k][3];
         }
         is2D = fabsf(wsum - 3.0f) < 1e-5f;
@@ -877,11 +880,12 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
     const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
     for (int face = 0; face < 2; face++) {
         if (!(info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT))) continue;
-        // Vertices are (re)loaded per pass rather than kept live across both.
+        // Vertex attributes are (re)loaded per pass rather than kept live
+        // across both; the positions come in P.
         Vtx v[3];
         for (int k = 0; k < 3; k++) {
             size_t vi = (size_t)t * 3 + k;
-            float4 p = A.pos[vi];
+            const float4 p = P[k];
             v[k].pos[0] = p.x; v[k].pos[1] = p.y; v[k].pos[2] = p.z; v[k].pos[3] = p.w;
             transform_pos(st, p, v[k].hom);
             v[k].col = (PHASE == 1) ? A.col[vi] : 0u;
@@ -1036,6 +1040,15 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         // Reset the tile ranges of this pass (the raster reads [begin, end)).
         for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
     }
+    // The triangle, its frame and positions are requested first, so their
+    // latency overlaps the block scans of PHASE 1.
+    uint32_t t = 0, info = 0, si = 0, frame = 0;
+    DFrame F;
+    float4 P[3] = { make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f) };
+    if (valid) {
+        pass_triangle(A, gid, t, info, si, F, frame);
+        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
+    }
     uint32_t rtBase = 0, pairBase = 0;
     if (PHASE == 1) {
         uint32_t tot;
@@ -1044,9 +1057,6 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         pairBase = block_excl_scan(cP, lds, tot) + A.blockPairs[blockIdx.x];
     }
     uint32_t nRT = 0, nPairs = 0;
-    uint32_t t = 0, info = 0, si = 0, frame = 0;
-    DFrame F;
-    if (valid) pass_triangle(A, gid, t, info, si, F, frame);
     // One iteration per distinct state in the wavefront: the state is then
     // uniform and read with scalar loads.
     uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
@@ -1057,8 +1067,8 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         const bool mine = valid && si == s0;
         todo &= ~__builtin_amdgcn_ballot_w64(mine);
         if (mine)
-            deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, rtBase, pairBase,
-                                                         nRT, nPairs, (CVtx<VF_FULL>*)nullptr,
+            deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, P, rtBase,
+                                                         pairBase, nRT, nPairs, (CVtx<VF_FULL>*)nullptr,
                                                          (CVtx<VF_FULL>*)nullptr);
     }
     if (PHASE == 0) {
@@ -1108,9 +1118,11 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
         const uint4 cr = A.clipList[i];
         const uint32_t gid = cr.x, t = cr.y, info = cr.z, si = info & PFF_STATE_MASK;
         const DFrame F = A.frames[cr.w];
+        float4 P[3];
+        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
         uint32_t nRT = 0, nPairs = 0;
         const uint32_t rtBase = PHASE == 1 ? A.cntRT[gid] : 0u, pairBase = PHASE == 1 ? A.cntPairs[gid] : 0u;
-        setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, rtBase, pairBase, nRT,
+        setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, P, rtBase, pairBase, nRT,
                                              nPairs, poly, tmp);
         if (PHASE == 0) {
             A.cntRT[gid] = nRT;
@@ -1313,10 +1325,21 @@ constexpr int BATCH = 32;
 // measured slower on the batched workload; 32 is kept.
 constexpr int REC_STRIDE = 32;
 
+// Staged RTriL records of per-fragment lit triangles; no LDS otherwise
+// (the queue of an unlit kernel then fits eight workgroups per CU).
+template <bool PHONG> struct PhongStage { uint32_t recL[BATCH][20]; };
+template <> struct PhongStage<false> {};
+
+template <bool PHONG, typename Q>
+__device__ __forceinline__ const float* staged_light(const Q& q, int j) {
+    if constexpr (PHONG) return reinterpret_cast<const float*>(&q.recL[j][0]);
+    else return nullptr;
+}
+
 template <bool PHONG>
-struct WaveQueueT {
+struct WaveQueueT : PhongStage<PHONG> {
     uint32_t rec[BATCH][REC_STRIDE];        // staged RTri records
-    uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
+    unsigned long long rmask[BATCH];        // 8x8 blocks: bbox pixels of each staged record (bit = lane)
     uint32_t recId[BATCH];                  // raster-triangle index of each staged record
     uint32_t fcol[64];
     float    fz[64];
@@ -1330,9 +1353,12 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Edge values at (px, py); small says every coefficient fits 24 bits
+// (META_SMALL), which allows 24-bit multiplies.
 template <typename TRI>
-__device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_t& w1, int32_t& w2, int32_t& w3) {
-    if (t.meta & META_SMALL) {
+__device__ __forceinline__ void edge_values_as(const TRI& t, bool small, int px, int py, int32_t& w1, int32_t& w2,
+                                               int32_t& w3) {
+    if (small) {
         // 24x24-bit multiplies keep the low 32 bits of the product: the same
         // value mod 2^32 as the full multiply when |A|, |B| < 2^23.
         w1 = __mul24(t.A[0], px) + __mul24(t.B[0], py) + t.C[0];
@@ -1344,6 +1370,11 @@ __device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_
     w1 = (int32_t)((uint32_t)t.A[0] * ux + (uint32_t)t.B[0] * uy + (uint32_t)t.C[0]);
     w2 = (int32_t)((uint32_t)t.A[1] * ux + (uint32_t)t.B[1] * uy + (uint32_t)t.C[1]);
     w3 = (int32_t)((uint32_t)t.A[2] * ux + (uint32_t)t.B[2] * uy + (uint32_t)t.C[2]);
+}
+
+template <typename TRI>
+__device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_t& w1, int32_t& w2, int32_t& w3) {
+    edge_values_as(t, (t.meta & META_SMALL) != 0, px, py, w1, w2, w3);
 }
 
 // Colour and depth of one fragment of raster triangle t at pixel (px, py)
@@ -1425,7 +1456,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
         const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
         uint32_t c;
         float z;
-        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p % TW),
+        shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), bx0 + (p % TW),
                               by0 + (p / TW), c, z);
         Q.fcol[lane] = c;
         Q.fz[lane] = z;
@@ -1443,6 +1474,13 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
         }
     }
     wave_sync();
+}
+
+// Pixels x0..x1 of rows y0..y1 of an 8x8 block as a 64-bit mask, bit
+// 8 y + x (the lane of pixel (x, y) when a wave holds one pixel per lane).
+__device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
+    const uint64_t row = (uint64_t)(((2u << x1) - 1u) & ~((1u << x0) - 1u));
+    return (row * 0x0101010101010101ull) & (~0ull >> (8 * (7 - y1))) & (~0ull << (8 * y0));
 }
 
 // Renders the block at (bx0, by0) of a tile whose pair list is [e0, e1):
@@ -1483,6 +1521,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         }
     }
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // Last column and row of the block inside the target.
+    const int xLim = min(bx0 + TW - 1, (int)F.W - 1), yLim = min(by0 + TW - 1, (int)F.H - 1);
     uint32_t frags = 0;
     for (uint32_t e = e0; e < e1; e += BATCH) {
         // Fetch up to 32 list entries and their records; keep the ones
@@ -1492,6 +1532,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
         uint4 q0 = z4, q1 = z4, q2 = z4, q3 = z4, q4 = z4, q5 = z4, q6 = z4, q7 = z4;
         uint32_t r = 0;
+        uint64_t rm = 0;
         if ((uint32_t)lane < nb) {
             r = A.vals[e + lane];
             const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
@@ -1514,20 +1555,36 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     ov = ov && hi >= 0;
                 }
             }
+            if (ov && K == 1) {
+                // The coverage rule's bbox part (x in [xMin, xMax), y in
+                // [yMin, yMax]) and the target, as a pixel mask: for exact
+                // triangles the edges imply it, for the others it is their
+                // bbox test.  An empty mask drops the record.
+                const int32_t rx0 = max(bx0, xMin), rx1 = min(xLim, xMax - 1);
+                const int32_t ry0 = max(by0, yMin), ry1 = min(yLim, yMax);
+                if (rx0 > rx1 || ry0 > ry1) ov = false;
+                else rm = rect_mask8(rx0 - bx0, rx1 - bx0, ry0 - by0, ry1 - by0);
+            }
         }
         const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
         const int nt = __popcll(OV);
+        // 24-bit edge multiplies for the whole batch when every staged
+        // record allows them (decided before the loop: no per-record branch).
+        const bool allSmall = __builtin_amdgcn_ballot_w64(ov && !(q5.x & META_SMALL)) == 0ull;
         if (ov) {
             const int slot = (int)__popcll(OV & ltMask);
             uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
             dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
             dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
             Q.recId[slot] = r;
-            if (PHONG && (q5.x & META_PHONG)) {
-                const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
-                uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
+            if (K == 1) Q.rmask[slot] = rm;
+            if constexpr (PHONG) {
+                if (q5.x & META_PHONG) {
+                    const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
+                    uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
 #pragma unroll
-                for (int k = 0; k < 5; k++) ld[k] = ls[k];
+                    for (int k = 0; k < 5; k++) ld[k] = ls[k];
+                }
             }
         }
         wave_sync();
@@ -1537,12 +1594,16 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         for (int k = 0; k < K; k++) pm[k] = 0ull;
         for (int j = 0; j < nt; j++) {
             const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
-            const bool exact = (t.meta & META_EXACT) != 0;
             // A lane's pixels are RS rows apart: each edge value steps by
             // RS * B, and stepping mod 2^32 gives the direct value exactly.
             int32_t w1, w2, w3;
-            edge_values(t, px, py[0], w1, w2, w3);
+            edge_values_as(t, allSmall, px, py[0], w1, w2, w3);
             const uint32_t d1 = (uint32_t)t.B[0] * RS, d2 = (uint32_t)t.B[1] * RS, d3 = (uint32_t)t.B[2] * RS;
+            // Branch-free bbox test (K = 4; K = 1 uses the staged mask):
+            // every field is read up front.
+            const uint64_t rmj = (K == 1) ? Q.rmask[j] : 0ull;
+            const bool exact = (t.meta & META_EXACT) != 0;
+            const int32_t xMin = t.xMin, yMin = t.yMin, xMax = t.xMax, yMax = t.yMax;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if (k > 0) {
@@ -1550,8 +1611,9 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     w2 = (int32_t)((uint32_t)w2 + d2);
                     w3 = (int32_t)((uint32_t)w3 + d3);
                 }
-                const bool cov = in[k] && ((w1 | w2 | w3) > 0) && px < t.xMax
-                                 && (exact || (px >= t.xMin && py[k] >= t.yMin && py[k] <= t.yMax));
+                const bool inBox = (K == 1) ? (((rmj >> lane) & 1ull) != 0)
+                                            : (in[k] & (px < xMax) & (exact | ((px >= xMin) & (py[k] >= yMin) & (py[k] <= yMax))));
+                const bool cov = inBox & ((w1 | w2 | w3) > 0);
                 const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
                 if (!M) continue;
                 const int c = __popcll(M);
@@ -1560,7 +1622,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     if (cov) {
                         uint32_t fc;
                         float fz;
-                        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), px, py[k],
+                        shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), px, py[k],
                                               fc, fz);
                         rop(t.meta, fc, fz, Q.recId[j], col[k], dep[k], win[k]);
                     }
@@ -1603,9 +1665,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
 // a mask of its queue slots, applied lowest slot first.  Small triangles
 // (a few pixels per block) then cost a few lanes instead of a wavefront.
 template <bool PHONG>
-struct WaveQueueC {
+struct WaveQueueC : PhongStage<PHONG> {
     uint32_t rec[BATCH][REC_STRIDE];        // staged RTri records
-    uint32_t recL[PHONG ? BATCH : 1][20];   // staged RTriL records (Phong only)
     uint32_t recId[BATCH];                  // raster-triangle index of each staged record
     uint32_t cinfo[BATCH];                  // first candidate | x0 << 12 | y0 << 15 | (width - 1) << 18
     uint32_t fcol[64];
@@ -1625,7 +1686,7 @@ __device__ __forceinline__ void flush_queue_c(const RasterArgs& A, WaveQueueC<PH
         const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
         uint32_t c;
         float z;
-        shade_fragment<PHONG>(A, t, reinterpret_cast<const float*>(&Q.recL[PHONG ? j : 0][0]), bx0 + (p & 7),
+        shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), bx0 + (p & 7),
                               by0 + (p >> 3), c, z);
         Q.fcol[lane] = c;
         Q.fz[lane] = z;
@@ -1714,11 +1775,13 @@ __device__ __forceinline__ uint32_t raster_block_c(const RasterArgs& A, const DF
             dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
             Q.recId[slot] = r;
             Q.cinfo[slot] = (incl - cnt) | rect;
-            if (PHONG && (q5.x & META_PHONG)) {
-                const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
-                uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
+            if constexpr (PHONG) {
+                if (q5.x & META_PHONG) {
+                    const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
+                    uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
 #pragma unroll
-                for (int k = 0; k < 5; k++) ld[k] = ls[k];
+                    for (int k = 0; k < 5; k++) ld[k] = ls[k];
+                }
             }
         }
         wave_sync();
