@@ -153,6 +153,9 @@ __device__ __forceinline__ int32_t cvt_rne_i32(float f) {
     if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
     return (int32_t)__builtin_rintf(f);
 }
+// cvt_rne_i32 for a value known to lie in int32 range (not NaN): a clamped
+// colour channel times 255, a clamped texture coordinate times the size.
+__device__ __forceinline__ int32_t cvt_rne_inrange(float f) { return (int32_t)__builtin_rintf(f); }
 __device__ __forceinline__ uint8_t f2u8(float f) { return (uint8_t)(uint32_t)cvtt_i32(f); }
 __device__ __forceinline__ float mn_ps(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float mx_ps(float a, float b) { return a > b ? a : b; }
@@ -254,9 +257,9 @@ __device__ __forceinline__ float pow_cephes(float b, float e) { return cephes_ex
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ uint32_t pack_f3(float r, float g, float b) {      // color.h:110-121, vecSize 3
-    uint32_t p = (uint32_t)cvt_rne_i32(clamp_ps(r, 0.0f, 1.0f) * 255.0f);
-    p |= (uint32_t)cvt_rne_i32(clamp_ps(g, 0.0f, 1.0f) * 255.0f) << 8;
-    p |= (uint32_t)cvt_rne_i32(clamp_ps(b, 0.0f, 1.0f) * 255.0f) << 16;
+    uint32_t p = (uint32_t)cvt_rne_inrange(clamp_ps(r, 0.0f, 1.0f) * 255.0f);
+    p |= (uint32_t)cvt_rne_inrange(clamp_ps(g, 0.0f, 1.0f) * 255.0f) << 8;
+    p |= (uint32_t)cvt_rne_inrange(clamp_ps(b, 0.0f, 1.0f) * 255.0f) << 16;
     return p;
 }
 
@@ -337,7 +340,7 @@ __device__ __forceinline__ int32_t wrap_coord(uint32_t wrap, float u, uint32_t s
         if (m > 1.0f) m = mirror;
         return cvt_rne_i32(m * tw + 0.5f);
     }
-    return cvt_rne_i32(clamp_ps(u, 0.0f, 1.0f) * tw + 0.5f);
+    return cvt_rne_inrange(clamp_ps(u, 0.0f, 1.0f) * tw + 0.5f);
 }
 
 __device__ __forceinline__ uint32_t lerp_color(uint32_t a, uint32_t b, float t) {   // color.h:137-144
@@ -348,9 +351,21 @@ __device__ __forceinline__ uint32_t lerp_color(uint32_t a, uint32_t b, float t) 
         float fa = (float)ch(a, i) * inv255;
         float fb = (float)ch(b, i) * inv255;
         float r = fa + t * (fb - fa);
-        p |= (uint32_t)cvt_rne_i32(clamp_ps(r, 0.0f, 1.0f) * 255.0f) << (8 * i);
+        p |= (uint32_t)cvt_rne_inrange(clamp_ps(r, 0.0f, 1.0f) * 255.0f) << (8 * i);
     }
     return p;
+}
+
+// 1 / w correctly rounded for an integer 0 <= w <= 65535, as __fdiv_rn(1.0f,
+// (float)w) but cheaper: the hardware reciprocal (within 1 ulp) and one
+// fused Newton step.  Checked for every w >= 1 and every start within
+// 1 ulp of the rounded reciprocal (tests/test_rcp_newton.py); w = 0 gives
+// +inf as the division does.
+__device__ __forceinline__ float rcp_int_rn(uint32_t w) {
+    const float fw = (float)w;
+    const float r0 = __builtin_amdgcn_rcpf(fw);
+    const float e = __builtin_fmaf(-fw, r0, 1.0f);
+    return w ? __builtin_fmaf(e, r0, r0) : __builtin_inff();
 }
 
 __device__ uint32_t sample_tex(const DTex& t, float u, float v) {
@@ -360,7 +375,7 @@ __device__ uint32_t sample_tex(const DTex& t, float u, float v) {
         int32_t y = wrap_coord(t.wrap, v, t.h);
         return texel_at(t, (int32_t)((uint32_t)y * w + (uint32_t)x));
     }
-    float tx = __fdiv_rn(1.0f, (float)t.w), ty = __fdiv_rn(1.0f, (float)t.h);
+    float tx = rcp_int_rn(t.w), ty = rcp_int_rn(t.h);   // texture sides are at most 65535
     int32_t x0 = wrap_coord(t.wrap, u, t.w);
     int32_t y0 = wrap_coord(t.wrap, v, t.h);
     int32_t x1 = wrap_coord(t.wrap, u + tx, t.w);
@@ -697,6 +712,14 @@ struct SetupArgs {
 // triangle: columns [xMin, xMax) and rows [yMin, yMax] clipped to the target.
 struct TileSpan { uint32_t tx0, tx1, ty0, ty1; };
 
+// The tile rectangle of one raster triangle's pairs: w x h tiles from tile0
+// (rows tilesX apart), kept by the clipper so that the wavefront writes the
+// pairs of all its lanes' triangles together.
+struct PairRect {
+    uint32_t tile0, wh;                 // wh = w | h << 16
+};
+constexpr int MAX_FAN_RECTS = 10;       // fan triangles of a polygon of at most 12 vertices
+
 // Builds (or only counts) the raster triangle of screen vertices a, b, c for
 // one culling pass (Rasterize_Triangle setup, triangles.c:287-349) and its
 // tile pairs.  Returns false when it is culled or covers no pixel.
@@ -704,7 +727,8 @@ template <int PHASE>
 __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, const DFrame& F,
                                             uint32_t stateIdx, uint32_t src, int face, bool is3D,
                                             const Vtx& a, const Vtx& b, const Vtx& c,
-                                            uint32_t rtSlot, uint32_t pairSlot, uint32_t& pairsOut) {
+                                            uint32_t rtSlot, uint32_t pairSlot, uint32_t& pairsOut,
+                                            PairRect* rect = nullptr) {
     int32_t x1 = cvtt_i32(a.scr[0]), y1 = cvtt_i32(a.scr[1]);
     int32_t x2 = cvtt_i32(b.scr[0]), y2 = cvtt_i32(b.scr[1]);
     int32_t x3 = cvtt_i32(c.scr[0]), y3 = cvtt_i32(c.scr[1]);
@@ -808,9 +832,13 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         d.pad = 0;
         A.rtd[rtSlot] = d;
     }
-    // Tile pairs of this raster triangle, in submission order.  (Writing them
-    // cooperatively across the wavefront measured slower: most triangles
-    // have a few pairs.)
+    // Tile pairs of this raster triangle, in submission order: handed back
+    // as a rectangle (clipped triangles: large, few) or written here (the
+    // rest: mostly a few pairs, where the per-lane loop is cheaper).
+    if (rect) {
+        *rect = PairRect{ F.tileBase + ty0 * F.tilesX + tx0, (tx1 - tx0 + 1) | ((ty1 - ty0 + 1) << 16) };
+        return true;
+    }
     uint32_t p = pairSlot;
     for (uint32_t ty = ty0; ty <= ty1; ty++)
         for (uint32_t tx = tx0; tx <= tx1; tx++) {
@@ -840,7 +868,7 @@ template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL>
 __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
                                                uint32_t t, uint32_t info, const float4* P, uint32_t rtBase,
                                                uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs, CVtx<VF>* cpoly,
-                                               CVtx<VF>* ctmp) {
+                                               CVtx<VF>* ctmp, PairRect* rects = nullptr) {
     // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
     // tests from the positions alone.  The clip planes are taken in the
     // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
@@ -970,7 +998,8 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
                 to_screen(stl, a);
                 to_screen(stl, b);
                 to_screen(stl, c);
-                if (emit_raster<PHASE>(A, st, F, si, t, face, true, a, b, c, rtBase + nRT, pairBase + nPairs, np)) {
+                if (emit_raster<PHASE>(A, st, F, si, t, face, true, a, b, c, rtBase + nRT, pairBase + nPairs, np,
+                                       (rects && nRT < MAX_FAN_RECTS) ? &rects[nRT] : nullptr)) {
                     nRT++;
                     nPairs += np;
                 }
@@ -1103,32 +1132,92 @@ template <int VF>
 constexpr int clip_words() { return 2 * 12 * (int)(sizeof(CVtx<VF>) / 4) + 1; }   // odd: lanes in distinct banks
 constexpr int CLIP_GRID = 1024;
 
+// The wavefront writes one pair rectangle per lane (w x h tiles from tile0,
+// rows tilesX apart, pairs from slot with raster triangle rt) together, 64
+// pairs per step: the clipped triangles of a wavefront are few but large,
+// and a lane looping over its own tiles would hold the others.  Every lane
+// of the wavefront must call it; lanes without a rectangle pass w = 0.
+__device__ __forceinline__ void write_pair_rects(const SetupArgs& A, uint32_t slot, uint32_t tile0, uint32_t w,
+                                                 uint32_t h, uint32_t tilesX, uint32_t rt) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = w * h;
+    const uint32_t incl = wave_incl_scan(n);
+    const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+    const uint32_t excl = incl - n;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t i = base + (uint32_t)lane;
+        // Owner: the last lane whose first pair is <= i (lanes without
+        // pairs share the next lane's start and are passed over).
+        int o = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const uint32_t e = (uint32_t)__shfl((int)excl, o + step, 64);
+            if (e <= i) o += step;
+        }
+        const uint32_t q = i - (uint32_t)__shfl((int)excl, o, 64);
+        const uint32_t oSlot = (uint32_t)__shfl((int)slot, o, 64);
+        const uint32_t oTile = (uint32_t)__shfl((int)tile0, o, 64);
+        const uint32_t oW = (uint32_t)__shfl((int)w, o, 64);
+        const uint32_t oTX = (uint32_t)__shfl((int)tilesX, o, 64);
+        const uint32_t oRt = (uint32_t)__shfl((int)rt, o, 64);
+        if (i < total) {
+            const uint32_t qy = q / oW, qx = q - qy * oW;
+            const uint32_t p = oSlot + q;
+            if (p < A.capPairs) {
+                A.keys[p] = oTile + qy * oTX + qx;
+                A.vals[p] = oRt;
+            }
+        }
+    }
+}
+
 template <int PHASE, bool LIT, int VF>
 __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
     constexpr int CLIP_WORDS = clip_words<VF>();
     __shared__ uint32_t mem[64 * CLIP_WORDS];
+    __shared__ PairRect rectMem[PHASE == 1 ? 64 * MAX_FAN_RECTS : 1];
     const int lane = threadIdx.x;
     CVtx<VF>* poly = reinterpret_cast<CVtx<VF>*>(&mem[lane * CLIP_WORDS]);
     CVtx<VF>* tmp = poly + 12;
+    PairRect* rects = PHASE == 1 ? &rectMem[lane * MAX_FAN_RECTS] : nullptr;
     const uint32_t n = *A.clipCount;
-    for (uint32_t i = blockIdx.x * 64 + lane; i < n; i += CLIP_GRID * 64) {
-        // The listed triangle with its recorded index, info and frame, so
-        // that its inputs are fetched without the segment lookup's chain
-        // of dependent loads.
-        const uint4 cr = A.clipList[i];
-        const uint32_t gid = cr.x, t = cr.y, info = cr.z, si = info & PFF_STATE_MASK;
-        const DFrame F = A.frames[cr.w];
-        float4 P[3];
-        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
-        uint32_t nRT = 0, nPairs = 0;
-        const uint32_t rtBase = PHASE == 1 ? A.cntRT[gid] : 0u, pairBase = PHASE == 1 ? A.cntPairs[gid] : 0u;
-        setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, P, rtBase, pairBase, nRT,
-                                             nPairs, poly, tmp);
-        if (PHASE == 0) {
-            A.cntRT[gid] = nRT;
-            A.cntPairs[gid] = nPairs;
-            if (nRT) atomicAdd(&A.blockRT[gid / 256], nRT);
-            if (nPairs) atomicAdd(&A.blockPairs[gid / 256], nPairs);
+    // Wavefront-uniform loop: the pair writing below needs every lane.
+    for (uint32_t i0 = blockIdx.x * 64; i0 < n; i0 += CLIP_GRID * 64) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        const bool valid = i < n;
+        uint32_t nRT = 0, nPairs = 0, rtBase = 0, pairBase = 0, tilesX = 0;
+        if (valid) {
+            // The listed triangle with its recorded index, info and frame, so
+            // that its inputs are fetched without the segment lookup's chain
+            // of dependent loads.
+            const uint4 cr = A.clipList[i];
+            const uint32_t gid = cr.x, t = cr.y, info = cr.z, si = info & PFF_STATE_MASK;
+            const DFrame F = A.frames[cr.w];
+            tilesX = F.tilesX;
+            float4 P[3];
+            for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
+            if (PHASE == 1) { rtBase = A.cntRT[gid]; pairBase = A.cntPairs[gid]; }
+            setup_triangle<PHASE, LIT, true, VF>(A, as_const(A.states + si), si, F, t, info, P, rtBase, pairBase,
+                                                 nRT, nPairs, poly, tmp, rects);
+            if (PHASE == 0) {
+                A.cntRT[gid] = nRT;
+                A.cntPairs[gid] = nPairs;
+                if (nRT) atomicAdd(&A.blockRT[gid / 256], nRT);
+                if (nPairs) atomicAdd(&A.blockPairs[gid / 256], nPairs);
+            }
+        }
+        if (PHASE == 1) {
+            // The lanes' k-th raster triangles' pairs, k = 0, 1, ...
+            uint32_t most = min(nRT, (uint32_t)MAX_FAN_RECTS);
+            for (int d = 1; d < 64; d <<= 1) most = max(most, (uint32_t)__shfl_xor((int)most, d, 64));
+            uint32_t slot = pairBase;
+            for (uint32_t k = 0; k < most; k++) {
+                const bool has = k < min(nRT, (uint32_t)MAX_FAN_RECTS);
+                const PairRect r = has ? rects[k] : PairRect{ 0u, 0u };
+                const uint32_t w = r.wh & 0xFFFFu, h = r.wh >> 16;
+                write_pair_rects(A, slot, r.tile0, w, h, tilesX, rtBase + k);
+                slot += w * h;
+            }
         }
     }
 }
@@ -1963,7 +2052,7 @@ __device__ __forceinline__ void tex_taps(const DTex& t, float u, float v, Taps& 
         tp.w[0] = 1.0f;
         return;
     }
-    float tx = __fdiv_rn(1.0f, (float)t.w), ty = __fdiv_rn(1.0f, (float)t.h);
+    float tx = rcp_int_rn(t.w), ty = rcp_int_rn(t.h);   // texture sides are at most 65535
     int32_t x0 = wrap_coord(t.wrap, u, t.w), y0 = wrap_coord(t.wrap, v, t.h);
     int32_t x1 = wrap_coord(t.wrap, u + tx, t.w), y1 = wrap_coord(t.wrap, v + ty, t.h);
     float fx = clamp_ps(u * (float)t.w - (float)x0, 0.0f, 1.0f);
