@@ -1078,13 +1078,10 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         pass_triangle(A, gid, t, info, si, F, frame);
         for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
     }
-    uint32_t rtBase = 0, pairBase = 0;
-    if (PHASE == 1) {
-        uint32_t tot;
-        uint32_t cR = valid ? A.cntRT[gid] : 0u, cP = valid ? A.cntPairs[gid] : 0u;
-        rtBase = block_excl_scan(cR, lds, tot) + A.blockRT[blockIdx.x];
-        pairBase = block_excl_scan(cP, lds, tot) + A.blockPairs[blockIdx.x];
-    }
+    // PHASE 1: the slots of this triangle's records and pairs (k_scan_local
+    // turned the counts into offsets).
+    const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
+    const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
     uint32_t nRT = 0, nPairs = 0;
     // One iteration per distinct state in the wavefront: the state is then
     // uniform and read with scalar loads.
@@ -1116,10 +1113,21 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         block_excl_scan(nRT, lds, totR);
         block_excl_scan(nPairs, lds, totP);
         if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
-    } else if (deferred) {
-        A.cntRT[gid] = rtBase;            // k_clip<1> writes from these slots
-        A.cntPairs[gid] = pairBase;
     }
+}
+
+// Per triangle of the pass, the first raster-triangle and pair slot: the
+// block's scanned sums plus a scan within the block, in place over the
+// counts.  Both k_setup<1> and k_clip<1> then find their slots there, so
+// the two run side by side.
+__global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    uint32_t tot;
+    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + A.blockRT[blockIdx.x];
+    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + A.blockPairs[blockIdx.x];
+    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
 }
 
 // The listed triangles that need clipping, one per lane, each lane
@@ -2408,6 +2416,8 @@ struct PassInfo {
 
 struct PFDjob {
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;           // k_clip<1> beside k_setup<1> (default priority)
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<PFFframe> src;            // host frame descriptors: only the target fields are
                                           // used after creation (the rest may be reused by the caller)
     std::vector<uint8_t>  leadClear;      // per frame: the first command is a clear
@@ -2553,14 +2563,21 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.capRT = numRT;
     sa.capPairs = numPairs;
     if (numRT) {
+        // Slots per triangle, then the clipped triangles on the side stream
+        // while the others are set up (both read the slots only).
+        hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, J->stream, sa);
+        HIP_TRY(hipEventRecord(J->fork, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
         if (J->anyLit) {
+            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
-            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
         } else {
+            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
+            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
             hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
-            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
         }
+        HIP_TRY(hipEventRecord(J->join, J->side), PFD_E_LAUNCH);
+        HIP_TRY(hipStreamWaitEvent(J->stream, J->join, 0), PFD_E_LAUNCH);
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
@@ -2715,6 +2732,13 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
     hipError_t e = hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { set_error("hipStreamCreate", e); delete J; return PFD_E_NODEVICE; }
+    if ((e = hipStreamCreateWithFlags(&J->side, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&J->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&J->join, hipEventDisableTiming)) != hipSuccess) {
+        set_error("hipStreamCreate", e);
+        pfdJobDestroy(J);
+        return PFD_E_NODEVICE;
+    }
 
     // Concatenate triangle streams, states and textures of all frames.
     uint32_t totalTris = 0, totalStates = 0;
@@ -3088,6 +3112,10 @@ void pfdJobDestroy(PFDjob* J) {
     for (auto& b : J->targets) release(b);
     for (auto& b : J->texData) release(b);
     for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); }
+    if (J->side) (void)hipStreamSynchronize(J->side);
+    if (J->fork) (void)hipEventDestroy(J->fork);
+    if (J->join) (void)hipEventDestroy(J->join);
+    if (J->side) (void)hipStreamDestroy(J->side);
     if (J->stream) (void)hipStreamDestroy(J->stream);
     delete J;
 }
