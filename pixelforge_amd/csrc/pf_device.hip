@@ -1275,18 +1275,21 @@ constexpr int RADIX_THREADS = 256;
 constexpr int RADIX_ROUNDS = 8;      // 2048 items per block: more blocks, shorter serial rounds
 constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ROUNDS;
 
+// Digits of DB bits: 2^DB buckets (DB = 8 is used).
+template <int DB>
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
                                                               uint32_t nb, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr uint32_t ND = 1u << DB;
+    __shared__ uint32_t h[ND];
+    for (uint32_t d = threadIdx.x; d < ND; d += RADIX_THREADS) h[d] = 0;
     __syncthreads();
     size_t base = (size_t)blockIdx.x * RADIX_TILE;
     for (int r = 0; r < RADIX_ROUNDS; r++) {
         size_t i = base + (size_t)r * RADIX_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (ND - 1u)], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    for (uint32_t d = threadIdx.x; d < ND; d += RADIX_THREADS) hist[(size_t)d * nb + blockIdx.x] = h[d];
 }
 
 // One workgroup per digit d: the exclusive prefix over radix blocks of
@@ -1315,35 +1318,49 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_colscan(uint32_t* __res
 
 // Stable scatter: items of a block are taken in index order, 256 per round;
 // an item's rank among equal digits is its rank within its wave (ballot
-// match on the 8 digit bits) plus the counts of lower waves and of earlier
+// match on the DB digit bits) plus the counts of lower waves and of earlier
 // rounds.
+template <int DB>
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                                                                  uint32_t n, uint32_t shift, uint32_t nb,
                                                                  const uint32_t* __restrict__ colPrefix,
                                                                  const uint32_t* __restrict__ digitTotal,
                                                                  uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t cnt[RADIX_THREADS / 64][256];
+    constexpr uint32_t ND = 1u << DB;
+    constexpr int PER = (int)ND / RADIX_THREADS;          // digits per thread
+    __shared__ uint32_t base[ND];
+    __shared__ uint32_t cnt[RADIX_THREADS / 64][ND];
     __shared__ uint32_t lds[64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // Items of digit d start at (items of lower digits) + (items of digit d
-    // in earlier blocks).
-    uint32_t all;
-    const uint32_t lower = block_excl_scan(digitTotal[threadIdx.x], lds, all);
-    base[threadIdx.x] = lower + colPrefix[(size_t)threadIdx.x * nb + blockIdx.x];
+    // in earlier blocks); thread t owns digits t*PER .. t*PER + PER-1.
+    {
+        uint32_t tot[PER], s = 0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) { tot[q] = digitTotal[threadIdx.x * PER + q]; s += tot[q]; }
+        uint32_t all;
+        uint32_t lower = block_excl_scan(s, lds, all);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint32_t d = threadIdx.x * PER + q;
+            base[d] = lower + colPrefix[(size_t)d * nb + blockIdx.x];
+            lower += tot[q];
+        }
+    }
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     size_t blockBase = (size_t)blockIdx.x * RADIX_TILE;
     for (int r = 0; r < RADIX_ROUNDS; r++) {
-        for (int w = 0; w < RADIX_THREADS / 64; w++) cnt[w][threadIdx.x] = 0;
+        for (int w = 0; w < RADIX_THREADS / 64; w++)
+            for (uint32_t d = threadIdx.x; d < ND; d += RADIX_THREADS) cnt[w][d] = 0;
         __syncthreads();
         size_t i = blockBase + (size_t)r * RADIX_THREADS + threadIdx.x;
         bool valid = i < n;
         uint32_t key = valid ? keysIn[i] : 0u;
         uint32_t val = valid ? valsIn[i] : 0u;
-        uint32_t d = (key >> shift) & 255u;
+        uint32_t d = (key >> shift) & (ND - 1u);
         uint64_t match = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < DB; b++) {
             uint64_t bb = __ballot((d >> b) & 1u);
             match &= ((d >> b) & 1u) ? bb : ~bb;
         }
@@ -1357,9 +1374,11 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
             valsOut[pos] = val;
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < RADIX_THREADS / 64; w++) add += cnt[w][threadIdx.x];
-        base[threadIdx.x] += add;
+        for (uint32_t dd = threadIdx.x; dd < ND; dd += RADIX_THREADS) {
+            uint32_t add = 0;
+            for (int w = 0; w < RADIX_THREADS / 64; w++) add += cnt[w][dd];
+            base[dd] += add;
+        }
         __syncthreads();
     }
 }
@@ -1443,6 +1462,34 @@ struct WaveQueueT : PhongStage<PHONG> {
     uint8_t  ftri[64];
     uint8_t  fpix[64];
 };
+
+// Staged records.  With ROT their 16-byte chunks are rotated by the slot
+// (chunk c of record j at position (c + j) mod 8): lanes reading one field
+// of different records then hit different LDS banks (records 128 bytes
+// apart put one field of every record in the same two banks).  Rotation
+// helps where a flush shades fragments of many triangles (small ones) and
+// costs where triangles are large, so it is a raster mode of its own.
+template <bool ROT, typename QT>
+__device__ __forceinline__ void stage_rec(QT& Q, int slot, const uint4* q) {
+    uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
+#pragma unroll
+    for (int c = 0; c < 8; c++) dst[ROT ? ((c + slot) & 7) : c] = q[c];
+}
+// Chunks 0..NC-1 of staged record j (its leading fields); the rest zero.
+template <bool ROT, int NC, typename QT>
+__device__ __forceinline__ RTri staged_rec(const QT& Q, int j) {
+    const uint4* src = reinterpret_cast<const uint4*>(&Q.rec[j][0]);
+    RTri r;
+    uint4* d = reinterpret_cast<uint4*>(&r);
+#pragma unroll
+    for (int c = 0; c < 8; c++) d[c] = c < NC ? src[ROT ? ((c + j) & 7) : c] : make_uint4(0u, 0u, 0u, 0u);
+    return r;
+}
+template <bool ROT, typename QT>
+__device__ __forceinline__ uint32_t staged_meta(const QT& Q, int j) {
+    static_assert(offsetof(RTri, meta) == 80, "meta opens chunk 5");
+    return Q.rec[j][(ROT ? ((5 + j) & 7) : 5) * 4];
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1542,7 +1589,7 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
 // Queued pixel p of a TW-wide block is (bx0 + p % TW, by0 + p / TW); lane
 // l owns pixels l + 64k, k < K, whose fragments sit in the queue slots set
 // in pm[k] (slots grow in submission order).
-template <bool PHONG, int K>
+template <bool PHONG, int K, bool ROT>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
                                             int nf, uint64_t* pm, uint32_t* col, float* dep, uint32_t* win) {
     constexpr int TW = (K == 1) ? 8 : 16;
@@ -1550,7 +1597,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
     if (lane < nf) {
         const int j = Q.ftri[lane];
         const int p = Q.fpix[lane];
-        const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+        const RTri t = staged_rec<ROT, 8>(Q, j);
         uint32_t c;
         float z;
         shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), bx0 + (p % TW),
@@ -1567,7 +1614,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
             const int s = __builtin_ctzll(m);
             m &= m - 1ull;
             const int j = Q.ftri[s];
-            rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.recId[j], col[k], dep[k], win[k]);
+            rop(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], col[k], dep[k], win[k]);
         }
     }
     wave_sync();
@@ -1587,6 +1634,8 @@ __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
 template <int MODE, bool PHONG, int K>
 __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
                                                  int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueT<PHONG>& Q) {
+    constexpr bool ROT = MODE == 4;                // mode 4: mode 0 with rotated staged records
+    constexpr int PM = MODE == 4 ? 0 : MODE;      // 0 packed, 1 in place
     constexpr int TW = (K == 1) ? 8 : 16;
     constexpr int RS = 64 / TW;                    // rows between a lane's pixels
     const int lane = threadIdx.x & 63;
@@ -1670,9 +1719,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         const bool allSmall = __builtin_amdgcn_ballot_w64(ov && !(q5.x & META_SMALL)) == 0ull;
         if (ov) {
             const int slot = (int)__popcll(OV & ltMask);
-            uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
-            dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
-            dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
+            const uint4 q[8] = { q0, q1, q2, q3, q4, q5, q6, q7 };
+            stage_rec<ROT>(Q, slot, q);
             Q.recId[slot] = r;
             if (K == 1) Q.rmask[slot] = rm;
             if constexpr (PHONG) {
@@ -1690,7 +1738,11 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
 #pragma unroll
         for (int k = 0; k < K; k++) pm[k] = 0ull;
         for (int j = 0; j < nt; j++) {
-            const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
+            // Edges and bbox (chunks 0..3) and meta; the whole record when
+            // this loop shades (in place).
+            RTri t;
+            if (PM == 1) t = staged_rec<ROT, 8>(Q, j);
+            else { t = staged_rec<ROT, 4>(Q, j); t.meta = staged_meta<ROT>(Q, j); }
             // A lane's pixels are RS rows apart: each edge value steps by
             // RS * B, and stepping mod 2^32 gives the direct value exactly.
             int32_t w1, w2, w3;
@@ -1715,7 +1767,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 if (!M) continue;
                 const int c = __popcll(M);
                 frags += (uint32_t)c;
-                if (MODE == 1) {
+                if (PM == 1) {
                     if (cov) {
                         uint32_t fc;
                         float fz;
@@ -1726,7 +1778,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     continue;
                 }
                 if (nf + c > 64) {
-                    flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
+                    flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
                     nf = 0;
                 }
                 if (cov) {
@@ -1738,7 +1790,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 nf += c;
             }
         }
-        if (MODE == 0 && nf > 0) flush_queue<PHONG, K>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
+        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
         wave_sync();
     }
 #pragma unroll
@@ -1750,191 +1802,6 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         }
     return frags;
 }
-
-// Compacted mode (4): one 8x8 block per wave, as mode 0, but coverage is
-// tested on candidate pixels instead of on every pixel of the block for
-// every staged triangle.  A staged triangle's candidates are the pixels of
-// its bounding box inside the block and the target (the bbox test of the
-// coverage rule, so only the edge test is left); the candidates of a batch
-// are numbered in list order, triangle by triangle, row by row, and tested
-// 64 at a time.  Covered candidates enter the fragment queue in that order,
-// so the fragments of one pixel stay in submission order; each pixel keeps
-// a mask of its queue slots, applied lowest slot first.  Small triangles
-// (a few pixels per block) then cost a few lanes instead of a wavefront.
-template <bool PHONG>
-struct WaveQueueC : PhongStage<PHONG> {
-    uint32_t rec[BATCH][REC_STRIDE];        // staged RTri records
-    uint32_t recId[BATCH];                  // raster-triangle index of each staged record
-    uint32_t cinfo[BATCH];                  // first candidate | x0 << 12 | y0 << 15 | (width - 1) << 18
-    uint32_t fcol[64];
-    float    fz[64];
-    unsigned long long pmask[64];           // per pixel: its queue slots
-    uint8_t  ftri[64];
-    uint8_t  fpix[64];
-};
-
-template <bool PHONG>
-__device__ __forceinline__ void flush_queue_c(const RasterArgs& A, WaveQueueC<PHONG>& Q, int bx0, int by0, int lane,
-                                              int nf, uint32_t& col, float& dep, uint32_t& win) {
-    wave_sync();
-    if (lane < nf) {
-        const int j = Q.ftri[lane];
-        const int p = Q.fpix[lane];
-        const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
-        uint32_t c;
-        float z;
-        shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), bx0 + (p & 7),
-                              by0 + (p >> 3), c, z);
-        Q.fcol[lane] = c;
-        Q.fz[lane] = z;
-    }
-    wave_sync();
-    unsigned long long m = Q.pmask[lane];
-    Q.pmask[lane] = 0ull;
-    while (m) {
-        const int s = __builtin_ctzll(m);
-        m &= m - 1ull;
-        const int j = Q.ftri[s];
-        rop(Q.rec[j][offsetof(RTri, meta) / 4], Q.fcol[s], Q.fz[s], Q.recId[j], col, dep, win);
-    }
-    wave_sync();
-}
-
-template <bool PHONG>
-__device__ __forceinline__ uint32_t raster_block_c(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
-                                                   int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueC<PHONG>& Q) {
-    const int lane = threadIdx.x & 63;
-    const int px = bx0 + (lane & 7), py = by0 + (lane >> 3);
-    const bool in = px < (int)F.W && py < (int)F.H;
-    const size_t npix = (size_t)F.W * F.H;
-    const size_t aligned = npix - (npix % 8);
-    const size_t pi = in ? (size_t)py * F.W + px : 0;
-    uint32_t col = 0u, win = 0u;
-    float dep = 0.0f;
-    if (in) {
-        if (sg.clear && pi >= 8 && pi < aligned) {                 // context.c:693-713
-            col = sg.clearColor;
-            dep = sg.clearDepth;
-        } else {
-            const size_t src = (sg.clear && pi >= aligned) ? 0 : pi;
-            col = F.color[src];
-            dep = F.depth[src];
-        }
-    }
-    // Last column and row of the block inside the target.
-    const int xLim = min(bx0 + 7, (int)F.W - 1), yLim = min(by0 + 7, (int)F.H - 1);
-    Q.pmask[lane] = 0ull;
-    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t frags = 0;
-    for (uint32_t e = e0; e < e1; e += BATCH) {
-        const uint32_t nb = min((uint32_t)BATCH, e1 - e);
-        bool ov = false;
-        const uint4 z4 = make_uint4(0u, 0u, 0u, 0u);
-        uint4 q0 = z4, q1 = z4, q2 = z4, q3 = z4, q4 = z4, q5 = z4, q6 = z4, q7 = z4;
-        uint32_t r = 0, cnt = 0, rect = 0;
-        if ((uint32_t)lane < nb) {
-            r = A.vals[e + lane];
-            const uint4* src = reinterpret_cast<const uint4*>(&A.rts[r]);
-            q0 = src[0]; q1 = src[1]; q2 = src[2]; q3 = src[3];
-            q4 = src[4]; q5 = src[5]; q6 = src[6]; q7 = src[7];
-            const int32_t xMin = (int32_t)q2.y, yMin = (int32_t)q2.z;
-            const int32_t xMax = (int32_t)q2.w, yMax = (int32_t)q3.x;
-            const int32_t rx0 = max(bx0, xMin), rx1 = min(xLim, xMax - 1);
-            const int32_t ry0 = max(by0, yMin), ry1 = min(yLim, yMax);
-            ov = rx0 <= rx1 && ry0 <= ry1;
-            // An edge negative over the whole block leaves it uncovered (see
-            // raster_block: NOWRAP keeps w free of int32 wrap near the bbox).
-            if (ov && (q5.x & META_NOWRAP)) {
-                const int32_t xs[2] = { bx0, bx0 + 7 }, ys[2] = { by0, by0 + 7 };
-                const int32_t eA[3] = { (int32_t)q0.x, (int32_t)q0.y, (int32_t)q0.z };
-                const int32_t eB[3] = { (int32_t)q0.w, (int32_t)q1.x, (int32_t)q1.y };
-                const int32_t eC[3] = { (int32_t)q1.z, (int32_t)q1.w, (int32_t)q2.x };
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int32_t hi = eA[k] * xs[eA[k] > 0] + eB[k] * ys[eB[k] > 0] + eC[k];
-                    ov = ov && hi >= 0;
-                }
-            }
-            if (ov) {
-                const int32_t w = rx1 - rx0 + 1;
-                cnt = (uint32_t)(w * (ry1 - ry0 + 1));
-                rect = ((uint32_t)(rx0 - bx0) << 12) | ((uint32_t)(ry0 - by0) << 15) | ((uint32_t)(w - 1) << 18);
-            }
-        }
-        const uint64_t OV = __builtin_amdgcn_ballot_w64(ov);
-        const int nt = __popcll(OV);
-        const uint32_t incl = wave_incl_scan(cnt);
-        const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-        if (ov) {
-            const int slot = (int)__popcll(OV & ltMask);
-            uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
-            dst[0] = q0; dst[1] = q1; dst[2] = q2; dst[3] = q3;
-            dst[4] = q4; dst[5] = q5; dst[6] = q6; dst[7] = q7;
-            Q.recId[slot] = r;
-            Q.cinfo[slot] = (incl - cnt) | rect;
-            if constexpr (PHONG) {
-                if (q5.x & META_PHONG) {
-                    const uint4* ls = reinterpret_cast<const uint4*>(&A.rtl[r]);
-                    uint4* ld = reinterpret_cast<uint4*>(&Q.recL[slot][0]);
-#pragma unroll
-                    for (int k = 0; k < 5; k++) ld[k] = ls[k];
-                }
-            }
-        }
-        wave_sync();
-        int nf = 0;
-        for (uint32_t base = 0; base < total; base += 64) {
-            const uint32_t s = base + (uint32_t)lane;
-            bool cov = false;
-            int j = 0, lp = 0;
-            if (s < total) {
-                // The staged record holding candidate s: the last one whose
-                // first candidate is <= s (first candidates increase).
-#pragma unroll
-                for (int step = BATCH / 2; step > 0; step >>= 1)
-                    if (j + step < nt && (Q.cinfo[j + step] & 0xFFFu) <= s) j += step;
-                const uint32_t ci = Q.cinfo[j];
-                const int q = (int)(s - (ci & 0xFFFu));
-                const int w = (int)((ci >> 18) & 7u) + 1;
-                // q / w for q < 64, w <= 8: (q + 0.5) / w lies at least 1/16
-                // from an integer, far beyond the reciprocal's error.
-                const int qy = (int)(((float)q + 0.5f) * __builtin_amdgcn_rcpf((float)w));
-                const int lx = (int)((ci >> 12) & 7u) + (q - qy * w), ly = (int)((ci >> 15) & 7u) + qy;
-                const RTri& t = *reinterpret_cast<const RTri*>(&Q.rec[j][0]);
-                int32_t w1, w2, w3;
-                edge_values(t, bx0 + lx, by0 + ly, w1, w2, w3);
-                cov = (w1 | w2 | w3) > 0;
-                lp = ly * 8 + lx;
-            }
-            const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
-            if (!M) continue;
-            const int c = __popcll(M);
-            frags += (uint32_t)c;
-            if (nf + c > 64) {
-                flush_queue_c<PHONG>(A, Q, bx0, by0, lane, nf, col, dep, win);
-                nf = 0;
-            }
-            if (cov) {
-                const int slot = nf + (int)__popcll(M & ltMask);
-                Q.ftri[slot] = (uint8_t)j;
-                Q.fpix[slot] = (uint8_t)lp;
-                atomicOr(&Q.pmask[lp], 1ull << slot);
-            }
-            nf += c;
-        }
-        if (nf > 0) flush_queue_c<PHONG>(A, Q, bx0, by0, lane, nf, col, dep, win);
-        wave_sync();
-    }
-    if (in) {
-        F.color[pi] = col;
-        F.depth[pi] = dep;
-        if (A.winner) A.winner[pi] = win;
-    }
-    return frags;
-}
-
-template <int MODE, bool PHONG> struct RasterQueue { typedef WaveQueueT<PHONG> type; };
-template <bool PHONG> struct RasterQueue<4, PHONG> { typedef WaveQueueC<PHONG> type; };
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
 // i mod 8), and each XCD has its own L2.  This maps the workgroups of each
@@ -1959,7 +1826,7 @@ __device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
 // workgroup, its blocks dealt round-robin to the four waves.
 template <int TS, int MODE, bool PHONG, int K>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
-    typedef typename RasterQueue<MODE, PHONG>::type Queue;
+    typedef WaveQueueT<PHONG> Queue;
     __shared__ Queue Qs[4];
     constexpr int BW = (K == 1) ? 8 : 16;
     static_assert(TS >= BW, "tile smaller than a block");
@@ -1983,8 +1850,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
         const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        if constexpr (MODE == 4) frags += raster_block_c<PHONG>(A, F, sg, bx0, by0, e0, e1, Q);
-        else frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
+        frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -2593,12 +2459,15 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         if ((rc = ensure(J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J->histScan, (size_t)nh * 4 + 4))) return rc;
         uint32_t* kIn = keys; uint32_t* vIn = vals;
         uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
+        // 8-bit digits (10-bit ones, which would sort 17..20-bit keys in two
+        // passes, measured slower than a third 8-bit pass).
         for (uint32_t shift = 0; shift < bits; shift += 8) {
-            hipLaunchKernelGGL(k_radix_hist, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb, (uint32_t*)J->hist.p);
+            hipLaunchKernelGGL(k_radix_hist<8>, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb,
+                               (uint32_t*)J->hist.p);
             hipLaunchKernelGGL(k_radix_colscan, dim3(256), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
                                (uint32_t*)J->histScan.p);
-            hipLaunchKernelGGL(k_radix_scatter, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift, rb,
-                               (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
+            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift,
+                               rb, (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
             std::swap(kIn, kOut);
             std::swap(vIn, vOut);
         }

@@ -49,3 +49,41 @@ def test_batch_job_every_tile_size(tile_shift, mode):
             assert np.array_equal(got, r), f"context {i} at tile {1 << tile_shift}, mode {mode}"
     finally:
         cap.close()
+
+
+def _run_capture(cap, tile_shift, mode=0):
+    lib = pf.lib()
+    cfg = pf.PFDconfig()
+    cfg.tileShift, cfg.rasterMode = tile_shift, mode
+    assert lib.pfdJobSetConfig(cap.job, C.byref(cfg)) == 0
+    for b in cap.buffers:
+        b[:] = 0
+    assert lib.pfdJobUploadTargets(cap.job) == 0
+    for _ in range(2):                                  # sizing run, then the graph
+        assert lib.pfdJobLaunch(cap.job) == 0
+    assert lib.pfdJobSync(cap.job) == 0
+    assert lib.pfdJobDownload(cap.job) == 0
+    return [b.copy() for b in cap.buffers]
+
+
+def test_benchmark_batch_wide_tile_keys():
+    """The 256-context benchmark batch at 16x16 tiles has ~278k tiles, so
+    its tile keys take 19 bits and the sort runs three 8-bit passes; at
+    64x64 tiles the keys take 15 bits (two passes).  Tile
+    size is a performance knob: both render the same images, and a sample
+    of contexts equals the oracle."""
+    params = pf.batch_workload(256)
+    cap = pf.capture_batch(params)
+    try:
+        wide = _run_capture(cap, 4)
+        narrow = _run_capture(cap, 6)
+    finally:
+        cap.close()
+    for i, p in enumerate(params):
+        n = p.width * p.height
+        assert np.array_equal(wide[i][:n], narrow[i][:n]), f"context {i} differs between 16x16 and 64x64 tiles"
+    for i in (0, 17, 101, 255):
+        p = params[i]
+        ref = render_oracle(p)[0]
+        got = wide[i][: p.width * p.height].reshape(p.height, p.width)
+        assert np.array_equal(got, ref), f"context {i} differs from the oracle"
