@@ -52,7 +52,7 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic.json")
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libpforacle.so")
 PEAK_HBM_GBS = 8000.0
 SETUP_BYTES_PER_TRI = 88          # position 3x16 + colour 3x4 + texcoord 3x8 + info 4
-RASTER_MODES = ["packed", "in-place", "packed, 4 px/lane", "in-place, 4 px/lane", "packed, rotated staging"]
+RASTER_MODES = ["packed", "in-place", "packed, 4 px/lane", "in-place, 4 px/lane"]
 
 
 def workload(width=1920, height=1080, seed=2):

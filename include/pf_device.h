@@ -38,10 +38,8 @@ typedef struct PFDjob PFDjob;
  * rasterMode: 0 packs the covered fragments of several triangles into full
  * wavefronts before shading; 1 shades each triangle in place on the lanes
  * it covers; 2 and 3 are 0 and 1 with four pixels per lane (a wave renders
- * 16x16 pixels per walk of the tile list; tiles of 8x8 fall back to 0/1);
- * 4 is 0 with the staged triangle records' 16-byte chunks rotated by
- * their slot in LDS (fewer bank conflicts when a flush shades fragments of
- * many small triangles).  Other values act as 0.
+ * 16x16 pixels per walk of the tile list; tiles of 8x8 fall back to 0/1).
+ * Other values act as 0.
  * Zero fields take defaults. */
 typedef struct {
     uint32_t tileShift;

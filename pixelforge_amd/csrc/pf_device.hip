@@ -1463,12 +1463,10 @@ struct WaveQueueT : PhongStage<PHONG> {
     uint8_t  fpix[64];
 };
 
-// Staged records.  With ROT their 16-byte chunks are rotated by the slot
-// (chunk c of record j at position (c + j) mod 8): lanes reading one field
-// of different records then hit different LDS banks (records 128 bytes
-// apart put one field of every record in the same two banks).  Rotation
-// helps where a flush shades fragments of many triangles (small ones) and
-// costs where triangles are large, so it is a raster mode of its own.
+// Staged records, read and written as 16-byte chunks.  With ROT the chunks
+// of record j would sit rotated by j (chunk c at (c + j) mod 8), which
+// spreads one field of different records over the LDS banks; it measured
+// slower with chunked reads and is off.
 template <bool ROT, typename QT>
 __device__ __forceinline__ void stage_rec(QT& Q, int slot, const uint4* q) {
     uint4* dst = reinterpret_cast<uint4*>(&Q.rec[slot][0]);
@@ -1634,8 +1632,8 @@ __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
 template <int MODE, bool PHONG, int K>
 __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
                                                  int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueT<PHONG>& Q) {
-    constexpr bool ROT = MODE == 4;                // mode 4: mode 0 with rotated staged records
-    constexpr int PM = MODE == 4 ? 0 : MODE;      // 0 packed, 1 in place
+    constexpr bool ROT = false;                   // staged records are not rotated (slower, measured)
+    constexpr int PM = MODE;                      // 0 packed, 1 in place
     constexpr int TW = (K == 1) ? 8 : 16;
     constexpr int RS = 64 / TW;                    // rows between a lane's pixels
     const int lane = threadIdx.x & 63;
@@ -1738,10 +1736,11 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
 #pragma unroll
         for (int k = 0; k < K; k++) pm[k] = 0ull;
         for (int j = 0; j < nt; j++) {
-            // Edges and bbox (chunks 0..3) and meta; the whole record when
-            // this loop shades (in place).
+            // Edges (chunks 0..2), for 16x16 blocks also bbox and meta; the
+            // whole record when this loop shades (in place).
             RTri t;
             if (PM == 1) t = staged_rec<ROT, 8>(Q, j);
+            else if (K == 1) t = staged_rec<ROT, 3>(Q, j);
             else { t = staged_rec<ROT, 4>(Q, j); t.meta = staged_meta<ROT>(Q, j); }
             // A lane's pixels are RS rows apart: each edge value steps by
             // RS * B, and stepping mod 2^32 gives the direct value exactly.
@@ -2495,7 +2494,6 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
-    case 4: launch_raster_ts<4, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
@@ -2938,8 +2936,8 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if (!J) return PFD_E_INVALID;
     float best = 1e30f;
     PFDconfig bestCfg = J->cfg;
-    for (uint32_t mode = 0; mode <= 4; mode++)
-        for (uint32_t ts = (mode == 2 || mode == 3 ? 4 : 3); ts <= 6; ts++) {   // four pixels per lane: 16x16 tiles
+    for (uint32_t mode = 0; mode <= 3; mode++)
+        for (uint32_t ts = (mode >= 2 ? 4 : 3); ts <= 6; ts++) {   // four pixels per lane: 16x16 tiles
             PFDconfig c = J->cfg;
             c.tileShift = ts;
             c.rasterMode = mode;

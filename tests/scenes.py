@@ -104,8 +104,18 @@ ARRAY_SCENES = [
     ("mesh_gouraud_cull", dict(_MESH, seed=83, arrays=1, lighting=1, cull=pf.PF_BACK)),
 ]
 
+# Targets smaller than a block or a tile, with sides that are no multiple of
+# any tile size, and a frame that only clears (GPU against the oracle).
+SIZE_SCENES = [
+    ("tiny_1x1", dict(kind=pf.RANDOM_NDC, width=1, height=1, ntris=20, seed=90)),
+    ("tiny_7x3", dict(kind=pf.RANDOM_NDC, width=7, height=3, ntris=20, seed=91)),
+    ("clear_only", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=0, seed=92)),
+    ("odd_129x77_3d", dict(kind=pf.RANDOM_3D, width=129, height=77, ntris=300, seed=93, texture=1, texW=31,
+                           texH=17, filter=pf.PF_BILINEAR)),
+]
+
 GOLDEN_SCENES = CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES
+ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 
 def params(kw):
