@@ -421,9 +421,9 @@ PF_API void*     pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFs
 
 /* ------------------------------------------------------------------------ */
 /* Outside the GPU triangle path.  Points, lines, rectangles, pixel          */
-/* transfers, fog, post-processing and render lists keep their signatures so */
-/* that callers link; their state setters store what the getters return, and */
-/* the drawing calls set PF_INVALID_OPERATION without drawing.               */
+/* transfers, fog and post-processing keep their signatures so that callers  */
+/* link; their state setters store what the getters return, and the drawing  */
+/* calls set PF_INVALID_OPERATION without drawing.                           */
 /* ------------------------------------------------------------------------ */
 
 PF_API void pfLineWidth(PFfloat width);                                    /* pixelforge.h:823 */
@@ -450,6 +450,15 @@ PF_API void pfFogiv(PFfogparam pname, PFint* param);                       /* pi
 PF_API void pfFogfv(PFfogparam pname, PFfloat* param);                     /* pixelforge.h:1689 */
 PF_API void pfFogProcess(void);                                            /* pixelforge.h:1697 */
 PF_API void pfPostProcess(PFpostprocessfunc postProcessFunction);          /* pixelforge.h:1735 */
+/* ------------------------------------------------------------------------ */
+/* Render lists.  Recording stores each pfBegin's draw mode, materials and   */
+/* texture and each vertex's position, colour, texture coordinate and       */
+/* normal; pfCallList replays them through immediate mode, so the triangles  */
+/* go to the GPU with the frame.  pfNewList and pfCallList save colour,      */
+/* normal, texture coordinate, materials, texture and enable bits in one     */
+/* backup slot, restored by pfEndList and at the end of pfCallList.          */
+/* ------------------------------------------------------------------------ */
+
 PF_API PFrenderlist pfGenList(void);                                       /* pixelforge.h:1768 */
 PF_API void pfDeleteList(PFrenderlist* renderList);                        /* pixelforge.h:1793 */
 PF_API void pfNewList(PFrenderlist renderList);                            /* pixelforge.h:1817 */

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libpixelforge_amd.so")
 SCENES_PATH = os.path.join(HERE, "libpf_scenes.so")
 
 # Scene kinds (csrc/pf_scenes.c)
-RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D, MESH = 1, 2, 3, 4, 5
+RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D, MESH, LIST = 1, 2, 3, 4, 5, 6
 
 # Enum values of include/pixelforge.h used by callers
 PF_FRONT, PF_BACK = 0, 1

@@ -71,6 +71,30 @@ typedef struct {                       /* PFIvertexattribbuffer, context.h:242-2
     PFenum      type;
 } Attrib;
 
+typedef struct {                       /* PFIrendercall, context.h:270-279 */
+    Material    faceMaterial[2];
+    Tex*        texture;
+    PFdrawmode  drawMode;
+    float*      pos;                   /* 4 floats per vertex */
+    float*      uv;                    /* 2 floats per vertex */
+    float*      nrm;                   /* 3 floats per vertex */
+    PFcolor*    col;
+    size_t      n, cap;
+} ListCall;
+
+typedef struct {                       /* PFIrenderlist, context.h:298 */
+    ListCall*   calls;
+    size_t      n, cap;
+} List;
+
+typedef struct {                       /* PFIctxbackup, context.h:300-316 */
+    Material    faceMaterial[2];
+    float       uv[2], nrm[3];
+    PFcolor     color;
+    Tex*        texture;
+    uint32_t    state;
+} Backup;
+
 typedef struct {                       /* the frame being recorded */
     uint32_t*  targetColor;
     float*     targetDepth;
@@ -121,6 +145,8 @@ typedef struct {
     uint32_t vcount;
     Tex*     curTex;
     Attrib   aPos, aNrm, aUV, aCol;    /* vertex arrays (context.c:1161-1229) */
+    List*    curList;                  /* list being recorded (pfNewList), NULL */
+    Backup   backup;                   /* one slot, shared by pfNewList and pfCallList */
 
     float    pointSize, lineWidth;     /* out-of-path state, kept for the getters */
     float    pixelZoom[2], rasterPos[4];
@@ -1202,10 +1228,14 @@ void pfColorMaterial(PFface face, PFenum mode)
 /* Immediate mode (context.c:1580-1913)                                      */
 /* ------------------------------------------------------------------------ */
 
+static void list_begin(Ctx* c, PFdrawmode mode);
+static void list_vertex(Ctx* c, const float* v);
+
 void pfBegin(PFdrawmode mode)
 {
     Ctx* c = G;
     if (!c) return;
+    if (c->curList) { list_begin(c, mode); return; }      /* context.c:1589-1602 */
     if (mode > PF_QUAD_STRIP) { c->err = PF_INVALID_ENUM; return; }
     update_matrices(c, !(mode == PF_POINTS || mode == PF_LINES));
     c->drawMode = mode;
@@ -1218,6 +1248,7 @@ void pfVertex4fv(const PFfloat* v)
 {
     Ctx* c = G;
     if (!c) return;
+    if (c->curList) { list_vertex(c, v); return; }         /* context.c:1675-1683 */
     Vert* d = &c->vb[c->vcount++];
     memcpy(d->position, v, 16);
     memcpy(d->normal, c->curNormal, 12);
@@ -1660,11 +1691,162 @@ void pfFogiv(PFfogparam pname, PFint* param) { (void)pname; (void)param; not_in_
 void pfFogfv(PFfogparam pname, PFfloat* param) { (void)pname; (void)param; not_in_path(); }
 void pfFogProcess(void) { not_in_path(); }
 void pfPostProcess(PFpostprocessfunc fn) { (void)fn; not_in_path(); }
-PFrenderlist pfGenList(void) { not_in_path(); return NULL; }
-void pfDeleteList(PFrenderlist* list) { (void)list; not_in_path(); }
-void pfNewList(PFrenderlist list) { (void)list; not_in_path(); }
-void pfEndList(void) { not_in_path(); }
-void pfCallList(const PFrenderlist list) { (void)list; not_in_path(); }
+
+/* ------------------------------------------------------------------------ */
+/* Render lists (renderlist.c:6-97)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* A list holds the pfBegin calls made while it was being recorded, each
+ * with the materials and texture current at that pfBegin and, per vertex,
+ * the position given to pfVertex and the colour, texture coordinate (after
+ * the texture matrix) and normal current at that moment.  pfCallList feeds
+ * them back through pfBegin / pfColor4ubv / pfTexCoordfv / pfNormal3fv /
+ * pfVertex4fv, so replayed triangles are recorded into the frame like any
+ * other and the matrices, lights and enables of the moment of the call
+ * apply.  Recording and replay each save the colour, texture coordinate,
+ * normal, materials, bound texture and enable bits in the context's one
+ * backup slot and restore them at pfEndList / the end of pfCallList
+ * (internal/context/context.c:26-47); a pfCallList made while recording
+ * overwrites the slot that pfEndList restores from, as in the reference. */
+
+static void make_backup(Ctx* c)
+{
+    memcpy(c->backup.faceMaterial, c->mat, sizeof c->mat);
+    memcpy(c->backup.uv, c->curUV, sizeof c->curUV);
+    memcpy(c->backup.nrm, c->curNormal, sizeof c->curNormal);
+    c->backup.color = c->curColor;
+    c->backup.texture = c->curTex;
+    c->backup.state = c->state;
+}
+
+static void restore_backup(Ctx* c)
+{
+    memcpy(c->mat, c->backup.faceMaterial, sizeof c->mat);
+    memcpy(c->curUV, c->backup.uv, sizeof c->curUV);
+    memcpy(c->curNormal, c->backup.nrm, sizeof c->curNormal);
+    c->curColor = c->backup.color;
+    c->curTex = c->backup.texture;
+    c->state = c->backup.state;
+    c->stateDirty = 1;
+}
+
+static void free_call(ListCall* k)
+{
+    free(k->pos); free(k->uv); free(k->nrm); free(k->col);
+    memset(k, 0, sizeof *k);
+}
+
+static void list_begin(Ctx* c, PFdrawmode mode)
+{
+    List* L = c->curList;
+    if (L->n == L->cap) {
+        size_t cap = L->cap ? 2 * L->cap : 8;
+        ListCall* p = (ListCall*)realloc(L->calls, cap * sizeof *p);
+        if (!p) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+        L->calls = p; L->cap = cap;
+    }
+    ListCall* k = &L->calls[L->n++];
+    memset(k, 0, sizeof *k);
+    memcpy(k->faceMaterial, c->mat, sizeof c->mat);
+    k->texture = c->curTex;
+    k->drawMode = mode;                /* checked when the list is called */
+}
+
+static void list_vertex(Ctx* c, const float* v)
+{
+    List* L = c->curList;
+    /* A vertex with no pfBegin since pfNewList has no call to join (the
+     * reference indexes before its first element). */
+    if (L->n == 0) { c->err = PF_INVALID_OPERATION; return; }
+    ListCall* k = &L->calls[L->n - 1];
+    if (k->n == k->cap) {
+        size_t cap = k->cap ? 2 * k->cap : 8;
+        float* p = (float*)realloc(k->pos, cap * 4 * sizeof(float));
+        if (p) k->pos = p;
+        float* u = (float*)realloc(k->uv, cap * 2 * sizeof(float));
+        if (u) k->uv = u;
+        float* q = (float*)realloc(k->nrm, cap * 3 * sizeof(float));
+        if (q) k->nrm = q;
+        PFcolor* o = (PFcolor*)realloc(k->col, cap * sizeof(PFcolor));
+        if (o) k->col = o;
+        if (!p || !u || !q || !o) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+        k->cap = cap;
+    }
+    memcpy(k->pos + 4 * k->n, v, 4 * sizeof(float));
+    memcpy(k->uv + 2 * k->n, c->curUV, 2 * sizeof(float));
+    memcpy(k->nrm + 3 * k->n, c->curNormal, 3 * sizeof(float));
+    k->col[k->n] = c->curColor;
+    k->n++;
+}
+
+PFrenderlist pfGenList(void)                                 /* renderlist.c:6-15 */
+{
+    return (PFrenderlist)calloc(1, sizeof(List));
+}
+
+void pfDeleteList(PFrenderlist* renderList)                  /* renderlist.c:17-36 */
+{
+    if (!renderList) return;
+    List* L = (List*)*renderList;
+    if (L) {
+        if (G && G->curList == L) G->curList = NULL;
+        for (size_t i = 0; i < L->n; i++) free_call(&L->calls[i]);
+        free(L->calls);
+        free(L);
+    }
+    *renderList = NULL;
+}
+
+void pfNewList(PFrenderlist renderList)                      /* renderlist.c:38-57 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!renderList) { c->err = PF_INVALID_VALUE; return; }
+    List* L = (List*)renderList;
+    for (size_t i = 0; i < L->n; i++) free_call(&L->calls[i]);
+    L->n = 0;
+    c->curList = L;
+    make_backup(c);
+}
+
+void pfEndList(void)                                         /* renderlist.c:59-67 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!c->curList) c->err = PF_INVALID_OPERATION;
+    c->curList = NULL;
+    restore_backup(c);
+}
+
+void pfCallList(const PFrenderlist renderList)              /* renderlist.c:69-97 */
+{
+    Ctx* c = G;
+    if (!c) return;
+    /* The reference dereferences a NULL list. */
+    if (!renderList) { c->err = PF_INVALID_VALUE; return; }
+    const List* L = (const List*)renderList;
+    make_backup(c);
+    /* Calls made while recording into this same list append to it: walk the
+     * count taken at entry. */
+    const size_t ncalls = L->n;
+    for (size_t i = 0; i < ncalls; i++) {
+        const ListCall* k = &L->calls[i];
+        const size_t nv = k->n;
+        memcpy(c->mat, k->faceMaterial, sizeof c->mat);
+        c->stateDirty = 1;
+        pfBindTexture((PFtexture)k->texture);
+        pfBegin(k->drawMode);
+        for (size_t j = 0; j < nv; j++) {
+            const ListCall* kk = &L->calls[i];                /* the array may move */
+            pfColor4ubv((const PFubyte*)&kk->col[j]);
+            pfTexCoordfv(kk->uv + 2 * j);
+            pfNormal3fv(kk->nrm + 3 * j);
+            pfVertex4fv(kk->pos + 4 * j);
+        }
+        pfEnd();
+    }
+    restore_backup(c);
+}
 
 /* ------------------------------------------------------------------------ */
 /* Readback and framebuffers                                                 */

@@ -23,6 +23,9 @@
  *                   per-vertex normals, texture coordinates and float
  *                   colours, drawn through pfDrawElements, pfDrawArrays or
  *                   the equivalent immediate-mode calls.
+ *   PFS_LIST        spheres recorded once into render lists and called at
+ *                   several model-view transforms (pfNewList / pfEndList /
+ *                   pfCallList), including a list recorded by calling another.
  */
 #include <dlfcn.h>
 #include <math.h>
@@ -37,6 +40,7 @@
 #define PFS_SPHERES      3
 #define PFS_RANDOM_3D    4
 #define PFS_MESH         5
+#define PFS_LIST         6
 
 /* Scene description; zero fields take the defaults noted. */
 typedef struct {
@@ -120,6 +124,11 @@ typedef struct {
     void (*ColorPointer)(PFint, PFenum, PFsizei, const void*);
     void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
+    PFrenderlist (*GenList)(void);
+    void (*DeleteList)(PFrenderlist*);
+    void (*NewList)(PFrenderlist);
+    void (*EndList)(void);
+    void (*CallList)(const PFrenderlist);
     void (*Finish)(void);                    /* absent from the reference */
     void (*SubmitBatch)(const PFcontext*, PFsizei);   /* absent from the reference */
 } PFSapi;
@@ -143,6 +152,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(TextureParameter); BIND(GetTexturePixels); BIND(GetError); BIND(Color4f);
     BIND(VertexPointer); BIND(NormalPointer); BIND(TexCoordPointer); BIND(ColorPointer);
     BIND(DrawElements); BIND(DrawArrays);
+    BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
     return a;
@@ -547,6 +557,78 @@ static void scene_spheres(const PFSapi* a, Rng* r, const PFSparams* p)
     a->End();
 }
 
+/* Render lists.  List A holds a unit sphere recorded in immediate mode: its
+ * upper half as PF_TRIANGLES with the scene texture bound, its lower half as
+ * PF_QUADS with no texture.  While A is recorded the colour, normal, texture
+ * coordinate, bound texture and PF_BLEND change; pfEndList restores them.
+ * List B is recorded as a colour change followed by pfCallList(A), whose
+ * context backup replaces the one pfNewList(B) made, so pfEndList(B) keeps
+ * that colour.  The spheres are then A or B called at random model-view
+ * transforms, and one immediate-mode triangle in the current colour ends
+ * the scene. */
+static void scene_list(const PFSapi* a, Rng* r, const PFSparams* p, PFtexture tex)
+{
+    int NS = p->spheres > 0 ? p->spheres : 8;
+    int NLAT = p->sphereLat > 0 ? p->sphereLat : 10, NLON = p->sphereLon > 0 ? p->sphereLon : 20;
+    perspective(a, p, 0.5f, 0.5f, 30.0f);
+    setup_lights(a, r, p, 0.0f, 0.0f, -7.0f, 3.0f);
+    size_t nv = (size_t)(NLAT + 1) * (NLON + 1);
+    uint32_t* col = (uint32_t*)malloc(nv * sizeof(uint32_t));
+    for (size_t v = 0; v < nv; v++) col[v] = (uint32_t)(rnext(r) >> 32) | 0xFF000000u;
+    PFrenderlist A = a->GenList(), B = a->GenList();
+    a->Color4ub(200, 100, 50, 255);
+    a->NewList(A);
+    for (int half = 0; half < 2; half++) {
+        a->BindTexture(half == 0 ? tex : NULL);
+        if (half == 1) a->Enable(PF_BLEND);
+        a->Begin(half == 0 ? PF_TRIANGLES : PF_QUADS);
+        for (int la = half ? NLAT / 2 : 0; la < (half ? NLAT : NLAT / 2); la++)
+            for (int lo = 0; lo < NLON; lo++) {
+                int cc[6][2], n;
+                if (half == 0) {
+                    int t[6][2] = { { la, lo }, { la + 1, lo }, { la + 1, lo + 1 },
+                                    { la, lo }, { la + 1, lo + 1 }, { la, lo + 1 } };
+                    memcpy(cc, t, sizeof t); n = 6;
+                } else {
+                    int q[4][2] = { { la, lo }, { la + 1, lo }, { la + 1, lo + 1 }, { la, lo + 1 } };
+                    memcpy(cc, q, sizeof q); n = 4;
+                }
+                for (int k = 0; k < n; k++) {
+                    float th = (float)cc[k][0] / NLAT * 3.14159265f;
+                    float phi = (float)cc[k][1] / NLON * 6.28318531f;
+                    float nx = sinf(th) * cosf(phi), ny = cosf(th), nz = sinf(th) * sinf(phi);
+                    uint32_t c = col[(size_t)cc[k][0] * (NLON + 1) + cc[k][1]];
+                    a->Color4ub((PFubyte)c, (PFubyte)(c >> 8), (PFubyte)(c >> 16), 255);
+                    a->Normal3f(nx, ny, nz);
+                    a->TexCoord2f((float)cc[k][1] / NLON, (float)cc[k][0] / NLAT);
+                    a->Vertex3f(nx, ny, nz);
+                }
+            }
+        a->End();
+    }
+    a->Color4ub(0, 0, 0, 255);
+    a->EndList();
+    a->NewList(B);
+    a->Color4ub(40, 220, 90, 255);
+    a->CallList(A);
+    a->EndList();
+    for (int s = 0; s < NS; s++) {
+        a->LoadIdentity();
+        a->Translatef(rrange(r, -2.5f, 2.5f), rrange(r, -1.6f, 1.6f), rrange(r, -9.0f, -5.0f));
+        a->Rotatef(rrange(r, 0.0f, 360.0f), rrange(r, -1.0f, 1.0f), rrange(r, -1.0f, 1.0f), rrange(r, 0.1f, 1.0f));
+        a->CallList(s % 3 == 2 ? B : A);
+    }
+    a->LoadIdentity();
+    a->Begin(PF_TRIANGLES);
+    a->Vertex3f(-1.5f, -1.2f, -6.0f);
+    a->Vertex3f(0.5f, -1.4f, -6.5f);
+    a->Vertex3f(-0.4f, 0.2f, -5.5f);
+    a->End();
+    a->DeleteList(&A);
+    a->DeleteList(&B);
+    free(col);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Entry points                                                              */
 /* ------------------------------------------------------------------------ */
@@ -570,6 +652,7 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
         setup_lights(a, r, p, 0.0f, 0.0f, -1.2f, 1.0f);
         scene_mesh(a, r, p);
         break;
+    case PFS_LIST: scene_list(a, r, p, tex->tex); break;
     default: break;
     }
 }

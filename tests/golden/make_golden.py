@@ -6,7 +6,9 @@ thread -- the reference's row-parallel loop races on shared 8-pixel stores
 (triangles.c:402, 434) -- and stores colour (packed RGBA8) and depth in one
 compressed .npz per scene together with the scene parameters.
 
-    OMP_NUM_THREADS=1 python tests/golden/make_golden.py
+    OMP_NUM_THREADS=1 python tests/golden/make_golden.py [NAME ...]
+
+With names, only those scenes are written.
 """
 import json
 import os
@@ -29,7 +31,10 @@ REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
 def main():
     if not os.path.exists(REF_SO):
         sys.exit(f"{REF_SO} missing: run make -C oracle ref")
+    only = set(sys.argv[1:])
     for name, kw in scenes.GOLDEN_SCENES:
+        if only and name not in only:
+            continue
         p = scenes.params(kw)
         color, depth, err = pf.render(p, REF_SO)
         if err:

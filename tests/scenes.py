@@ -104,6 +104,17 @@ ARRAY_SCENES = [
     ("mesh_gouraud_cull", dict(_MESH, seed=83, arrays=1, lighting=1, cull=pf.PF_BACK)),
 ]
 
+# Render lists (pfNewList / pfEndList / pfCallList): spheres recorded once and
+# called at several transforms, a list recorded by calling another, state
+# restored at pfEndList; unlit, Gouraud, and Phong with colour material.
+_LIST = dict(kind=pf.LIST, width=W, height=H, texture=1, texW=33, texH=33, filter=pf.PF_NEAREST)
+LIST_SCENES = [
+    ("list_unlit", dict(_LIST, seed=100, cull=pf.PF_BACK)),
+    ("list_gouraud", dict(_LIST, seed=101, lighting=1, numLights=2, cull=pf.PF_BACK)),
+    ("list_phong_color_material", dict(_LIST, seed=102, lighting=2, numLights=2, colorMaterial=1, cull=-1)),
+    ("list_flat_blend", dict(_LIST, seed=103, shade=pf.PF_FLAT, blend=1, blendMode=1, cull=pf.PF_FRONT)),
+]
+
 # Targets smaller than a block or a tile, with sides that are no multiple of
 # any tile size, and a frame that only clears (GPU against the oracle).
 SIZE_SCENES = [
@@ -114,7 +125,8 @@ SIZE_SCENES = [
                            texH=17, filter=pf.PF_BILINEAR)),
 ]
 
-GOLDEN_SCENES = CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
+GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
+                 + LIST_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 
