@@ -37,6 +37,35 @@
 /* Texture pixel formats accepted by the pipeline. */
 #define PFF_TEX_RGBA8    0u         /* pixel.h:696-700, 2909-2913 */
 #define PFF_TEX_RGBAF32  1u         /* pixel.h:735-746, 2983-3008 */
+#define PFF_TEX_SOURCE   2u         /* any other (PFpixelformat, PFdatatype)
+                                     * pair of the reference's texel getter
+                                     * table (pixel.h:3050-3095), named by
+                                     * PFFtexture.source; its texels are read
+                                     * as the AVX2 getters read them
+                                     * (pixel.h:2250-3040) */
+
+/* PFFtexture.source: PFpixelformat in bits 0-7, PFdatatype in bits 8-15. */
+#define PFF_SOURCE(format, type) ((uint32_t)(format) | ((uint32_t)(type) << 8))
+
+/* Bytes per texel of a source pair (pfiGetPixelBytes, pixel.h:3155-3212);
+ * 0 for a pair the getter table lacks.  Formats: RED 0, GREEN 1, BLUE 2,
+ * ALPHA 3, LUMINANCE 4, LUMINANCE_ALPHA 5, RGB 6, RGBA 7, BGR 8, BGRA 9;
+ * types: UNSIGNED_BYTE 0, UNSIGNED_SHORT_5_6_5 2, _5_5_5_1 3, _4_4_4_4 4,
+ * HALF_FLOAT 9, FLOAT 10. */
+static inline uint32_t pff_source_texel_bytes(uint32_t source)
+{
+    uint32_t f = source & 0xFFu, t = (source >> 8) & 0xFFu;
+    if (f > 9u) return 0u;
+    uint32_t comps = f <= 4u ? 1u : f == 5u ? 2u : (f == 6u || f == 8u) ? 3u : 4u;
+    switch (t) {
+    case 0u:  return comps;
+    case 9u:  return 2u * comps;
+    case 10u: return 4u * comps;
+    case 2u:  return (f == 6u || f == 8u) ? 2u : 0u;          /* RGB / BGR only  */
+    case 3u: case 4u: return (f == 7u || f == 9u) ? 2u : 0u;  /* RGBA / BGRA only */
+    default:  return 0u;
+    }
+}
 
 typedef struct {
     float    position[3];
@@ -81,7 +110,7 @@ typedef struct {
     uint32_t    format;        /* PFF_TEX_* */
     uint32_t    wrap;          /* PFtexturewrap */
     uint32_t    filter;        /* PFtexturefilter */
-    uint32_t    pad;
+    uint32_t    source;        /* PFF_TEX_SOURCE: PFF_SOURCE(format, type) */
 } PFFtexture;
 
 #define PFF_CMD_CLEAR 1u

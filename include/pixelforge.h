@@ -15,10 +15,11 @@
  *     pfSwapBuffers(), pfSetMainBuffer(), pfDeleteContext(), and any change
  *     of render target.  Code that reads a colour buffer directly must call
  *     pfFinish() first (new entry point, no reference counterpart).
- *   - Only RGBA / UNSIGNED_BYTE render targets are accepted; textures may be
- *     RGBA / UNSIGNED_BYTE or RGBA / FLOAT.
- *   - Lines, points, rects, DrawPixels, fog, post-processing, render lists
- *     and vertex arrays are not part of this library.
+ *   - Only RGBA / UNSIGNED_BYTE render targets are accepted.  Textures may
+ *     use every (format, type) pair that has a texel getter in the reference
+ *     (pixel.h:3050-3095); pairs without one are PF_INVALID_ENUM.
+ *   - Lines, points, rects, DrawPixels, fog and post-processing are not part
+ *     of this library (their calls set PF_INVALID_OPERATION).
  *
  * Errors follow the reference convention: a sticky last-error code that
  * pfGetError() reads and clears (context.c:380-385).  GPU failures map to

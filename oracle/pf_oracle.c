@@ -32,6 +32,10 @@
  *       viewport larger than, or offset from, the framebuffer in 3D, where
  *       the bounding box is not clamped) are skipped instead of written out
  *       of bounds.
+ *   D6  The texel getters of the 1-, 2- and 3-byte formats gather 32-bit
+ *       words that run up to 3 bytes past the last texel; those bytes read
+ *       as 0 here (the reference reads whatever follows the caller's
+ *       buffer).
  * Parity of this oracle against the compiled reference is pinned by
  * tests/test_oracle_golden.py (golden frames) and tests/test_oracle_units.py
  * (closed forms and invariants).
@@ -279,8 +283,152 @@ static uint32_t modulate(uint32_t texel, uint32_t frag) { return pforacle_blend(
 /* Texture sampling (sampler.h:202-410)                                      */
 /* ------------------------------------------------------------------------ */
 
-/* RGBA8 gather (pixel.h:2909-2913) and RGBA float gather with cvt_rne(f*255)
- * and no clamp (pixel.h:2983-3008). */
+/* PFpixelformat / PFdatatype values (pixelforge.h:140-153, 406-417). */
+enum { SRC_RED, SRC_GREEN, SRC_BLUE, SRC_ALPHA, SRC_LUMINANCE, SRC_LUMINANCE_ALPHA,
+       SRC_RGB, SRC_RGBA, SRC_BGR, SRC_BGRA };
+enum { SRC_UBYTE = 0, SRC_565 = 2, SRC_5551 = 3, SRC_4444 = 4, SRC_HALF = 9, SRC_FLOAT = 10 };
+
+/* pfmHalfToFloatI (pfm.h:128-145). */
+static float half_bits_to_float(uint32_t h)
+{
+    uint32_t s = (h & 0x8000u) << 16;
+    int32_t em = (int32_t)(h & 0x7FFFu);
+    int32_t r = (em + (112 << 10)) << 13;
+    if (em < (1 << 10)) r = 0;                       /* subnormal: signed zero */
+    if (em >= (31 << 10)) r += 112 << 23;            /* infinity / NaN */
+    uint32_t u = s | (uint32_t)r;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+/* _mm256_i32gather_epi32 lane: the little-endian word at byte o of the
+ * texel bytes; bytes past the last texel read as 0 (D6). */
+static uint32_t gather_word(const uint8_t* p, size_t size, size_t o)
+{
+    uint32_t w = 0;
+    for (int k = 0; k < 4; k++)
+        if (o + (size_t)k < size) w |= (uint32_t)p[o + (size_t)k] << (8 * k);
+    return w;
+}
+
+/* One component: _mm256_cvtps_epi32(v * 255.0f), as an unsigned lane. */
+static uint32_t comp8(const uint8_t* p, size_t size, size_t o, int half)
+{
+    uint32_t w = gather_word(p, size, o);
+    float v;
+    if (half) v = half_bits_to_float(w & 0xFFFFu);
+    else memcpy(&v, &w, 4);
+    return (uint32_t)cvt_rne_i32(v * 255.0f);
+}
+
+/* The AVX2 texel getters of every (format, type) pair other than RGBA8 and
+ * RGBA float (GC_pixelGetters_simd, pixel.h:3050-3095), texel i of the
+ * texture's bytes.  Each gathers 32-bit words at byte i * stride (+ the
+ * component offset) and packs r | g << 8 | b << 16 | a << 24 by ORs of
+ * unmasked lanes. */
+static uint32_t source_texel(const PFFtexture* t, size_t i)
+{
+    const uint8_t* p = (const uint8_t*)t->pixels;
+    const uint32_t f = t->source & 0xFFu, ty = (t->source >> 8) & 0xFFu;
+    const size_t size = (size_t)t->width * t->height * pff_source_texel_bytes(t->source);
+    const uint32_t A = 0xFF000000u;
+    if (ty == SRC_UBYTE) {
+        uint32_t x;
+        switch (f) {
+        case SRC_RED:   return (gather_word(p, size, i) >> 24) | A;                 /* pixel.h:2389-2397 */
+        case SRC_GREEN: return ((gather_word(p, size, i) >> 24) << 8) | A;          /* pixel.h:2400-2407 */
+        case SRC_BLUE:  return ((gather_word(p, size, i) >> 24) << 16) | A;         /* pixel.h:2410-2417 */
+        case SRC_ALPHA: return gather_word(p, size, i) << 24;                       /* pixel.h:2420-2424 */
+        case SRC_LUMINANCE: {                                                       /* pixel.h:2253-2263 */
+            uint32_t lum = (gather_word(p, size, i) >> 24) | A;
+            uint32_t last = lum & 255u;
+            return lum | (last << 16) | (last << 8);
+        }
+        case SRC_LUMINANCE_ALPHA: {                /* pixel.h:2309-2325: bytes 2,2,2,3 of each lane */
+            x = gather_word(p, size, 2 * i);
+            uint32_t b2 = (x >> 16) & 0xFFu, b3 = x >> 24;
+            return b2 | (b2 << 8) | (b2 << 16) | (b3 << 24);
+        }
+        case SRC_RGB: {                                                             /* pixel.h:2604-2632 */
+            x = gather_word(p, size, 3 * i);
+            uint32_t r = (x & 0xFF0000u) >> 16, g = (x & 0x00FF00u) >> 8, b = x & 0xFFu;
+            return (r << 16) | (g << 8) | b | A;
+        }
+        case SRC_BGR: {                                                             /* pixel.h:2635-2663 */
+            x = gather_word(p, size, 3 * i);
+            uint32_t b = (x & 0xFF0000u) >> 16, g = (x & 0x00FF00u) >> 8, r = x & 0xFFu;
+            return (r << 16) | (g << 8) | b | A;
+        }
+        case SRC_BGRA: {                           /* pixel.h:2916-2920: byte shuffle 2,1,0,3 */
+            x = gather_word(p, size, 4 * i);
+            return ((x >> 16) & 0xFFu) | (x & 0xFF00u) | ((x & 0xFFu) << 16) | (x & 0xFF000000u);
+        }
+        default: return 0;
+        }
+    }
+    if (ty == SRC_HALF || ty == SRC_FLOAT) {
+        const int h = ty == SRC_HALF;
+        const size_t cs = h ? 2 : 4;
+        switch (f) {
+        case SRC_RED:   return comp8(p, size, cs * i, h) | A;                       /* pixel.h:2427-2435, 2480-2490 */
+        case SRC_GREEN: return (comp8(p, size, cs * i, h) << 8) | A;                /* pixel.h:2440-2451, 2493-2504 */
+        case SRC_BLUE:  return (comp8(p, size, cs * i, h) << 16) | A;               /* pixel.h:2454-2465, 2507-2518 */
+        case SRC_ALPHA: return comp8(p, size, cs * i, h) << 24;                     /* pixel.h:2468-2477, 2521-2530 */
+        case SRC_LUMINANCE: {                                                       /* pixel.h:2266-2303 */
+            uint32_t l = comp8(p, size, cs * i, h);
+            return (A | (l << 16)) | ((l << 8) | l);
+        }
+        case SRC_LUMINANCE_ALPHA: {                                                 /* pixel.h:2328-2383 */
+            uint32_t l = comp8(p, size, 2 * cs * i, h), a = comp8(p, size, 2 * cs * i + cs, h);
+            return ((a << 24) | (l << 16)) | ((l << 8) | l);
+        }
+        case SRC_RGB: {                                                             /* pixel.h:2666-2694, 2728-2757 */
+            uint32_t r = comp8(p, size, 3 * cs * i, h), g = comp8(p, size, 3 * cs * i + cs, h);
+            uint32_t b = comp8(p, size, 3 * cs * i + 2 * cs, h);
+            return (A | (b << 16)) | ((g << 8) | r);
+        }
+        case SRC_BGR: {            /* pixel.h:2697-2725, 2759-2790: the first component lands in red */
+            uint32_t b = comp8(p, size, 3 * cs * i, h), g = comp8(p, size, 3 * cs * i + cs, h);
+            uint32_t r = comp8(p, size, 3 * cs * i + 2 * cs, h);
+            return (A | (r << 16)) | ((g << 8) | b);
+        }
+        case SRC_RGBA: {                                                            /* pixel.h:2923-2950 */
+            uint32_t r = comp8(p, size, 4 * cs * i, h), g = comp8(p, size, 4 * cs * i + cs, h);
+            uint32_t b = comp8(p, size, 4 * cs * i + 2 * cs, h), a = comp8(p, size, 4 * cs * i + 3 * cs, h);
+            return ((a << 24) | (b << 16)) | ((g << 8) | r);
+        }
+        case SRC_BGRA: {                                                            /* pixel.h:2953-2980, 3013-3040 */
+            uint32_t b = comp8(p, size, 4 * cs * i, h), g = comp8(p, size, 4 * cs * i + cs, h);
+            uint32_t r = comp8(p, size, 4 * cs * i + 2 * cs, h), a = comp8(p, size, 4 * cs * i + 3 * cs, h);
+            return ((a << 24) | (b << 16)) | ((g << 8) | r);
+        }
+        default: return 0;
+        }
+    }
+    const uint32_t x = gather_word(p, size, 2 * i);
+    if (ty == SRC_565) {                                           /* pixel.h:2536-2601 */
+        uint32_t top = ((x & 0xF800u) >> 11) << 3, g = ((x & 0x07E0u) >> 5) << 2, low = (x & 0x001Fu) << 3;
+        if (f == SRC_RGB) return (A | (low << 16)) | ((g << 8) | top);
+        return (A | (top << 16)) | ((g << 8) | low);                              /* BGR */
+    }
+    if (ty == SRC_5551) {                                         /* pixel.h:2794-2849 */
+        uint32_t c11 = ((x >> 11) & 0x1Fu) * (255u / 31u), c6 = ((x >> 6) & 0x1Fu) * (255u / 31u);
+        uint32_t c1 = ((x >> 1) & 0x1Fu) * (255u / 31u), a = (x & 0x1u) * 255u;
+        if (f == SRC_RGBA) return ((a << 24) | (c1 << 16)) | ((c6 << 8) | c11);
+        return ((a << 24) | (c11 << 16)) | ((c6 << 8) | c1);                      /* BGRA */
+    }
+    if (ty == SRC_4444) {                                         /* pixel.h:2852-2907 */
+        uint32_t c12 = ((x >> 12) & 0xFu) * (255u / 15u), c8 = ((x >> 8) & 0xFu) * (255u / 15u);
+        uint32_t c4 = ((x >> 4) & 0xFu) * (255u / 15u), a = (x & 0xFu) * (255u / 15u);
+        if (f == SRC_RGBA) return ((a << 24) | (c4 << 16)) | ((c8 << 8) | c12);
+        return ((a << 24) | (c12 << 16)) | ((c8 << 8) | c4);                      /* BGRA */
+    }
+    return 0;
+}
+
+/* RGBA8 gather (pixel.h:2909-2913), RGBA float gather with cvt_rne(f*255)
+ * and no clamp (pixel.h:2983-3008), and the other pairs (source_texel). */
 static uint32_t texel_at(const PFFtexture* t, int32_t off)
 {
     int64_t n = (int64_t)t->width * t->height;
@@ -288,6 +436,7 @@ static uint32_t texel_at(const PFFtexture* t, int32_t off)
     if (o < 0) o = 0;                       /* D3 */
     if (o >= n) o = n - 1;
     if (t->format == PFF_TEX_RGBA8) return ((const uint32_t*)t->pixels)[o];
+    if (t->format == PFF_TEX_SOURCE) return source_texel(t, (size_t)o);
     const float* p = (const float*)t->pixels + 4 * o;
     int32_t r = cvt_rne_i32(p[0] * 255.0f), g = cvt_rne_i32(p[1] * 255.0f);
     int32_t b = cvt_rne_i32(p[2] * 255.0f), a = cvt_rne_i32(p[3] * 255.0f);
@@ -1050,8 +1199,8 @@ static void diff_taps(const PFFtexture* t, float u, float v, DTaps* tp)
 static void texel_d(const PFFtexture* t, const PFOdiffParams* p, int param, int32_t off, double* o)
 {
     if (param) { for (int c = 0; c < 4; c++) o[c] = p->texels[(size_t)off * 4 + c]; return; }
-    if (t->format == PFF_TEX_RGBA8) {
-        uint32_t c = ((const uint32_t*)t->pixels)[off];
+    if (t->format == PFF_TEX_RGBA8 || t->format == PFF_TEX_SOURCE) {
+        uint32_t c = texel_at(t, off);
         for (int i = 0; i < 4; i++) o[i] = ch(c, i) / 255.0;
         return;
     }

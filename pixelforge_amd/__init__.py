@@ -45,6 +45,7 @@ class SceneParams(C.Structure):
         ("triScale", C.c_float), ("flushEvery", C.c_int32), ("useQuads", C.c_int32),
         ("arrays", C.c_int32), ("indexType", C.c_int32), ("posType", C.c_int32),
         ("colType", C.c_int32), ("colSize", C.c_int32),
+        ("texFormat", C.c_int32), ("texType", C.c_int32), ("texSwizzle", C.c_int32),
     ]
 
 

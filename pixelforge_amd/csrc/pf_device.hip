@@ -2218,6 +2218,111 @@ __global__ __launch_bounds__(256) void k_diff_light_sum(const float* __restrict_
     if (threadIdx.x == 0) out[i] = red[0];
 }
 
+// ---------------------------------------------------------------------------
+// Texels of the other source formats, decoded once to RGBA8 as the
+// reference's AVX2 texel getters read them (pixel.h:2250-3040).  Each getter
+// is a function of the texture bytes alone, so decoding at upload and
+// sampling RGBA8 gives the texels the reference's sampler sees.  The getters
+// gather 32-bit words at byte offset (texel index x a stride); the
+// single-byte formats keep the top byte of that word (the texel three
+// ahead), LUMINANCE_ALPHA bytes the third and fourth (the next texel), ALPHA
+// bytes leaves red, green and blue at 0, and BGR half / float pack their
+// first component into red.  Half and float components are converted by
+// cvt_rne(v * 255) with no clamp, ORed into place.  Bytes past the last
+// texel read as 0 (the source copy carries 4 zero bytes).
+// ---------------------------------------------------------------------------
+
+// pfmHalfToFloatI (pfm.h:128-145): subnormals flush to signed zero;
+// infinities and NaNs keep their payload.
+__device__ __forceinline__ float half_to_float_ref(uint32_t h) {
+    const uint32_t s = (h & 0x8000u) << 16;
+    const int32_t em = (int32_t)(h & 0x7FFFu);
+    int32_t r = (em + (112 << 10)) << 13;
+    r = em < (1 << 10) ? 0 : r;
+    r += em >= (31 << 10) ? (112 << 23) : 0;
+    return __uint_as_float(s | (uint32_t)r);
+}
+
+// Little-endian 32-bit word at byte offset o (any alignment).
+__device__ __forceinline__ uint32_t src_word(const uint8_t* __restrict__ p, size_t o) {
+    return (uint32_t)p[o] | ((uint32_t)p[o + 1] << 8) | ((uint32_t)p[o + 2] << 16) | ((uint32_t)p[o + 3] << 24);
+}
+__device__ __forceinline__ uint32_t c8_half(const uint8_t* __restrict__ p, size_t o) {
+    return (uint32_t)cvt_rne_i32(half_to_float_ref(src_word(p, o) & 0xFFFFu) * 255.0f);
+}
+__device__ __forceinline__ uint32_t c8_float(const uint8_t* __restrict__ p, size_t o) {
+    return (uint32_t)cvt_rne_i32(__uint_as_float(src_word(p, o)) * 255.0f);
+}
+
+__device__ uint32_t decode_source_texel(const uint8_t* __restrict__ p, size_t i, uint32_t source) {
+    const uint32_t f = source & 0xFFu, t = (source >> 8) & 0xFFu;
+    const uint32_t A = 0xFF000000u;
+    if (t == 0u) {                                                    // UNSIGNED_BYTE
+        switch (f) {
+        case 0u: return (src_word(p, i) >> 24) | A;                   // pixel.h:2389-2397
+        case 1u: return ((src_word(p, i) >> 24) << 8) | A;            // :2400-2407
+        case 2u: return ((src_word(p, i) >> 24) << 16) | A;           // :2410-2417
+        case 3u: return src_word(p, i) << 24;                         // :2420-2424
+        case 4u: { const uint32_t g = src_word(p, i) >> 24;           // :2253-2263
+                   return A | (g << 16) | (g << 8) | g; }
+        case 5u: { const uint32_t x = src_word(p, 2 * i), g = (x >> 16) & 0xFFu;   // :2309-2325
+                   return (x & 0xFF000000u) | (g << 16) | (g << 8) | g; }
+        case 6u: return (src_word(p, 3 * i) & 0x00FFFFFFu) | A;       // :2604-2632
+        case 8u: { const uint32_t x = src_word(p, 3 * i);             // :2635-2663
+                   return A | ((x & 0xFFu) << 16) | (x & 0xFF00u) | ((x >> 16) & 0xFFu); }
+        case 9u: { const uint32_t x = src_word(p, 4 * i);             // :2916-2920
+                   return (x & 0xFF00FF00u) | ((x & 0xFFu) << 16) | ((x >> 16) & 0xFFu); }
+        default: return 0u;
+        }
+    }
+    if (t == 9u || t == 10u) {                                        // HALF_FLOAT, FLOAT
+        const bool h = t == 9u;
+        const size_t cs = h ? 2 : 4;                                  // component stride
+        auto c = [&](size_t o) { return h ? c8_half(p, o) : c8_float(p, o); };
+        switch (f) {
+        case 0u: return c(cs * i) | A;                                // :2427-2435, :2480-2490
+        case 1u: return (c(cs * i) << 8) | A;                         // :2440-2451, :2493-2504
+        case 2u: return (c(cs * i) << 16) | A;                        // :2454-2465, :2507-2518
+        case 3u: return c(cs * i) << 24;                              // :2468-2477, :2521-2530
+        case 4u: { const uint32_t l = c(cs * i);                      // :2266-2283, :2286-2303
+                   return A | (l << 16) | (l << 8) | l; }
+        case 5u: { const uint32_t l = c(2 * cs * i), a = c(2 * cs * i + cs);       // :2328-2354, :2357-2383
+                   return (a << 24) | (l << 16) | (l << 8) | l; }
+        case 6u: case 8u: {                                           // :2666-2694, :2697-2725, :2728-2790
+                   const size_t o = 3 * cs * i;
+                   return A | (c(o + 2 * cs) << 16) | (c(o + cs) << 8) | c(o); }
+        case 7u: { const size_t o = 4 * cs * i;                       // :2923-2950 (RGBA float is PFF_TEX_RGBAF32)
+                   return (c(o + 3 * cs) << 24) | (c(o + 2 * cs) << 16) | (c(o + cs) << 8) | c(o); }
+        case 9u: { const size_t o = 4 * cs * i;                       // :2953-2980, :3013-3040
+                   return (c(o + 3 * cs) << 24) | (c(o) << 16) | (c(o + cs) << 8) | c(o + 2 * cs); }
+        default: return 0u;
+        }
+    }
+    const uint32_t x = src_word(p, 2 * i);                            // packed 16-bit texels
+    if (t == 2u) {                                                    // 5_6_5, :2536-2601
+        const uint32_t hi = ((x & 0xF800u) >> 11) << 3, g = ((x & 0x07E0u) >> 5) << 2, lo = (x & 0x1Fu) << 3;
+        return f == 6u ? (A | (lo << 16) | (g << 8) | hi) : (A | (hi << 16) | (g << 8) | lo);
+    }
+    if (t == 3u) {                                                    // 5_5_5_1, :2794-2849
+        const uint32_t c0 = ((x >> 11) & 0x1Fu) * 8u, c1 = ((x >> 6) & 0x1Fu) * 8u, c2 = ((x >> 1) & 0x1Fu) * 8u;
+        const uint32_t a = (x & 1u) * 255u;
+        return f == 7u ? ((a << 24) | (c2 << 16) | (c1 << 8) | c0) : ((a << 24) | (c0 << 16) | (c1 << 8) | c2);
+    }
+    if (t == 4u) {                                                    // 4_4_4_4, :2852-2907
+        const uint32_t c0 = ((x >> 12) & 0xFu) * 17u, c1 = ((x >> 8) & 0xFu) * 17u, c2 = ((x >> 4) & 0xFu) * 17u;
+        const uint32_t a = (x & 0xFu) * 17u;
+        return f == 7u ? ((a << 24) | (c2 << 16) | (c1 << 8) | c0) : ((a << 24) | (c0 << 16) | (c1 << 8) | c2);
+    }
+    return 0u;
+}
+
+// One thread per texel; src holds n texels of the source pair plus 4 bytes.
+__global__ __launch_bounds__(256) void k_decode_texels(const uint8_t* __restrict__ src, size_t n, uint32_t source,
+                                                       uint32_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = decode_source_texel(src, i, source);
+}
+
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -2664,13 +2769,38 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             DTex d;
             memset(&d, 0, sizeof d);
             d.w = T.width; d.h = T.height; d.format = T.format; d.wrap = T.wrap; d.filter = T.filter;
-            size_t bytes = (size_t)T.width * T.height * (T.format == PFF_TEX_RGBA8 ? 4 : 16);
+            const size_t texels = (size_t)T.width * T.height;
+            const uint32_t srcBytes = T.format == PFF_TEX_SOURCE ? pff_source_texel_bytes(T.source) : 0u;
+            if (T.format > PFF_TEX_SOURCE || (T.format == PFF_TEX_SOURCE && srcBytes == 0u)) {
+                snprintf(g_err, sizeof g_err, "texture format %u / source 0x%x not supported", T.format, T.source);
+                pfdJobDestroy(J);
+                return PFD_E_INVALID;
+            }
+            size_t bytes = texels * (T.format == PFF_TEX_RGBAF32 ? 16 : 4);
             J->texData.emplace_back();
             if ((rc = ensure(J->texData.back(), bytes ? bytes : 4))) { pfdJobDestroy(J); return rc; }
-            if (bytes) {
+            if (bytes && T.format == PFF_TEX_SOURCE) {
+                // Raw texels + 4 zero bytes (the getters' word reads run up
+                // to 3 bytes past the last texel), decoded to RGBA8.
+                DevBuf raw;
+                const size_t rawBytes = texels * srcBytes;
+                if ((rc = ensure(raw, rawBytes + 4))) { pfdJobDestroy(J); return rc; }
+                e = hipMemcpy(raw.p, T.pixels, rawBytes, hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = hipMemset((uint8_t*)raw.p + rawBytes, 0, 4);
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL(k_decode_texels, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, J->stream,
+                                       (const uint8_t*)raw.p, texels, T.source, (uint32_t*)J->texData.back().p);
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+                release(raw);
+                if (e != hipSuccess) { set_error("texture decode", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+                d.format = PFF_TEX_RGBA8;
+            } else if (bytes) {
                 e = hipMemcpy(J->texData.back().p, T.pixels, bytes, hipMemcpyHostToDevice);
                 if (e != hipSuccess) { set_error("texture upload", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
             }
+            if (T.format == PFF_TEX_SOURCE) d.format = PFF_TEX_RGBA8;
             d.pixels = J->texData.back().p;
             htex.push_back(d);
         }

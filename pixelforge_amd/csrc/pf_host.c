@@ -398,7 +398,9 @@ static int rec_texture(Ctx* c, const Tex* t)
     e->pixels = t->pixels;
     e->width = t->w;
     e->height = t->h;
-    e->format = (t->type == PF_FLOAT) ? PFF_TEX_RGBAF32 : PFF_TEX_RGBA8;
+    if (t->format == PF_RGBA && t->type == PF_UNSIGNED_BYTE) e->format = PFF_TEX_RGBA8;
+    else if (t->format == PF_RGBA && t->type == PF_FLOAT) e->format = PFF_TEX_RGBAF32;
+    else { e->format = PFF_TEX_SOURCE; e->source = PFF_SOURCE(t->format, t->type); }
     e->wrap = t->wrap;
     e->filter = t->filter;
     r->texKeys[r->ntex] = t;
@@ -648,9 +650,13 @@ static void update_matrices(Ctx* c, int withNormal)
 /* Textures (texture.c:30-123)                                               */
 /* ------------------------------------------------------------------------ */
 
+/* Texture formats: every (format, type) pair with a texel getter
+ * (pixel.h:3050-3095).  The reference accepts any pair in range
+ * (pfiIsPixelFormatValid, pixel.h:3149-3153) and calls a NULL getter when a
+ * pair without one is sampled; here such a pair is PF_INVALID_ENUM. */
 static int format_ok(PFpixelformat f, PFdatatype t)
 {
-    return f == PF_RGBA && (t == PF_UNSIGNED_BYTE || t == PF_FLOAT);
+    return pff_source_texel_bytes(PFF_SOURCE(f, t)) != 0u;
 }
 
 PFtexture pfGenTexture(void* pixels, PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)

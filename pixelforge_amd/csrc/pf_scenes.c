@@ -75,6 +75,10 @@ typedef struct {
     int32_t  posType;        /* mesh arrays: 0 float, 1 double positions             */
     int32_t  colType;        /* mesh arrays: 0 float, 1 ubyte, 2 ushort, 3 uint, 4 double */
     int32_t  colSize;        /* mesh arrays: 3 or 4 colour components (0: 4)         */
+    int32_t  texFormat;      /* texture = 3: PFpixelformat of the texture            */
+    int32_t  texType;        /* texture = 3: PFdatatype of the texture               */
+    int32_t  texSwizzle;     /* texture = 3, BGRA / UNSIGNED_BYTE: hand the same
+                              * texels over as RGBA / UNSIGNED_BYTE instead      */
 } PFSparams;
 
 typedef struct {
@@ -198,13 +202,59 @@ typedef struct {
     void*     pixels;
 } SceneTex;
 
+/* Bytes per texel of a (format, type) pair with a texel getter. */
+static size_t texel_bytes(int f, int t)
+{
+    size_t comps = f <= PF_LUMINANCE ? 1 : f == PF_LUMINANCE_ALPHA ? 2 : (f == PF_RGB || f == PF_BGR) ? 3 : 4;
+    if (t == PF_UNSIGNED_SHORT_5_6_5 || t == PF_UNSIGNED_SHORT_5_5_5_1 || t == PF_UNSIGNED_SHORT_4_4_4_4) return 2;
+    return comps * (t == PF_UNSIGNED_BYTE ? 1 : t == PF_HALF_FLOAT ? 2 : 4);
+}
+
+/* Half-float bit patterns: mostly values in [0, 1], one in 16 a special
+ * (above 1, negative, infinite, NaN, subnormal, 255). */
+static uint16_t rand_half(Rng* r)
+{
+    static const uint16_t special[8] = { 0x3E00, 0xBC00, 0x7C00, 0x7E00, 0x0001, 0x5BF8, 0x3BFF, 0x8000 };
+    uint64_t v = rnext(r);
+    if ((v & 15u) == 0) return special[(v >> 4) & 7u];
+    return (uint16_t)((v >> 16) % 0x3C01u);
+}
+
+static float rand_float(Rng* r)
+{
+    static const float special[8] = { 1.5f, -0.25f, 300.0f, 1e10f, -1e10f, 0.998f, 1.0f / 510.0f, 0.0f };
+    uint64_t v = rnext(r);
+    if ((v & 15u) == 0) {
+        uint32_t k = (uint32_t)(v >> 4) & 7u;
+        if (k == 7u) { uint32_t nan = 0x7FC00000u; float f; memcpy(&f, &nan, 4); return f; }
+        return special[k];
+    }
+    return (float)(v >> 40) * (1.0f / 16777216.0f);
+}
+
 static SceneTex make_texture(const PFSapi* a, Rng* r, const PFSparams* p)
 {
     SceneTex st = { NULL, NULL };
     if (!p->texture) return st;
     uint32_t w = p->texW ? (uint32_t)p->texW : 256, h = p->texH ? (uint32_t)p->texH : 256;
     size_t n = (size_t)w * h;
-    if (p->texture == 2) {
+    if (p->texture == 3) {
+        /* Any (format, type) pair: random texel bytes, 64 zero bytes after
+         * the last texel (the reference's getters read up to 3 past it). */
+        int f = p->texFormat, t = p->texType;
+        size_t bpp = texel_bytes(f, t);
+        uint8_t* px = (uint8_t*)calloc(n * bpp + 64, 1);
+        size_t comps = (t == PF_HALF_FLOAT) ? n * bpp / 2 : (t == PF_FLOAT) ? n * bpp / 4 : 0;
+        if (t == PF_HALF_FLOAT) for (size_t i = 0; i < comps; i++) ((uint16_t*)px)[i] = rand_half(r);
+        else if (t == PF_FLOAT) for (size_t i = 0; i < comps; i++) ((float*)px)[i] = rand_float(r);
+        else for (size_t i = 0; i < n * bpp; i++) px[i] = (uint8_t)(rnext(r) >> 56);
+        if (p->texSwizzle && f == PF_BGRA && t == PF_UNSIGNED_BYTE) {
+            for (size_t i = 0; i < n; i++) { uint8_t b = px[4 * i]; px[4 * i] = px[4 * i + 2]; px[4 * i + 2] = b; }
+            f = PF_RGBA;
+        }
+        st.pixels = px;
+        st.tex = a->GenTexture(px, w, h, (PFpixelformat)f, (PFdatatype)t);
+    } else if (p->texture == 2) {
         float* px = (float*)malloc(n * 4 * sizeof(float) + 64);
         for (size_t i = 0; i < 4 * n; i++) px[i] = runit(r);
         st.pixels = px;

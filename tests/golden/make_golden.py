@@ -8,7 +8,9 @@ compressed .npz per scene together with the scene parameters.
 
     OMP_NUM_THREADS=1 python tests/golden/make_golden.py [NAME ...]
 
-With names, only those scenes are written.
+With names, only those scenes are written.  A scene listed in
+scenes.GOLDEN_REF_OVERRIDES is rendered by the reference with those extra
+parameters (the file keeps the scene's own parameters).
 """
 import json
 import os
@@ -35,7 +37,7 @@ def main():
     for name, kw in scenes.GOLDEN_SCENES:
         if only and name not in only:
             continue
-        p = scenes.params(kw)
+        p = scenes.params(dict(kw, **scenes.GOLDEN_REF_OVERRIDES.get(name, {})))
         color, depth, err = pf.render(p, REF_SO)
         if err:
             sys.exit(f"{name}: reference reported error {err}")

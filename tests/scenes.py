@@ -104,6 +104,31 @@ ARRAY_SCENES = [
     ("mesh_gouraud_cull", dict(_MESH, seed=83, arrays=1, lighting=1, cull=pf.PF_BACK)),
 ]
 
+# Texture formats: every (format, type) pair with a texel getter other than
+# RGBA8 and RGBA float (covered above), sampled NEAREST at a small size.
+# Names: tex_<format>_<type>.  Half and float texels include values above 1,
+# negative, infinite and NaN.
+_FORMATS = dict(red=0, green=1, blue=2, alpha=3, lum=4, lumalpha=5, rgb=6, rgba=7, bgr=8, bgra=9)
+_TYPES = dict(ubyte=0, u565=2, u5551=3, u4444=4, half=9, float=10)
+_PAIRS = ([(f, t) for f in ("red", "green", "blue", "alpha", "lum", "lumalpha") for t in ("ubyte", "half", "float")]
+          + [(f, t) for f in ("rgb", "bgr") for t in ("ubyte", "u565", "half", "float")]
+          + [("rgba", t) for t in ("u5551", "u4444", "half")]
+          + [("bgra", t) for t in ("ubyte", "u5551", "u4444", "half", "float")])
+FORMAT_SCENES = [
+    (f"tex_{f}_{t}", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=200 + k, texture=3,
+                          texFormat=_FORMATS[f], texType=_TYPES[t], texW=13, texH=11, filter=pf.PF_NEAREST))
+    for k, (f, t) in enumerate(_PAIRS)
+]
+
+# Golden frames rendered by the reference with extra scene parameters.
+# BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
+# repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
+# simd.h:564-574), so each group of four SIMD lanes gets its first lane's
+# texel; this library decodes every texel on its own (as the scalar getter,
+# pixel.h:702-706), which equals the reference sampling the same texels
+# handed over as RGBA / UNSIGNED_BYTE.
+GOLDEN_REF_OVERRIDES = {"tex_bgra_ubyte": dict(texSwizzle=1)}
+
 # Render lists (pfNewList / pfEndList / pfCallList): spheres recorded once and
 # called at several transforms, a list recorded by calling another, state
 # restored at pfEndList; unlit, Gouraud, and Phong with colour material.
@@ -126,7 +151,7 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

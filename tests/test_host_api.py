@@ -73,6 +73,34 @@ def test_unsupported_formats_are_rejected():
         L2.pfDeleteContext(ctx)
 
 
+def test_texture_formats_with_a_getter_are_accepted():
+    """pfGenTexture takes exactly the (format, type) pairs of the reference's
+    texel getter table (pixel.h:3050-3095): 10 formats; UNSIGNED_BYTE,
+    HALF_FLOAT and FLOAT for all, 5_6_5 for RGB/BGR, 5_5_5_1 and 4_4_4_4
+    for RGBA/BGRA."""
+    L, ctx, _ = _ctx()
+    buf = np.zeros(64 * 16, np.uint32)
+    L.pfGenTexture.restype = C.c_void_p
+    L.pfDeleteTexture.argtypes = [C.POINTER(C.c_void_p), C.c_bool]
+    try:
+        accepted = set()
+        for f in range(10):
+            for t in range(12):
+                tex = C.c_void_p(L.pfGenTexture(buf.ctypes.data, 4, 4, f, t))
+                err = L.pfGetError()
+                if tex.value:
+                    accepted.add((f, t))
+                    assert err == 0
+                    L.pfDeleteTexture(C.byref(tex), False)
+                else:
+                    assert err == 1                    # PF_INVALID_ENUM
+        expect = {(f, t) for f in range(10) for t in (0, 9, 10)}
+        expect |= {(f, 2) for f in (6, 8)} | {(f, t) for f in (7, 9) for t in (3, 4)}
+        assert accepted == expect and len(expect) == 36
+    finally:
+        L.pfDeleteContext(ctx)
+
+
 EXEC = pf.EXECUTOR
 
 
