@@ -1863,13 +1863,115 @@ static void finish_target(const void* colorPixels)
     if (G && G->rec.ncmd && G->rec.targetColor == colorPixels) submit(G);
 }
 
+/* pfmFloatToHalfI (pfm.h:107-126): mantissa rounded half up, results below
+ * 2^-14 flushed to signed zero, overflow to infinity, every NaN to 0x7E00. */
+static uint16_t float_to_half_ref(float x)
+{
+    uint32_t ui;
+    memcpy(&ui, &x, 4);
+    int32_t s = (int32_t)((ui >> 16) & 0x8000u);
+    int32_t em = (int32_t)(ui & 0x7FFFFFFFu);
+    int32_t h = (em - (112 << 23) + (1 << 12)) >> 13;
+    if (em < (113 << 23)) h = 0;
+    if (em >= (143 << 23)) h = 0x7C00;
+    if (em > (255 << 23)) h = 0x7E00;
+    return (uint16_t)(s | h);
+}
+
+/* PF_COLOR_GARYSCALE (pixel.h:69-74), in float, left to right. */
+static float gray_of(PFcolor c)
+{
+    const float inv = (float)PF_INV_255_D;
+    return c.r * inv * 0.299f + c.g * inv * 0.587f + c.b * inv * 0.114f;
+}
+
+/* One pixel through the reference's scalar setter of (format, type)
+ * (GC_pixelSetters, pixel.h:815-858; setters pixel.h:78-400).  The
+ * normalising macros multiply by (float)(1/255) (pixel.h:39-67) -- the BGRA
+ * one does not swap red and blue, so the BGRA 5_5_5_1, 4_4_4_4, HALF and
+ * FLOAT setters store RGBA order -- while the single-channel HALF/FLOAT
+ * setters divide by 255.0f. */
+static void set_pixel(void* dst, size_t i, PFpixelformat f, PFdatatype t, PFcolor c)
+{
+    const float inv = (float)PF_INV_255_D;
+    uint8_t* b = (uint8_t*)dst;
+    uint16_t* h = (uint16_t*)dst;
+    float* fl = (float*)dst;
+    if (f <= PF_ALPHA) {                                                   /* pixel.h:131-196 */
+        const uint8_t v = f == PF_RED ? c.r : f == PF_GREEN ? c.g : f == PF_BLUE ? c.b : c.a;
+        if (t == PF_UNSIGNED_BYTE) b[i] = v;
+        else if (t == PF_HALF_FLOAT) h[i] = float_to_half_ref(v / 255.0f);
+        else fl[i] = v / 255.0f;
+        return;
+    }
+    if (f == PF_LUMINANCE) {                                               /* pixel.h:78-97 */
+        const float g = gray_of(c);
+        if (t == PF_UNSIGNED_BYTE) b[i] = (uint8_t)(255.0f * g);
+        else if (t == PF_HALF_FLOAT) h[i] = float_to_half_ref(g);
+        else fl[i] = g;
+        return;
+    }
+    if (f == PF_LUMINANCE_ALPHA) {                                         /* pixel.h:102-127 */
+        const float g = gray_of(c);
+        if (t == PF_UNSIGNED_BYTE) { b[2 * i] = (uint8_t)(255 * g); b[2 * i + 1] = c.a; }
+        else if (t == PF_HALF_FLOAT) { h[2 * i] = float_to_half_ref(g); h[2 * i + 1] = float_to_half_ref(c.a * inv); }
+        else { fl[2 * i] = g; fl[2 * i + 1] = c.a * inv; }
+        return;
+    }
+    if (f == PF_RGB || f == PF_BGR) {                                      /* pixel.h:206-262 */
+        const int swap = f == PF_BGR;
+        const uint8_t c0 = swap ? c.b : c.r, c2 = swap ? c.r : c.b;
+        const float n0 = c0 * inv, n1 = c.g * inv, n2 = c2 * inv;
+        if (t == PF_UNSIGNED_BYTE) { b[3 * i] = c0; b[3 * i + 1] = c.g; b[3 * i + 2] = c2; }
+        else if (t == PF_UNSIGNED_SHORT_5_6_5)
+            h[i] = (uint16_t)(((uint16_t)(uint8_t)roundf(n0 * 31.0f) << 11) | ((uint16_t)(uint8_t)roundf(n1 * 63.0f) << 5)
+                              | (uint16_t)(uint8_t)roundf(n2 * 31.0f));
+        else if (t == PF_HALF_FLOAT) { h[3 * i] = float_to_half_ref(n0); h[3 * i + 1] = float_to_half_ref(n1); h[3 * i + 2] = float_to_half_ref(n2); }
+        else { fl[3 * i] = n0; fl[3 * i + 1] = n1; fl[3 * i + 2] = n2; }
+        return;
+    }
+    /* RGBA / BGRA (pixel.h:293-400). */
+    const float n0 = c.r * inv, n1 = c.g * inv, n2 = c.b * inv, n3 = c.a * inv;
+    if (t == PF_UNSIGNED_BYTE) {
+        if (f == PF_RGBA) { b[4 * i] = c.r; b[4 * i + 1] = c.g; b[4 * i + 2] = c.b; b[4 * i + 3] = c.a; }
+        else { b[4 * i] = c.b; b[4 * i + 1] = c.g; b[4 * i + 2] = c.r; b[4 * i + 3] = c.a; }
+    } else if (t == PF_UNSIGNED_SHORT_5_5_5_1) {
+        const uint16_t a1 = (double)n3 > (double)(float)50 * PF_INV_255_D ? 1 : 0;   /* config.h:45-47 */
+        h[i] = (uint16_t)(((uint16_t)(uint8_t)roundf(n0 * 31.0f) << 11) | ((uint16_t)(uint8_t)roundf(n1 * 31.0f) << 6)
+                          | ((uint16_t)(uint8_t)roundf(n2 * 31.0f) << 1) | a1);
+    } else if (t == PF_UNSIGNED_SHORT_4_4_4_4) {
+        h[i] = (uint16_t)(((uint16_t)(uint8_t)roundf(n0 * 15.0f) << 12) | ((uint16_t)(uint8_t)roundf(n1 * 15.0f) << 8)
+                          | ((uint16_t)(uint8_t)roundf(n2 * 15.0f) << 4) | (uint16_t)(uint8_t)roundf(n3 * 15.0f));
+    } else if (t == PF_HALF_FLOAT) {
+        h[4 * i] = float_to_half_ref(n0); h[4 * i + 1] = float_to_half_ref(n1);
+        h[4 * i + 2] = float_to_half_ref(n2); h[4 * i + 3] = float_to_half_ref(n3);
+    } else {
+        fl[4 * i] = n0; fl[4 * i + 1] = n1; fl[4 * i + 2] = n2; fl[4 * i + 3] = n3;
+    }
+}
+
 void pfReadPixels(PFint x, PFint y, PFsizei width, PFsizei height,
                   PFpixelformat format, PFdatatype type, void* pixels)   /* context.c:2349-2395 */
 {
     Ctx* c = G;
     if (!c) return;
-    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { c->err = PF_INVALID_ENUM; return; }
+    /* Every pair with a pixel setter (pixel.h:815-858); the reference checks
+     * the same table (context.c:2351-2361). */
+    if (pff_source_texel_bytes(PFF_SOURCE(format, type)) == 0u) { c->err = PF_INVALID_ENUM; return; }
     submit(c);
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) {
+        const Tex* t = fb_tex(c->curFB);
+        int32_t xMin = x < 0 ? 0 : (x > (int32_t)t->w - 1 ? (int32_t)t->w - 1 : x);
+        int32_t yMin = y < 0 ? 0 : (y > (int32_t)t->h - 1 ? (int32_t)t->h - 1 : y);
+        int64_t xe = (int64_t)x + (int32_t)width, ye = (int64_t)y + (int32_t)height;
+        int32_t xMax = (int32_t)(xe < 0 ? 0 : (xe > t->w ? t->w : xe));
+        int32_t yMax = (int32_t)(ye < 0 ? 0 : (ye > t->h ? t->h : ye));
+        const PFcolor* src = (const PFcolor*)t->pixels;
+        for (int32_t ys = yMin, yd = 0; ys < yMax; ys++, yd++)
+            for (int32_t xs = xMin, xd = 0; xs < xMax; xs++, xd++)
+                set_pixel(pixels, (size_t)yd * width + xd, format, type, src[(size_t)ys * t->w + xs]);
+        return;
+    }
     const Tex* t = fb_tex(c->curFB);
     int32_t xMin = x < 0 ? 0 : (x > (int32_t)t->w - 1 ? (int32_t)t->w - 1 : x);
     int32_t yMin = y < 0 ? 0 : (y > (int32_t)t->h - 1 ? (int32_t)t->h - 1 : y);
