@@ -113,8 +113,25 @@ typedef struct {
     uint32_t    source;        /* PFF_TEX_SOURCE: PFF_SOURCE(format, type) */
 } PFFtexture;
 
-#define PFF_CMD_CLEAR 1u
-#define PFF_CMD_DRAW  2u
+#define PFF_CMD_CLEAR  1u
+#define PFF_CMD_DRAW   2u
+#define PFF_CMD_PIXELS 3u      /* pfDrawPixels: PFFframe.pixelOps[first]      */
+
+/* One pfDrawPixels call (context.c:1987-2080), with everything the call
+ * computes from the context before its pixel loop. */
+typedef struct {
+    const void* pixels;        /* copy of the caller's image: width*height
+                                  texels of `source`                          */
+    uint32_t    width, height;
+    uint32_t    source;        /* PFF_SOURCE(format, type)                    */
+    int32_t     xScreen, yScreen;          /* raster position on screen       */
+    int32_t     xMin, yMin, xMax, yMax;    /* destination rectangle, inclusive */
+    float       zPos;          /* depth written and tested                     */
+    float       invXLen, invYLen;          /* 1 / (width * zoom), 1 / (height * zoom) */
+    uint32_t    flags;         /* PFF_DEPTH_TEST, PFF_BLEND                    */
+    uint32_t    depthFunc;     /* PFdepthmode */
+    uint32_t    blendMode;     /* PFblendmode */
+} PFFpixels;
 
 typedef struct {
     uint32_t kind;             /* PFF_CMD_* */
@@ -145,6 +162,7 @@ typedef struct {
     const PFFstate*    states;   uint32_t numStates;
     const PFFtexture*  textures; uint32_t numTextures;
     const PFFcmd*      cmds;     uint32_t numCmds;
+    const PFFpixels*   pixelOps; uint32_t numPixelOps;
 } PFFframe;
 
 #endif /* PF_FRAME_H */

@@ -36,6 +36,13 @@
  *       words that run up to 3 bytes past the last texel; those bytes read
  *       as 0 here (the reference reads whatever follows the caller's
  *       buffer).
+ *   D7  BGRA / UNSIGNED_BYTE texels are decoded one by one (bytes 2,1,0,3);
+ *       the AVX2 getter's shuffle (pixel.h:2916-2920 with the mask of
+ *       simd.h:564-574) gives every group of four SIMD lanes its first
+ *       lane's texel, which depends on how pixels are grouped.
+ *   D8  A pfDrawPixels source offset past the image (possible only for a
+ *       raster position beyond the viewport's far edge) reads the image's
+ *       last texel; the reference reads past the caller's buffer.
  * Parity of this oracle against the compiled reference is pinned by
  * tests/test_oracle_golden.py (golden frames) and tests/test_oracle_units.py
  * (closed forms and invariants).
@@ -1000,6 +1007,154 @@ static void clear(const PFFframe* f, const PFFcmd* c)
 /* Renders a recorded frame into frame->color / frame->depth.  Returns 0 on
  * success; *fragments receives the number of covered samples (before the
  * depth test) summed over rasterised triangles. */
+/* ------------------------------------------------------------------------ */
+/* pfDrawPixels (context.c:1987-2080)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* (PFsizei)(float): gcc converts through cvttss2si to a 64-bit register and
+ * keeps the low 32 bits; NaN and values outside int64 give 0. */
+static uint32_t f2u32_x86(float f)
+{
+    if (!(f >= -9223372036854775808.0f && f < 9223372036854775808.0f)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)f;
+}
+
+static uint32_t rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a)
+{
+    return (r & 255u) | ((g & 255u) << 8) | ((b & 255u) << 16) | ((a & 255u) << 24);
+}
+
+/* The scalar texel getters (GC_pixelGetters, pixel.h:765-810), texel i of
+ * an image: float and half components become (PFubyte)(v * 255), packed
+ * fields (float)(field) * (255.0f / max) truncated. */
+static uint32_t scalar_texel(const uint8_t* p, uint32_t source, size_t i)
+{
+    const uint32_t f = source & 0xFFu, t = (source >> 8) & 0xFFu;
+    uint16_t hv[4];
+    float fv[4];
+    if (t == SRC_UBYTE) {
+        switch (f) {
+        case SRC_RED:   return rgba(p[i], 0, 0, 255);                              /* pixel.h:462-466 */
+        case SRC_GREEN: return rgba(0, p[i], 0, 255);                              /* pixel.h:469-473 */
+        case SRC_BLUE:  return rgba(0, 0, p[i], 255);                              /* pixel.h:476-480 */
+        case SRC_ALPHA: return rgba(255, 255, 255, p[i]);                          /* pixel.h:483-487 */
+        case SRC_LUMINANCE: return rgba(p[i], p[i], p[i], 255);                    /* pixel.h:410-414 */
+        case SRC_LUMINANCE_ALPHA: return rgba(p[2 * i], p[2 * i], p[2 * i], p[2 * i + 1]);   /* pixel.h:434-438 */
+        case SRC_RGB: return rgba(p[3 * i], p[3 * i + 1], p[3 * i + 2], 255);     /* pixel.h:575-580 */
+        case SRC_BGR: return rgba(p[3 * i + 2], p[3 * i + 1], p[3 * i], 255);     /* pixel.h:582-587 */
+        case SRC_RGBA: return rgba(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);       /* pixel.h:696-700 */
+        case SRC_BGRA: return rgba(p[4 * i + 2], p[4 * i + 1], p[4 * i], p[4 * i + 3]);       /* pixel.h:702-707 */
+        default: return 0;
+        }
+    }
+    if (t == SRC_HALF || t == SRC_FLOAT) {
+        const size_t comps = f <= SRC_LUMINANCE ? 1 : f == SRC_LUMINANCE_ALPHA ? 2 : (f == SRC_RGB || f == SRC_BGR) ? 3 : 4;
+        for (size_t k = 0; k < comps; k++) {
+            if (t == SRC_HALF) { memcpy(&hv[k], p + 2 * (comps * i + k), 2); fv[k] = half_bits_to_float(hv[k]); }
+            else memcpy(&fv[k], p + 4 * (comps * i + k), 4);
+        }
+        uint32_t c[4];
+        for (size_t k = 0; k < comps; k++) c[k] = f2u8(fv[k] * 255.0f);          /* 255*v = v*255 */
+        switch (f) {
+        case SRC_RED:   return rgba(c[0], 0, 0, 255);                              /* pixel.h:490-532 */
+        case SRC_GREEN: return rgba(0, c[0], 0, 255);
+        case SRC_BLUE:  return rgba(0, 0, c[0], 255);
+        case SRC_ALPHA: return rgba(255, 255, 255, c[0]);
+        case SRC_LUMINANCE: return rgba(c[0], c[0], c[0], 255);                    /* pixel.h:417-431 */
+        case SRC_LUMINANCE_ALPHA: return rgba(c[0], c[0], c[0], c[1]);             /* pixel.h:441-457 */
+        case SRC_RGB: return rgba(c[0], c[1], c[2], 255);                          /* pixel.h:589-639 */
+        case SRC_BGR: return rgba(c[2], c[1], c[0], 255);
+        case SRC_RGBA: return rgba(c[0], c[1], c[2], c[3]);                        /* pixel.h:709-759 */
+        case SRC_BGRA: return rgba(c[2], c[1], c[0], c[3]);
+        default: return 0;
+        }
+    }
+    uint16_t x;
+    memcpy(&x, p + 2 * i, 2);
+    if (t == SRC_565) {                                                            /* pixel.h:549-573 */
+        uint32_t hi = f2u8((float)((x & 0xF800u) >> 11) * (255.0f / 31));
+        uint32_t g = f2u8((float)((x & 0x07E0u) >> 5) * (255.0f / 63));
+        uint32_t lo = f2u8((float)(x & 0x001Fu) * (255.0f / 31));
+        return f == SRC_RGB ? rgba(hi, g, lo, 255) : rgba(lo, g, hi, 255);
+    }
+    if (t == SRC_5551) {                                                           /* pixel.h:644-668 */
+        uint32_t c11 = f2u8((float)((x & 0xF800u) >> 11) * (255.0f / 31));
+        uint32_t c6 = f2u8((float)((x & 0x07C0u) >> 6) * (255.0f / 31));
+        uint32_t c1 = f2u8((float)((x & 0x003Eu) >> 1) * (255.0f / 31));
+        uint32_t a = (uint32_t)((x & 0x1u) * 255u);
+        return f == SRC_RGBA ? rgba(c11, c6, c1, a) : rgba(c1, c6, c11, a);
+    }
+    if (t == SRC_4444) {                                                           /* pixel.h:670-694 */
+        uint32_t c12 = f2u8((float)((x & 0xF000u) >> 12) * (255.0f / 15));
+        uint32_t c8 = f2u8((float)((x & 0x0F00u) >> 8) * (255.0f / 15));
+        uint32_t c4 = f2u8((float)((x & 0x00F0u) >> 4) * (255.0f / 15));
+        uint32_t a = f2u8((float)(x & 0x000Fu) * (255.0f / 15));
+        return f == SRC_RGBA ? rgba(c12, c8, c4, a) : rgba(c4, c8, c12, a);
+    }
+    return 0;
+}
+
+/* The scalar depth tests (depth.h:29-63): NOTEQUAL is a real !=. */
+static int depth_scalar(uint32_t func, float src, float dst)
+{
+    switch (func) {
+    case 0: return src == dst;
+    case 1: return src != dst;
+    case 2: return src < dst;
+    case 3: return src <= dst;
+    case 4: return src > dst;
+    default: return src >= dst;
+    }
+}
+
+/* The scalar blend functions (blend.h:29-118), source over destination. */
+static uint32_t blend_scalar(uint32_t mode, uint32_t s, uint32_t d)
+{
+    uint32_t o[4];
+    for (int i = 0; i < 4; i++) {
+        int32_t a = (int32_t)((s >> (8 * i)) & 255u), b = (int32_t)((d >> (8 * i)) & 255u), v;
+        switch (mode) {
+        case 0: v = (a + b) >> 1; break;                                           /* average */
+        case 1: {                                                                  /* alpha */
+            int32_t al = (int32_t)(s >> 24) + 1, inv = 256 - al;
+            v = i < 3 ? (al * a + inv * b) >> 8 : (al * 255 + inv * b) >> 8;
+            break;
+        }
+        case 2: v = b + a > 255 ? 255 : b + a; break;                              /* add */
+        case 3: v = b - a < 0 ? 0 : b - a; break;                                  /* sub */
+        case 4: v = (a * b) / 255; break;                                          /* mul */
+        case 5: { int32_t w = ((b * (255 - a)) >> 8) + a; v = w > 255 ? 255 : w; break; }   /* screen */
+        case 6: v = a > b ? a : b; break;                                          /* lighten */
+        default: v = a < b ? a : b; break;                                         /* darken */
+        }
+        o[i] = (uint32_t)v & 255u;
+    }
+    return rgba(o[0], o[1], o[2], o[3]);
+}
+
+/* One pfDrawPixels call: every destination pixel of the inclusive
+ * rectangle, top to bottom, left to right (context.c:2048-2080). */
+static void draw_pixels(const PFFframe* f, const PFFpixels* o)
+{
+    const uint8_t* img = (const uint8_t*)o->pixels;
+    const uint32_t n = o->width * o->height;
+    for (int32_t y = o->yMin; y <= o->yMax; y++) {
+        const float v = (float)(y - o->yScreen) * o->invYLen;
+        const uint32_t ySrc = f2u32_x86(v * (float)(o->height - 1u)) * o->width;
+        for (int32_t x = o->xMin; x <= o->xMax; x++) {
+            if (x < 0 || y < 0 || (uint32_t)x >= f->width || (uint32_t)y >= f->height) continue;   /* D5 */
+            const size_t dst = (size_t)y * f->width + (size_t)x;
+            if ((o->flags & PFF_DEPTH_TEST) && !depth_scalar(o->depthFunc, o->zPos, f->depth[dst])) continue;
+            const float u = (float)(x - o->xScreen) * o->invXLen;
+            uint32_t src = ySrc + f2u32_x86(u * (float)(o->width - 1u));
+            if (src >= n) src = n - 1u;                                                          /* D8 */
+            f->depth[dst] = o->zPos;
+            const uint32_t c = scalar_texel(img, o->source, src);
+            f->color[dst] = (o->flags & PFF_BLEND) ? blend_scalar(o->blendMode, c, f->color[dst]) : c;
+        }
+    }
+}
+
 static int render_frame(RCtx* rc);
 
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
@@ -1019,6 +1174,11 @@ static int render_frame(RCtx* rcp)
     for (uint32_t c = 0; c < f->numCmds; c++) {
         const PFFcmd* cmd = &f->cmds[c];
         if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
+        if (cmd->kind == PFF_CMD_PIXELS) {
+            if (cmd->first >= f->numPixelOps) return -1;
+            draw_pixels(f, &f->pixelOps[cmd->first]);
+            continue;
+        }
         if (cmd->kind != PFF_CMD_DRAW) return -1;
         for (uint32_t t = cmd->first; t < cmd->last; t++) {
             uint32_t info = f->tris.info[t];

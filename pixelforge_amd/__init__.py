@@ -46,6 +46,7 @@ class SceneParams(C.Structure):
         ("arrays", C.c_int32), ("indexType", C.c_int32), ("posType", C.c_int32),
         ("colType", C.c_int32), ("colSize", C.c_int32),
         ("texFormat", C.c_int32), ("texType", C.c_int32), ("texSwizzle", C.c_int32),
+        ("pixelOps", C.c_int32), ("pixFormat", C.c_int32), ("pixType", C.c_int32),
     ]
 
 

@@ -76,7 +76,7 @@ struct DSeg {               // the part of one frame a pass renders
     uint32_t clear;               // 1: a clear precedes the draws
     uint32_t clearColor;
     float    clearDepth;
-    uint32_t pad;
+    uint32_t pixOp;               // 1 + index into the job's pfDrawPixels calls, 0 for none
 };
 
 // Raster triangle: what the tile kernel needs for coverage, depth and
@@ -2323,6 +2323,160 @@ __global__ __launch_bounds__(256) void k_decode_texels(const uint8_t* __restrict
     if (i < n) out[i] = decode_source_texel(src, i, source);
 }
 
+// ---------------------------------------------------------------------------
+// pfDrawPixels (context.c:1987-2080): the image is decoded to RGBA8 with the
+// scalar getters (pixel.h:405-760) when the job is created; after the raster
+// kernel of the pass that holds the call, one thread per destination pixel
+// runs the scalar depth test (depth.h:29-63, NOTEQUAL a real !=), writes
+// the depth and the scalar blend (blend.h:29-118) of the source texel.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+    return (r & 255u) | ((g & 255u) << 8) | ((b & 255u) << 16) | ((a & 255u) << 24);
+}
+
+__device__ uint32_t decode_scalar_texel(const uint8_t* __restrict__ p, size_t i, uint32_t source) {
+    const uint32_t f = source & 0xFFu, t = (source >> 8) & 0xFFu;
+    if (t == 0u) {                                                    // UNSIGNED_BYTE
+        switch (f) {
+        case 0u: return pack4(p[i], 0, 0, 255);                      // pixel.h:462-487
+        case 1u: return pack4(0, p[i], 0, 255);
+        case 2u: return pack4(0, 0, p[i], 255);
+        case 3u: return pack4(255, 255, 255, p[i]);
+        case 4u: return pack4(p[i], p[i], p[i], 255);                 // :410-414
+        case 5u: return pack4(p[2 * i], p[2 * i], p[2 * i], p[2 * i + 1]);          // :434-438
+        case 6u: return pack4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 255);           // :575-587
+        case 8u: return pack4(p[3 * i + 2], p[3 * i + 1], p[3 * i], 255);
+        case 7u: return pack4(p[4 * i], p[4 * i + 1], p[4 * i + 2], p[4 * i + 3]);  // :696-707
+        case 9u: return pack4(p[4 * i + 2], p[4 * i + 1], p[4 * i], p[4 * i + 3]);
+        default: return 0u;
+        }
+    }
+    if (t == 9u || t == 10u) {                                        // HALF_FLOAT, FLOAT: (PFubyte)(v * 255)
+        const uint32_t comps = f <= 4u ? 1u : f == 5u ? 2u : (f == 6u || f == 8u) ? 3u : 4u;
+        uint32_t c[4] = { 0u, 0u, 0u, 0u };
+        for (uint32_t k = 0; k < comps; k++) {
+            const size_t o = (t == 9u ? 2 : 4) * ((size_t)comps * i + k);
+            const float v = t == 9u ? half_to_float_ref((uint32_t)p[o] | ((uint32_t)p[o + 1] << 8)) : __uint_as_float(src_word(p, o));
+            c[k] = f2u8(v * 255.0f);
+        }
+        switch (f) {
+        case 0u: return pack4(c[0], 0, 0, 255);                       // :490-532
+        case 1u: return pack4(0, c[0], 0, 255);
+        case 2u: return pack4(0, 0, c[0], 255);
+        case 3u: return pack4(255, 255, 255, c[0]);
+        case 4u: return pack4(c[0], c[0], c[0], 255);                 // :417-431
+        case 5u: return pack4(c[0], c[0], c[0], c[1]);                // :441-457
+        case 6u: return pack4(c[0], c[1], c[2], 255);                 // :589-639
+        case 8u: return pack4(c[2], c[1], c[0], 255);
+        case 7u: return pack4(c[0], c[1], c[2], c[3]);                // :709-759
+        case 9u: return pack4(c[2], c[1], c[0], c[3]);
+        default: return 0u;
+        }
+    }
+    const uint32_t x = (uint32_t)p[2 * i] | ((uint32_t)p[2 * i + 1] << 8);
+    if (t == 2u) {                                                    // 5_6_5, :549-573
+        const uint32_t hi = f2u8((float)((x & 0xF800u) >> 11) * (255.0f / 31));
+        const uint32_t g = f2u8((float)((x & 0x07E0u) >> 5) * (255.0f / 63));
+        const uint32_t lo = f2u8((float)(x & 0x001Fu) * (255.0f / 31));
+        return f == 6u ? pack4(hi, g, lo, 255) : pack4(lo, g, hi, 255);
+    }
+    if (t == 3u) {                                                    // 5_5_5_1, :644-668
+        const uint32_t c11 = f2u8((float)((x & 0xF800u) >> 11) * (255.0f / 31));
+        const uint32_t c6 = f2u8((float)((x & 0x07C0u) >> 6) * (255.0f / 31));
+        const uint32_t c1 = f2u8((float)((x & 0x003Eu) >> 1) * (255.0f / 31));
+        const uint32_t a = (x & 1u) * 255u;
+        return f == 7u ? pack4(c11, c6, c1, a) : pack4(c1, c6, c11, a);
+    }
+    if (t == 4u) {                                                    // 4_4_4_4, :670-694
+        const uint32_t c12 = f2u8((float)((x & 0xF000u) >> 12) * (255.0f / 15));
+        const uint32_t c8 = f2u8((float)((x & 0x0F00u) >> 8) * (255.0f / 15));
+        const uint32_t c4 = f2u8((float)((x & 0x00F0u) >> 4) * (255.0f / 15));
+        const uint32_t a = f2u8((float)(x & 0x000Fu) * (255.0f / 15));
+        return f == 7u ? pack4(c12, c8, c4, a) : pack4(c4, c8, c12, a);
+    }
+    return 0u;
+}
+
+__global__ __launch_bounds__(256) void k_decode_image(const uint8_t* __restrict__ src, size_t n, uint32_t source,
+                                                      uint32_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = decode_scalar_texel(src, i, source);
+}
+
+struct DPixOp {
+    const uint32_t* texels;       // w*h RGBA8
+    uint32_t* color;              // render target
+    float*    depth;
+    uint32_t  W, H;
+    uint32_t  w, h;
+    int32_t   xScreen, yScreen, xMin, yMin, xMax, yMax;
+    float     zPos, invXLen, invYLen;
+    uint32_t  flags, depthFunc, blendMode;
+};
+
+// (PFsizei)(float) of gcc on x86-64: cvttss2si into a 64-bit register, low
+// 32 bits kept; NaN and values outside int64 give 0.
+__device__ __forceinline__ uint32_t f2u32_x86(float f) {
+    if (!(f >= -9223372036854775808.0f && f < 9223372036854775808.0f)) return 0u;
+    return (uint32_t)(uint64_t)(int64_t)f;
+}
+
+__device__ __forceinline__ bool depth_scalar(uint32_t func, float s, float d) {
+    switch (func) {
+    case 0u: return s == d;
+    case 1u: return s != d;
+    case 2u: return s < d;
+    case 3u: return s <= d;
+    case 4u: return s > d;
+    default: return s >= d;
+    }
+}
+
+__device__ __forceinline__ uint32_t blend_scalar(uint32_t mode, uint32_t s, uint32_t d) {
+    uint32_t o = 0u;
+    const int32_t al = (int32_t)(s >> 24) + 1, inv = 256 - al;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int32_t a = (int32_t)((s >> (8 * i)) & 255u), b = (int32_t)((d >> (8 * i)) & 255u);
+        int32_t v;
+        switch (mode) {
+        case 0u: v = (a + b) >> 1; break;
+        case 1u: v = i < 3 ? (al * a + inv * b) >> 8 : (al * 255 + inv * b) >> 8; break;
+        case 2u: v = min(a + b, 255); break;
+        case 3u: v = max(b - a, 0); break;
+        case 4u: v = (a * b) / 255; break;
+        case 5u: v = min(((b * (255 - a)) >> 8) + a, 255); break;
+        case 6u: v = max(a, b); break;
+        default: v = min(a, b); break;
+        }
+        o |= ((uint32_t)v & 255u) << (8 * i);
+    }
+    return o;
+}
+
+// grid.y = the pass's calls; each thread strides over its call's rectangle.
+__global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ ops) {
+    const DPixOp o = ops[blockIdx.y];
+    if (o.xMax < o.xMin || o.yMax < o.yMin) return;
+    const uint32_t rw = (uint32_t)(o.xMax - o.xMin) + 1u, rh = (uint32_t)(o.yMax - o.yMin) + 1u;
+    const uint64_t area = (uint64_t)rw * rh;
+    const uint32_t n = o.w * o.h;
+    for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < area; k += (uint64_t)gridDim.x * 256) {
+        const int32_t x = o.xMin + (int32_t)(k % rw), y = o.yMin + (int32_t)(k / rw);
+        if (x < 0 || y < 0 || (uint32_t)x >= o.W || (uint32_t)y >= o.H) continue;
+        const size_t dst = (size_t)y * o.W + (size_t)x;
+        if ((o.flags & PFF_DEPTH_TEST) && !depth_scalar(o.depthFunc, o.zPos, o.depth[dst])) continue;
+        const float v = (float)(y - o.yScreen) * o.invYLen;
+        const float u = (float)(x - o.xScreen) * o.invXLen;
+        uint32_t src = f2u32_x86(v * (float)(o.h - 1u)) * o.w + f2u32_x86(u * (float)(o.w - 1u));
+        if (src >= n) src = n - 1u;
+        o.depth[dst] = o.zPos;
+        const uint32_t c = o.texels[src];
+        o.color[dst] = (o.flags & PFF_BLEND) ? blend_scalar(o.blendMode, c, o.color[dst]) : c;
+    }
+}
+
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -2380,6 +2534,9 @@ struct PassInfo {
     uint32_t numPairs = 0;
     bool     sized = false;
     DevBuf   dSegs, dFrameSeg, dBlockSeg, dTileDesc;
+    std::vector<DPixOp> pix;   // pfDrawPixels calls run after this pass's raster
+    uint32_t pixGrid = 0;
+    DevBuf   dPix;
 };
 
 }  // namespace
@@ -2398,6 +2555,7 @@ struct PFDjob {
     std::vector<DevBuf> targets;          // colour + depth per frame
     DevBuf pos, col, uv, nrm, info, states, texTable;
     std::vector<DevBuf> texData;
+    std::vector<DevBuf> pixData;   // decoded pfDrawPixels images
     uint32_t numIn = 0;
     bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
     std::vector<PassInfo> passes;
@@ -2601,6 +2759,9 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     }
+    if (!P.pix.empty())
+        hipLaunchKernelGGL(k_draw_pixels, dim3(P.pixGrid, (uint32_t)P.pix.size()), dim3(256), 0, J->stream,
+                           (const DPixOp*)P.dPix.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
     P.sized = true;
@@ -2847,6 +3008,55 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 cur.clearColor = cmd.clearColor;
                 cur.clearDepth = cmd.clearDepth;
                 open = true;
+            } else if (cmd.kind == PFF_CMD_PIXELS) {
+                // A pass of its own: no triangles, the call runs after the
+                // pass's raster kernel, so it lands between the draws
+                // recorded before and after it.
+                if (!fr.pixelOps || cmd.first >= fr.numPixelOps) {
+                    snprintf(g_err, sizeof g_err, "bad pixel command");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                const PFFpixels& op = fr.pixelOps[cmd.first];
+                const uint32_t bpp = pff_source_texel_bytes(op.source);
+                if (bpp == 0u || op.width == 0u || op.height == 0u || !op.pixels) {
+                    snprintf(g_err, sizeof g_err, "bad pixel image");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                const size_t texels = (size_t)op.width * op.height;
+                J->pixData.emplace_back();
+                DevBuf raw;
+                if ((rc = ensure(J->pixData.back(), texels * 4)) || (rc = ensure(raw, texels * bpp + 4))) {
+                    release(raw);
+                    pfdJobDestroy(J);
+                    return rc;
+                }
+                e = hipMemcpy(raw.p, op.pixels, texels * bpp, hipMemcpyHostToDevice);
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL(k_decode_image, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, J->stream,
+                                       (const uint8_t*)raw.p, texels, op.source, (uint32_t*)J->pixData.back().p);
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+                release(raw);
+                if (e != hipSuccess) { set_error("pixel image decode", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+                DPixOp d;
+                memset(&d, 0, sizeof d);
+                d.texels = (const uint32_t*)J->pixData.back().p;
+                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.w = op.width; d.h = op.height;
+                d.xScreen = op.xScreen; d.yScreen = op.yScreen;
+                d.xMin = op.xMin; d.yMin = op.yMin; d.xMax = op.xMax; d.yMax = op.yMax;
+                d.zPos = op.zPos; d.invXLen = op.invXLen; d.invYLen = op.invYLen;
+                d.flags = op.flags; d.depthFunc = op.depthFunc; d.blendMode = op.blendMode;
+                flush_seg();
+                cur.triFirst = cur.triLast = triBase;
+                open = true;
+                if (J->passes.size() <= segNo) J->passes.emplace_back();
+                J->passes[segNo].pix.push_back(d);
+                cur.pixOp = (uint32_t)J->passes[segNo].pix.size();
+                flush_seg();
             } else if (cmd.kind == PFF_CMD_DRAW) {
                 if (cmd.first > cmd.last || cmd.last > fr.tris.count) {
                     snprintf(g_err, sizeof g_err, "bad draw range");
@@ -2898,6 +3108,17 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             bs[b] = (uint32_t)sgi;
         }
         if ((rc = up(P.dBlockSeg, bs.data(), bs.size() * 4))) { pfdJobDestroy(J); return rc; }
+        if (!P.pix.empty()) {
+            uint64_t maxArea = 1;
+            for (const DPixOp& d : P.pix) {
+                if (d.xMax < d.xMin || d.yMax < d.yMin) continue;
+                const uint64_t a = (uint64_t)(uint32_t)(d.xMax - d.xMin + 1) * (uint64_t)(uint32_t)(d.yMax - d.yMin + 1);
+                if (a > maxArea) maxArea = a;
+            }
+            const uint64_t g = (maxArea + 255) / 256;
+            P.pixGrid = (uint32_t)(g < 4096 ? g : 4096);
+            if ((rc = up(P.dPix, P.pix.data(), P.pix.size() * sizeof(DPixOp)))) { pfdJobDestroy(J); return rc; }
+        }
         if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
             || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
             pfdJobDestroy(J);
@@ -3108,7 +3329,8 @@ void pfdJobDestroy(PFDjob* J) {
     for (DevBuf* b : bufs) release(*b);
     for (auto& b : J->targets) release(b);
     for (auto& b : J->texData) release(b);
-    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); }
+    for (auto& b : J->pixData) release(b);
+    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); }
     if (J->side) (void)hipStreamSynchronize(J->side);
     if (J->fork) (void)hipEventDestroy(J->fork);
     if (J->join) (void)hipEventDestroy(J->join);
@@ -3166,8 +3388,10 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     *out = nullptr;
     // One pass: at most one clear, before every draw.
     for (uint32_t c = 0; c < frame->numCmds; c++)
-        if (frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) {
-            snprintf(g_err, sizeof g_err, "differentiable frame: a clear after the first command");
+        if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS) {
+            snprintf(g_err, sizeof g_err, "differentiable frame: %s",
+                     frame->cmds[c].kind == PFF_CMD_PIXELS ? "pfDrawPixels is not differentiable here"
+                                                           : "a clear after the first command");
             return PFD_E_INVALID;
         }
     for (uint32_t i = 0; i < frame->numStates; i++) {

@@ -104,6 +104,7 @@ typedef struct {                       /* the frame being recorded */
     PFFstate*  states;   size_t nstates, capstates;
     PFFtexture* texs;    const Tex** texKeys; size_t ntex, captex;
     PFFcmd*    cmds;     size_t ncmd, capcmd;
+    PFFpixels* pix;      size_t npix, cappix;   /* pfDrawPixels calls; images owned */
     int        anyTex, anyLight, anyVid;
 } Rec;
 
@@ -301,6 +302,8 @@ static int grow(void** p, size_t* cap, size_t need, size_t elem)
 
 static void rec_reset(Rec* r)
 {
+    for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
+    r->npix = 0;
     r->ntri = r->nstates = r->ntex = r->ncmd = 0;
     r->anyTex = r->anyLight = r->anyVid = 0;
     r->targetColor = NULL;
@@ -309,6 +312,8 @@ static void rec_reset(Rec* r)
 
 static void rec_free(Rec* r)
 {
+    for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
+    free(r->pix);
     free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info); free(r->vid);
     free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
     memset(r, 0, sizeof *r);
@@ -343,6 +348,8 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->numTextures = (uint32_t)r->ntex;
     f->cmds = r->cmds;
     f->numCmds = (uint32_t)r->ncmd;
+    f->pixelOps = r->pix;
+    f->numPixelOps = (uint32_t)r->npix;
 }
 
 /* Executes the recorded frame, if any, and starts an empty one. */
@@ -1689,8 +1696,77 @@ void pfRects(PFshort x1, PFshort y1, PFshort x2, PFshort y2) { (void)x1; (void)y
 void pfRectsv(const PFshort* v1, const PFshort* v2) { (void)v1; (void)v2; not_in_path(); }
 void pfRectf(PFfloat x1, PFfloat y1, PFfloat x2, PFfloat y2) { (void)x1; (void)y1; (void)x2; (void)y2; not_in_path(); }
 void pfRectfv(const PFfloat* v1, const PFfloat* v2) { (void)v1; (void)v2; not_in_path(); }
+
+/* (PFint)(float) as gcc emits it on x86-64 (cvttss2si): NaN and values out
+ * of range give INT32_MIN. */
+static int32_t f2i_x86(float f)
+{
+    if (!(f > -2147483904.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+
+/* pfDrawPixels (context.c:1987-2080).  The call computes, from the current
+ * context, the raster position on screen (the MVP-transformed raster
+ * position with no perspective division), the destination rectangle
+ * clipped to the viewport, the source steps and the depth / blend state,
+ * and records them with a copy of the image as a PFF_CMD_PIXELS command:
+ * the pixels are written on the GPU in order with the frame's triangles,
+ * through the reference's scalar getter, depth test and blend function. */
 void pfDrawPixels(PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type, const void* pixels)
-{ (void)width; (void)height; (void)format; (void)type; (void)pixels; not_in_path(); }
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (width == 0 || height == 0) { c->err = PF_INVALID_VALUE; return; }
+    const uint32_t source = PFF_SOURCE(format, type);
+    const uint32_t bpp = pff_source_texel_bytes(source);
+    if (bpp == 0u) { c->err = PF_INVALID_ENUM; return; }              /* context.c:1996-2006 */
+    if (!pixels) { c->err = PF_INVALID_VALUE; return; }
+    if (!ensure_target(c)) { c->err = PF_INVALID_OPERATION; return; }
+    update_matrices(c, 0);
+    const float* m = c->matMVP;                                      /* pfmVec4Transform, pfm.h:1996-2006 */
+    const float* v = c->rasterPos;
+    float rp[3];
+    for (int i = 0; i < 3; i++) rp[i] = m[i] * v[0] + m[4 + i] * v[1] + m[8 + i] * v[2] + m[12 + i] * v[3];
+    PFFpixels op;
+    memset(&op, 0, sizeof op);
+    op.width = width;
+    op.height = height;
+    op.source = source;
+    op.xScreen = f2i_x86((float)c->vpPos[0] + (rp[0] + 1.0f) * 0.5f * (float)c->vpDim[0]);
+    op.yScreen = f2i_x86((float)c->vpPos[1] + (1.0f - rp[1]) * 0.5f * (float)c->vpDim[1]);
+    op.zPos = rp[2];
+    /* PF_CLAMP of ints (context.c:2024-2025) and of floats (2026-2027). */
+    op.xMin = op.xScreen < c->vpMin[0] ? c->vpMin[0] : (op.xScreen > c->vpMax[0] ? c->vpMax[0] : op.xScreen);
+    op.yMin = op.yScreen < c->vpMin[1] ? c->vpMin[1] : (op.yScreen > c->vpMax[1] ? c->vpMax[1] : op.yScreen);
+    const float xe = (float)op.xScreen + (float)width * c->pixelZoom[0];
+    const float ye = (float)op.yScreen + (float)height * c->pixelZoom[1];
+    op.xMax = f2i_x86(xe < (float)c->vpMin[0] ? (float)c->vpMin[0] : (xe > (float)c->vpMax[0] ? (float)c->vpMax[0] : xe));
+    op.yMax = f2i_x86(ye < (float)c->vpMin[1] ? (float)c->vpMin[1] : (ye > (float)c->vpMax[1] ? (float)c->vpMax[1] : ye));
+    op.invXLen = 1.0f / ((float)width * c->pixelZoom[0]);
+    op.invYLen = 1.0f / ((float)height * c->pixelZoom[1]);
+    op.flags = ((c->state & PF_DEPTH_TEST) ? PFF_DEPTH_TEST : 0u) | ((c->state & PF_BLEND) ? PFF_BLEND : 0u);
+    op.depthFunc = (uint32_t)c->depthFunc;
+    op.blendMode = (uint32_t)c->blendMode;
+    const size_t bytes = (size_t)width * height * bpp;
+    void* copy = malloc(bytes + 4);
+    if (!copy) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+    memcpy(copy, pixels, bytes);
+    memset((uint8_t*)copy + bytes, 0, 4);
+    op.pixels = copy;
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->pix, &r->cappix, r->npix + 1, sizeof(PFFpixels))) {
+        free(copy);
+        c->err = PF_ERROR_OUT_OF_MEMORY;
+        return;
+    }
+    r->pix[r->npix] = op;
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_PIXELS;
+    cmd.first = (uint32_t)r->npix;
+    r->npix++;
+    push_cmd(c, &cmd);
+}
 void pfFogi(PFfogparam pname, PFint param) { (void)pname; (void)param; not_in_path(); }
 void pfFogf(PFfogparam pname, PFfloat param) { (void)pname; (void)param; not_in_path(); }
 void pfFogiv(PFfogparam pname, PFint* param) { (void)pname; (void)param; not_in_path(); }

@@ -79,6 +79,10 @@ typedef struct {
     int32_t  texType;        /* texture = 3: PFdatatype of the texture               */
     int32_t  texSwizzle;     /* texture = 3, BGRA / UNSIGNED_BYTE: hand the same
                               * texels over as RGBA / UNSIGNED_BYTE instead      */
+    int32_t  pixelOps;       /* >0: that many pfDrawPixels calls between the first
+                              * and second half of the triangles (random kinds) */
+    int32_t  pixFormat;      /* pfDrawPixels image PFpixelformat                  */
+    int32_t  pixType;        /* pfDrawPixels image PFdatatype                     */
 } PFSparams;
 
 typedef struct {
@@ -128,6 +132,9 @@ typedef struct {
     void (*ColorPointer)(PFint, PFenum, PFsizei, const void*);
     void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
+    void (*RasterPos3f)(PFfloat, PFfloat, PFfloat);
+    void (*PixelZoom)(PFfloat, PFfloat);
+    void (*DrawPixels)(PFsizei, PFsizei, PFpixelformat, PFdatatype, const void*);
     PFrenderlist (*GenList)(void);
     void (*DeleteList)(PFrenderlist*);
     void (*NewList)(PFrenderlist);
@@ -156,6 +163,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(TextureParameter); BIND(GetTexturePixels); BIND(GetError); BIND(Color4f);
     BIND(VertexPointer); BIND(NormalPointer); BIND(TexCoordPointer); BIND(ColorPointer);
     BIND(DrawElements); BIND(DrawArrays);
+    BIND(RasterPos3f); BIND(PixelZoom); BIND(DrawPixels);
     BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
@@ -232,6 +240,41 @@ static float rand_float(Rng* r)
     return (float)(v >> 40) * (1.0f / 16777216.0f);
 }
 
+/* Random texel bytes of a (format, type) pair for n texels, 64 zero bytes
+ * after the last. */
+static uint8_t* random_texels(Rng* r, int f, int t, size_t n)
+{
+    size_t bpp = texel_bytes(f, t);
+    uint8_t* px = (uint8_t*)calloc(n * bpp + 64, 1);
+    if (!px) return NULL;
+    size_t comps = (t == PF_HALF_FLOAT) ? n * bpp / 2 : (t == PF_FLOAT) ? n * bpp / 4 : 0;
+    if (t == PF_HALF_FLOAT) for (size_t i = 0; i < comps; i++) ((uint16_t*)px)[i] = rand_half(r);
+    else if (t == PF_FLOAT) for (size_t i = 0; i < comps; i++) ((float*)px)[i] = rand_float(r);
+    else for (size_t i = 0; i < n * bpp; i++) px[i] = (uint8_t)(rnext(r) >> 56);
+    return px;
+}
+
+/* pfDrawPixels calls: images of random size in the scene's pixel format at
+ * random raster positions; the second call zooms by (2, 1.5) and blends,
+ * the third zooms by (0.75, 0.5) without the depth test. */
+static void draw_pixel_ops(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    for (int k = 0; k < p->pixelOps; k++) {
+        uint32_t w = 5 + (uint32_t)(rnext(r) % 24), h = 4 + (uint32_t)(rnext(r) % 18);
+        uint8_t* px = random_texels(r, p->pixFormat, p->pixType, (size_t)w * h);
+        a->RasterPos3f(rrange(r, -0.9f, 0.5f), rrange(r, -0.5f, 0.9f), rrange(r, -0.6f, 0.6f));
+        int kind = k % 3;
+        a->PixelZoom(kind == 1 ? 2.0f : kind == 2 ? 0.75f : 1.0f, kind == 1 ? 1.5f : kind == 2 ? 0.5f : 1.0f);
+        if (kind == 1) { a->Enable(PF_BLEND); a->BlendFunc((PFblendmode)((p->seed + k) % 8)); }
+        if (kind == 2) a->Disable(PF_DEPTH_TEST);
+        a->DrawPixels(w, h, (PFpixelformat)p->pixFormat, (PFdatatype)p->pixType, px);
+        if (kind == 1) { a->Disable(PF_BLEND); a->BlendFunc((PFblendmode)p->blendMode); }
+        if (kind == 2 && p->depthTest) a->Enable(PF_DEPTH_TEST);
+        free(px);
+    }
+    a->PixelZoom(1.0f, 1.0f);
+}
+
 static SceneTex make_texture(const PFSapi* a, Rng* r, const PFSparams* p)
 {
     SceneTex st = { NULL, NULL };
@@ -242,12 +285,7 @@ static SceneTex make_texture(const PFSapi* a, Rng* r, const PFSparams* p)
         /* Any (format, type) pair: random texel bytes, 64 zero bytes after
          * the last texel (the reference's getters read up to 3 past it). */
         int f = p->texFormat, t = p->texType;
-        size_t bpp = texel_bytes(f, t);
-        uint8_t* px = (uint8_t*)calloc(n * bpp + 64, 1);
-        size_t comps = (t == PF_HALF_FLOAT) ? n * bpp / 2 : (t == PF_FLOAT) ? n * bpp / 4 : 0;
-        if (t == PF_HALF_FLOAT) for (size_t i = 0; i < comps; i++) ((uint16_t*)px)[i] = rand_half(r);
-        else if (t == PF_FLOAT) for (size_t i = 0; i < comps; i++) ((float*)px)[i] = rand_float(r);
-        else for (size_t i = 0; i < n * bpp; i++) px[i] = (uint8_t)(rnext(r) >> 56);
+        uint8_t* px = random_texels(r, f, t, n);
         if (p->texSwizzle && f == PF_BGRA && t == PF_UNSIGNED_BYTE) {
             for (size_t i = 0; i < n; i++) { uint8_t b = px[4 * i]; px[4 * i] = px[4 * i + 2]; px[4 * i + 2] = b; }
             f = PF_RGBA;
@@ -690,6 +728,16 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
     if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
     if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
+    if (p->pixelOps > 0 && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
+        PFSparams half = *p;
+        half.ntris = p->ntris / 2;
+        if (p->kind == PFS_RANDOM_3D) setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
+        if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
+        draw_pixel_ops(a, r, p);
+        half.ntris = p->ntris - half.ntris;
+        if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
+        return;
+    }
     switch (p->kind) {
     case PFS_RANDOM_NDC: scene_random_ndc(a, r, p); break;
     case PFS_HEIGHTFIELD: scene_heightfield(a, r, p); break;

@@ -120,6 +120,19 @@ FORMAT_SCENES = [
     for k, (f, t) in enumerate(_PAIRS)
 ]
 
+# pfDrawPixels: three calls (plain; zoomed and blended; zoomed without the
+# depth test) between two halves of the triangles, images in every (format,
+# type) pair with a pixel getter; one 3D scene.
+_ALL_PAIRS = _PAIRS + [("rgba", "ubyte"), ("rgba", "float")]
+PIXEL_SCENES = [
+    (f"pixels_{f}_{t}", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=300 + k, pixelOps=3,
+                             pixFormat=_FORMATS[f], pixType=_TYPES[t], depthFunc=pf.PF_LEQUAL))
+    for k, (f, t) in enumerate(_ALL_PAIRS)
+] + [
+    ("pixels_3d_rgba_ubyte", dict(kind=pf.RANDOM_3D, width=96, height=64, ntris=120, seed=340, pixelOps=6,
+                                  pixFormat=7, pixType=0)),
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -151,7 +164,7 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES + FORMAT_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

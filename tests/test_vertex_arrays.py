@@ -292,15 +292,13 @@ def test_framebuffer_pixel_setters():
         L.pfDeleteContext(ctx)
 
 
-@pytest.mark.parametrize("call", ["pfFogProcess", "pfEndList", "pfRectf", "pfDrawPixels", "pfPostProcess"])
+@pytest.mark.parametrize("call", ["pfFogProcess", "pfRectf", "pfPostProcess"])
 def test_out_of_path_calls_report_invalid_operation(call):
     L, ctx, _ = _ctx()
     try:
         fn = getattr(L, call)
         if call == "pfRectf":
             fn(*_fl(0, 0, 1, 1))
-        elif call == "pfDrawPixels":
-            fn(1, 1, 7, 0, None)
         elif call == "pfPostProcess":
             fn(None)
         else:
