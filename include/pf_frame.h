@@ -116,9 +116,12 @@ typedef struct {
 #define PFF_CMD_CLEAR  1u
 #define PFF_CMD_DRAW   2u
 #define PFF_CMD_PIXELS 3u      /* pfDrawPixels: PFFframe.pixelOps[first]      */
+#define PFF_CMD_FOG    4u      /* pfFogProcess: PFFframe.pixelOps[first]      */
 
 /* One pfDrawPixels call (context.c:1987-2080), with everything the call
- * computes from the context before its pixel loop. */
+ * computes from the context before its pixel loop; or, for PFF_CMD_FOG,
+ * one pfFogProcess call (context.c:2275-2343) and the fog state it reads
+ * (image fields unused). */
 typedef struct {
     const void* pixels;        /* copy of the caller's image: width*height
                                   texels of `source`                          */
@@ -131,6 +134,9 @@ typedef struct {
     uint32_t    flags;         /* PFF_DEPTH_TEST, PFF_BLEND                    */
     uint32_t    depthFunc;     /* PFdepthmode */
     uint32_t    blendMode;     /* PFblendmode */
+    uint32_t    fogMode;       /* PFfogmode (any value; unknown ones give t = 0) */
+    float       fogDensity, fogStart, fogEnd;
+    uint32_t    fogColor;      /* packed RGBA8 */
 } PFFpixels;
 
 typedef struct {

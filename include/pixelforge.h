@@ -18,8 +18,10 @@
  *   - Only RGBA / UNSIGNED_BYTE render targets are accepted.  Textures may
  *     use every (format, type) pair that has a texel getter in the reference
  *     (pixel.h:3050-3095); pairs without one are PF_INVALID_ENUM.
- *   - Lines, points, rects, fog and post-processing are not part of this
- *     library (their calls set PF_INVALID_OPERATION).
+ *   - Lines, points and rects are not part of this library (their calls
+ *     set PF_INVALID_OPERATION).  pfFogProcess runs on the GPU in order
+ *     with the draws; pfPostProcess runs the caller's function on the host
+ *     after the frame has been drawn.
  *
  * Errors follow the reference convention: a sticky last-error code that
  * pfGetError() reads and clears (context.c:380-385).  GPU failures map to
@@ -421,12 +423,12 @@ PF_API void*     pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFs
                                     PFpixelformat* format, PFdatatype* type); /* pixelforge.h:2085 */
 
 /* ------------------------------------------------------------------------ */
-/* Outside the GPU triangle path.  Points, lines, rectangles, fog and        */
-/* post-processing keep their signatures so that callers link; their state   */
-/* setters store what the getters return, and the drawing calls set          */
-/* PF_INVALID_OPERATION without drawing.  pfDrawPixels (below, with the      */
-/* raster position and zoom) is recorded and drawn on the GPU in order with  */
-/* the frame's triangles.                                                    */
+/* Outside the GPU triangle path.  Points, lines and rectangles keep their   */
+/* signatures so that callers link; their state setters store what the      */
+/* getters return, and the drawing calls set PF_INVALID_OPERATION without    */
+/* drawing.  pfDrawPixels (with the raster position and zoom) and            */
+/* pfFogProcess are recorded and run on the GPU in order with the frame's    */
+/* triangles; pfPostProcess runs on the host after the frame.                */
 /* ------------------------------------------------------------------------ */
 
 PF_API void pfLineWidth(PFfloat width);                                    /* pixelforge.h:823 */

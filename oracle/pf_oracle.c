@@ -26,8 +26,9 @@
  *       even texture size -- are clamped to the buffer instead of reading
  *       past it.
  *   D4  The glibc powf of the Gouraud path (lighting.c:115) is evaluated as
- *       (float)pow((double)x, (double)y), i.e. the correctly rounded powf
- *       in all but vanishingly rare cases.
+ *       (float)pow((double)x, (double)y), and the expf / exp2f of the fog
+ *       pass (context.c:2327-2331) as (float)exp / exp2 of the double, i.e.
+ *       the correctly rounded results in all but vanishingly rare cases.
  *   D5  Pixels outside the render target (possible in the reference with a
  *       viewport larger than, or offset from, the framebuffer in 3D, where
  *       the bounding box is not clamped) are skipped instead of written out
@@ -1155,6 +1156,34 @@ static void draw_pixels(const PFFframe* f, const PFFpixels* o)
     }
 }
 
+/* pfFogProcess (context.c:2275-2343): the fog colour blended (scalar ALPHA)
+ * into every pixel from its depth; at or beyond the end with the fog
+ * colour's own alpha, between start and end with t * alpha.  expf / exp2f
+ * are evaluated as (float)exp((double)x) (D4). */
+static void fog_process(const PFFframe* f, const PFFpixels* o)
+{
+    const uint32_t fc = o->fogColor, alpha = fc >> 24;
+    const float start = o->fogStart, end = o->fogEnd, density = o->fogDensity;
+    const float invLen = 1 / (end - start);
+    const size_t n = (size_t)f->width * f->height;
+    for (size_t i = 0; i < n; i++) {
+        const float depth = f->depth[i];
+        if (depth >= end) {
+            f->color[i] = alpha == 255u ? fc : blend_scalar(1, fc, f->color[i]);
+        } else if (depth > start) {
+            float t = 0.0f;
+            switch (o->fogMode) {
+            case 0: t = (depth - start) * invLen; break;
+            case 1: t = 1.0f - (float)exp((double)(-density * (depth - start))); break;
+            case 2: t = 1.0f - (float)exp2((double)(-density * (depth - start))); break;
+            default: break;
+            }
+            const uint32_t a = f2u8(t * (float)alpha);
+            f->color[i] = blend_scalar(1, (fc & 0x00FFFFFFu) | (a << 24), f->color[i]);
+        }
+    }
+}
+
 static int render_frame(RCtx* rc);
 
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
@@ -1174,9 +1203,10 @@ static int render_frame(RCtx* rcp)
     for (uint32_t c = 0; c < f->numCmds; c++) {
         const PFFcmd* cmd = &f->cmds[c];
         if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
-        if (cmd->kind == PFF_CMD_PIXELS) {
+        if (cmd->kind == PFF_CMD_PIXELS || cmd->kind == PFF_CMD_FOG) {
             if (cmd->first >= f->numPixelOps) return -1;
-            draw_pixels(f, &f->pixelOps[cmd->first]);
+            if (cmd->kind == PFF_CMD_PIXELS) draw_pixels(f, &f->pixelOps[cmd->first]);
+            else fog_process(f, &f->pixelOps[cmd->first]);
             continue;
         }
         if (cmd->kind != PFF_CMD_DRAW) return -1;

@@ -47,6 +47,7 @@ class SceneParams(C.Structure):
         ("colType", C.c_int32), ("colSize", C.c_int32),
         ("texFormat", C.c_int32), ("texType", C.c_int32), ("texSwizzle", C.c_int32),
         ("pixelOps", C.c_int32), ("pixFormat", C.c_int32), ("pixType", C.c_int32),
+        ("fog", C.c_int32), ("fogStart", C.c_float), ("fogEnd", C.c_float), ("fogAlpha", C.c_int32),
     ]
 
 

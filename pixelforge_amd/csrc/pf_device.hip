@@ -2405,6 +2405,7 @@ __global__ __launch_bounds__(256) void k_decode_image(const uint8_t* __restrict_
 }
 
 struct DPixOp {
+    uint32_t  kind;               // 0 pfDrawPixels, 1 pfFogProcess
     const uint32_t* texels;       // w*h RGBA8
     uint32_t* color;              // render target
     float*    depth;
@@ -2413,6 +2414,8 @@ struct DPixOp {
     int32_t   xScreen, yScreen, xMin, yMin, xMax, yMax;
     float     zPos, invXLen, invYLen;
     uint32_t  flags, depthFunc, blendMode;
+    uint32_t  fogMode, fogColor;
+    float     fogDensity, fogStart, fogEnd;
 };
 
 // (PFsizei)(float) of gcc on x86-64: cvttss2si into a 64-bit register, low
@@ -2455,9 +2458,32 @@ __device__ __forceinline__ uint32_t blend_scalar(uint32_t mode, uint32_t s, uint
     return o;
 }
 
-// grid.y = the pass's calls; each thread strides over its call's rectangle.
+// pfFogProcess (context.c:2275-2343) on one pixel: the fog colour blended
+// (scalar ALPHA) from the pixel's depth; expf / exp2f as the float rounding
+// of the double results.
+__device__ __forceinline__ uint32_t fog_pixel(const DPixOp& o, float depth, uint32_t dst) {
+    const uint32_t fc = o.fogColor, alpha = fc >> 24;
+    if (depth >= o.fogEnd) return alpha == 255u ? fc : blend_scalar(1u, fc, dst);
+    if (!(depth > o.fogStart)) return dst;
+    float t = 0.0f;
+    const float d = -o.fogDensity * (depth - o.fogStart);
+    if (o.fogMode == 0u) t = (depth - o.fogStart) * __fdiv_rn(1.0f, o.fogEnd - o.fogStart);
+    else if (o.fogMode == 1u) t = 1.0f - (float)exp((double)d);
+    else if (o.fogMode == 2u) t = 1.0f - (float)exp2((double)d);
+    const uint32_t a = f2u8(t * (float)alpha);
+    return blend_scalar(1u, (fc & 0x00FFFFFFu) | (a << 24), dst);
+}
+
+// grid.y = the pass's calls; each thread strides over its call's rectangle
+// (the whole target for fog).
 __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ ops) {
     const DPixOp o = ops[blockIdx.y];
+    if (o.kind == 1u) {
+        const size_t n = (size_t)o.W * o.H;
+        for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+            o.color[k] = fog_pixel(o, o.depth[k], o.color[k]);
+        return;
+    }
     if (o.xMax < o.xMin || o.yMax < o.yMin) return;
     const uint32_t rw = (uint32_t)(o.xMax - o.xMin) + 1u, rh = (uint32_t)(o.yMax - o.yMin) + 1u;
     const uint64_t area = (uint64_t)rw * rh;
@@ -3008,6 +3034,28 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 cur.clearColor = cmd.clearColor;
                 cur.clearDepth = cmd.clearDepth;
                 open = true;
+            } else if (cmd.kind == PFF_CMD_FOG) {
+                // A pass of its own, like a pfDrawPixels call.
+                if (!fr.pixelOps || cmd.first >= fr.numPixelOps) {
+                    snprintf(g_err, sizeof g_err, "bad fog command");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                const PFFpixels& op = fr.pixelOps[cmd.first];
+                DPixOp d;
+                memset(&d, 0, sizeof d);
+                d.kind = 1u;
+                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.fogMode = op.fogMode; d.fogColor = op.fogColor;
+                d.fogDensity = op.fogDensity; d.fogStart = op.fogStart; d.fogEnd = op.fogEnd;
+                d.xMin = 0; d.yMin = 0; d.xMax = (int32_t)D.W - 1; d.yMax = (int32_t)D.H - 1;
+                flush_seg();
+                cur.triFirst = cur.triLast = triBase;
+                open = true;
+                if (J->passes.size() <= segNo) J->passes.emplace_back();
+                J->passes[segNo].pix.push_back(d);
+                cur.pixOp = (uint32_t)J->passes[segNo].pix.size();
+                flush_seg();
             } else if (cmd.kind == PFF_CMD_PIXELS) {
                 // A pass of its own: no triangles, the call runs after the
                 // pass's raster kernel, so it lands between the draws
@@ -3388,10 +3436,11 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     *out = nullptr;
     // One pass: at most one clear, before every draw.
     for (uint32_t c = 0; c < frame->numCmds; c++)
-        if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS) {
+        if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS
+            || frame->cmds[c].kind == PFF_CMD_FOG) {
             snprintf(g_err, sizeof g_err, "differentiable frame: %s",
-                     frame->cmds[c].kind == PFF_CMD_PIXELS ? "pfDrawPixels is not differentiable here"
-                                                           : "a clear after the first command");
+                     frame->cmds[c].kind == PFF_CMD_CLEAR ? "a clear after the first command"
+                                                          : "pfDrawPixels / pfFogProcess are not differentiable here");
             return PFD_E_INVALID;
         }
     for (uint32_t i = 0; i < frame->numStates; i++) {

@@ -151,6 +151,9 @@ typedef struct {
 
     float    pointSize, lineWidth;     /* out-of-path state, kept for the getters */
     float    pixelZoom[2], rasterPos[4];
+    int32_t  fogMode;                  /* PFIfog, context.h:250-256 */
+    float    fogDensity, fogStart, fogEnd;
+    PFcolor  fogColor;
 
     PFerrcode err;
     Rec      rec;
@@ -776,6 +779,11 @@ PFcontext pfCreateContext(void* targetBuffer, PFsizei width, PFsizei height, PFp
     c->followMode = PF_AMBIENT_AND_DIFFUSE;
     c->pointSize = c->lineWidth = 1.0f;                                 /* context.c:166-180 */
     c->pixelZoom[0] = c->pixelZoom[1] = 1.0f;
+    c->fogMode = PF_LINEAR;                                   /* context.c:203-209 */
+    c->fogDensity = 1.0f;
+    c->fogStart = 0.0f;
+    c->fogEnd = 1.0f;
+    c->fogColor = (PFcolor){ 0, 0, 0, 0 };
     c->rasterPos[3] = 1.0f;
 
     c->matMode = PF_MODELVIEW;                                         /* context.c:232-239 */
@@ -1767,12 +1775,115 @@ void pfDrawPixels(PFsizei width, PFsizei height, PFpixelformat format, PFdatatyp
     r->npix++;
     push_cmd(c, &cmd);
 }
-void pfFogi(PFfogparam pname, PFint param) { (void)pname; (void)param; not_in_path(); }
-void pfFogf(PFfogparam pname, PFfloat param) { (void)pname; (void)param; not_in_path(); }
-void pfFogiv(PFfogparam pname, PFint* param) { (void)pname; (void)param; not_in_path(); }
-void pfFogfv(PFfogparam pname, PFfloat* param) { (void)pname; (void)param; not_in_path(); }
-void pfFogProcess(void) { not_in_path(); }
-void pfPostProcess(PFpostprocessfunc fn) { (void)fn; not_in_path(); }
+/* Fog state setters (context.c:2161-2273), quirks kept: every DENSITY
+ * branch stores into the mode (so the density stays 1), pfFogiv and pfFogfv
+ * accept any MODE / DENSITY value (their range tests use ||). */
+void pfFogi(PFfogparam pname, PFint param)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_FOG_MODE:
+        if (param >= PF_LINEAR && param <= PF_EXP2) c->fogMode = param; else c->err = PF_INVALID_VALUE;
+        break;
+    case PF_FOG_DENSITY:
+        if (param == 0 || param == 1) c->fogMode = param; else c->err = PF_INVALID_VALUE;
+        break;
+    case PF_FOG_START: c->fogStart = (float)param; break;
+    case PF_FOG_END: c->fogEnd = (float)param; break;
+    default: c->err = PF_INVALID_ENUM; break;
+    }
+}
+
+void pfFogf(PFfogparam pname, PFfloat param)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_FOG_DENSITY:
+        if (param >= 0 && param <= 1) c->fogMode = f2i_x86(param); else c->err = PF_INVALID_VALUE;
+        break;
+    case PF_FOG_START: c->fogStart = param; break;
+    case PF_FOG_END: c->fogEnd = param; break;
+    default: c->err = PF_INVALID_ENUM; break;
+    }
+}
+
+void pfFogiv(PFfogparam pname, PFint* param)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_FOG_MODE: case PF_FOG_DENSITY: c->fogMode = *param; break;
+    case PF_FOG_START: c->fogStart = (float)*param; break;
+    case PF_FOG_END: c->fogEnd = (float)*param; break;
+    case PF_FOG_COLOR:
+        c->fogColor = (PFcolor){ (PFubyte)param[0], (PFubyte)param[1], (PFubyte)param[2], (PFubyte)param[3] };
+        break;
+    default: c->err = PF_INVALID_ENUM; break;
+    }
+}
+
+void pfFogfv(PFfogparam pname, PFfloat* param)
+{
+    Ctx* c = G;
+    if (!c) return;
+    switch (pname) {
+    case PF_FOG_DENSITY: c->fogMode = f2i_x86(*param); break;
+    case PF_FOG_START: c->fogStart = *param; break;
+    case PF_FOG_END: c->fogEnd = *param; break;
+    case PF_FOG_COLOR:
+        c->fogColor = (PFcolor){ f2u8(255 * param[0]), f2u8(255 * param[1]), f2u8(255 * param[2]), f2u8(255 * param[3]) };
+        break;
+    default: c->err = PF_INVALID_ENUM; break;
+    }
+}
+
+/* pfFogProcess (context.c:2275-2343): recorded with the fog state as a
+ * PFF_CMD_FOG command; the GPU blends the fog colour into every pixel from
+ * its depth after the draws recorded before the call. */
+void pfFogProcess(void)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!ensure_target(c)) { c->err = PF_INVALID_OPERATION; return; }
+    PFFpixels op;
+    memset(&op, 0, sizeof op);
+    op.fogMode = (uint32_t)c->fogMode;
+    op.fogDensity = c->fogDensity;
+    op.fogStart = c->fogStart;
+    op.fogEnd = c->fogEnd;
+    op.fogColor = pack(c->fogColor);
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->pix, &r->cappix, r->npix + 1, sizeof(PFFpixels))) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+    r->pix[r->npix] = op;
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_FOG;
+    cmd.first = (uint32_t)r->npix;
+    r->npix++;
+    push_cmd(c, &cmd);
+}
+
+/* pfPostProcess (context.c:2397-2440): the caller's function is host code,
+ * so the recorded frame is executed first and the function then runs over
+ * the render target in host memory, pixel by pixel with its depth. */
+void pfPostProcess(PFpostprocessfunc fn)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!fn) { c->err = PF_INVALID_VALUE; return; }
+    submit(c);
+    Tex* t = fb_tex(c->curFB);
+    if (!t || !t->pixels || !c->curFB->zbuffer) { c->err = PF_INVALID_OPERATION; return; }
+    PFcolor* px = (PFcolor*)t->pixels;
+    const float* z = c->curFB->zbuffer;
+    for (int32_t y = 0; y < (int32_t)t->h; y++)
+        for (int32_t x = 0; x < (int32_t)t->w; x++) {
+            const size_t i = (size_t)y * t->w + (size_t)x;
+            px[i] = fn(x, y, z[i], px[i]);
+        }
+}
 
 /* ------------------------------------------------------------------------ */
 /* Render lists (renderlist.c:6-97)                                          */

@@ -83,6 +83,12 @@ typedef struct {
                               * and second half of the triangles (random kinds) */
     int32_t  pixFormat;      /* pfDrawPixels image PFpixelformat                  */
     int32_t  pixType;        /* pfDrawPixels image PFdatatype                     */
+    int32_t  fog;            /* pfFogProcess after the scene (random kinds: between
+                              * the two halves): 1 LINEAR, 2 EXP, 3 EXP2, 4 the
+                              * setters' quirks (DENSITY stores the mode, an
+                              * unknown mode through pfFogiv)                   */
+    float    fogStart, fogEnd;
+    int32_t  fogAlpha;       /* fog colour alpha, 0..255                          */
 } PFSparams;
 
 typedef struct {
@@ -132,6 +138,11 @@ typedef struct {
     void (*ColorPointer)(PFint, PFenum, PFsizei, const void*);
     void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
+    void (*Fogi)(PFfogparam, PFint);
+    void (*Fogf)(PFfogparam, PFfloat);
+    void (*Fogiv)(PFfogparam, PFint*);
+    void (*Fogfv)(PFfogparam, PFfloat*);
+    void (*FogProcess)(void);
     void (*RasterPos3f)(PFfloat, PFfloat, PFfloat);
     void (*PixelZoom)(PFfloat, PFfloat);
     void (*DrawPixels)(PFsizei, PFsizei, PFpixelformat, PFdatatype, const void*);
@@ -164,6 +175,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(VertexPointer); BIND(NormalPointer); BIND(TexCoordPointer); BIND(ColorPointer);
     BIND(DrawElements); BIND(DrawArrays);
     BIND(RasterPos3f); BIND(PixelZoom); BIND(DrawPixels);
+    BIND(Fogi); BIND(Fogf); BIND(Fogiv); BIND(Fogfv); BIND(FogProcess);
     BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
@@ -273,6 +285,22 @@ static void draw_pixel_ops(const PFSapi* a, Rng* r, const PFSparams* p)
         free(px);
     }
     a->PixelZoom(1.0f, 1.0f);
+}
+
+static void apply_fog(const PFSapi* a, const PFSparams* p)
+{
+    if (p->fog == 4) {
+        a->Fogf(PF_FOG_DENSITY, 0.7f);                /* stores (int)0.7 = LINEAR in the mode */
+        PFint m = 7;
+        a->Fogiv(PF_FOG_MODE, &m);                    /* accepted: t = 0 for this mode */
+    } else {
+        a->Fogi(PF_FOG_MODE, p->fog - 1);
+    }
+    a->Fogf(PF_FOG_START, p->fogStart);
+    a->Fogf(PF_FOG_END, p->fogEnd);
+    PFfloat col[4] = { 0.8f, 0.7f, 0.9f, (float)p->fogAlpha / 255.0f };
+    a->Fogfv(PF_FOG_COLOR, col);
+    a->FogProcess();
 }
 
 static SceneTex make_texture(const PFSapi* a, Rng* r, const PFSparams* p)
@@ -728,12 +756,13 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
     if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
     if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
-    if (p->pixelOps > 0 && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
+    if ((p->pixelOps > 0 || p->fog) && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
         PFSparams half = *p;
         half.ntris = p->ntris / 2;
         if (p->kind == PFS_RANDOM_3D) setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
         draw_pixel_ops(a, r, p);
+        if (p->fog) apply_fog(a, p);
         half.ntris = p->ntris - half.ntris;
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
         return;
@@ -753,6 +782,7 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     case PFS_LIST: scene_list(a, r, p, tex->tex); break;
     default: break;
     }
+    if (p->fog) apply_fog(a, p);
 }
 
 /* Renders one scene into an offscreen framebuffer of the bound library and

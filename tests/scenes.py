@@ -133,6 +133,21 @@ PIXEL_SCENES = [
                                   pixFormat=7, pixType=0)),
 ]
 
+# pfFogProcess after lit spheres (depths 2.5..4.3, background at FLT_MAX):
+# each fog mode, an opaque fog colour, the setters' quirks, and fog between
+# two halves of a 3D scene's triangles.
+_FOG = dict(kind=pf.SPHERES, width=64, height=48, spheres=5, sphereLat=8, sphereLon=14, lighting=1, numLights=2,
+            cull=pf.PF_BACK, fogStart=2.6, fogEnd=3.4, fogAlpha=200)
+FOG_SCENES = [
+    ("fog_linear", dict(_FOG, seed=400, fog=1)),
+    ("fog_exp", dict(_FOG, seed=401, fog=2, fogStart=2.4, fogEnd=3.0)),
+    ("fog_exp2", dict(_FOG, seed=402, fog=3)),
+    ("fog_opaque", dict(_FOG, seed=403, fog=1, fogAlpha=255)),
+    ("fog_setter_quirks", dict(_FOG, seed=404, fog=4)),
+    ("fog_between_draws_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=80, seed=405, fog=1,
+                                  fogStart=-0.2, fogEnd=1.5, fogAlpha=180)),
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -164,7 +179,7 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

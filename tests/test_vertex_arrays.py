@@ -292,15 +292,13 @@ def test_framebuffer_pixel_setters():
         L.pfDeleteContext(ctx)
 
 
-@pytest.mark.parametrize("call", ["pfFogProcess", "pfRectf", "pfPostProcess"])
+@pytest.mark.parametrize("call", ["pfRectf"])
 def test_out_of_path_calls_report_invalid_operation(call):
     L, ctx, _ = _ctx()
     try:
         fn = getattr(L, call)
         if call == "pfRectf":
             fn(*_fl(0, 0, 1, 1))
-        elif call == "pfPostProcess":
-            fn(None)
         else:
             fn()
         assert L.pfGetError() == INVALID_OPERATION
