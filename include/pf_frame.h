@@ -117,6 +117,19 @@ typedef struct {
 #define PFF_CMD_DRAW   2u
 #define PFF_CMD_PIXELS 3u      /* pfDrawPixels: PFFframe.pixelOps[first]      */
 #define PFF_CMD_FOG    4u      /* pfFogProcess: PFFframe.pixelOps[first]      */
+#define PFF_CMD_POINTS 5u      /* points [first, last) of PFFframe.points      */
+
+/* One point (pfVertex in PF_POINTS mode, or a corner of a triangle or quad
+ * in PF_POINT polygon mode; points.c:35-57), projected and rasterised on
+ * the device (points.c:59-183). */
+typedef struct {
+    float    position[4];      /* object space */
+    uint32_t color;            /* packed RGBA8 */
+    uint32_t state;            /* index into PFFframe.states (MVP, viewport,
+                                  depth test and blending) */
+    float    size;             /* pfPointSize */
+    uint32_t pad;
+} PFFpoint;
 
 /* One pfDrawPixels call (context.c:1987-2080), with everything the call
  * computes from the context before its pixel loop; or, for PFF_CMD_FOG,
@@ -169,6 +182,7 @@ typedef struct {
     const PFFtexture*  textures; uint32_t numTextures;
     const PFFcmd*      cmds;     uint32_t numCmds;
     const PFFpixels*   pixelOps; uint32_t numPixelOps;
+    const PFFpoint*    points;   uint32_t numPoints;
 } PFFframe;
 
 #endif /* PF_FRAME_H */

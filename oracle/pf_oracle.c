@@ -1184,6 +1184,55 @@ static void fog_process(const PFFframe* f, const PFFpixels* o)
     }
 }
 
+/* One point: Process_ProjectPoint (points.c:59-83) -- MVP, the clip test
+ * and the division of x and y only when w != 1, the screen mapping with its
+ * +0.5 (ictx.c:51-65), the viewport test -- then Rasterize_Point_DEPTH /
+ * _NODEPTH (points.c:85-183): one pixel at size <= 1, else every pixel of
+ * the disc x^2 + y^2 <= (size/2)^2 inside the target; the scalar depth test
+ * and blending; the depth is written also without the depth test. */
+static void plot_point(const PFFframe* f, const PFFstate* st, size_t off, float z, uint32_t color)
+{
+    if ((st->flags & PFF_DEPTH_TEST) && !depth_scalar(st->depthFunc, z, f->depth[off])) return;
+    f->color[off] = (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, color, f->color[off]) : color;
+    f->depth[off] = z;
+}
+
+static void draw_point(const PFFframe* f, const PFFpoint* pt)
+{
+    if (pt->state >= f->numStates) return;
+    const PFFstate* st = &f->states[pt->state];
+    const float* m = st->mvp;
+    const float* v = pt->position;
+    float h[4];
+    for (int i = 0; i < 4; i++) h[i] = m[i] * v[0] + m[4 + i] * v[1] + m[8 + i] * v[2] + m[12 + i] * v[3];
+    if (h[3] != 1.0f) {
+        for (int i = 0; i < 3; i++)
+            if (h[i] < -h[3] || h[i] > h[3]) return;
+        const float invW = 1.0f / h[3];
+        h[0] *= invW;
+        h[1] *= invW;
+    }
+    const float sx = ((float)st->vpPos[0] + ((h[0] + 1.0f) * 0.5f) * (float)st->vpDim[0]) + 0.5f;
+    const float sy = ((float)st->vpPos[1] + ((1.0f - h[1]) * 0.5f) * (float)st->vpDim[1]) + 0.5f;
+    if (!(sx >= (float)st->vpMin[0] && sy >= (float)st->vpMin[1] && sx <= (float)st->vpMax[0] && sy <= (float)st->vpMax[1]))
+        return;
+    const int32_t cx = cvtt_i32(sx), cy = cvtt_i32(sy);
+    const float z = h[2];
+    if (pt->size <= 1.0f) {
+        if (cx < 0 || cy < 0 || (uint32_t)cx >= f->width || (uint32_t)cy >= f->height) return;   /* D5 */
+        plot_point(f, st, (size_t)cy * f->width + (size_t)cx, z, pt->color);
+        return;
+    }
+    const float r = pt->size * 0.5f, rSq = r * r;
+    const int32_t ir = cvtt_i32(r);
+    for (int32_t y = -ir; y <= ir; y++)
+        for (int32_t x = -ir; x <= ir; x++) {
+            if (!((float)(y * y + x * x) <= rSq)) continue;
+            const uint32_t px = (uint32_t)(cx + x), py = (uint32_t)(cy + y);
+            if (px < f->width && py < f->height) plot_point(f, st, (size_t)py * f->width + px, z, pt->color);
+        }
+}
+
 static int render_frame(RCtx* rc);
 
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
@@ -1203,6 +1252,11 @@ static int render_frame(RCtx* rcp)
     for (uint32_t c = 0; c < f->numCmds; c++) {
         const PFFcmd* cmd = &f->cmds[c];
         if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
+        if (cmd->kind == PFF_CMD_POINTS) {
+            if (cmd->first > cmd->last || cmd->last > f->numPoints) return -1;
+            for (uint32_t k = cmd->first; k < cmd->last; k++) draw_point(f, &f->points[k]);
+            continue;
+        }
         if (cmd->kind == PFF_CMD_PIXELS || cmd->kind == PFF_CMD_FOG) {
             if (cmd->first >= f->numPixelOps) return -1;
             if (cmd->kind == PFF_CMD_PIXELS) draw_pixels(f, &f->pixelOps[cmd->first]);

@@ -48,6 +48,7 @@ class SceneParams(C.Structure):
         ("texFormat", C.c_int32), ("texType", C.c_int32), ("texSwizzle", C.c_int32),
         ("pixelOps", C.c_int32), ("pixFormat", C.c_int32), ("pixType", C.c_int32),
         ("fog", C.c_int32), ("fogStart", C.c_float), ("fogEnd", C.c_float), ("fogAlpha", C.c_int32),
+        ("points", C.c_int32), ("pointSize", C.c_float), ("polyPoint", C.c_int32),
     ]
 
 

@@ -2503,6 +2503,105 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Points (points.c:35-183).  k_point_setup projects every point of the job
+// once: MVP, the clip test and the division of x and y when w != 1, the
+// screen mapping (ictx.c:51-65) and the viewport test.  After the raster
+// kernel of the pass a run of points occupies, k_points gives each 16x16
+// tile of the target to a workgroup whose threads own one pixel each and
+// walk the run's points in order: one pixel at size <= 1, else the disc
+// x^2 + y^2 <= (size/2)^2; scalar depth test and blending; the depth is
+// written also without the depth test.
+// ---------------------------------------------------------------------------
+
+struct DPointScr {
+    int32_t  cx, cy;              // screen pixel
+    float    z;                   // clip z (not divided)
+    uint32_t color;
+    int32_t  ir;                  // (int)(size / 2), 0 at size <= 1
+    float    rSq;                 // (size / 2)^2
+    uint32_t flags;               // PFF_DEPTH_TEST | PFF_BLEND; 0x80000000: rejected
+    uint16_t depthFunc, blendMode;
+};
+
+__global__ __launch_bounds__(256) void k_point_setup(const PFFpoint* __restrict__ pts, uint32_t n,
+                                                     const PFFstate* __restrict__ states,
+                                                     DPointScr* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const PFFpoint pt = pts[i];
+    const PFFstate& st = states[pt.state];
+    DPointScr o;
+    o.flags = 0x80000000u;
+    o.cx = o.cy = 0; o.z = 0.0f; o.color = pt.color; o.ir = 0; o.rSq = 0.0f; o.depthFunc = 0; o.blendMode = 0;
+    float h[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        h[k] = st.mvp[k] * pt.position[0] + st.mvp[4 + k] * pt.position[1] + st.mvp[8 + k] * pt.position[2]
+             + st.mvp[12 + k] * pt.position[3];
+    bool ok = true;
+    if (h[3] != 1.0f) {
+        for (int k = 0; k < 3; k++) ok = ok && !(h[k] < -h[3] || h[k] > h[3]);
+        const float invW = __fdiv_rn(1.0f, h[3]);
+        h[0] *= invW;
+        h[1] *= invW;
+    }
+    const float sx = ((float)st.vpPos[0] + ((h[0] + 1.0f) * 0.5f) * (float)st.vpDim[0]) + 0.5f;
+    const float sy = ((float)st.vpPos[1] + ((1.0f - h[1]) * 0.5f) * (float)st.vpDim[1]) + 0.5f;
+    ok = ok && sx >= (float)st.vpMin[0] && sy >= (float)st.vpMin[1] && sx <= (float)st.vpMax[0] && sy <= (float)st.vpMax[1];
+    if (ok) {
+        o.cx = cvtt_i32(sx);
+        o.cy = cvtt_i32(sy);
+        o.z = h[2];
+        if (pt.size > 1.0f) {
+            const float r = pt.size * 0.5f;
+            o.rSq = r * r;
+            o.ir = cvtt_i32(r);
+        }
+        o.flags = st.flags & (PFF_DEPTH_TEST | PFF_BLEND);
+        o.depthFunc = (uint16_t)st.depthFunc;
+        o.blendMode = (uint16_t)st.blendMode;
+    }
+    out[i] = o;
+}
+
+struct DPtOp {
+    uint32_t* color;
+    float*    depth;
+    uint32_t  W, H, tilesX, tilesY;
+    uint32_t  first, last;        // job point indices
+};
+
+__global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, const DPointScr* __restrict__ scr) {
+    const DPtOp o = ops[blockIdx.y];
+    if (blockIdx.x >= o.tilesX * o.tilesY) return;
+    const int32_t x0 = (int32_t)(blockIdx.x % o.tilesX) * 16, y0 = (int32_t)(blockIdx.x / o.tilesX) * 16;
+    const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
+    const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
+    const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
+    uint32_t c = inside ? o.color[off] : 0u;
+    float d = inside ? o.depth[off] : 0.0f;
+    bool dirty = false;
+    for (uint32_t i = o.first; i < o.last; i++) {
+        const DPointScr s = scr[i];
+        if (s.flags & 0x80000000u) continue;
+        if (s.cx + s.ir < x0 || s.cx - s.ir > x0 + 15 || s.cy + s.ir < y0 || s.cy - s.ir > y0 + 15) continue;
+        const int32_t dx = px - s.cx, dy = py - s.cy;
+        const bool cov = s.ir == 0 && s.rSq == 0.0f
+                       ? (dx == 0 && dy == 0)
+                       : (dx >= -s.ir && dx <= s.ir && dy >= -s.ir && dy <= s.ir && (float)(dy * dy + dx * dx) <= s.rSq);
+        if (!inside || !cov) continue;
+        if ((s.flags & PFF_DEPTH_TEST) && !depth_scalar(s.depthFunc, s.z, d)) continue;
+        c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, c) : s.color;
+        d = s.z;
+        dirty = true;
+    }
+    if (dirty) {
+        o.color[off] = c;
+        o.depth[off] = d;
+    }
+}
+
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -2563,6 +2662,9 @@ struct PassInfo {
     std::vector<DPixOp> pix;   // pfDrawPixels calls run after this pass's raster
     uint32_t pixGrid = 0;
     DevBuf   dPix;
+    std::vector<DPtOp> pts;    // runs of points, after the pixel calls
+    uint32_t ptTiles = 0;
+    DevBuf   dPts;
 };
 
 }  // namespace
@@ -2582,6 +2684,7 @@ struct PFDjob {
     DevBuf pos, col, uv, nrm, info, states, texTable;
     std::vector<DevBuf> texData;
     std::vector<DevBuf> pixData;   // decoded pfDrawPixels images
+    DevBuf   points, pointScr;     // PFFpoint of all frames, their projections
     uint32_t numIn = 0;
     bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
     std::vector<PassInfo> passes;
@@ -2788,6 +2891,9 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (!P.pix.empty())
         hipLaunchKernelGGL(k_draw_pixels, dim3(P.pixGrid, (uint32_t)P.pix.size()), dim3(256), 0, J->stream,
                            (const DPixOp*)P.dPix.p);
+    if (!P.pts.empty())
+        hipLaunchKernelGGL(k_points, dim3(P.ptTiles, (uint32_t)P.pts.size()), dim3(256), 0, J->stream,
+                           (const DPtOp*)P.dPts.p, (const DPointScr*)J->pointScr.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
     P.sized = true;
@@ -2922,7 +3028,8 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     std::vector<float> huv(anyUV ? (size_t)totalTris * 6 : 0), hnrm(anyNrm ? (size_t)totalTris * 9 : 0);
     std::vector<PFFstate> hstates(totalStates ? totalStates : 1);
     std::vector<DTex> htex;
-    uint32_t triBase = 0, stateBase = 0;
+    std::vector<PFFpoint> hpts;
+    uint32_t triBase = 0, stateBase = 0, ptBase = 0;
     J->src.assign(frames, frames + count);
     J->leadClear.resize(count);
     for (uint32_t f = 0; f < count; f++)
@@ -3010,6 +3117,12 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         D.color = (uint32_t*)J->targets[2 * f].p;
         D.depth = (float*)J->targets[2 * f + 1].p;
 
+        for (uint32_t k = 0; k < fr.numPoints; k++) {
+            PFFpoint q = fr.points[k];
+            if (q.state >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad point state"); pfdJobDestroy(J); return PFD_E_INVALID; }
+            q.state += stateBase;
+            hpts.push_back(q);
+        }
         // Commands -> segments; the k-th segment of a frame runs in pass k.
         uint32_t segNo = 0;
         DSeg cur;
@@ -3034,6 +3147,24 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 cur.clearColor = cmd.clearColor;
                 cur.clearDepth = cmd.clearDepth;
                 open = true;
+            } else if (cmd.kind == PFF_CMD_POINTS) {
+                // A pass of its own; the run's points are applied in order.
+                if (!fr.points || cmd.first > cmd.last || cmd.last > fr.numPoints) {
+                    snprintf(g_err, sizeof g_err, "bad points command");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                DPtOp d;
+                memset(&d, 0, sizeof d);
+                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
+                d.first = ptBase + cmd.first; d.last = ptBase + cmd.last;
+                flush_seg();
+                cur.triFirst = cur.triLast = triBase;
+                open = true;
+                if (J->passes.size() <= segNo) J->passes.emplace_back();
+                J->passes[segNo].pts.push_back(d);
+                flush_seg();
             } else if (cmd.kind == PFF_CMD_FOG) {
                 // A pass of its own, like a pfDrawPixels call.
                 if (!fr.pixelOps || cmd.first >= fr.numPixelOps) {
@@ -3121,6 +3252,7 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         flush_seg();
         triBase += (uint32_t)nt;
         stateBase += fr.numStates;
+        ptBase += fr.numPoints;
     }
     // Device copies of the inputs.
     auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
@@ -3139,6 +3271,20 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex)))) {
         pfdJobDestroy(J);
         return rc;
+    }
+    if (!hpts.empty()) {
+        // Points are projected once; every replay reads the projections.
+        if ((rc = up(J->points, hpts.data(), hpts.size() * sizeof(PFFpoint)))
+            || (rc = ensure(J->pointScr, hpts.size() * sizeof(DPointScr)))) {
+            pfdJobDestroy(J);
+            return rc;
+        }
+        const uint32_t np = (uint32_t)hpts.size();
+        hipLaunchKernelGGL(k_point_setup, dim3((np + 255) / 256), dim3(256), 0, J->stream, (const PFFpoint*)J->points.p,
+                           np, (const PFFstate*)J->states.p, (DPointScr*)J->pointScr.p);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+        if (e != hipSuccess) { set_error("point setup", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
     }
     // Per-pass tables.
     for (auto& P : J->passes) {
@@ -3166,6 +3312,12 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             const uint64_t g = (maxArea + 255) / 256;
             P.pixGrid = (uint32_t)(g < 4096 ? g : 4096);
             if ((rc = up(P.dPix, P.pix.data(), P.pix.size() * sizeof(DPixOp)))) { pfdJobDestroy(J); return rc; }
+        }
+        if (!P.pts.empty()) {
+            uint32_t maxTiles = 1;
+            for (const DPtOp& d : P.pts) maxTiles = std::max(maxTiles, d.tilesX * d.tilesY);
+            P.ptTiles = maxTiles;
+            if ((rc = up(P.dPts, P.pts.data(), P.pts.size() * sizeof(DPtOp)))) { pfdJobDestroy(J); return rc; }
         }
         if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
             || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
@@ -3378,7 +3530,9 @@ void pfdJobDestroy(PFDjob* J) {
     for (auto& b : J->targets) release(b);
     for (auto& b : J->texData) release(b);
     for (auto& b : J->pixData) release(b);
-    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); }
+    release(J->points);
+    release(J->pointScr);
+    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); release(P.dPts); }
     if (J->side) (void)hipStreamSynchronize(J->side);
     if (J->fork) (void)hipEventDestroy(J->fork);
     if (J->join) (void)hipEventDestroy(J->join);
@@ -3437,10 +3591,10 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     // One pass: at most one clear, before every draw.
     for (uint32_t c = 0; c < frame->numCmds; c++)
         if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS
-            || frame->cmds[c].kind == PFF_CMD_FOG) {
+            || frame->cmds[c].kind == PFF_CMD_FOG || frame->cmds[c].kind == PFF_CMD_POINTS) {
             snprintf(g_err, sizeof g_err, "differentiable frame: %s",
                      frame->cmds[c].kind == PFF_CMD_CLEAR ? "a clear after the first command"
-                                                          : "pfDrawPixels / pfFogProcess are not differentiable here");
+                                                          : "points, pfDrawPixels and pfFogProcess are not differentiable here");
             return PFD_E_INVALID;
         }
     for (uint32_t i = 0; i < frame->numStates; i++) {

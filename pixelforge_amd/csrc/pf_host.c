@@ -105,6 +105,7 @@ typedef struct {                       /* the frame being recorded */
     PFFtexture* texs;    const Tex** texKeys; size_t ntex, captex;
     PFFcmd*    cmds;     size_t ncmd, capcmd;
     PFFpixels* pix;      size_t npix, cappix;   /* pfDrawPixels calls; images owned */
+    PFFpoint*  pts;      size_t npts, cappts;
     int        anyTex, anyLight, anyVid;
 } Rec;
 
@@ -307,6 +308,7 @@ static void rec_reset(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
     r->npix = 0;
+    r->npts = 0;
     r->ntri = r->nstates = r->ntex = r->ncmd = 0;
     r->anyTex = r->anyLight = r->anyVid = 0;
     r->targetColor = NULL;
@@ -317,6 +319,7 @@ static void rec_free(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
     free(r->pix);
+    free(r->pts);
     free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info); free(r->vid);
     free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
     memset(r, 0, sizeof *r);
@@ -353,6 +356,8 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->numCmds = (uint32_t)r->ncmd;
     f->pixelOps = r->pix;
     f->numPixelOps = (uint32_t)r->npix;
+    f->points = r->pts;
+    f->numPoints = (uint32_t)r->npts;
 }
 
 /* Executes the recorded frame, if any, and starts an empty one. */
@@ -565,21 +570,56 @@ static uint32_t face_pass(int face) { return face == PF_FRONT ? PFF_PASS_FRONT :
 
 /* pfiProcessAndRasterize (internal/context/context.c:94-244), triangle
  * modes.  Point and line polygon modes are not part of this library. */
+/* Appends one point (points.c:35-57); consecutive points share one
+ * PFF_CMD_POINTS command. */
+static void emit_point(Ctx* c, const Vert* v)
+{
+    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); return; }
+    int si = current_state(c);
+    if (si < 0) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->pts, &r->cappts, r->npts + 1, sizeof(PFFpoint))) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    PFFpoint* p = &r->pts[r->npts];
+    memcpy(p->position, v->position, 16);
+    p->color = pack(v->color);
+    p->state = (uint32_t)si;
+    p->size = c->pointSize;
+    p->pad = 0;
+    const uint32_t k = (uint32_t)r->npts++;
+    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_POINTS && r->cmds[r->ncmd - 1].last == k) {
+        r->cmds[r->ncmd - 1].last = k + 1;
+        return;
+    }
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_POINTS;
+    cmd.first = k;
+    cmd.last = k + 1;
+    push_cmd(c, &cmd);
+}
+
+/* pfiProcessRasterize_POLY_POINTS (points.c:45-57): the n corners. */
+static void emit_poly_points(Ctx* c, int n)
+{
+    for (int i = 0; i < n; i++) emit_point(c, &c->vb[i]);
+}
+
 static void process_primitive(Ctx* c)
 {
     int culled = (c->state & PF_CULL_FACE) != 0;
     int face = culled ? !c->cullFace : PF_FRONT_AND_BACK;
     switch (c->drawMode) {
-    case PF_TRIANGLES:
-        if (face == PF_FRONT_AND_BACK) {
-            uint32_t passes = 0;
-            if (c->polyMode[0] == PF_FILL) passes |= PFF_PASS_FRONT;
-            if (c->polyMode[1] == PF_FILL) passes |= PFF_PASS_BACK;
-            if (passes != (PFF_PASS_FRONT | PFF_PASS_BACK)) set_err(PF_INVALID_OPERATION);
-            if (passes) emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], passes);
-        } else if (c->polyMode[face] == PF_FILL) {
-            emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], face_pass(face));
-        } else set_err(PF_INVALID_OPERATION);
+    case PF_TRIANGLES:                       /* ictx.c:104-138: each face by its polygon mode */
+        if (face == PF_FRONT_AND_BACK && c->polyMode[0] == PF_FILL && c->polyMode[1] == PF_FILL) {
+            emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], PFF_PASS_FRONT | PFF_PASS_BACK);
+            break;
+        }
+        for (int f = 0; f < 2; f++) {
+            if (face != PF_FRONT_AND_BACK && face != f) continue;
+            if (c->polyMode[f] == PF_FILL) emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], face_pass(f));
+            else if (c->polyMode[f] == PF_POINT) emit_poly_points(c, 3);
+            else set_err(PF_INVALID_OPERATION);                          /* PF_LINE */
+        }
         break;
     case PF_TRIANGLE_FAN:
     case PF_QUAD_FAN:
@@ -597,10 +637,14 @@ static void process_primitive(Ctx* c)
         for (int f = 0; f < 2; f++) {
             if (face != PF_FRONT_AND_BACK && face != f) continue;
             if (c->polyMode[f] == PF_FILL) emit_fan(c, face_pass(f), 2);
-            else set_err(PF_INVALID_OPERATION);
+            else if (c->polyMode[f] == PF_POINT) emit_poly_points(c, 4);
+            else set_err(PF_INVALID_OPERATION);                          /* PF_LINE */
         }
         break;
-    default:                                 /* PF_POINTS, PF_LINES */
+    case PF_POINTS:                          /* ictx.c:98-100 */
+        emit_point(c, &c->vb[0]);
+        break;
+    default:                                 /* PF_LINES */
         set_err(PF_INVALID_OPERATION);
         break;
     }

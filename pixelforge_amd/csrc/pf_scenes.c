@@ -89,6 +89,10 @@ typedef struct {
                               * unknown mode through pfFogiv)                   */
     float    fogStart, fogEnd;
     int32_t  fogAlpha;       /* fog colour alpha, 0..255                          */
+    int32_t  points;         /* >0: that many PF_POINTS vertices between the two
+                              * halves of the triangles (random kinds)          */
+    float    pointSize;      /* >0: pfPointSize                                   */
+    int32_t  polyPoint;      /* bit 0: front faces PF_POINT, bit 1: back faces   */
 } PFSparams;
 
 typedef struct {
@@ -138,6 +142,8 @@ typedef struct {
     void (*ColorPointer)(PFint, PFenum, PFsizei, const void*);
     void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
+    void (*PointSize)(PFfloat);
+    void (*PolygonMode)(PFface, PFpolygonmode);
     void (*Fogi)(PFfogparam, PFint);
     void (*Fogf)(PFfogparam, PFfloat);
     void (*Fogiv)(PFfogparam, PFint*);
@@ -176,6 +182,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(DrawElements); BIND(DrawArrays);
     BIND(RasterPos3f); BIND(PixelZoom); BIND(DrawPixels);
     BIND(Fogi); BIND(Fogf); BIND(Fogiv); BIND(Fogfv); BIND(FogProcess);
+    BIND(PointSize); BIND(PolygonMode);
     BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
@@ -287,6 +294,18 @@ static void draw_pixel_ops(const PFSapi* a, Rng* r, const PFSparams* p)
     a->PixelZoom(1.0f, 1.0f);
 }
 
+/* PF_POINTS: random positions in [-1, 1]^3 of the current matrices, random
+ * colours. */
+static void draw_points(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    a->Begin(PF_POINTS);
+    for (int i = 0; i < p->points; i++) {
+        a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
+        a->Vertex3f(rrange(r, -1, 1), rrange(r, -1, 1), rrange(r, -1, 1));
+    }
+    a->End();
+}
+
 static void apply_fog(const PFSapi* a, const PFSparams* p)
 {
     if (p->fog == 4) {
@@ -344,6 +363,9 @@ static void free_texture(const PFSapi* a, SceneTex* st)
 
 static void common_state(const PFSapi* a, const PFSparams* p)
 {
+    if (p->pointSize > 0) a->PointSize(p->pointSize);
+    if (p->polyPoint & 1) a->PolygonMode(PF_FRONT, PF_POINT);
+    if (p->polyPoint & 2) a->PolygonMode(PF_BACK, PF_POINT);
     if (p->cull < 0) a->Disable(PF_CULL_FACE);
     else { a->Enable(PF_CULL_FACE); a->CullFace((PFface)p->cull); }
     if (p->depthTest) { a->Enable(PF_DEPTH_TEST); a->DepthFunc((PFdepthmode)p->depthFunc); }
@@ -756,12 +778,13 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
     if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
     if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
-    if ((p->pixelOps > 0 || p->fog) && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
+    if ((p->pixelOps > 0 || p->fog || p->points > 0) && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
         PFSparams half = *p;
         half.ntris = p->ntris / 2;
         if (p->kind == PFS_RANDOM_3D) setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
         draw_pixel_ops(a, r, p);
+        if (p->points > 0) draw_points(a, r, p);
         if (p->fog) apply_fog(a, p);
         half.ntris = p->ntris - half.ntris;
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);

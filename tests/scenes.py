@@ -148,6 +148,26 @@ FOG_SCENES = [
                                   fogStart=-0.2, fogEnd=1.5, fogAlpha=180)),
 ]
 
+# Points: PF_POINTS between two halves of the triangles (single pixels,
+# discs, blended discs), and the PF_POINT polygon mode for triangles (front,
+# both faces twice, culled) and for quads (the render-list sphere's lower
+# half).
+POINT_SCENES = [
+    ("points_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=500, points=300,
+                        depthFunc=pf.PF_LEQUAL)),
+    ("points_discs_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=501, points=120, pointSize=5.5)),
+    ("points_blend", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=502, points=150, pointSize=3.0,
+                          blend=1, blendMode=3)),
+    ("poly_point_front", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=80, seed=503, polyPoint=1,
+                              pointSize=2.0)),
+    ("poly_point_both", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=80, seed=504, polyPoint=3, blend=1,
+                             blendMode=2)),
+    ("poly_point_culled", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=80, seed=505, polyPoint=2,
+                               cull=pf.PF_FRONT, pointSize=4.0)),
+    ("poly_point_quads_list", dict(kind=pf.LIST, width=W, height=H, texture=1, texW=33, texH=33,
+                                   filter=pf.PF_NEAREST, seed=506, polyPoint=3, pointSize=3.0, cull=-1)),
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -179,7 +199,7 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 
