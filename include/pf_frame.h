@@ -118,6 +118,20 @@ typedef struct {
 #define PFF_CMD_PIXELS 3u      /* pfDrawPixels: PFFframe.pixelOps[first]      */
 #define PFF_CMD_FOG    4u      /* pfFogProcess: PFFframe.pixelOps[first]      */
 #define PFF_CMD_POINTS 5u      /* points [first, last) of PFFframe.points      */
+#define PFF_CMD_LINES  6u      /* lines [first, last) of PFFframe.lines        */
+
+/* One line (a pair of PF_LINES vertices, or an edge of a triangle or quad
+ * in PF_LINE polygon mode; lines.c:56-105), projected, clipped and
+ * rasterised on the device (lines.c:107-532).  The edges of one polygon
+ * form a group: an edge the clipper rejects ends its group (lines.c:93). */
+typedef struct {
+    float    position[2][4];   /* object space */
+    uint32_t color[2];         /* packed RGBA8 */
+    uint32_t state;            /* index into PFFframe.states */
+    float    width;            /* pfLineWidth */
+    uint32_t groupStart;       /* 1 for the first line of a group */
+    uint32_t pad;
+} PFFline;
 
 /* One point (pfVertex in PF_POINTS mode, or a corner of a triangle or quad
  * in PF_POINT polygon mode; points.c:35-57), projected and rasterised on
@@ -183,6 +197,7 @@ typedef struct {
     const PFFcmd*      cmds;     uint32_t numCmds;
     const PFFpixels*   pixelOps; uint32_t numPixelOps;
     const PFFpoint*    points;   uint32_t numPoints;
+    const PFFline*     lines;    uint32_t numLines;
 } PFFframe;
 
 #endif /* PF_FRAME_H */

@@ -18,9 +18,8 @@
  *   - Only RGBA / UNSIGNED_BYTE render targets are accepted.  Textures may
  *     use every (format, type) pair that has a texel getter in the reference
  *     (pixel.h:3050-3095); pairs without one are PF_INVALID_ENUM.
- *   - Lines (PF_LINES, the PF_LINE polygon mode) and rects are not part of
- *     this library (they set PF_INVALID_OPERATION).  Points are drawn on
- *     the GPU in order with the triangles.  pfFogProcess runs on the GPU in order
+ *   - Rects are not part of this library (they set PF_INVALID_OPERATION).
+ *     Points and lines are drawn on the GPU in order with the triangles.  pfFogProcess runs on the GPU in order
  *     with the draws; pfPostProcess runs the caller's function on the host
  *     after the frame has been drawn.
  *
@@ -424,10 +423,9 @@ PF_API void*     pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFs
                                     PFpixelformat* format, PFdatatype* type); /* pixelforge.h:2085 */
 
 /* ------------------------------------------------------------------------ */
-/* Outside the GPU triangle path.  Lines and rectangles keep their          */
-/* signatures so that callers link; their state setters store what the      */
-/* getters return, and the drawing calls set PF_INVALID_OPERATION without    */
-/* drawing; pfPointSize applies to points.  pfDrawPixels (with the raster position and zoom) and            */
+/* Outside the GPU triangle path.  Rectangles keep their signatures so that  */
+/* callers link and set PF_INVALID_OPERATION without drawing; pfLineWidth    */
+/* and pfPointSize apply to lines and points.  pfDrawPixels (with the raster position and zoom) and            */
 /* pfFogProcess are recorded and run on the GPU in order with the frame's    */
 /* triangles; pfPostProcess runs on the host after the frame.                */
 /* ------------------------------------------------------------------------ */

@@ -41,6 +41,13 @@
  *       the AVX2 getter's shuffle (pixel.h:2916-2920 with the mask of
  *       simd.h:564-574) gives every group of four SIMD lanes its first
  *       lane's texel, which depends on how pixels are grouped.
+ *   D9  The 2D line clipper (lines.c:140-189) may not terminate for some
+ *       lines (its BOTTOM / TOP steps move a point to the opposite edge);
+ *       here a line still outside after 64 steps is rejected.  Line pixels
+ *       are addressed as y * width + x, as in the reference, so an x past a
+ *       side of the target lands in the neighbouring row; offsets outside
+ *       the buffer are skipped.  Thick lines keep at most 16384 copies per
+ *       side, and copies wrapping by more than one row are not drawn.
  *   D8  A pfDrawPixels source offset past the image (possible only for a
  *       raster position beyond the viewport's far edge) reads the image's
  *       last texel; the reference reads past the caller's buffer.
@@ -1233,6 +1240,180 @@ static void draw_point(const PFFframe* f, const PFFpoint* pt)
         }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Lines (lines.c:56-532)                                                    */
+/* ------------------------------------------------------------------------ */
+
+#define LINE_CLIP_STEPS 64            /* D9 */
+
+typedef struct {
+    float    hom[4];
+    float    scr[2];
+    uint32_t color;
+} LVtx;
+
+static void line_to_screen(const PFFstate* st, LVtx* v)            /* ictx.c:51-65 */
+{
+    v->scr[0] = ((float)st->vpPos[0] + ((v->hom[0] + 1.0f) * 0.5f) * (float)st->vpDim[0]) + 0.5f;
+    v->scr[1] = ((float)st->vpPos[1] + ((1.0f - v->hom[1]) * 0.5f) * (float)st->vpDim[1]) + 0.5f;
+}
+
+static uint32_t clip_code2d(const float* s, const PFFstate* st)   /* lines.c:107-115 */
+{
+    uint32_t c = 0;
+    if (s[0] < (float)st->vpMin[0]) c |= 1u;      /* LEFT */
+    if (s[0] > (float)st->vpMax[0]) c |= 2u;      /* RIGHT */
+    if (s[1] < (float)st->vpMin[1]) c |= 8u;      /* TOP */
+    if (s[1] > (float)st->vpMax[1]) c |= 4u;      /* BOTTOM */
+    return c;
+}
+
+/* Process_ClipLine2D (lines.c:140-189): Cohen-Sutherland on the screen
+ * positions, with the reference's mapping of BOTTOM (below the viewport) to
+ * its top edge and TOP to its bottom edge. */
+static int clip_line2d(const PFFstate* st, LVtx* v1, LVtx* v2)
+{
+    float m = 0;
+    if (v1->scr[0] != v2->scr[0]) m = (v2->scr[1] - v1->scr[1]) / (v2->scr[0] - v1->scr[0]);
+    for (int step = 0; step < LINE_CLIP_STEPS; step++) {
+        uint32_t c0 = clip_code2d(v1->scr, st), c1 = clip_code2d(v2->scr, st);
+        if ((c0 | c1) == 0) return 1;
+        if (c0 & c1) return 0;
+        if (c0 == 0) { LVtx t = *v1; *v1 = *v2; *v2 = t; c0 = c1; }
+        if (c0 & 1u) { v1->scr[1] += ((float)st->vpMin[0] - v1->scr[0]) * m; v1->scr[0] = (float)st->vpMin[0]; }
+        else if (c0 & 2u) { v1->scr[1] += ((float)st->vpMax[0] - v1->scr[0]) * m; v1->scr[0] = (float)st->vpMax[0]; }
+        else if (c0 & 4u) { if (m != 0.0f) v1->scr[0] += ((float)st->vpMin[1] - v1->scr[1]) / m; v1->scr[1] = (float)st->vpMin[1]; }
+        else if (c0 & 8u) { if (m != 0.0f) v1->scr[0] += ((float)st->vpMax[1] - v1->scr[1]) / m; v1->scr[1] = (float)st->vpMax[1]; }
+    }
+    return 0;
+}
+
+static int clip_coord3d(float q, float p, float* t1, float* t2)    /* lines.c:117-138 */
+{
+    if (fabsf(p) < 1e-5f) return !(q < -1e-5f);
+    const float r = q / p;
+    if (p < 0) {
+        if (r > *t2) return 0;
+        if (r > *t1) *t1 = r;
+    } else {
+        if (r < *t1) return 0;
+        if (r < *t2) *t2 = r;
+    }
+    return 1;
+}
+
+static int clip_line3d(LVtx* v1, LVtx* v2)                         /* lines.c:191-220 */
+{
+    float t1 = 0, t2 = 1, d[4];
+    for (int i = 0; i < 4; i++) d[i] = v2->hom[i] - v1->hom[i];
+    const float* h = v1->hom;
+    if (!clip_coord3d(h[3] - h[0], -d[3] + d[0], &t1, &t2)) return 0;
+    if (!clip_coord3d(h[3] + h[0], -d[3] - d[0], &t1, &t2)) return 0;
+    if (!clip_coord3d(h[3] - h[1], -d[3] + d[1], &t1, &t2)) return 0;
+    if (!clip_coord3d(h[3] + h[1], -d[3] - d[1], &t1, &t2)) return 0;
+    if (!clip_coord3d(h[3] - h[2], -d[3] + d[2], &t1, &t2)) return 0;
+    if (!clip_coord3d(h[3] + h[2], -d[3] - d[2], &t1, &t2)) return 0;
+    if (t2 < 1) for (int i = 0; i < 4; i++) v2->hom[i] = v1->hom[i] + d[i] * t2;
+    if (t1 > 0) for (int i = 0; i < 4; i++) v1->hom[i] = v1->hom[i] + d[i] * t1;
+    return 1;
+}
+
+/* Process_ProjectAndClipLine (lines.c:222-258). */
+static int project_line(const PFFstate* st, const PFFline* l, LVtx* v)
+{
+    for (int k = 0; k < 2; k++) {
+        const float* m = st->mvp;
+        const float* p = l->position[k];
+        for (int i = 0; i < 4; i++) v[k].hom[i] = m[i] * p[0] + m[4 + i] * p[1] + m[8 + i] * p[2] + m[12 + i] * p[3];
+        v[k].color = l->color[k];
+    }
+    if (v[0].hom[3] == 1.0f && v[1].hom[3] == 1.0f) {
+        line_to_screen(st, &v[0]);
+        line_to_screen(st, &v[1]);
+        return clip_line2d(st, &v[0], &v[1]);
+    }
+    if (!clip_line3d(&v[0], &v[1])) return 0;
+    for (int k = 0; k < 2; k++) {
+        const float invW = 1.0f / v[k].hom[3];
+        v[k].hom[0] *= invW;
+        v[k].hom[1] *= invW;
+    }
+    line_to_screen(st, &v[0]);
+    line_to_screen(st, &v[1]);
+    return 1;
+}
+
+/* Rasterize_Line_DEPTH / _NODEPTH (lines.c:260-445) of screen endpoints
+ * (s1, s2): fixed-point DDA over the longer axis, last pixel excluded;
+ * t = i / (end), depth z1 + t (z2 - z1), colour pfiColorLerpSmooth
+ * (color.h:29-37). */
+static void thin_line(const PFFframe* f, const PFFstate* st, const float* s1, const float* s2,
+                      float z1, float z2, uint32_t c1, uint32_t c2, int depth)
+{
+    const int32_t x1 = cvtt_i32(s1[0]), y1 = cvtt_i32(s1[1]), x2 = cvtt_i32(s2[0]), y2 = cvtt_i32(s2[1]);
+    int32_t shortLen = y2 - y1, longLen = x2 - x1;
+    int yLonger = 0;
+    if (abs(shortLen) > abs(longLen)) { int32_t t = shortLen; shortLen = longLen; longLen = t; yLonger = 1; }
+    const float invEndVal = 1.0f / (float)longLen;
+    const int32_t endVal = longLen;
+    int32_t sgnInc = 1;
+    if (longLen < 0) { longLen = -longLen; sgnInc = -1; }
+    const int32_t decInc = longLen == 0 ? 0 : (int32_t)((uint32_t)shortLen << 16) / longLen;
+    int32_t j = 0;
+    for (int32_t i = 0; i != endVal; i += sgnInc, j += decInc) {
+        const float t = (float)i * invEndVal;
+        const int32_t x = yLonger ? x1 + (j >> 16) : x1 + i;
+        const int32_t y = yLonger ? y1 + i : y1 + (j >> 16);
+        const float z = z1 + t * (z2 - z1);
+        /* pOffset = y * w + x (lines.c:315): an x past a side of the target
+         * lands in the neighbouring row; offsets outside the buffer are
+         * skipped (D9). */
+        const int64_t o = (int64_t)y * f->width + x;
+        if (o < 0 || o >= (int64_t)f->width * f->height) continue;
+        if (x < -(int32_t)f->width || x >= 2 * (int32_t)f->width) continue;         /* D9 */
+        const size_t off = (size_t)o;
+        if (depth && !depth_scalar(st->depthFunc, z, f->depth[off])) continue;
+        uint32_t col = 0;
+        for (int k = 0; k < 4; k++) {
+            const int32_t a = (int32_t)((c1 >> (8 * k)) & 255u), b = (int32_t)((c2 >> (8 * k)) & 255u);
+            col |= (uint32_t)f2u8((float)a + t * (float)(b - a)) << (8 * k);
+        }
+        f->color[off] = (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, col, f->color[off]) : col;
+        f->depth[off] = z;
+    }
+}
+
+/* One line: thin, or above width 1.5 the depth-tested centre line and
+ * copies shifted by 1..w pixels across the shorter axis, each side in turn
+ * (Rasterize_Line_THICK_*, lines.c:447-532; rsqrtf is 1 / sqrtf, pfm.h:34). */
+static void draw_line(const PFFframe* f, const PFFstate* st, const LVtx* v, float width)
+{
+    const int depth = (st->flags & PFF_DEPTH_TEST) != 0;
+    const float z1 = v[0].hom[2], z2 = v[1].hom[2];
+    if (!(width > 1.5f)) { thin_line(f, st, v[0].scr, v[1].scr, z1, z2, v[0].color, v[1].color, depth); return; }
+    const int32_t x1 = cvtt_i32(v[0].scr[0]), y1 = cvtt_i32(v[0].scr[1]);
+    const int32_t x2 = cvtt_i32(v[1].scr[0]), y2 = cvtt_i32(v[1].scr[1]);
+    const int32_t dx = x2 - x1, dy = y2 - y1;
+    thin_line(f, st, v[0].scr, v[1].scr, z1, z2, v[0].color, v[1].color, 1);
+    int axis = -1;
+    int32_t w = 0;
+    if (dx != 0 && abs(dy / dx) < 1) {
+        w = cvtt_i32((width - 1.0f) * (1.0f / sqrtf((float)(dx * dx + dy * dy))) * (float)abs(dx)) >> 1;
+        axis = 1;
+    } else if (dy != 0) {
+        w = cvtt_i32((width - 1.0f) * (1.0f / sqrtf((float)(dx * dx + dy * dy))) * (float)abs(dy)) >> 1;
+        axis = 0;
+    }
+    if (w > 16384) w = 16384;                                          /* D9 */
+    for (int32_t i = 1; axis >= 0 && i <= w; i++)
+        for (int side = -1; side <= 1; side += 2) {
+            float a[2] = { v[0].scr[0], v[0].scr[1] }, b[2] = { v[1].scr[0], v[1].scr[1] };
+            a[axis] += (float)(side * i);
+            b[axis] += (float)(side * i);
+            thin_line(f, st, a, b, z1, z2, v[0].color, v[1].color, depth);
+        }
+}
+
 static int render_frame(RCtx* rc);
 
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
@@ -1252,6 +1433,20 @@ static int render_frame(RCtx* rcp)
     for (uint32_t c = 0; c < f->numCmds; c++) {
         const PFFcmd* cmd = &f->cmds[c];
         if (cmd->kind == PFF_CMD_CLEAR) { clear(f, cmd); continue; }
+        if (cmd->kind == PFF_CMD_LINES) {
+            if (cmd->first > cmd->last || cmd->last > f->numLines) return -1;
+            int groupDead = 0;
+            for (uint32_t k = cmd->first; k < cmd->last; k++) {
+                const PFFline* l = &f->lines[k];
+                if (l->groupStart) groupDead = 0;
+                if (groupDead || l->state >= f->numStates) continue;
+                const PFFstate* st = &f->states[l->state];
+                LVtx v[2];
+                if (!project_line(st, l, v)) { groupDead = 1; continue; }   /* lines.c:93 */
+                draw_line(f, st, v, l->width);
+            }
+            continue;
+        }
         if (cmd->kind == PFF_CMD_POINTS) {
             if (cmd->first > cmd->last || cmd->last > f->numPoints) return -1;
             for (uint32_t k = cmd->first; k < cmd->last; k++) draw_point(f, &f->points[k]);

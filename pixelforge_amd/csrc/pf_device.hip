@@ -2602,6 +2602,250 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lines (lines.c:56-532).  k_line_setup runs one thread per group (a
+// PF_LINES pair or the edges of one polygon): projection, the 2D clipper
+// (Cohen-Sutherland on screen positions, BOTTOM mapped to the top edge and
+// TOP to the bottom one as in the reference, at most 64 steps) or the 3D
+// one (Liang-Barsky on clip coordinates), the division of x and y, the
+// screen mapping; a rejected edge ends its group.  k_lines gives each 16x16
+// tile to a workgroup whose threads walk the run's lines in order.  A line
+// is thin, or above width 1.5 a depth-tested centre line plus copies
+// shifted by 1..w across the shorter axis.  Each thin line is the
+// reference's fixed-point DDA: step k of the longer axis lands on the
+// shorter-axis coordinate start + ((k * decInc) >> 16), so a pixel knows in
+// O(1) which step, if any, covers it.  Pixels are addressed y * W + x as in
+// the reference (an x past a side wraps to the neighbouring row), so each
+// thread also tests the two wrapped positions of its offset and applies the
+// steps it owns in step order.
+// ---------------------------------------------------------------------------
+
+struct DLineScr {
+    float    s1[2], s2[2];        // screen end points after clipping
+    float    z1, z2;
+    uint32_t c1, c2;
+    uint32_t flags;               // PFF_DEPTH_TEST | PFF_BLEND; 0x80000000 inactive; 0x40000000 thick
+    uint16_t depthFunc, blendMode;
+    int32_t  w, axis;             // thick copies: 1..w each side, along axis (0 x, 1 y)
+    int32_t  bx0, by0, bx1, by1;  // pixel bounds of every copy
+};
+
+__device__ __forceinline__ void line_screen(const PFFstate& st, const float* h, float* sc) {
+    sc[0] = ((float)st.vpPos[0] + ((h[0] + 1.0f) * 0.5f) * (float)st.vpDim[0]) + 0.5f;
+    sc[1] = ((float)st.vpPos[1] + ((1.0f - h[1]) * 0.5f) * (float)st.vpDim[1]) + 0.5f;
+}
+
+__device__ __forceinline__ uint32_t line_code(const PFFstate& st, const float* sc) {
+    uint32_t c = 0u;
+    if (sc[0] < (float)st.vpMin[0]) c |= 1u;
+    if (sc[0] > (float)st.vpMax[0]) c |= 2u;
+    if (sc[1] < (float)st.vpMin[1]) c |= 8u;
+    if (sc[1] > (float)st.vpMax[1]) c |= 4u;
+    return c;
+}
+
+__device__ __forceinline__ bool clip_coord3d(float q, float p, float& t1, float& t2) {
+    if (fabsf(p) < 1e-5f) return !(q < -1e-5f);
+    const float r = __fdiv_rn(q, p);
+    if (p < 0.0f) {
+        if (r > t2) return false;
+        if (r > t1) t1 = r;
+    } else {
+        if (r < t1) return false;
+        if (r < t2) t2 = r;
+    }
+    return true;
+}
+
+// Process_ProjectAndClipLine (lines.c:140-258) of one line.
+__device__ bool project_line(const PFFstate& st, const PFFline& l, float* h1, float* h2, float* s1, float* s2,
+                             uint32_t& c1, uint32_t& c2) {
+    float h[2][4];
+    for (int v = 0; v < 2; v++)
+        for (int k = 0; k < 4; k++)
+            h[v][k] = st.mvp[k] * l.position[v][0] + st.mvp[4 + k] * l.position[v][1] + st.mvp[8 + k] * l.position[v][2]
+                    + st.mvp[12 + k] * l.position[v][3];
+    c1 = l.color[0];
+    c2 = l.color[1];
+    if (h[0][3] == 1.0f && h[1][3] == 1.0f) {
+        float a[2], b[2];
+        line_screen(st, h[0], a);
+        line_screen(st, h[1], b);
+        float m = 0.0f;
+        if (a[0] != b[0]) m = __fdiv_rn(b[1] - a[1], b[0] - a[0]);
+        bool accept = false;
+        for (int step = 0; step < 64; step++) {
+            uint32_t k0 = line_code(st, a), k1 = line_code(st, b);
+            if ((k0 | k1) == 0u) { accept = true; break; }
+            if (k0 & k1) break;
+            if (k0 == 0u) {
+                for (int q = 0; q < 2; q++) { float t = a[q]; a[q] = b[q]; b[q] = t; }
+                for (int q = 0; q < 4; q++) { float t = h[0][q]; h[0][q] = h[1][q]; h[1][q] = t; }
+                uint32_t t = c1; c1 = c2; c2 = t;
+                k0 = k1;
+            }
+            if (k0 & 1u) { a[1] += ((float)st.vpMin[0] - a[0]) * m; a[0] = (float)st.vpMin[0]; }
+            else if (k0 & 2u) { a[1] += ((float)st.vpMax[0] - a[0]) * m; a[0] = (float)st.vpMax[0]; }
+            else if (k0 & 4u) { if (m != 0.0f) a[0] += __fdiv_rn((float)st.vpMin[1] - a[1], m); a[1] = (float)st.vpMin[1]; }
+            else if (k0 & 8u) { if (m != 0.0f) a[0] += __fdiv_rn((float)st.vpMax[1] - a[1], m); a[1] = (float)st.vpMax[1]; }
+        }
+        if (!accept) return false;
+        for (int q = 0; q < 4; q++) { h1[q] = h[0][q]; h2[q] = h[1][q]; }
+        s1[0] = a[0]; s1[1] = a[1]; s2[0] = b[0]; s2[1] = b[1];
+        return true;
+    }
+    float t1 = 0.0f, t2 = 1.0f, d[4];
+    for (int q = 0; q < 4; q++) d[q] = h[1][q] - h[0][q];
+    const float* g = h[0];
+    if (!clip_coord3d(g[3] - g[0], -d[3] + d[0], t1, t2)) return false;
+    if (!clip_coord3d(g[3] + g[0], -d[3] - d[0], t1, t2)) return false;
+    if (!clip_coord3d(g[3] - g[1], -d[3] + d[1], t1, t2)) return false;
+    if (!clip_coord3d(g[3] + g[1], -d[3] - d[1], t1, t2)) return false;
+    if (!clip_coord3d(g[3] - g[2], -d[3] + d[2], t1, t2)) return false;
+    if (!clip_coord3d(g[3] + g[2], -d[3] - d[2], t1, t2)) return false;
+    if (t2 < 1.0f) for (int q = 0; q < 4; q++) h[1][q] = h[0][q] + d[q] * t2;
+    if (t1 > 0.0f) for (int q = 0; q < 4; q++) h[0][q] = h[0][q] + d[q] * t1;
+    for (int v = 0; v < 2; v++) {
+        const float invW = __fdiv_rn(1.0f, h[v][3]);
+        h[v][0] *= invW;
+        h[v][1] *= invW;
+    }
+    line_screen(st, h[0], s1);
+    line_screen(st, h[1], s2);
+    for (int q = 0; q < 4; q++) { h1[q] = h[0][q]; h2[q] = h[1][q]; }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_line_setup(const PFFline* __restrict__ lines, uint32_t n,
+                                                    const PFFstate* __restrict__ states, DLineScr* __restrict__ out) {
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= n || !lines[g].groupStart) return;
+    bool dead = false;
+    for (uint32_t i = g; i < n && (i == g || !lines[i].groupStart); i++) {
+        const PFFline l = lines[i];
+        const PFFstate& st = states[l.state];
+        DLineScr o;
+        memset(&o, 0, sizeof o);
+        o.flags = 0x80000000u;
+        float h1[4], h2[4];
+        if (!dead && project_line(st, l, h1, h2, o.s1, o.s2, o.c1, o.c2)) {
+            o.z1 = h1[2];
+            o.z2 = h2[2];
+            o.flags = st.flags & (PFF_DEPTH_TEST | PFF_BLEND);
+            o.depthFunc = (uint16_t)st.depthFunc;
+            o.blendMode = (uint16_t)st.blendMode;
+            const int32_t x1 = cvtt_i32(o.s1[0]), y1 = cvtt_i32(o.s1[1]), x2 = cvtt_i32(o.s2[0]), y2 = cvtt_i32(o.s2[1]);
+            o.w = 0;
+            o.axis = 0;
+            if (l.width > 1.5f) {
+                o.flags |= 0x40000000u;
+                const int32_t dx = x2 - x1, dy = y2 - y1;
+                if (dx != 0 && abs(dy / dx) < 1) {
+                    o.w = cvtt_i32((l.width - 1.0f) * __fdiv_rn(1.0f, __fsqrt_rn((float)(dx * dx + dy * dy))) * (float)abs(dx)) >> 1;
+                    o.axis = 1;
+                } else if (dy != 0) {
+                    o.w = cvtt_i32((l.width - 1.0f) * __fdiv_rn(1.0f, __fsqrt_rn((float)(dx * dx + dy * dy))) * (float)abs(dy)) >> 1;
+                    o.axis = 0;
+                }
+                if (o.w < 0) o.w = 0;
+                if (o.w > 16384) o.w = 16384;                   // copies past any target draw nothing
+            }
+            const int32_t ex = o.axis == 0 ? o.w + 1 : 1, ey = o.axis == 1 ? o.w + 1 : 1;
+            o.bx0 = min(x1, x2) - ex; o.bx1 = max(x1, x2) + ex;
+            o.by0 = min(y1, y2) - ey; o.by1 = max(y1, y2) + ey;
+        } else {
+            dead = true;                                        // lines.c:93
+        }
+        out[i] = o;
+    }
+}
+
+struct DLnOp {
+    uint32_t* color;
+    float*    depth;
+    uint32_t  W, H, tilesX, tilesY;
+    uint32_t  first, last;        // job line indices
+};
+
+// Step of a thin line (ints X1..Y2) covering line pixel (x, y), or -1.
+__device__ __forceinline__ int32_t line_step(int32_t X1, int32_t Y1, int32_t X2, int32_t Y2, int32_t x, int32_t y,
+                                             int32_t& iOut, float& invEnd) {
+    int32_t shortLen = Y2 - Y1, longLen = X2 - X1;
+    const bool yl = abs(shortLen) > abs(longLen);
+    if (yl) { const int32_t t = shortLen; shortLen = longLen; longLen = t; }
+    invEnd = __fdiv_rn(1.0f, (float)longLen);
+    const int32_t endVal = longLen;
+    int32_t sgn = 1;
+    if (longLen < 0) { longLen = -longLen; sgn = -1; }
+    const int32_t decInc = longLen == 0 ? 0 : (int32_t)((uint32_t)shortLen << 16) / longLen;
+    const int32_t i = yl ? y - Y1 : x - X1;
+    const int32_t k = i * sgn;
+    if (k < 0 || k >= (endVal < 0 ? -endVal : endVal)) return -1;
+    const int32_t minor = (yl ? X1 : Y1) + ((k * decInc) >> 16);
+    if ((yl ? x : y) != minor) return -1;
+    iOut = i;
+    return k;
+}
+
+__global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, const DLineScr* __restrict__ scr) {
+    const DLnOp o = ops[blockIdx.y];
+    if (blockIdx.x >= o.tilesX * o.tilesY) return;
+    const int32_t W = (int32_t)o.W;
+    const int32_t x0 = (int32_t)(blockIdx.x % o.tilesX) * 16, y0 = (int32_t)(blockIdx.x / o.tilesX) * 16;
+    const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
+    const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
+    const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
+    uint32_t c = inside ? o.color[off] : 0u;
+    float d = inside ? o.depth[off] : 0.0f;
+    bool dirty = false;
+    // Line positions whose offset y * W + x is this pixel's.
+    const int32_t cxs[3] = { px, px + W, px - W }, cys[3] = { py, py - 1, py + 1 };
+    for (uint32_t li = o.first; li < o.last; li++) {
+        const DLineScr s = scr[li];
+        if (s.flags & 0x80000000u) continue;
+        const bool wraps = s.bx0 < 0 || s.bx1 >= W;
+        if (s.by1 < y0 - 1 || s.by0 > y0 + 16) continue;
+        if (!wraps && (s.bx1 < x0 || s.bx0 > x0 + 15)) continue;
+        if (!inside) continue;
+        const bool thick = (s.flags & 0x40000000u) != 0;
+        const int32_t copies = thick ? 1 + 2 * s.w : 1;
+        for (int32_t cp = 0; cp < copies; cp++) {
+            // Copy 0 is the centre line; then -1, +1, -2, +2, ... along axis.
+            const int32_t sh = cp == 0 ? 0 : ((cp + 1) >> 1) * ((cp & 1) ? -1 : 1);
+            float a0 = s.s1[0], a1 = s.s1[1], b0 = s.s2[0], b1 = s.s2[1];
+            if (s.axis == 0) { a0 += (float)sh; b0 += (float)sh; } else { a1 += (float)sh; b1 += (float)sh; }
+            const int32_t X1 = cvtt_i32(a0), Y1 = cvtt_i32(a1), X2 = cvtt_i32(b0), Y2 = cvtt_i32(b1);
+            const bool depthOn = (cp == 0 && thick) || (s.flags & PFF_DEPTH_TEST);
+            int32_t ks[3], is[3];
+            float invEnd = 0.0f;
+            for (int q = 0; q < 3; q++) ks[q] = line_step(X1, Y1, X2, Y2, cxs[q], cys[q], is[q], invEnd);
+            for (int pass = 0; pass < 3; pass++) {
+                int best = -1;
+                for (int q = 0; q < 3; q++)
+                    if (ks[q] >= 0 && (best < 0 || ks[q] < ks[best])) best = q;
+                if (best < 0) break;
+                const float t = (float)is[best] * invEnd;
+                ks[best] = -1;
+                const float z = s.z1 + t * (s.z2 - s.z1);
+                if (depthOn && !depth_scalar(s.depthFunc, z, d)) continue;
+                uint32_t col = 0u;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int32_t ca = (int32_t)((s.c1 >> (8 * q)) & 255u), cb = (int32_t)((s.c2 >> (8 * q)) & 255u);
+                    col |= (uint32_t)f2u8((float)ca + t * (float)(cb - ca)) << (8 * q);
+                }
+                c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, c) : col;
+                d = z;
+                dirty = true;
+            }
+        }
+    }
+    if (dirty) {
+        o.color[off] = c;
+        o.depth[off] = d;
+    }
+}
+
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -2665,6 +2909,9 @@ struct PassInfo {
     std::vector<DPtOp> pts;    // runs of points, after the pixel calls
     uint32_t ptTiles = 0;
     DevBuf   dPts;
+    std::vector<DLnOp> lns;    // runs of lines, after the points
+    uint32_t lnTiles = 0;
+    DevBuf   dLns;
 };
 
 }  // namespace
@@ -2685,6 +2932,7 @@ struct PFDjob {
     std::vector<DevBuf> texData;
     std::vector<DevBuf> pixData;   // decoded pfDrawPixels images
     DevBuf   points, pointScr;     // PFFpoint of all frames, their projections
+    DevBuf   lines, lineScr;       // PFFline of all frames, their clipped screen forms
     uint32_t numIn = 0;
     bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
     std::vector<PassInfo> passes;
@@ -2894,6 +3142,9 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (!P.pts.empty())
         hipLaunchKernelGGL(k_points, dim3(P.ptTiles, (uint32_t)P.pts.size()), dim3(256), 0, J->stream,
                            (const DPtOp*)P.dPts.p, (const DPointScr*)J->pointScr.p);
+    if (!P.lns.empty())
+        hipLaunchKernelGGL(k_lines, dim3(P.lnTiles, (uint32_t)P.lns.size()), dim3(256), 0, J->stream,
+                           (const DLnOp*)P.dLns.p, (const DLineScr*)J->lineScr.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
     P.sized = true;
@@ -3029,7 +3280,8 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     std::vector<PFFstate> hstates(totalStates ? totalStates : 1);
     std::vector<DTex> htex;
     std::vector<PFFpoint> hpts;
-    uint32_t triBase = 0, stateBase = 0, ptBase = 0;
+    std::vector<PFFline> hlns;
+    uint32_t triBase = 0, stateBase = 0, ptBase = 0, lnBase = 0;
     J->src.assign(frames, frames + count);
     J->leadClear.resize(count);
     for (uint32_t f = 0; f < count; f++)
@@ -3123,6 +3375,13 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             q.state += stateBase;
             hpts.push_back(q);
         }
+        for (uint32_t k = 0; k < fr.numLines; k++) {
+            PFFline q = fr.lines[k];
+            if (q.state >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad line state"); pfdJobDestroy(J); return PFD_E_INVALID; }
+            q.state += stateBase;
+            if (k == 0) q.groupStart = 1u;             // groups never span frames
+            hlns.push_back(q);
+        }
         // Commands -> segments; the k-th segment of a frame runs in pass k.
         uint32_t segNo = 0;
         DSeg cur;
@@ -3147,6 +3406,24 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 cur.clearColor = cmd.clearColor;
                 cur.clearDepth = cmd.clearDepth;
                 open = true;
+            } else if (cmd.kind == PFF_CMD_LINES) {
+                // A pass of its own; the run's lines are applied in order.
+                if (!fr.lines || cmd.first > cmd.last || cmd.last > fr.numLines) {
+                    snprintf(g_err, sizeof g_err, "bad lines command");
+                    pfdJobDestroy(J);
+                    return PFD_E_INVALID;
+                }
+                DLnOp d;
+                memset(&d, 0, sizeof d);
+                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
+                d.first = lnBase + cmd.first; d.last = lnBase + cmd.last;
+                flush_seg();
+                cur.triFirst = cur.triLast = triBase;
+                open = true;
+                if (J->passes.size() <= segNo) J->passes.emplace_back();
+                J->passes[segNo].lns.push_back(d);
+                flush_seg();
             } else if (cmd.kind == PFF_CMD_POINTS) {
                 // A pass of its own; the run's points are applied in order.
                 if (!fr.points || cmd.first > cmd.last || cmd.last > fr.numPoints) {
@@ -3253,6 +3530,7 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         triBase += (uint32_t)nt;
         stateBase += fr.numStates;
         ptBase += fr.numPoints;
+        lnBase += fr.numLines;
     }
     // Device copies of the inputs.
     auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
@@ -3286,6 +3564,20 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
         if (e != hipSuccess) { set_error("point setup", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
     }
+    if (!hlns.empty()) {
+        // Lines are projected and clipped once; every replay reads them.
+        if ((rc = up(J->lines, hlns.data(), hlns.size() * sizeof(PFFline)))
+            || (rc = ensure(J->lineScr, hlns.size() * sizeof(DLineScr)))) {
+            pfdJobDestroy(J);
+            return rc;
+        }
+        const uint32_t nl = (uint32_t)hlns.size();
+        hipLaunchKernelGGL(k_line_setup, dim3((nl + 255) / 256), dim3(256), 0, J->stream, (const PFFline*)J->lines.p,
+                           nl, (const PFFstate*)J->states.p, (DLineScr*)J->lineScr.p);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
+        if (e != hipSuccess) { set_error("line setup", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+    }
     // Per-pass tables.
     for (auto& P : J->passes) {
         uint32_t acc = 0;
@@ -3318,6 +3610,12 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             for (const DPtOp& d : P.pts) maxTiles = std::max(maxTiles, d.tilesX * d.tilesY);
             P.ptTiles = maxTiles;
             if ((rc = up(P.dPts, P.pts.data(), P.pts.size() * sizeof(DPtOp)))) { pfdJobDestroy(J); return rc; }
+        }
+        if (!P.lns.empty()) {
+            uint32_t maxTiles = 1;
+            for (const DLnOp& d : P.lns) maxTiles = std::max(maxTiles, d.tilesX * d.tilesY);
+            P.lnTiles = maxTiles;
+            if ((rc = up(P.dLns, P.lns.data(), P.lns.size() * sizeof(DLnOp)))) { pfdJobDestroy(J); return rc; }
         }
         if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
             || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
@@ -3532,7 +3830,9 @@ void pfdJobDestroy(PFDjob* J) {
     for (auto& b : J->pixData) release(b);
     release(J->points);
     release(J->pointScr);
-    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); release(P.dPts); }
+    release(J->lines);
+    release(J->lineScr);
+    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); release(P.dPts); release(P.dLns); }
     if (J->side) (void)hipStreamSynchronize(J->side);
     if (J->fork) (void)hipEventDestroy(J->fork);
     if (J->join) (void)hipEventDestroy(J->join);
@@ -3591,10 +3891,11 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     // One pass: at most one clear, before every draw.
     for (uint32_t c = 0; c < frame->numCmds; c++)
         if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS
-            || frame->cmds[c].kind == PFF_CMD_FOG || frame->cmds[c].kind == PFF_CMD_POINTS) {
+            || frame->cmds[c].kind == PFF_CMD_FOG || frame->cmds[c].kind == PFF_CMD_POINTS
+            || frame->cmds[c].kind == PFF_CMD_LINES) {
             snprintf(g_err, sizeof g_err, "differentiable frame: %s",
                      frame->cmds[c].kind == PFF_CMD_CLEAR ? "a clear after the first command"
-                                                          : "points, pfDrawPixels and pfFogProcess are not differentiable here");
+                                                          : "points, lines, pfDrawPixels and pfFogProcess are not differentiable here");
             return PFD_E_INVALID;
         }
     for (uint32_t i = 0; i < frame->numStates; i++) {

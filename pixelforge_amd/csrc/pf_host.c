@@ -106,6 +106,7 @@ typedef struct {                       /* the frame being recorded */
     PFFcmd*    cmds;     size_t ncmd, capcmd;
     PFFpixels* pix;      size_t npix, cappix;   /* pfDrawPixels calls; images owned */
     PFFpoint*  pts;      size_t npts, cappts;
+    PFFline*   lns;      size_t nlns, caplns;
     int        anyTex, anyLight, anyVid;
 } Rec;
 
@@ -309,6 +310,7 @@ static void rec_reset(Rec* r)
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
     r->npix = 0;
     r->npts = 0;
+    r->nlns = 0;
     r->ntri = r->nstates = r->ntex = r->ncmd = 0;
     r->anyTex = r->anyLight = r->anyVid = 0;
     r->targetColor = NULL;
@@ -320,6 +322,7 @@ static void rec_free(Rec* r)
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
     free(r->pix);
     free(r->pts);
+    free(r->lns);
     free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info); free(r->vid);
     free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
     memset(r, 0, sizeof *r);
@@ -358,6 +361,8 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->numPixelOps = (uint32_t)r->npix;
     f->points = r->pts;
     f->numPoints = (uint32_t)r->npts;
+    f->lines = r->lns;
+    f->numLines = (uint32_t)r->nlns;
 }
 
 /* Executes the recorded frame, if any, and starts an empty one. */
@@ -598,6 +603,43 @@ static void emit_point(Ctx* c, const Vert* v)
     push_cmd(c, &cmd);
 }
 
+/* Appends one line (lines.c:56-105); consecutive lines share one
+ * PFF_CMD_LINES command. */
+static void emit_line(Ctx* c, const Vert* a, const Vert* b, int groupStart)
+{
+    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); return; }
+    int si = current_state(c);
+    if (si < 0) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->lns, &r->caplns, r->nlns + 1, sizeof(PFFline))) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    PFFline* l = &r->lns[r->nlns];
+    memcpy(l->position[0], a->position, 16);
+    memcpy(l->position[1], b->position, 16);
+    l->color[0] = pack(a->color);
+    l->color[1] = pack(b->color);
+    l->state = (uint32_t)si;
+    l->width = c->lineWidth;
+    l->groupStart = groupStart ? 1u : 0u;
+    l->pad = 0;
+    const uint32_t k = (uint32_t)r->nlns++;
+    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_LINES && r->cmds[r->ncmd - 1].last == k) {
+        r->cmds[r->ncmd - 1].last = k + 1;
+        return;
+    }
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_LINES;
+    cmd.first = k;
+    cmd.last = k + 1;
+    push_cmd(c, &cmd);
+}
+
+/* pfiProcessRasterize_POLY_LINES (lines.c:79-105): the n edges, one group. */
+static void emit_poly_lines(Ctx* c, int n)
+{
+    for (int i = 0; i < n; i++) emit_line(c, &c->vb[i], &c->vb[(i + 1) % n], i == 0);
+}
+
 /* pfiProcessRasterize_POLY_POINTS (points.c:45-57): the n corners. */
 static void emit_poly_points(Ctx* c, int n)
 {
@@ -618,7 +660,7 @@ static void process_primitive(Ctx* c)
             if (face != PF_FRONT_AND_BACK && face != f) continue;
             if (c->polyMode[f] == PF_FILL) emit_tri(c, &c->vb[0], &c->vb[1], &c->vb[2], face_pass(f));
             else if (c->polyMode[f] == PF_POINT) emit_poly_points(c, 3);
-            else set_err(PF_INVALID_OPERATION);                          /* PF_LINE */
+            else emit_poly_lines(c, 3);
         }
         break;
     case PF_TRIANGLE_FAN:
@@ -638,14 +680,16 @@ static void process_primitive(Ctx* c)
             if (face != PF_FRONT_AND_BACK && face != f) continue;
             if (c->polyMode[f] == PF_FILL) emit_fan(c, face_pass(f), 2);
             else if (c->polyMode[f] == PF_POINT) emit_poly_points(c, 4);
-            else set_err(PF_INVALID_OPERATION);                          /* PF_LINE */
+            else emit_poly_lines(c, 4);
         }
         break;
     case PF_POINTS:                          /* ictx.c:98-100 */
         emit_point(c, &c->vb[0]);
         break;
-    default:                                 /* PF_LINES */
-        set_err(PF_INVALID_OPERATION);
+    case PF_LINES:                           /* ictx.c:101-103 */
+        emit_line(c, &c->vb[0], &c->vb[1], 1);
+        break;
+    default:
         break;
     }
 }

@@ -93,6 +93,10 @@ typedef struct {
                               * halves of the triangles (random kinds)          */
     float    pointSize;      /* >0: pfPointSize                                   */
     int32_t  polyPoint;      /* bit 0: front faces PF_POINT, bit 1: back faces   */
+    int32_t  lines;          /* >0: that many PF_LINES pairs between the two halves
+                              * of the triangles (random kinds)                  */
+    float    lineWidth;      /* >0: pfLineWidth                                   */
+    int32_t  polyLine;       /* bit 0: front faces PF_LINE, bit 1: back faces    */
 } PFSparams;
 
 typedef struct {
@@ -143,6 +147,7 @@ typedef struct {
     void (*DrawElements)(PFdrawmode, PFsizei, PFdatatype, const void*);
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
     void (*PointSize)(PFfloat);
+    void (*LineWidth)(PFfloat);
     void (*PolygonMode)(PFface, PFpolygonmode);
     void (*Fogi)(PFfogparam, PFint);
     void (*Fogf)(PFfogparam, PFfloat);
@@ -182,7 +187,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(DrawElements); BIND(DrawArrays);
     BIND(RasterPos3f); BIND(PixelZoom); BIND(DrawPixels);
     BIND(Fogi); BIND(Fogf); BIND(Fogiv); BIND(Fogfv); BIND(FogProcess);
-    BIND(PointSize); BIND(PolygonMode);
+    BIND(PointSize); BIND(PolygonMode); BIND(LineWidth);
     BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
@@ -306,6 +311,18 @@ static void draw_points(const PFSapi* a, Rng* r, const PFSparams* p)
     a->End();
 }
 
+/* PF_LINES: random end points in [-1.3, 1.3]^3 of the current matrices (some
+ * leave the viewport and are clipped), random colours. */
+static void draw_lines(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    a->Begin(PF_LINES);
+    for (int i = 0; i < 2 * p->lines; i++) {
+        a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
+        a->Vertex3f(rrange(r, -1.3f, 1.3f), rrange(r, -1.3f, 1.3f), rrange(r, -1.3f, 1.3f));
+    }
+    a->End();
+}
+
 static void apply_fog(const PFSapi* a, const PFSparams* p)
 {
     if (p->fog == 4) {
@@ -366,6 +383,9 @@ static void common_state(const PFSapi* a, const PFSparams* p)
     if (p->pointSize > 0) a->PointSize(p->pointSize);
     if (p->polyPoint & 1) a->PolygonMode(PF_FRONT, PF_POINT);
     if (p->polyPoint & 2) a->PolygonMode(PF_BACK, PF_POINT);
+    if (p->lineWidth > 0) a->LineWidth(p->lineWidth);
+    if (p->polyLine & 1) a->PolygonMode(PF_FRONT, PF_LINE);
+    if (p->polyLine & 2) a->PolygonMode(PF_BACK, PF_LINE);
     if (p->cull < 0) a->Disable(PF_CULL_FACE);
     else { a->Enable(PF_CULL_FACE); a->CullFace((PFface)p->cull); }
     if (p->depthTest) { a->Enable(PF_DEPTH_TEST); a->DepthFunc((PFdepthmode)p->depthFunc); }
@@ -778,13 +798,15 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
     if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
     if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
-    if ((p->pixelOps > 0 || p->fog || p->points > 0) && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
+    if ((p->pixelOps > 0 || p->fog || p->points > 0 || p->lines > 0)
+        && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
         PFSparams half = *p;
         half.ntris = p->ntris / 2;
         if (p->kind == PFS_RANDOM_3D) setup_lights(a, r, p, 0.0f, 0.0f, -4.0f, 3.0f);
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);
         draw_pixel_ops(a, r, p);
         if (p->points > 0) draw_points(a, r, p);
+        if (p->lines > 0) draw_lines(a, r, p);
         if (p->fog) apply_fog(a, p);
         half.ntris = p->ntris - half.ntris;
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);

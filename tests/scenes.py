@@ -168,6 +168,26 @@ POINT_SCENES = [
                                    filter=pf.PF_NEAREST, seed=506, polyPoint=3, pointSize=3.0, cull=-1)),
 ]
 
+# Lines: PF_LINES between two halves of the triangles (thin, thick, depth
+# test off, blending; 2D and 3D clipping), and the PF_LINE polygon mode for
+# triangles and quads.
+LINE_SCENES = [
+    ("lines_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=600, lines=80,
+                       depthFunc=pf.PF_LEQUAL)),
+    ("lines_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=601, lines=80)),
+    ("lines_thick_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=602, lines=40,
+                             lineWidth=4.5)),
+    ("lines_thick_nodepth", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=603, lines=40,
+                                 lineWidth=3.0, depthTest=0)),
+    ("lines_blend_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=604, lines=60, lineWidth=2.5,
+                            blend=1, blendMode=5)),
+    ("poly_line_both", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=605, polyLine=3)),
+    ("poly_line_back_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=60, seed=606, polyLine=2,
+                                lineWidth=2.0)),
+    ("poly_line_quads_list", dict(kind=pf.LIST, width=W, height=H, texture=1, texW=33, texH=33,
+                                  filter=pf.PF_NEAREST, seed=607, polyLine=3, cull=-1)),
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -199,7 +219,7 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

@@ -20,9 +20,10 @@ NAMES = [n for n, _ in scenes.GOLDEN_SCENES]
 
 
 def _points_only(kw):
-    """Scenes whose every rendered triangle face is in PF_POINT mode (no
-    triangle fragments): both faces, or the one face culling leaves."""
-    pp, cull = kw.get("polyPoint", 0), kw.get("cull", -1)
+    """Scenes whose every rendered triangle face is in PF_POINT or PF_LINE
+    mode (no triangle fragments): both faces, or the one face culling
+    leaves."""
+    pp, cull = kw.get("polyPoint", 0) | kw.get("polyLine", 0), kw.get("cull", -1)
     return pp == 3 or (pp == 2 and cull == pf.PF_FRONT) or (pp == 1 and cull == pf.PF_BACK)
 
 
