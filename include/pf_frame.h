@@ -145,6 +145,10 @@ typedef struct {
     uint32_t pad;
 } PFFpoint;
 
+/* PFFpixels.flags: a pfRect* fill (context.c:1938-1978) -- the 1x1 image
+ * holds the current colour, both steps are 0, no depth is written. */
+#define PFF_PIXELS_RECT  0x80000000u
+
 /* One pfDrawPixels call (context.c:1987-2080), with everything the call
  * computes from the context before its pixel loop; or, for PFF_CMD_FOG,
  * one pfFogProcess call (context.c:2275-2343) and the fog state it reads
@@ -158,7 +162,7 @@ typedef struct {
     int32_t     xMin, yMin, xMax, yMax;    /* destination rectangle, inclusive */
     float       zPos;          /* depth written and tested                     */
     float       invXLen, invYLen;          /* 1 / (width * zoom), 1 / (height * zoom) */
-    uint32_t    flags;         /* PFF_DEPTH_TEST, PFF_BLEND                    */
+    uint32_t    flags;         /* PFF_DEPTH_TEST, PFF_BLEND, PFF_PIXELS_RECT   */
     uint32_t    depthFunc;     /* PFdepthmode */
     uint32_t    blendMode;     /* PFblendmode */
     uint32_t    fogMode;       /* PFfogmode (any value; unknown ones give t = 0) */

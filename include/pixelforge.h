@@ -18,8 +18,9 @@
  *   - Only RGBA / UNSIGNED_BYTE render targets are accepted.  Textures may
  *     use every (format, type) pair that has a texel getter in the reference
  *     (pixel.h:3050-3095); pairs without one are PF_INVALID_ENUM.
- *   - Rects are not part of this library (they set PF_INVALID_OPERATION).
- *     Points and lines are drawn on the GPU in order with the triangles.  pfFogProcess runs on the GPU in order
+ *   - Points, lines, rects, pfDrawPixels and pfFogProcess are drawn on the
+ *     GPU in order with the triangles; pfPostProcess runs the caller's
+ *     function on the host after the frame.  pfFogProcess runs on the GPU in order
  *     with the draws; pfPostProcess runs the caller's function on the host
  *     after the frame has been drawn.
  *
@@ -423,11 +424,11 @@ PF_API void*     pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFs
                                     PFpixelformat* format, PFdatatype* type); /* pixelforge.h:2085 */
 
 /* ------------------------------------------------------------------------ */
-/* Outside the GPU triangle path.  Rectangles keep their signatures so that  */
-/* callers link and set PF_INVALID_OPERATION without drawing; pfLineWidth    */
-/* and pfPointSize apply to lines and points.  pfDrawPixels (with the raster position and zoom) and            */
-/* pfFogProcess are recorded and run on the GPU in order with the frame's    */
-/* triangles; pfPostProcess runs on the host after the frame.                */
+/* Beside the triangle path: pfLineWidth and pfPointSize apply to lines and  */
+/* points; rectangles fill with the current colour.  pfDrawPixels (with the  */
+/* raster position and zoom) and pfFogProcess are recorded and run on the    */
+/* GPU in order with the frame's triangles; pfPostProcess runs on the host   */
+/* after the frame.                                                          */
 /* ------------------------------------------------------------------------ */
 
 PF_API void pfLineWidth(PFfloat width);                                    /* pixelforge.h:823 */

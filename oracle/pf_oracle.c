@@ -1156,7 +1156,7 @@ static void draw_pixels(const PFFframe* f, const PFFpixels* o)
             const float u = (float)(x - o->xScreen) * o->invXLen;
             uint32_t src = ySrc + f2u32_x86(u * (float)(o->width - 1u));
             if (src >= n) src = n - 1u;                                                          /* D8 */
-            f->depth[dst] = o->zPos;
+            if (!(o->flags & PFF_PIXELS_RECT)) f->depth[dst] = o->zPos;    /* pfRect writes no depth */
             const uint32_t c = scalar_texel(img, o->source, src);
             f->color[dst] = (o->flags & PFF_BLEND) ? blend_scalar(o->blendMode, c, f->color[dst]) : c;
         }

@@ -2497,7 +2497,7 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
         const float u = (float)(x - o.xScreen) * o.invXLen;
         uint32_t src = f2u32_x86(v * (float)(o.h - 1u)) * o.w + f2u32_x86(u * (float)(o.w - 1u));
         if (src >= n) src = n - 1u;
-        o.depth[dst] = o.zPos;
+        if (!(o.flags & PFF_PIXELS_RECT)) o.depth[dst] = o.zPos;      // pfRect writes no depth
         const uint32_t c = o.texels[src];
         o.color[dst] = (o.flags & PFF_BLEND) ? blend_scalar(o.blendMode, c, o.color[dst]) : c;
     }

@@ -1787,11 +1787,7 @@ void pfRasterPos4i(PFint x, PFint y, PFint z, PFint w) { raster_pos((float)x, (f
 void pfRasterPos4f(PFfloat x, PFfloat y, PFfloat z, PFfloat w) { raster_pos(x, y, z, w); }
 void pfRasterPos4fv(const PFfloat* v) { raster_pos(v[0], v[1], v[2], v[3]); }
 
-static void not_in_path(void) { if (G) G->err = PF_INVALID_OPERATION; }
-void pfRects(PFshort x1, PFshort y1, PFshort x2, PFshort y2) { (void)x1; (void)y1; (void)x2; (void)y2; not_in_path(); }
-void pfRectsv(const PFshort* v1, const PFshort* v2) { (void)v1; (void)v2; not_in_path(); }
-void pfRectf(PFfloat x1, PFfloat y1, PFfloat x2, PFfloat y2) { (void)x1; (void)y1; (void)x2; (void)y2; not_in_path(); }
-void pfRectfv(const PFfloat* v1, const PFfloat* v2) { (void)v1; (void)v2; not_in_path(); }
+
 
 /* (PFint)(float) as gcc emits it on x86-64 (cvttss2si): NaN and values out
  * of range give INT32_MIN. */
@@ -1863,6 +1859,63 @@ void pfDrawPixels(PFsizei width, PFsizei height, PFpixelformat format, PFdatatyp
     r->npix++;
     push_cmd(c, &cmd);
 }
+/* pfRectf (context.c:1938-1978): the corners through the MVP (no division),
+ * to screen, sorted and clamped to the viewport; recorded as a pixel
+ * command whose 1x1 image is the current colour, with no depth test, depth
+ * write or blending, so it fills in order with the draws. */
+void pfRectf(PFfloat x1, PFfloat y1, PFfloat x2, PFfloat y2)
+{
+    Ctx* c = G;
+    if (!c) return;
+    if (!ensure_target(c)) { c->err = PF_INVALID_OPERATION; return; }
+    update_matrices(c, 0);
+    const float* m = c->matMVP;
+    const float in[2][2] = { { x1, y1 }, { x2, y2 } };
+    int32_t ix[2], iy[2];
+    for (int k = 0; k < 2; k++) {
+        const float v[4] = { in[k][0], in[k][1], 0.0f, 1.0f };
+        float t[2];
+        for (int i = 0; i < 2; i++) t[i] = m[i] * v[0] + m[4 + i] * v[1] + m[8 + i] * v[2] + m[12 + i] * v[3];
+        ix[k] = f2i_x86((float)c->vpPos[0] + (t[0] + 1.0f) * 0.5f * (float)c->vpDim[0]);
+        iy[k] = f2i_x86((float)c->vpPos[1] + (1.0f - t[1]) * 0.5f * (float)c->vpDim[1]);
+    }
+    if (ix[1] < ix[0]) { int32_t t = ix[0]; ix[0] = ix[1]; ix[1] = t; }
+    if (iy[1] < iy[0]) { int32_t t = iy[0]; iy[0] = iy[1]; iy[1] = t; }
+    for (int k = 0; k < 2; k++) {
+        ix[k] = ix[k] < c->vpMin[0] ? c->vpMin[0] : (ix[k] > c->vpMax[0] ? c->vpMax[0] : ix[k]);
+        iy[k] = iy[k] < c->vpMin[1] ? c->vpMin[1] : (iy[k] > c->vpMax[1] ? c->vpMax[1] : iy[k]);
+    }
+    uint32_t* px = (uint32_t*)malloc(8);
+    if (!px) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
+    px[0] = pack(c->curColor);
+    px[1] = 0u;
+    PFFpixels op;
+    memset(&op, 0, sizeof op);
+    op.pixels = px;
+    op.width = op.height = 1;
+    op.source = PFF_SOURCE(PF_RGBA, PF_UNSIGNED_BYTE);
+    op.xScreen = ix[0]; op.yScreen = iy[0];
+    op.xMin = ix[0]; op.yMin = iy[0]; op.xMax = ix[1]; op.yMax = iy[1];
+    op.flags = PFF_PIXELS_RECT;
+    Rec* r = &c->rec;
+    if (!grow((void**)&r->pix, &r->cappix, r->npix + 1, sizeof(PFFpixels))) {
+        free(px);
+        c->err = PF_ERROR_OUT_OF_MEMORY;
+        return;
+    }
+    r->pix[r->npix] = op;
+    PFFcmd cmd;
+    memset(&cmd, 0, sizeof cmd);
+    cmd.kind = PFF_CMD_PIXELS;
+    cmd.first = (uint32_t)r->npix;
+    r->npix++;
+    push_cmd(c, &cmd);
+}
+
+void pfRects(PFshort x1, PFshort y1, PFshort x2, PFshort y2) { pfRectf((float)x1, (float)y1, (float)x2, (float)y2); }
+void pfRectsv(const PFshort* v1, const PFshort* v2) { pfRectf((float)v1[0], (float)v1[1], (float)v2[0], (float)v2[1]); }
+void pfRectfv(const PFfloat* v1, const PFfloat* v2) { pfRectf(v1[0], v1[1], v2[0], v2[1]); }
+
 /* Fog state setters (context.c:2161-2273), quirks kept: every DENSITY
  * branch stores into the mode (so the density stays 1), pfFogiv and pfFogfv
  * accept any MODE / DENSITY value (their range tests use ||). */

@@ -97,6 +97,8 @@ typedef struct {
                               * of the triangles (random kinds)                  */
     float    lineWidth;      /* >0: pfLineWidth                                   */
     int32_t  polyLine;       /* bit 0: front faces PF_LINE, bit 1: back faces    */
+    int32_t  rects;          /* >0: that many pfRectf fills between the two halves
+                              * of the triangles (random kinds)                  */
 } PFSparams;
 
 typedef struct {
@@ -148,6 +150,7 @@ typedef struct {
     void (*DrawArrays)(PFdrawmode, PFint, PFsizei);
     void (*PointSize)(PFfloat);
     void (*LineWidth)(PFfloat);
+    void (*Rectf)(PFfloat, PFfloat, PFfloat, PFfloat);
     void (*PolygonMode)(PFface, PFpolygonmode);
     void (*Fogi)(PFfogparam, PFint);
     void (*Fogf)(PFfogparam, PFfloat);
@@ -187,7 +190,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(DrawElements); BIND(DrawArrays);
     BIND(RasterPos3f); BIND(PixelZoom); BIND(DrawPixels);
     BIND(Fogi); BIND(Fogf); BIND(Fogiv); BIND(Fogfv); BIND(FogProcess);
-    BIND(PointSize); BIND(PolygonMode); BIND(LineWidth);
+    BIND(PointSize); BIND(PolygonMode); BIND(LineWidth); BIND(Rectf);
     BIND(GenList); BIND(DeleteList); BIND(NewList); BIND(EndList); BIND(CallList);
     a->Finish = (void (*)(void))dlsym(h, "pfFinish");
     a->SubmitBatch = (void (*)(const PFcontext*, PFsizei))dlsym(h, "pfSubmitBatch");
@@ -321,6 +324,16 @@ static void draw_lines(const PFSapi* a, Rng* r, const PFSparams* p)
         a->Vertex3f(rrange(r, -1.3f, 1.3f), rrange(r, -1.3f, 1.3f), rrange(r, -1.3f, 1.3f));
     }
     a->End();
+}
+
+/* pfRectf: corners in [-1.2, 1.2]^2 of the current matrices (some clamped
+ * by the viewport), in either order, random colours. */
+static void draw_rects(const PFSapi* a, Rng* r, const PFSparams* p)
+{
+    for (int i = 0; i < p->rects; i++) {
+        a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
+        a->Rectf(rrange(r, -1.2f, 1.2f), rrange(r, -1.2f, 1.2f), rrange(r, -1.2f, 1.2f), rrange(r, -1.2f, 1.2f));
+    }
 }
 
 static void apply_fog(const PFSapi* a, const PFSparams* p)
@@ -798,7 +811,7 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (tex->tex) { a->BindTexture(tex->tex); a->Enable(PF_TEXTURE_2D); }
     if (p->blend && p->kind != PFS_HEIGHTFIELD) a->Enable(PF_BLEND);
     if (p->clearFirst) { a->ClearColor(20, 30, 40, 255); a->Clear(PF_COLOR_BUFFER_BIT | PF_DEPTH_BUFFER_BIT); }
-    if ((p->pixelOps > 0 || p->fog || p->points > 0 || p->lines > 0)
+    if ((p->pixelOps > 0 || p->fog || p->points > 0 || p->lines > 0 || p->rects > 0)
         && (p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D)) {
         PFSparams half = *p;
         half.ntris = p->ntris / 2;
@@ -807,6 +820,7 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
         draw_pixel_ops(a, r, p);
         if (p->points > 0) draw_points(a, r, p);
         if (p->lines > 0) draw_lines(a, r, p);
+        if (p->rects > 0) draw_rects(a, r, p);
         if (p->fog) apply_fog(a, p);
         half.ntris = p->ntris - half.ntris;
         if (p->kind == PFS_RANDOM_NDC) scene_random_ndc(a, r, &half); else scene_random_3d(a, r, &half);

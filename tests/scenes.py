@@ -188,6 +188,13 @@ LINE_SCENES = [
                                   filter=pf.PF_NEAREST, seed=607, polyLine=3, cull=-1)),
 ]
 
+# pfRectf fills between two halves of the triangles (2D, and 3D where the
+# corners go through a perspective MVP without division).
+RECT_SCENES = [
+    ("rects_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=700, rects=6)),
+    ("rects_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=701, rects=5)),
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -219,7 +226,8 @@ SIZE_SCENES = [
 ]
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
-                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES)
+                 + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES
+                 + RECT_SCENES)
 ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

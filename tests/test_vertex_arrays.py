@@ -290,17 +290,3 @@ def test_framebuffer_pixel_setters():
         L.pfDeleteFramebuffer(C.byref(fb))
     finally:
         L.pfDeleteContext(ctx)
-
-
-@pytest.mark.parametrize("call", ["pfRectf"])
-def test_out_of_path_calls_report_invalid_operation(call):
-    L, ctx, _ = _ctx()
-    try:
-        fn = getattr(L, call)
-        if call == "pfRectf":
-            fn(*_fl(0, 0, 1, 1))
-        else:
-            fn()
-        assert L.pfGetError() == INVALID_OPERATION
-    finally:
-        L.pfDeleteContext(ctx)
