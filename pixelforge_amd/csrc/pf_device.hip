@@ -2514,6 +2514,23 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
 // written also without the depth test.
 // ---------------------------------------------------------------------------
 
+// Ordered compaction of one chunk of up to 256 primitives: `hit` per thread
+// (primitive base + threadIdx.x); the indices of the hits land in idx[] in
+// primitive order and the count is returned.  Every thread of the
+// workgroup must call it (it synchronises the workgroup).
+__device__ __forceinline__ uint32_t compact_hits(bool hit, uint32_t i, uint32_t* idx, uint32_t* waveCount) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(hit);
+    if (lane == 0) waveCount[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; w++) before += waveCount[w];
+    const uint32_t total = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+    if (hit) idx[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    __syncthreads();
+    return total;
+}
+
 struct DPointScr {
     int32_t  cx, cy;              // screen pixel
     float    z;                   // clip z (not divided)
@@ -2582,10 +2599,20 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
     uint32_t c = inside ? o.color[off] : 0u;
     float d = inside ? o.depth[off] : 0.0f;
     bool dirty = false;
-    for (uint32_t i = o.first; i < o.last; i++) {
-        const DPointScr s = scr[i];
-        if (s.flags & 0x80000000u) continue;
-        if (s.cx + s.ir < x0 || s.cx - s.ir > x0 + 15 || s.cy + s.ir < y0 || s.cy - s.ir > y0 + 15) continue;
+    __shared__ uint32_t idx[256];
+    __shared__ uint32_t waveCount[4];
+    for (uint32_t base = o.first; base < o.last; base += 256) {
+      // The chunk's points that touch this tile, in order.
+      const uint32_t mine = base + threadIdx.x;
+      bool hit = false;
+      if (mine < o.last) {
+          const DPointScr t = scr[mine];
+          hit = !(t.flags & 0x80000000u)
+             && !(t.cx + t.ir < x0 || t.cx - t.ir > x0 + 15 || t.cy + t.ir < y0 || t.cy - t.ir > y0 + 15);
+      }
+      const uint32_t nHit = compact_hits(hit, mine, idx, waveCount);
+      for (uint32_t h = 0; h < nHit; h++) {
+        const DPointScr s = scr[idx[h]];
         const int32_t dx = px - s.cx, dy = py - s.cy;
         const bool cov = s.ir == 0 && s.rSq == 0.0f
                        ? (dx == 0 && dy == 0)
@@ -2595,6 +2622,8 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
         c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, c) : s.color;
         d = s.z;
         dirty = true;
+      }
+      __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
         o.color[off] = c;
@@ -2800,12 +2829,22 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
     bool dirty = false;
     // Line positions whose offset y * W + x is this pixel's.
     const int32_t cxs[3] = { px, px + W, px - W }, cys[3] = { py, py - 1, py + 1 };
-    for (uint32_t li = o.first; li < o.last; li++) {
-        const DLineScr s = scr[li];
-        if (s.flags & 0x80000000u) continue;
-        const bool wraps = s.bx0 < 0 || s.bx1 >= W;
-        if (s.by1 < y0 - 1 || s.by0 > y0 + 16) continue;
-        if (!wraps && (s.bx1 < x0 || s.bx0 > x0 + 15)) continue;
+    __shared__ uint32_t idx[256];
+    __shared__ uint32_t waveCount[4];
+    for (uint32_t base = o.first; base < o.last; base += 256) {
+      // The chunk's lines that can touch this tile (directly or by a row
+      // wrap), in order.
+      const uint32_t mine = base + threadIdx.x;
+      bool hit = false;
+      if (mine < o.last) {
+          const DLineScr t = scr[mine];
+          const bool wraps = t.bx0 < 0 || t.bx1 >= W;
+          hit = !(t.flags & 0x80000000u) && !(t.by1 < y0 - 1 || t.by0 > y0 + 16)
+             && (wraps || !(t.bx1 < x0 || t.bx0 > x0 + 15));
+      }
+      const uint32_t nHit = compact_hits(hit, mine, idx, waveCount);
+      for (uint32_t hI = 0; hI < nHit; hI++) {
+        const DLineScr s = scr[idx[hI]];
         if (!inside) continue;
         const bool thick = (s.flags & 0x40000000u) != 0;
         const int32_t copies = thick ? 1 + 2 * s.w : 1;
@@ -2839,6 +2878,8 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
                 dirty = true;
             }
         }
+      }
+      __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
         o.color[off] = c;

@@ -184,6 +184,8 @@ LINE_SCENES = [
     ("poly_line_both", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=605, polyLine=3)),
     ("poly_line_back_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=60, seed=606, polyLine=2,
                                 lineWidth=2.0)),
+    ("lines_many_ndc", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=20, seed=608, lines=600,
+                            blend=1, blendMode=1)),
     ("poly_line_quads_list", dict(kind=pf.LIST, width=W, height=H, texture=1, texW=33, texH=33,
                                   filter=pf.PF_NEAREST, seed=607, polyLine=3, cull=-1)),
 ]
