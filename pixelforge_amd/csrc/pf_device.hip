@@ -1120,13 +1120,25 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
 // block's scanned sums plus a scan within the block, in place over the
 // counts.  Both k_setup<1> and k_clip<1> then find their slots there, so
 // the two run side by side.
+// FUSED: the block sums are still raw (k_scan_top did not run); each block
+// adds up those of the blocks before it itself (few blocks).
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
     __shared__ uint32_t lds[64];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = gid < A.numPassTris;
-    uint32_t tot;
-    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + A.blockRT[blockIdx.x];
-    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + A.blockPairs[blockIdx.x];
+    uint32_t tot, baseR, baseP;
+    if (FUSED) {
+        uint32_t sr = 0, sp = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
+        (void)block_excl_scan(sr, lds, baseR);
+        (void)block_excl_scan(sp, lds, baseP);
+    } else {
+        baseR = A.blockRT[blockIdx.x];
+        baseP = A.blockPairs[blockIdx.x];
+    }
+    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
+    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
     if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
 }
 
@@ -3084,8 +3096,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
     hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
     hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
-                       (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
+    // Replays with few blocks leave the block sums raw for k_scan_local<true>.
+    const bool fusedScan = !discover && nb <= 1024;
+    if (!fusedScan)
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
+                           (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (discover) {
         uint32_t tot[2] = {0, 0};
@@ -3111,7 +3126,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (numRT) {
         // Slots per triangle, then the clipped triangles on the side stream
         // while the others are set up (both read the slots only).
-        hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(256), 0, J->stream, sa);
+        if (fusedScan) hipLaunchKernelGGL(k_scan_local<true>, dim3(nb), dim3(256), 0, J->stream, sa);
+        else hipLaunchKernelGGL(k_scan_local<false>, dim3(nb), dim3(256), 0, J->stream, sa);
         HIP_TRY(hipEventRecord(J->fork, J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
         if (J->anyLit) {
