@@ -1395,8 +1395,12 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
     }
 }
 
+// Also clears the clip-list counter for the next pass (both clip kernels of
+// this pass have read it by now).
 __global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
-                              uint32_t* __restrict__ tileBegin, uint32_t* __restrict__ tileEnd) {
+                              uint32_t* __restrict__ tileBegin, uint32_t* __restrict__ tileEnd,
+                              uint32_t* __restrict__ clipCount) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clipCount = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t k = keys[i];
         if (i == 0 || keys[i - 1] != k) tileBegin[k] = (uint32_t)i;
@@ -3093,7 +3097,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.clipCount = (uint32_t*)J->clipCount.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
+    // Every pass leaves the clip-list counter at zero (k_tile_bounds, or the
+    // fill at its end); the discovering first run clears it here as well.
+    if (discover)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
     hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
     hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
     // Replays with few blocks leave the block sums raw for k_scan_local<true>.
@@ -3170,8 +3177,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         keys = kIn;
         vals = vIn;
         hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(numPairs, 256 * 4)), dim3(256), 0, J->stream,
-                           keys, numPairs, (uint32_t*)J->tileBegin.p, (uint32_t*)J->tileEnd.p);
+                           keys, numPairs, (uint32_t*)J->tileBegin.p, (uint32_t*)J->tileEnd.p,
+                           (uint32_t*)J->clipCount.p);
     }
+    if (!numPairs)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[2], J->stream), PFD_E_LAUNCH);
 
