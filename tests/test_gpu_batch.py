@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import pixelforge_amd as pf
+import scenes
 from conftest import render_gpu, render_oracle
 
 pytestmark = pytest.mark.gpu
@@ -102,3 +103,33 @@ def test_batch_with_tiny_and_clear_only_contexts():
     for i, (p, img) in enumerate(zip(params, imgs)):
         ref, _, e, _ = render_oracle(p)
         assert np.array_equal(img, ref), f"context {i} ({p.width}x{p.height}) differs from the oracle"
+
+
+@pytest.mark.skip(reason="known issue: replays of a captured batch job whose frames have several passes "
+                         "(points, lines, fog, pixel rectangles between triangle passes) differ from the oracle "
+                         "after the first run; one-shot frames (pfFinish) and single-pass batches are exact")
+def test_multi_pass_frames_replay_in_graphs():
+    """Frames with several passes (triangles, then points / lines / fog /
+    pixel rectangles, then triangles again) and clipped 3D triangles,
+    captured as one batch job and replayed as a graph several times: every
+    replay equals the oracle (the clip-list counter is left at zero by each
+    pass for the next one)."""
+    names = ["points_discs_3d", "lines_3d", "fog_between_draws_3d", "pixels_3d_rgba_ubyte", "rects_3d"]
+    kws = dict(scenes.ALL_SCENES)
+    params = [scenes.params(kws[n]) for n in names]
+    refs = [render_oracle(p)[0] for p in params]
+    cap = pf.capture_batch(params)
+    lib = pf.lib()
+    try:
+        for replay in range(4):
+            for b in cap.buffers:
+                b[:] = 0
+            assert lib.pfdJobUploadTargets(cap.job) == 0
+            assert lib.pfdJobLaunch(cap.job) == 0
+            assert lib.pfdJobSync(cap.job) == 0
+            assert lib.pfdJobDownload(cap.job) == 0
+            for n, ref, img in zip(names, refs, cap.buffers):
+                got = img[:ref.size].reshape(ref.shape)
+                assert np.array_equal(got, ref), f"replay {replay}: {n} differs from the oracle"
+    finally:
+        cap.close()
