@@ -50,7 +50,7 @@ class SceneParams(C.Structure):
         ("fog", C.c_int32), ("fogStart", C.c_float), ("fogEnd", C.c_float), ("fogAlpha", C.c_int32),
         ("points", C.c_int32), ("pointSize", C.c_float), ("polyPoint", C.c_int32),
         ("lines", C.c_int32), ("lineWidth", C.c_float), ("polyLine", C.c_int32),
-        ("rects", C.c_int32),
+        ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float),
     ]
 
 
@@ -126,6 +126,9 @@ def lib():
         L.pfMakeCurrent.argtypes = [C.c_void_p]
         L.pfDeleteContext.argtypes = [C.c_void_p]
         L.pfGetError.restype = C.c_int
+        L.pfdJobTarget.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.pfdJobFrameCount.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -213,6 +216,37 @@ class Captured:
     def __init__(self, job, ctxs, buffers, textures):
         self.job = C.c_void_p(job)
         self.ctxs, self.buffers, self.textures = ctxs, buffers, textures
+
+    def targets(self):
+        """The job's host render targets as (colour uint32[n], depth
+        float32[n]) views of the buffers the job uploads and downloads."""
+        L = lib()
+        out = []
+        for f in range(L.pfdJobFrameCount(self.job)):
+            c, d, w, h = C.c_void_p(), C.c_void_p(), C.c_uint32(), C.c_uint32()
+            if L.pfdJobTarget(self.job, f, C.byref(c), C.byref(d), C.byref(w), C.byref(h)):
+                raise RuntimeError("pfdJobTarget failed")
+            n = w.value * h.value
+            out.append((np.ctypeslib.as_array(C.cast(c, C.POINTER(C.c_uint32)), (n,)),
+                        np.ctypeslib.as_array(C.cast(d, C.POINTER(C.c_float)), (n,))))
+        return out
+
+    def snapshot(self):
+        """Copies of the host render targets (to restore between runs)."""
+        return [(c.copy(), d.copy()) for c, d in self.targets()]
+
+    def restore(self, snap):
+        """Writes a snapshot back into the host render targets and uploads
+        them, so that the next run starts from the same pixels."""
+        for (c, d), (sc, sd) in zip(self.targets(), snap):
+            c[:] = sc
+            d[:] = sd
+        if lib().pfdJobUploadTargets(self.job):
+            raise RuntimeError("pfdJobUploadTargets failed")
+
+    def images(self):
+        """Copies of the host colour targets after a download."""
+        return [c.copy() for c, _ in self.targets()]
 
     def close(self):
         L = lib()

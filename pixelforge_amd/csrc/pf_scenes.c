@@ -99,6 +99,8 @@ typedef struct {
     int32_t  polyLine;       /* bit 0: front faces PF_LINE, bit 1: back faces    */
     int32_t  rects;          /* >0: that many pfRectf fills between the two halves
                               * of the triangles (random kinds)                  */
+    float    uvMin, uvMax;   /* random kinds: texture coordinates uniform in
+                              * [uvMin, uvMax) when uvMax > uvMin, else [-2, 2) */
 } PFSparams;
 
 typedef struct {
@@ -457,6 +459,13 @@ static void maybe_flush(const PFSapi* a, const PFSparams* p, uint32_t drawn)
     if (p->flushEvery > 0 && a->Finish && drawn % (uint32_t)p->flushEvery == 0) a->Finish();
 }
 
+/* One texture coordinate of a random-kind vertex (v is drawn before u, the
+ * order in which the scenes have always consumed the generator). */
+static float rand_uv(Rng* r, const PFSparams* p)
+{
+    return p->uvMax > p->uvMin ? rrange(r, p->uvMin, p->uvMax) : rrange(r, -2, 2);
+}
+
 /* Random triangles with vertices uniform in NDC [-1,1]^3 (2D path). */
 static void scene_random_ndc(const PFSapi* a, Rng* r, const PFSparams* p)
 {
@@ -468,7 +477,7 @@ static void scene_random_ndc(const PFSapi* a, Rng* r, const PFSparams* p)
             float x = rrange(r, -1, 1), y = rrange(r, -1, 1), z = rrange(r, -1, 1);
             if (s != 1.0f) { x = cx + s * x; y = cy + s * y; z = cz + s * z; }
             a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
-            if (p->texture) a->TexCoord2f(rrange(r, -2, 2), rrange(r, -2, 2));
+            if (p->texture) { float v = rand_uv(r, p), u = rand_uv(r, p); a->TexCoord2f(u, v); }
             a->Vertex3f(x, y, z);
         }
         maybe_flush(a, p, t + 1);
@@ -488,7 +497,7 @@ static void scene_random_3d(const PFSapi* a, Rng* r, const PFSparams* p)
         for (int k = 0; k < 3; k++) {
             float x = cx + s * rrange(r, -2, 2), y = cy + s * rrange(r, -2, 2), z = cz + s * rrange(r, -2, 2);
             a->Color4ub(rbyte(r), rbyte(r), rbyte(r), rbyte(r));
-            if (p->texture) a->TexCoord2f(rrange(r, -2, 2), rrange(r, -2, 2));
+            if (p->texture) { float v = rand_uv(r, p), u = rand_uv(r, p); a->TexCoord2f(u, v); }
             if (p->lighting) a->Normal3f(rrange(r, -1, 1), rrange(r, -1, 1), rrange(r, -1, 1));
             a->Vertex3f(x, y, z);
         }

@@ -2,8 +2,9 @@
 
 Each entry is (name, keyword arguments of pixelforge_amd.scene()).  Sizes
 are small so that the oracle and the compiled reference finish in well
-under a second per scene.  `BILINEAR` scenes have no reference golden: the
-reference's bilinear filter reads an uninitialised vector (color.h:141).
+under a second per scene.  The goldens of the `BILINEAR` scenes come from the
+reference built with its bilinear colour lerp fixed (color.h:141 reads an
+uninitialised vector; oracle/Makefile `ref-bilinear`).
 """
 import pixelforge_amd as pf
 
@@ -28,6 +29,9 @@ FEATURE_SCENES = [
     ("no_depth_test", dict(kind=pf.RANDOM_NDC, width=W, height=H, ntris=200, seed=22, depthTest=0)),
     ("no_clear", dict(kind=pf.RANDOM_NDC, width=W, height=H, ntris=150, seed=23, clearFirst=0)),
     ("odd_size_clear", dict(kind=pf.RANDOM_NDC, width=W - 3, height=H - 5, ntris=150, seed=24)),
+    # W*H = 7 mod 8: the clear copies pixel 0 into the last 7 pixels, and
+    # triangles of the same frame then draw over pixel 0 itself
+    ("odd_clear_pixel0_3d", dict(kind=pf.RANDOM_3D, width=125, height=91, ntris=300, seed=85)),
     ("clip_3d", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=400, seed=25)),
     ("clip_3d_cull_front", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=400, seed=26, cull=pf.PF_FRONT)),
     ("clip_3d_cull_back", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=400, seed=27, cull=pf.PF_BACK)),
@@ -69,21 +73,45 @@ WRAP_SCENES = [
 
 # Even sizes in MIRRORED / CLAMP_TO_EDGE: out-of-texture reads, which the
 # reference leaves undefined and this library clamps to the last texel.
+# Bilinear CLAMP_TO_EDGE at an even side reads past the texture as soon as
+# u + 1/w reaches 1 (the second tap's index rounds to w).
 EDGE_READ_SCENES = [
     (f"edge_read_{w}", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=250, seed=66 + w, texture=1,
                             texW=48, texH=40, wrap=w, filter=pf.PF_NEAREST))
     for w in (1, 2)
+] + [
+    ("edge_read_bilinear_2", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=250, seed=72, texture=1,
+                                  texW=64, texH=64, wrap=pf.PF_CLAMP_TO_EDGE, filter=pf.PF_BILINEAR)),
 ]
 
+# Bilinear filtering in every wrap mode (odd sides for MIRRORED and
+# CLAMP_TO_EDGE; CLAMP_TO_EDGE at even sides on the 2D path with vertex
+# coordinates below 1 - 2/w, so that no tap leaves the texture -- the 2D
+# path interpolates them affinely, while the 3D path divides by the
+# interpolated 1/z of clip space, which can leave the vertices' range),
+# float texels, the config-2 height field and a batch-style additive
+# context.  Goldens: the reference with the colour-lerp fix (module doc).
+_BL = dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=250, texture=1, filter=pf.PF_BILINEAR)
 BILINEAR_SCENES = [
-    (f"bilinear_{w}", dict(kind=pf.RANDOM_3D, width=W, height=H, ntris=250, seed=70 + w, texture=1,
-                           texW=64, texH=64, wrap=w, filter=pf.PF_BILINEAR))
-    for w in range(3)
-] + [
+    ("bilinear_0", dict(_BL, seed=70, texW=64, texH=64, wrap=pf.PF_REPEAT)),
+    ("bilinear_1", dict(_BL, seed=71, texW=63, texH=47, wrap=pf.PF_MIRRORED_REPEAT)),
+    ("bilinear_1_even", dict(_BL, seed=75, texW=64, texH=64, wrap=pf.PF_MIRRORED_REPEAT)),
+    ("bilinear_2", dict(_BL, seed=73, texW=63, texH=47, wrap=pf.PF_CLAMP_TO_EDGE)),
+    ("bilinear_2_even_inner", dict(kind=pf.RANDOM_NDC, width=W, height=H, ntris=200, seed=76, texture=1,
+                                   filter=pf.PF_BILINEAR, texW=64, texH=48, wrap=pf.PF_CLAMP_TO_EDGE,
+                                   uvMin=-0.5, uvMax=1.0 - 3.0 / 64.0)),
+    ("bilinear_2d", dict(kind=pf.RANDOM_NDC, width=W - 7, height=H - 9, ntris=200, seed=77, texture=1, texW=45,
+                         texH=29, wrap=pf.PF_REPEAT, filter=pf.PF_BILINEAR)),
+    ("bilinear_float", dict(_BL, seed=78, texture=2, texW=33, texH=17, wrap=pf.PF_REPEAT)),
     ("c2_heightfield_bilinear", dict(kind=pf.HEIGHTFIELD, width=W, height=H, gridX=60, gridZ=12, seed=74,
                                      texture=1, texW=64, texH=64, wrap=pf.PF_REPEAT, filter=pf.PF_BILINEAR,
                                      depthFunc=pf.PF_LEQUAL, blendFraction=0.3, cull=-1)),
+    ("bilinear_batch_add", dict(kind=pf.HEIGHTFIELD, width=113, height=87, gridX=40, gridZ=6, seed=79, texture=1,
+                                texW=96, texH=72, wrap=pf.PF_REPEAT, filter=pf.PF_BILINEAR,
+                                depthFunc=pf.PF_LEQUAL, cull=pf.PF_BACK, blend=1, blendMode=pf.PF_BLEND_ADD,
+                                blendFraction=1.0)),
 ]
+BILINEAR_GOLDEN_NAMES = frozenset(n for n, _ in BILINEAR_SCENES)
 
 # Vertex arrays (pfDrawElements / pfDrawArrays) against the same mesh drawn
 # in immediate mode; every index, position and colour type.
@@ -229,8 +257,8 @@ SIZE_SCENES = [
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
                  + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES
-                 + RECT_SCENES)
-ALL_SCENES = GOLDEN_SCENES + BILINEAR_SCENES + EDGE_READ_SCENES + SIZE_SCENES
+                 + RECT_SCENES + BILINEAR_SCENES)
+ALL_SCENES = GOLDEN_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 
 def params(kw):
