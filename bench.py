@@ -29,6 +29,7 @@ chain (setup, binning, tile raster) as one HIP graph.
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import statistics
@@ -103,6 +104,47 @@ def time_replays(job, steps, warmup):
     return time.perf_counter() - t0
 
 
+def verified_replays(cap, steps, warmup):
+    """Runs the job once from its captured targets (sizing run, then the
+    first graph replay from the same targets), then times `steps` graph
+    replays after `warmup` more, and downloads the frames again: the timed
+    replays must leave exactly the first replay's images, or the line
+    fails.  Returns (seconds, first images, sha1 of the first images)."""
+    lib = pf.lib()
+    job = cap.job
+    snap = cap.snapshot()
+    for _ in range(2):
+        cap.restore(snap)
+        check(lib.pfdJobLaunch(job), "launch")
+        check(lib.pfdJobSync(job), "sync")
+    check(lib.pfdJobDownload(job), "download")
+    first = cap.images()
+    elapsed = time_replays(job, steps, warmup)
+    check(lib.pfdJobDownload(job), "download")
+    for i, (a, b) in enumerate(zip(first, cap.images())):
+        if not np.array_equal(a, b):
+            raise RuntimeError(f"frame {i}: the timed replays left {np.count_nonzero(a != b)} pixels different "
+                               "from the first replay")
+    cap.restore(snap)
+    digest = hashlib.sha1(b"".join(a.tobytes() for a in first)).hexdigest()[:16]
+    return elapsed, first, digest
+
+
+def oracle_check(params, image):
+    """cpu_baseline leg: the oracle renders the same recorded frame; True
+    when the GPU's first replay equals it bit for bit."""
+    if not os.path.exists(ORACLE_SO):
+        return None
+    o = C.CDLL(ORACLE_SO)
+    lib = pf.lib()
+    lib.pfxSetExecutor(C.cast(o.pforacle_executor, C.c_void_p), None)
+    try:
+        ref, _, err = pf.render(params)
+    finally:
+        lib.pfxSetExecutor(None, None)
+    return err == 0 and bool(np.array_equal(ref.ravel(), image))
+
+
 def phase_times(job, runs):
     lib = pf.lib()
     raster, setup, binning = [], [], []
@@ -148,7 +190,7 @@ def bench_frame(args):
     lib = pf.lib()
     check(lib.pfdJobUploadTargets(job), "upload")
     cfg = configure(job, args.tile_shift, args.raster_mode)
-    elapsed = time_replays(job, args.steps, args.warmup)
+    elapsed, first, digest = verified_replays(cap, args.steps, args.warmup)
     st = stats(job)
     frags = int(st.fragments)
     setup_ms, bin_ms, raster_ms = phase_times(job, args.profile_runs)
@@ -170,7 +212,10 @@ def bench_frame(args):
                      "kernel": "k_raster", "kernel_ms": raster_ms, "algorithmic_bytes": raster_bytes},
         "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
         "record_s": rec_s,
+        "replays": {"timed_equal_first": True, "frame_sha1": digest},
     }
+    if not args.no_cpu:
+        out["replays"]["first_equals_oracle"] = oracle_check(params, first[0])
     if not args.no_cpu and os.path.exists(REF_SO):
         med = cpu_frame_rate([params], frames=5)
         out["cpu_baseline"] = {
@@ -195,7 +240,7 @@ def bench_batch(args):
     lib = pf.lib()
     check(lib.pfdJobUploadTargets(job), "upload")
     cfg = configure(job, args.batch_tile_shift, args.batch_raster_mode)
-    elapsed = time_replays(job, args.batch_steps, 3)
+    elapsed, first, digest = verified_replays(cap, args.batch_steps, 3)
     st = stats(job)
     setup_ms, bin_ms, raster_ms = phase_times(job, max(3, args.profile_runs // 4))
     ms = elapsed * 1e3 / args.batch_steps
@@ -205,6 +250,7 @@ def bench_batch(args):
     algo = px * 8 + tex + tris * SETUP_BYTES_PER_TRI
     out = {
         "value": len(params) * args.batch_steps / elapsed, "unit": "frames/s", "contexts": len(params),
+        "replays": {"timed_equal_first": True, "frames_sha1": digest},
         "ms_per_batch": ms, "fragments": int(st.fragments), "input_triangles": tris,
         "pixels": px, "tile": 1 << int(st.tileShift), "raster_mode": RASTER_MODES[cfg.rasterMode],
         "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
@@ -216,6 +262,9 @@ def bench_batch(args):
                                        "traffic": measured_traffic(f"k_raster batch{len(params)} tile "
                                                                    f"{1 << int(st.tileShift)} mode {cfg.rasterMode}")}},
     }
+    if not args.no_cpu:
+        out["replays"]["first_equals_oracle"] = all(oracle_check(p, first[i]) for i, p in
+                                                    enumerate(params[: args.cpu_contexts]))
     if not args.no_cpu and os.path.exists(REF_SO):
         sample = params[: args.cpu_contexts]
         med = cpu_frame_rate(sample, frames=3)

@@ -94,6 +94,11 @@ int  pfdJobDownload(PFDjob* job);                /* device -> host target buffer
 int  pfdJobRunTimed(PFDjob* job, PFDtimes* t);   /* one run with per-phase events */
 int  pfdJobTimeReplays(PFDjob* job, uint32_t iters, float* msTotal); /* graph replays */
 int  pfdJobStats(PFDjob* job, PFDstats* st);
+/* Host render target of frame `frame` (the buffers pfdJobUploadTargets reads
+ * and pfdJobDownload writes): a caller replaying a job restores them between
+ * runs.  Any output pointer may be NULL. */
+int  pfdJobTarget(PFDjob* job, uint32_t frame, uint32_t** color, float** depth, uint32_t* width, uint32_t* height);
+int  pfdJobFrameCount(PFDjob* job);               /* frames in the job, or < 0 */
 int  pfdJobAutotune(PFDjob* job, PFDconfig* chosen);
 void pfdJobDestroy(PFDjob* job);
 

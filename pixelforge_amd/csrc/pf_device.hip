@@ -706,6 +706,8 @@ struct SetupArgs {
     uint32_t        capPairs;
     uint4*          clipList;       // triangles to clip {gid, t, info, frame} (PHASE 0 appends)
     uint32_t*       clipCount;
+    uint32_t        clipCap;        // entries clipList holds
+    uint32_t*       clearSnap;      // per frame: colour and depth bits of pixel 0 when the pass began
 };
 
 // Tiles [tx0..tx1] x [ty0..ty1] overlapped by the covered pixels of a
@@ -1068,6 +1070,19 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     if (PHASE == 0) {
         // Reset the tile ranges of this pass (the raster reads [begin, end)).
         for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
+        // pfClear copies pixel 0 into the pixels past the last multiple of 8
+        // before any later draw (context.c:707-710).  The raster kernel of
+        // this pass may already be writing pixel 0 while other tiles clear
+        // their tail, so the tail takes pixel 0 as it was when the pass
+        // began: every earlier pass has finished by the time this runs.
+        if (blockIdx.x == 0)
+            for (uint32_t s = threadIdx.x; s < A.numSegs; s += blockDim.x) {
+                const DSeg sg = A.segs[s];
+                if (!sg.clear) continue;
+                const DFrame F = A.frames[sg.frame];
+                A.clearSnap[2 * sg.frame] = F.color[0];
+                A.clearSnap[2 * sg.frame + 1] = __float_as_uint(F.depth[0]);
+            }
     }
     // The triangle, its frame and positions are requested first, so their
     // latency overlaps the block scans of PHASE 1.
@@ -1106,7 +1121,11 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
             if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
             base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
             const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            if (deferred) A.clipList[base + __popcll(D & ltMask)] = make_uint4(gid, t, info, frame);
+            // The list holds every triangle of the pass once the counter
+            // starts at zero; the bound keeps a counter left over by a failed
+            // run from writing past it.
+            const uint32_t at = base + __popcll(D & ltMask);
+            if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
         }
         if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
         uint32_t totR, totP;
@@ -1200,7 +1219,7 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
     CVtx<VF>* poly = reinterpret_cast<CVtx<VF>*>(&mem[lane * CLIP_WORDS]);
     CVtx<VF>* tmp = poly + 12;
     PairRect* rects = PHASE == 1 ? &rectMem[lane * MAX_FAN_RECTS] : nullptr;
-    const uint32_t n = *A.clipCount;
+    const uint32_t n = min(*A.clipCount, A.clipCap);
     // Wavefront-uniform loop: the pair writing below needs every lane.
     for (uint32_t i0 = blockIdx.x * 64; i0 < n; i0 += CLIP_GRID * 64) {
         const uint32_t i = i0 + (uint32_t)lane;
@@ -1427,6 +1446,7 @@ struct RasterArgs {
                                     // the raster itself)
     uint32_t*       winner;         // per pixel of frame 0: 1 + raster triangle of the last
                                     // fragment written, 0 for none; null unless differentiable
+    const uint32_t* clearSnap;      // per frame: pixel 0 (colour, depth bits) when the pass began
 };
 
 // ---------------------------------------------------------------------------
@@ -1647,7 +1667,8 @@ __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
 // of the list serves 256 pixels.  Returns the covered samples it saw.
 template <int MODE, bool PHONG, int K>
 __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
-                                                 int bx0, int by0, uint32_t e0, uint32_t e1, WaveQueueT<PHONG>& Q) {
+                                                 uint32_t frame, int bx0, int by0, uint32_t e0, uint32_t e1,
+                                                 WaveQueueT<PHONG>& Q) {
     constexpr bool ROT = false;                   // staged records are not rotated (slower, measured)
     constexpr int PM = MODE;                      // 0 packed, 1 in place
     constexpr int TW = (K == 1) ? 8 : 16;
@@ -1673,10 +1694,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             if (sg.clear && pi[k] >= 8 && pi[k] < aligned) {       // context.c:693-713
                 col[k] = sg.clearColor;
                 dep[k] = sg.clearDepth;
+            } else if (sg.clear && pi[k] >= aligned) {              // the tail copies pixel 0
+                col[k] = A.clearSnap[2 * frame];
+                dep[k] = __uint_as_float(A.clearSnap[2 * frame + 1]);
             } else {
-                const size_t src = (sg.clear && pi[k] >= aligned) ? 0 : pi[k];
-                col[k] = F.color[src];
-                dep[k] = F.depth[src];
+                col[k] = F.color[pi[k]];
+                dep[k] = F.depth[pi[k]];
             }
         }
     }
@@ -1854,7 +1877,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     if (tile >= A.numTiles) return;
     uint32_t* fragOut = A.tileFrags + (size_t)tile * 4 + (NB == 1 ? 0 : wave);
     const bool lead = (threadIdx.x & 63) == 0;
-    const DTileFrame sg = A.tileDesc[A.tileFrame[tile]];
+    const uint32_t frame = A.tileFrame[tile];
+    const DTileFrame sg = A.tileDesc[frame];
     if (!sg.active) { if (lead) *fragOut = 0u; return; }
     const DFrame& F = sg.F;
     const uint32_t e0 = A.tileBegin[tile], e1 = A.tileEnd[tile];
@@ -1865,7 +1889,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
         const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        frags += raster_block<MODE, PHONG, K>(A, F, sg, bx0, by0, e0, e1, Q);
+        frags += raster_block<MODE, PHONG, K>(A, F, sg, frame, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -2998,7 +3022,7 @@ struct PFDjob {
     uint32_t numTiles = 0;
     // scratch
     DevBuf cnt, off, blockRT, blockPairs, rts, rtl, keysA, valsA, keysB, valsB;
-    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags, dTileFrame, clipList, clipCount;
+    DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags, dTileFrame, clipList, clipCount, clearSnap;
     uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
     // graph
     hipGraph_t graph = nullptr;
@@ -3058,6 +3082,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
     if ((rc = ensure(J->scanTotal, 16))) return rc;
     if ((rc = ensure(J->clipList, (size_t)n * 16 + 16)) || (rc = ensure(J->clipCount, 16))) return rc;
+    if ((rc = ensure(J->clearSnap, J->frames.size() * 8))) return rc;
     const size_t fragSlots = (size_t)J->numTiles * 4;
     const size_t passIdx = (size_t)(&P - J->passes.data());
     if (discover) {
@@ -3095,6 +3120,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.tileEnd = (uint32_t*)J->tileEnd.p;
     sa.clipList = (uint4*)J->clipList.p;
     sa.clipCount = (uint32_t*)J->clipCount.p;
+    sa.clipCap = n;
+    sa.clearSnap = (uint32_t*)J->clearSnap.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
     // Every pass leaves the clip-list counter at zero (k_tile_bounds, or the
@@ -3197,6 +3224,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.numTiles = J->numTiles;
     ra.tileFrags = (uint32_t*)J->tileFrags.p + passIdx * fragSlots;
     ra.winner = J->diff ? (uint32_t*)J->winner.p : nullptr;
+    ra.clearSnap = (const uint32_t*)J->clearSnap.p;
     switch (J->cfg.rasterMode) {
     case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
     case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
@@ -3221,8 +3249,39 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
 int run_all(PFDjob* J, bool discover) {
     int rc = 0;
     for (auto& P : J->passes)
-        if ((rc = run_pass(J, P, discover || !P.sized, nullptr))) return rc;
+        if ((rc = run_pass(J, P, discover || !P.sized, nullptr))) {
+            // A run that stopped part-way may leave device counters (the
+            // clip list's) off zero; the next run rediscovers, which clears
+            // them before use.
+            for (auto& Q : J->passes) Q.sized = false;
+            return rc;
+        }
     return 0;
+}
+
+// Device time of `iters` runs launched straight onto the job's stream (no
+// graph), after a sizing run if one is due.
+int time_direct(PFDjob* J, uint32_t iters, float* ms) {
+    int rc = 0;
+    bool allSized = true;
+    for (auto& P : J->passes) allSized = allSized && P.sized;
+    if (!allSized && (rc = run_all(J, true))) return rc;
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, J->stream);
+    for (uint32_t i = 0; e == hipSuccess && !rc && i < iters; i++) rc = run_all(J, false);
+    if (e == hipSuccess && !rc) e = hipEventRecord(b, J->stream);
+    if (e == hipSuccess && !rc) e = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (e == hipSuccess && !rc) e = hipEventElapsedTime(&t, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) { set_error("time_direct", e); return PFD_E_LAUNCH; }
+    *ms = t;
+    return PFD_OK;
 }
 
 void layout_tiles(PFDjob* J) {
@@ -3739,6 +3798,8 @@ int pfdJobLaunch(PFDjob* J) {
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(J->stream, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess)
+            for (auto& P : J->passes) P.sized = false;
         if (e != hipSuccess) { set_error("hipStreamEndCapture", e); return PFD_E_LAUNCH; }
         J->graph = g;
         e = hipGraphInstantiate(&J->graphExec, g, nullptr, nullptr, 0);
@@ -3808,6 +3869,18 @@ int pfdJobTimeReplays(PFDjob* J, uint32_t iters, float* msTotal) {
     return PFD_OK;
 }
 
+int pfdJobTarget(PFDjob* J, uint32_t frame, uint32_t** color, float** depth, uint32_t* width, uint32_t* height) {
+    if (!J || frame >= J->src.size()) return PFD_E_INVALID;
+    const PFFframe& fr = J->src[frame];
+    if (color) *color = fr.color;
+    if (depth) *depth = fr.depth;
+    if (width) *width = fr.width;
+    if (height) *height = fr.height;
+    return PFD_OK;
+}
+
+int pfdJobFrameCount(PFDjob* J) { return J ? (int)J->src.size() : PFD_E_INVALID; }
+
 int pfdJobStats(PFDjob* J, PFDstats* st) {
     if (!J || !st) return PFD_E_INVALID;
     memset(st, 0, sizeof *st);
@@ -3846,8 +3919,12 @@ int pfdJobStats(PFDjob* J, PFDstats* st) {
 }
 
 // Times every tile size and raster mode on the job (one sizing run plus a
-// few graph replays each) and keeps the fastest.  The target buffers are
-// restored from the host afterwards so the job's output is unchanged.
+// few runs each) and keeps the fastest.  The target buffers are restored
+// from the host afterwards so the job's output is unchanged.  Candidates are
+// timed as runs launched straight onto the stream, not as graphs: a graph
+// instantiated and destroyed per candidate (14 per job) bought nothing in
+// the ranking, and the one graph of the chosen configuration is built by
+// the next pfdJobLaunch.
 int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if (!J) return PFD_E_INVALID;
     float best = 1e30f;
@@ -3863,13 +3940,13 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
             // Replays worth about 10 ms, best of three trials: single short
             // timings mis-ranked configurations a few percent apart.
             float est = 0;
-            if ((rc = pfdJobTimeReplays(J, 3, &est))) return rc;
+            if ((rc = time_direct(J, 3, &est))) return rc;
             const float per = est / 3.0f;
             const uint32_t iters = per > 0.0f ? (uint32_t)fminf(50.0f, fmaxf(3.0f, 10.0f / per)) : 3u;
             float ms = 1e30f;
             for (int trial = 0; trial < 3; trial++) {
                 float t = 0;
-                if ((rc = pfdJobTimeReplays(J, iters, &t))) return rc;
+                if ((rc = time_direct(J, iters, &t))) return rc;
                 ms = fminf(ms, t / (float)iters);
             }
             if (ms < best) { best = ms; bestCfg = c; }
@@ -3888,7 +3965,7 @@ void pfdJobDestroy(PFDjob* J) {
     drop_graph(J);
     DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
                        &J->rts, &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
-                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList, &J->clipCount,
+                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList, &J->clipCount, &J->clearSnap,
                        &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams, &J->lightParams, &J->image,
                        &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
     for (DevBuf* b : bufs) release(*b);

@@ -37,3 +37,17 @@ def test_oracle_matches_golden(name):
     assert err == 0
     assert frags > 0 or _points_only(kw)
     assert_same(g["color"], g["depth"], color, depth, name)
+
+
+def test_clear_tail_keeps_pixel0_from_before_the_draws():
+    """The odd-size scene draws over pixel 0 after its clear, while the
+    pixels of the tail that no triangle reaches keep what the clear copied
+    there: pixel 0 from before the draws (context.c:707-710), which in a
+    fresh framebuffer is colour 0 and depth FLT_MAX."""
+    g = np.load(os.path.join(GOLDEN, "odd_clear_pixel0_3d.npz"), allow_pickle=False)
+    color, depth = g["color"].ravel(), g["depth"].ravel()
+    tail = color.size % 8
+    assert tail == 7
+    assert color[0] != 0 and depth[0] < 1.0
+    kept = (color[-tail:] == 0) & (depth[-tail:] == np.finfo(np.float32).max)
+    assert kept.any()
