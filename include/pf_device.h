@@ -78,7 +78,10 @@ const char* pfdLastError(void);
 int pfdSetDefaultConfig(const PFDconfig* cfg);
 
 /* One-shot execution used by the host library's default executor: uploads
- * the frame and the render target, runs it and downloads the target. */
+ * the frame and the render target, runs it and downloads the target.  The
+ * library keeps one job's streams and device buffers between calls
+ * (buffers only grow), so a frame no larger than an earlier one creates no
+ * stream and calls no hipMalloc or hipFree. */
 int pfdExecuteFrame(const PFFframe* frame);
 
 /* Same for several frames (distinct render targets) in one batched job. */

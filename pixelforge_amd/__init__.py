@@ -129,6 +129,7 @@ def lib():
         L.pfdJobTarget.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.pfdJobFrameCount.argtypes = [C.c_void_p]
+        L.pfdExecuteFrame.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

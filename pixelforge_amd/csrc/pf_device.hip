@@ -32,6 +32,7 @@
 #include <math.h>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "pf_frame.h"
 #include "pf_device.h"
@@ -2952,29 +2953,16 @@ int g_initState = 0;   // 0 unknown, 1 ok, -1 failed
 struct DevBuf {
     void*  p = nullptr;
     size_t bytes = 0;
+    bool   borrowed = false;   // carved from an arena: never freed
 };
 
-int ensure(DevBuf& b, size_t bytes) {
-    if (bytes <= b.bytes && b.p) return 0;
-    if (b.p) {
-        // Kernels already queued may still read the old block.
-        (void)hipDeviceSynchronize();
-        (void)hipFree(b.p);
-        b.p = nullptr;
-        b.bytes = 0;
-    }
-    size_t want = bytes < 256 ? 256 : bytes;
-    hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess) { set_error("hipMalloc", e); b.p = nullptr; return PFD_E_NOMEM; }
-    b.bytes = want;
-    return 0;
-}
-
-void release(DevBuf& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-}
+// Device memory a job carves its buffers from instead of calling hipMalloc.
+// Bump allocation; reset when the job is reset.
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t   cap = 0, used = 0;
+    const char* owner = "";
+};
 
 struct PassInfo {
     std::vector<DSeg> segs;
@@ -2983,24 +2971,33 @@ struct PassInfo {
     uint32_t numRT = 0;        // known after the first run
     uint32_t numPairs = 0;
     bool     sized = false;
-    DevBuf   dSegs, dFrameSeg, dBlockSeg, dTileDesc;
+    std::vector<uint32_t> blockSeg;   // per 256-triangle block: segment of its first triangle
     std::vector<DPixOp> pix;   // pfDrawPixels calls run after this pass's raster
     uint32_t pixGrid = 0;
-    DevBuf   dPix;
     std::vector<DPtOp> pts;    // runs of points, after the pixel calls
     uint32_t ptTiles = 0;
-    DevBuf   dPts;
     std::vector<DLnOp> lns;    // runs of lines, after the points
     uint32_t lnTiles = 0;
-    DevBuf   dLns;
+    // Device copies of the tables above, inside the job's passTab buffer,
+    // and of the pass's per-frame tile descriptors, inside tileDescs.
+    const DSeg*       dSegs = nullptr;
+    const uint32_t*   dBlockSeg = nullptr;
+    const DPixOp*     dPix = nullptr;
+    const DPtOp*      dPts = nullptr;
+    const DLnOp*      dLns = nullptr;
+    const DTileFrame* dTileDesc = nullptr;
 };
 
 }  // namespace
 
+// Every device buffer of a job is a member of this struct (no vectors of
+// buffers), so a job can be reset and planned again for the next frame
+// while keeping its memory: buffers only grow.
 struct PFDjob {
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;           // k_clip<1> beside k_setup<1> (default priority)
+    hipStream_t side = nullptr;           // k_clip<1> beside k_setup<1> (default priority); may be null
     hipEvent_t fork = nullptr, join = nullptr;
+    Arena* arena = nullptr;               // non-null: buffers come from the arena, never from hipMalloc
     std::vector<PFFframe> src;            // host frame descriptors: only the target fields are
                                           // used after creation (the rest may be reused by the caller)
     std::vector<uint8_t>  leadClear;      // per frame: the first command is a clear
@@ -3008,10 +3005,12 @@ struct PFDjob {
     std::vector<PFFstate>   srcStates;
     std::vector<DFrame>   frames;
     DevBuf dFrames;
-    std::vector<DevBuf> targets;          // colour + depth per frame
+    DevBuf targetPool;                    // colour + depth of every frame
     DevBuf pos, col, uv, nrm, info, states, texTable;
-    std::vector<DevBuf> texData;
-    std::vector<DevBuf> pixData;   // decoded pfDrawPixels images
+    DevBuf texPool;                       // texels of every texture (RGBA8 or RGBA float)
+    DevBuf pixPool;                       // decoded pfDrawPixels images
+    DevBuf stage;                         // raw texels / pixel images before decoding
+    DevBuf passTab, tileDescs;            // per-pass tables (PassInfo::d*)
     DevBuf   points, pointScr;     // PFFpoint of all frames, their projections
     DevBuf   lines, lineScr;       // PFFline of all frames, their clipped screen forms
     uint32_t numIn = 0;
@@ -3037,6 +3036,56 @@ struct PFDjob {
 };
 
 namespace {
+
+template <typename F>
+void for_each_buf(PFDjob* J, F f) {
+    DevBuf* bufs[] = { &J->dFrames, &J->targetPool, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states,
+                       &J->texTable, &J->texPool, &J->pixPool, &J->stage, &J->passTab, &J->tileDescs, &J->points,
+                       &J->pointScr, &J->lines, &J->lineScr, &J->cnt, &J->off, &J->blockRT, &J->blockPairs, &J->rts,
+                       &J->rtl, &J->keysA, &J->valsA, &J->keysB, &J->valsB, &J->scanBlocks, &J->scanTotal, &J->hist,
+                       &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList,
+                       &J->clipCount, &J->clearSnap, &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams,
+                       &J->lightParams, &J->image, &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
+    for (DevBuf* b : bufs) f(*b);
+}
+
+constexpr size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Makes `b` hold at least `bytes`.  Without an arena the buffer grows by
+// hipMalloc (after the device has finished with the old block); with one,
+// a new block is carved from the arena, and running out of it is an error.
+int ensure(PFDjob* J, DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return 0;
+    const size_t want = align256(bytes < 256 ? 256 : bytes);
+    if (J->arena) {
+        Arena& A = *J->arena;
+        if (want > A.cap - A.used) {
+            snprintf(g_err, sizeof g_err, "%s: the frame needs more than the %zu-byte static scratch "
+                     "(%zu bytes in use, %zu more asked)", A.owner, A.cap, A.used, want);
+            return PFD_E_NOMEM;
+        }
+        b.p = A.base + A.used;
+        b.bytes = want;
+        b.borrowed = true;
+        A.used += want;
+        return 0;
+    }
+    if (b.p && !b.borrowed) {
+        // Kernels already queued may still read the old block.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(b.p);
+    }
+    b = DevBuf();
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) { set_error("hipMalloc", e); b.p = nullptr; return PFD_E_NOMEM; }
+    b.bytes = want;
+    return 0;
+}
+
+void release(DevBuf& b) {
+    if (b.p && !b.borrowed) (void)hipFree(b.p);
+    b = DevBuf();
+}
 
 uint32_t read_total(PFDjob* J, int* rc) {
     uint32_t v = 0;
@@ -3078,20 +3127,20 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (P.numPassTris == 0 && P.segs.empty()) return 0;
     const uint32_t n = P.numPassTris;
     const uint32_t nb = grid_for(n, 256);
-    if ((rc = ensure(J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J->off, (size_t)n * 4 + 4))) return rc;
-    if ((rc = ensure(J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J->blockPairs, (size_t)nb * 4 + 4))) return rc;
-    if ((rc = ensure(J->scanTotal, 16))) return rc;
-    if ((rc = ensure(J->clipList, (size_t)n * 16 + 16)) || (rc = ensure(J->clipCount, 16))) return rc;
-    if ((rc = ensure(J->clearSnap, J->frames.size() * 8))) return rc;
+    if ((rc = ensure(J, J->cnt, (size_t)n * 4 + 4)) || (rc = ensure(J, J->off, (size_t)n * 4 + 4))) return rc;
+    if ((rc = ensure(J, J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J, J->blockPairs, (size_t)nb * 4 + 4))) return rc;
+    if ((rc = ensure(J, J->scanTotal, 16))) return rc;
+    if ((rc = ensure(J, J->clipList, (size_t)n * 16 + 16)) || (rc = ensure(J, J->clipCount, 16))) return rc;
+    if ((rc = ensure(J, J->clearSnap, J->frames.size() * 8))) return rc;
     const size_t fragSlots = (size_t)J->numTiles * 4;
     const size_t passIdx = (size_t)(&P - J->passes.data());
     if (discover) {
         // Slots no wave writes (TS = 8 uses one of four) must read as zero.
-        if ((rc = ensure(J->tileFrags, fragSlots * J->passes.size() * 4 + 4))) return rc;
+        if ((rc = ensure(J, J->tileFrags, fragSlots * J->passes.size() * 4 + 4))) return rc;
         hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(fragSlots, 256 * 8)), dim3(256), 0, J->stream,
                            (uint32_t*)J->tileFrags.p + passIdx * fragSlots, (uint32_t)fragSlots, 0u);
     }
-    if ((rc = ensure(J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J->tileEnd, (size_t)J->numTiles * 4 + 4)))
+    if ((rc = ensure(J, J->tileBegin, (size_t)J->numTiles * 4 + 4)) || (rc = ensure(J, J->tileEnd, (size_t)J->numTiles * 4 + 4)))
         return rc;
     if (!discover) {
         // sizes known from the sizing run; buffers already hold them
@@ -3105,10 +3154,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.info = (const uint32_t*)J->info.p;
     sa.states = (const PFFstate*)J->states.p;
     sa.texs = (const DTex*)J->texTable.p;
-    sa.segs = (const DSeg*)P.dSegs.p;
+    sa.segs = P.dSegs;
     sa.frames = (const DFrame*)J->dFrames.p;
     sa.numSegs = (uint32_t)P.segs.size();
-    sa.blockSeg = (const uint32_t*)P.dBlockSeg.p;
+    sa.blockSeg = P.dBlockSeg;
     sa.numPassTris = n;
     sa.tileShift = J->tileShift;
     sa.numTiles = J->numTiles;
@@ -3144,11 +3193,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         P.numPairs = tot[1];
     }
     const uint32_t numRT = P.numRT, numPairs = P.numPairs;
-    if ((rc = ensure(J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
-    if (J->anyPhong && (rc = ensure(J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
-    if (J->diff && (rc = ensure(J->rtd, (size_t)numRT * sizeof(RTriD) + sizeof(RTriD)))) return rc;
+    if ((rc = ensure(J, J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
+    if (J->anyPhong && (rc = ensure(J, J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
+    if (J->diff && (rc = ensure(J, J->rtd, (size_t)numRT * sizeof(RTriD) + sizeof(RTriD)))) return rc;
     const size_t pb = (size_t)numPairs * 4 + 4;
-    if ((rc = ensure(J->keysA, pb)) || (rc = ensure(J->valsA, pb)) || (rc = ensure(J->keysB, pb)) || (rc = ensure(J->valsB, pb)))
+    if ((rc = ensure(J, J->keysA, pb)) || (rc = ensure(J, J->valsA, pb)) || (rc = ensure(J, J->keysB, pb)) || (rc = ensure(J, J->valsB, pb)))
         return rc;
     sa.rts = (RTri*)J->rts.p;
     sa.rtl = (RTriL*)J->rtl.p;
@@ -3162,18 +3211,24 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         // while the others are set up (both read the slots only).
         if (fusedScan) hipLaunchKernelGGL(k_scan_local<true>, dim3(nb), dim3(256), 0, J->stream, sa);
         else hipLaunchKernelGGL(k_scan_local<false>, dim3(nb), dim3(256), 0, J->stream, sa);
-        HIP_TRY(hipEventRecord(J->fork, J->stream), PFD_E_LAUNCH);
-        HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
+        // A job without a side stream runs the clipper after k_setup<1>.
+        hipStream_t clipStream = J->side ? J->side : J->stream;
+        if (J->side) {
+            HIP_TRY(hipEventRecord(J->fork, J->stream), PFD_E_LAUNCH);
+            HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
+        }
         if (J->anyLit) {
-            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
+            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
         } else {
-            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
-            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, J->side, sa);
+            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
+            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
             hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
         }
-        HIP_TRY(hipEventRecord(J->join, J->side), PFD_E_LAUNCH);
-        HIP_TRY(hipStreamWaitEvent(J->stream, J->join, 0), PFD_E_LAUNCH);
+        if (J->side) {
+            HIP_TRY(hipEventRecord(J->join, J->side), PFD_E_LAUNCH);
+            HIP_TRY(hipStreamWaitEvent(J->stream, J->join, 0), PFD_E_LAUNCH);
+        }
     }
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
@@ -3186,7 +3241,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         while (bits < 32 && (1u << bits) < J->numTiles) bits++;
         const uint32_t rb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;
         const uint32_t nh = rb * 256;
-        if ((rc = ensure(J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J->histScan, (size_t)nh * 4 + 4))) return rc;
+        if ((rc = ensure(J, J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J, J->histScan, (size_t)nh * 4 + 4))) return rc;
         uint32_t* kIn = keys; uint32_t* vIn = vals;
         uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
         // 8-bit digits (10-bit ones, which would sort 17..20-bit keys in two
@@ -3217,7 +3272,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.rtl = (const RTriL*)J->rtl.p;
     ra.states = (const PFFstate*)J->states.p;
     ra.tileFrame = (const uint32_t*)J->dTileFrame.p;
-    ra.tileDesc = (const DTileFrame*)P.dTileDesc.p;
+    ra.tileDesc = P.dTileDesc;
     ra.tileBegin = (const uint32_t*)J->tileBegin.p;
     ra.tileEnd = (const uint32_t*)J->tileEnd.p;
     ra.vals = vals;
@@ -3233,13 +3288,13 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     }
     if (!P.pix.empty())
         hipLaunchKernelGGL(k_draw_pixels, dim3(P.pixGrid, (uint32_t)P.pix.size()), dim3(256), 0, J->stream,
-                           (const DPixOp*)P.dPix.p);
+                           P.dPix);
     if (!P.pts.empty())
         hipLaunchKernelGGL(k_points, dim3(P.ptTiles, (uint32_t)P.pts.size()), dim3(256), 0, J->stream,
-                           (const DPtOp*)P.dPts.p, (const DPointScr*)J->pointScr.p);
+                           P.dPts, (const DPointScr*)J->pointScr.p);
     if (!P.lns.empty())
         hipLaunchKernelGGL(k_lines, dim3(P.lnTiles, (uint32_t)P.lns.size()), dim3(256), 0, J->stream,
-                           (const DLnOp*)P.dLns.p, (const DLineScr*)J->lineScr.p);
+                           P.dLns, (const DLineScr*)J->lineScr.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[3], J->stream), PFD_E_LAUNCH);
     P.sized = true;
@@ -3295,8 +3350,10 @@ void layout_tiles(PFDjob* J) {
     J->numTiles = base;
 }
 
+// Frame table, tile -> frame map and the per-pass tile descriptors; all
+// depend on the tile size, so they are uploaded again when it changes.
 int upload_frames_table(PFDjob* J) {
-    int rc = ensure(J->dFrames, J->frames.size() * sizeof(DFrame));
+    int rc = ensure(J, J->dFrames, J->frames.size() * sizeof(DFrame));
     if (rc) return rc;
     HIP_TRY(hipMemcpy(J->dFrames.p, J->frames.data(), J->frames.size() * sizeof(DFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
     std::vector<uint32_t> tf(J->numTiles ? J->numTiles : 1, 0u);
@@ -3304,24 +3361,30 @@ int upload_frames_table(PFDjob* J) {
         const DFrame& F = J->frames[f];
         for (uint32_t t = 0; t < F.tilesX * F.tilesY; t++) tf[F.tileBase + t] = f;
     }
-    if ((rc = ensure(J->dTileFrame, tf.size() * 4))) return rc;
+    if ((rc = ensure(J, J->dTileFrame, tf.size() * 4))) return rc;
     HIP_TRY(hipMemcpy(J->dTileFrame.p, tf.data(), tf.size() * 4, hipMemcpyHostToDevice), PFD_E_LAUNCH);
-    for (auto& P : J->passes) {
-        std::vector<DTileFrame> d(J->frames.size());
-        for (size_t f = 0; f < d.size(); f++) {
-            memset(&d[f], 0, sizeof d[f]);
-            d[f].F = J->frames[f];
+    const size_t nf = J->frames.size();
+    std::vector<DTileFrame> d(J->passes.size() * nf);
+    memset(d.data(), 0, d.size() * sizeof(DTileFrame));
+    for (size_t p = 0; p < J->passes.size(); p++) {
+        const PassInfo& P = J->passes[p];
+        for (size_t f = 0; f < nf; f++) {
+            DTileFrame& t = d[p * nf + f];
+            t.F = J->frames[f];
             const int32_t si = f < P.frameSeg.size() ? P.frameSeg[f] : -1;
             if (si < 0) continue;
             const DSeg& sg = P.segs[si];
-            d[f].active = 1;
-            d[f].clear = sg.clear;
-            d[f].clearColor = sg.clearColor;
-            d[f].clearDepth = sg.clearDepth;
+            t.active = 1;
+            t.clear = sg.clear;
+            t.clearColor = sg.clearColor;
+            t.clearDepth = sg.clearDepth;
         }
-        if ((rc = ensure(P.dTileDesc, d.size() * sizeof(DTileFrame)))) return rc;
-        HIP_TRY(hipMemcpy(P.dTileDesc.p, d.data(), d.size() * sizeof(DTileFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
     }
+    if (d.empty()) return 0;
+    if ((rc = ensure(J, J->tileDescs, d.size() * sizeof(DTileFrame)))) return rc;
+    HIP_TRY(hipMemcpy(J->tileDescs.p, d.data(), d.size() * sizeof(DTileFrame), hipMemcpyHostToDevice), PFD_E_LAUNCH);
+    for (size_t p = 0; p < J->passes.size(); p++)
+        J->passes[p].dTileDesc = (const DTileFrame*)J->tileDescs.p + p * nf;
     return 0;
 }
 
@@ -3364,50 +3427,119 @@ int pfdSetDefaultConfig(const PFDconfig* cfg) {
     return PFD_OK;
 }
 
-int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out) {
-    if (!out || !frames || count == 0) { snprintf(g_err, sizeof g_err, "pfdJobCreate: bad arguments"); return PFD_E_INVALID; }
-    *out = nullptr;
-    int rc = pfdInit();
-    if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// A job with no frames yet: its stream and, with `withSide`, the side
+// stream and the fork/join events of the clipper.
+int job_shell(PFDjob** out, bool withSide) {
     PFDjob* J = new PFDjob();
-    J->cfg = cfg ? *cfg : g_defaultCfg;
-    J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
     hipError_t e = hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { set_error("hipStreamCreate", e); delete J; return PFD_E_NODEVICE; }
-    if ((e = hipStreamCreateWithFlags(&J->side, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&J->fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&J->join, hipEventDisableTiming)) != hipSuccess) {
+    if (e == hipSuccess && withSide) {
+        if ((e = hipStreamCreateWithFlags(&J->side, hipStreamNonBlocking)) == hipSuccess
+            && (e = hipEventCreateWithFlags(&J->fork, hipEventDisableTiming)) == hipSuccess)
+            e = hipEventCreateWithFlags(&J->join, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
         set_error("hipStreamCreate", e);
         pfdJobDestroy(J);
         return PFD_E_NODEVICE;
     }
+    *out = J;
+    return PFD_OK;
+}
 
-    // Concatenate triangle streams, states and textures of all frames.
+// Forgets the frames of a job and keeps its streams and device buffers
+// (which the next plan reuses; an arena job's buffers are re-carved).
+void job_reset(PFDjob* J) {
+    if (J->stream) (void)hipStreamSynchronize(J->stream);
+    if (J->side) (void)hipStreamSynchronize(J->side);
+    drop_graph(J);
+    J->src.clear();
+    J->leadClear.clear();
+    J->srcTex.clear();
+    J->srcStates.clear();
+    J->frames.clear();
+    J->passes.clear();
+    J->numIn = 0;
+    J->hasUV = J->hasNrm = J->anyPhong = J->anyLit = false;
+    J->numTiles = 0;
+    J->sortedInB = 0;
+    J->lastFragments = J->totalRT = J->totalPairs = 0;
+    J->diff = J->diffRan = false;
+    J->numVertices = 0;
+    J->paramTex = -1;
+    if (J->arena) {
+        J->arena->used = 0;
+        for_each_buf(J, [](DevBuf& b) { b = DevBuf(); });
+    }
+}
+
+#define PLAN_FAIL(code, ...) do { snprintf(g_err, sizeof g_err, __VA_ARGS__); return (code); } while (0)
+
+// Uploads the frames into the job's buffers and builds its passes:
+// triangle streams, states and textures of all frames concatenated, render
+// targets, decoded pfDrawPixels images, points and lines (projected once),
+// and per pass its segments (one per frame at most) and pixel / point /
+// line operations.  The job's configuration must be set.
+int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count) {
+    int rc = 0;
+    hipError_t e = hipSuccess;
+
+    // Sizes and validation, before anything is copied.
     uint32_t totalTris = 0, totalStates = 0;
     bool anyUV = false, anyNrm = false;
+    size_t targetBytes = 0, texBytes = 0, pixBytes = 0, stageBytes = 0;
     for (uint32_t f = 0; f < count; f++) {
         const PFFframe& fr = frames[f];
-        if (!fr.color || !fr.depth || fr.width == 0 || fr.height == 0) {
-            snprintf(g_err, sizeof g_err, "frame %u has no render target", f);
-            pfdJobDestroy(J);
-            return PFD_E_INVALID;
-        }
+        if (!fr.color || !fr.depth || fr.width == 0 || fr.height == 0)
+            PLAN_FAIL(PFD_E_INVALID, "frame %u has no render target", f);
+        if (fr.tris.count && (!fr.tris.position || !fr.tris.color || !fr.tris.info))
+            PLAN_FAIL(PFD_E_INVALID, "frame %u: triangle streams missing", f);
+        for (size_t t = 0; t < fr.tris.count; t++)
+            if ((fr.tris.info[t] & PFF_STATE_MASK) >= fr.numStates) PLAN_FAIL(PFD_E_INVALID, "bad state index");
         totalTris += fr.tris.count;
         totalStates += fr.numStates;
         anyUV |= fr.tris.texcoord != nullptr;
         anyNrm |= fr.tris.normal != nullptr;
+        targetBytes += 2 * align256((size_t)fr.width * fr.height * 4);
+        for (uint32_t k = 0; k < fr.numTextures; k++) {
+            const PFFtexture& T = fr.textures[k];
+            if (T.width > 0xFFFFu || T.height > 0xFFFFu)        // raster records keep 16-bit sides
+                PLAN_FAIL(PFD_E_INVALID, "texture %ux%u: sides above 65535 are not supported", T.width, T.height);
+            const uint32_t srcBytes = T.format == PFF_TEX_SOURCE ? pff_source_texel_bytes(T.source) : 0u;
+            if (T.format > PFF_TEX_SOURCE || (T.format == PFF_TEX_SOURCE && srcBytes == 0u))
+                PLAN_FAIL(PFD_E_INVALID, "texture format %u / source 0x%x not supported", T.format, T.source);
+            const size_t texels = (size_t)T.width * T.height;
+            texBytes += align256(texels * (T.format == PFF_TEX_RGBAF32 ? 16 : 4));
+            if (T.format == PFF_TEX_SOURCE) stageBytes = std::max(stageBytes, texels * srcBytes + 4);
+        }
+        for (uint32_t c = 0; c < fr.numCmds; c++) {
+            const PFFcmd& cmd = fr.cmds[c];
+            if (cmd.kind != PFF_CMD_PIXELS) continue;
+            if (!fr.pixelOps || cmd.first >= fr.numPixelOps) PLAN_FAIL(PFD_E_INVALID, "bad pixel command");
+            const PFFpixels& op = fr.pixelOps[cmd.first];
+            const uint32_t bpp = pff_source_texel_bytes(op.source);
+            if (bpp == 0u || op.width == 0u || op.height == 0u || !op.pixels) PLAN_FAIL(PFD_E_INVALID, "bad pixel image");
+            const size_t texels = (size_t)op.width * op.height;
+            pixBytes += align256(texels * 4);
+            stageBytes = std::max(stageBytes, texels * bpp + 4);
+        }
     }
+    if ((rc = ensure(J, J->targetPool, targetBytes)) || (rc = ensure(J, J->texPool, texBytes))
+        || (pixBytes && (rc = ensure(J, J->pixPool, pixBytes))) || (stageBytes && (rc = ensure(J, J->stage, stageBytes))))
+        return rc;
+
     J->numIn = totalTris;
     J->hasUV = anyUV;
     J->hasNrm = anyNrm;
-    std::vector<float> hpos((size_t)totalTris * 12);
-    std::vector<uint32_t> hcol((size_t)totalTris * 3), hinfo(totalTris);
-    std::vector<float> huv(anyUV ? (size_t)totalTris * 6 : 0), hnrm(anyNrm ? (size_t)totalTris * 9 : 0);
     std::vector<PFFstate> hstates(totalStates ? totalStates : 1);
     std::vector<DTex> htex;
     std::vector<PFFpoint> hpts;
     std::vector<PFFline> hlns;
     uint32_t triBase = 0, stateBase = 0, ptBase = 0, lnBase = 0;
+    size_t targetOff = 0, texOff = 0, pixOff = 0;
     J->src.assign(frames, frames + count);
     J->leadClear.resize(count);
     for (uint32_t f = 0; f < count; f++)
@@ -3415,65 +3547,38 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     J->srcTex.assign(frames[0].textures, frames[0].textures + frames[0].numTextures);
     J->srcStates.assign(frames[0].states, frames[0].states + frames[0].numStates);
     J->frames.resize(count);
-    J->targets.resize(2 * count);
     for (uint32_t f = 0; f < count; f++) {
         const PFFframe& fr = frames[f];
-        size_t nt = fr.tris.count;
-        if (nt) {
-            memcpy(&hpos[(size_t)triBase * 12], fr.tris.position, nt * 12 * sizeof(float));
-            memcpy(&hcol[(size_t)triBase * 3], fr.tris.color, nt * 3 * sizeof(uint32_t));
-            if (anyUV && fr.tris.texcoord) memcpy(&huv[(size_t)triBase * 6], fr.tris.texcoord, nt * 6 * sizeof(float));
-            if (anyNrm && fr.tris.normal) memcpy(&hnrm[(size_t)triBase * 9], fr.tris.normal, nt * 9 * sizeof(float));
-            for (size_t t = 0; t < nt; t++) {
-                uint32_t si = fr.tris.info[t] & PFF_STATE_MASK;
-                if (si >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad state index"); pfdJobDestroy(J); return PFD_E_INVALID; }
-                hinfo[triBase + t] = (fr.tris.info[t] & ~PFF_STATE_MASK) | (si + stateBase);
-            }
-        }
-        uint32_t texBase = (uint32_t)htex.size();
+        const size_t nt = fr.tris.count;
+        const uint32_t texBase = (uint32_t)htex.size();
         for (uint32_t k = 0; k < fr.numTextures; k++) {
             const PFFtexture& T = fr.textures[k];
-            if (T.width > 0xFFFFu || T.height > 0xFFFFu) {       // raster records keep 16-bit sides
-                snprintf(g_err, sizeof g_err, "texture %ux%u: sides above 65535 are not supported", T.width, T.height);
-                pfdJobDestroy(J);
-                return PFD_E_INVALID;
-            }
             DTex d;
             memset(&d, 0, sizeof d);
             d.w = T.width; d.h = T.height; d.format = T.format; d.wrap = T.wrap; d.filter = T.filter;
             const size_t texels = (size_t)T.width * T.height;
-            const uint32_t srcBytes = T.format == PFF_TEX_SOURCE ? pff_source_texel_bytes(T.source) : 0u;
-            if (T.format > PFF_TEX_SOURCE || (T.format == PFF_TEX_SOURCE && srcBytes == 0u)) {
-                snprintf(g_err, sizeof g_err, "texture format %u / source 0x%x not supported", T.format, T.source);
-                pfdJobDestroy(J);
-                return PFD_E_INVALID;
-            }
-            size_t bytes = texels * (T.format == PFF_TEX_RGBAF32 ? 16 : 4);
-            J->texData.emplace_back();
-            if ((rc = ensure(J->texData.back(), bytes ? bytes : 4))) { pfdJobDestroy(J); return rc; }
+            const size_t bytes = texels * (T.format == PFF_TEX_RGBAF32 ? 16 : 4);
+            uint8_t* dst = (uint8_t*)J->texPool.p + texOff;
+            texOff += align256(bytes);
             if (bytes && T.format == PFF_TEX_SOURCE) {
                 // Raw texels + 4 zero bytes (the getters' word reads run up
                 // to 3 bytes past the last texel), decoded to RGBA8.
-                DevBuf raw;
-                const size_t rawBytes = texels * srcBytes;
-                if ((rc = ensure(raw, rawBytes + 4))) { pfdJobDestroy(J); return rc; }
-                e = hipMemcpy(raw.p, T.pixels, rawBytes, hipMemcpyHostToDevice);
-                if (e == hipSuccess) e = hipMemset((uint8_t*)raw.p + rawBytes, 0, 4);
+                const size_t rawBytes = texels * pff_source_texel_bytes(T.source);
+                e = hipMemcpy(J->stage.p, T.pixels, rawBytes, hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = hipMemsetAsync((uint8_t*)J->stage.p + rawBytes, 0, 4, J->stream);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL(k_decode_texels, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, J->stream,
-                                       (const uint8_t*)raw.p, texels, T.source, (uint32_t*)J->texData.back().p);
+                                       (const uint8_t*)J->stage.p, texels, T.source, (uint32_t*)dst);
                     e = hipGetLastError();
                 }
-                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
-                release(raw);
-                if (e != hipSuccess) { set_error("texture decode", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
-                d.format = PFF_TEX_RGBA8;
+                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);   // the stage is reused
+                if (e != hipSuccess) { set_error("texture decode", e); return PFD_E_LAUNCH; }
             } else if (bytes) {
-                e = hipMemcpy(J->texData.back().p, T.pixels, bytes, hipMemcpyHostToDevice);
-                if (e != hipSuccess) { set_error("texture upload", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+                e = hipMemcpy(dst, T.pixels, bytes, hipMemcpyHostToDevice);
+                if (e != hipSuccess) { set_error("texture upload", e); return PFD_E_LAUNCH; }
             }
             if (T.format == PFF_TEX_SOURCE) d.format = PFF_TEX_RGBA8;
-            d.pixels = J->texData.back().p;
+            d.pixels = dst;
             htex.push_back(d);
         }
         for (uint32_t s = 0; s < fr.numStates; s++) {
@@ -3490,20 +3595,21 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         memset(&D, 0, sizeof D);
         D.W = fr.width;
         D.H = fr.height;
-        size_t npx = (size_t)fr.width * fr.height;
-        if ((rc = ensure(J->targets[2 * f], npx * 4)) || (rc = ensure(J->targets[2 * f + 1], npx * 4))) { pfdJobDestroy(J); return rc; }
-        D.color = (uint32_t*)J->targets[2 * f].p;
-        D.depth = (float*)J->targets[2 * f + 1].p;
+        const size_t npx = (size_t)fr.width * fr.height;
+        D.color = (uint32_t*)((uint8_t*)J->targetPool.p + targetOff);
+        targetOff += align256(npx * 4);
+        D.depth = (float*)((uint8_t*)J->targetPool.p + targetOff);
+        targetOff += align256(npx * 4);
 
         for (uint32_t k = 0; k < fr.numPoints; k++) {
             PFFpoint q = fr.points[k];
-            if (q.state >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad point state"); pfdJobDestroy(J); return PFD_E_INVALID; }
+            if (q.state >= fr.numStates) PLAN_FAIL(PFD_E_INVALID, "bad point state");
             q.state += stateBase;
             hpts.push_back(q);
         }
         for (uint32_t k = 0; k < fr.numLines; k++) {
             PFFline q = fr.lines[k];
-            if (q.state >= fr.numStates) { snprintf(g_err, sizeof g_err, "bad line state"); pfdJobDestroy(J); return PFD_E_INVALID; }
+            if (q.state >= fr.numStates) PLAN_FAIL(PFD_E_INVALID, "bad line state");
             q.state += stateBase;
             if (k == 0) q.groupStart = 1u;             // groups never span frames
             hlns.push_back(q);
@@ -3524,6 +3630,14 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             open = false;
             hasDraw = false;
         };
+        // A command that runs alone after the raster kernel of its own pass.
+        auto own_pass = [&](void) -> PassInfo& {
+            flush_seg();
+            cur.triFirst = cur.triLast = triBase;
+            open = true;
+            if (J->passes.size() <= segNo) J->passes.emplace_back();
+            return J->passes[segNo];
+        };
         for (uint32_t c = 0; c < fr.numCmds; c++) {
             const PFFcmd& cmd = fr.cmds[c];
             if (cmd.kind == PFF_CMD_CLEAR) {
@@ -3533,48 +3647,27 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 cur.clearDepth = cmd.clearDepth;
                 open = true;
             } else if (cmd.kind == PFF_CMD_LINES) {
-                // A pass of its own; the run's lines are applied in order.
-                if (!fr.lines || cmd.first > cmd.last || cmd.last > fr.numLines) {
-                    snprintf(g_err, sizeof g_err, "bad lines command");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
+                // The run's lines are applied in order.
+                if (!fr.lines || cmd.first > cmd.last || cmd.last > fr.numLines) PLAN_FAIL(PFD_E_INVALID, "bad lines command");
                 DLnOp d;
                 memset(&d, 0, sizeof d);
                 d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
                 d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
                 d.first = lnBase + cmd.first; d.last = lnBase + cmd.last;
-                flush_seg();
-                cur.triFirst = cur.triLast = triBase;
-                open = true;
-                if (J->passes.size() <= segNo) J->passes.emplace_back();
-                J->passes[segNo].lns.push_back(d);
+                own_pass().lns.push_back(d);
                 flush_seg();
             } else if (cmd.kind == PFF_CMD_POINTS) {
-                // A pass of its own; the run's points are applied in order.
-                if (!fr.points || cmd.first > cmd.last || cmd.last > fr.numPoints) {
-                    snprintf(g_err, sizeof g_err, "bad points command");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
+                // The run's points are applied in order.
+                if (!fr.points || cmd.first > cmd.last || cmd.last > fr.numPoints) PLAN_FAIL(PFD_E_INVALID, "bad points command");
                 DPtOp d;
                 memset(&d, 0, sizeof d);
                 d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
                 d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
                 d.first = ptBase + cmd.first; d.last = ptBase + cmd.last;
-                flush_seg();
-                cur.triFirst = cur.triLast = triBase;
-                open = true;
-                if (J->passes.size() <= segNo) J->passes.emplace_back();
-                J->passes[segNo].pts.push_back(d);
+                own_pass().pts.push_back(d);
                 flush_seg();
             } else if (cmd.kind == PFF_CMD_FOG) {
-                // A pass of its own, like a pfDrawPixels call.
-                if (!fr.pixelOps || cmd.first >= fr.numPixelOps) {
-                    snprintf(g_err, sizeof g_err, "bad fog command");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
+                if (!fr.pixelOps || cmd.first >= fr.numPixelOps) PLAN_FAIL(PFD_E_INVALID, "bad fog command");
                 const PFFpixels& op = fr.pixelOps[cmd.first];
                 DPixOp d;
                 memset(&d, 0, sizeof d);
@@ -3583,68 +3676,42 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
                 d.fogMode = op.fogMode; d.fogColor = op.fogColor;
                 d.fogDensity = op.fogDensity; d.fogStart = op.fogStart; d.fogEnd = op.fogEnd;
                 d.xMin = 0; d.yMin = 0; d.xMax = (int32_t)D.W - 1; d.yMax = (int32_t)D.H - 1;
-                flush_seg();
-                cur.triFirst = cur.triLast = triBase;
-                open = true;
-                if (J->passes.size() <= segNo) J->passes.emplace_back();
-                J->passes[segNo].pix.push_back(d);
-                cur.pixOp = (uint32_t)J->passes[segNo].pix.size();
+                PassInfo& P = own_pass();
+                P.pix.push_back(d);
+                cur.pixOp = (uint32_t)P.pix.size();
                 flush_seg();
             } else if (cmd.kind == PFF_CMD_PIXELS) {
-                // A pass of its own: no triangles, the call runs after the
-                // pass's raster kernel, so it lands between the draws
-                // recorded before and after it.
-                if (!fr.pixelOps || cmd.first >= fr.numPixelOps) {
-                    snprintf(g_err, sizeof g_err, "bad pixel command");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
+                // No triangles: the call runs after the pass's raster
+                // kernel, so it lands between the draws recorded before and
+                // after it.  Validated in the size pass above.
                 const PFFpixels& op = fr.pixelOps[cmd.first];
                 const uint32_t bpp = pff_source_texel_bytes(op.source);
-                if (bpp == 0u || op.width == 0u || op.height == 0u || !op.pixels) {
-                    snprintf(g_err, sizeof g_err, "bad pixel image");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
                 const size_t texels = (size_t)op.width * op.height;
-                J->pixData.emplace_back();
-                DevBuf raw;
-                if ((rc = ensure(J->pixData.back(), texels * 4)) || (rc = ensure(raw, texels * bpp + 4))) {
-                    release(raw);
-                    pfdJobDestroy(J);
-                    return rc;
-                }
-                e = hipMemcpy(raw.p, op.pixels, texels * bpp, hipMemcpyHostToDevice);
+                uint32_t* img = (uint32_t*)((uint8_t*)J->pixPool.p + pixOff);
+                pixOff += align256(texels * 4);
+                e = hipMemcpy(J->stage.p, op.pixels, texels * bpp, hipMemcpyHostToDevice);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL(k_decode_image, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, J->stream,
-                                       (const uint8_t*)raw.p, texels, op.source, (uint32_t*)J->pixData.back().p);
+                                       (const uint8_t*)J->stage.p, texels, op.source, img);
                     e = hipGetLastError();
                 }
-                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
-                release(raw);
-                if (e != hipSuccess) { set_error("pixel image decode", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+                if (e == hipSuccess) e = hipStreamSynchronize(J->stream);   // the stage is reused
+                if (e != hipSuccess) { set_error("pixel image decode", e); return PFD_E_LAUNCH; }
                 DPixOp d;
                 memset(&d, 0, sizeof d);
-                d.texels = (const uint32_t*)J->pixData.back().p;
+                d.texels = img;
                 d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
                 d.w = op.width; d.h = op.height;
                 d.xScreen = op.xScreen; d.yScreen = op.yScreen;
                 d.xMin = op.xMin; d.yMin = op.yMin; d.xMax = op.xMax; d.yMax = op.yMax;
                 d.zPos = op.zPos; d.invXLen = op.invXLen; d.invYLen = op.invYLen;
                 d.flags = op.flags; d.depthFunc = op.depthFunc; d.blendMode = op.blendMode;
-                flush_seg();
-                cur.triFirst = cur.triLast = triBase;
-                open = true;
-                if (J->passes.size() <= segNo) J->passes.emplace_back();
-                J->passes[segNo].pix.push_back(d);
-                cur.pixOp = (uint32_t)J->passes[segNo].pix.size();
+                PassInfo& P = own_pass();
+                P.pix.push_back(d);
+                cur.pixOp = (uint32_t)P.pix.size();
                 flush_seg();
             } else if (cmd.kind == PFF_CMD_DRAW) {
-                if (cmd.first > cmd.last || cmd.last > fr.tris.count) {
-                    snprintf(g_err, sizeof g_err, "bad draw range");
-                    pfdJobDestroy(J);
-                    return PFD_E_INVALID;
-                }
+                if (cmd.first > cmd.last || cmd.last > fr.tris.count) PLAN_FAIL(PFD_E_INVALID, "bad draw range");
                 if (hasDraw && cur.triLast != triBase + cmd.first) flush_seg();
                 if (!hasDraw) { cur.triFirst = triBase + cmd.first; cur.triLast = triBase + cmd.first; }
                 cur.triLast = triBase + cmd.last;
@@ -3658,9 +3725,10 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         ptBase += fr.numPoints;
         lnBase += fr.numLines;
     }
+
     // Device copies of the inputs.
     auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
-        int r = ensure(b, bytes ? bytes : 4);
+        int r = ensure(J, b, bytes ? bytes : 4);
         if (r) return r;
         if (bytes) {
             hipError_t ee = hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice);
@@ -3668,43 +3736,70 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
         }
         return 0;
     };
-    if ((rc = up(J->pos, hpos.data(), hpos.size() * 4)) || (rc = up(J->col, hcol.data(), hcol.size() * 4))
-        || (rc = up(J->info, hinfo.data(), hinfo.size() * 4)) || (rc = up(J->uv, huv.data(), huv.size() * 4))
-        || (rc = up(J->nrm, hnrm.data(), hnrm.size() * 4))
-        || (rc = up(J->states, hstates.data(), hstates.size() * sizeof(PFFstate)))
-        || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex)))) {
-        pfdJobDestroy(J);
-        return rc;
+    if (count == 1) {
+        // One frame: its streams go up as they are (state indices need no
+        // offset and were checked above).
+        const PFFtriangles& T = frames[0].tris;
+        const size_t n = T.count;
+        if ((rc = up(J->pos, T.position, n * 48)) || (rc = up(J->col, T.color, n * 12))
+            || (rc = up(J->info, T.info, n * 4)) || (rc = up(J->uv, T.texcoord, T.texcoord ? n * 24 : 0))
+            || (rc = up(J->nrm, T.normal, T.normal ? n * 36 : 0)))
+            return rc;
+    } else {
+        std::vector<float> hpos((size_t)totalTris * 12);
+        std::vector<uint32_t> hcol((size_t)totalTris * 3), hinfo(totalTris);
+        std::vector<float> huv(anyUV ? (size_t)totalTris * 6 : 0), hnrm(anyNrm ? (size_t)totalTris * 9 : 0);
+        uint32_t tb = 0, sb = 0;
+        for (uint32_t f = 0; f < count; f++) {
+            const PFFframe& fr = frames[f];
+            const size_t nt = fr.tris.count;
+            if (nt) {
+                memcpy(&hpos[(size_t)tb * 12], fr.tris.position, nt * 12 * sizeof(float));
+                memcpy(&hcol[(size_t)tb * 3], fr.tris.color, nt * 3 * sizeof(uint32_t));
+                if (anyUV && fr.tris.texcoord) memcpy(&huv[(size_t)tb * 6], fr.tris.texcoord, nt * 6 * sizeof(float));
+                if (anyNrm && fr.tris.normal) memcpy(&hnrm[(size_t)tb * 9], fr.tris.normal, nt * 9 * sizeof(float));
+                for (size_t t = 0; t < nt; t++)
+                    hinfo[tb + t] = (fr.tris.info[t] & ~PFF_STATE_MASK) | ((fr.tris.info[t] & PFF_STATE_MASK) + sb);
+            }
+            tb += (uint32_t)nt;
+            sb += fr.numStates;
+        }
+        if ((rc = up(J->pos, hpos.data(), hpos.size() * 4)) || (rc = up(J->col, hcol.data(), hcol.size() * 4))
+            || (rc = up(J->info, hinfo.data(), hinfo.size() * 4)) || (rc = up(J->uv, huv.data(), huv.size() * 4))
+            || (rc = up(J->nrm, hnrm.data(), hnrm.size() * 4)))
+            return rc;
     }
+    if ((rc = up(J->states, hstates.data(), hstates.size() * sizeof(PFFstate)))
+        || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex))))
+        return rc;
     if (!hpts.empty()) {
         // Points are projected once; every replay reads the projections.
         if ((rc = up(J->points, hpts.data(), hpts.size() * sizeof(PFFpoint)))
-            || (rc = ensure(J->pointScr, hpts.size() * sizeof(DPointScr)))) {
-            pfdJobDestroy(J);
+            || (rc = ensure(J, J->pointScr, hpts.size() * sizeof(DPointScr))))
             return rc;
-        }
         const uint32_t np = (uint32_t)hpts.size();
         hipLaunchKernelGGL(k_point_setup, dim3((np + 255) / 256), dim3(256), 0, J->stream, (const PFFpoint*)J->points.p,
                            np, (const PFFstate*)J->states.p, (DPointScr*)J->pointScr.p);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
-        if (e != hipSuccess) { set_error("point setup", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+        if (e != hipSuccess) { set_error("point setup", e); return PFD_E_LAUNCH; }
     }
     if (!hlns.empty()) {
         // Lines are projected and clipped once; every replay reads them.
         if ((rc = up(J->lines, hlns.data(), hlns.size() * sizeof(PFFline)))
-            || (rc = ensure(J->lineScr, hlns.size() * sizeof(DLineScr)))) {
-            pfdJobDestroy(J);
+            || (rc = ensure(J, J->lineScr, hlns.size() * sizeof(DLineScr))))
             return rc;
-        }
         const uint32_t nl = (uint32_t)hlns.size();
         hipLaunchKernelGGL(k_line_setup, dim3((nl + 255) / 256), dim3(256), 0, J->stream, (const PFFline*)J->lines.p,
                            nl, (const PFFstate*)J->states.p, (DLineScr*)J->lineScr.p);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(J->stream);
-        if (e != hipSuccess) { set_error("line setup", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
+        if (e != hipSuccess) { set_error("line setup", e); return PFD_E_LAUNCH; }
     }
-    // Per-pass tables.
+    // Per-pass tables, all passes in one buffer (one copy).
+    std::vector<size_t> offs;
+    size_t tabBytes = 0;
+    auto place = [&](size_t bytes) { offs.push_back(tabBytes); tabBytes += align256(bytes); };
     for (auto& P : J->passes) {
         uint32_t acc = 0;
         P.frameSeg.assign(count, -1);
@@ -3714,12 +3809,11 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             P.frameSeg[P.segs[s].frame] = (int32_t)s;
         }
         P.numPassTris = acc;
-        std::vector<uint32_t> bs((acc + 255) / 256 + 1, 0u);
-        for (size_t b = 0, sgi = 0; b < bs.size(); b++) {
+        P.blockSeg.assign((acc + 255) / 256 + 1, 0u);
+        for (size_t b = 0, sgi = 0; b < P.blockSeg.size(); b++) {
             while (sgi + 1 < P.segs.size() && P.segs[sgi + 1].passTriBase <= b * 256) sgi++;
-            bs[b] = (uint32_t)sgi;
+            P.blockSeg[b] = (uint32_t)sgi;
         }
-        if ((rc = up(P.dBlockSeg, bs.data(), bs.size() * 4))) { pfdJobDestroy(J); return rc; }
         if (!P.pix.empty()) {
             uint64_t maxArea = 1;
             for (const DPixOp& d : P.pix) {
@@ -3729,28 +3823,60 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
             }
             const uint64_t g = (maxArea + 255) / 256;
             P.pixGrid = (uint32_t)(g < 4096 ? g : 4096);
-            if ((rc = up(P.dPix, P.pix.data(), P.pix.size() * sizeof(DPixOp)))) { pfdJobDestroy(J); return rc; }
         }
-        if (!P.pts.empty()) {
-            uint32_t maxTiles = 1;
-            for (const DPtOp& d : P.pts) maxTiles = std::max(maxTiles, d.tilesX * d.tilesY);
-            P.ptTiles = maxTiles;
-            if ((rc = up(P.dPts, P.pts.data(), P.pts.size() * sizeof(DPtOp)))) { pfdJobDestroy(J); return rc; }
+        P.ptTiles = 1;
+        for (const DPtOp& d : P.pts) P.ptTiles = std::max(P.ptTiles, d.tilesX * d.tilesY);
+        P.lnTiles = 1;
+        for (const DLnOp& d : P.lns) P.lnTiles = std::max(P.lnTiles, d.tilesX * d.tilesY);
+        place(P.segs.size() * sizeof(DSeg));
+        place(P.blockSeg.size() * 4);
+        place(P.pix.size() * sizeof(DPixOp));
+        place(P.pts.size() * sizeof(DPtOp));
+        place(P.lns.size() * sizeof(DLnOp));
+    }
+    if (tabBytes) {
+        std::vector<uint8_t> tab(tabBytes, 0);
+        size_t k = 0;
+        for (auto& P : J->passes) {
+            memcpy(&tab[offs[k + 0]], P.segs.data(), P.segs.size() * sizeof(DSeg));
+            memcpy(&tab[offs[k + 1]], P.blockSeg.data(), P.blockSeg.size() * 4);
+            if (!P.pix.empty()) memcpy(&tab[offs[k + 2]], P.pix.data(), P.pix.size() * sizeof(DPixOp));
+            if (!P.pts.empty()) memcpy(&tab[offs[k + 3]], P.pts.data(), P.pts.size() * sizeof(DPtOp));
+            if (!P.lns.empty()) memcpy(&tab[offs[k + 4]], P.lns.data(), P.lns.size() * sizeof(DLnOp));
+            k += 5;
         }
-        if (!P.lns.empty()) {
-            uint32_t maxTiles = 1;
-            for (const DLnOp& d : P.lns) maxTiles = std::max(maxTiles, d.tilesX * d.tilesY);
-            P.lnTiles = maxTiles;
-            if ((rc = up(P.dLns, P.lns.data(), P.lns.size() * sizeof(DLnOp)))) { pfdJobDestroy(J); return rc; }
-        }
-        if ((rc = up(P.dSegs, P.segs.data(), P.segs.size() * sizeof(DSeg)))
-            || (rc = up(P.dFrameSeg, P.frameSeg.data(), P.frameSeg.size() * 4))) {
-            pfdJobDestroy(J);
-            return rc;
+        if ((rc = up(J->passTab, tab.data(), tabBytes))) return rc;
+        const uint8_t* base = (const uint8_t*)J->passTab.p;
+        k = 0;
+        for (auto& P : J->passes) {
+            P.dSegs = (const DSeg*)(base + offs[k + 0]);
+            P.dBlockSeg = (const uint32_t*)(base + offs[k + 1]);
+            P.dPix = (const DPixOp*)(base + offs[k + 2]);
+            P.dPts = (const DPtOp*)(base + offs[k + 3]);
+            P.dLns = (const DLnOp*)(base + offs[k + 4]);
+            k += 5;
         }
     }
     layout_tiles(J);
-    if ((rc = upload_frames_table(J))) { pfdJobDestroy(J); return rc; }
+    return upload_frames_table(J);
+}
+
+#undef PLAN_FAIL
+
+}  // namespace
+
+extern "C" {
+
+int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out) {
+    if (!out || !frames || count == 0) { snprintf(g_err, sizeof g_err, "pfdJobCreate: bad arguments"); return PFD_E_INVALID; }
+    *out = nullptr;
+    int rc = pfdInit();
+    if (rc) return rc;
+    PFDjob* J = nullptr;
+    if ((rc = job_shell(&J, true))) return rc;
+    J->cfg = cfg ? *cfg : g_defaultCfg;
+    J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
+    if ((rc = job_plan(J, frames, count))) { pfdJobDestroy(J); return rc; }
     *out = J;
     return PFD_OK;
 }
@@ -3898,12 +4024,7 @@ int pfdJobStats(PFDjob* J, PFDstats* st) {
     st->numTiles = J->numTiles;
     st->tileShift = J->tileShift;
     size_t bytes = 0;
-    DevBuf* bufs[] = { &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off, &J->rts,
-                       &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
-                       &J->scanBlocks, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd };
-    for (DevBuf* b : bufs) bytes += b->bytes;
-    for (auto& b : J->targets) bytes += b.bytes;
-    for (auto& b : J->texData) bytes += b.bytes;
+    for_each_buf(J, [&](DevBuf& b) { bytes += b.bytes; });
     st->deviceBytes = bytes;
     if (J->tileBegin.p && J->numTiles) {
         std::vector<uint32_t> b(J->numTiles), e(J->numTiles);
@@ -3962,22 +4083,9 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
 void pfdJobDestroy(PFDjob* J) {
     if (!J) return;
     if (J->stream) (void)hipStreamSynchronize(J->stream);
-    drop_graph(J);
-    DevBuf* bufs[] = { &J->dFrames, &J->pos, &J->col, &J->uv, &J->nrm, &J->info, &J->states, &J->texTable, &J->cnt, &J->off,
-                       &J->rts, &J->rtl, &J->blockRT, &J->blockPairs, &J->keysA, &J->valsA, &J->keysB, &J->valsB,
-                       &J->scanBlocks, &J->scanTotal, &J->hist, &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList, &J->clipCount, &J->clearSnap,
-                       &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams, &J->lightParams, &J->image,
-                       &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
-    for (DevBuf* b : bufs) release(*b);
-    for (auto& b : J->targets) release(b);
-    for (auto& b : J->texData) release(b);
-    for (auto& b : J->pixData) release(b);
-    release(J->points);
-    release(J->pointScr);
-    release(J->lines);
-    release(J->lineScr);
-    for (auto& P : J->passes) { release(P.dSegs); release(P.dFrameSeg); release(P.dBlockSeg); release(P.dTileDesc); release(P.dPix); release(P.dPts); release(P.dLns); }
     if (J->side) (void)hipStreamSynchronize(J->side);
+    drop_graph(J);
+    for_each_buf(J, [](DevBuf& b) { release(b); });
     if (J->fork) (void)hipEventDestroy(J->fork);
     if (J->join) (void)hipEventDestroy(J->join);
     if (J->side) (void)hipStreamDestroy(J->side);
@@ -3985,12 +4093,46 @@ void pfdJobDestroy(PFDjob* J) {
     delete J;
 }
 
-int pfdExecuteFrames(const PFFframe* frames, uint32_t count) {
-    PFDjob* J = nullptr;
-    int rc = pfdJobCreate(frames, count, nullptr, &J);
-    if (rc) return rc;
-    if (!(rc = pfdJobUploadTargets(J)) && !(rc = pfdJobLaunch(J)) && !(rc = pfdJobSync(J)))
+}  // extern "C"
+
+namespace {
+
+// Plans and runs frames once on a reset job: upload, one discovering run
+// (no graph), download.
+int run_one_shot(PFDjob* J, const PFFframe* frames, uint32_t count) {
+    J->cfg = g_defaultCfg;
+    J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
+    int rc = job_plan(J, frames, count);
+    if (!rc && !(rc = pfdJobUploadTargets(J)) && !(rc = run_all(J, true)) && !(rc = pfdJobSync(J)))
         rc = pfdJobDownload(J);
+    return rc;
+}
+
+// The job of pfdExecuteFrame(s), kept between calls: its streams, device
+// buffers (which only grow) and host tables serve the next frame, so a
+// frame of the same size as an earlier one costs no hipMalloc, hipFree or
+// stream creation.  A call made while another thread runs one uses a job
+// of its own.
+std::mutex g_oneShotLock;
+PFDjob*    g_oneShot = nullptr;
+
+}  // namespace
+
+extern "C" {
+
+int pfdExecuteFrames(const PFFframe* frames, uint32_t count) {
+    if (!frames || count == 0) { snprintf(g_err, sizeof g_err, "pfdExecuteFrames: bad arguments"); return PFD_E_INVALID; }
+    int rc = pfdInit();
+    if (rc) return rc;
+    std::unique_lock<std::mutex> lk(g_oneShotLock, std::try_to_lock);
+    if (lk.owns_lock()) {
+        if (!g_oneShot && (rc = job_shell(&g_oneShot, true))) return rc;
+        job_reset(g_oneShot);
+        return run_one_shot(g_oneShot, frames, count);
+    }
+    PFDjob* J = nullptr;
+    if ((rc = job_shell(&J, true))) return rc;
+    rc = run_one_shot(J, frames, count);
     pfdJobDestroy(J);
     return rc;
 }
@@ -4058,13 +4200,13 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     if (rc) return rc;
     J->diff = true;
     const size_t npx = (size_t)frame->width * frame->height;
-    if ((rc = ensure(J->winner, npx * 4)) || (rc = ensure(J->image, npx * 16)) || (rc = ensure(J->dImage, npx * 16))) {
+    if ((rc = ensure(J, J->winner, npx * 4)) || (rc = ensure(J, J->image, npx * 16)) || (rc = ensure(J, J->dImage, npx * 16))) {
         pfdJobDestroy(J);
         return rc;
     }
     if (frame->tris.vertexId && frame->tris.count) {
         const size_t bytes = (size_t)frame->tris.count * 12;
-        if ((rc = ensure(J->vid, bytes))) { pfdJobDestroy(J); return rc; }
+        if ((rc = ensure(J, J->vid, bytes))) { pfdJobDestroy(J); return rc; }
         hipError_t e = hipMemcpy(J->vid.p, frame->tris.vertexId, bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) { set_error("vertex ids", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
     }
@@ -4095,14 +4237,14 @@ int diff_args(PFDjob* J, const PFDdiffParams* p, DiffArgs& a) {
             texels = (const float*)T.pixels;
         }
         const size_t bytes = (size_t)T.width * T.height * 16;
-        if ((rc = ensure(J->texParams, bytes))) return rc;
+        if ((rc = ensure(J, J->texParams, bytes))) return rc;
         HIP_TRY(hipMemcpyAsync(J->texParams.p, texels, bytes, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
         a.paramTex = p->texture;
         a.texParams = (const float*)J->texParams.p;
     }
     if (p->vertexColors && p->numVertices) {
         const size_t bytes = (size_t)p->numVertices * 16;
-        if ((rc = ensure(J->colParams, bytes))) return rc;
+        if ((rc = ensure(J, J->colParams, bytes))) return rc;
         HIP_TRY(hipMemcpyAsync(J->colParams.p, p->vertexColors, bytes, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
         a.colParams = (const float*)J->colParams.p;
         a.numVertices = p->numVertices;
@@ -4124,7 +4266,7 @@ int diff_args(PFDjob* J, const PFDdiffParams* p, DiffArgs& a) {
             }
         if (p->lightDiffuse) memcpy(lp, p->lightDiffuse, PFF_MAX_LIGHTS * 3 * sizeof(float));
         if (p->lightSpecular) memcpy(lp + PFF_MAX_LIGHTS * 3, p->lightSpecular, PFF_MAX_LIGHTS * 3 * sizeof(float));
-        if ((rc = ensure(J->lightParams, sizeof lp))) return rc;
+        if ((rc = ensure(J, J->lightParams, sizeof lp))) return rc;
         HIP_TRY(hipMemcpyAsync(J->lightParams.p, lp, sizeof lp, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);   // lp is on this stack frame
         a.lightParams = (const float*)J->lightParams.p;
@@ -4164,18 +4306,18 @@ int diff_backward_launch(PFDjob* J, DiffArgs& a, bool wantTex, bool wantCol, boo
     if (wantTex && a.paramTex >= 0) {
         const PFFtexture& T = J->srcTex[a.paramTex];
         const size_t n = (size_t)T.width * T.height * 4;
-        if ((rc = ensure(J->gTex, n * 4))) return rc;
+        if ((rc = ensure(J, J->gTex, n * 4))) return rc;
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256 * 8)), dim3(256), 0, J->stream, (float*)J->gTex.p, n);
         a.gTex = (float*)J->gTex.p;
     }
     if (wantCol && a.colParams) {
         const size_t n = (size_t)a.numVertices * 4;
-        if ((rc = ensure(J->gCol, n * 4))) return rc;
+        if ((rc = ensure(J, J->gCol, n * 4))) return rc;
         hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256 * 8)), dim3(256), 0, J->stream, (float*)J->gCol.p, n);
         a.gCol = (float*)J->gCol.p;
     }
     if (wantLight) {
-        if ((rc = ensure(J->gLightPart, (size_t)blocks * DIFF_LIGHT_VALS * 4)) || (rc = ensure(J->gLight, DIFF_LIGHT_VALS * 4)))
+        if ((rc = ensure(J, J->gLightPart, (size_t)blocks * DIFF_LIGHT_VALS * 4)) || (rc = ensure(J, J->gLight, DIFF_LIGHT_VALS * 4)))
             return rc;
         a.gLightPart = (float*)J->gLightPart.p;
     }
