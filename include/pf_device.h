@@ -87,6 +87,22 @@ int pfdExecuteFrame(const PFFframe* frame);
 /* Same for several frames (distinct render targets) in one batched job. */
 int pfdExecuteFrames(const PFFframe* frames, uint32_t count);
 
+/* pfdExecuteFrame without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * around HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Every device buffer of the frame -- its uploaded inputs and
+ * render target, raster records, tile pairs, sort and scan buffers and
+ * counters -- is carved from a static device array of PFD_SCRATCH_BYTES
+ * (128 MiB) per GPU, which the library holds from its set-up (pfdInit)
+ * until the process ends; it calls no hipMalloc or hipFree.  A frame that
+ * needs more fails with PFD_E_NOMEM and a message naming the limit, before
+ * its raster pass (a 1920x1080 frame of 200k textured triangles needs about
+ * 75 MB).  Calls on one GPU share that array, which keeps the last call's
+ * contents; overlapping calls would race on it, so the library serialises
+ * the calls of a process with a lock.  Results equal pfdExecuteFrame's. */
+#define PFD_SCRATCH_BYTES_MIB 128
+int pfdExecuteFrameNoAlloc(const PFFframe* frame);
+
 /* Resident jobs, for replay and measurement. */
 int  pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out);
 int  pfdJobSetConfig(PFDjob* job, const PFDconfig* cfg);

@@ -130,6 +130,7 @@ def lib():
                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.pfdJobFrameCount.argtypes = [C.c_void_p]
         L.pfdExecuteFrame.argtypes = [C.c_void_p]
+        L.pfdExecuteFrameNoAlloc.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

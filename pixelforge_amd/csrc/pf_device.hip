@@ -2956,8 +2956,9 @@ struct DevBuf {
     bool   borrowed = false;   // carved from an arena: never freed
 };
 
-// Device memory a job carves its buffers from instead of calling hipMalloc.
-// Bump allocation; reset when the job is reset.
+// Device memory a job carves its buffers from instead of calling hipMalloc
+// (the static scratch array of pfdExecuteFrameNoAlloc).  Bump allocation;
+// reset when the job is reset.
 struct Arena {
     uint8_t* base = nullptr;
     size_t   cap = 0, used = 0;
@@ -2989,6 +2990,16 @@ struct PassInfo {
 };
 
 }  // namespace
+
+// Device memory of pfdExecuteFrameNoAlloc: every buffer of its frames
+// (inputs, render target, raster records, tile pairs, sort and scan
+// buffers, counters) is carved from this array, so that entry point never
+// allocates.  Each GPU holds its own copy until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (no_alloc_setup); the array's name is not a device
+// address on the host.
+#define PFD_SCRATCH_BYTES ((size_t)PFD_SCRATCH_BYTES_MIB << 20)
+static __device__ __align__(256) uint8_t scratch[PFD_SCRATCH_BYTES];
 
 // Every device buffer of a job is a member of this struct (no vectors of
 // buffers), so a job can be reset and planned again for the next frame
@@ -3393,6 +3404,41 @@ void drop_graph(PFDjob* J) {
     if (J->graph) { (void)hipGraphDestroy(J->graph); J->graph = nullptr; }
 }
 
+int job_shell(PFDjob** out, bool withSide);
+
+// State of pfdExecuteFrameNoAlloc, per device: the address of its copy of
+// `scratch`, the arena over it and a job (host tables and one stream) made
+// once at set-up.  Guarded by g_noAllocLock.
+constexpr int PFD_MAX_DEVICES = 64;
+uint8_t*   g_scratchPtr[PFD_MAX_DEVICES];
+Arena      g_noAllocArena[PFD_MAX_DEVICES];
+PFDjob*    g_noAllocJob[PFD_MAX_DEVICES];
+std::mutex g_noAllocLock;
+
+// Set-up of the current device `dev` for pfdExecuteFrameNoAlloc, once per
+// device, with g_noAllocLock held.
+int no_alloc_setup(int dev) {
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) { set_error("hipGetSymbolAddress(scratch)", e); return PFD_E_NODEVICE; }
+        g_scratchPtr[dev] = (uint8_t*)p;
+    }
+    if (!g_noAllocJob[dev]) {
+        PFDjob* J = nullptr;
+        int rc = job_shell(&J, false);    // the clipper then runs on the job's one stream
+        if (rc) return rc;
+        Arena& A = g_noAllocArena[dev];
+        A.base = g_scratchPtr[dev];
+        A.cap = PFD_SCRATCH_BYTES;
+        A.used = 0;
+        A.owner = "pfdExecuteFrameNoAlloc";
+        J->arena = &A;
+        g_noAllocJob[dev] = J;
+    }
+    return PFD_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -3413,6 +3459,11 @@ int pfdInit(void) {
     }
     e = hipSetDevice(0);
     if (e != hipSuccess) { set_error("hipSetDevice", e); g_initState = -1; return PFD_E_NODEVICE; }
+    {
+        std::lock_guard<std::mutex> lk(g_noAllocLock);
+        const int rc = no_alloc_setup(0);
+        if (rc) { g_initState = -1; return rc; }
+    }
     g_initState = 1;
     return PFD_OK;
 }
@@ -4138,6 +4189,25 @@ int pfdExecuteFrames(const PFFframe* frames, uint32_t count) {
 }
 
 int pfdExecuteFrame(const PFFframe* frame) { return pfdExecuteFrames(frame, 1); }
+
+int pfdExecuteFrameNoAlloc(const PFFframe* frame) {
+    if (!frame) { snprintf(g_err, sizeof g_err, "pfdExecuteFrameNoAlloc: no frame"); return PFD_E_INVALID; }
+    int rc = pfdInit();
+    if (rc) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), PFD_E_NODEVICE);
+    if (dev < 0 || dev >= PFD_MAX_DEVICES) {
+        snprintf(g_err, sizeof g_err, "pfdExecuteFrameNoAlloc: device %d out of range", dev);
+        return PFD_E_NODEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_noAllocLock);
+    if ((rc = no_alloc_setup(dev))) return rc;
+    PFDjob* J = g_noAllocJob[dev];
+    job_reset(J);                       // also waits for the previous call's work on the stream
+    rc = run_one_shot(J, frame, 1);
+    if (rc) (void)hipStreamSynchronize(J->stream);
+    return rc;
+}
 
 static PFDjob* g_captured = nullptr;
 
