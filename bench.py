@@ -25,6 +25,12 @@ Scenes are recorded once through the PixelForge C API into jobs that stay
 resident on the GPU (inputs in HBM); each step replays the whole kernel
 chain (setup, binning, tile raster) as one HIP graph.
 
+`end_to_end` times the 1080p frame as an application of the reference sees
+it: pfClear, the scene's API calls and pfFinish (upload of the recorded
+frame, the kernels, colour and depth copied back), replayed from a call
+tape so that the scene generator is not timed; `cpu_baseline` times the
+reference on the same tape.
+
     python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
 """
 import argparse
@@ -164,17 +170,53 @@ def stats(job):
 
 
 def cpu_frame_rate(params_list, frames=3):
-    """The reference's own CPU path (oracle/_ref) on the given scenes:
-    median seconds per pass over all of them (API calls included)."""
-    for p in params_list:
-        pf.render(p, REF_SO)                        # warm-up
-    times = []
-    for _ in range(frames):
-        t0 = time.perf_counter()
-        for p in params_list:
-            pf.render(p, REF_SO)
-        times.append(time.perf_counter() - t0)
-    return statistics.median(times)
+    """The reference's own CPU path (oracle/_ref) on the given scenes,
+    replayed from call tapes (the scene generator is not timed): median
+    seconds per pass over all of them, each frame being pfClear and the
+    scene's draws, after one warm-up pass."""
+    tapes = [pf.Tape(p) for p in params_list]
+    try:
+        per = np.zeros((len(tapes), frames))
+        for i, t in enumerate(tapes):
+            _, ms, err = t.time(frames + 1, REF_SO)
+            if err:
+                raise RuntimeError(f"reference frame failed ({err})")
+            per[i] = ms[1:]
+        return float(np.median(per.sum(axis=0))) * 1e-3
+    finally:
+        for t in tapes:
+            t.close()
+
+
+def end_to_end(params, frames, expect):
+    """The drop-in path as an application sees it: per frame, the scene's
+    API calls (pfClear, state, pfBegin / pfColor / pfTexCoord / pfVertex /
+    pfEnd) then pfFinish, which uploads the recorded frame, runs it and
+    copies colour and depth back into the application's buffers.  Replayed
+    from a call tape, so the scene generator is not timed.  The first frame
+    is a warm-up.  The host part alone is timed with an executor that
+    discards the frame.  The last frame must equal `expect` (the resident
+    job's image)."""
+    tape = pf.Tape(params)
+    try:
+        img, ms, err = tape.time(frames + 1)
+        check(err, "end-to-end frame")
+        if not np.array_equal(img.ravel(), expect):
+            raise RuntimeError(f"end-to-end frame differs from the resident job's on "
+                               f"{np.count_nonzero(img.ravel() != expect)} pixels")
+        lib = pf.lib()
+        cb = pf.EXECUTOR(lambda frame, user: 0)
+        lib.pfxSetExecutor(C.cast(cb, C.c_void_p), None)
+        try:
+            _, rec_ms, err = tape.time(frames + 1)
+        finally:
+            lib.pfxSetExecutor(None, None)
+        check(err, "recording")
+        return {"ms_per_frame": float(np.median(ms[1:])), "frames": frames,
+                "host_record_ms": float(np.median(rec_ms[1:])), "api_calls_per_frame": tape.calls,
+                "equals_resident_job": True}
+    finally:
+        tape.close()
 
 
 # ---------------------------------------------------------------------------
@@ -216,13 +258,91 @@ def bench_frame(args):
     }
     if not args.no_cpu:
         out["replays"]["first_equals_oracle"] = oracle_check(params, first[0])
+    e2e = end_to_end(params, args.e2e_frames, first[0])
+    e2e["fragments_per_s"] = frags / (e2e["ms_per_frame"] * 1e-3)
+    out["end_to_end"] = e2e
     if not args.no_cpu and os.path.exists(REF_SO):
         med = cpu_frame_rate([params], frames=5)
         out["cpu_baseline"] = {
             "value": frags / med, "unit": "fragments/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": "reference",
-            "sample": f"1 warm-up + median of 5 full frames of the same scene (API calls incl. scene generation), "
-                      f"reference built -O3 -mavx2 -fopenmp; median {med * 1e3:.1f} ms/frame"}
+            "sample": f"1 warm-up + median of 5 frames of the same scene replayed from a call tape (pfClear + "
+                      f"draws; scene generation not timed), reference built -O3 -mavx2 -fopenmp; "
+                      f"median {med * 1e3:.1f} ms/frame"}
+        e2e["cpu_reference_ms_per_frame"] = med * 1e3
+        e2e["speedup_vs_reference"] = med * 1e3 / e2e["ms_per_frame"]
+    else:
+        out["cpu_baseline"] = None
+    cap.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# 1b. the other single-frame configurations
+# ---------------------------------------------------------------------------
+
+def config1():
+    """800x600, 10,000 triangles with vertices uniform in NDC [-1, 1]^3 and
+    the default matrices (the reference's 2D path), random RGBA8 vertex
+    colours interpolated (smooth), depth test LESS, culling disabled, no
+    texture, no blending; the frame starts with pfClear."""
+    return pf.scene(kind=pf.RANDOM_NDC, width=800, height=600, ntris=10000, seed=1, cull=-1,
+                    depthFunc=pf.PF_LESS)
+
+
+def config3():
+    """3840x2160, 256 spheres of 31 x 63 x 2 = 3,906 triangles (999,936 in
+    all) with per-vertex normals, lit per fragment (Phong) by 4 point
+    lights, shininess 32; 512x512 RGBA8 texture, NEAREST, CLAMP_TO_EDGE;
+    back faces culled, depth test LESS; the frame starts with pfClear."""
+    return pf.scene(kind=pf.SPHERES, width=3840, height=2160, spheres=256, sphereLat=31, sphereLon=63, seed=3,
+                    lighting=2, numLights=4, texture=1, texW=512, texH=512, wrap=pf.PF_CLAMP_TO_EDGE,
+                    filter=pf.PF_NEAREST, cull=pf.PF_BACK)
+
+
+def bench_config(params, workload, tex_bytes, args, cpu_frames, e2e_frames):
+    """Resident replays of one frame (autotuned), phase times, the raster
+    kernel's roofline, the end-to-end frame and the reference's CPU time on
+    the same scene (call tape)."""
+    cap = pf.capture(params)
+    job = cap.job
+    lib = pf.lib()
+    check(lib.pfdJobUploadTargets(job), "upload")
+    cfg = configure(job, 0, 0)
+    elapsed, first, digest = verified_replays(cap, args.config_steps, 3)
+    st = stats(job)
+    frags = int(st.fragments)
+    setup_ms, bin_ms, raster_ms = phase_times(job, max(3, args.profile_runs // 4))
+    W, H = params.width, params.height
+    raster_bytes = W * H * 8 + tex_bytes
+    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9
+    ms = elapsed * 1e3 / args.config_steps
+    out = {
+        "value": frags / (ms * 1e-3), "unit": "fragments/s", "ms_per_frame": ms, "steps": args.config_steps,
+        "config": {"workload": workload, "fragments_per_frame": frags, "raster_triangles": int(st.rasterTriangles),
+                   "tile_pairs": int(st.tilePairs), "tile": 1 << int(st.tileShift),
+                   "raster_mode": RASTER_MODES[cfg.rasterMode]},
+        "phases_ms": {"setup": setup_ms, "binning": bin_ms, "raster": raster_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "kernel": "k_raster", "kernel_ms": raster_ms,
+                     "algorithmic_bytes": raster_bytes},
+        "replays": {"timed_equal_first": True, "frame_sha1": digest},
+    }
+    if e2e_frames:
+        e2e = end_to_end(params, e2e_frames, first[0])
+        e2e["fragments_per_s"] = frags / (e2e["ms_per_frame"] * 1e-3)
+        out["end_to_end"] = e2e
+    if not args.no_cpu and os.path.exists(REF_SO):
+        med = cpu_frame_rate([params], frames=cpu_frames)
+        out["cpu_baseline"] = {
+            "value": frags / med, "unit": "fragments/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "reference",
+            "sample": f"1 warm-up + median of {cpu_frames} frames of the same scene replayed from a call tape "
+                      f"(scene generation not timed), reference built -O3 -mavx2 -fopenmp; "
+                      f"median {med * 1e3:.1f} ms/frame"}
+        if e2e_frames:
+            out["end_to_end"]["cpu_reference_ms_per_frame"] = med * 1e3
+            out["end_to_end"]["speedup_vs_reference"] = med * 1e3 / out["end_to_end"]["ms_per_frame"]
     else:
         out["cpu_baseline"] = None
     cap.close()
@@ -272,7 +392,8 @@ def bench_batch(args):
             "value": len(sample) / med, "unit": "frames/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
             "kind": "reference",
             "sample": f"first {len(sample)} of the {len(params)} contexts, one after another (the reference has no "
-                      f"batched path), 1 warm-up + median of 3; {med * 1e3:.0f} ms per pass"}
+                      f"batched path), replayed from call tapes (scene generation not timed), 1 warm-up + median "
+                      f"of 3; {med * 1e3:.0f} ms per pass"}
     else:
         out["cpu_baseline"] = None
     cap.close()
@@ -374,6 +495,9 @@ def main():
     ap.add_argument("--diff-iters", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--e2e-frames", type=int, default=10, help="timed end-to-end frames (API calls + pfFinish)")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 800x600 and 4K single-frame workloads")
+    ap.add_argument("--config-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -382,7 +506,8 @@ def main():
 
     check(pf.lib().pfdInit(), "pfdInit")
     frame = bench_frame(args)
-    print(f"1080p: {frame['value']:.4g} fragments/s, {frame['ms_per_step']:.3f} ms/frame", file=sys.stderr, flush=True)
+    print(f"1080p: {frame['value']:.4g} fragments/s, {frame['ms_per_step']:.3f} ms/frame; end to end "
+          f"{frame['end_to_end']['ms_per_frame']:.2f} ms/frame", file=sys.stderr, flush=True)
     out = {
         "metric": "shaded fragments/s at 1080p (1 GPU) + batched frames/s; fwd+bwd time at 1e-3 grad err",
         "value": frame["value"],
@@ -401,7 +526,18 @@ def main():
         "cpu_baseline": frame["cpu_baseline"],
         "phases_ms": frame["phases_ms"],
         "record_s": frame["record_s"],
+        "end_to_end": frame["end_to_end"],
     }
+    if not args.no_configs:
+        out["config1"] = bench_config(config1(), "800x600, 10k random NDC triangles (2D path), smooth colours, LESS, "
+                                      "culling off", 0, args, cpu_frames=3, e2e_frames=args.e2e_frames)
+        print(f"config 1: {out['config1']['value']:.4g} fragments/s, {out['config1']['ms_per_frame']:.3f} ms/frame",
+              file=sys.stderr, flush=True)
+        out["config3"] = bench_config(config3(), "3840x2160, 256 spheres (1M triangles), Phong with 4 point lights, "
+                                      "512^2 RGBA8 NEAREST CLAMP, back faces culled", 512 * 512 * 4, args,
+                                      cpu_frames=2, e2e_frames=0)
+        print(f"config 3: {out['config3']['value']:.4g} fragments/s, {out['config3']['ms_per_frame']:.3f} ms/frame",
+              file=sys.stderr, flush=True)
     if not args.no_batch:
         out["batched"] = bench_batch(args)
         print(f"batched: {out['batched']['value']:.4g} frames/s", file=sys.stderr, flush=True)

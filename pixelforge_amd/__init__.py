@@ -152,6 +152,14 @@ def scenes():
         S.pfs_record.restype = C.c_int
         S.pfs_free_texture.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         S.pfs_free_texture.restype = None
+        S.pfs_tape_make.restype = C.c_void_p
+        S.pfs_tape_make.argtypes = [C.POINTER(SceneParams)]
+        S.pfs_tape_free.argtypes = [C.c_void_p]
+        S.pfs_tape_free.restype = None
+        S.pfs_tape_calls.argtypes = [C.c_void_p]
+        S.pfs_tape_calls.restype = C.c_uint64
+        S.pfs_tape_time.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        S.pfs_tape_time.restype = C.c_int
         _scenes = S
     return _scenes
 
@@ -189,6 +197,35 @@ def render_batch(params_list, path=LIB_PATH):
     ptrs = (C.c_void_p * n)(*[b.ctypes.data for b in bufs])
     err = scenes().pfs_render_batch(api(path), arr, n, ptrs)
     return [b[: p.height * p.width].reshape(p.height, p.width) for b, p in zip(bufs, params_list)], err
+
+
+class Tape:
+    """A scene's API calls recorded once (csrc/pf_scenes.c, call tapes), to
+    be replayed through a library: timing a replay measures the library, not
+    the scene generator."""
+
+    def __init__(self, params):
+        self.params = params
+        self.ptr = scenes().pfs_tape_make(C.byref(params))
+        if not self.ptr:
+            raise ValueError("scene kind or option not covered by call tapes")
+        self.calls = int(scenes().pfs_tape_calls(self.ptr))
+
+    def time(self, frames, path=LIB_PATH):
+        """Renders the scene `frames` times through the library at `path`
+        into an application colour buffer (pfClear + draws + pfFinish per
+        frame); returns (color uint32[H, W] of the last frame, per-frame
+        milliseconds, error code)."""
+        h, w = self.params.height, self.params.width
+        color = np.zeros((h * w + 16,), np.uint32)
+        ms = np.zeros(frames, np.float64)
+        err = scenes().pfs_tape_time(api(path), self.ptr, frames, color.ctypes.data, ms.ctypes.data)
+        return color[: h * w].reshape(h, w), ms, err
+
+    def close(self):
+        if self.ptr:
+            scenes().pfs_tape_free(self.ptr)
+            self.ptr = None
 
 
 def set_default_config(tile_shift=3, raster_mode=0):

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-C_FLAGS = ["-std=gnu99", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+C_FLAGS = ["-std=gnu99", "-O3", "-fPIC", "-fno-semantic-interposition", "-ffp-contract=off", "-fno-fast-math",
            "-Wall", "-Wextra", "-Wno-unused-parameter", "-fvisibility=hidden"]
 
 

@@ -168,7 +168,9 @@ typedef struct {
     uint32_t   diffNumTex;
 } Ctx;
 
-static __thread Ctx* G = NULL;        /* current context, one per thread */
+/* Current context, one per thread.  Initial-exec TLS: every API call reads
+ * it, and the default model for a shared library calls __tls_get_addr. */
+static __thread Ctx* G __attribute__((tls_model("initial-exec"))) = NULL;
 static PFXexecutor g_exec = NULL;
 static void* g_execUser = NULL;
 static PFXbatchExecutor g_batchExec = NULL;
@@ -504,44 +506,63 @@ static void push_cmd(Ctx* c, const PFFcmd* cmd)
     r->cmds[r->ncmd++] = *cmd;
 }
 
-/* Appends one triangle with the given culling passes (PFF_PASS_*). */
+/* Grows the triangle streams to hold at least one more triangle. */
+static int rec_grow(Rec* r)
+{
+    size_t nc = r->captri ? r->captri : 1024;
+    while (nc < r->ntri + 1) nc *= 2;
+    float* pos = realloc(r->pos, nc * 12 * sizeof(float));
+    if (pos) r->pos = pos;
+    uint32_t* col = realloc(r->col, nc * 3 * sizeof(uint32_t));
+    if (col) r->col = col;
+    float* uv = realloc(r->uv, nc * 6 * sizeof(float));
+    if (uv) r->uv = uv;
+    float* nrm = realloc(r->nrm, nc * 9 * sizeof(float));
+    if (nrm) r->nrm = nrm;
+    uint32_t* info = realloc(r->info, nc * sizeof(uint32_t));
+    if (info) r->info = info;
+    uint32_t* vid = realloc(r->vid, nc * 3 * sizeof(uint32_t));
+    if (vid) r->vid = vid;
+    if (!pos || !col || !uv || !nrm || !info || !vid) return 0;
+    r->captri = nc;
+    return 1;
+}
+
+/* Appends one triangle with the given culling passes (PFF_PASS_*).  Only
+ * the streams its state reads are written: texture coordinates for a
+ * textured state, normals for a lit one (the kernels and the oracle read
+ * them for no other triangle), colour-array indices once any corner of the
+ * frame has one (the triangles before it then get PFF_NO_VERTEX). */
 static void emit_tri(Ctx* c, const Vert* a, const Vert* b, const Vert* v3, uint32_t passes)
 {
     if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); return; }
     int si = current_state(c);
     if (si < 0) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
     Rec* r = &c->rec;
-    if (r->ntri + 1 > r->captri) {
-        size_t cap = r->captri;
-        size_t need = r->ntri + 1;
-        size_t nc = cap ? cap : 1024;
-        while (nc < need) nc *= 2;
-        float* pos = realloc(r->pos, nc * 12 * sizeof(float));
-        if (pos) r->pos = pos;
-        uint32_t* col = realloc(r->col, nc * 3 * sizeof(uint32_t));
-        if (col) r->col = col;
-        float* uv = realloc(r->uv, nc * 6 * sizeof(float));
-        if (uv) r->uv = uv;
-        float* nrm = realloc(r->nrm, nc * 9 * sizeof(float));
-        if (nrm) r->nrm = nrm;
-        uint32_t* info = realloc(r->info, nc * sizeof(uint32_t));
-        if (info) r->info = info;
-        uint32_t* vid = realloc(r->vid, nc * 3 * sizeof(uint32_t));
-        if (vid) r->vid = vid;
-        if (!pos || !col || !uv || !nrm || !info || !vid) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
-        r->captri = nc;
-    }
-    size_t t = r->ntri;
+    if (r->ntri + 1 > r->captri && !rec_grow(r)) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    const size_t t = r->ntri;
     const Vert* v[3] = { a, b, v3 };
+    float* pos = r->pos + 12 * t;
+    uint32_t* col = r->col + 3 * t;
     for (int k = 0; k < 3; k++) {
-        size_t vi = t * 3 + k;
-        memcpy(r->pos + 4 * vi, v[k]->position, 16);
-        r->col[vi] = pack(v[k]->color);
-        memcpy(r->uv + 2 * vi, v[k]->texcoord, 8);
-        memcpy(r->nrm + 3 * vi, v[k]->normal, 12);
-        r->vid[vi] = v[k]->vid;
-        if (v[k]->vid != PFF_NO_VERTEX) r->anyVid = 1;
+        memcpy(pos + 4 * k, v[k]->position, 16);
+        col[k] = pack(v[k]->color);
     }
+    const PFFstate* st = &r->states[si];
+    if (st->texture >= 0) {
+        float* uv = r->uv + 6 * t;
+        for (int k = 0; k < 3; k++) memcpy(uv + 2 * k, v[k]->texcoord, 8);
+    }
+    if (st->flags & PFF_LIGHTING) {
+        float* nrm = r->nrm + 9 * t;
+        for (int k = 0; k < 3; k++) memcpy(nrm + 3 * k, v[k]->normal, 12);
+    }
+    if (!r->anyVid && (a->vid != PFF_NO_VERTEX || b->vid != PFF_NO_VERTEX || v3->vid != PFF_NO_VERTEX)) {
+        memset(r->vid, 0xFF, t * 3 * sizeof(uint32_t));
+        r->anyVid = 1;
+    }
+    if (r->anyVid)
+        for (int k = 0; k < 3; k++) r->vid[3 * t + k] = v[k]->vid;
     r->info[t] = (uint32_t)si | passes;
     r->ntri++;
     if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_DRAW && r->cmds[r->ncmd - 1].last == t) {

@@ -27,7 +27,9 @@
  *                   several model-view transforms (pfNewList / pfEndList /
  *                   pfCallList), including a list recorded by calling another.
  */
+#define _GNU_SOURCE
 #include <dlfcn.h>
+#include <time.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -125,6 +127,7 @@ typedef struct {
     void (*Clear)(PFclearflag);
     void (*ClearColor)(PFubyte, PFubyte, PFubyte, PFubyte);
     void (*EnableLight)(PFsizei);
+    void (*DisableLight)(PFsizei);
     void (*Lightf)(PFsizei, PFenum, PFfloat);
     void (*Lightfv)(PFsizei, PFenum, const void*);
     void (*Materialf)(PFface, PFenum, PFfloat);
@@ -184,6 +187,7 @@ PFSapi* pfs_bind(const char* path)
     BIND(MatrixMode); BIND(LoadIdentity); BIND(Translatef); BIND(Rotatef); BIND(Frustum);
     BIND(CullFace); BIND(BlendFunc); BIND(DepthFunc); BIND(ShadeModel); BIND(LightModel);
     BIND(BindTexture); BIND(BindFramebuffer); BIND(Clear); BIND(ClearColor); BIND(EnableLight);
+    BIND(DisableLight);
     BIND(Lightf); BIND(Lightfv); BIND(Materialf); BIND(Materialfv); BIND(ColorMaterial);
     BIND(Begin); BIND(End); BIND(Vertex3f); BIND(Color4ub); BIND(TexCoord2f); BIND(Normal3f);
     BIND(GenFramebuffer); BIND(DeleteFramebuffer); BIND(GenTexture); BIND(DeleteTexture);
@@ -943,4 +947,277 @@ void pfs_free_texture(const PFSapi* a, PFtexture tex, void* pixels)
 {
     if (tex) a->DeleteTexture(&tex, 0);
     free(pixels);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Call tapes                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* A tape is the sequence of API calls one scene makes, with their
+ * arguments, generated once by running the scene code against a recorder.
+ * Replaying it through a library issues exactly those calls, so timing a
+ * replay measures the library (its API calls and pfFinish) and not the
+ * scene generator (random numbers, sinf).  Textures the scene creates are
+ * kept as slots on the tape and created in the library once, before the
+ * timed frames.  Tapes cover the scene kinds whose calls the recorder
+ * knows: random NDC and 3D triangles, height fields and spheres, with no
+ * pixel operations, points, lines, rects, fog, flushes or vertex arrays. */
+
+enum {
+    T_ENABLE = 1, T_DISABLE, T_CULLFACE, T_DEPTHFUNC, T_SHADEMODEL, T_BLENDFUNC, T_MATRIXMODE, T_CLEAR,
+    T_BEGIN, T_LIGHTMODEL, T_ENABLELIGHT, T_LOADIDENTITY, T_END, T_TRANSLATE, T_VERTEX3, T_NORMAL3,
+    T_ROTATE, T_FRUSTUM, T_TEXCOORD2, T_COLOR4UB, T_CLEARCOLOR, T_LIGHTFV, T_LIGHTF, T_MATERIALFV,
+    T_MATERIALF, T_COLORMATERIAL, T_BINDTEXTURE, T_TEXPARAM, T_DISABLELIGHT
+};
+
+#define TAPE_MAX_TEX 4
+
+typedef struct {
+    uint32_t* w;                 /* opcodes and arguments, floats as bits */
+    size_t    n, cap;
+    int       bad;               /* the scene made a call the recorder does not know */
+    uint32_t  ntex;
+    void*     texPixels[TAPE_MAX_TEX];
+    uint32_t  texW[TAPE_MAX_TEX], texH[TAPE_MAX_TEX];
+    int32_t   texFormat[TAPE_MAX_TEX], texType[TAPE_MAX_TEX];
+    PFtexture bound[TAPE_MAX_TEX];   /* the library's textures while the tape is bound */
+    size_t    body;              /* words of the scene's calls; an epilogue follows */
+    uint32_t  enabled;           /* PFstate bits the scene enables */
+    uint32_t  lights;            /* lights the scene enables, by bit */
+    uint32_t  width, height;
+    uint64_t  calls;             /* API calls per replay */
+} PFStape;
+
+static PFStape* g_tape;          /* the tape being recorded */
+
+static void tput(uint32_t v)
+{
+    PFStape* t = g_tape;
+    if (t->bad) return;
+    if (t->n == t->cap) {
+        size_t nc = t->cap ? t->cap * 2 : 4096;
+        uint32_t* w = (uint32_t*)realloc(t->w, nc * sizeof(uint32_t));
+        if (!w) { t->bad = 1; return; }
+        t->w = w;
+        t->cap = nc;
+    }
+    t->w[t->n++] = v;
+}
+
+static void tputf(float f) { uint32_t v; memcpy(&v, &f, 4); tput(v); }
+static void top(uint32_t op) { tput(op); g_tape->calls++; }
+
+static uint32_t tex_slot(PFtexture tex) { return (uint32_t)(uintptr_t)tex; }   /* slot + 1, 0: none */
+
+static void r_Enable(PFstate s) { top(T_ENABLE); tput((uint32_t)s); g_tape->enabled |= (uint32_t)s; }
+static void r_Disable(PFstate s) { top(T_DISABLE); tput((uint32_t)s); }
+static void r_CullFace(PFface f) { top(T_CULLFACE); tput((uint32_t)f); }
+static void r_DepthFunc(PFdepthmode m) { top(T_DEPTHFUNC); tput((uint32_t)m); }
+static void r_ShadeModel(PFshademode m) { top(T_SHADEMODEL); tput((uint32_t)m); }
+static void r_BlendFunc(PFblendmode m) { top(T_BLENDFUNC); tput((uint32_t)m); }
+static void r_MatrixMode(PFmatrixmode m) { top(T_MATRIXMODE); tput((uint32_t)m); }
+static void r_Clear(PFclearflag f) { top(T_CLEAR); tput((uint32_t)f); }
+static void r_Begin(PFdrawmode m) { top(T_BEGIN); tput((uint32_t)m); }
+static void r_LightModel(PFlightmode m) { top(T_LIGHTMODEL); tput((uint32_t)m); }
+static void r_EnableLight(PFsizei l) { top(T_ENABLELIGHT); tput((uint32_t)l); if (l < 32) g_tape->lights |= 1u << l; }
+static void r_LoadIdentity(void) { top(T_LOADIDENTITY); }
+static void r_End(void) { top(T_END); }
+static void r_Translatef(PFfloat x, PFfloat y, PFfloat z) { top(T_TRANSLATE); tputf(x); tputf(y); tputf(z); }
+static void r_Vertex3f(PFfloat x, PFfloat y, PFfloat z) { top(T_VERTEX3); tputf(x); tputf(y); tputf(z); }
+static void r_Normal3f(PFfloat x, PFfloat y, PFfloat z) { top(T_NORMAL3); tputf(x); tputf(y); tputf(z); }
+static void r_Rotatef(PFfloat a, PFfloat x, PFfloat y, PFfloat z) { top(T_ROTATE); tputf(a); tputf(x); tputf(y); tputf(z); }
+static void r_Frustum(PFfloat l, PFfloat r, PFfloat b, PFfloat t, PFfloat n, PFfloat f)
+{
+    top(T_FRUSTUM); tputf(l); tputf(r); tputf(b); tputf(t); tputf(n); tputf(f);
+}
+static void r_TexCoord2f(PFfloat u, PFfloat v) { top(T_TEXCOORD2); tputf(u); tputf(v); }
+static void r_Color4ub(PFubyte r, PFubyte g, PFubyte b, PFubyte a)
+{
+    top(T_COLOR4UB); tput((uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24));
+}
+static void r_ClearColor(PFubyte r, PFubyte g, PFubyte b, PFubyte a)
+{
+    top(T_CLEARCOLOR); tput((uint32_t)r | ((uint32_t)g << 8) | ((uint32_t)b << 16) | ((uint32_t)a << 24));
+}
+static void r_Lightfv(PFsizei l, PFenum p, const void* v)     /* three floats in every scene call */
+{
+    const float* f = (const float*)v;
+    top(T_LIGHTFV); tput((uint32_t)l); tput((uint32_t)p); tputf(f[0]); tputf(f[1]); tputf(f[2]);
+}
+static void r_Lightf(PFsizei l, PFenum p, PFfloat v) { top(T_LIGHTF); tput((uint32_t)l); tput((uint32_t)p); tputf(v); }
+static void r_Materialfv(PFface face, PFenum p, const void* v)
+{
+    const float* f = (const float*)v;
+    top(T_MATERIALFV); tput((uint32_t)face); tput((uint32_t)p); tputf(f[0]); tputf(f[1]); tputf(f[2]);
+}
+static void r_Materialf(PFface face, PFenum p, PFfloat v) { top(T_MATERIALF); tput((uint32_t)face); tput((uint32_t)p); tputf(v); }
+static void r_ColorMaterial(PFface face, PFenum m) { top(T_COLORMATERIAL); tput((uint32_t)face); tput((uint32_t)m); }
+static void r_BindTexture(PFtexture t) { top(T_BINDTEXTURE); tput(tex_slot(t)); }
+static void r_TextureParameter(PFtexture t, PFtexturewrap w, PFtexturefilter f)
+{
+    top(T_TEXPARAM); tput(tex_slot(t)); tput((uint32_t)w); tput((uint32_t)f);
+}
+static PFtexture r_GenTexture(void* px, PFsizei w, PFsizei h, PFpixelformat f, PFdatatype t)
+{
+    PFStape* tp = g_tape;
+    if (tp->ntex == TAPE_MAX_TEX) { tp->bad = 1; return NULL; }
+    uint32_t k = tp->ntex++;
+    tp->texPixels[k] = px;
+    tp->texW[k] = w; tp->texH[k] = h;
+    tp->texFormat[k] = f; tp->texType[k] = t;
+    return (PFtexture)(uintptr_t)(k + 1);
+}
+static PFerrcode r_GetError(void) { return PF_NO_ERROR; }
+
+/* Records scene `p` into a new tape; NULL for a scene kind or option the
+ * recorder does not cover. */
+PFStape* pfs_tape_make(const PFSparams* p)
+{
+    if (!p || !(p->kind == PFS_RANDOM_NDC || p->kind == PFS_RANDOM_3D || p->kind == PFS_HEIGHTFIELD
+                || p->kind == PFS_SPHERES))
+        return NULL;
+    if (p->pixelOps > 0 || p->fog || p->points > 0 || p->lines > 0 || p->rects > 0 || p->flushEvery > 0
+        || p->arrays || p->pointSize > 0 || p->lineWidth > 0 || p->polyPoint || p->polyLine)
+        return NULL;
+    PFStape* t = (PFStape*)calloc(1, sizeof(PFStape));
+    if (!t) return NULL;
+    PFSapi r;
+    memset(&r, 0, sizeof r);
+    r.Enable = r_Enable; r.Disable = r_Disable; r.CullFace = r_CullFace; r.DepthFunc = r_DepthFunc;
+    r.ShadeModel = r_ShadeModel; r.BlendFunc = r_BlendFunc; r.MatrixMode = r_MatrixMode; r.Clear = r_Clear;
+    r.Begin = r_Begin; r.LightModel = r_LightModel; r.EnableLight = r_EnableLight; r.LoadIdentity = r_LoadIdentity;
+    r.End = r_End; r.Translatef = r_Translatef; r.Vertex3f = r_Vertex3f; r.Normal3f = r_Normal3f;
+    r.Rotatef = r_Rotatef; r.Frustum = r_Frustum; r.TexCoord2f = r_TexCoord2f; r.Color4ub = r_Color4ub;
+    r.ClearColor = r_ClearColor; r.Lightfv = r_Lightfv; r.Lightf = r_Lightf; r.Materialfv = r_Materialfv;
+    r.Materialf = r_Materialf; r.ColorMaterial = r_ColorMaterial; r.BindTexture = r_BindTexture;
+    r.TextureParameter = r_TextureParameter; r.GenTexture = r_GenTexture; r.GetError = r_GetError;
+    g_tape = t;
+    Rng rng = { p->seed };
+    SceneTex tex;
+    draw_scene(&r, &rng, p, &tex);
+    /* Epilogue, played after each timed frame: disables every state and
+     * light the scene enabled, so that the next frame starts from the state
+     * the first one started from (the scene sets the rest itself). */
+    t->body = t->n;
+    for (uint32_t b = 0; b < 32; b++)
+        if (t->enabled & (1u << b)) { tput(T_DISABLE); tput(1u << b); }
+    for (uint32_t l = 0; l < 32; l++)
+        if (t->lights & (1u << l)) { tput(T_DISABLELIGHT); tput(l); }
+    g_tape = NULL;
+    t->width = p->width;
+    t->height = p->height;
+    if (t->bad) {
+        for (uint32_t k = 0; k < t->ntex; k++) free(t->texPixels[k]);
+        free(t->w);
+        free(t);
+        return NULL;
+    }
+    return t;
+}
+
+void pfs_tape_free(PFStape* t)
+{
+    if (!t) return;
+    for (uint32_t k = 0; k < t->ntex; k++) free(t->texPixels[k]);
+    free(t->w);
+    free(t);
+}
+
+uint64_t pfs_tape_calls(const PFStape* t) { return t ? t->calls : 0; }
+
+static PFtexture tape_tex(const PFStape* t, uint32_t slot) { return (slot && slot <= t->ntex) ? t->bound[slot - 1] : NULL; }
+
+static float word_f(uint32_t v) { float f; memcpy(&f, &v, 4); return f; }
+
+/* Issues the tape's calls [from, to) (word offsets) through library `a`. */
+static void tape_play(const PFSapi* a, const PFStape* t, size_t from, size_t to)
+{
+    const uint32_t* w = t->w + from;
+    const uint32_t* end = t->w + to;
+#define F(i) word_f(w[i])
+    while (w < end) {
+        switch (*w) {
+        case T_ENABLE: a->Enable((PFstate)w[1]); w += 2; break;
+        case T_DISABLE: a->Disable((PFstate)w[1]); w += 2; break;
+        case T_CULLFACE: a->CullFace((PFface)w[1]); w += 2; break;
+        case T_DEPTHFUNC: a->DepthFunc((PFdepthmode)w[1]); w += 2; break;
+        case T_SHADEMODEL: a->ShadeModel((PFshademode)w[1]); w += 2; break;
+        case T_BLENDFUNC: a->BlendFunc((PFblendmode)w[1]); w += 2; break;
+        case T_MATRIXMODE: a->MatrixMode((PFmatrixmode)w[1]); w += 2; break;
+        case T_CLEAR: a->Clear((PFclearflag)w[1]); w += 2; break;
+        case T_BEGIN: a->Begin((PFdrawmode)w[1]); w += 2; break;
+        case T_LIGHTMODEL: a->LightModel((PFlightmode)w[1]); w += 2; break;
+        case T_ENABLELIGHT: a->EnableLight((PFsizei)w[1]); w += 2; break;
+        case T_DISABLELIGHT: a->DisableLight((PFsizei)w[1]); w += 2; break;
+        case T_LOADIDENTITY: a->LoadIdentity(); w += 1; break;
+        case T_END: a->End(); w += 1; break;
+        case T_TRANSLATE: a->Translatef(F(1), F(2), F(3)); w += 4; break;
+        case T_VERTEX3: a->Vertex3f(F(1), F(2), F(3)); w += 4; break;
+        case T_NORMAL3: a->Normal3f(F(1), F(2), F(3)); w += 4; break;
+        case T_ROTATE: a->Rotatef(F(1), F(2), F(3), F(4)); w += 5; break;
+        case T_FRUSTUM: a->Frustum(F(1), F(2), F(3), F(4), F(5), F(6)); w += 7; break;
+        case T_TEXCOORD2: a->TexCoord2f(F(1), F(2)); w += 3; break;
+        case T_COLOR4UB:
+            a->Color4ub((PFubyte)w[1], (PFubyte)(w[1] >> 8), (PFubyte)(w[1] >> 16), (PFubyte)(w[1] >> 24));
+            w += 2;
+            break;
+        case T_CLEARCOLOR:
+            a->ClearColor((PFubyte)w[1], (PFubyte)(w[1] >> 8), (PFubyte)(w[1] >> 16), (PFubyte)(w[1] >> 24));
+            w += 2;
+            break;
+        case T_LIGHTFV: { float v[3] = { F(3), F(4), F(5) }; a->Lightfv((PFsizei)w[1], (PFenum)w[2], v); w += 6; break; }
+        case T_LIGHTF: a->Lightf((PFsizei)w[1], (PFenum)w[2], F(3)); w += 4; break;
+        case T_MATERIALFV: { float v[3] = { F(3), F(4), F(5) }; a->Materialfv((PFface)w[1], (PFenum)w[2], v); w += 6; break; }
+        case T_MATERIALF: a->Materialf((PFface)w[1], (PFenum)w[2], F(3)); w += 4; break;
+        case T_COLORMATERIAL: a->ColorMaterial((PFface)w[1], (PFenum)w[2]); w += 3; break;
+        case T_BINDTEXTURE: a->BindTexture(tape_tex(t, w[1])); w += 2; break;
+        case T_TEXPARAM:
+            a->TextureParameter(tape_tex(t, w[1]), (PFtexturewrap)w[2], (PFtexturefilter)w[3]);
+            w += 4;
+            break;
+        default: return;                     /* not written by the recorder */
+        }
+    }
+#undef F
+}
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+/* Renders the tape's scene `frames` times as an application would: one
+ * context of library `a` over the caller's colour buffer (W*H+16 pixels),
+ * the tape's textures created in the library once, then per frame the
+ * scene's calls (pfClear and every draw) followed by pfFinish where the
+ * library has one, and the tape's epilogue (untimed), which undoes what
+ * the scene enabled so that every frame starts from the same state.
+ * ms[i] receives frame i's wall time (CLOCK_MONOTONIC) from its first call
+ * to the return of pfFinish (to the return of the last draw for the
+ * reference, which draws inside the calls).  The buffer holds the last
+ * frame.  Returns the library's error code, -1 on setup failure. */
+int pfs_tape_time(const PFSapi* a, PFStape* t, uint32_t frames, uint32_t* color, double* ms)
+{
+    if (!a || !t || !color) return -1;
+    PFcontext ctx = a->CreateContext(color, t->width, t->height, PF_RGBA, PF_UNSIGNED_BYTE);
+    if (!ctx) return -1;
+    a->MakeCurrent(ctx);
+    for (uint32_t k = 0; k < t->ntex; k++)
+        t->bound[k] = a->GenTexture(t->texPixels[k], t->texW[k], t->texH[k], (PFpixelformat)t->texFormat[k],
+                                    (PFdatatype)t->texType[k]);
+    int err = 0;
+    for (uint32_t f = 0; f < frames && err == 0; f++) {
+        double t0 = now_ms();
+        tape_play(a, t, 0, t->body);
+        if (a->Finish) a->Finish();
+        if (ms) ms[f] = now_ms() - t0;
+        tape_play(a, t, t->body, t->n);
+        err = (int)a->GetError();
+    }
+    for (uint32_t k = 0; k < t->ntex; k++) { a->DeleteTexture(&t->bound[k], 0); t->bound[k] = NULL; }
+    a->DeleteContext(ctx);
+    a->MakeCurrent(NULL);
+    return err;
 }
