@@ -40,7 +40,10 @@ typedef struct PFDjob PFDjob;
  * it covers; 2 and 3 are 0 and 1 with four pixels per lane (a wave renders
  * 16x16 pixels per walk of the tile list; tiles of 8x8 fall back to 0/1).
  * Other values act as 0.
- * Zero fields take defaults. */
+ * Zero fields take defaults.  A zero tileShift in the default configuration
+ * (pfdSetDefaultConfig; the initial one) makes one-shot frames choose their
+ * tiles from the first pass's counts: larger tiles and in-place shading
+ * when triangles span many 8x8 tiles. */
 typedef struct {
     uint32_t tileShift;
     uint32_t rasterMode;
@@ -74,7 +77,7 @@ int         pfdInit(void);
 const char* pfdLastError(void);
 
 /* Configuration of jobs created without one (pfdExecuteFrame(s), the host
- * library's default executor). */
+ * library's default executor); initially automatic (tileShift 0). */
 int pfdSetDefaultConfig(const PFDconfig* cfg);
 
 /* One-shot execution used by the host library's default executor: uploads

@@ -228,9 +228,10 @@ class Tape:
             self.ptr = None
 
 
-def set_default_config(tile_shift=3, raster_mode=0):
+def set_default_config(tile_shift=0, raster_mode=0):
     """Tile size (2**tile_shift) and raster mode of frames run by the
-    default executor."""
+    default executor; tile_shift 0 (the initial setting) chooses both per
+    frame."""
     c = PFDconfig()
     c.tileShift = tile_shift
     c.rasterMode = raster_mode

@@ -3009,6 +3009,7 @@ struct PFDjob {
     hipStream_t side = nullptr;           // k_clip<1> beside k_setup<1> (default priority); may be null
     hipEvent_t fork = nullptr, join = nullptr;
     Arena* arena = nullptr;               // non-null: buffers come from the arena, never from hipMalloc
+    bool autoTiles = false;               // choose tile size and raster mode from the first counts
     std::vector<PFFframe> src;            // host frame descriptors: only the target fields are
                                           // used after creation (the rest may be reused by the caller)
     std::vector<uint8_t>  leadClear;      // per frame: the first command is a clear
@@ -3129,6 +3130,30 @@ void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, b
     }
 }
 
+void layout_tiles(PFDjob* J);
+int upload_frames_table(PFDjob* J);
+
+// Automatic configuration from the first pass's counts at 8x8 tiles: while
+// raster triangles span more than 32 tiles on average, tiles double in
+// side (each doubling divides the pairs of large triangles by about four),
+// as long as at least 1024 tiles remain to spread over the chip's 256 CUs;
+// triangles still spanning more than 16 tiles are shaded in place (raster
+// mode 1), the rest packed (mode 0).  Returns the tile shift.
+uint32_t auto_config(PFDjob* J, uint32_t numRT, uint32_t numPairs) {
+    uint32_t s = J->tileShift;
+    double per = (double)numPairs / (double)(numRT ? numRT : 1);
+    while (s < 6 && per > 32.0) {
+        uint64_t tiles = 0;
+        for (const DFrame& F : J->frames)
+            tiles += (uint64_t)((F.W + (2u << s) - 1) >> (s + 1)) * ((F.H + (2u << s) - 1) >> (s + 1));
+        if (tiles < 1024) break;
+        s++;
+        per *= 0.25;
+    }
+    J->cfg.rasterMode = per > 16.0 ? 1u : 0u;
+    return s;
+}
+
 // Issues one pass.  With discover set, the raster-triangle and pair counts
 // are read back (synchronising) and scratch buffers are sized; otherwise
 // the sizes of the previous run are used and nothing synchronises, so the
@@ -3184,24 +3209,37 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.clearSnap = (uint32_t*)J->clearSnap.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
-    // Every pass leaves the clip-list counter at zero (k_tile_bounds, or the
-    // fill at its end); the discovering first run clears it here as well.
-    if (discover)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
-    hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-    hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
     // Replays with few blocks leave the block sums raw for k_scan_local<true>.
     const bool fusedScan = !discover && nb <= 1024;
-    if (!fusedScan)
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
-                           (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
-    HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
-    if (discover) {
+    for (;;) {
+        // Every pass leaves the clip-list counter at zero (k_tile_bounds, or
+        // the fill at its end); a discovering run clears it here as well.
+        if (discover)
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
+        hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+        hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+        if (!fusedScan)
+            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
+                               (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
+        HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+        if (!discover) break;
         uint32_t tot[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(tot, J->scanTotal.p, 8, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
         P.numRT = tot[0];
         P.numPairs = tot[1];
+        // A job with an automatic configuration chooses its tiles from the
+        // first pass's counts, before anything is drawn: the counts are
+        // taken again at the chosen size (the clip list is refilled).
+        if (!J->autoTiles || passIdx != 0) break;
+        J->autoTiles = false;
+        const uint32_t ts = auto_config(J, tot[0], tot[1]);
+        if (ts == J->tileShift) break;
+        J->tileShift = ts;
+        layout_tiles(J);
+        if ((rc = upload_frames_table(J))) return rc;
+        sa.tileShift = J->tileShift;
+        sa.numTiles = J->numTiles;
     }
     const uint32_t numRT = P.numRT, numPairs = P.numPairs;
     if ((rc = ensure(J, J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
@@ -3470,7 +3508,7 @@ int pfdInit(void) {
 
 const char* pfdLastError(void) { return g_err; }
 
-static PFDconfig g_defaultCfg = {3u, 0u, {0, 0, 0, 0, 0, 0}};
+static PFDconfig g_defaultCfg = {0u, 0u, {0, 0, 0, 0, 0, 0}};   // automatic (auto_config)
 
 int pfdSetDefaultConfig(const PFDconfig* cfg) {
     if (!cfg) return PFD_E_INVALID;
@@ -3516,6 +3554,7 @@ void job_reset(PFDjob* J) {
     J->numIn = 0;
     J->hasUV = J->hasNrm = J->anyPhong = J->anyLit = false;
     J->numTiles = 0;
+    J->autoTiles = false;
     J->sortedInB = 0;
     J->lastFragments = J->totalRT = J->totalPairs = 0;
     J->diff = J->diffRan = false;
@@ -4153,6 +4192,7 @@ namespace {
 int run_one_shot(PFDjob* J, const PFFframe* frames, uint32_t count) {
     J->cfg = g_defaultCfg;
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
+    J->autoTiles = J->cfg.tileShift == 0;
     int rc = job_plan(J, frames, count);
     if (!rc && !(rc = pfdJobUploadTargets(J)) && !(rc = run_all(J, true)) && !(rc = pfdJobSync(J)))
         rc = pfdJobDownload(J);

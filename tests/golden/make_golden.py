@@ -1,4 +1,5 @@
-"""Writes the golden frames of tests/scenes.py GOLDEN_SCENES.
+"""Writes the golden frames of tests/scenes.py GOLDEN_SCENES, and the
+digests of its FULL_SIZE_SCENES (full_sizes.json).
 
 Renders each scene through a CPU build of the reference (oracle/_ref,
 built by `make -C oracle ref` from the reference sources) with one OpenMP
@@ -34,11 +35,35 @@ REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
 REF_BILINEAR_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref_bilinear.so")
 
 
+def full_sizes(only):
+    """Digests of the benchmark configurations at full size
+    (scenes.FULL_SIZE_SCENES) into full_sizes.json."""
+    path = os.path.join(HERE, "full_sizes.json")
+    out = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            out = json.load(f)
+    for name, kw in scenes.FULL_SIZE_SCENES:
+        if only and name not in only:
+            continue
+        so = REF_BILINEAR_SO if name in scenes.FULL_SIZE_BILINEAR else REF_SO
+        color, depth, err = pf.render(scenes.params(kw), so)
+        if err:
+            sys.exit(f"{name}: reference reported error {err}")
+        c, d = scenes.digest(color, depth)
+        out[name] = {"params": kw, "color_sha256": c, "depth_sha256": d,
+                     "reference": "bilinear colour-lerp fix" if name in scenes.FULL_SIZE_BILINEAR else "plain"}
+        print(name, color.shape)
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def main():
     for so in (REF_SO, REF_BILINEAR_SO):
         if not os.path.exists(so):
             sys.exit(f"{so} missing: run make -C oracle ref")
     only = set(sys.argv[1:])
+    full_sizes(only)
     for name, kw in scenes.GOLDEN_SCENES:
         if only and name not in only:
             continue

@@ -263,3 +263,25 @@ ALL_SCENES = GOLDEN_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 def params(kw):
     return pf.scene(**kw)
+
+
+# Benchmark configurations at full size.  Their goldens are SHA-256 digests
+# of the reference's colour and depth bytes (tests/golden/full_sizes.json):
+# the images are megabytes each.  Config 2 samples bilinearly, so its digest
+# comes from the reference with the colour-lerp fix; config 3 is not here,
+# because its even-sided CLAMP texture is read past its end by the
+# reference at coordinate 1.0 (deviation D3).
+FULL_SIZE_SCENES = [
+    ("c1_full_800x600", dict(kind=pf.RANDOM_NDC, width=800, height=600, ntris=10000, seed=1, cull=-1,
+                             depthFunc=pf.PF_LESS)),
+    ("c2_full_1080p", dict(kind=pf.HEIGHTFIELD, width=1920, height=1080, seed=2, gridX=1000, gridZ=100,
+                           texture=1, texW=1024, texH=1024, wrap=pf.PF_REPEAT, filter=pf.PF_BILINEAR,
+                           depthFunc=pf.PF_LEQUAL, blendFraction=0.3, cull=pf.PF_BACK)),
+]
+FULL_SIZE_BILINEAR = frozenset({"c2_full_1080p"})
+
+
+def digest(color, depth):
+    """(colour, depth) SHA-256 hex digests of the raw bytes."""
+    import hashlib
+    return (hashlib.sha256(color.tobytes()).hexdigest(), hashlib.sha256(depth.tobytes()).hexdigest())

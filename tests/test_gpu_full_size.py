@@ -1,0 +1,99 @@
+"""The benchmark configurations at full size on the GPU.
+
+- Configs 1 and 2 through the drop-in path (pfFinish, automatic tile
+  choice) equal the reference's digests (tests/golden/full_sizes.json).
+- Config 3 (3840x2160, 1M Phong-lit triangles): its image does not depend
+  on the tile size or raster mode, and the frame rendered in a batch with
+  another context equals the frame rendered alone (size-independent
+  properties; the oracle is too slow at this size).  At a reduced size the
+  same scene equals the oracle bit for bit.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import pixelforge_amd as pf
+import scenes
+from conftest import GOLDEN, assert_same, render_gpu, render_oracle
+
+pytestmark = pytest.mark.gpu
+
+with open(os.path.join(GOLDEN, "full_sizes.json")) as f:
+    FULL = json.load(f)
+
+
+@pytest.mark.parametrize("name,kw", scenes.FULL_SIZE_SCENES, ids=[n for n, _ in scenes.FULL_SIZE_SCENES])
+def test_full_size_matches_reference(name, kw):
+    print(f"full-size {name}", flush=True)
+    c, d, e = render_gpu(scenes.params(kw))
+    assert e == 0, pf.lib().pfdLastError()
+    assert scenes.digest(c, d) == (FULL[name]["color_sha256"], FULL[name]["depth_sha256"])
+
+
+def config3(width=3840, height=2160, spheres=256):
+    return pf.scene(kind=pf.SPHERES, width=width, height=height, spheres=spheres, sphereLat=31, sphereLon=63,
+                    seed=3, lighting=2, numLights=4, texture=1, texW=512, texH=512, wrap=pf.PF_CLAMP_TO_EDGE,
+                    filter=pf.PF_NEAREST, cull=pf.PF_BACK)
+
+
+def run_job(cap, tile_shift, mode):
+    """Runs a captured job once from its captured targets at one
+    configuration; returns copies of every frame's colour and depth."""
+    lib = pf.lib()
+    cfg = pf.PFDconfig()
+    cfg.tileShift, cfg.rasterMode = tile_shift, mode
+    assert lib.pfdJobSetConfig(cap.job, C.byref(cfg)) == 0
+    snap = cap.snapshot()
+    cap.restore(snap)
+    assert lib.pfdJobLaunch(cap.job) == 0
+    assert lib.pfdJobSync(cap.job) == 0
+    assert lib.pfdJobDownload(cap.job) == 0
+    out = [(c.copy(), d.copy()) for c, d in cap.targets()]
+    cap.restore(snap)
+    return out
+
+
+def test_config3_full_size_independent_of_tiles():
+    cap = pf.capture(config3())
+    try:
+        base = None
+        for ts, mode in ((3, 0), (6, 1), (4, 2), (5, 3)):
+            print(f"config 3 at tile {1 << ts}, mode {mode}", flush=True)
+            (c, d), = run_job(cap, ts, mode)
+            if base is None:
+                base = (c, d)
+                assert np.count_nonzero(c != c[0]) > 1_000_000, "config 3 draws too little"
+            else:
+                assert_same(base[0], base[1], c, d, f"tile {1 << ts} mode {mode}")
+    finally:
+        cap.close()
+
+
+def test_config3_full_size_batched_equals_single():
+    other = scenes.params(dict(scenes.CONFIG_SCENES)["c2_heightfield_nearest"])
+    single = pf.capture(config3())
+    try:
+        (c, d), = run_job(single, 3, 0)
+    finally:
+        single.close()
+    print("config 3 in a batch", flush=True)
+    batch = pf.capture_batch([other, config3()])
+    try:
+        frames = run_job(batch, 3, 0)
+        assert_same(c, d, frames[1][0], frames[1][1], "config 3 batched vs single")
+        oc, od, _, _ = render_oracle(other)
+        assert_same(oc.ravel(), od.ravel(), frames[0][0][: oc.size], frames[0][1][: od.size], "other context")
+    finally:
+        batch.close()
+
+
+def test_config3_reduced_size_equals_oracle():
+    p = config3(width=1920, height=1080, spheres=96)
+    oc, od, oe, frags = render_oracle(p)
+    gc, gd, ge = render_gpu(p)
+    assert oe == 0 and ge == 0
+    assert frags > 1_000_000
+    assert_same(oc, od, gc, gd, "config 3 at 1920x1080, 96 spheres")

@@ -51,4 +51,4 @@ def test_every_tile_size_and_raster_mode(tile_shift, mode):
             assert ge == 0
             assert_same(oc, od, gc, gd, f"{name} tile {1 << tile_shift} mode {mode}")
     finally:
-        pf.set_default_config(3, 0)
+        pf.set_default_config()
