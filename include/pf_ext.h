@@ -23,6 +23,12 @@ PF_API void pfxSetExecutor(PFXexecutor fn, void* user);
 /* Number of frames the current context has executed so far. */
 PF_API PFuint pfxFrameCount(void);
 
+/* Whether pfDrawElements / pfDrawArrays calls whose primitives all become
+ * filled triangles are recorded whole, their triangles formed on the GPU
+ * (the default), or vertex by vertex on the host as immediate-mode calls
+ * are.  Both record the same triangles; a test harness compares them. */
+PF_API void pfxSetArrayDraws(PFboolean enabled);
+
 /* Runs the frames of one pfSubmitBatch call together; returns 0 on success. */
 typedef int (*PFXbatchExecutor)(const PFFframe* frames, uint32_t count, void* user);
 

@@ -191,6 +191,49 @@ typedef struct {
                                   no corner has one                          */
 } PFFtriangles;
 
+/* PFdatatype values of the array element types (pixelforge.h). */
+#define PFF_UBYTE   0u
+#define PFF_USHORT  1u
+#define PFF_UINT    5u
+#define PFF_SHORT   7u
+#define PFF_INT     8u
+#define PFF_FLOAT  10u
+#define PFF_DOUBLE 11u
+
+/* One pfDrawElements / pfDrawArrays call whose primitives all become filled
+ * triangles (context.c:1231-1575 with ictx.c:94-244): the elements it reads
+ * are copied into the frame when it is called, and its triangles fill the
+ * slots [triFirst, triFirst + triCount) of PFFtriangles, which the host
+ * leaves unwritten.  The vertex buffer of `mode` is filled with one element
+ * per index; each time it is full (3 vertices for PF_TRIANGLES, 4 for
+ * PF_TRIANGLE_FAN / _STRIP and PF_QUADS, 6 for PF_QUAD_FAN / _STRIP) it
+ * gives, for each face group in order, the mode's triangles (1, 2, 2 or 4)
+ * with that group's PFF_PASS_* bits, and the buffer keeps its last 1 (fans,
+ * strips) or 2 (quad fans, quad strips) vertices.  A corner takes position
+ * (x, y, z, w) from `position` (missing components 0, w 1), normal and
+ * texture coordinate from their arrays or zero (no texture matrix, no
+ * PF_NORMALIZE), and colour from `colors` (missing alpha 255, colour-array
+ * index = element + elementBase) or `color` (index PFF_NO_VERTEX). */
+typedef struct {
+    uint32_t    mode;          /* PFdrawmode: PF_TRIANGLES .. PF_QUAD_STRIP    */
+    uint32_t    count;         /* vertices the call issues                     */
+    uint32_t    triFirst;      /* first triangle slot of the call              */
+    uint32_t    triCount;
+    uint32_t    numFaces;      /* face groups per full buffer, 1 or 2          */
+    uint32_t    facePasses[2]; /* PFF_PASS_* bits of each group, in order      */
+    uint32_t    state;         /* index into PFFframe.states                   */
+    uint32_t    color;         /* packed RGBA8: corners without a colour array */
+    uint32_t    elementBase;   /* colour-array index of element 0 (pfDrawArrays' first) */
+    uint32_t    numElements;   /* elements in each array copy                  */
+    const void* position;  int32_t posSize; uint32_t posType;   /* PF_SHORT, PF_INT, PF_FLOAT, PF_DOUBLE */
+    const void* normal;    uint32_t nrmType;                    /* PF_FLOAT, PF_DOUBLE; NULL: none */
+    const void* texcoord;  uint32_t uvType;                     /* PF_FLOAT, PF_DOUBLE; NULL: none */
+    const void* colors;    int32_t colSize; uint32_t colType;   /* PF_UNSIGNED_BYTE/SHORT/INT, PF_FLOAT,
+                                                                   PF_DOUBLE; NULL: none */
+    const void* indices;   uint32_t idxType;                    /* PF_UNSIGNED_BYTE/SHORT/INT; NULL:
+                                                                   vertex i is element i */
+} PFFarrays;
+
 typedef struct {
     uint32_t           width, height;    /* render target, RGBA8 + float32 depth */
     uint32_t*          color;            /* host buffers of the render target    */
@@ -202,6 +245,7 @@ typedef struct {
     const PFFpixels*   pixelOps; uint32_t numPixelOps;
     const PFFpoint*    points;   uint32_t numPoints;
     const PFFline*     lines;    uint32_t numLines;
+    const PFFarrays*   arrays;   uint32_t numArrays;   /* in triFirst order */
 } PFFframe;
 
 #endif /* PF_FRAME_H */

@@ -1414,14 +1414,177 @@ static void draw_line(const PFFframe* f, const PFFstate* st, const LVtx* v, floa
         }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Array draws (pfDrawElements / pfDrawArrays recorded whole)                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {                       /* the PFIvertex fields array draws set */
+    float    pos[4], normal[3], uv[2];
+    uint32_t color, vid;
+} AVert;
+
+static float elem_f(const void* b, uint32_t type, size_t i)    /* context.c:1274-1352 */
+{
+    switch (type) {
+    case PFF_SHORT:  return (float)((const int16_t*)b)[i];
+    case PFF_INT:    return (float)((const int32_t*)b)[i];
+    case PFF_DOUBLE: return (float)((const double*)b)[i];
+    default:         return ((const float*)b)[i];
+    }
+}
+
+/* (PFubyte) conversions of context.c:1356-1392 as gcc emits them on x86-64:
+ * the integer shifts, or cvttss2si / cvttsd2si (INT_MIN out of range) and
+ * the low byte. */
+static uint32_t elem_u8(const void* b, uint32_t type, size_t i)
+{
+    switch (type) {
+    case PFF_UBYTE:  return ((const uint8_t*)b)[i];
+    case PFF_USHORT: return ((const uint16_t*)b)[i] >> 8;
+    case PFF_UINT:   return ((const uint32_t*)b)[i] >> 24;
+    case PFF_FLOAT:  return f2u8(((const float*)b)[i] * 255);
+    default: {
+        double d = ((const double*)b)[i] * 255;
+        int32_t v = (d > -2147483649.0 && d < 2147483648.0) ? (int32_t)d : INT32_MIN;
+        return (uint32_t)v & 0xFFu;
+    }
+    }
+}
+
+static void put_tri(const PFFarrays* d, const AVert* a, const AVert* b, const AVert* c, uint32_t passes,
+                    size_t t, float* pos, uint32_t* col, float* uv, float* nrm, uint32_t* info, uint32_t* vid)
+{
+    const AVert* v[3] = { a, b, c };
+    for (int k = 0; k < 3; k++) {
+        size_t vi = t * 3 + k;
+        memcpy(pos + 4 * vi, v[k]->pos, 16);
+        col[vi] = v[k]->color;
+        if (uv) memcpy(uv + 2 * vi, v[k]->uv, 8);
+        if (nrm) memcpy(nrm + 3 * vi, v[k]->normal, 12);
+        if (vid) vid[vi] = v[k]->vid;
+    }
+    info[t] = d->state | passes;
+}
+
+/* The reference's per-vertex loop over one call (context.c:1245-1404,
+ * 1427-1575): the mode's vertex buffer (sizes context.c:61-74) is reset to
+ * zero attributes and the current colour (:1254-1257), filled one element
+ * at a time, and each time it is full its primitive goes through
+ * pfiProcessAndRasterize (ictx.c:94-244: each face group in turn; fans and
+ * quads as triangle fans, strips alternating winding, triangles.c:125-152)
+ * before the buffer keeps its last vertices (context.c:41-59). */
+static void expand_arrays(const PFFarrays* d, float* pos, uint32_t* col, float* uv, float* nrm, uint32_t* info,
+                          uint32_t* vid)
+{
+    const uint32_t m = d->mode;
+    const uint32_t bufSize = m == 2 ? 3 : (m == 6 || m == 7) ? 6 : 4;
+    AVert vb[6];
+    memset(vb, 0, sizeof vb);
+    for (uint32_t i = 0; i < bufSize; i++) { vb[i].color = d->color; vb[i].vid = PFF_NO_VERTEX; }
+    uint32_t n = 0;
+    size_t t = d->triFirst;
+    for (uint32_t i = 0; i < d->count; i++) {
+        uint32_t j = i;
+        if (d->indices) j = d->idxType == PFF_UBYTE ? ((const uint8_t*)d->indices)[i]
+                          : d->idxType == PFF_USHORT ? ((const uint16_t*)d->indices)[i] : ((const uint32_t*)d->indices)[i];
+        AVert* v = &vb[n++];
+        v->pos[0] = v->pos[1] = v->pos[2] = 0.0f;
+        v->pos[3] = 1.0f;
+        for (int k = 0; k < d->posSize; k++) v->pos[k] = elem_f(d->position, d->posType, (size_t)j * d->posSize + k);
+        if (d->normal) for (int k = 0; k < 3; k++) v->normal[k] = elem_f(d->normal, d->nrmType, (size_t)j * 3 + k);
+        if (d->texcoord) for (int k = 0; k < 2; k++) v->uv[k] = elem_f(d->texcoord, d->uvType, (size_t)j * 2 + k);
+        if (d->colors) {
+            uint32_t ch[4] = { 255, 255, 255, 255 };
+            for (int k = 0; k < d->colSize; k++) ch[k] = elem_u8(d->colors, d->colType, (size_t)j * d->colSize + k);
+            v->color = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (ch[3] << 24);
+            v->vid = j + d->elementBase;
+        }
+        if (n < bufSize) continue;
+        for (uint32_t g = 0; g < d->numFaces; g++) {
+            const uint32_t ps = d->facePasses[g];
+            if (m == 2) {
+                put_tri(d, &vb[0], &vb[1], &vb[2], ps, t++, pos, col, uv, nrm, info, vid);
+            } else if (m == 4 || m == 7) {                       /* strips */
+                for (uint32_t q = 0; q < bufSize - 2; q++) {
+                    if (q % 2 == 0) put_tri(d, &vb[q], &vb[q + 1], &vb[q + 2], ps, t++, pos, col, uv, nrm, info, vid);
+                    else put_tri(d, &vb[q + 2], &vb[q + 1], &vb[q], ps, t++, pos, col, uv, nrm, info, vid);
+                }
+            } else {                                             /* fans and quads */
+                for (uint32_t q = 0; q < bufSize - 2; q++)
+                    put_tri(d, &vb[0], &vb[q + 1], &vb[q + 2], ps, t++, pos, col, uv, nrm, info, vid);
+            }
+        }
+        if (m == 3 || m == 4) { vb[0] = vb[3]; n = 1; }
+        else if (m == 6 || m == 7) { vb[0] = vb[4]; vb[1] = vb[5]; n = 2; }
+        else n = 0;
+    }
+}
+
+typedef struct {                       /* streams of a frame with its array draws expanded */
+    float* pos; uint32_t* col; float* uv; float* nrm; uint32_t* info; uint32_t* vid;
+} Expanded;
+
+static void expanded_free(Expanded* x)
+{
+    free(x->pos); free(x->col); free(x->uv); free(x->nrm); free(x->info); free(x->vid);
+    memset(x, 0, sizeof *x);
+}
+
+/* *out = *f with every array draw's triangles in its streams (copies). */
+static int expand_frame(const PFFframe* f, PFFframe* out, Expanded* x)
+{
+    memset(x, 0, sizeof *x);
+    *out = *f;
+    if (!f->numArrays) return 0;
+    const size_t n = f->tris.count;
+    x->pos = (float*)malloc(n * 48 + 16);
+    x->col = (uint32_t*)malloc(n * 12 + 16);
+    x->info = (uint32_t*)malloc(n * 4 + 16);
+    if (f->tris.texcoord) x->uv = (float*)malloc(n * 24 + 16);
+    if (f->tris.normal) x->nrm = (float*)malloc(n * 36 + 16);
+    if (f->tris.vertexId) x->vid = (uint32_t*)malloc(n * 12 + 16);
+    if (!x->pos || !x->col || !x->info || (f->tris.texcoord && !x->uv) || (f->tris.normal && !x->nrm)
+        || (f->tris.vertexId && !x->vid)) {
+        expanded_free(x);
+        return -3;
+    }
+    if (n) {
+        memcpy(x->pos, f->tris.position, n * 48);
+        memcpy(x->col, f->tris.color, n * 12);
+        memcpy(x->info, f->tris.info, n * 4);
+        if (x->uv) memcpy(x->uv, f->tris.texcoord, n * 24);
+        if (x->nrm) memcpy(x->nrm, f->tris.normal, n * 36);
+        if (x->vid) memcpy(x->vid, f->tris.vertexId, n * 12);
+    }
+    for (uint32_t k = 0; k < f->numArrays; k++) {
+        const PFFarrays* d = &f->arrays[k];
+        if ((uint64_t)d->triFirst + d->triCount > n || d->mode < 2 || d->mode > 7) { expanded_free(x); return -1; }
+        expand_arrays(d, x->pos, x->col, x->uv, x->nrm, x->info, x->vid);
+    }
+    out->tris.position = x->pos;
+    out->tris.color = x->col;
+    out->tris.info = x->info;
+    out->tris.texcoord = x->uv;
+    out->tris.normal = x->nrm;
+    out->tris.vertexId = x->vid;
+    out->arrays = NULL;
+    out->numArrays = 0;
+    return 0;
+}
+
 static int render_frame(RCtx* rc);
 
 int pforacle_render(const PFFframe* f, uint64_t* fragments)
 {
+    PFFframe g;
+    Expanded x;
+    int r = expand_frame(f, &g, &x);
+    if (r) return r;
     RCtx rc;
     memset(&rc, 0, sizeof rc);
-    rc.f = f;
-    int r = render_frame(&rc);
+    rc.f = &g;
+    r = render_frame(&rc);
+    expanded_free(&x);
     if (fragments) *fragments = rc.fragments;
     return r;
 }
@@ -1483,6 +1646,18 @@ static int render_frame(RCtx* rcp)
     return rc.oom ? -3 : 0;
 }
 
+/* For the recording tests: *out = *f with its array draws expanded into
+ * streams this oracle owns until the next call or pforacle_expand_free. */
+static Expanded g_expanded;
+
+int pforacle_expand(const PFFframe* f, PFFframe* out)
+{
+    expanded_free(&g_expanded);
+    return expand_frame(f, out, &g_expanded);
+}
+
+void pforacle_expand_free(void) { expanded_free(&g_expanded); }
+
 /* Frame executor with the signature of PFXexecutor (pf_ext.h), so that a
  * test harness can run the host library's recorded frames through this
  * oracle.  `user`, when not NULL, points to a uint64_t that accumulates the
@@ -1542,10 +1717,14 @@ static void diff_job_free(DiffJob* J)
 
 /* Renders the frame with winner tracking into private buffers.  Returns a
  * handle, or NULL. */
-void* pforacle_diff_prepare(const PFFframe* f)
+void* pforacle_diff_prepare(const PFFframe* f0)
 {
+    PFFframe fx;
+    Expanded x;
+    if (expand_frame(f0, &fx, &x)) return NULL;
+    const PFFframe* f = &fx;
     DiffJob* J = (DiffJob*)calloc(1, sizeof(DiffJob));
-    if (!J) return NULL;
+    if (!J) { expanded_free(&x); return NULL; }
     size_t n = (size_t)f->width * f->height;
     J->f = *f;
     size_t nt = (size_t)f->tris.count * 3;
@@ -1579,9 +1758,11 @@ void* pforacle_diff_prepare(const PFFframe* f)
     J->f.tris.normal = NULL;
     J->f.tris.info = NULL;
     J->f.cmds = NULL;
+    expanded_free(&x);
     return J;
 fail:
     diff_job_free(J);
+    expanded_free(&x);
     return NULL;
 }
 

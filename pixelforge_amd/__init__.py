@@ -20,6 +20,7 @@ RANDOM_NDC, HEIGHTFIELD, SPHERES, RANDOM_3D, MESH, LIST = 1, 2, 3, 4, 5, 6
 
 # Enum values of include/pixelforge.h used by callers
 PF_FRONT, PF_BACK = 0, 1
+PF_TRIANGLES, PF_TRIANGLE_FAN, PF_TRIANGLE_STRIP, PF_QUADS, PF_QUAD_FAN, PF_QUAD_STRIP = range(2, 8)
 PF_EQUAL, PF_NOTEQUAL, PF_LESS, PF_LEQUAL, PF_GREATER, PF_GEQUAL = range(6)
 (PF_BLEND_AVERAGE, PF_BLEND_ALPHA, PF_BLEND_ADD, PF_BLEND_SUB, PF_BLEND_MUL,
  PF_BLEND_SCREEN, PF_BLEND_LIGHTEN, PF_BLEND_DARKEN) = range(8)
@@ -50,7 +51,7 @@ class SceneParams(C.Structure):
         ("fog", C.c_int32), ("fogStart", C.c_float), ("fogEnd", C.c_float), ("fogAlpha", C.c_int32),
         ("points", C.c_int32), ("pointSize", C.c_float), ("polyPoint", C.c_int32),
         ("lines", C.c_int32), ("lineWidth", C.c_float), ("polyLine", C.c_int32),
-        ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float),
+        ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float), ("arrayMode", C.c_int32),
     ]
 
 

@@ -2928,6 +2928,116 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Array draws (pfDrawElements / pfDrawArrays recorded whole, PFFarrays):
+// one thread per triangle forms it from the copied elements, in the order
+// the reference's vertex loop emits triangles (context.c:1231-1575: the
+// vertex buffer of the mode, ictx.c:94-244 per face, triangles.c:125-152
+// for fans and strips), and writes it into the job's triangle streams.
+// ---------------------------------------------------------------------------
+
+struct DArrays {
+    const uint8_t* pos;
+    const uint8_t* nrm;        // null: zero normal
+    const uint8_t* uv;         // null: zero texture coordinate
+    const uint8_t* col;        // null: `color`, no colour-array index
+    const uint8_t* idx;        // null: vertex v is element v
+    uint32_t posSize, posType, nrmType, uvType, colSize, colType, idxType;
+    uint32_t mode;
+    uint32_t perBuf;           // triangles per full buffer and face group
+    uint32_t stride;           // vertices a full buffer advances (buffer size minus the kept ones)
+    uint32_t numFaces, facePass0, facePass1;
+    uint32_t state;            // global state index
+    uint32_t color, elementBase, numElements;
+    uint32_t slot0;            // global slot of the draw's first triangle
+    uint32_t cum;              // array triangles of the job before this draw
+};
+
+__device__ __forceinline__ float arr_f(const uint8_t* b, uint32_t type, size_t i) {
+    switch (type) {                                // context.c:1274-1310 (float conversions)
+    case PFF_SHORT:  return (float)((const int16_t*)b)[i];
+    case PFF_INT:    return (float)((const int32_t*)b)[i];
+    case PFF_DOUBLE: return (float)((const double*)b)[i];
+    default:         return ((const float*)b)[i];
+    }
+}
+
+// Colour component as the reference converts it (context.c:1356-1392); a
+// float or double outside int32 converts to 0 (cvttss2si / cvttsd2si give
+// INT_MIN, whose low byte is 0).
+__device__ __forceinline__ uint32_t arr_u8(const uint8_t* b, uint32_t type, size_t i) {
+    switch (type) {
+    case PFF_UBYTE:  return b[i];
+    case PFF_USHORT: return ((const uint16_t*)b)[i] >> 8;
+    case PFF_UINT:   return ((const uint32_t*)b)[i] >> 24;
+    case PFF_FLOAT: {
+        const float f = ((const float*)b)[i] * 255.0f;
+        return (f > -2147483904.0f && f < 2147483648.0f) ? ((uint32_t)(int32_t)f & 0xFFu) : 0u;
+    }
+    default: {
+        const double d = ((const double*)b)[i] * 255.0;
+        return (d > -2147483649.0 && d < 2147483648.0) ? ((uint32_t)(int32_t)d & 0xFFu) : 0u;
+    }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_deindex(const DArrays* __restrict__ D, uint32_t numDraws, uint32_t total,
+                                                 float4* __restrict__ pos, uint32_t* __restrict__ col,
+                                                 float2* __restrict__ uv, float* __restrict__ nrm,
+                                                 uint32_t* __restrict__ info, uint32_t* __restrict__ vid) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= total) return;
+    uint32_t lo = 0, hi = numDraws;                // D[lo].cum <= g < D[hi].cum
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (D[mid].cum <= g) lo = mid; else hi = mid;
+    }
+    const DArrays& A = D[lo];
+    const uint32_t w = g - A.cum;
+    const uint32_t perFill = A.perBuf * A.numFaces;
+    const uint32_t fill = w / perFill, r = w - fill * perFill;
+    const uint32_t grp = r / A.perBuf, i = r - grp * A.perBuf;
+    uint32_t sl[3];
+    if (A.mode == 2u) {                            // PF_TRIANGLES
+        sl[0] = 0; sl[1] = 1; sl[2] = 2;
+    } else if (A.mode == 4u || A.mode == 7u) {     // strips: odd triangles reversed
+        if (i & 1u) { sl[0] = i + 2; sl[1] = i + 1; sl[2] = i; }
+        else        { sl[0] = i; sl[1] = i + 1; sl[2] = i + 2; }
+    } else {                                       // fans and quads: around slot 0
+        sl[0] = 0; sl[1] = i + 1; sl[2] = i + 2;
+    }
+    const size_t t = (size_t)A.slot0 + w;
+    for (int k = 0; k < 3; k++) {
+        const uint32_t v = fill * A.stride + sl[k];
+        uint32_t j = v;
+        if (A.idx) j = A.idxType == PFF_UBYTE ? A.idx[v] : A.idxType == PFF_USHORT ? ((const uint16_t*)A.idx)[v]
+                                                                                  : ((const uint32_t*)A.idx)[v];
+        if (j >= A.numElements) j = A.numElements - 1;        // not produced by the host library
+        float p[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+        for (uint32_t c = 0; c < A.posSize; c++) p[c] = arr_f(A.pos, A.posType, (size_t)j * A.posSize + c);
+        pos[3 * t + k] = make_float4(p[0], p[1], p[2], p[3]);
+        uint32_t cw = A.color, vi = PFF_NO_VERTEX;
+        if (A.col) {
+            uint32_t ch[4] = {255u, 255u, 255u, 255u};
+            for (uint32_t c = 0; c < A.colSize; c++) ch[c] = arr_u8(A.col, A.colType, (size_t)j * A.colSize + c);
+            cw = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (ch[3] << 24);
+            vi = j + A.elementBase;
+        }
+        col[3 * t + k] = cw;
+        if (vid) vid[3 * t + k] = vi;
+        if (uv) {
+            float2 q = make_float2(0.0f, 0.0f);
+            if (A.uv) { q.x = arr_f(A.uv, A.uvType, (size_t)j * 2); q.y = arr_f(A.uv, A.uvType, (size_t)j * 2 + 1); }
+            uv[3 * t + k] = q;
+        }
+        if (nrm) {
+            for (int c = 0; c < 3; c++)
+                nrm[9 * t + 3 * k + c] = A.nrm ? arr_f(A.nrm, A.nrmType, (size_t)j * 3 + c) : 0.0f;
+        }
+    }
+    info[t] = A.state | (grp ? A.facePass1 : A.facePass0);
+}
+
 __global__ void k_fill_f32(float* __restrict__ p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0f;
 }
@@ -3043,6 +3153,7 @@ struct PFDjob {
     // differentiable jobs
     bool diff = false, diffRan = false;
     DevBuf rtd, winner, vid, texParams, colParams, lightParams, image, dImage, gTex, gCol, gLightPart, gLight;
+    DevBuf arrPool, arrTable;             // array draws: element copies and their DArrays
     uint32_t numVertices = 0;
     int32_t paramTex = -1;
 };
@@ -3057,7 +3168,8 @@ void for_each_buf(PFDjob* J, F f) {
                        &J->rtl, &J->keysA, &J->valsA, &J->keysB, &J->valsB, &J->scanBlocks, &J->scanTotal, &J->hist,
                        &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList,
                        &J->clipCount, &J->clearSnap, &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams,
-                       &J->lightParams, &J->image, &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight };
+                       &J->lightParams, &J->image, &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight,
+                       &J->arrPool, &J->arrTable };
     for (DevBuf* b : bufs) f(*b);
 }
 
@@ -3568,27 +3680,96 @@ void job_reset(PFDjob* J) {
 
 #define PLAN_FAIL(code, ...) do { snprintf(g_err, sizeof g_err, __VA_ARGS__); return (code); } while (0)
 
+// Calls f(begin, end) for the triangle ranges of a frame that the host
+// recorded itself, i.e. outside its array draws (whose slots the host
+// leaves unwritten and k_deindex fills).
+template <typename F>
+void for_host_ranges(const PFFframe& fr, F f) {
+    uint32_t at = 0;
+    for (uint32_t k = 0; k < fr.numArrays; k++) {
+        const PFFarrays& a = fr.arrays[k];
+        if (a.triFirst > at) f(at, a.triFirst);
+        at = a.triFirst + a.triCount;
+    }
+    if (fr.tris.count > at) f(at, fr.tris.count);
+}
+
+size_t elem_bytes(uint32_t type) {
+    switch (type) {
+    case PFF_UBYTE: return 1;
+    case PFF_USHORT: case PFF_SHORT: return 2;
+    case PFF_DOUBLE: return 8;
+    default: return 4;
+    }
+}
+
+// Checks an array draw of a frame with `nt` triangles and `ns` states
+// against the rules of PFFarrays; returns a message or null.
+const char* check_arrays(const PFFarrays& a, uint32_t prevEnd, uint32_t nt, uint32_t ns) {
+    if (a.mode < 2u || a.mode > 7u) return "array draw: bad mode";
+    if (a.numFaces < 1u || a.numFaces > 2u || a.state >= ns) return "array draw: bad faces or state";
+    if (!a.position || a.posSize < 2 || a.posSize > 4
+        || !(a.posType == PFF_SHORT || a.posType == PFF_INT || a.posType == PFF_FLOAT || a.posType == PFF_DOUBLE))
+        return "array draw: bad positions";
+    if (a.normal && !(a.nrmType == PFF_FLOAT || a.nrmType == PFF_DOUBLE)) return "array draw: bad normals";
+    if (a.texcoord && !(a.uvType == PFF_FLOAT || a.uvType == PFF_DOUBLE)) return "array draw: bad texcoords";
+    if (a.colors && (a.colSize < 3 || a.colSize > 4
+                     || !(a.colType == PFF_UBYTE || a.colType == PFF_USHORT || a.colType == PFF_UINT
+                          || a.colType == PFF_FLOAT || a.colType == PFF_DOUBLE)))
+        return "array draw: bad colours";
+    if (a.indices && !(a.idxType == PFF_UBYTE || a.idxType == PFF_USHORT || a.idxType == PFF_UINT))
+        return "array draw: bad index type";
+    if (a.numElements == 0u || (!a.indices && a.count > a.numElements)) return "array draw: too few elements";
+    const uint32_t buf = a.mode == 2u ? 3u : (a.mode == 6u || a.mode == 7u) ? 6u : 4u;
+    const uint32_t keep = (a.mode == 3u || a.mode == 4u) ? 1u : (a.mode == 6u || a.mode == 7u) ? 2u : 0u;
+    const uint32_t per = a.mode == 2u ? 1u : (a.mode == 6u || a.mode == 7u) ? 4u : 2u;
+    const uint64_t fills = a.count < buf ? 0 : 1 + (uint64_t)(a.count - buf) / (buf - keep);
+    if (fills * a.numFaces * per != a.triCount) return "array draw: triangle count does not match";
+    if (a.triFirst < prevEnd || (uint64_t)a.triFirst + a.triCount > nt) return "array draw: bad triangle range";
+    return nullptr;
+}
+
 // Uploads the frames into the job's buffers and builds its passes:
 // triangle streams, states and textures of all frames concatenated, render
 // targets, decoded pfDrawPixels images, points and lines (projected once),
 // and per pass its segments (one per frame at most) and pixel / point /
 // line operations.  The job's configuration must be set.
-int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count) {
+int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
     int rc = 0;
     hipError_t e = hipSuccess;
 
     // Sizes and validation, before anything is copied.
     uint32_t totalTris = 0, totalStates = 0;
     bool anyUV = false, anyNrm = false;
-    size_t targetBytes = 0, texBytes = 0, pixBytes = 0, stageBytes = 0;
+    size_t targetBytes = 0, texBytes = 0, pixBytes = 0, stageBytes = 0, arrBytes = 0;
+    uint64_t arrTris = 0;
+    uint32_t numDraws = 0;
     for (uint32_t f = 0; f < count; f++) {
         const PFFframe& fr = frames[f];
         if (!fr.color || !fr.depth || fr.width == 0 || fr.height == 0)
             PLAN_FAIL(PFD_E_INVALID, "frame %u has no render target", f);
         if (fr.tris.count && (!fr.tris.position || !fr.tris.color || !fr.tris.info))
             PLAN_FAIL(PFD_E_INVALID, "frame %u: triangle streams missing", f);
-        for (size_t t = 0; t < fr.tris.count; t++)
-            if ((fr.tris.info[t] & PFF_STATE_MASK) >= fr.numStates) PLAN_FAIL(PFD_E_INVALID, "bad state index");
+        if (fr.numArrays && !fr.arrays) PLAN_FAIL(PFD_E_INVALID, "frame %u: array draws missing", f);
+        uint32_t prevEnd = 0;
+        for (uint32_t k = 0; k < fr.numArrays; k++) {
+            const PFFarrays& a = fr.arrays[k];
+            const char* why = check_arrays(a, prevEnd, fr.tris.count, fr.numStates);
+            if (why) PLAN_FAIL(PFD_E_INVALID, "frame %u: %s", f, why);
+            prevEnd = a.triFirst + a.triCount;
+            const size_t n = a.numElements;
+            arrBytes += align256(n * a.posSize * elem_bytes(a.posType)) + align256(a.normal ? n * 3 * elem_bytes(a.nrmType) : 0)
+                      + align256(a.texcoord ? n * 2 * elem_bytes(a.uvType) : 0)
+                      + align256(a.colors ? n * a.colSize * elem_bytes(a.colType) : 0)
+                      + align256(a.indices ? (size_t)a.count * elem_bytes(a.idxType) : 0);
+            arrTris += a.triCount;
+            numDraws++;
+        }
+        bool badState = false;
+        for_host_ranges(fr, [&](uint32_t b, uint32_t e2) {
+            for (uint32_t t = b; t < e2; t++) badState |= (fr.tris.info[t] & PFF_STATE_MASK) >= fr.numStates;
+        });
+        if (badState) PLAN_FAIL(PFD_E_INVALID, "bad state index");
         totalTris += fr.tris.count;
         totalStates += fr.numStates;
         anyUV |= fr.tris.texcoord != nullptr;
@@ -3826,15 +4007,31 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count) {
         }
         return 0;
     };
+    const bool vidOn = wantVid && count == 1 && frames[0].tris.vertexId != nullptr;
     if (count == 1) {
-        // One frame: its streams go up as they are (state indices need no
-        // offset and were checked above).
-        const PFFtriangles& T = frames[0].tris;
+        // One frame: the host-recorded ranges of its streams go up as they
+        // are (state indices need no offset and were checked above).
+        const PFFframe& fr = frames[0];
+        const PFFtriangles& T = fr.tris;
         const size_t n = T.count;
-        if ((rc = up(J->pos, T.position, n * 48)) || (rc = up(J->col, T.color, n * 12))
-            || (rc = up(J->info, T.info, n * 4)) || (rc = up(J->uv, T.texcoord, T.texcoord ? n * 24 : 0))
-            || (rc = up(J->nrm, T.normal, T.normal ? n * 36 : 0)))
+        if ((rc = ensure(J, J->pos, n * 48 + 4)) || (rc = ensure(J, J->col, n * 12 + 4)) || (rc = ensure(J, J->info, n * 4 + 4))
+            || (rc = ensure(J, J->uv, T.texcoord ? n * 24 + 4 : 4)) || (rc = ensure(J, J->nrm, T.normal ? n * 36 + 4 : 4))
+            || (vidOn && (rc = ensure(J, J->vid, n * 12 + 4))))
             return rc;
+        for_host_ranges(fr, [&](uint32_t b, uint32_t e2) {
+            auto part = [&](DevBuf& d, const void* h, size_t per) {
+                if (e == hipSuccess && h)
+                    e = hipMemcpy((uint8_t*)d.p + b * per, (const uint8_t*)h + b * per, (size_t)(e2 - b) * per,
+                                  hipMemcpyHostToDevice);
+            };
+            part(J->pos, T.position, 48);
+            part(J->col, T.color, 12);
+            part(J->info, T.info, 4);
+            part(J->uv, T.texcoord, 24);
+            part(J->nrm, T.normal, 36);
+            if (vidOn) part(J->vid, T.vertexId, 12);
+        });
+        if (e != hipSuccess) { set_error("upload", e); return PFD_E_LAUNCH; }
     } else {
         std::vector<float> hpos((size_t)totalTris * 12);
         std::vector<uint32_t> hcol((size_t)totalTris * 3), hinfo(totalTris);
@@ -3842,22 +4039,80 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count) {
         uint32_t tb = 0, sb = 0;
         for (uint32_t f = 0; f < count; f++) {
             const PFFframe& fr = frames[f];
-            const size_t nt = fr.tris.count;
-            if (nt) {
-                memcpy(&hpos[(size_t)tb * 12], fr.tris.position, nt * 12 * sizeof(float));
-                memcpy(&hcol[(size_t)tb * 3], fr.tris.color, nt * 3 * sizeof(uint32_t));
-                if (anyUV && fr.tris.texcoord) memcpy(&huv[(size_t)tb * 6], fr.tris.texcoord, nt * 6 * sizeof(float));
-                if (anyNrm && fr.tris.normal) memcpy(&hnrm[(size_t)tb * 9], fr.tris.normal, nt * 9 * sizeof(float));
-                for (size_t t = 0; t < nt; t++)
+            for_host_ranges(fr, [&](uint32_t b, uint32_t e2) {
+                const size_t nt = e2 - b, o = (size_t)tb + b;
+                memcpy(&hpos[o * 12], fr.tris.position + (size_t)b * 12, nt * 12 * sizeof(float));
+                memcpy(&hcol[o * 3], fr.tris.color + (size_t)b * 3, nt * 3 * sizeof(uint32_t));
+                if (anyUV && fr.tris.texcoord) memcpy(&huv[o * 6], fr.tris.texcoord + (size_t)b * 6, nt * 6 * sizeof(float));
+                if (anyNrm && fr.tris.normal) memcpy(&hnrm[o * 9], fr.tris.normal + (size_t)b * 9, nt * 9 * sizeof(float));
+                for (uint32_t t = b; t < e2; t++)
                     hinfo[tb + t] = (fr.tris.info[t] & ~PFF_STATE_MASK) | ((fr.tris.info[t] & PFF_STATE_MASK) + sb);
-            }
-            tb += (uint32_t)nt;
+            });
+            tb += fr.tris.count;
             sb += fr.numStates;
         }
         if ((rc = up(J->pos, hpos.data(), hpos.size() * 4)) || (rc = up(J->col, hcol.data(), hcol.size() * 4))
             || (rc = up(J->info, hinfo.data(), hinfo.size() * 4)) || (rc = up(J->uv, huv.data(), huv.size() * 4))
             || (rc = up(J->nrm, hnrm.data(), hnrm.size() * 4)))
             return rc;
+    }
+    if (numDraws) {
+        // Array draws: element copies into one pool, then their triangles.
+        if ((rc = ensure(J, J->arrPool, arrBytes)) || (rc = ensure(J, J->arrTable, (size_t)numDraws * sizeof(DArrays))))
+            return rc;
+        std::vector<DArrays> tab;
+        tab.reserve(numDraws);
+        uint8_t* dst = (uint8_t*)J->arrPool.p;
+        uint32_t tb = 0, sb = 0, cum = 0;
+        auto put = [&](const void* h, size_t bytes) -> const uint8_t* {
+            if (!h) return nullptr;
+            uint8_t* d = dst;
+            if (bytes && e == hipSuccess) e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+            dst += align256(bytes);
+            return d;
+        };
+        for (uint32_t f = 0; f < count; f++) {
+            const PFFframe& fr = frames[f];
+            for (uint32_t k = 0; k < fr.numArrays; k++) {
+                const PFFarrays& a = fr.arrays[k];
+                const size_t n = a.numElements;
+                DArrays d;
+                memset(&d, 0, sizeof d);
+                d.pos = put(a.position, n * a.posSize * elem_bytes(a.posType));
+                d.nrm = put(a.normal, n * 3 * elem_bytes(a.nrmType));
+                d.uv = put(a.texcoord, n * 2 * elem_bytes(a.uvType));
+                d.col = put(a.colors, n * a.colSize * elem_bytes(a.colType));
+                d.idx = put(a.indices, (size_t)a.count * elem_bytes(a.idxType));
+                d.posSize = (uint32_t)a.posSize; d.posType = a.posType; d.nrmType = a.nrmType; d.uvType = a.uvType;
+                d.colSize = (uint32_t)a.colSize; d.colType = a.colType; d.idxType = a.idxType;
+                d.mode = a.mode;
+                d.perBuf = a.mode == 2u ? 1u : (a.mode == 6u || a.mode == 7u) ? 4u : 2u;
+                d.stride = (a.mode == 2u || a.mode == 3u || a.mode == 4u) ? 3u : 4u;
+                d.numFaces = a.numFaces;
+                d.facePass0 = a.facePasses[0];
+                d.facePass1 = a.facePasses[1];
+                d.state = a.state + sb;
+                d.color = a.color;
+                d.elementBase = a.elementBase;
+                d.numElements = a.numElements;
+                d.slot0 = tb + a.triFirst;
+                d.cum = cum;
+                cum += a.triCount;
+                tab.push_back(d);
+            }
+            tb += fr.tris.count;
+            sb += fr.numStates;
+        }
+        if (e == hipSuccess) e = hipMemcpy(J->arrTable.p, tab.data(), tab.size() * sizeof(DArrays), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { set_error("array draw upload", e); return PFD_E_LAUNCH; }
+        if (arrTris) {
+            hipLaunchKernelGGL(k_deindex, dim3((uint32_t)((arrTris + 255) / 256)), dim3(256), 0, J->stream,
+                               (const DArrays*)J->arrTable.p, numDraws, (uint32_t)arrTris, (float4*)J->pos.p,
+                               (uint32_t*)J->col.p, anyUV ? (float2*)J->uv.p : nullptr,
+                               anyNrm ? (float*)J->nrm.p : nullptr, (uint32_t*)J->info.p,
+                               vidOn ? (uint32_t*)J->vid.p : nullptr);
+            HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
+        }
     }
     if ((rc = up(J->states, hstates.data(), hstates.size() * sizeof(PFFstate)))
         || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex))))
@@ -3966,7 +4221,7 @@ int pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, P
     if ((rc = job_shell(&J, true))) return rc;
     J->cfg = cfg ? *cfg : g_defaultCfg;
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
-    if ((rc = job_plan(J, frames, count))) { pfdJobDestroy(J); return rc; }
+    if ((rc = job_plan(J, frames, count, false))) { pfdJobDestroy(J); return rc; }
     *out = J;
     return PFD_OK;
 }
@@ -4193,7 +4448,7 @@ int run_one_shot(PFDjob* J, const PFFframe* frames, uint32_t count) {
     J->cfg = g_defaultCfg;
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
     J->autoTiles = J->cfg.tileShift == 0;
-    int rc = job_plan(J, frames, count);
+    int rc = job_plan(J, frames, count, false);
     if (!rc && !(rc = pfdJobUploadTargets(J)) && !(rc = run_all(J, true)) && !(rc = pfdJobSync(J)))
         rc = pfdJobDownload(J);
     return rc;
@@ -4305,20 +4560,20 @@ int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
             return PFD_E_INVALID;
         }
     }
-    PFDjob* J = nullptr;
-    int rc = pfdJobCreate(frame, 1, nullptr, &J);
+    int rc = pfdInit();
     if (rc) return rc;
+    PFDjob* J = nullptr;
+    if ((rc = job_shell(&J, true))) return rc;
+    J->cfg = g_defaultCfg;
+    J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
+    // The corners' colour-array indices (host-recorded or formed from array
+    // draws) address the colour parameters.
+    if ((rc = job_plan(J, frame, 1, true))) { pfdJobDestroy(J); return rc; }
     J->diff = true;
     const size_t npx = (size_t)frame->width * frame->height;
     if ((rc = ensure(J, J->winner, npx * 4)) || (rc = ensure(J, J->image, npx * 16)) || (rc = ensure(J, J->dImage, npx * 16))) {
         pfdJobDestroy(J);
         return rc;
-    }
-    if (frame->tris.vertexId && frame->tris.count) {
-        const size_t bytes = (size_t)frame->tris.count * 12;
-        if ((rc = ensure(J, J->vid, bytes))) { pfdJobDestroy(J); return rc; }
-        hipError_t e = hipMemcpy(J->vid.p, frame->tris.vertexId, bytes, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { set_error("vertex ids", e); pfdJobDestroy(J); return PFD_E_LAUNCH; }
     }
     *out = J;
     return PFD_OK;

@@ -107,6 +107,7 @@ typedef struct {                       /* the frame being recorded */
     PFFpixels* pix;      size_t npix, cappix;   /* pfDrawPixels calls; images owned */
     PFFpoint*  pts;      size_t npts, cappts;
     PFFline*   lns;      size_t nlns, caplns;
+    PFFarrays* arr;      size_t narr, caparr;   /* array draws; element copies owned */
     int        anyTex, anyLight, anyVid;
 } Rec;
 
@@ -176,6 +177,7 @@ static void* g_execUser = NULL;
 static PFXbatchExecutor g_batchExec = NULL;
 static void* g_batchUser = NULL;
 static uint32_t g_texGen = 1;          /* bumped by pfTextureParameter */
+static int g_arrayDraws = 1;           /* pfxSetArrayDraws: record array draws whole */
 
 /* ------------------------------------------------------------------------ */
 /* Matrix helpers (pfm.h); same evaluation order as the reference.           */
@@ -310,6 +312,8 @@ static int grow(void** p, size_t* cap, size_t need, size_t elem)
 static void rec_reset(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
+    for (size_t i = 0; i < r->narr; i++) free((void*)r->arr[i].position);   /* one block per draw */
+    r->narr = 0;
     r->npix = 0;
     r->npts = 0;
     r->nlns = 0;
@@ -322,6 +326,8 @@ static void rec_reset(Rec* r)
 static void rec_free(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
+    for (size_t i = 0; i < r->narr; i++) free((void*)r->arr[i].position);
+    free(r->arr);
     free(r->pix);
     free(r->pts);
     free(r->lns);
@@ -365,6 +371,8 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->numPoints = (uint32_t)r->npts;
     f->lines = r->lns;
     f->numLines = (uint32_t)r->nlns;
+    f->arrays = r->arr;
+    f->numArrays = (uint32_t)r->narr;
 }
 
 /* Executes the recorded frame, if any, and starts an empty one. */
@@ -506,11 +514,11 @@ static void push_cmd(Ctx* c, const PFFcmd* cmd)
     r->cmds[r->ncmd++] = *cmd;
 }
 
-/* Grows the triangle streams to hold at least one more triangle. */
-static int rec_grow(Rec* r)
+/* Grows the triangle streams to hold at least `need` triangles. */
+static int rec_grow(Rec* r, size_t need)
 {
     size_t nc = r->captri ? r->captri : 1024;
-    while (nc < r->ntri + 1) nc *= 2;
+    while (nc < need) nc *= 2;
     float* pos = realloc(r->pos, nc * 12 * sizeof(float));
     if (pos) r->pos = pos;
     uint32_t* col = realloc(r->col, nc * 3 * sizeof(uint32_t));
@@ -539,7 +547,7 @@ static void emit_tri(Ctx* c, const Vert* a, const Vert* b, const Vert* v3, uint3
     int si = current_state(c);
     if (si < 0) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
     Rec* r = &c->rec;
-    if (r->ntri + 1 > r->captri && !rec_grow(r)) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
+    if (r->ntri + 1 > r->captri && !rec_grow(r, r->ntri + 1)) { set_err(PF_ERROR_OUT_OF_MEMORY); return; }
     const size_t t = r->ntri;
     const Vert* v[3] = { a, b, v3 };
     float* pos = r->pos + 12 * t;
@@ -1585,6 +1593,133 @@ static void draw_array_vertex(Ctx* c, size_t j)
     }
 }
 
+static size_t type_bytes(PFenum t)
+{
+    switch (t) {
+    case PF_UNSIGNED_BYTE: return 1;
+    case PF_UNSIGNED_SHORT: case PF_SHORT: return 2;
+    case PF_DOUBLE: return 8;
+    default: return 4;
+    }
+}
+
+/* A pfDrawElements / pfDrawArrays call whose primitives all become filled
+ * triangles is recorded as one array draw (PFFarrays): the elements it
+ * reads (0 .. the largest index, or first .. first + count - 1) are copied
+ * into the frame now, since the caller may change the arrays afterwards,
+ * and the device forms the triangles, in the order the per-vertex loop
+ * below would record them.  Returns 0 when the call needs that loop: a
+ * list is being recorded, a point or line polygon mode applies, or
+ * pfDrawArrays starts before element 0. */
+static int record_array_draw(Ctx* c, PFdrawmode mode, PFsizei count, PFdatatype itype, const void* indices,
+                             int64_t first)
+{
+    if (c->curList || count <= 0 || mode < PF_TRIANGLES || mode > PF_QUAD_STRIP || first < 0) return 0;
+    const int culled = (c->state & PF_CULL_FACE) != 0;
+    const int face = culled ? !c->cullFace : PF_FRONT_AND_BACK;
+    uint32_t groups = 1, passes[2] = { 0, 0 };
+    const int polyModes = (mode == PF_TRIANGLES || mode == PF_QUADS);   /* ictx.c:104-136 */
+    if (face == PF_FRONT_AND_BACK) {
+        if (polyModes && (c->polyMode[0] != PF_FILL || c->polyMode[1] != PF_FILL)) return 0;
+        if (mode == PF_TRIANGLES) passes[0] = PFF_PASS_FRONT | PFF_PASS_BACK;
+        else { groups = 2; passes[0] = PFF_PASS_FRONT; passes[1] = PFF_PASS_BACK; }
+    } else {
+        if (polyModes && c->polyMode[face] != PF_FILL) return 0;
+        passes[0] = face_pass(face);
+    }
+    const uint32_t bufSize = mode_vertex_count(mode);
+    const uint32_t keep = (mode == PF_TRIANGLE_FAN || mode == PF_TRIANGLE_STRIP) ? 1u
+                        : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 2u : 0u;
+    const uint32_t perBuffer = (mode == PF_TRIANGLES) ? 1u : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 4u : 2u;
+    const uint64_t fills = (uint64_t)count < bufSize ? 0 : 1 + ((uint64_t)count - bufSize) / (bufSize - keep);
+    const uint64_t tris = fills * groups * perBuffer;
+    pfBegin(mode);                                   /* matrices, as the loop's pfBegin */
+    if (tris == 0) { pfEnd(); return 1; }
+    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); pfEnd(); return 1; }
+    int si = current_state(c);
+    Rec* r = &c->rec;
+    if (si < 0 || r->ntri + tris > 0xFFFFFF00u) { set_err(PF_ERROR_OUT_OF_MEMORY); pfEnd(); return 1; }
+    /* Elements read. */
+    uint64_t lo = 0, n = 0;
+    const size_t isz = indices ? type_bytes(itype) : 0;
+    if (indices) {
+        uint32_t mx = 0;
+        for (PFsizei i = 0; i < count; i++) {
+            uint32_t j = itype == PF_UNSIGNED_BYTE ? ((const PFubyte*)indices)[i]
+                       : itype == PF_UNSIGNED_SHORT ? ((const PFushort*)indices)[i] : ((const PFuint*)indices)[i];
+            if (j > mx) mx = j;
+        }
+        n = (uint64_t)mx + 1;
+    } else {
+        lo = (uint64_t)first;
+        n = (uint64_t)count;
+    }
+    const int useUV = (c->state & PF_TEXTURE_COORD_ARRAY) && c->aUV.buffer;
+    const int useN = (c->state & PF_NORMAL_ARRAY) && c->aNrm.buffer;
+    const int useC = (c->state & PF_COLOR_ARRAY) && c->aCol.buffer;
+    const size_t pb = (size_t)c->aPos.size * type_bytes(c->aPos.type);
+    const size_t nb = useN ? 3 * type_bytes(c->aNrm.type) : 0;
+    const size_t tb = useUV ? 2 * type_bytes(c->aUV.type) : 0;
+    const size_t cb = useC ? (size_t)c->aCol.size * type_bytes(c->aCol.type) : 0;
+#define ALIGN16(x) (((x) + 15) & ~(size_t)15)
+    const size_t total = ALIGN16(n * pb) + ALIGN16(n * nb) + ALIGN16(n * tb) + ALIGN16(n * cb) + (size_t)count * isz + 16;
+    uint8_t* blk = (uint8_t*)malloc(total);
+    if (!blk || !grow((void**)&r->arr, &r->caparr, r->narr + 1, sizeof(PFFarrays))
+        || (r->ntri + tris > r->captri && !rec_grow(r, r->ntri + tris))) {
+        free(blk);
+        set_err(PF_ERROR_OUT_OF_MEMORY);
+        pfEnd();
+        return 1;
+    }
+    PFFarrays* d = &r->arr[r->narr];
+    memset(d, 0, sizeof *d);
+    uint8_t* q = blk;
+    memcpy(q, (const uint8_t*)c->aPos.buffer + lo * pb, n * pb);
+    d->position = q; d->posSize = c->aPos.size; d->posType = (uint32_t)c->aPos.type;
+    q += ALIGN16(n * pb);
+    if (useN) { memcpy(q, (const uint8_t*)c->aNrm.buffer + lo * nb, n * nb); d->normal = q; d->nrmType = (uint32_t)c->aNrm.type; }
+    q += ALIGN16(n * nb);
+    if (useUV) { memcpy(q, (const uint8_t*)c->aUV.buffer + lo * tb, n * tb); d->texcoord = q; d->uvType = (uint32_t)c->aUV.type; }
+    q += ALIGN16(n * tb);
+    if (useC) {
+        memcpy(q, (const uint8_t*)c->aCol.buffer + lo * cb, n * cb);
+        d->colors = q; d->colSize = c->aCol.size; d->colType = (uint32_t)c->aCol.type;
+    }
+    q += ALIGN16(n * cb);
+    if (indices) { memcpy(q, indices, (size_t)count * isz); d->indices = q; d->idxType = (uint32_t)itype; }
+#undef ALIGN16
+    d->mode = (uint32_t)mode;
+    d->count = (uint32_t)count;
+    d->triFirst = (uint32_t)r->ntri;
+    d->triCount = (uint32_t)tris;
+    d->numFaces = groups;
+    d->facePasses[0] = passes[0];
+    d->facePasses[1] = passes[1];
+    d->state = (uint32_t)si;
+    d->color = pack(c->curColor);
+    d->elementBase = (uint32_t)lo;
+    d->numElements = (uint32_t)n;
+    r->narr++;
+    if (useC && !r->anyVid) {                        /* corners before this draw have no index */
+        memset(r->vid, 0xFF, r->ntri * 3 * sizeof(uint32_t));
+        r->anyVid = 1;
+    }
+    const size_t t = r->ntri;
+    r->ntri += (size_t)tris;
+    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_DRAW && r->cmds[r->ncmd - 1].last == t) {
+        r->cmds[r->ncmd - 1].last = (uint32_t)r->ntri;
+    } else {
+        PFFcmd cmd;
+        memset(&cmd, 0, sizeof cmd);
+        cmd.kind = PFF_CMD_DRAW;
+        cmd.first = (uint32_t)t;
+        cmd.last = (uint32_t)r->ntri;
+        push_cmd(c, &cmd);
+    }
+    pfEnd();
+    return 1;
+}
+
 static int draw_array_ready(Ctx* c, PFdrawmode mode)
 {
     if (!(c->state & PF_VERTEX_ARRAY)) { c->err = PF_INVALID_OPERATION; return 0; }
@@ -1603,6 +1738,7 @@ void pfDrawElements(PFdrawmode mode, PFsizei count, PFdatatype type, const void*
         return;
     }
     if (!draw_array_ready(c, mode)) return;
+    if (g_arrayDraws && record_array_draw(c, mode, count, type, indices, 0)) return;
     draw_array_begin(c, mode);
     for (PFsizei i = 0; i < count; i++) {
         size_t j = type == PF_UNSIGNED_BYTE ? ((const PFubyte*)indices)[i]
@@ -1618,6 +1754,7 @@ void pfDrawArrays(PFdrawmode mode, PFint first, PFsizei count)
     Ctx* c = G;
     if (!c) return;
     if (!draw_array_ready(c, mode)) return;
+    if (g_arrayDraws && record_array_draw(c, mode, count, PF_UNSIGNED_INT, NULL, first)) return;
     draw_array_begin(c, mode);
     for (PFsizei i = 0; i < count; i++) draw_array_vertex(c, (size_t)((int64_t)first + i));
     pfEnd();
@@ -2492,6 +2629,8 @@ void pfxSetBatchExecutor(PFXbatchExecutor fn, void* user)
 }
 
 PFuint pfxFrameCount(void) { return G ? G->frames : 0; }
+
+void pfxSetArrayDraws(PFboolean enabled) { g_arrayDraws = enabled ? 1 : 0; }
 
 /* ------------------------------------------------------------------------ */
 /* Differentiable rendering                                                  */

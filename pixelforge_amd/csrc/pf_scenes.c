@@ -103,6 +103,11 @@ typedef struct {
                               * of the triangles (random kinds)                  */
     float    uvMin, uvMax;   /* random kinds: texture coordinates uniform in
                               * [uvMin, uvMax) when uvMax > uvMin, else [-2, 2) */
+    int32_t  arrayMode;      /* mesh arrays: 0 PF_TRIANGLES; a PFdrawmode from
+                              * PF_TRIANGLE_FAN to PF_QUAD_STRIP draws each row of
+                              * cells as one call in that mode (strip order:
+                              * v(i, j), v(i, j+1), v(i+1, j), ...; PF_QUADS: the
+                              * cells' four corners)                            */
 } PFSparams;
 
 typedef struct {
@@ -643,16 +648,29 @@ static void scene_mesh(const PFSapi* a, Rng* r, const PFSparams* p)
                 Cf[4 * v + k] = (float)((int)(c * 256.0f)) / 256.0f;
             }
         }
-    size_t ni = (size_t)NX * NZ * 6;
+    const int rowMode = p->arrays && p->arrayMode >= PF_TRIANGLE_FAN && p->arrayMode <= PF_QUAD_STRIP;
+    const size_t perRow = !rowMode ? (size_t)NX * 6 : p->arrayMode == PF_QUADS ? (size_t)NX * 4 : (size_t)(NX + 1) * 2;
+    size_t ni = perRow * NZ;
     uint32_t* I = (uint32_t*)malloc(ni * sizeof(uint32_t));
     size_t q = 0;
-    for (int j = 0; j < NZ; j++)
+    for (int j = 0; j < NZ; j++) {
+        if (rowMode && p->arrayMode != PF_QUADS) {
+            for (int i = 0; i <= NX; i++) {
+                I[q++] = (uint32_t)(j * (NX + 1) + i);
+                I[q++] = (uint32_t)((j + 1) * (NX + 1) + i);
+            }
+            continue;
+        }
         for (int i = 0; i < NX; i++) {
             uint32_t v00 = (uint32_t)(j * (NX + 1) + i), v10 = v00 + 1;
             uint32_t v01 = v00 + (uint32_t)(NX + 1), v11 = v01 + 1;
+            if (rowMode) { I[q++] = v00; I[q++] = v10; I[q++] = v11; I[q++] = v01; continue; }
             I[q++] = v00; I[q++] = v10; I[q++] = v11;
             I[q++] = v00; I[q++] = v11; I[q++] = v01;
         }
+    }
+    const PFdrawmode dm = rowMode ? (PFdrawmode)p->arrayMode : PF_TRIANGLES;
+    const size_t calls = rowMode ? (size_t)NZ : 1, perCall = rowMode ? perRow : ni;
     if (p->arrays == 0) {
         a->Begin(PF_TRIANGLES);
         for (size_t k = 0; k < ni; k++) {
@@ -686,7 +704,8 @@ static void scene_mesh(const PFSapi* a, Rng* r, const PFSparams* p)
         a->TexCoordPointer(PF_FLOAT, 0, ta);
         a->ColorPointer(colSize, (PFenum)colType, 0, ca);
         if (expand) {
-            a->DrawArrays(PF_TRIANGLES, (PFint)first, (PFsizei)ni);
+            for (size_t k = 0; k < calls; k++)
+                a->DrawArrays(dm, (PFint)(first + k * perCall), (PFsizei)perCall);
         } else {
             int it = idxTypes[(p->indexType >= 0 && p->indexType < 3) ? p->indexType : 0];
             void* ix = malloc(ni * type_size(it));
@@ -695,7 +714,8 @@ static void scene_mesh(const PFSapi* a, Rng* r, const PFSparams* p)
                 else if (it == PF_UNSIGNED_SHORT) ((uint16_t*)ix)[k] = (uint16_t)I[k];
                 else ((uint32_t*)ix)[k] = I[k];
             }
-            a->DrawElements(PF_TRIANGLES, (PFsizei)ni, (PFdatatype)it, ix);
+            for (size_t k = 0; k < calls; k++)
+                a->DrawElements(dm, (PFsizei)perCall, (PFdatatype)it, (const uint8_t*)ix + k * perCall * type_size(it));
             free(ix);
         }
         /* The reference reads the arrays inside the draw call; this library

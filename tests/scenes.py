@@ -130,6 +130,16 @@ ARRAY_SCENES = [
     ("mesh_colors_double", dict(_MESH, seed=81, arrays=1, colType=4)),
     ("mesh_colors_rgb", dict(_MESH, seed=82, arrays=1, colSize=3)),
     ("mesh_gouraud_cull", dict(_MESH, seed=83, arrays=1, lighting=1, cull=pf.PF_BACK)),
+    # every other triangle mode, one draw call per row of cells (strip order,
+    # or the cells' corners for PF_QUADS), culling off, front or back
+    ("mesh_strip_elements", dict(_MESH, seed=84, arrays=1, arrayMode=pf.PF_TRIANGLE_STRIP)),
+    ("mesh_strip_cull_back", dict(_MESH, seed=84, arrays=1, arrayMode=pf.PF_TRIANGLE_STRIP, cull=pf.PF_BACK)),
+    ("mesh_fan_cull_front", dict(_MESH, seed=85, arrays=1, arrayMode=pf.PF_TRIANGLE_FAN, cull=pf.PF_FRONT,
+                                 indexType=2)),
+    ("mesh_quads_elements", dict(_MESH, seed=86, arrays=1, arrayMode=pf.PF_QUADS)),
+    ("mesh_quads_arrays_cull", dict(_MESH, seed=86, arrays=2, arrayMode=pf.PF_QUADS, cull=pf.PF_BACK)),
+    ("mesh_quad_strip_arrays", dict(_MESH, seed=87, arrays=2, arrayMode=pf.PF_QUAD_STRIP, lighting=1)),
+    ("mesh_quad_fan_elements", dict(_MESH, seed=88, arrays=1, arrayMode=pf.PF_QUAD_FAN, colType=1)),
 ]
 
 # Texture formats: every (format, type) pair with a texel getter other than

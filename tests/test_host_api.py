@@ -134,10 +134,15 @@ class Cmd(C.Structure):
 
 
 class Frame(C.Structure):
+    """PFFframe (include/pf_frame.h)."""
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("color", C.c_void_p), ("depth", C.c_void_p),
                 ("tris", Tris), ("states", C.c_void_p), ("numStates", C.c_uint32),
                 ("textures", C.c_void_p), ("numTextures", C.c_uint32),
-                ("cmds", C.POINTER(Cmd)), ("numCmds", C.c_uint32)]
+                ("cmds", C.POINTER(Cmd)), ("numCmds", C.c_uint32),
+                ("pixelOps", C.c_void_p), ("numPixelOps", C.c_uint32),
+                ("points", C.c_void_p), ("numPoints", C.c_uint32),
+                ("lines", C.c_void_p), ("numLines", C.c_uint32),
+                ("arrays", C.c_void_p), ("numArrays", C.c_uint32)]
 
 
 def test_recorded_frame_structure():

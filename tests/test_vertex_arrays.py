@@ -16,7 +16,7 @@ import pytest
 
 import pixelforge_amd as pf
 import scenes
-from conftest import GOLDEN
+from conftest import GOLDEN, _oracle
 from test_host_api import Frame, EXEC
 
 PF_FLOAT, PF_DOUBLE, PF_UBYTE, PF_USHORT, PF_UINT, PF_SHORT, PF_INT = 10, 11, 0, 1, 5, 7, 8
@@ -34,20 +34,37 @@ def _ctx(w=32, h=24):
     return L, ctx, buf
 
 
+@pytest.fixture(params=[True, False], ids=["array-draws", "vertex-loop"])
+def array_draws(request):
+    """Runs a test with pfDrawElements / pfDrawArrays recorded whole (the
+    default: the triangles are formed later, here by the oracle's
+    restatement of the reference's vertex loop) and vertex by vertex on the
+    host; both must record the same triangles."""
+    pf.lib().pfxSetArrayDraws(1 if request.param else 0)
+    yield request.param
+    pf.lib().pfxSetArrayDraws(1)
+
+
 class Capture:
-    """Executor keeping copies of each frame's triangle streams."""
+    """Executor keeping copies of each frame's triangle streams, with array
+    draws expanded by the oracle."""
 
     def __init__(self):
         self.frames = []
+        self.arrays = []
         self.cb = EXEC(self._run)
 
     def _run(self, fp, user):
-        f = C.cast(fp, C.POINTER(Frame)).contents
+        raw = C.cast(fp, C.POINTER(Frame)).contents
+        self.arrays.append(raw.numArrays)
+        f = Frame()
+        assert _oracle.pforacle_expand(C.c_void_p(fp), C.byref(f)) == 0
         n = f.tris.count
         get = lambda p, k, t: np.ctypeslib.as_array(C.cast(p, C.POINTER(t)), (n * k,)).copy() if p else None
         self.frames.append(dict(pos=get(f.tris.position, 12, C.c_float), col=get(f.tris.color, 3, C.c_uint32),
                                 uv=get(f.tris.texcoord, 6, C.c_float), nrm=get(f.tris.normal, 9, C.c_float),
                                 vid=get(f.tris.vertexId, 3, C.c_uint32)))
+        _oracle.pforacle_expand_free()
         return 0
 
     def __enter__(self):
@@ -84,7 +101,7 @@ def test_array_goldens_equal_immediate_mode():
         assert np.array_equal(load(n)[0], c1), n
 
 
-def test_draw_elements_records_the_immediate_mode_triangles():
+def test_draw_elements_records_the_immediate_mode_triangles(array_draws):
     L, ctx, _ = _ctx()
     rng = np.random.default_rng(5)
     pos = rng.uniform(-1, 1, (6, 3)).astype(np.float32)
@@ -121,13 +138,14 @@ def test_draw_elements_records_the_immediate_mode_triangles():
     finally:
         L.pfDeleteContext(ctx)
     imm, arr = cap.frames
+    assert cap.arrays == [0, 1 if array_draws else 0]
     for k in ("pos", "col", "uv", "nrm"):
         assert np.array_equal(imm[k], arr[k]), k
     assert imm["vid"] is None
     assert arr["vid"].tolist() == idx.tolist()
 
 
-def test_draw_arrays_types_and_missing_attributes():
+def test_draw_arrays_types_and_missing_attributes(array_draws):
     """Positions of every type, w = 1 and zero z for size 2; attributes with
     no enabled array: zero normal and texcoord, current colour, no index."""
     L, ctx, _ = _ctx()
@@ -153,7 +171,7 @@ def test_draw_arrays_types_and_missing_attributes():
     assert f["vid"] is None
 
 
-def test_color_array_conversions():
+def test_color_array_conversions(array_draws):
     """ubyte as is, ushort >> 8, uint >> 24, float/double (ubyte)(x * 255);
     missing alpha is 255 (context.c:1356-1392)."""
     L, ctx, _ = _ctx()
@@ -290,3 +308,20 @@ def test_framebuffer_pixel_setters():
         L.pfDeleteFramebuffer(C.byref(fb))
     finally:
         L.pfDeleteContext(ctx)
+
+
+@pytest.mark.parametrize("name,kw", scenes.ARRAY_SCENES, ids=[n for n, _ in scenes.ARRAY_SCENES])
+def test_vertex_loop_recording_matches_golden(name, kw):
+    """The array scenes recorded vertex by vertex on the host (the path the
+    library keeps for lists and point / line polygon modes) give the
+    reference's image too; test_oracle_golden covers them recorded whole."""
+    from conftest import render_oracle
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    pf.lib().pfxSetArrayDraws(0)
+    try:
+        c, d, e, _ = render_oracle(scenes.params(kw))
+    finally:
+        pf.lib().pfxSetArrayDraws(1)
+    assert e == 0
+    assert np.count_nonzero(c != c[0, 0]) > 1000
+    assert np.array_equal(c, g["color"]) and np.array_equal(d.view(np.uint32), g["depth"].view(np.uint32))
