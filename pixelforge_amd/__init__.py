@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpixelforge_amd.so")
+# PIXELFORGE_AMD_LIB selects another build of the library (the sanitized
+# host layer of `python -m pixelforge_amd.build --sanitize`).
+LIB_PATH = os.path.abspath(os.environ.get("PIXELFORGE_AMD_LIB", os.path.join(HERE, "libpixelforge_amd.so")))
 SCENES_PATH = os.path.join(HERE, "libpf_scenes.so")
 
 # Scene kinds (csrc/pf_scenes.c)

@@ -1,7 +1,8 @@
 """Builds the native libraries of pixelforge_amd in-tree.
 
-    python -m pixelforge_amd.build          # everything
-    python -m pixelforge_amd.build --check  # exit 1 if a library is missing
+    python -m pixelforge_amd.build             # everything
+    python -m pixelforge_amd.build --check     # exit 1 if a library is missing
+    python -m pixelforge_amd.build --sanitize  # host layer under ASan/UBSan (_san/)
 
 Outputs (next to this file):
     libpixelforge_amd.so  drop-in PixelForge API: host C state machine
@@ -50,6 +51,35 @@ def hipcc():
     return path
 
 
+SAN_DIR = os.path.join(HERE, "_san")
+SAN_LIB = os.path.join(SAN_DIR, "libpixelforge_amd.so")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined", "-g"]
+
+
+def build_sanitized():
+    """The host layer (pf_host.c) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer into _san/libpixelforge_amd.so, linked with
+    the same device object.  Run the CPU tests against it with
+
+        LD_PRELOAD=$(gcc -print-file-name=libasan.so) ASAN_OPTIONS=detect_leaks=0 \
+        PIXELFORGE_AMD_LIB=pixelforge_amd/_san/libpixelforge_amd.so \
+        python -m pytest tests -m "not gpu" -p no:xdist
+
+    (the sanitizer runtime must be loaded first, hence LD_PRELOAD; leaks of
+    the Python interpreter itself are not reported)."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    dev_o = os.path.join(BUILD, "pf_device.o")
+    if not os.path.exists(dev_o):
+        build()
+    host_o = os.path.join(SAN_DIR, "pf_host.o")
+    _run(["gcc"] + C_FLAGS + SAN_FLAGS + ["-I", INCLUDE, "-c", os.path.join(CSRC, "pf_host.c"), "-o", host_o])
+    tmp = SAN_LIB + ".tmp"
+    _run(["g++", "-shared", "-fPIC", "-o", tmp, dev_o, host_o, "-L/opt/rocm/lib", "-lamdhip64", "-lubsan",
+          "-Wl,-Bsymbolic", "-lm"])
+    os.replace(tmp, SAN_LIB)
+    return SAN_LIB
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     dev_o = os.path.join(BUILD, "pf_device.o")
@@ -74,4 +104,7 @@ def built():
 if __name__ == "__main__":
     if "--check" in sys.argv:
         sys.exit(0 if built() else 1)
-    build()
+    if "--sanitize" in sys.argv:
+        build_sanitized()
+    else:
+        build()
