@@ -18,6 +18,7 @@
 #ifndef PF_DEVICE_H
 #define PF_DEVICE_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include "pf_frame.h"
 
@@ -105,6 +106,15 @@ int pfdExecuteFrames(const PFFframe* frames, uint32_t count);
  * the calls of a process with a lock.  Results equal pfdExecuteFrame's. */
 #define PFD_SCRATCH_BYTES_MIB 128
 int pfdExecuteFrameNoAlloc(const PFFframe* frame);
+
+/* Device copies of host bytes that outlive frames (no reference
+ * counterpart): render lists keep their recorded vertices resident, and
+ * frames that call them reference the copy (PFFarrays.device) instead of
+ * uploading the vertices again. */
+typedef struct PFDresident PFDresident;
+int         pfdResidentCreate(const void* host, size_t bytes, PFDresident** out);
+const void* pfdResidentPointer(const PFDresident* r);
+void        pfdResidentDestroy(PFDresident* r);
 
 /* Resident jobs, for replay and measurement. */
 int  pfdJobCreate(const PFFframe* frames, uint32_t count, const PFDconfig* cfg, PFDjob** out);

@@ -232,7 +232,22 @@ typedef struct {
                                                                    PF_DOUBLE; NULL: none */
     const void* indices;   uint32_t idxType;                    /* PF_UNSIGNED_BYTE/SHORT/INT; NULL:
                                                                    vertex i is element i */
+    uint32_t    flags;         /* PFF_ARRAYS_* */
+    float       uvMatrix[6];   /* PFF_ARRAYS_TEXMAT: u' = (m0 u + m2 v) + m4,
+                                  v' = (m1 u + m3 v) + m5 (pfmVec2Transform) */
+    const void* device;        /* non-NULL: a device copy of the elements, the
+                                  arrays at the same offsets from it as from
+                                  `position` on the host */
 } PFFarrays;
+
+/* PFFarrays.flags.  A call of a render list (renderlist.c:69-97) is an
+ * array draw of the list's recorded vertices whose colours come through
+ * pfColor4ubv (no colour-array index), texture coordinates through
+ * pfTexCoordfv (the current texture matrix applies) and normals through
+ * pfNormal3fv (normalised under PF_NORMALIZE, unless of zero length). */
+#define PFF_ARRAYS_LIST       0x1u  /* colours carry no colour-array index */
+#define PFF_ARRAYS_TEXMAT     0x2u  /* texture coordinates through uvMatrix */
+#define PFF_ARRAYS_NORMALIZE  0x4u  /* normals normalised */
 
 typedef struct {
     uint32_t           width, height;    /* render target, RGBA8 + float32 depth */

@@ -1497,7 +1497,23 @@ static void expand_arrays(const PFFarrays* d, float* pos, uint32_t* col, float* 
             uint32_t ch[4] = { 255, 255, 255, 255 };
             for (int k = 0; k < d->colSize; k++) ch[k] = elem_u8(d->colors, d->colType, (size_t)j * d->colSize + k);
             v->color = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (ch[3] << 24);
-            v->vid = j + d->elementBase;
+            if (!(d->flags & PFF_ARRAYS_LIST)) v->vid = j + d->elementBase;
+        }
+        /* A render-list call replays each vertex through pfTexCoordfv and
+         * pfNormal3fv (renderlist.c:84-89): the current texture matrix
+         * (pfm.h:954-962) and PF_NORMALIZE (context.c:1895-1913) apply. */
+        if (d->flags & PFF_ARRAYS_TEXMAT) {
+            const float* m = d->uvMatrix;
+            float u = v->uv[0], w = v->uv[1];
+            v->uv[0] = m[0] * u + m[2] * w + m[4];
+            v->uv[1] = m[1] * u + m[3] * w + m[5];
+        }
+        if (d->flags & PFF_ARRAYS_NORMALIZE) {
+            float sq = v->normal[0] * v->normal[0] + v->normal[1] * v->normal[1] + v->normal[2] * v->normal[2];
+            if (sq != 0.0f) {
+                float inv = 1.0f / sqrtf(sq);
+                for (int k = 0; k < 3; k++) v->normal[k] = v->normal[k] * inv;
+            }
         }
         if (n < bufSize) continue;
         for (uint32_t g = 0; g < d->numFaces; g++) {

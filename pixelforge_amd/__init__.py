@@ -53,7 +53,8 @@ class SceneParams(C.Structure):
         ("fog", C.c_int32), ("fogStart", C.c_float), ("fogEnd", C.c_float), ("fogAlpha", C.c_int32),
         ("points", C.c_int32), ("pointSize", C.c_float), ("polyPoint", C.c_int32),
         ("lines", C.c_int32), ("lineWidth", C.c_float), ("polyLine", C.c_int32),
-        ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float), ("arrayMode", C.c_int32),
+        ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float), ("listOptions", C.c_int32),
+        ("arrayMode", C.c_int32),
     ]
 
 

@@ -2951,6 +2951,8 @@ struct DArrays {
     uint32_t color, elementBase, numElements;
     uint32_t slot0;            // global slot of the draw's first triangle
     uint32_t cum;              // array triangles of the job before this draw
+    uint32_t flags;            // PFF_ARRAYS_*
+    float    uvm[6];           // texture matrix (PFF_ARRAYS_TEXMAT)
 };
 
 __device__ __forceinline__ float arr_f(const uint8_t* b, uint32_t type, size_t i) {
@@ -3021,18 +3023,31 @@ __global__ __launch_bounds__(256) void k_deindex(const DArrays* __restrict__ D, 
             uint32_t ch[4] = {255u, 255u, 255u, 255u};
             for (uint32_t c = 0; c < A.colSize; c++) ch[c] = arr_u8(A.col, A.colType, (size_t)j * A.colSize + c);
             cw = ch[0] | (ch[1] << 8) | (ch[2] << 16) | (ch[3] << 24);
-            vi = j + A.elementBase;
+            if (!(A.flags & PFF_ARRAYS_LIST)) vi = j + A.elementBase;
         }
         col[3 * t + k] = cw;
         if (vid) vid[3 * t + k] = vi;
         if (uv) {
             float2 q = make_float2(0.0f, 0.0f);
             if (A.uv) { q.x = arr_f(A.uv, A.uvType, (size_t)j * 2); q.y = arr_f(A.uv, A.uvType, (size_t)j * 2 + 1); }
+            if (A.flags & PFF_ARRAYS_TEXMAT) {         // pfTexCoord2f (pfm.h:954-962)
+                const float u = q.x, v = q.y;
+                q.x = A.uvm[0] * u + A.uvm[2] * v + A.uvm[4];
+                q.y = A.uvm[1] * u + A.uvm[3] * v + A.uvm[5];
+            }
             uv[3 * t + k] = q;
         }
         if (nrm) {
-            for (int c = 0; c < 3; c++)
-                nrm[9 * t + 3 * k + c] = A.nrm ? arr_f(A.nrm, A.nrmType, (size_t)j * 3 + c) : 0.0f;
+            float n[3];
+            for (int c = 0; c < 3; c++) n[c] = A.nrm ? arr_f(A.nrm, A.nrmType, (size_t)j * 3 + c) : 0.0f;
+            if (A.flags & PFF_ARRAYS_NORMALIZE) {      // pfNormal3f under PF_NORMALIZE (pfm.h:1183)
+                const float sq = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+                if (sq != 0.0f) {
+                    const float inv = 1.0f / sqrtf(sq);
+                    for (int c = 0; c < 3; c++) n[c] = n[c] * inv;
+                }
+            }
+            for (int c = 0; c < 3; c++) nrm[9 * t + 3 * k + c] = n[c];
         }
     }
     info[t] = A.state | (grp ? A.facePass1 : A.facePass0);
@@ -3758,10 +3773,13 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
             if (why) PLAN_FAIL(PFD_E_INVALID, "frame %u: %s", f, why);
             prevEnd = a.triFirst + a.triCount;
             const size_t n = a.numElements;
-            arrBytes += align256(n * a.posSize * elem_bytes(a.posType)) + align256(a.normal ? n * 3 * elem_bytes(a.nrmType) : 0)
-                      + align256(a.texcoord ? n * 2 * elem_bytes(a.uvType) : 0)
-                      + align256(a.colors ? n * a.colSize * elem_bytes(a.colType) : 0)
-                      + align256(a.indices ? (size_t)a.count * elem_bytes(a.idxType) : 0);
+            if (!a.device)
+                arrBytes += align256(n * a.posSize * elem_bytes(a.posType)) + align256(a.normal ? n * 3 * elem_bytes(a.nrmType) : 0)
+                          + align256(a.texcoord ? n * 2 * elem_bytes(a.uvType) : 0)
+                          + align256(a.colors ? n * a.colSize * elem_bytes(a.colType) : 0)
+                          + align256(a.indices ? (size_t)a.count * elem_bytes(a.idxType) : 0);
+            else if (a.indices)
+                PLAN_FAIL(PFD_E_INVALID, "frame %u: a device-resident array draw has indices", f);
             arrTris += a.triCount;
             numDraws++;
         }
@@ -4062,6 +4080,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
             return rc;
         std::vector<DArrays> tab;
         tab.reserve(numDraws);
+        bool usedResident = false;
         uint8_t* dst = (uint8_t*)J->arrPool.p;
         uint32_t tb = 0, sb = 0, cum = 0;
         auto put = [&](const void* h, size_t bytes) -> const uint8_t* {
@@ -4078,11 +4097,27 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 const size_t n = a.numElements;
                 DArrays d;
                 memset(&d, 0, sizeof d);
-                d.pos = put(a.position, n * a.posSize * elem_bytes(a.posType));
-                d.nrm = put(a.normal, n * 3 * elem_bytes(a.nrmType));
-                d.uv = put(a.texcoord, n * 2 * elem_bytes(a.uvType));
-                d.col = put(a.colors, n * a.colSize * elem_bytes(a.colType));
-                d.idx = put(a.indices, (size_t)a.count * elem_bytes(a.idxType));
+                if (a.device) {
+                    // Resident copy (a render list's vertices): same layout as on the host.
+                    const uint8_t* dev = (const uint8_t*)a.device;
+                    const uint8_t* h0 = (const uint8_t*)a.position;
+                    auto at = [&](const void* h) -> const uint8_t* {
+                        return h ? dev + ((const uint8_t*)h - h0) : nullptr;
+                    };
+                    d.pos = dev;
+                    d.nrm = at(a.normal);
+                    d.uv = at(a.texcoord);
+                    d.col = at(a.colors);
+                    usedResident = true;
+                } else {
+                    d.pos = put(a.position, n * a.posSize * elem_bytes(a.posType));
+                    d.nrm = put(a.normal, n * 3 * elem_bytes(a.nrmType));
+                    d.uv = put(a.texcoord, n * 2 * elem_bytes(a.uvType));
+                    d.col = put(a.colors, n * a.colSize * elem_bytes(a.colType));
+                    d.idx = put(a.indices, (size_t)a.count * elem_bytes(a.idxType));
+                }
+                d.flags = a.flags;
+                memcpy(d.uvm, a.uvMatrix, sizeof d.uvm);
                 d.posSize = (uint32_t)a.posSize; d.posType = a.posType; d.nrmType = a.nrmType; d.uvType = a.uvType;
                 d.colSize = (uint32_t)a.colSize; d.colType = a.colType; d.idxType = a.idxType;
                 d.mode = a.mode;
@@ -4113,6 +4148,9 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                                vidOn ? (uint32_t*)J->vid.p : nullptr);
             HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
         }
+        // Resident copies belong to their lists, which may go once the
+        // frame has been handed over: read them before returning.
+        if (usedResident) HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
     }
     if ((rc = up(J->states, hstates.data(), hstates.size() * sizeof(PFFstate)))
         || (rc = up(J->texTable, htex.data(), htex.size() * sizeof(DTex))))
@@ -4423,6 +4461,38 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     if ((rc = pfdJobUploadTargets(J))) return rc;
     if (chosen) *chosen = J->cfg;
     return PFD_OK;
+}
+
+struct PFDresident {
+    void*  p;
+    size_t bytes;
+};
+
+int pfdResidentCreate(const void* host, size_t bytes, PFDresident** out) {
+    if (!out || (!host && bytes)) return PFD_E_INVALID;
+    *out = nullptr;
+    int rc = pfdInit();
+    if (rc) return rc;
+    PFDresident* r = new PFDresident();
+    r->bytes = bytes;
+    hipError_t e = hipMalloc(&r->p, bytes ? bytes : 16);
+    if (e != hipSuccess) { set_error("hipMalloc", e); delete r; return PFD_E_NOMEM; }
+    if (bytes && (e = hipMemcpy(r->p, host, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        set_error("resident upload", e);
+        (void)hipFree(r->p);
+        delete r;
+        return PFD_E_LAUNCH;
+    }
+    *out = r;
+    return PFD_OK;
+}
+
+const void* pfdResidentPointer(const PFDresident* r) { return r ? r->p : nullptr; }
+
+void pfdResidentDestroy(PFDresident* r) {
+    if (!r) return;
+    (void)hipFree(r->p);
+    delete r;
 }
 
 void pfdJobDestroy(PFDjob* J) {

@@ -80,11 +80,23 @@ typedef struct {                       /* PFIrendercall, context.h:270-279 */
     float*      nrm;                   /* 3 floats per vertex */
     PFcolor*    col;
     size_t      n, cap;
+    void*       packed;                /* sealed: pos, nrm, uv, col in one block */
+    PFDresident* resident;             /* sealed: its device copy, or NULL */
 } ListCall;
+
+/* The calls of a list once pfEndList has sealed them: immutable, shared by
+ * the list and the pending frames that call it (reference counted, so that
+ * pfNewList / pfDeleteList while a frame is pending leave it intact). */
+typedef struct {
+    ListCall*   calls;
+    size_t      n;
+    int         refs;
+} ListStore;
 
 typedef struct {                       /* PFIrenderlist, context.h:298 */
     ListCall*   calls;
     size_t      n, cap;
+    ListStore*  store;                 /* sealed calls (owns `calls`), or NULL */
 } List;
 
 typedef struct {                       /* PFIctxbackup, context.h:300-316 */
@@ -108,6 +120,8 @@ typedef struct {                       /* the frame being recorded */
     PFFpoint*  pts;      size_t npts, cappts;
     PFFline*   lns;      size_t nlns, caplns;
     PFFarrays* arr;      size_t narr, caparr;   /* array draws; element copies owned */
+    uint8_t*   arrOwned;  size_t capowned;      /* per array draw: 1 if its block is owned */
+    ListStore** stores;  size_t nstores, capstores;   /* lists called, one reference each */
     int        anyTex, anyLight, anyVid;
 } Rec;
 
@@ -309,11 +323,16 @@ static int grow(void** p, size_t* cap, size_t need, size_t elem)
 /* Recording                                                                 */
 /* ------------------------------------------------------------------------ */
 
+static void store_release(ListStore* st);
+
 static void rec_reset(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
-    for (size_t i = 0; i < r->narr; i++) free((void*)r->arr[i].position);   /* one block per draw */
+    for (size_t i = 0; i < r->narr; i++)
+        if (r->arrOwned[i]) free((void*)r->arr[i].position);             /* one block per draw */
     r->narr = 0;
+    for (size_t i = 0; i < r->nstores; i++) store_release(r->stores[i]);
+    r->nstores = 0;
     r->npix = 0;
     r->npts = 0;
     r->nlns = 0;
@@ -326,7 +345,11 @@ static void rec_reset(Rec* r)
 static void rec_free(Rec* r)
 {
     for (size_t i = 0; i < r->npix; i++) free((void*)r->pix[i].pixels);
-    for (size_t i = 0; i < r->narr; i++) free((void*)r->arr[i].position);
+    for (size_t i = 0; i < r->narr; i++)
+        if (r->arrOwned[i]) free((void*)r->arr[i].position);
+    for (size_t i = 0; i < r->nstores; i++) store_release(r->stores[i]);
+    free(r->stores);
+    free(r->arrOwned);
     free(r->arr);
     free(r->pix);
     free(r->pts);
@@ -1603,6 +1626,88 @@ static size_t type_bytes(PFenum t)
     }
 }
 
+/* Face groups of a draw in a triangle mode under the current culling and
+ * polygon modes (ictx.c:94-244): the PFF_PASS_* bits of each group, in the
+ * order the reference renders them.  Returns 0 when a point or line
+ * polygon mode applies (such draws are recorded vertex by vertex). */
+static int face_groups(const Ctx* c, PFdrawmode mode, uint32_t passes[2], uint32_t* groups)
+{
+    const int culled = (c->state & PF_CULL_FACE) != 0;
+    const int face = culled ? !c->cullFace : PF_FRONT_AND_BACK;
+    const int polyModes = (mode == PF_TRIANGLES || mode == PF_QUADS);   /* ictx.c:104-136 */
+    *groups = 1;
+    passes[1] = 0;
+    if (face == PF_FRONT_AND_BACK) {
+        if (polyModes && (c->polyMode[0] != PF_FILL || c->polyMode[1] != PF_FILL)) return 0;
+        if (mode == PF_TRIANGLES) passes[0] = PFF_PASS_FRONT | PFF_PASS_BACK;
+        else { *groups = 2; passes[0] = PFF_PASS_FRONT; passes[1] = PFF_PASS_BACK; }
+    } else {
+        if (polyModes && c->polyMode[face] != PF_FILL) return 0;
+        passes[0] = face_pass(face);
+    }
+    return 1;
+}
+
+/* Triangles `count` vertices give per face group in a triangle mode: one
+ * primitive per full vertex buffer (context.c:41-74), 1, 2 or 4 triangles
+ * each. */
+static uint64_t mode_triangles(PFdrawmode mode, uint64_t count)
+{
+    const uint32_t bufSize = mode_vertex_count(mode);
+    const uint32_t keep = (mode == PF_TRIANGLE_FAN || mode == PF_TRIANGLE_STRIP) ? 1u
+                        : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 2u : 0u;
+    const uint32_t perBuffer = (mode == PF_TRIANGLES) ? 1u : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 4u : 2u;
+    const uint64_t fills = count < bufSize ? 0 : 1 + (count - bufSize) / (bufSize - keep);
+    return fills * perBuffer;
+}
+
+/* Appends an array draw of `tris` triangles (after pfBegin): target, state,
+ * triangle slots and draw command.  Returns the record with its element
+ * fields still to fill, or NULL after setting the error.  `block` (the
+ * element copy, or NULL) is freed with the frame when `owned`. */
+static PFFarrays* push_array_draw(Ctx* c, PFdrawmode mode, uint32_t count, uint64_t tris, uint32_t groups,
+                                  const uint32_t passes[2], void* block, int owned)
+{
+    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); if (owned) free(block); return NULL; }
+    int si = current_state(c);
+    Rec* r = &c->rec;
+    if (si < 0 || r->ntri + tris > 0xFFFFFF00u
+        || !grow((void**)&r->arrOwned, &r->capowned, r->narr + 1, 1)
+        || !grow((void**)&r->arr, &r->caparr, r->narr + 1, sizeof(PFFarrays))
+        || (r->ntri + tris > r->captri && !rec_grow(r, r->ntri + tris))) {
+        if (owned) free(block);
+        set_err(PF_ERROR_OUT_OF_MEMORY);
+        return NULL;
+    }
+    PFFarrays* d = &r->arr[r->narr];
+    memset(d, 0, sizeof *d);
+    r->arrOwned[r->narr] = owned ? 1 : 0;
+    d->position = block;
+    d->mode = (uint32_t)mode;
+    d->count = count;
+    d->triFirst = (uint32_t)r->ntri;
+    d->triCount = (uint32_t)tris;
+    d->numFaces = groups;
+    d->facePasses[0] = passes[0];
+    d->facePasses[1] = passes[1];
+    d->state = (uint32_t)si;
+    d->color = pack(c->curColor);
+    r->narr++;
+    const size_t t = r->ntri;
+    r->ntri += (size_t)tris;
+    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_DRAW && r->cmds[r->ncmd - 1].last == t) {
+        r->cmds[r->ncmd - 1].last = (uint32_t)r->ntri;
+    } else {
+        PFFcmd cmd;
+        memset(&cmd, 0, sizeof cmd);
+        cmd.kind = PFF_CMD_DRAW;
+        cmd.first = (uint32_t)t;
+        cmd.last = (uint32_t)r->ntri;
+        push_cmd(c, &cmd);
+    }
+    return d;
+}
+
 /* A pfDrawElements / pfDrawArrays call whose primitives all become filled
  * triangles is recorded as one array draw (PFFarrays): the elements it
  * reads (0 .. the largest index, or first .. first + count - 1) are copied
@@ -1615,30 +1720,11 @@ static int record_array_draw(Ctx* c, PFdrawmode mode, PFsizei count, PFdatatype 
                              int64_t first)
 {
     if (c->curList || count <= 0 || mode < PF_TRIANGLES || mode > PF_QUAD_STRIP || first < 0) return 0;
-    const int culled = (c->state & PF_CULL_FACE) != 0;
-    const int face = culled ? !c->cullFace : PF_FRONT_AND_BACK;
-    uint32_t groups = 1, passes[2] = { 0, 0 };
-    const int polyModes = (mode == PF_TRIANGLES || mode == PF_QUADS);   /* ictx.c:104-136 */
-    if (face == PF_FRONT_AND_BACK) {
-        if (polyModes && (c->polyMode[0] != PF_FILL || c->polyMode[1] != PF_FILL)) return 0;
-        if (mode == PF_TRIANGLES) passes[0] = PFF_PASS_FRONT | PFF_PASS_BACK;
-        else { groups = 2; passes[0] = PFF_PASS_FRONT; passes[1] = PFF_PASS_BACK; }
-    } else {
-        if (polyModes && c->polyMode[face] != PF_FILL) return 0;
-        passes[0] = face_pass(face);
-    }
-    const uint32_t bufSize = mode_vertex_count(mode);
-    const uint32_t keep = (mode == PF_TRIANGLE_FAN || mode == PF_TRIANGLE_STRIP) ? 1u
-                        : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 2u : 0u;
-    const uint32_t perBuffer = (mode == PF_TRIANGLES) ? 1u : (mode == PF_QUAD_FAN || mode == PF_QUAD_STRIP) ? 4u : 2u;
-    const uint64_t fills = (uint64_t)count < bufSize ? 0 : 1 + ((uint64_t)count - bufSize) / (bufSize - keep);
-    const uint64_t tris = fills * groups * perBuffer;
+    uint32_t groups, passes[2];
+    if (!face_groups(c, mode, passes, &groups)) return 0;
+    const uint64_t tris = mode_triangles(mode, (uint64_t)count) * groups;
     pfBegin(mode);                                   /* matrices, as the loop's pfBegin */
     if (tris == 0) { pfEnd(); return 1; }
-    if (!ensure_target(c)) { set_err(PF_INVALID_OPERATION); pfEnd(); return 1; }
-    int si = current_state(c);
-    Rec* r = &c->rec;
-    if (si < 0 || r->ntri + tris > 0xFFFFFF00u) { set_err(PF_ERROR_OUT_OF_MEMORY); pfEnd(); return 1; }
     /* Elements read. */
     uint64_t lo = 0, n = 0;
     const size_t isz = indices ? type_bytes(itype) : 0;
@@ -1664,18 +1750,12 @@ static int record_array_draw(Ctx* c, PFdrawmode mode, PFsizei count, PFdatatype 
 #define ALIGN16(x) (((x) + 15) & ~(size_t)15)
     const size_t total = ALIGN16(n * pb) + ALIGN16(n * nb) + ALIGN16(n * tb) + ALIGN16(n * cb) + (size_t)count * isz + 16;
     uint8_t* blk = (uint8_t*)malloc(total);
-    if (!blk || !grow((void**)&r->arr, &r->caparr, r->narr + 1, sizeof(PFFarrays))
-        || (r->ntri + tris > r->captri && !rec_grow(r, r->ntri + tris))) {
-        free(blk);
-        set_err(PF_ERROR_OUT_OF_MEMORY);
-        pfEnd();
-        return 1;
-    }
-    PFFarrays* d = &r->arr[r->narr];
-    memset(d, 0, sizeof *d);
+    if (!blk) { set_err(PF_ERROR_OUT_OF_MEMORY); pfEnd(); return 1; }
+    PFFarrays* d = push_array_draw(c, mode, (uint32_t)count, tris, groups, passes, blk, 1);
+    if (!d) { pfEnd(); return 1; }
     uint8_t* q = blk;
     memcpy(q, (const uint8_t*)c->aPos.buffer + lo * pb, n * pb);
-    d->position = q; d->posSize = c->aPos.size; d->posType = (uint32_t)c->aPos.type;
+    d->posSize = c->aPos.size; d->posType = (uint32_t)c->aPos.type;
     q += ALIGN16(n * pb);
     if (useN) { memcpy(q, (const uint8_t*)c->aNrm.buffer + lo * nb, n * nb); d->normal = q; d->nrmType = (uint32_t)c->aNrm.type; }
     q += ALIGN16(n * nb);
@@ -1688,33 +1768,12 @@ static int record_array_draw(Ctx* c, PFdrawmode mode, PFsizei count, PFdatatype 
     q += ALIGN16(n * cb);
     if (indices) { memcpy(q, indices, (size_t)count * isz); d->indices = q; d->idxType = (uint32_t)itype; }
 #undef ALIGN16
-    d->mode = (uint32_t)mode;
-    d->count = (uint32_t)count;
-    d->triFirst = (uint32_t)r->ntri;
-    d->triCount = (uint32_t)tris;
-    d->numFaces = groups;
-    d->facePasses[0] = passes[0];
-    d->facePasses[1] = passes[1];
-    d->state = (uint32_t)si;
-    d->color = pack(c->curColor);
     d->elementBase = (uint32_t)lo;
     d->numElements = (uint32_t)n;
-    r->narr++;
+    Rec* r = &c->rec;
     if (useC && !r->anyVid) {                        /* corners before this draw have no index */
-        memset(r->vid, 0xFF, r->ntri * 3 * sizeof(uint32_t));
+        memset(r->vid, 0xFF, (size_t)d->triFirst * 3 * sizeof(uint32_t));
         r->anyVid = 1;
-    }
-    const size_t t = r->ntri;
-    r->ntri += (size_t)tris;
-    if (r->ncmd && r->cmds[r->ncmd - 1].kind == PFF_CMD_DRAW && r->cmds[r->ncmd - 1].last == t) {
-        r->cmds[r->ncmd - 1].last = (uint32_t)r->ntri;
-    } else {
-        PFFcmd cmd;
-        memset(&cmd, 0, sizeof cmd);
-        cmd.kind = PFF_CMD_DRAW;
-        cmd.first = (uint32_t)t;
-        cmd.last = (uint32_t)r->ntri;
-        push_cmd(c, &cmd);
     }
     pfEnd();
     return 1;
@@ -2224,8 +2283,120 @@ static void restore_backup(Ctx* c)
 
 static void free_call(ListCall* k)
 {
-    free(k->pos); free(k->uv); free(k->nrm); free(k->col);
+    if (k->packed) free(k->packed);
+    else { free(k->pos); free(k->uv); free(k->nrm); free(k->col); }
+    if (k->resident) pfdResidentDestroy(k->resident);
     memset(k, 0, sizeof *k);
+}
+
+static void store_release(ListStore* st)
+{
+    if (!st || __atomic_sub_fetch(&st->refs, 1, __ATOMIC_ACQ_REL) != 0) return;
+    for (size_t i = 0; i < st->n; i++) free_call(&st->calls[i]);
+    free(st->calls);
+    free(st);
+}
+
+/* pfEndList: the recorded calls become an immutable store.  Each call's
+ * vertices are packed into one block (positions, normals, texture
+ * coordinates, colours) and copied to the GPU once, so that every later
+ * pfCallList references them instead of replaying them (a call keeps no
+ * device copy when no GPU is usable). */
+static void seal_list(List* L)
+{
+    ListStore* st = (ListStore*)calloc(1, sizeof *st);
+    if (!st) return;                                     /* stays replayable on the host */
+    for (size_t i = 0; i < L->n; i++) {
+        ListCall* k = &L->calls[i];
+        const size_t n = k->n;
+#define ALIGN16(x) (((x) + 15) & ~(size_t)15)
+        const size_t onrm = ALIGN16(n * 16), ouv = onrm + ALIGN16(n * 12), ocol = ouv + ALIGN16(n * 8);
+        const size_t bytes = ocol + n * 4;
+#undef ALIGN16
+        uint8_t* blk = (uint8_t*)malloc(bytes + 16);
+        if (!blk) continue;                              /* this call stays in its own arrays */
+        if (n) {
+            memcpy(blk, k->pos, n * 16);
+            memcpy(blk + onrm, k->nrm, n * 12);
+            memcpy(blk + ouv, k->uv, n * 8);
+            memcpy(blk + ocol, k->col, n * 4);
+        }
+        free(k->pos); free(k->uv); free(k->nrm); free(k->col);
+        k->packed = blk;
+        k->pos = (float*)blk;
+        k->nrm = (float*)(blk + onrm);
+        k->uv = (float*)(blk + ouv);
+        k->col = (PFcolor*)(blk + ocol);
+        k->cap = n;
+        if (n && g_arrayDraws && pfdResidentCreate(blk, bytes, &k->resident) != 0) k->resident = NULL;
+    }
+    st->calls = L->calls;
+    st->n = L->n;
+    st->refs = 1;
+    L->store = st;
+}
+
+/* Forgets a list's calls: its store is released (pending frames keep their
+ * references), unsealed calls are freed. */
+static void list_clear(List* L)
+{
+    if (L->store) store_release(L->store);
+    else {
+        for (size_t i = 0; i < L->n; i++) free_call(&L->calls[i]);
+        free(L->calls);
+    }
+    L->store = NULL;
+    L->calls = NULL;
+    L->n = L->cap = 0;
+}
+
+/* One sealed call of a render list recorded as an array draw of its
+ * vertices, when every primitive becomes a filled triangle and the vertex
+ * colours do not drive the material (PF_COLOR_MATERIAL would change the
+ * state vertex by vertex): the frame references the list's block and its
+ * device copy, and takes a reference on the store.  Returns 0 when the call
+ * must be replayed through immediate mode. */
+static int call_as_array_draw(Ctx* c, ListStore* st, const ListCall* k)
+{
+    if (!g_arrayDraws || !k->packed || (c->state & PF_COLOR_MATERIAL) || k->drawMode < PF_TRIANGLES
+        || k->drawMode > PF_QUAD_STRIP || k->n > 0xFFFFFF00u)
+        return 0;
+    uint32_t groups, passes[2];
+    if (!face_groups(c, k->drawMode, passes, &groups)) return 0;
+    memcpy(c->mat, k->faceMaterial, sizeof c->mat);         /* renderlist.c:82-83 */
+    c->stateDirty = 1;
+    pfBindTexture((PFtexture)k->texture);
+    pfBegin(k->drawMode);
+    const uint64_t tris = mode_triangles(k->drawMode, k->n) * groups;
+    if (tris) {
+        Rec* r = &c->rec;
+        int held = 0;
+        for (size_t i = 0; i < r->nstores && !held; i++) held = r->stores[i] == st;
+        if (!held) {
+            if (!grow((void**)&r->stores, &r->capstores, r->nstores + 1, sizeof(ListStore*))) {
+                set_err(PF_ERROR_OUT_OF_MEMORY);
+                pfEnd();
+                return 1;
+            }
+            __atomic_add_fetch(&st->refs, 1, __ATOMIC_ACQ_REL);
+            r->stores[r->nstores++] = st;
+        }
+        PFFarrays* d = push_array_draw(c, k->drawMode, (uint32_t)k->n, tris, groups, passes, k->packed, 0);
+        if (d) {
+            d->posSize = 4; d->posType = PFF_FLOAT;
+            d->normal = k->nrm; d->nrmType = PFF_FLOAT;
+            d->texcoord = k->uv; d->uvType = PFF_FLOAT;
+            d->colors = k->col; d->colSize = 4; d->colType = PFF_UBYTE;
+            d->numElements = (uint32_t)(k->n ? k->n : 1);
+            d->flags = PFF_ARRAYS_LIST | PFF_ARRAYS_TEXMAT | ((c->state & PF_NORMALIZE) ? PFF_ARRAYS_NORMALIZE : 0u);
+            const float* m = c->matTex;
+            d->uvMatrix[0] = m[0]; d->uvMatrix[1] = m[1]; d->uvMatrix[2] = m[4];
+            d->uvMatrix[3] = m[5]; d->uvMatrix[4] = m[12]; d->uvMatrix[5] = m[13];
+            d->device = k->resident ? pfdResidentPointer(k->resident) : NULL;
+        }
+    }
+    pfEnd();
+    return 1;
 }
 
 static void list_begin(Ctx* c, PFdrawmode mode)
@@ -2282,8 +2453,7 @@ void pfDeleteList(PFrenderlist* renderList)                  /* renderlist.c:17-
     List* L = (List*)*renderList;
     if (L) {
         if (G && G->curList == L) G->curList = NULL;
-        for (size_t i = 0; i < L->n; i++) free_call(&L->calls[i]);
-        free(L->calls);
+        list_clear(L);
         free(L);
     }
     *renderList = NULL;
@@ -2295,8 +2465,7 @@ void pfNewList(PFrenderlist renderList)                      /* renderlist.c:38-
     if (!c) return;
     if (!renderList) { c->err = PF_INVALID_VALUE; return; }
     List* L = (List*)renderList;
-    for (size_t i = 0; i < L->n; i++) free_call(&L->calls[i]);
-    L->n = 0;
+    list_clear(L);
     c->curList = L;
     make_backup(c);
 }
@@ -2306,6 +2475,7 @@ void pfEndList(void)                                         /* renderlist.c:59-
     Ctx* c = G;
     if (!c) return;
     if (!c->curList) c->err = PF_INVALID_OPERATION;
+    else seal_list(c->curList);
     c->curList = NULL;
     restore_backup(c);
 }
@@ -2316,13 +2486,18 @@ void pfCallList(const PFrenderlist renderList)              /* renderlist.c:69-9
     if (!c) return;
     /* The reference dereferences a NULL list. */
     if (!renderList) { c->err = PF_INVALID_VALUE; return; }
-    const List* L = (const List*)renderList;
+    List* L = (List*)renderList;
     make_backup(c);
+    /* A sealed list called outside recording: each call whose triangles
+     * the GPU can form from the list's resident vertices is recorded as an
+     * array draw; the others replay through immediate mode below. */
+    ListStore* st = (!c->curList && L->store) ? L->store : NULL;
     /* Calls made while recording into this same list append to it: walk the
      * count taken at entry. */
     const size_t ncalls = L->n;
     for (size_t i = 0; i < ncalls; i++) {
         const ListCall* k = &L->calls[i];
+        if (st && call_as_array_draw(c, st, k)) continue;
         const size_t nv = k->n;
         memcpy(c->mat, k->faceMaterial, sizeof c->mat);
         c->stateDirty = 1;

@@ -103,6 +103,11 @@ typedef struct {
                               * of the triangles (random kinds)                  */
     float    uvMin, uvMax;   /* random kinds: texture coordinates uniform in
                               * [uvMin, uvMax) when uvMax > uvMin, else [-2, 2) */
+    int32_t  listOptions;    /* lists: bit 0 a texture matrix (rotate, scale,
+                              * translate) and bit 1 PF_NORMALIZE when they are
+                              * called; bit 2 list A re-recorded empty after its
+                              * last call (the lists are deleted before pfFinish
+                              * in any case)                                    */
     int32_t  arrayMode;      /* mesh arrays: 0 PF_TRIANGLES; a PFdrawmode from
                               * PF_TRIANGLE_FAN to PF_QUAD_STRIP draws each row of
                               * cells as one call in that mode (strip order:
@@ -120,6 +125,7 @@ typedef struct {
     void (*MatrixMode)(PFmatrixmode);
     void (*LoadIdentity)(void);
     void (*Translatef)(PFfloat, PFfloat, PFfloat);
+    void (*Scalef)(PFfloat, PFfloat, PFfloat);
     void (*Rotatef)(PFfloat, PFfloat, PFfloat, PFfloat);
     void (*Frustum)(PFfloat, PFfloat, PFfloat, PFfloat, PFfloat, PFfloat);
     void (*CullFace)(PFface);
@@ -189,7 +195,7 @@ PFSapi* pfs_bind(const char* path)
     if (!a) return NULL;
     a->h = h;
     BIND(CreateContext); BIND(DeleteContext); BIND(MakeCurrent); BIND(Enable); BIND(Disable);
-    BIND(MatrixMode); BIND(LoadIdentity); BIND(Translatef); BIND(Rotatef); BIND(Frustum);
+    BIND(MatrixMode); BIND(LoadIdentity); BIND(Translatef); BIND(Scalef); BIND(Rotatef); BIND(Frustum);
     BIND(CullFace); BIND(BlendFunc); BIND(DepthFunc); BIND(ShadeModel); BIND(LightModel);
     BIND(BindTexture); BIND(BindFramebuffer); BIND(Clear); BIND(ClearColor); BIND(EnableLight);
     BIND(DisableLight);
@@ -816,6 +822,14 @@ static void scene_list(const PFSapi* a, Rng* r, const PFSparams* p, PFtexture te
     a->Color4ub(40, 220, 90, 255);
     a->CallList(A);
     a->EndList();
+    if (p->listOptions & 1) {
+        a->MatrixMode(PF_TEXTURE);
+        a->Rotatef(30.0f, 0.0f, 0.0f, 1.0f);
+        a->Scalef(1.7f, 0.6f, 1.0f);
+        a->Translatef(0.25f, -0.4f, 0.0f);
+        a->MatrixMode(PF_MODELVIEW);
+    }
+    if (p->listOptions & 2) a->Enable(PF_NORMALIZE);
     for (int s = 0; s < NS; s++) {
         a->LoadIdentity();
         a->Translatef(rrange(r, -2.5f, 2.5f), rrange(r, -1.6f, 1.6f), rrange(r, -9.0f, -5.0f));
@@ -828,6 +842,10 @@ static void scene_list(const PFSapi* a, Rng* r, const PFSparams* p, PFtexture te
     a->Vertex3f(0.5f, -1.4f, -6.5f);
     a->Vertex3f(-0.4f, 0.2f, -5.5f);
     a->End();
+    if (p->listOptions & 4) {                       /* re-recorded: the old calls must survive */
+        a->NewList(A);
+        a->EndList();
+    }
     a->DeleteList(&A);
     a->DeleteList(&B);
     free(col);

@@ -253,6 +253,10 @@ LIST_SCENES = [
     ("list_gouraud", dict(_LIST, seed=101, lighting=1, numLights=2, cull=pf.PF_BACK)),
     ("list_phong_color_material", dict(_LIST, seed=102, lighting=2, numLights=2, colorMaterial=1, cull=-1)),
     ("list_flat_blend", dict(_LIST, seed=103, shade=pf.PF_FLAT, blend=1, blendMode=1, cull=pf.PF_FRONT)),
+    # called under a texture matrix and PF_NORMALIZE (both applied when the
+    # calls replay), culling off, list A re-recorded before pfFinish
+    ("list_texmat_normalize", dict(_LIST, seed=104, lighting=2, numLights=2, cull=-1, listOptions=7)),
+    ("list_texmat_gouraud", dict(_LIST, seed=105, lighting=1, numLights=3, cull=pf.PF_BACK, listOptions=3)),
 ]
 
 # Targets smaller than a block or a tile, with sides that are no multiple of

@@ -203,3 +203,40 @@ def test_list_undefined_reference_cases_are_errors():
     finally:
         L.pfDeleteContext(ctx)
         L.pfMakeCurrent(None)
+
+
+@pytest.mark.parametrize("name", [n for n, _ in __import__("scenes").LIST_SCENES])
+def test_list_calls_recorded_as_array_draws(name):
+    """A sealed list called with filled triangle modes and no colour
+    material is recorded as array draws of its vertices (the GPU forms the
+    triangles); replayed through immediate mode instead, the image is the
+    same, and both equal the reference's golden."""
+    import os
+    import scenes
+    from conftest import GOLDEN, render_oracle
+    kw = dict(scenes.LIST_SCENES)[name]
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    images = []
+    for on in (1, 0):
+        pf.lib().pfxSetArrayDraws(on)
+        try:
+            c, d, e, _ = render_oracle(scenes.params(kw))
+        finally:
+            pf.lib().pfxSetArrayDraws(1)
+        assert e == 0
+        images.append((c, d))
+    for c, d in images:
+        assert np.array_equal(c, g["color"]) and np.array_equal(d.view(np.uint32), g["depth"].view(np.uint32))
+    # The calls that can be array draws are: the recorded frame says so.
+    from test_host_api import Frame, EXEC
+    counts = []
+    cb = EXEC(lambda fp, user: (counts.append(C.cast(fp, C.POINTER(Frame)).contents.numArrays), 0)[1])
+    pf.lib().pfxSetExecutor(C.cast(cb, C.c_void_p), None)
+    try:
+        pf.render(scenes.params(kw))
+    finally:
+        pf.lib().pfxSetExecutor(None, None)
+    if kw.get("colorMaterial"):
+        assert counts == [0]                      # colour material: immediate-mode replay
+    else:
+        assert counts and counts[0] > 0
