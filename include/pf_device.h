@@ -127,9 +127,10 @@ int  pfdJobRunTimed(PFDjob* job, PFDtimes* t);   /* one run with per-phase event
 int  pfdJobTimeReplays(PFDjob* job, uint32_t iters, float* msTotal); /* graph replays */
 int  pfdJobStats(PFDjob* job, PFDstats* st);
 /* Host render target of frame `frame` (the buffers pfdJobUploadTargets reads
- * and pfdJobDownload writes): a caller replaying a job restores them between
- * runs.  Any output pointer may be NULL. */
-int  pfdJobTarget(PFDjob* job, uint32_t frame, uint32_t** color, float** depth, uint32_t* width, uint32_t* height);
+ * and pfdJobDownload writes; colour pixels in the frame's PFF_TARGET_*
+ * format): a caller replaying a job restores them between runs.  Any output
+ * pointer may be NULL. */
+int  pfdJobTarget(PFDjob* job, uint32_t frame, void** color, float** depth, uint32_t* width, uint32_t* height);
 int  pfdJobFrameCount(PFDjob* job);               /* frames in the job, or < 0 */
 int  pfdJobAutotune(PFDjob* job, PFDconfig* chosen);
 void pfdJobDestroy(PFDjob* job);

@@ -249,10 +249,52 @@ typedef struct {
 #define PFF_ARRAYS_TEXMAT     0x2u  /* texture coordinates through uvMatrix */
 #define PFF_ARRAYS_NORMALIZE  0x4u  /* normals normalised */
 
+/* Pixel formats of a render target's colour buffer (PFFframe.target): the
+ * (format, type) pairs whose reference accessors read and write only their
+ * own pixel -- the SIMD ones of the triangle raster and pfClear
+ * (pixel.h:1492-2067, 2536-2913) and the scalar ones of points, lines,
+ * pfDrawPixels, rects and fog (pixel.h:206-400, 549-707).  BGRA /
+ * UNSIGNED_BYTE (its SIMD accessors copy one pixel's bytes into its
+ * neighbours, pixel.h:2070-2078), the single-channel, luminance and
+ * half/float pairs are not render targets here. */
+#define PFF_TARGET_RGBA8     0u     /* RGBA / UNSIGNED_BYTE           */
+#define PFF_TARGET_RGB8      1u     /* RGB / UNSIGNED_BYTE (3 bytes)  */
+#define PFF_TARGET_BGR8      2u     /* BGR / UNSIGNED_BYTE (3 bytes)  */
+#define PFF_TARGET_RGB565    3u     /* RGB / UNSIGNED_SHORT_5_6_5     */
+#define PFF_TARGET_BGR565    4u     /* BGR / UNSIGNED_SHORT_5_6_5     */
+#define PFF_TARGET_RGBA5551  5u     /* RGBA / UNSIGNED_SHORT_5_5_5_1  */
+#define PFF_TARGET_BGRA5551  6u     /* BGRA / UNSIGNED_SHORT_5_5_5_1  */
+#define PFF_TARGET_RGBA4444  7u     /* RGBA / UNSIGNED_SHORT_4_4_4_4  */
+#define PFF_TARGET_BGRA4444  8u     /* BGRA / UNSIGNED_SHORT_4_4_4_4  */
+#define PFF_TARGET_NONE      0xFFFFFFFFu
+
+/* The target of a PFF_SOURCE pair, or PFF_TARGET_NONE. */
+static inline uint32_t pff_target_of(uint32_t source)
+{
+    switch (source) {
+    case PFF_SOURCE(7, 0): return PFF_TARGET_RGBA8;
+    case PFF_SOURCE(6, 0): return PFF_TARGET_RGB8;
+    case PFF_SOURCE(8, 0): return PFF_TARGET_BGR8;
+    case PFF_SOURCE(6, 2): return PFF_TARGET_RGB565;
+    case PFF_SOURCE(8, 2): return PFF_TARGET_BGR565;
+    case PFF_SOURCE(7, 3): return PFF_TARGET_RGBA5551;
+    case PFF_SOURCE(9, 3): return PFF_TARGET_BGRA5551;
+    case PFF_SOURCE(7, 4): return PFF_TARGET_RGBA4444;
+    case PFF_SOURCE(9, 4): return PFF_TARGET_BGRA4444;
+    default: return PFF_TARGET_NONE;
+    }
+}
+
+/* Bytes per pixel of a target (0 for an unknown one). */
+static inline uint32_t pff_target_bytes(uint32_t target)
+{
+    return target == PFF_TARGET_RGBA8 ? 4u : target <= PFF_TARGET_BGR8 ? 3u : target <= PFF_TARGET_BGRA4444 ? 2u : 0u;
+}
+
 typedef struct {
-    uint32_t           width, height;    /* render target, RGBA8 + float32 depth */
-    uint32_t*          color;            /* host buffers of the render target    */
-    float*             depth;
+    uint32_t           width, height;    /* render target + float32 depth        */
+    void*              color;            /* host buffers of the render target:   */
+    float*             depth;            /* width*height pixels of `target`      */
     PFFtriangles       tris;
     const PFFstate*    states;   uint32_t numStates;
     const PFFtexture*  textures; uint32_t numTextures;
@@ -261,6 +303,7 @@ typedef struct {
     const PFFpoint*    points;   uint32_t numPoints;
     const PFFline*     lines;    uint32_t numLines;
     const PFFarrays*   arrays;   uint32_t numArrays;   /* in triFirst order */
+    uint32_t           target;   /* PFF_TARGET_*: pixel format of `color`  */
 } PFFframe;
 
 #endif /* PF_FRAME_H */

@@ -303,6 +303,10 @@ enum { SRC_RED, SRC_GREEN, SRC_BLUE, SRC_ALPHA, SRC_LUMINANCE, SRC_LUMINANCE_ALP
        SRC_RGB, SRC_RGBA, SRC_BGR, SRC_BGRA };
 enum { SRC_UBYTE = 0, SRC_565 = 2, SRC_5551 = 3, SRC_4444 = 4, SRC_HALF = 9, SRC_FLOAT = 10 };
 
+/* Render-target pixel accessors, defined with the pixel operations below. */
+static uint32_t tget_simd(const PFFframe* f, size_t i);
+static void tset_simd(const PFFframe* f, size_t i, uint32_t c);
+
 /* pfmHalfToFloatI (pfm.h:128-145). */
 static float half_bits_to_float(uint32_t h)
 {
@@ -936,8 +940,8 @@ static void raster_tri(RCtx* rc, const PFFstate* st, int face, int is3D,
                 }
                 frag = light_fragment(st, mat, frag, P, N);
             }
-            if (blend) frag = pforacle_blend(st->blendMode, frag, f->color[off]);
-            f->color[off] = frag;
+            if (blend) frag = pforacle_blend(st->blendMode, frag, tget_simd(f, off));   /* tri.c:523-529 */
+            tset_simd(f, off, frag);
             f->depth[off] = zq;
             if (rc->win) {
                 if (rc->cur < 0) {
@@ -1002,14 +1006,17 @@ static void process_pass(RCtx* rc, const PFFstate* st, int face, const Vtx* in3)
         raster_tri(rc, st, face, is3D, &poly[0], &poly[i + 1], &poly[i + 2]);
 }
 
-/* pfClear, SIMD build (context.c:680-716): pixels 0..7 keep their value and
- * the tail past the last multiple of 8 copies pixel 0. */
+/* pfClear, SIMD build (context.c:680-716): pixels 0..7 keep their value,
+ * pixels 8 up to the last multiple of 8 go through the target's SIMD
+ * setter and the tail copies pixel 0's bytes. */
 static void clear(const PFFframe* f, const PFFcmd* c)
 {
     size_t size = (size_t)f->width * f->height;
     size_t aligned = size - (size % 8);
-    for (size_t i = 8; i < aligned; i++) { f->color[i] = c->clearColor; f->depth[i] = c->clearDepth; }
-    for (size_t i = aligned; i < size; i++) { f->color[i] = f->color[0]; f->depth[i] = f->depth[0]; }
+    const size_t bpp = pff_target_bytes(f->target);
+    uint8_t* px = (uint8_t*)f->color;
+    for (size_t i = 8; i < aligned; i++) { tset_simd(f, i, c->clearColor); f->depth[i] = c->clearDepth; }
+    for (size_t i = aligned; i < size; i++) { memcpy(px + i * bpp, px, bpp); f->depth[i] = f->depth[0]; }
 }
 
 /* Renders a recorded frame into frame->color / frame->depth.  Returns 0 on
@@ -1102,6 +1109,124 @@ static uint32_t scalar_texel(const uint8_t* p, uint32_t source, size_t i)
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Render targets (PFFframe.target)                                          */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t target_source(uint32_t target)
+{
+    static const uint32_t src[9] = {
+        PFF_SOURCE(SRC_RGBA, SRC_UBYTE), PFF_SOURCE(SRC_RGB, SRC_UBYTE), PFF_SOURCE(SRC_BGR, SRC_UBYTE),
+        PFF_SOURCE(SRC_RGB, SRC_565), PFF_SOURCE(SRC_BGR, SRC_565), PFF_SOURCE(SRC_RGBA, SRC_5551),
+        PFF_SOURCE(SRC_BGRA, SRC_5551), PFF_SOURCE(SRC_RGBA, SRC_4444), PFF_SOURCE(SRC_BGRA, SRC_4444) };
+    return src[target];
+}
+
+/* The triangle raster's read of a pixel: the target's AVX2 getter
+ * (pixel.h:2536-2913), one lane of it. */
+static uint32_t tget_simd(const PFFframe* f, size_t i)
+{
+    if (f->target == PFF_TARGET_RGBA8) return ((const uint32_t*)f->color)[i];       /* pixel.h:2910-2913 */
+    PFFtexture t;
+    memset(&t, 0, sizeof t);
+    t.pixels = f->color;
+    t.width = f->width;
+    t.height = f->height;
+    t.format = PFF_TEX_SOURCE;
+    t.source = target_source(f->target);
+    return source_texel(&t, i);
+}
+
+/* The raster's and pfClear's write: one lane of the target's AVX2 setter
+ * with its mask set (pixel.h:1492-2067).  Channels scale as
+ * (c * (2^n - 1)) >> 8; RGB bytes are packed high to low and written
+ * byte by byte. */
+static void tset_simd(const PFFframe* f, size_t i, uint32_t c)
+{
+    uint8_t* p = (uint8_t*)f->color;
+    const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
+    uint32_t v;
+    uint16_t h;
+    switch (f->target) {
+    case PFF_TARGET_RGB8:                                                          /* pixel.h:1590-1638 */
+    case PFF_TARGET_BGR8:                                                          /* pixel.h:1641-1689 */
+        v = f->target == PFF_TARGET_RGB8 ? ((r << 16) | (g << 8) | b) : ((b << 16) | (g << 8) | r);
+        p[3 * i] = (uint8_t)((v >> 16) & 0xFFu);
+        p[3 * i + 1] = (uint8_t)((v >> 8) & 0xFFu);
+        p[3 * i + 2] = (uint8_t)(v & 0xFFu);
+        return;
+    case PFF_TARGET_RGB565:                                                        /* pixel.h:1492-1538 */
+        h = (uint16_t)((((r * 31u) >> 8) << 11) | (((g * 63u) >> 8) << 5) | ((b * 31u) >> 8));
+        break;
+    case PFF_TARGET_BGR565:                                                        /* pixel.h:1541-1587 */
+        h = (uint16_t)((((b * 31u) >> 8) << 11) | (((g * 63u) >> 8) << 5) | ((r * 31u) >> 8));
+        break;
+    case PFF_TARGET_RGBA5551:                                                      /* pixel.h:1851-1900 */
+        h = (uint16_t)((((r * 31u) >> 8) << 11) | (((g * 31u) >> 8) << 6) | (((b * 31u) >> 8) << 1)
+                       | ((int32_t)a > 127 ? 1u : 0u));
+        break;
+    case PFF_TARGET_BGRA5551:                                                      /* pixel.h:1903-1952 */
+        h = (uint16_t)((((b * 31u) >> 8) << 11) | (((g * 31u) >> 8) << 6) | (((r * 31u) >> 8) << 1)
+                       | ((int32_t)a > 127 ? 1u : 0u));
+        break;
+    case PFF_TARGET_RGBA4444:                                                      /* pixel.h:1955-2005 */
+        h = (uint16_t)((((r * 15u) >> 8) << 12) | (((g * 15u) >> 8) << 8) | (((b * 15u) >> 8) << 4) | ((a * 15u) >> 8));
+        break;
+    case PFF_TARGET_BGRA4444:                                                      /* pixel.h:2008-2058 */
+        h = (uint16_t)((((b * 15u) >> 8) << 12) | (((g * 15u) >> 8) << 8) | (((r * 15u) >> 8) << 4) | ((a * 15u) >> 8));
+        break;
+    default:                                                                       /* pixel.h:2061-2067 */
+        memcpy(p + 4 * i, &c, 4);
+        return;
+    }
+    memcpy(p + 2 * i, &h, 2);
+}
+
+/* Points, lines, pfDrawPixels, rects and fog read through the scalar
+ * getter (pixel.h:549-707) ... */
+static uint32_t tget(const PFFframe* f, size_t i)
+{
+    return scalar_texel((const uint8_t*)f->color, target_source(f->target), i);
+}
+
+/* ... and write through the scalar setter (pixel.h:206-400): channels
+ * times (float)PF_INV_255 (pixelforge.h:93, 1.0 / 255), times the field's
+ * maximum, roundf; the 5_5_5_1 alpha bit is a > 50 * PF_INV_255 compared
+ * in double (config.h:45-47).  The BGRA normaliser does not swap red and
+ * blue (pixel.h:59-67), so BGRA 5_5_5_1 / 4_4_4_4 store RGBA order. */
+static void tset(const PFFframe* f, size_t i, uint32_t c)
+{
+    uint8_t* p = (uint8_t*)f->color;
+    const uint8_t r = (uint8_t)c, g = (uint8_t)(c >> 8), b = (uint8_t)(c >> 16), a = (uint8_t)(c >> 24);
+    const float inv = (float)(1.0 / 255);
+    const float nr = r * inv, ng = g * inv, nb = b * inv, na = a * inv;
+    uint16_t h;
+    switch (f->target) {
+    case PFF_TARGET_RGB8: p[3 * i] = r; p[3 * i + 1] = g; p[3 * i + 2] = b; return;          /* pixel.h:243-249 */
+    case PFF_TARGET_BGR8: p[3 * i] = b; p[3 * i + 1] = g; p[3 * i + 2] = r; return;          /* pixel.h:251-257 */
+    case PFF_TARGET_RGB565:                                                                   /* pixel.h:206-217 */
+        h = (uint16_t)(((uint16_t)(uint8_t)roundf(nr * 31.0f) << 11) | ((uint16_t)(uint8_t)roundf(ng * 63.0f) << 5)
+                       | (uint16_t)(uint8_t)roundf(nb * 31.0f));
+        break;
+    case PFF_TARGET_BGR565:                                                                   /* pixel.h:219-230 */
+        h = (uint16_t)(((uint16_t)(uint8_t)roundf(nb * 31.0f) << 11) | ((uint16_t)(uint8_t)roundf(ng * 63.0f) << 5)
+                       | (uint16_t)(uint8_t)roundf(nr * 31.0f));
+        break;
+    case PFF_TARGET_RGBA5551: case PFF_TARGET_BGRA5551:                                       /* pixel.h:293-318 */
+        h = (uint16_t)(((uint16_t)(uint8_t)roundf(nr * 31.0f) << 11) | ((uint16_t)(uint8_t)roundf(ng * 31.0f) << 6)
+                       | ((uint16_t)(uint8_t)roundf(nb * 31.0f) << 1) | (na > (float)50 * (1.0 / 255) ? 1u : 0u));
+        break;
+    case PFF_TARGET_RGBA4444: case PFF_TARGET_BGRA4444:                                       /* pixel.h:320-345 */
+        h = (uint16_t)(((uint16_t)(uint8_t)roundf(nr * 15.0f) << 12) | ((uint16_t)(uint8_t)roundf(ng * 15.0f) << 8)
+                       | ((uint16_t)(uint8_t)roundf(nb * 15.0f) << 4) | (uint16_t)(uint8_t)roundf(na * 15.0f));
+        break;
+    default:
+        memcpy(p + 4 * i, &c, 4);
+        return;
+    }
+    memcpy(p + 2 * i, &h, 2);
+}
+
 /* The scalar depth tests (depth.h:29-63): NOTEQUAL is a real !=. */
 static int depth_scalar(uint32_t func, float src, float dst)
 {
@@ -1158,7 +1283,7 @@ static void draw_pixels(const PFFframe* f, const PFFpixels* o)
             if (src >= n) src = n - 1u;                                                          /* D8 */
             if (!(o->flags & PFF_PIXELS_RECT)) f->depth[dst] = o->zPos;    /* pfRect writes no depth */
             const uint32_t c = scalar_texel(img, o->source, src);
-            f->color[dst] = (o->flags & PFF_BLEND) ? blend_scalar(o->blendMode, c, f->color[dst]) : c;
+            tset(f, dst, (o->flags & PFF_BLEND) ? blend_scalar(o->blendMode, c, tget(f, dst)) : c);
         }
     }
 }
@@ -1176,7 +1301,7 @@ static void fog_process(const PFFframe* f, const PFFpixels* o)
     for (size_t i = 0; i < n; i++) {
         const float depth = f->depth[i];
         if (depth >= end) {
-            f->color[i] = alpha == 255u ? fc : blend_scalar(1, fc, f->color[i]);
+            tset(f, i, alpha == 255u ? fc : blend_scalar(1, fc, tget(f, i)));
         } else if (depth > start) {
             float t = 0.0f;
             switch (o->fogMode) {
@@ -1186,7 +1311,7 @@ static void fog_process(const PFFframe* f, const PFFpixels* o)
             default: break;
             }
             const uint32_t a = f2u8(t * (float)alpha);
-            f->color[i] = blend_scalar(1, (fc & 0x00FFFFFFu) | (a << 24), f->color[i]);
+            tset(f, i, blend_scalar(1, (fc & 0x00FFFFFFu) | (a << 24), tget(f, i)));
         }
     }
 }
@@ -1200,7 +1325,7 @@ static void fog_process(const PFFframe* f, const PFFpixels* o)
 static void plot_point(const PFFframe* f, const PFFstate* st, size_t off, float z, uint32_t color)
 {
     if ((st->flags & PFF_DEPTH_TEST) && !depth_scalar(st->depthFunc, z, f->depth[off])) return;
-    f->color[off] = (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, color, f->color[off]) : color;
+    tset(f, off, (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, color, tget(f, off)) : color);
     f->depth[off] = z;
 }
 
@@ -1378,7 +1503,7 @@ static void thin_line(const PFFframe* f, const PFFstate* st, const float* s1, co
             const int32_t a = (int32_t)((c1 >> (8 * k)) & 255u), b = (int32_t)((c2 >> (8 * k)) & 255u);
             col |= (uint32_t)f2u8((float)a + t * (float)(b - a)) << (8 * k);
         }
-        f->color[off] = (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, col, f->color[off]) : col;
+        tset(f, off, (st->flags & PFF_BLEND) ? blend_scalar(st->blendMode, col, tget(f, off)) : col);
         f->depth[off] = z;
     }
 }
@@ -1594,6 +1719,7 @@ int pforacle_render(const PFFframe* f, uint64_t* fragments)
 {
     PFFframe g;
     Expanded x;
+    if (pff_target_bytes(f->target) == 0u) return -1;
     int r = expand_frame(f, &g, &x);
     if (r) return r;
     RCtx rc;

@@ -54,7 +54,7 @@ class SceneParams(C.Structure):
         ("points", C.c_int32), ("pointSize", C.c_float), ("polyPoint", C.c_int32),
         ("lines", C.c_int32), ("lineWidth", C.c_float), ("polyLine", C.c_int32),
         ("rects", C.c_int32), ("uvMin", C.c_float), ("uvMax", C.c_float), ("listOptions", C.c_int32),
-        ("arrayMode", C.c_int32),
+        ("arrayMode", C.c_int32), ("fbFormat", C.c_int32), ("fbType", C.c_int32), ("fbMain", C.c_int32),
     ]
 
 
@@ -182,13 +182,21 @@ def api(path=LIB_PATH):
     return _apis[path]
 
 
+# Bytes per pixel of the render-target (format, type) pairs.
+TARGET_BYTES = {(7, 0): 4, (6, 0): 3, (8, 0): 3, (6, 2): 2, (8, 2): 2, (7, 3): 2, (9, 3): 2, (7, 4): 2, (9, 4): 2}
+
+
 def render(params, path=LIB_PATH):
     """Renders one scene through the library at `path`; returns
-    (color uint32[H, W], depth float32[H, W], error code)."""
+    (color, depth float32[H, W], error code): color is uint32[H, W] for an
+    RGBA8 target, else the target's bytes as uint8[H, W, bytes per pixel]."""
     h, w = params.height, params.width
     color = np.zeros((h * w + 16,), np.uint32)
     depth = np.zeros((h * w + 16,), np.float32)
     err = scenes().pfs_render(api(path), C.byref(params), color.ctypes.data, depth.ctypes.data)
+    if params.fbFormat > 0:
+        bpp = TARGET_BYTES[(params.fbFormat, params.fbType)]
+        return color.view(np.uint8)[: h * w * bpp].reshape(h, w, bpp).copy(), depth[: h * w].reshape(h, w), err
     return color[: h * w].reshape(h, w), depth[: h * w].reshape(h, w), err
 
 

@@ -53,11 +53,89 @@ struct DTex {
 };
 
 struct DFrame {
-    uint32_t* color;        // device render target
+    uint8_t*  color;        // device render target: W*H pixels of `kind`
     float*    depth;
     uint32_t  W, H;
-    uint32_t  tilesX, tilesY, tileBase, pad;
+    uint32_t  tilesX, tilesY, tileBase, kind;   // kind: PFF_TARGET_*
 };
+
+// ---------------------------------------------------------------------------
+// Render-target pixels.  A pixel is held as the word of its stored bytes
+// (byte i of the pixel in bits 8i..8i+7: RGBA8 words, RGB/BGR byte triples,
+// 16-bit words).  The triangle raster and pfClear go through the
+// reference's SIMD accessors (pixel.h:1492-2067 setters, 2536-2913
+// getters); points, lines, pfDrawPixels, rects and fog through its scalar
+// ones (tgt_get_scalar / tgt_set_scalar below).
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t tgt_load(const uint8_t* p, uint32_t kind, size_t i) {
+    if (kind == PFF_TARGET_RGBA8) return ((const uint32_t*)p)[i];
+    if (kind <= PFF_TARGET_BGR8) {
+        const uint8_t* q = p + 3 * i;
+        return (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+    }
+    return ((const uint16_t*)p)[i];
+}
+
+__device__ __forceinline__ void tgt_store(uint8_t* p, uint32_t kind, size_t i, uint32_t v) {
+    if (kind == PFF_TARGET_RGBA8) { ((uint32_t*)p)[i] = v; return; }
+    if (kind <= PFF_TARGET_BGR8) {
+        uint8_t* q = p + 3 * i;
+        q[0] = (uint8_t)v;
+        q[1] = (uint8_t)(v >> 8);
+        q[2] = (uint8_t)(v >> 16);
+        return;
+    }
+    ((uint16_t*)p)[i] = (uint16_t)v;
+}
+
+// The SIMD setter's encoding of RGBA8 colour c.  Channels scale by
+// (x * (2^n - 1)) >> 8; the 5_5_5_1 alpha bit is a > 127.
+__host__ __device__ __forceinline__ uint32_t tgt_set_simd(uint32_t kind, uint32_t c) {
+    const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
+    switch (kind) {
+    case PFF_TARGET_RGB8:     return c & 0xFFFFFFu;                                  // :1590-1638
+    case PFF_TARGET_BGR8:     return b | (g << 8) | (r << 16);                       // :1641-1689
+    case PFF_TARGET_RGB565:   return (((r * 31u) >> 8) << 11) | (((g * 63u) >> 8) << 5) | ((b * 31u) >> 8);  // :1492-1538
+    case PFF_TARGET_BGR565:   return (((b * 31u) >> 8) << 11) | (((g * 63u) >> 8) << 5) | ((r * 31u) >> 8);  // :1541-1587
+    case PFF_TARGET_RGBA5551: return (((r * 31u) >> 8) << 11) | (((g * 31u) >> 8) << 6) | (((b * 31u) >> 8) << 1)
+                                     | (a > 127u ? 1u : 0u);                         // :1851-1900
+    case PFF_TARGET_BGRA5551: return (((b * 31u) >> 8) << 11) | (((g * 31u) >> 8) << 6) | (((r * 31u) >> 8) << 1)
+                                     | (a > 127u ? 1u : 0u);                         // :1903-1952
+    case PFF_TARGET_RGBA4444: return (((r * 15u) >> 8) << 12) | (((g * 15u) >> 8) << 8) | (((b * 15u) >> 8) << 4)
+                                     | ((a * 15u) >> 8);                             // :1955-2005
+    case PFF_TARGET_BGRA4444: return (((b * 15u) >> 8) << 12) | (((g * 15u) >> 8) << 8) | (((r * 15u) >> 8) << 4)
+                                     | ((a * 15u) >> 8);                             // :2008-2058
+    default:                  return c;                                              // :2061-2067
+    }
+}
+
+// The SIMD getter's RGBA8 reading of stored pixel v: 16-bit fields widened
+// by shifts (5_6_5) or integer products (x 8 for 5 bits, x 17 for 4 bits).
+__device__ __forceinline__ uint32_t tgt_get_simd(uint32_t kind, uint32_t v) {
+    switch (kind) {
+    case PFF_TARGET_RGB8:     return v | 0xFF000000u;                                // :2604-2632
+    case PFF_TARGET_BGR8:     return ((v >> 16) & 255u) | (v & 0xFF00u) | ((v & 255u) << 16) | 0xFF000000u;  // :2635-2663
+    case PFF_TARGET_RGB565: case PFF_TARGET_BGR565: {                                // :2536-2601
+        const uint32_t hi = ((v & 0xF800u) >> 11) << 3, g = ((v & 0x07E0u) >> 5) << 2, lo = (v & 0x1Fu) << 3;
+        return kind == PFF_TARGET_RGB565 ? (0xFF000000u | (lo << 16) | (g << 8) | hi)
+                                         : (0xFF000000u | (hi << 16) | (g << 8) | lo);
+    }
+    case PFF_TARGET_RGBA5551: case PFF_TARGET_BGRA5551: {                            // :2794-2849
+        const uint32_t c0 = ((v >> 11) & 0x1Fu) * 8u, c1 = ((v >> 6) & 0x1Fu) * 8u, c2 = ((v >> 1) & 0x1Fu) * 8u;
+        const uint32_t a = (v & 1u) * 255u;
+        return kind == PFF_TARGET_RGBA5551 ? ((a << 24) | (c2 << 16) | (c1 << 8) | c0)
+                                           : ((a << 24) | (c0 << 16) | (c1 << 8) | c2);
+    }
+    case PFF_TARGET_RGBA4444: case PFF_TARGET_BGRA4444: {                            // :2852-2907
+        const uint32_t c0 = ((v >> 12) & 0xFu) * 17u, c1 = ((v >> 8) & 0xFu) * 17u, c2 = ((v >> 4) & 0xFu) * 17u;
+        const uint32_t a = (v & 0xFu) * 17u;
+        return kind == PFF_TARGET_RGBA4444 ? ((a << 24) | (c2 << 16) | (c1 << 8) | c0)
+                                           : ((a << 24) | (c0 << 16) | (c1 << 8) | c2);
+    }
+    default:                  return v;                                              // :2910-2913
+    }
+}
 
 // What the raster kernel needs of a tile's frame in one pass: the frame
 // and its segment's clear, in one record (one scalar load per wave).
@@ -65,7 +143,7 @@ struct DTileFrame {
     DFrame   F;
     uint32_t active;        // the frame has a segment in this pass
     uint32_t clear;
-    uint32_t clearColor;
+    uint32_t clearColor;    // as the target stores it (tgt_set_simd)
     float    clearDepth;
     uint32_t pad[2];
 };
@@ -1081,7 +1159,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
                 const DSeg sg = A.segs[s];
                 if (!sg.clear) continue;
                 const DFrame F = A.frames[sg.frame];
-                A.clearSnap[2 * sg.frame] = F.color[0];
+                A.clearSnap[2 * sg.frame] = tgt_load(F.color, F.kind, 0);
                 A.clearSnap[2 * sg.frame + 1] = __float_as_uint(F.depth[0]);
             }
     }
@@ -1598,8 +1676,8 @@ __device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& 
 
 // Depth test and blend of one fragment against the lane's pixel
 // (depth.h:80-124, blend.h:137-274, triangles.c:434-438, 523-529).
-__device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t& col, float& dep,
-                                    uint32_t& win) {
+__device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t kind, uint32_t& col,
+                                    float& dep, uint32_t& win) {
     if (meta & META_DEPTH) {
         uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
         bool ok;
@@ -1612,8 +1690,16 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
         }
         if (!ok) return;
     }
-    if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
-    col = c;
+    if (kind == PFF_TARGET_RGBA8) {
+        if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
+        col = c;
+    } else {
+        // col holds the target's stored pixel: blending reads it through the
+        // SIMD getter and the result goes back through the SIMD setter
+        // (triangles.c:523-529).
+        if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, tgt_get_simd(kind, col));
+        col = tgt_set_simd(kind, c);
+    }
     dep = z;
     win = id + 1u;
 }
@@ -1626,7 +1712,8 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
 // in pm[k] (slots grow in submission order).
 template <bool PHONG, int K, bool ROT>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
-                                            int nf, uint64_t* pm, uint32_t* col, float* dep, uint32_t* win) {
+                                            int nf, uint64_t* pm, uint32_t kind, uint32_t* col, float* dep,
+                                            uint32_t* win) {
     constexpr int TW = (K == 1) ? 8 : 16;
     wave_sync();
     if (lane < nf) {
@@ -1649,7 +1736,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
             const int s = __builtin_ctzll(m);
             m &= m - 1ull;
             const int j = Q.ftri[s];
-            rop(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], col[k], dep[k], win[k]);
+            rop(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], kind, col[k], dep[k], win[k]);
         }
     }
     wave_sync();
@@ -1693,13 +1780,13 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         dep[k] = 0.0f;
         if (in[k]) {
             if (sg.clear && pi[k] >= 8 && pi[k] < aligned) {       // context.c:693-713
-                col[k] = sg.clearColor;
+                col[k] = sg.clearColor;                              // as the SIMD setter stores it
                 dep[k] = sg.clearDepth;
             } else if (sg.clear && pi[k] >= aligned) {              // the tail copies pixel 0
                 col[k] = A.clearSnap[2 * frame];
                 dep[k] = __uint_as_float(A.clearSnap[2 * frame + 1]);
             } else {
-                col[k] = F.color[pi[k]];
+                col[k] = tgt_load(F.color, F.kind, pi[k]);
                 dep[k] = F.depth[pi[k]];
             }
         }
@@ -1812,12 +1899,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                         float fz;
                         shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), px, py[k],
                                               fc, fz);
-                        rop(t.meta, fc, fz, Q.recId[j], col[k], dep[k], win[k]);
+                        rop(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
                     }
                     continue;
                 }
                 if (nf + c > 64) {
-                    flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
+                    flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
                     nf = 0;
                 }
                 if (cov) {
@@ -1829,13 +1916,13 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 nf += c;
             }
         }
-        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, col, dep, win);
+        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
         wave_sync();
     }
 #pragma unroll
     for (int k = 0; k < K; k++)
         if (in[k]) {
-            F.color[pi[k]] = col[k];
+            tgt_store(F.color, F.kind, pi[k], col[k]);
             F.depth[pi[k]] = dep[k];
             if (A.winner) A.winner[pi[k]] = win[k];
         }
@@ -2445,11 +2532,75 @@ __global__ __launch_bounds__(256) void k_decode_image(const uint8_t* __restrict_
     if (i < n) out[i] = decode_scalar_texel(src, i, source);
 }
 
+// The scalar getter of a target (pixel.h:549-707) on stored pixel v:
+// 16-bit fields scaled by 255.0f/31, /63 or /15 in float and truncated.
+__device__ __forceinline__ uint32_t tgt_get_scalar(uint32_t kind, uint32_t v) {
+    switch (kind) {
+    case PFF_TARGET_RGB8: return v | 0xFF000000u;
+    case PFF_TARGET_BGR8: return ((v >> 16) & 255u) | (v & 0xFF00u) | ((v & 255u) << 16) | 0xFF000000u;
+    case PFF_TARGET_RGB565: case PFF_TARGET_BGR565: {
+        const uint32_t hi = f2u8((float)((v & 0xF800u) >> 11) * (255.0f / 31));
+        const uint32_t g = f2u8((float)((v & 0x07E0u) >> 5) * (255.0f / 63));
+        const uint32_t lo = f2u8((float)(v & 0x001Fu) * (255.0f / 31));
+        return kind == PFF_TARGET_RGB565 ? pack4(hi, g, lo, 255) : pack4(lo, g, hi, 255);
+    }
+    case PFF_TARGET_RGBA5551: case PFF_TARGET_BGRA5551: {
+        const uint32_t c11 = f2u8((float)((v & 0xF800u) >> 11) * (255.0f / 31));
+        const uint32_t c6 = f2u8((float)((v & 0x07C0u) >> 6) * (255.0f / 31));
+        const uint32_t c1 = f2u8((float)((v & 0x003Eu) >> 1) * (255.0f / 31));
+        const uint32_t a = (v & 1u) * 255u;
+        return kind == PFF_TARGET_RGBA5551 ? pack4(c11, c6, c1, a) : pack4(c1, c6, c11, a);
+    }
+    case PFF_TARGET_RGBA4444: case PFF_TARGET_BGRA4444: {
+        const uint32_t c12 = f2u8((float)((v & 0xF000u) >> 12) * (255.0f / 15));
+        const uint32_t c8 = f2u8((float)((v & 0x0F00u) >> 8) * (255.0f / 15));
+        const uint32_t c4 = f2u8((float)((v & 0x00F0u) >> 4) * (255.0f / 15));
+        const uint32_t a = f2u8((float)(v & 0x000Fu) * (255.0f / 15));
+        return kind == PFF_TARGET_RGBA4444 ? pack4(c12, c8, c4, a) : pack4(c4, c8, c12, a);
+    }
+    default: return v;
+    }
+}
+
+// The scalar setter of a target (pixel.h:206-400) for RGBA8 colour c.
+// Channels are normalised by (float)(1/255) and rounded (roundf) to the
+// field; the 5_5_5_1 alpha bit is n_a > 50/255 (config.h:45-47, compared
+// in double).  The BGRA 5_5_5_1 and 4_4_4_4 setters normalise without the
+// red/blue swap (pixel.h:59-67), so they store RGBA order.
+__device__ __forceinline__ uint32_t tgt_set_scalar(uint32_t kind, uint32_t c) {
+    const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
+    const float inv = (float)(1.0 / 255.0);
+    switch (kind) {
+    case PFF_TARGET_RGB8: return r | (g << 8) | (b << 16);
+    case PFF_TARGET_BGR8: return b | (g << 8) | (r << 16);
+    case PFF_TARGET_RGB565: case PFF_TARGET_BGR565: {
+        const uint32_t c0 = kind == PFF_TARGET_RGB565 ? r : b, c2 = kind == PFF_TARGET_RGB565 ? b : r;
+        return (((uint32_t)(uint8_t)roundf((float)c0 * inv * 31.0f)) << 11)
+             | (((uint32_t)(uint8_t)roundf((float)g * inv * 63.0f)) << 5)
+             | (uint32_t)(uint8_t)roundf((float)c2 * inv * 31.0f);
+    }
+    case PFF_TARGET_RGBA5551: case PFF_TARGET_BGRA5551: {
+        const float na = (float)a * inv;
+        return (((uint32_t)(uint8_t)roundf((float)r * inv * 31.0f)) << 11)
+             | (((uint32_t)(uint8_t)roundf((float)g * inv * 31.0f)) << 6)
+             | (((uint32_t)(uint8_t)roundf((float)b * inv * 31.0f)) << 1)
+             | ((double)na > 50.0 * (1.0 / 255.0) ? 1u : 0u);
+    }
+    case PFF_TARGET_RGBA4444: case PFF_TARGET_BGRA4444:
+        return (((uint32_t)(uint8_t)roundf((float)r * inv * 15.0f)) << 12)
+             | (((uint32_t)(uint8_t)roundf((float)g * inv * 15.0f)) << 8)
+             | (((uint32_t)(uint8_t)roundf((float)b * inv * 15.0f)) << 4)
+             | (uint32_t)(uint8_t)roundf((float)a * inv * 15.0f);
+    default: return c;
+    }
+}
+
 struct DPixOp {
     uint32_t  kind;               // 0 pfDrawPixels, 1 pfFogProcess
     const uint32_t* texels;       // w*h RGBA8
-    uint32_t* color;              // render target
+    uint8_t*  color;              // render target
     float*    depth;
+    uint32_t  target;             // PFF_TARGET_* of color
     uint32_t  W, H;
     uint32_t  w, h;
     int32_t   xScreen, yScreen, xMin, yMin, xMax, yMax;
@@ -2505,7 +2656,6 @@ __device__ __forceinline__ uint32_t blend_scalar(uint32_t mode, uint32_t s, uint
 __device__ __forceinline__ uint32_t fog_pixel(const DPixOp& o, float depth, uint32_t dst) {
     const uint32_t fc = o.fogColor, alpha = fc >> 24;
     if (depth >= o.fogEnd) return alpha == 255u ? fc : blend_scalar(1u, fc, dst);
-    if (!(depth > o.fogStart)) return dst;
     float t = 0.0f;
     const float d = -o.fogDensity * (depth - o.fogStart);
     if (o.fogMode == 0u) t = (depth - o.fogStart) * __fdiv_rn(1.0f, o.fogEnd - o.fogStart);
@@ -2521,8 +2671,12 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
     const DPixOp o = ops[blockIdx.y];
     if (o.kind == 1u) {
         const size_t n = (size_t)o.W * o.H;
-        for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
-            o.color[k] = fog_pixel(o, o.depth[k], o.color[k]);
+        for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
+            const float z = o.depth[k];
+            if (!(z >= o.fogEnd) && !(z > o.fogStart)) continue;      // untouched: no getter or setter call
+            const uint32_t c = fog_pixel(o, z, tgt_get_scalar(o.target, tgt_load(o.color, o.target, k)));
+            tgt_store(o.color, o.target, k, tgt_set_scalar(o.target, c));
+        }
         return;
     }
     if (o.xMax < o.xMin || o.yMax < o.yMin) return;
@@ -2539,8 +2693,9 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
         uint32_t src = f2u32_x86(v * (float)(o.h - 1u)) * o.w + f2u32_x86(u * (float)(o.w - 1u));
         if (src >= n) src = n - 1u;
         if (!(o.flags & PFF_PIXELS_RECT)) o.depth[dst] = o.zPos;      // pfRect writes no depth
-        const uint32_t c = o.texels[src];
-        o.color[dst] = (o.flags & PFF_BLEND) ? blend_scalar(o.blendMode, c, o.color[dst]) : c;
+        uint32_t c = o.texels[src];
+        if (o.flags & PFF_BLEND) c = blend_scalar(o.blendMode, c, tgt_get_scalar(o.target, tgt_load(o.color, o.target, dst)));
+        tgt_store(o.color, o.target, dst, tgt_set_scalar(o.target, c));
     }
 }
 
@@ -2624,8 +2779,9 @@ __global__ __launch_bounds__(256) void k_point_setup(const PFFpoint* __restrict_
 }
 
 struct DPtOp {
-    uint32_t* color;
+    uint8_t*  color;
     float*    depth;
+    uint32_t  target;             // PFF_TARGET_* of color
     uint32_t  W, H, tilesX, tilesY;
     uint32_t  first, last;        // job point indices
 };
@@ -2637,7 +2793,7 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
     const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
     const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
     const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
-    uint32_t c = inside ? o.color[off] : 0u;
+    uint32_t c = inside ? tgt_load(o.color, o.target, off) : 0u;      // stored form
     float d = inside ? o.depth[off] : 0.0f;
     bool dirty = false;
     __shared__ uint32_t idx[256];
@@ -2660,14 +2816,15 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
                        : (dx >= -s.ir && dx <= s.ir && dy >= -s.ir && dy <= s.ir && (float)(dy * dy + dx * dx) <= s.rSq);
         if (!inside || !cov) continue;
         if ((s.flags & PFF_DEPTH_TEST) && !depth_scalar(s.depthFunc, s.z, d)) continue;
-        c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, c) : s.color;
+        c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, tgt_get_scalar(o.target, c))
+                                                            : s.color);
         d = s.z;
         dirty = true;
       }
       __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
-        o.color[off] = c;
+        tgt_store(o.color, o.target, off, c);
         o.depth[off] = d;
     }
 }
@@ -2831,8 +2988,9 @@ __global__ __launch_bounds__(256) void k_line_setup(const PFFline* __restrict__ 
 }
 
 struct DLnOp {
-    uint32_t* color;
+    uint8_t*  color;
     float*    depth;
+    uint32_t  target;             // PFF_TARGET_* of color
     uint32_t  W, H, tilesX, tilesY;
     uint32_t  first, last;        // job line indices
 };
@@ -2865,7 +3023,7 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
     const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
     const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
     const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
-    uint32_t c = inside ? o.color[off] : 0u;
+    uint32_t c = inside ? tgt_load(o.color, o.target, off) : 0u;      // stored form
     float d = inside ? o.depth[off] : 0.0f;
     bool dirty = false;
     // Line positions whose offset y * W + x is this pixel's.
@@ -2914,7 +3072,8 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
                     const int32_t ca = (int32_t)((s.c1 >> (8 * q)) & 255u), cb = (int32_t)((s.c2 >> (8 * q)) & 255u);
                     col |= (uint32_t)f2u8((float)ca + t * (float)(cb - ca)) << (8 * q);
                 }
-                c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, c) : col;
+                c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, tgt_get_scalar(o.target, c))
+                                                                    : col);
                 d = z;
                 dirty = true;
             }
@@ -2923,7 +3082,7 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
       __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
-        o.color[off] = c;
+        tgt_store(o.color, o.target, off, c);
         o.depth[off] = d;
     }
 }
@@ -3552,7 +3711,7 @@ int upload_frames_table(PFDjob* J) {
             const DSeg& sg = P.segs[si];
             t.active = 1;
             t.clear = sg.clear;
-            t.clearColor = sg.clearColor;
+            t.clearColor = tgt_set_simd(t.F.kind, sg.clearColor);        // stored form of the clear colour
             t.clearDepth = sg.clearDepth;
         }
     }
@@ -3792,7 +3951,9 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         totalStates += fr.numStates;
         anyUV |= fr.tris.texcoord != nullptr;
         anyNrm |= fr.tris.normal != nullptr;
-        targetBytes += 2 * align256((size_t)fr.width * fr.height * 4);
+        if (pff_target_bytes(fr.target) == 0u) PLAN_FAIL(PFD_E_INVALID, "frame %u: unknown render target format %u", f, fr.target);
+        targetBytes += align256((size_t)fr.width * fr.height * pff_target_bytes(fr.target))
+                     + align256((size_t)fr.width * fr.height * 4);
         for (uint32_t k = 0; k < fr.numTextures; k++) {
             const PFFtexture& T = fr.textures[k];
             if (T.width > 0xFFFFu || T.height > 0xFFFFu)        // raster records keep 16-bit sides
@@ -3885,8 +4046,9 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         D.W = fr.width;
         D.H = fr.height;
         const size_t npx = (size_t)fr.width * fr.height;
-        D.color = (uint32_t*)((uint8_t*)J->targetPool.p + targetOff);
-        targetOff += align256(npx * 4);
+        D.kind = fr.target;
+        D.color = (uint8_t*)J->targetPool.p + targetOff;
+        targetOff += align256(npx * pff_target_bytes(fr.target));
         D.depth = (float*)((uint8_t*)J->targetPool.p + targetOff);
         targetOff += align256(npx * 4);
 
@@ -3940,7 +4102,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 if (!fr.lines || cmd.first > cmd.last || cmd.last > fr.numLines) PLAN_FAIL(PFD_E_INVALID, "bad lines command");
                 DLnOp d;
                 memset(&d, 0, sizeof d);
-                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.color = D.color; d.depth = D.depth; d.target = D.kind; d.W = D.W; d.H = D.H;
                 d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
                 d.first = lnBase + cmd.first; d.last = lnBase + cmd.last;
                 own_pass().lns.push_back(d);
@@ -3950,7 +4112,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 if (!fr.points || cmd.first > cmd.last || cmd.last > fr.numPoints) PLAN_FAIL(PFD_E_INVALID, "bad points command");
                 DPtOp d;
                 memset(&d, 0, sizeof d);
-                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.color = D.color; d.depth = D.depth; d.target = D.kind; d.W = D.W; d.H = D.H;
                 d.tilesX = (D.W + 15) / 16; d.tilesY = (D.H + 15) / 16;
                 d.first = ptBase + cmd.first; d.last = ptBase + cmd.last;
                 own_pass().pts.push_back(d);
@@ -3961,7 +4123,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 DPixOp d;
                 memset(&d, 0, sizeof d);
                 d.kind = 1u;
-                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.color = D.color; d.depth = D.depth; d.target = D.kind; d.W = D.W; d.H = D.H;
                 d.fogMode = op.fogMode; d.fogColor = op.fogColor;
                 d.fogDensity = op.fogDensity; d.fogStart = op.fogStart; d.fogEnd = op.fogEnd;
                 d.xMin = 0; d.yMin = 0; d.xMax = (int32_t)D.W - 1; d.yMax = (int32_t)D.H - 1;
@@ -3989,7 +4151,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 DPixOp d;
                 memset(&d, 0, sizeof d);
                 d.texels = img;
-                d.color = D.color; d.depth = D.depth; d.W = D.W; d.H = D.H;
+                d.color = D.color; d.depth = D.depth; d.target = D.kind; d.W = D.W; d.H = D.H;
                 d.w = op.width; d.h = op.height;
                 d.xScreen = op.xScreen; d.yScreen = op.yScreen;
                 d.xMin = op.xMin; d.yMin = op.yMin; d.xMax = op.xMax; d.yMax = op.yMax;
@@ -4289,7 +4451,8 @@ int pfdJobUploadTargets(PFDjob* J) {
         size_t npx = (size_t)fr.width * fr.height;
         bool leadClear = J->leadClear[f] != 0;
         size_t n = leadClear ? (npx < 8 ? npx : 8) : npx;
-        HIP_TRY(hipMemcpyAsync(J->frames[f].color, fr.color, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpyAsync(J->frames[f].color, fr.color, n * pff_target_bytes(fr.target), hipMemcpyHostToDevice,
+                               J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipMemcpyAsync(J->frames[f].depth, fr.depth, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
     }
     HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
@@ -4329,7 +4492,8 @@ int pfdJobDownload(PFDjob* J) {
     for (size_t f = 0; f < J->frames.size(); f++) {
         const PFFframe& fr = J->src[f];
         size_t npx = (size_t)fr.width * fr.height;
-        HIP_TRY(hipMemcpyAsync(fr.color, J->frames[f].color, npx * 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpyAsync(fr.color, J->frames[f].color, npx * pff_target_bytes(fr.target), hipMemcpyDeviceToHost,
+                               J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipMemcpyAsync(fr.depth, J->frames[f].depth, npx * 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
     }
     HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
@@ -4378,7 +4542,7 @@ int pfdJobTimeReplays(PFDjob* J, uint32_t iters, float* msTotal) {
     return PFD_OK;
 }
 
-int pfdJobTarget(PFDjob* J, uint32_t frame, uint32_t** color, float** depth, uint32_t* width, uint32_t* height) {
+int pfdJobTarget(PFDjob* J, uint32_t frame, void** color, float** depth, uint32_t* width, uint32_t* height) {
     if (!J || frame >= J->src.size()) return PFD_E_INVALID;
     const PFFframe& fr = J->src[frame];
     if (color) *color = fr.color;
@@ -4609,6 +4773,10 @@ PFDjob* pfdTakeCapturedJob(void) {
 int pfdDiffCreate(const PFFframe* frame, PFDjob** out) {
     if (!frame || !out) return PFD_E_INVALID;
     *out = nullptr;
+    if (frame->target != PFF_TARGET_RGBA8) {
+        snprintf(g_err, sizeof g_err, "differentiable frame: the render target must be RGBA / UNSIGNED_BYTE");
+        return PFD_E_INVALID;
+    }
     // One pass: at most one clear, before every draw.
     for (uint32_t c = 0; c < frame->numCmds; c++)
         if ((frame->cmds[c].kind == PFF_CMD_CLEAR && c != 0) || frame->cmds[c].kind == PFF_CMD_PIXELS
@@ -4710,7 +4878,7 @@ int diff_args(PFDjob* J, const PFDdiffParams* p, DiffArgs& a) {
     a.rtl = (const RTriL*)J->rtl.p;
     a.rtd = (const RTriD*)J->rtd.p;
     a.winner = (const uint32_t*)J->winner.p;
-    a.color = J->frames[0].color;
+    a.color = (const uint32_t*)J->frames[0].color;
     a.srcCol = (const uint32_t*)J->col.p;
     a.vid = J->vid.p ? (const uint32_t*)J->vid.p : nullptr;
     a.states = (const PFFstate*)J->states.p;

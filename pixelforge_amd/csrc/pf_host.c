@@ -108,9 +108,9 @@ typedef struct {                       /* PFIctxbackup, context.h:300-316 */
 } Backup;
 
 typedef struct {                       /* the frame being recorded */
-    uint32_t*  targetColor;
+    void*      targetColor;
     float*     targetDepth;
-    uint32_t   targetW, targetH;
+    uint32_t   targetW, targetH, targetKind;
     float*     pos;  uint32_t* col;  float* uv;  float* nrm;  uint32_t* info;  uint32_t* vid;
     size_t     ntri, captri;
     PFFstate*  states;   size_t nstates, capstates;
@@ -375,6 +375,7 @@ static void build_frame(Ctx* c, PFFframe* f)
     f->height = r->targetH;
     f->color = r->targetColor;
     f->depth = r->targetDepth;
+    f->target = r->targetKind;
     f->tris.count = (uint32_t)r->ntri;
     f->tris.position = r->pos;
     f->tris.color = r->col;
@@ -422,14 +423,17 @@ static int ensure_target(Ctx* c)
     Tex* t = fb_tex(c->curFB);
     if (!t || !t->pixels || !c->curFB->zbuffer) return 0;
     Rec* r = &c->rec;
-    if (r->ncmd && (r->targetColor != (uint32_t*)t->pixels || r->targetDepth != c->curFB->zbuffer
-                    || r->targetW != t->w || r->targetH != t->h))
+    const uint32_t kind = pff_target_of(PFF_SOURCE(t->format, t->type));
+    if (kind == PFF_TARGET_NONE) return 0;
+    if (r->ncmd && (r->targetColor != t->pixels || r->targetDepth != c->curFB->zbuffer
+                    || r->targetW != t->w || r->targetH != t->h || r->targetKind != kind))
         submit(c);
     if (r->ncmd == 0) {
-        r->targetColor = (uint32_t*)t->pixels;
+        r->targetColor = t->pixels;
         r->targetDepth = c->curFB->zbuffer;
         r->targetW = t->w;
         r->targetH = t->h;
+        r->targetKind = kind;
     }
     return 1;
 }
@@ -871,7 +875,7 @@ void* pfGetTexturePixels(const PFtexture texture, PFsizei* width, PFsizei* heigh
 
 PFcontext pfCreateContext(void* targetBuffer, PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
 {
-    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { set_err(PF_INVALID_ENUM); return NULL; }
+    if (pff_target_of(PFF_SOURCE(format, type)) == PFF_TARGET_NONE) { set_err(PF_INVALID_ENUM); return NULL; }
     Ctx* c = (Ctx*)calloc(1, sizeof(Ctx));
     if (!c) return NULL;
     size_t n = (size_t)width * height;
@@ -962,7 +966,7 @@ void pfSetMainBuffer(void* targetBuffer, PFsizei width, PFsizei height, PFpixelf
     Ctx* c = G;
     if (!c) return;
     if (targetBuffer == NULL || width == 0 || height == 0) { c->err = PF_INVALID_VALUE; return; }
-    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { c->err = PF_INVALID_ENUM; return; }
+    if (pff_target_of(PFF_SOURCE(format, type)) == PFF_TARGET_NONE) { c->err = PF_INVALID_ENUM; return; }
     submit(c);
     Tex* old = fb_tex(&c->mainFB);
     uint32_t oldW = old->w, oldH = old->h;
@@ -984,6 +988,8 @@ void pfSetMainBuffer(void* targetBuffer, PFsizei width, PFsizei height, PFpixelf
     old->pixels = targetBuffer;
     old->w = width;
     old->h = height;
+    old->format = format;
+    old->type = type;
     c->auxBuffer = NULL;
 }
 
@@ -2223,6 +2229,9 @@ void pfFogProcess(void)
     push_cmd(c, &cmd);
 }
 
+static void set_pixel(void* dst, size_t i, PFpixelformat f, PFdatatype t, PFcolor c);
+static PFcolor get_pixel(const void* src, size_t i, PFpixelformat f, PFdatatype t);
+
 /* pfPostProcess (context.c:2397-2440): the caller's function is host code,
  * so the recorded frame is executed first and the function then runs over
  * the render target in host memory, pixel by pixel with its depth. */
@@ -2234,12 +2243,11 @@ void pfPostProcess(PFpostprocessfunc fn)
     submit(c);
     Tex* t = fb_tex(c->curFB);
     if (!t || !t->pixels || !c->curFB->zbuffer) { c->err = PF_INVALID_OPERATION; return; }
-    PFcolor* px = (PFcolor*)t->pixels;
     const float* z = c->curFB->zbuffer;
     for (int32_t y = 0; y < (int32_t)t->h; y++)
         for (int32_t x = 0; x < (int32_t)t->w; x++) {
             const size_t i = (size_t)y * t->w + (size_t)x;
-            px[i] = fn(x, y, z[i], px[i]);
+            set_pixel(t->pixels, i, t->format, t->type, fn(x, y, z[i], get_pixel(t->pixels, i, t->format, t->type)));
         }
 }
 
@@ -2611,6 +2619,41 @@ static void set_pixel(void* dst, size_t i, PFpixelformat f, PFdatatype t, PFcolo
     }
 }
 
+/* One pixel of a render target through the reference's scalar getter of
+ * its pair (pixel.h:549-707): RGB/BGR bytes, and the 16-bit fields scaled
+ * by 255.0f/31, /63 or /15 in float and truncated. */
+static PFcolor get_pixel(const void* src, size_t i, PFpixelformat f, PFdatatype t)
+{
+    const uint8_t* b = (const uint8_t*)src;
+    if (t == PF_UNSIGNED_BYTE) {
+        if (f == PF_RGB) return (PFcolor){ b[3 * i], b[3 * i + 1], b[3 * i + 2], 255 };
+        if (f == PF_BGR) return (PFcolor){ b[3 * i + 2], b[3 * i + 1], b[3 * i], 255 };
+        PFcolor c;
+        memcpy(&c, b + 4 * i, 4);
+        return c;
+    }
+    const uint32_t x = ((const uint16_t*)src)[i];
+    const float s5 = 255.0f / 31, s6 = 255.0f / 63, s4 = 255.0f / 15;
+    uint8_t hi, mid, lo, a;
+    if (t == PF_UNSIGNED_SHORT_5_6_5) {
+        hi = (uint8_t)((float)((x & 0xF800u) >> 11) * s5);
+        mid = (uint8_t)((float)((x & 0x07E0u) >> 5) * s6);
+        lo = (uint8_t)((float)(x & 0x001Fu) * s5);
+        a = 255;
+    } else if (t == PF_UNSIGNED_SHORT_5_5_5_1) {
+        hi = (uint8_t)((float)((x & 0xF800u) >> 11) * s5);
+        mid = (uint8_t)((float)((x & 0x07C0u) >> 6) * s5);
+        lo = (uint8_t)((float)((x & 0x003Eu) >> 1) * s5);
+        a = (uint8_t)((x & 1u) * 255u);
+    } else {
+        hi = (uint8_t)((float)((x & 0xF000u) >> 12) * s4);
+        mid = (uint8_t)((float)((x & 0x0F00u) >> 8) * s4);
+        lo = (uint8_t)((float)((x & 0x00F0u) >> 4) * s4);
+        a = (uint8_t)((float)(x & 0x000Fu) * s4);
+    }
+    return (f == PF_BGR || f == PF_BGRA) ? (PFcolor){ lo, mid, hi, a } : (PFcolor){ hi, mid, lo, a };
+}
+
 void pfReadPixels(PFint x, PFint y, PFsizei width, PFsizei height,
                   PFpixelformat format, PFdatatype type, void* pixels)   /* context.c:2349-2395 */
 {
@@ -2620,17 +2663,18 @@ void pfReadPixels(PFint x, PFint y, PFsizei width, PFsizei height,
      * the same table (context.c:2351-2361). */
     if (pff_source_texel_bytes(PFF_SOURCE(format, type)) == 0u) { c->err = PF_INVALID_ENUM; return; }
     submit(c);
-    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) {
-        const Tex* t = fb_tex(c->curFB);
+    const Tex* tt = fb_tex(c->curFB);
+    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE && tt->format == PF_RGBA && tt->type == PF_UNSIGNED_BYTE)) {
+        const Tex* t = tt;
         int32_t xMin = x < 0 ? 0 : (x > (int32_t)t->w - 1 ? (int32_t)t->w - 1 : x);
         int32_t yMin = y < 0 ? 0 : (y > (int32_t)t->h - 1 ? (int32_t)t->h - 1 : y);
         int64_t xe = (int64_t)x + (int32_t)width, ye = (int64_t)y + (int32_t)height;
         int32_t xMax = (int32_t)(xe < 0 ? 0 : (xe > t->w ? t->w : xe));
         int32_t yMax = (int32_t)(ye < 0 ? 0 : (ye > t->h ? t->h : ye));
-        const PFcolor* src = (const PFcolor*)t->pixels;
         for (int32_t ys = yMin, yd = 0; ys < yMax; ys++, yd++)
             for (int32_t xs = xMin, xd = 0; xs < xMax; xs++, xd++)
-                set_pixel(pixels, (size_t)yd * width + xd, format, type, src[(size_t)ys * t->w + xs]);
+                set_pixel(pixels, (size_t)yd * width + xd, format, type,
+                          get_pixel(t->pixels, (size_t)ys * t->w + xs, t->format, t->type));
         return;
     }
     const Tex* t = fb_tex(c->curFB);
@@ -2649,9 +2693,10 @@ void pfReadPixels(PFint x, PFint y, PFsizei width, PFsizei height,
 PFframebuffer pfGenFramebuffer(PFsizei width, PFsizei height, PFpixelformat format, PFdatatype type)
 {
     PFframebuffer fb = { NULL, NULL };
-    if (!(format == PF_RGBA && type == PF_UNSIGNED_BYTE)) { set_err(PF_INVALID_ENUM); return fb; }
+    const uint32_t kind = pff_target_of(PFF_SOURCE(format, type));
+    if (kind == PFF_TARGET_NONE) { set_err(PF_INVALID_ENUM); return fb; }
     size_t n = (size_t)width * height;
-    void* px = calloc(n ? n : 1, 4);
+    void* px = calloc(n ? n : 1, pff_target_bytes(kind));
     if (!px) return fb;
     PFtexture t = pfGenTexture(px, width, height, format, type);
     if (!t) { free(px); set_err(PF_INVALID_ENUM); return fb; }
@@ -2688,17 +2733,14 @@ void pfClearFramebuffer(PFframebuffer* fb, PFcolor color, PFfloat depth)   /* fr
     if (!t) return;
     finish_target(t->pixels);
     size_t n = (size_t)t->w * t->h;
-    uint32_t c = pack(color);
-    for (size_t i = 0; i < n; i++) { ((uint32_t*)t->pixels)[i] = c; fb->zbuffer[i] = depth; }
+    for (size_t i = 0; i < n; i++) { set_pixel(t->pixels, i, t->format, t->type, color); fb->zbuffer[i] = depth; }
 }
 
 PFcolor pfGetFramebufferPixel(const PFframebuffer* fb, PFsizei x, PFsizei y)
 {
     const Tex* t = fb_tex(fb);
     finish_target(t->pixels);
-    PFcolor c;
-    memcpy(&c, (const uint32_t*)t->pixels + (size_t)y * t->w + x, 4);
-    return c;
+    return get_pixel(t->pixels, (size_t)y * t->w + x, t->format, t->type);
 }
 
 PFfloat pfGetFramebufferDepth(const PFframebuffer* fb, PFsizei x, PFsizei y)
@@ -2728,7 +2770,7 @@ void pfSetFramebufferPixelDepthTest(PFframebuffer* fb, PFsizei x, PFsizei y, PFf
     case PF_GEQUAL: pass = z >= d; break;
     }
     if (pass) {
-        ((uint32_t*)t->pixels)[off] = pack(color);
+        set_pixel(t->pixels, off, t->format, t->type, color);
         fb->zbuffer[off] = z;
     }
 }
@@ -2738,7 +2780,7 @@ void pfSetFramebufferPixelDepth(PFframebuffer* fb, PFsizei x, PFsizei y, PFfloat
     Tex* t = fb_tex(fb);
     finish_target(t->pixels);
     size_t off = (size_t)y * t->w + x;
-    ((uint32_t*)t->pixels)[off] = pack(color);
+    set_pixel(t->pixels, off, t->format, t->type, color);
     fb->zbuffer[off] = z;
 }
 
@@ -2746,7 +2788,7 @@ void pfSetFramebufferPixel(PFframebuffer* fb, PFsizei x, PFsizei y, PFcolor colo
 {
     Tex* t = fb_tex(fb);
     finish_target(t->pixels);
-    ((uint32_t*)t->pixels)[(size_t)y * t->w + x] = pack(color);
+    set_pixel(t->pixels, (size_t)y * t->w + x, t->format, t->type, color);
 }
 
 /* ------------------------------------------------------------------------ */

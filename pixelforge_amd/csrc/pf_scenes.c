@@ -113,6 +113,12 @@ typedef struct {
                               * cells as one call in that mode (strip order:
                               * v(i, j), v(i, j+1), v(i+1, j), ...; PF_QUADS: the
                               * cells' four corners)                            */
+    int32_t  fbFormat;       /* >0: PFpixelformat of the render target (with
+                              * fbType); 0: RGBA / UNSIGNED_BYTE               */
+    int32_t  fbType;         /* PFdatatype of the render target               */
+    int32_t  fbMain;         /* 1: draw into the context's main buffer, created
+                              * in that format, instead of an offscreen
+                              * framebuffer (no depth is copied out)          */
 } PFSparams;
 
 typedef struct {
@@ -895,33 +901,58 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
     if (p->fog) apply_fog(a, p);
 }
 
-/* Renders one scene into an offscreen framebuffer of the bound library and
- * copies its colour (RGBA8) and depth into the caller's arrays.  The
- * context's own buffer is `mainColor` (W*H+16 pixels, may be NULL).
- * Returns the library's last error code, or -1 on setup failure. */
-int pfs_render(const PFSapi* a, const PFSparams* p, uint32_t* colorOut, float* depthOut)
+/* Bytes per pixel of the render-target pairs scenes use (RGBA8 for the
+ * others). */
+static size_t target_bytes(PFpixelformat f, PFdatatype t)
+{
+    if (t != PF_UNSIGNED_BYTE) return 2;
+    return (f == PF_RGB || f == PF_BGR) ? 3 : 4;
+}
+
+/* Renders one scene into an offscreen framebuffer of the bound library (or,
+ * with fbMain, into the context's own buffer) and copies its colour (W*H
+ * pixels of the target's format) and depth into the caller's arrays.  The
+ * context's own buffer holds W*H+16 pixels.  Returns the library's last
+ * error code, or -1 on setup failure. */
+int pfs_render(const PFSapi* a, const PFSparams* p, void* colorOut, float* depthOut)
 {
     if (!a || !p || !p->width || !p->height) return -1;
     size_t n = (size_t)p->width * p->height;
-    uint32_t* mainColor = (uint32_t*)calloc(n + 16, 4);
+    const PFpixelformat ff = p->fbFormat > 0 ? (PFpixelformat)p->fbFormat : PF_RGBA;
+    const PFdatatype ft = p->fbFormat > 0 ? (PFdatatype)p->fbType : PF_UNSIGNED_BYTE;
+    const size_t bpp = target_bytes(ff, ft);
+    void* mainColor = calloc(n + 16, p->fbMain ? bpp : 4);
     if (!mainColor) return -1;
-    PFcontext ctx = a->CreateContext(mainColor, p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
+    PFcontext ctx = p->fbMain ? a->CreateContext(mainColor, p->width, p->height, ff, ft)
+                              : a->CreateContext(mainColor, p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
     if (!ctx) { free(mainColor); return -1; }
     a->MakeCurrent(ctx);
-    PFframebuffer fb = a->GenFramebuffer(p->width, p->height, PF_RGBA, PF_UNSIGNED_BYTE);
-    a->BindFramebuffer(&fb);
-    a->Enable(PF_FRAMEBUFFER);
+    PFframebuffer fb = { 0 };
+    if (!p->fbMain) {
+        fb = a->GenFramebuffer(p->width, p->height, ff, ft);
+        if (!fb.texture) {
+            int err = (int)a->GetError();
+            a->DeleteContext(ctx);
+            a->MakeCurrent(NULL);
+            free(mainColor);
+            return err ? err : -1;
+        }
+        a->BindFramebuffer(&fb);
+        a->Enable(PF_FRAMEBUFFER);
+    }
     Rng r = { p->seed };
     SceneTex tex;
     draw_scene(a, &r, p, &tex);
     if (a->Finish) a->Finish();
     int err = (int)a->GetError();
-    uint32_t* px = (uint32_t*)a->GetTexturePixels(fb.texture, NULL, NULL, NULL, NULL);
-    if (colorOut) memcpy(colorOut, px, n * 4);
-    if (depthOut) memcpy(depthOut, fb.zbuffer, n * 4);
-    a->Disable(PF_FRAMEBUFFER);
-    a->BindFramebuffer(NULL);
-    a->DeleteFramebuffer(&fb);
+    const void* px = p->fbMain ? mainColor : a->GetTexturePixels(fb.texture, NULL, NULL, NULL, NULL);
+    if (colorOut) memcpy(colorOut, px, n * bpp);
+    if (depthOut && !p->fbMain) memcpy(depthOut, fb.zbuffer, n * 4);
+    if (!p->fbMain) {
+        a->Disable(PF_FRAMEBUFFER);
+        a->BindFramebuffer(NULL);
+        a->DeleteFramebuffer(&fb);
+    }
     free_texture(a, &tex);
     a->DeleteContext(ctx);
     a->MakeCurrent(NULL);

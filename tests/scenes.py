@@ -235,6 +235,31 @@ RECT_SCENES = [
     ("rects_3d", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60, seed=701, rects=5)),
 ]
 
+# Render targets other than RGBA8 (the render-target pairs of the
+# library): per pair, a textured 3D scene drawn blended into a 61x45
+# offscreen framebuffer (SIMD getter and setter of the raster, the clear
+# and its one-pixel tail), a 2D scene with pfDrawPixels, points, lines,
+# rects and fog between its halves (the scalar accessors), and an
+# unblended 3D scene in the context's main buffer created in that pair.
+_TARGETS = [("rgb", "ubyte"), ("bgr", "ubyte"), ("rgb", "u565"), ("bgr", "u565"),
+            ("rgba", "u5551"), ("bgra", "u5551"), ("rgba", "u4444"), ("bgra", "u4444")]
+TARGET_SCENES = [
+    (f"target_{f}_{t}_raster", dict(kind=pf.RANDOM_3D, width=61, height=45, ntris=150, seed=800 + k, texture=1,
+                                    texW=17, texH=13, blend=1, blendMode=1 + k % 7,
+                                    fbFormat=_FORMATS[f], fbType=_TYPES[t]))
+    for k, (f, t) in enumerate(_TARGETS)
+] + [
+    (f"target_{f}_{t}_prims", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=820 + k, pixelOps=3,
+                                   pixFormat=7, pixType=0, points=120, pointSize=3.0, lines=40, lineWidth=2.0,
+                                   rects=3, fog=1, fogStart=-0.5, fogEnd=0.8, fogAlpha=200, blend=1, blendMode=1,
+                                   depthFunc=pf.PF_LEQUAL, fbFormat=_FORMATS[f], fbType=_TYPES[t]))
+    for k, (f, t) in enumerate(_TARGETS)
+] + [
+    (f"target_{f}_{t}_main", dict(kind=pf.RANDOM_3D, width=41, height=29, ntris=100, seed=840 + k,
+                                  fbFormat=_FORMATS[f], fbType=_TYPES[t], fbMain=1))
+    for k, (f, t) in enumerate(_TARGETS + [("rgba", "ubyte")])
+]
+
 # Golden frames rendered by the reference with extra scene parameters.
 # BGRA / UNSIGNED_BYTE: the reference's AVX2 getter shuffles with a mask that
 # repeats 2,1,0,3 at every 4 bytes of a 128-bit register (pixel.h:2916-2920,
@@ -271,7 +296,7 @@ SIZE_SCENES = [
 
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
                  + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES
-                 + RECT_SCENES + BILINEAR_SCENES)
+                 + RECT_SCENES + BILINEAR_SCENES + TARGET_SCENES)
 ALL_SCENES = GOLDEN_SCENES + EDGE_READ_SCENES + SIZE_SCENES
 
 

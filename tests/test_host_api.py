@@ -61,9 +61,36 @@ def test_error_codes_are_sticky_and_cleared():
 
 
 def test_unsupported_formats_are_rejected():
+    """Render targets: the nine pairs whose reference accessors touch only
+    their own pixel (RGBA, RGB and BGR bytes; RGB/BGR 5_6_5; RGBA/BGRA
+    5_5_5_1 and 4_4_4_4); every other pair is PF_INVALID_ENUM for
+    pfCreateContext and pfGenFramebuffer."""
     L = pf.lib()
     buf = np.zeros(64, np.uint32)
-    assert not L.pfCreateContext(buf.ctypes.data, 8, 8, 6, 0)     # PF_RGB
+    targets = {(7, 0), (6, 0), (8, 0), (6, 2), (8, 2), (7, 3), (9, 3), (7, 4), (9, 4)}
+    accepted = set()
+    for f in range(10):
+        for t in range(12):
+            ctx = L.pfCreateContext(buf.ctypes.data, 8, 8, f, t)
+            if ctx:
+                accepted.add((f, t))
+                L.pfDeleteContext(ctx)
+    assert accepted == targets
+
+    class FB(C.Structure):
+        _fields_ = [("texture", C.c_void_p), ("zbuffer", C.c_void_p)]
+    L2, ctx, _ = _ctx()
+    L2.pfGenFramebuffer.restype = FB
+    try:
+        for f, t in [(9, 0), (4, 0), (7, 10), (6, 9), (0, 0)]:
+            fb = L2.pfGenFramebuffer(8, 8, f, t)
+            assert not fb.texture and L2.pfGetError() == 1
+        for f, t in sorted(targets):
+            fb = L2.pfGenFramebuffer(8, 8, f, t)
+            assert fb.texture and L2.pfGetError() == 0
+            L2.pfDeleteFramebuffer(C.byref(fb))
+    finally:
+        L2.pfDeleteContext(ctx)
     L2, ctx, _ = _ctx()
     try:
         L2.pfGenTexture.restype = C.c_void_p
@@ -142,7 +169,7 @@ class Frame(C.Structure):
                 ("pixelOps", C.c_void_p), ("numPixelOps", C.c_uint32),
                 ("points", C.c_void_p), ("numPoints", C.c_uint32),
                 ("lines", C.c_void_p), ("numLines", C.c_uint32),
-                ("arrays", C.c_void_p), ("numArrays", C.c_uint32)]
+                ("arrays", C.c_void_p), ("numArrays", C.c_uint32), ("target", C.c_uint32)]
 
 
 def test_recorded_frame_structure():
