@@ -107,6 +107,15 @@ int pfdExecuteFrames(const PFFframe* frames, uint32_t count);
 #define PFD_SCRATCH_BYTES_MIB 128
 int pfdExecuteFrameNoAlloc(const PFFframe* frame);
 
+/* Host memory for the host library's own buffers: recorded triangle
+ * streams, depth buffers and offscreen colour buffers (no reference
+ * counterpart).  Page-locked when a GPU is present, so a frame's uploads
+ * and downloads of them are direct DMA transfers; plain heap memory
+ * otherwise.  pfdHostFree takes only pointers from pfdHostAlloc (NULL is
+ * ignored). */
+void* pfdHostAlloc(size_t bytes);
+void  pfdHostFree(void* p);
+
 /* Device copies of host bytes that outlive frames (no reference
  * counterpart): render lists keep their recorded vertices resident, and
  * frames that call them reference the copy (PFFarrays.device) instead of

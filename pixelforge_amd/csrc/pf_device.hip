@@ -33,6 +33,9 @@
 #include <vector>
 #include <algorithm>
 #include <mutex>
+#include <atomic>
+#include <condition_variable>
+#include <thread>
 
 #include "pf_frame.h"
 #include "pf_device.h"
@@ -3294,6 +3297,8 @@ struct PFDjob {
     hipEvent_t fork = nullptr, join = nullptr;
     Arena* arena = nullptr;               // non-null: buffers come from the arena, never from hipMalloc
     bool autoTiles = false;               // choose tile size and raster mode from the first counts
+    uint64_t autoKey = 0;                 // one-shot job: shape of the last automatically configured
+    uint32_t autoShift = 0, autoMode = 0; // frame and its choice (kept across job_reset)
     std::vector<PFFframe> src;            // host frame descriptors: only the target fields are
                                           // used after creation (the rest may be reused by the caller)
     std::vector<uint8_t>  leadClear;      // per frame: the first command is a clear
@@ -3330,6 +3335,10 @@ struct PFDjob {
     DevBuf arrPool, arrTable;             // array draws: element copies and their DArrays
     uint32_t numVertices = 0;
     int32_t paramTex = -1;
+    // Page-locked staging for copies from and to pageable host memory
+    // (staged_h2d / staged_d2h), made on first use.
+    void* hstage[2] = { nullptr, nullptr };
+    hipEvent_t hstageEv[2] = { nullptr, nullptr };
 };
 
 namespace {
@@ -3383,6 +3392,146 @@ int ensure(PFDjob* J, DevBuf& b, size_t bytes) {
 void release(DevBuf& b) {
     if (b.p && !b.borrowed) (void)hipFree(b.p);
     b = DevBuf();
+}
+
+// ---------------------------------------------------------------------------
+// Copies from and to pageable host memory (the application's colour buffer,
+// its textures).  hipMemcpy stages pageable memory through the runtime's
+// buffers with one thread (about 10 GB/s here); these copies go through two
+// page-locked chunks of the job instead, the host side of each chunk copied
+// by a few threads while the DMA of the neighbouring chunk runs.
+// ---------------------------------------------------------------------------
+
+constexpr size_t HSTAGE_CHUNK = (size_t)2 << 20;
+
+// A few threads that split one memcpy between them and the caller.  Made
+// on first use and kept for the life of the process (never destroyed, so
+// no thread outlives the state it waits on).
+class CopyPool {
+  public:
+    static CopyPool& get() {
+        static CopyPool* pool = new CopyPool();
+        return *pool;
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        if (workers_.empty() || bytes < ((size_t)256 << 10)) { memcpy(dst, src, bytes); return; }
+        std::lock_guard<std::mutex> one(callLock_);          // one parallel copy at a time
+        const size_t parts = workers_.size() + 1;
+        const size_t per = ((bytes + parts - 1) / parts + 63) & ~(size_t)63;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            dst_ = (uint8_t*)dst; src_ = (const uint8_t*)src; bytes_ = bytes; per_ = per;
+            pending_ = workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        part(0);                                              // the caller's own share
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+  private:
+    CopyPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned n = hw >= 16 ? 3u : hw >= 4 ? 1u : 0u;
+        for (unsigned i = 0; i < n; i++) workers_.emplace_back([this, i] { run(i + 1); });
+        for (auto& t : workers_) t.detach();
+    }
+    void part(size_t k) {
+        const size_t b = k * per_;
+        if (b < bytes_) memcpy(dst_ + b, src_ + b, std::min(per_, bytes_ - b));
+    }
+    void run(size_t k) {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            part(k);
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex callLock_, m_;
+    std::condition_variable cv_, done_;
+    uint8_t* dst_ = nullptr;
+    const uint8_t* src_ = nullptr;
+    size_t bytes_ = 0, per_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+};
+
+// True when `p` is page-locked host memory (a direct DMA source or target).
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t a;
+    memset(&a, 0, sizeof a);
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeHost;
+}
+
+// The job's staging chunks; false when they cannot be made (or the job may
+// not allocate), and the caller copies directly.
+bool hstage_ready(PFDjob* J) {
+    if (J->arena) return false;                              // pfdExecuteFrameNoAlloc: no allocation
+    if (J->hstage[0]) return true;
+    for (int i = 0; i < 2; i++) {
+        if (hipHostMalloc(&J->hstage[i], HSTAGE_CHUNK, hipHostMallocDefault) != hipSuccess
+            || hipEventCreateWithFlags(&J->hstageEv[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (int k = 0; k < 2; k++) {
+                if (J->hstage[k]) (void)hipHostFree(J->hstage[k]);
+                if (J->hstageEv[k]) (void)hipEventDestroy(J->hstageEv[k]);
+                J->hstage[k] = nullptr;
+                J->hstageEv[k] = nullptr;
+            }
+            return false;
+        }
+    }
+    return true;
+}
+
+// Host -> device on the job's stream; returns with the host memory read.
+hipError_t staged_h2d(PFDjob* J, void* dev, const void* host, size_t bytes) {
+    if (bytes < HSTAGE_CHUNK || host_pinned(host) || !hstage_ready(J)) {
+        hipError_t e = hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, J->stream);
+        return e == hipSuccess ? hipStreamSynchronize(J->stream) : e;
+    }
+    hipError_t e = hipSuccess;
+    for (size_t off = 0, i = 0; e == hipSuccess && off < bytes; off += HSTAGE_CHUNK, i++) {
+        const size_t n = std::min(HSTAGE_CHUNK, bytes - off), b = i & 1;
+        if (i >= 2) e = hipEventSynchronize(J->hstageEv[b]);   // this chunk's last DMA is done
+        if (e != hipSuccess) break;
+        CopyPool::get().copy(J->hstage[b], (const uint8_t*)host + off, n);
+        e = hipMemcpyAsync((uint8_t*)dev + off, J->hstage[b], n, hipMemcpyHostToDevice, J->stream);
+        if (e == hipSuccess) e = hipEventRecord(J->hstageEv[b], J->stream);
+    }
+    return e == hipSuccess ? hipStreamSynchronize(J->stream) : e;
+}
+
+// Device -> host after the work queued on the job's stream; returns with
+// the host memory written.
+hipError_t staged_d2h(PFDjob* J, void* host, const void* dev, size_t bytes) {
+    if (bytes < HSTAGE_CHUNK || host_pinned(host) || !hstage_ready(J)) {
+        hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, J->stream);
+        return e == hipSuccess ? hipStreamSynchronize(J->stream) : e;
+    }
+    const size_t chunks = (bytes + HSTAGE_CHUNK - 1) / HSTAGE_CHUNK;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t off = i * HSTAGE_CHUNK, n = std::min(HSTAGE_CHUNK, bytes - off);
+        hipError_t ee = hipMemcpyAsync(J->hstage[i & 1], (const uint8_t*)dev + off, n, hipMemcpyDeviceToHost, J->stream);
+        return ee == hipSuccess ? hipEventRecord(J->hstageEv[i & 1], J->stream) : ee;
+    };
+    hipError_t e = issue(0);
+    for (size_t i = 0; e == hipSuccess && i < chunks; i++) {
+        if (i + 1 < chunks) e = issue(i + 1);                  // its chunk's host copy (i - 1) is done
+        if (e == hipSuccess) e = hipEventSynchronize(J->hstageEv[i & 1]);
+        if (e != hipSuccess) break;
+        const size_t off = i * HSTAGE_CHUNK;
+        CopyPool::get().copy((uint8_t*)host + off, J->hstage[i & 1], std::min(HSTAGE_CHUNK, bytes - off));
+    }
+    return e;
 }
 
 uint32_t read_total(PFDjob* J, int* rc) {
@@ -4014,7 +4163,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 // Raw texels + 4 zero bytes (the getters' word reads run up
                 // to 3 bytes past the last texel), decoded to RGBA8.
                 const size_t rawBytes = texels * pff_source_texel_bytes(T.source);
-                e = hipMemcpy(J->stage.p, T.pixels, rawBytes, hipMemcpyHostToDevice);
+                e = staged_h2d(J, J->stage.p, T.pixels, rawBytes);
                 if (e == hipSuccess) e = hipMemsetAsync((uint8_t*)J->stage.p + rawBytes, 0, 4, J->stream);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL(k_decode_texels, dim3((uint32_t)((texels + 255) / 256)), dim3(256), 0, J->stream,
@@ -4024,7 +4173,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
                 if (e == hipSuccess) e = hipStreamSynchronize(J->stream);   // the stage is reused
                 if (e != hipSuccess) { set_error("texture decode", e); return PFD_E_LAUNCH; }
             } else if (bytes) {
-                e = hipMemcpy(dst, T.pixels, bytes, hipMemcpyHostToDevice);
+                e = staged_h2d(J, dst, T.pixels, bytes);
                 if (e != hipSuccess) { set_error("texture upload", e); return PFD_E_LAUNCH; }
             }
             if (T.format == PFF_TEX_SOURCE) d.format = PFF_TEX_RGBA8;
@@ -4198,11 +4347,15 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
             || (rc = ensure(J, J->uv, T.texcoord ? n * 24 + 4 : 4)) || (rc = ensure(J, J->nrm, T.normal ? n * 36 + 4 : 4))
             || (vidOn && (rc = ensure(J, J->vid, n * 12 + 4))))
             return rc;
+        // Asynchronous on the job's stream: the host library records into
+        // page-locked memory, so these are DMA transfers queued back to back
+        // (no round trip each); the stream is synchronised at the end of
+        // this function, before the caller may touch the streams again.
         for_host_ranges(fr, [&](uint32_t b, uint32_t e2) {
             auto part = [&](DevBuf& d, const void* h, size_t per) {
                 if (e == hipSuccess && h)
-                    e = hipMemcpy((uint8_t*)d.p + b * per, (const uint8_t*)h + b * per, (size_t)(e2 - b) * per,
-                                  hipMemcpyHostToDevice);
+                    e = hipMemcpyAsync((uint8_t*)d.p + b * per, (const uint8_t*)h + b * per, (size_t)(e2 - b) * per,
+                                       hipMemcpyHostToDevice, J->stream);
             };
             part(J->pos, T.position, 48);
             part(J->col, T.color, 12);
@@ -4403,7 +4556,11 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         }
     }
     layout_tiles(J);
-    return upload_frames_table(J);
+    if ((rc = upload_frames_table(J))) return rc;
+    // The caller may change its streams once this returns: the asynchronous
+    // uploads above must have read them.
+    HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
+    return PFD_OK;
 }
 
 #undef PLAN_FAIL
@@ -4451,9 +4608,8 @@ int pfdJobUploadTargets(PFDjob* J) {
         size_t npx = (size_t)fr.width * fr.height;
         bool leadClear = J->leadClear[f] != 0;
         size_t n = leadClear ? (npx < 8 ? npx : 8) : npx;
-        HIP_TRY(hipMemcpyAsync(J->frames[f].color, fr.color, n * pff_target_bytes(fr.target), hipMemcpyHostToDevice,
-                               J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipMemcpyAsync(J->frames[f].depth, fr.depth, n * 4, hipMemcpyHostToDevice, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(staged_h2d(J, J->frames[f].color, fr.color, n * pff_target_bytes(fr.target)), PFD_E_LAUNCH);
     }
     HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
     return PFD_OK;
@@ -4492,9 +4648,10 @@ int pfdJobDownload(PFDjob* J) {
     for (size_t f = 0; f < J->frames.size(); f++) {
         const PFFframe& fr = J->src[f];
         size_t npx = (size_t)fr.width * fr.height;
-        HIP_TRY(hipMemcpyAsync(fr.color, J->frames[f].color, npx * pff_target_bytes(fr.target), hipMemcpyDeviceToHost,
-                               J->stream), PFD_E_LAUNCH);
+        // Depth is the library's page-locked buffer: queued first, it
+        // overlaps the staged copy of an application's pageable colour.
         HIP_TRY(hipMemcpyAsync(fr.depth, J->frames[f].depth, npx * 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(staged_d2h(J, fr.color, J->frames[f].color, npx * pff_target_bytes(fr.target)), PFD_E_LAUNCH);
     }
     HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
     return PFD_OK;
@@ -4627,6 +4784,41 @@ int pfdJobAutotune(PFDjob* J, PFDconfig* chosen) {
     return PFD_OK;
 }
 
+// Host buffers of the host library.  A 64-byte header in front of the
+// caller's bytes records how they were allocated, so pfdHostFree needs no
+// table and the caller's pointer keeps 64-byte alignment.
+namespace {
+constexpr uint64_t HOST_MAGIC = 0x50464448u;   // "PFDH"
+struct HostHdr {
+    uint64_t magic;
+    uint64_t pinned;
+    uint64_t pad[6];
+};
+static_assert(sizeof(HostHdr) == 64, "host header");
+}  // namespace
+
+void* pfdHostAlloc(size_t bytes) {
+    void* base = nullptr;
+    uint64_t pinned = 0;
+    if (pfdInit() == PFD_OK && hipHostMalloc(&base, bytes + sizeof(HostHdr), hipHostMallocDefault) == hipSuccess)
+        pinned = 1;
+    else if (!(base = malloc(bytes + sizeof(HostHdr))))
+        return nullptr;
+    HostHdr* h = static_cast<HostHdr*>(base);
+    h->magic = HOST_MAGIC;
+    h->pinned = pinned;
+    return h + 1;
+}
+
+void pfdHostFree(void* p) {
+    if (!p) return;
+    HostHdr* h = static_cast<HostHdr*>(p) - 1;
+    if (h->magic != HOST_MAGIC) return;            // not ours: left alone rather than corrupting the heap
+    h->magic = 0;
+    if (h->pinned) (void)hipHostFree(h);
+    else free(h);
+}
+
 struct PFDresident {
     void*  p;
     size_t bytes;
@@ -4665,6 +4857,10 @@ void pfdJobDestroy(PFDjob* J) {
     if (J->side) (void)hipStreamSynchronize(J->side);
     drop_graph(J);
     for_each_buf(J, [](DevBuf& b) { release(b); });
+    for (int i = 0; i < 2; i++) {
+        if (J->hstage[i]) (void)hipHostFree(J->hstage[i]);
+        if (J->hstageEv[i]) (void)hipEventDestroy(J->hstageEv[i]);
+    }
     if (J->fork) (void)hipEventDestroy(J->fork);
     if (J->join) (void)hipEventDestroy(J->join);
     if (J->side) (void)hipStreamDestroy(J->side);
@@ -4678,13 +4874,40 @@ namespace {
 
 // Plans and runs frames once on a reset job: upload, one discovering run
 // (no graph), download.
+// Shape of a set of frames for reusing an automatic configuration: a frame
+// of the same targets, triangle and command counts as the last one is
+// taken to want the same tiles (a performance choice only).
+uint64_t frame_shape(const PFFframe* frames, uint32_t count) {
+    uint64_t h = 1469598103934665603ull ^ count;
+    auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+    for (uint32_t f = 0; f < count; f++) {
+        mix(frames[f].width); mix(frames[f].height); mix(frames[f].tris.count);
+        mix(frames[f].numCmds); mix(frames[f].numArrays); mix(frames[f].target);
+    }
+    return h | 1u;
+}
+
 int run_one_shot(PFDjob* J, const PFFframe* frames, uint32_t count) {
     J->cfg = g_defaultCfg;
     J->tileShift = (J->cfg.tileShift >= 3 && J->cfg.tileShift <= 6) ? J->cfg.tileShift : 3;
     J->autoTiles = J->cfg.tileShift == 0;
+    const uint64_t shape = J->autoTiles ? frame_shape(frames, count) : 0;
+    if (J->autoTiles && shape == J->autoKey) {
+        // The previous frame had this shape: its choice, without counting
+        // the first pass twice.
+        J->autoTiles = false;
+        J->tileShift = J->autoShift;
+        J->cfg.rasterMode = J->autoMode;
+    }
+    const bool choosing = J->autoTiles;
     int rc = job_plan(J, frames, count, false);
     if (!rc && !(rc = pfdJobUploadTargets(J)) && !(rc = run_all(J, true)) && !(rc = pfdJobSync(J)))
         rc = pfdJobDownload(J);
+    if (!rc && choosing) {
+        J->autoKey = shape;
+        J->autoShift = J->tileShift;
+        J->autoMode = J->cfg.rasterMode;
+    }
     return rc;
 }
 

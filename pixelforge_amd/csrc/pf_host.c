@@ -41,6 +41,7 @@ typedef struct {                       /* struct PFItex, context.h:160-176 */
     PFdatatype      type;
     PFtexturewrap   wrap;
     PFtexturefilter filter;
+    int             hostPixels;        /* pixels from pfdHostAlloc (pfGenFramebuffer) */
 } Tex;
 
 typedef struct {                       /* PFIlight, context.h:216-229 */
@@ -354,7 +355,8 @@ static void rec_free(Rec* r)
     free(r->pix);
     free(r->pts);
     free(r->lns);
-    free(r->pos); free(r->col); free(r->uv); free(r->nrm); free(r->info); free(r->vid);
+    pfdHostFree(r->pos); pfdHostFree(r->col); pfdHostFree(r->uv); pfdHostFree(r->nrm);
+    pfdHostFree(r->info); pfdHostFree(r->vid);
     free(r->states); free(r->texs); free(r->texKeys); free(r->cmds);
     memset(r, 0, sizeof *r);
 }
@@ -541,24 +543,32 @@ static void push_cmd(Ctx* c, const PFFcmd* cmd)
     r->cmds[r->ncmd++] = *cmd;
 }
 
-/* Grows the triangle streams to hold at least `need` triangles. */
+/* realloc for pfdHostAlloc memory: *p grows from `used` to `bytes`. */
+static int host_grow(void** p, size_t used, size_t bytes)
+{
+    void* q = pfdHostAlloc(bytes);
+    if (!q) return 0;
+    if (*p && used) memcpy(q, *p, used);
+    pfdHostFree(*p);
+    *p = q;
+    return 1;
+}
+
+/* Grows the triangle streams to hold at least `need` triangles.  They live
+ * in pfdHostAlloc memory (page-locked with a GPU), from which a frame's
+ * upload is a direct DMA transfer. */
 static int rec_grow(Rec* r, size_t need)
 {
     size_t nc = r->captri ? r->captri : 1024;
     while (nc < need) nc *= 2;
-    float* pos = realloc(r->pos, nc * 12 * sizeof(float));
-    if (pos) r->pos = pos;
-    uint32_t* col = realloc(r->col, nc * 3 * sizeof(uint32_t));
-    if (col) r->col = col;
-    float* uv = realloc(r->uv, nc * 6 * sizeof(float));
-    if (uv) r->uv = uv;
-    float* nrm = realloc(r->nrm, nc * 9 * sizeof(float));
-    if (nrm) r->nrm = nrm;
-    uint32_t* info = realloc(r->info, nc * sizeof(uint32_t));
-    if (info) r->info = info;
-    uint32_t* vid = realloc(r->vid, nc * 3 * sizeof(uint32_t));
-    if (vid) r->vid = vid;
-    if (!pos || !col || !uv || !nrm || !info || !vid) return 0;
+    const size_t old = r->captri;
+    int ok = host_grow((void**)&r->pos, old * 48, nc * 48)
+          && host_grow((void**)&r->col, old * 12, nc * 12)
+          && host_grow((void**)&r->uv, old * 24, nc * 24)
+          && host_grow((void**)&r->nrm, old * 36, nc * 36)
+          && host_grow((void**)&r->info, old * 4, nc * 4)
+          && host_grow((void**)&r->vid, old * 12, nc * 12);
+    if (!ok) return 0;
     r->captri = nc;
     return 1;
 }
@@ -836,7 +846,10 @@ void pfDeleteTexture(PFtexture* texture, PFboolean freeBuffer)
         if (G->curTex == t) G->curTex = NULL;
         if (G->rec.ncmd) submit(G);      /* recorded draws may read it */
     }
-    if (freeBuffer && t->pixels) free(t->pixels);
+    if (freeBuffer && t->pixels) {
+        if (t->hostPixels) pfdHostFree(t->pixels);
+        else free(t->pixels);
+    }
     free(t);
     *texture = NULL;
 }
@@ -880,9 +893,9 @@ PFcontext pfCreateContext(void* targetBuffer, PFsizei width, PFsizei height, PFp
     if (!c) return NULL;
     size_t n = (size_t)width * height;
     c->mainFB.texture = pfGenTexture(targetBuffer, width, height, format, type);
-    c->mainFB.zbuffer = (float*)malloc((n ? n : 1) * sizeof(float));
+    c->mainFB.zbuffer = (float*)pfdHostAlloc((n ? n : 1) * sizeof(float));
     if (!c->mainFB.texture || !c->mainFB.zbuffer) {
-        free(c->mainFB.texture); free(c->mainFB.zbuffer); free(c);
+        free(c->mainFB.texture); pfdHostFree(c->mainFB.zbuffer); free(c);
         return NULL;
     }
     for (size_t i = 0; i < n; i++) c->mainFB.zbuffer[i] = FLT_MAX;    /* context.c:136-138 */
@@ -954,7 +967,7 @@ void pfDeleteContext(PFcontext ctx)
     submit(c);
     pfdJobDestroy(c->diffJob);
     free(c->diffTex);
-    free(c->mainFB.zbuffer);
+    pfdHostFree(c->mainFB.zbuffer);
     free(c->mainFB.texture);
     rec_free(&c->rec);
     if (G == c) G = NULL;
@@ -973,7 +986,7 @@ void pfSetMainBuffer(void* targetBuffer, PFsizei width, PFsizei height, PFpixelf
     if (oldW != width || oldH != height) {
         /* context.c:279-312, with the depth buffer sized in floats (the
          * reference reallocates it to width*height bytes). */
-        float* z = (float*)malloc((size_t)width * height * sizeof(float));
+        float* z = (float*)pfdHostAlloc((size_t)width * height * sizeof(float));
         if (!z) { c->err = PF_ERROR_OUT_OF_MEMORY; return; }
         for (uint32_t y = 0; y < height; y++)
             for (uint32_t x = 0; x < width; x++) {
@@ -982,7 +995,7 @@ void pfSetMainBuffer(void* targetBuffer, PFsizei width, PFsizei height, PFpixelf
                 z[i] = (oi < (size_t)oldW * oldH) ? c->mainFB.zbuffer[oi] : c->clearDepth;
                 if (x >= oldW || y >= oldH) z[i] = c->clearDepth;
             }
-        free(c->mainFB.zbuffer);
+        pfdHostFree(c->mainFB.zbuffer);
         c->mainFB.zbuffer = z;
     }
     old->pixels = targetBuffer;
@@ -2696,11 +2709,14 @@ PFframebuffer pfGenFramebuffer(PFsizei width, PFsizei height, PFpixelformat form
     const uint32_t kind = pff_target_of(PFF_SOURCE(format, type));
     if (kind == PFF_TARGET_NONE) { set_err(PF_INVALID_ENUM); return fb; }
     size_t n = (size_t)width * height;
-    void* px = calloc(n ? n : 1, pff_target_bytes(kind));
+    const size_t bytes = (n ? n : 1) * pff_target_bytes(kind);
+    void* px = pfdHostAlloc(bytes);                 /* page-locked with a GPU: DMA downloads */
     if (!px) return fb;
+    memset(px, 0, bytes);
     PFtexture t = pfGenTexture(px, width, height, format, type);
-    if (!t) { free(px); set_err(PF_INVALID_ENUM); return fb; }
-    float* z = (float*)malloc((n ? n : 1) * sizeof(float));
+    if (!t) { pfdHostFree(px); set_err(PF_INVALID_ENUM); return fb; }
+    ((Tex*)t)->hostPixels = 1;
+    float* z = (float*)pfdHostAlloc((n ? n : 1) * sizeof(float));
     if (!z) { set_err(PF_ERROR_OUT_OF_MEMORY); pfDeleteTexture(&t, PF_TRUE); return fb; }
     for (size_t i = 0; i < n; i++) z[i] = FLT_MAX;
     fb.texture = t;
@@ -2716,7 +2732,7 @@ void pfDeleteFramebuffer(PFframebuffer* fb)
     if (G && G->boundFB == fb) G->boundFB = NULL;
     if (G && G->curFB == fb) G->curFB = &G->mainFB;
     pfDeleteTexture(&fb->texture, PF_TRUE);
-    free(fb->zbuffer);
+    pfdHostFree(fb->zbuffer);
     fb->zbuffer = NULL;
     fb->texture = NULL;
 }
