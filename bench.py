@@ -195,8 +195,10 @@ def end_to_end(params, frames, expect):
     copies colour and depth back into the application's buffers.  Replayed
     from a call tape, so the scene generator is not timed.  The first frame
     is a warm-up.  The host part alone is timed with an executor that
-    discards the frame.  The last frame must equal `expect` (the resident
-    job's image)."""
+    discards the frame, and the application's own loop (decoding the tape,
+    calling through function pointers) by replaying the tape through no-op
+    functions: library_ms is the frame time less that loop.  The last frame
+    must equal `expect` (the resident job's image)."""
     tape = pf.Tape(params)
     try:
         img, ms, err = tape.time(frames + 1)
@@ -212,8 +214,12 @@ def end_to_end(params, frames, expect):
         finally:
             lib.pfxSetExecutor(None, None)
         check(err, "recording")
-        return {"ms_per_frame": float(np.median(ms[1:])), "frames": frames,
+        loop_ms = tape.time_loop(frames + 1)
+        e2e_ms = float(np.median(ms[1:]))
+        app_ms = float(np.median(loop_ms[1:]))
+        return {"ms_per_frame": e2e_ms, "frames": frames,
                 "host_record_ms": float(np.median(rec_ms[1:])), "api_calls_per_frame": tape.calls,
+                "app_loop_ms": app_ms, "library_ms": e2e_ms - app_ms,
                 "equals_resident_job": True}
     finally:
         tape.close()

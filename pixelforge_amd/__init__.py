@@ -164,6 +164,8 @@ def scenes():
         S.pfs_tape_calls.restype = C.c_uint64
         S.pfs_tape_time.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
         S.pfs_tape_time.restype = C.c_int
+        S.pfs_tape_time_loop.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        S.pfs_tape_time_loop.restype = C.c_int
         _scenes = S
     return _scenes
 
@@ -233,6 +235,13 @@ class Tape:
         ms = np.zeros(frames, np.float64)
         err = scenes().pfs_tape_time(api(path), self.ptr, frames, color.ctypes.data, ms.ctypes.data)
         return color[: h * w].reshape(h, w), ms, err
+
+    def time_loop(self, frames):
+        """Per-frame milliseconds of replaying the frame's calls through
+        no-op functions: the application loop's own share of time()."""
+        ms = np.zeros(frames, np.float64)
+        scenes().pfs_tape_time_loop(self.ptr, frames, ms.ctypes.data)
+        return ms
 
     def close(self):
         if self.ptr:

@@ -425,11 +425,15 @@ static int ensure_target(Ctx* c)
     Tex* t = fb_tex(c->curFB);
     if (!t || !t->pixels || !c->curFB->zbuffer) return 0;
     Rec* r = &c->rec;
+    /* The frame being recorded already targets these buffers (a format
+     * change of the same buffers goes through pfSetMainBuffer, which
+     * submits the frame first). */
+    if (r->ncmd && r->targetColor == t->pixels && r->targetDepth == c->curFB->zbuffer
+        && r->targetW == t->w && r->targetH == t->h)
+        return 1;
     const uint32_t kind = pff_target_of(PFF_SOURCE(t->format, t->type));
     if (kind == PFF_TARGET_NONE) return 0;
-    if (r->ncmd && (r->targetColor != t->pixels || r->targetDepth != c->curFB->zbuffer
-                    || r->targetW != t->w || r->targetH != t->h || r->targetKind != kind))
-        submit(c);
+    if (r->ncmd) submit(c);
     if (r->ncmd == 0) {
         r->targetColor = t->pixels;
         r->targetDepth = c->curFB->zbuffer;

@@ -1257,6 +1257,61 @@ static double now_ms(void)
     return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
 }
 
+/* No-op stand-ins for every call a tape makes: replaying a tape through
+ * them times the application's own loop (decoding the tape and calling
+ * through function pointers) without any library work. */
+#define NOOP __attribute__((noinline, noipa))
+static NOOP void n_st(PFstate s) { (void)s; }
+static NOOP void n_face(PFface f) { (void)f; }
+static NOOP void n_dfunc(PFdepthmode m) { (void)m; }
+static NOOP void n_shade(PFshademode m) { (void)m; }
+static NOOP void n_blend(PFblendmode m) { (void)m; }
+static NOOP void n_mat(PFmatrixmode m) { (void)m; }
+static NOOP void n_clear(PFclearflag f) { (void)f; }
+static NOOP void n_begin(PFdrawmode m) { (void)m; }
+static NOOP void n_lmodel(PFlightmode m) { (void)m; }
+static NOOP void n_light(PFsizei l) { (void)l; }
+static NOOP void n_void(void) { }
+static NOOP void n_f3(PFfloat x, PFfloat y, PFfloat z) { (void)x; (void)y; (void)z; }
+static NOOP void n_f4(PFfloat x, PFfloat y, PFfloat z, PFfloat w) { (void)x; (void)y; (void)z; (void)w; }
+static NOOP void n_f6(PFfloat a, PFfloat b, PFfloat c, PFfloat d, PFfloat e, PFfloat f)
+{ (void)a; (void)b; (void)c; (void)d; (void)e; (void)f; }
+static NOOP void n_f2(PFfloat u, PFfloat v) { (void)u; (void)v; }
+static NOOP void n_ub4(PFubyte r, PFubyte g, PFubyte b, PFubyte a) { (void)r; (void)g; (void)b; (void)a; }
+static NOOP void n_lfv(PFsizei l, PFenum p, const void* v) { (void)l; (void)p; (void)v; }
+static NOOP void n_lf(PFsizei l, PFenum p, PFfloat v) { (void)l; (void)p; (void)v; }
+static NOOP void n_mfv(PFface f, PFenum p, const void* v) { (void)f; (void)p; (void)v; }
+static NOOP void n_mf(PFface f, PFenum p, PFfloat v) { (void)f; (void)p; (void)v; }
+static NOOP void n_cmat(PFface f, PFenum m) { (void)f; (void)m; }
+static NOOP void n_tex(PFtexture t) { (void)t; }
+static NOOP void n_texp(PFtexture t, PFtexturewrap w, PFtexturefilter f) { (void)t; (void)w; (void)f; }
+#undef NOOP
+
+/* Wall time per frame (ms[i]) of replaying the tape's frame calls through
+ * the no-op stand-ins: the part of pfs_tape_time's frame time that is the
+ * application's loop rather than the library. */
+int pfs_tape_time_loop(PFStape* t, uint32_t frames, double* ms)
+{
+    if (!t) return -1;
+    PFSapi a;
+    memset(&a, 0, sizeof a);
+    a.Enable = n_st; a.Disable = n_st; a.CullFace = n_face; a.DepthFunc = n_dfunc; a.ShadeModel = n_shade;
+    a.BlendFunc = n_blend; a.MatrixMode = n_mat; a.Clear = n_clear; a.Begin = n_begin; a.LightModel = n_lmodel;
+    a.EnableLight = n_light; a.DisableLight = n_light; a.LoadIdentity = n_void; a.End = n_void;
+    a.Translatef = n_f3; a.Vertex3f = n_f3; a.Normal3f = n_f3; a.Rotatef = n_f4; a.Frustum = n_f6;
+    a.TexCoord2f = n_f2; a.Color4ub = n_ub4; a.ClearColor = n_ub4; a.Lightfv = n_lfv; a.Lightf = n_lf;
+    a.Materialfv = n_mfv; a.Materialf = n_mf; a.ColorMaterial = n_cmat; a.BindTexture = n_tex;
+    a.TextureParameter = n_texp;
+    const PFSapi* volatile ap = &a;           /* the calls stay indirect, as through a loaded library */
+    for (uint32_t f = 0; f < frames; f++) {
+        double t0 = now_ms();
+        tape_play(ap, t, 0, t->body);
+        if (ms) ms[f] = now_ms() - t0;
+        tape_play(ap, t, t->body, t->n);
+    }
+    return 0;
+}
+
 /* Renders the tape's scene `frames` times as an application would: one
  * context of library `a` over the caller's colour buffer (W*H+16 pixels),
  * the tape's textures created in the library once, then per frame the
