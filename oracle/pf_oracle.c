@@ -831,6 +831,11 @@ typedef struct {
     float    z[3], uv[3][2], nrm[3][3], pos[3][3], bw[3][3];
 } DRec;
 
+/* Per recorded triangle covered samples, for measuring a workload's
+ * triangle screen areas (pforacle_set_coverage); NULL: not counted. */
+static uint32_t* g_cov;
+static uint32_t  g_covN;
+
 typedef struct {
     const PFFframe* f;
     uint64_t fragments;      /* covered samples, before the depth test */
@@ -918,6 +923,7 @@ static void raster_tri(RCtx* rc, const PFFstate* st, int face, int is3D,
             if ((w1 | w2 | w3) <= 0) continue;
             if (x < 0 || y < 0 || x >= W || y >= H) continue;   /* D5 */
             rc->fragments++;
+            if (g_cov && rc->src < g_covN) g_cov[rc->src]++;
             size_t off = (size_t)y * (size_t)W + (size_t)x;
             float b1 = (float)w1 * wInvSum, b2 = (float)w2 * wInvSum, b3 = (float)w3 * wInvSum;
             float zq = rcp_((v1->hom[2] * b1 + v2->hom[2] * b2) + v3->hom[2] * b3);
@@ -1804,6 +1810,15 @@ void pforacle_expand_free(void) { expanded_free(&g_expanded); }
  * test harness can run the host library's recorded frames through this
  * oracle.  `user`, when not NULL, points to a uint64_t that accumulates the
  * fragment count. */
+/* Frames rendered after this add, per recorded triangle t < n, the samples
+ * it covers (both culling passes, clipped fans summed) to counts[t];
+ * counts NULL stops counting. */
+void pforacle_set_coverage(uint32_t* counts, uint32_t n)
+{
+    g_cov = counts;
+    g_covN = counts ? n : 0;
+}
+
 int pforacle_executor(const PFFframe* f, void* user)
 {
     uint64_t frags = 0;
