@@ -252,11 +252,15 @@ typedef struct {
 /* Pixel formats of a render target's colour buffer (PFFframe.target): the
  * (format, type) pairs whose reference accessors read and write only their
  * own pixel -- the SIMD ones of the triangle raster and pfClear
- * (pixel.h:1492-2067, 2536-2913) and the scalar ones of points, lines,
- * pfDrawPixels, rects and fog (pixel.h:206-400, 549-707).  BGRA /
- * UNSIGNED_BYTE (its SIMD accessors copy one pixel's bytes into its
- * neighbours, pixel.h:2070-2078), the single-channel, luminance and
- * half/float pairs are not render targets here. */
+ * (pixel.h:1013-1090, 1492-2250, 2328-2383, 2536-3040) and the scalar ones
+ * of points, lines, pfDrawPixels, rects and fog (pixel.h:78-400,
+ * 405-760).  BGRA / UNSIGNED_BYTE (its SIMD accessors copy one pixel's
+ * bytes into its neighbours, pixel.h:2070-2078, 2916-2920), the
+ * single-channel and luminance pairs (their SIMD getters read a
+ * neighbour's byte, or their SIMD setters rewrite unmasked neighbours
+ * through 8 bits, pixel.h:873-965, 1092-1487, 2253-2530) are not render
+ * targets here.  Kinds from PFF_TARGET_LA_HALF on are "wide": more than 4
+ * bytes, or not held as one stored word by the raster. */
 #define PFF_TARGET_RGBA8     0u     /* RGBA / UNSIGNED_BYTE           */
 #define PFF_TARGET_RGB8      1u     /* RGB / UNSIGNED_BYTE (3 bytes)  */
 #define PFF_TARGET_BGR8      2u     /* BGR / UNSIGNED_BYTE (3 bytes)  */
@@ -266,6 +270,17 @@ typedef struct {
 #define PFF_TARGET_BGRA5551  6u     /* BGRA / UNSIGNED_SHORT_5_5_5_1  */
 #define PFF_TARGET_RGBA4444  7u     /* RGBA / UNSIGNED_SHORT_4_4_4_4  */
 #define PFF_TARGET_BGRA4444  8u     /* BGRA / UNSIGNED_SHORT_4_4_4_4  */
+#define PFF_TARGET_LA_HALF   9u     /* LUMINANCE_ALPHA / HALF_FLOAT   */
+#define PFF_TARGET_LA_FLOAT  10u    /* LUMINANCE_ALPHA / FLOAT        */
+#define PFF_TARGET_RGB_HALF  11u    /* RGB / HALF_FLOAT               */
+#define PFF_TARGET_BGR_HALF  12u    /* BGR / HALF_FLOAT               */
+#define PFF_TARGET_RGB_FLOAT 13u    /* RGB / FLOAT                    */
+#define PFF_TARGET_BGR_FLOAT 14u    /* BGR / FLOAT                    */
+#define PFF_TARGET_RGBA_HALF 15u    /* RGBA / HALF_FLOAT              */
+#define PFF_TARGET_BGRA_HALF 16u    /* BGRA / HALF_FLOAT              */
+#define PFF_TARGET_RGBA_FLOAT 17u   /* RGBA / FLOAT                   */
+#define PFF_TARGET_BGRA_FLOAT 18u   /* BGRA / FLOAT                   */
+#define PFF_TARGET_FIRST_WIDE PFF_TARGET_LA_HALF
 #define PFF_TARGET_NONE      0xFFFFFFFFu
 
 /* The target of a PFF_SOURCE pair, or PFF_TARGET_NONE. */
@@ -281,14 +296,35 @@ static inline uint32_t pff_target_of(uint32_t source)
     case PFF_SOURCE(9, 3): return PFF_TARGET_BGRA5551;
     case PFF_SOURCE(7, 4): return PFF_TARGET_RGBA4444;
     case PFF_SOURCE(9, 4): return PFF_TARGET_BGRA4444;
+    case PFF_SOURCE(5, 9): return PFF_TARGET_LA_HALF;
+    case PFF_SOURCE(5, 10): return PFF_TARGET_LA_FLOAT;
+    case PFF_SOURCE(6, 9): return PFF_TARGET_RGB_HALF;
+    case PFF_SOURCE(8, 9): return PFF_TARGET_BGR_HALF;
+    case PFF_SOURCE(6, 10): return PFF_TARGET_RGB_FLOAT;
+    case PFF_SOURCE(8, 10): return PFF_TARGET_BGR_FLOAT;
+    case PFF_SOURCE(7, 9): return PFF_TARGET_RGBA_HALF;
+    case PFF_SOURCE(9, 9): return PFF_TARGET_BGRA_HALF;
+    case PFF_SOURCE(7, 10): return PFF_TARGET_RGBA_FLOAT;
+    case PFF_SOURCE(9, 10): return PFF_TARGET_BGRA_FLOAT;
     default: return PFF_TARGET_NONE;
     }
+}
+
+/* PFF_SOURCE pair of a target (PFF_TARGET_NONE for an unknown one). */
+static inline uint32_t pff_target_source(uint32_t target)
+{
+    static const uint32_t src[19] = {
+        PFF_SOURCE(7, 0), PFF_SOURCE(6, 0), PFF_SOURCE(8, 0), PFF_SOURCE(6, 2), PFF_SOURCE(8, 2),
+        PFF_SOURCE(7, 3), PFF_SOURCE(9, 3), PFF_SOURCE(7, 4), PFF_SOURCE(9, 4), PFF_SOURCE(5, 9),
+        PFF_SOURCE(5, 10), PFF_SOURCE(6, 9), PFF_SOURCE(8, 9), PFF_SOURCE(6, 10), PFF_SOURCE(8, 10),
+        PFF_SOURCE(7, 9), PFF_SOURCE(9, 9), PFF_SOURCE(7, 10), PFF_SOURCE(9, 10) };
+    return target < 19u ? src[target] : PFF_TARGET_NONE;
 }
 
 /* Bytes per pixel of a target (0 for an unknown one). */
 static inline uint32_t pff_target_bytes(uint32_t target)
 {
-    return target == PFF_TARGET_RGBA8 ? 4u : target <= PFF_TARGET_BGR8 ? 3u : target <= PFF_TARGET_BGRA4444 ? 2u : 0u;
+    return target < 19u ? pff_source_texel_bytes(pff_target_source(target)) : 0u;
 }
 
 typedef struct {

@@ -1121,11 +1121,51 @@ static uint32_t scalar_texel(const uint8_t* p, uint32_t source, size_t i)
 
 static uint32_t target_source(uint32_t target)
 {
-    static const uint32_t src[9] = {
+    static const uint32_t src[PFF_TARGET_BGRA_FLOAT + 1] = {
         PFF_SOURCE(SRC_RGBA, SRC_UBYTE), PFF_SOURCE(SRC_RGB, SRC_UBYTE), PFF_SOURCE(SRC_BGR, SRC_UBYTE),
         PFF_SOURCE(SRC_RGB, SRC_565), PFF_SOURCE(SRC_BGR, SRC_565), PFF_SOURCE(SRC_RGBA, SRC_5551),
-        PFF_SOURCE(SRC_BGRA, SRC_5551), PFF_SOURCE(SRC_RGBA, SRC_4444), PFF_SOURCE(SRC_BGRA, SRC_4444) };
+        PFF_SOURCE(SRC_BGRA, SRC_5551), PFF_SOURCE(SRC_RGBA, SRC_4444), PFF_SOURCE(SRC_BGRA, SRC_4444),
+        PFF_SOURCE(SRC_LUMINANCE_ALPHA, SRC_HALF), PFF_SOURCE(SRC_LUMINANCE_ALPHA, SRC_FLOAT),
+        PFF_SOURCE(SRC_RGB, SRC_HALF), PFF_SOURCE(SRC_BGR, SRC_HALF),
+        PFF_SOURCE(SRC_RGB, SRC_FLOAT), PFF_SOURCE(SRC_BGR, SRC_FLOAT),
+        PFF_SOURCE(SRC_RGBA, SRC_HALF), PFF_SOURCE(SRC_BGRA, SRC_HALF),
+        PFF_SOURCE(SRC_RGBA, SRC_FLOAT), PFF_SOURCE(SRC_BGRA, SRC_FLOAT) };
     return src[target];
+}
+
+/* pfmFloatToHalfI (pfm.h:107-126): round half up on the 13 dropped
+ * mantissa bits, flush below 2^-14 to signed zero, overflow to infinity,
+ * every NaN to 0x7E00. */
+static uint16_t float_to_half_bits(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const int32_t s = (int32_t)((u >> 16) & 0x8000u);
+    const int32_t em = (int32_t)(u & 0x7FFFFFFFu);
+    int32_t h = (em - (112 << 23) + (1 << 12)) >> 13;
+    if (em < (113 << 23)) h = 0;
+    if (em >= (143 << 23)) h = 0x7C00;
+    if (em > (255 << 23)) h = 0x7E00;
+    return (uint16_t)(s | h);
+}
+
+/* Half and float targets: component count, and the store of `n` components
+ * of pixel i in memory order. */
+static int target_wide(uint32_t target, size_t* comps, int* half)
+{
+    const uint32_t src = target_source(target), fmt = src & 0xFFu, typ = (src >> 8) & 0xFFu;
+    if (typ != SRC_HALF && typ != SRC_FLOAT) return 0;
+    *comps = fmt == SRC_LUMINANCE_ALPHA ? 2 : (fmt == SRC_RGB || fmt == SRC_BGR) ? 3 : 4;
+    *half = typ == SRC_HALF;
+    return 1;
+}
+
+static void wide_store(uint8_t* p, size_t i, size_t n, int half, const float* v)
+{
+    for (size_t k = 0; k < n; k++) {
+        if (half) { const uint16_t h = float_to_half_bits(v[k]); memcpy(p + 2 * (n * i + k), &h, 2); }
+        else memcpy(p + 4 * (n * i + k), &v[k], 4);
+    }
 }
 
 /* The triangle raster's read of a pixel: the target's AVX2 getter
@@ -1153,6 +1193,31 @@ static void tset_simd(const PFFframe* f, size_t i, uint32_t c)
     const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
     uint32_t v;
     uint16_t h;
+    size_t n;
+    int half;
+    if (target_wide(f->target, &n, &half)) {
+        /* Half and float setters: the channels converted to float and
+         * multiplied by 1.0f/255.0f (simd.h:88).  Luminance-alpha stores the
+         * integer grey (77 r + 150 g + 29 b) >> 8 (color.h:205-240) and alpha
+         * (pixel.h:1012-1090); BGR stores red, green, blue in that order --
+         * its "b" lane is the colour's low byte (pixel.h:1729-1848); BGRA
+         * stores blue, green, red, alpha (pixel.h:2121-2250). */
+        const float inv = 1.0f / 255.0f;
+        float w[4];
+        if (n == 2) {
+            const uint32_t grey = ((b * 29u + g * 150u) + r * 77u) >> 8;
+            w[0] = (float)(grey & 255u) * inv;
+            w[1] = (float)a * inv;
+        } else {
+            const int bgra = f->target == PFF_TARGET_BGRA_HALF || f->target == PFF_TARGET_BGRA_FLOAT;
+            w[0] = (float)(bgra ? b : r) * inv;
+            w[1] = (float)g * inv;
+            w[2] = (float)(bgra ? r : b) * inv;
+            w[3] = (float)a * inv;
+        }
+        wide_store(p, i, n, half, w);
+        return;
+    }
     switch (f->target) {
     case PFF_TARGET_RGB8:                                                          /* pixel.h:1590-1638 */
     case PFF_TARGET_BGR8:                                                          /* pixel.h:1641-1689 */
@@ -1207,6 +1272,28 @@ static void tset(const PFFframe* f, size_t i, uint32_t c)
     const float inv = (float)(1.0 / 255);
     const float nr = r * inv, ng = g * inv, nb = b * inv, na = a * inv;
     uint16_t h;
+    size_t n;
+    int half;
+    if (target_wide(f->target, &n, &half)) {
+        /* Half and float setters (pixel.h:102-127, 233-290, 347-400): the
+         * normalised channels; luminance-alpha the grey
+         * r*inv*0.299f + g*inv*0.587f + b*inv*0.114f in float (pixel.h:69-74);
+         * BGR swaps red and blue (pixel.h:46-51), the BGRA normaliser does
+         * not (pixel.h:61-67). */
+        float w[4];
+        if (n == 2) {
+            w[0] = r * inv * 0.299f + g * inv * 0.587f + b * inv * 0.114f;
+            w[1] = na;
+        } else {
+            const int bgr = f->target == PFF_TARGET_BGR_HALF || f->target == PFF_TARGET_BGR_FLOAT;
+            w[0] = bgr ? nb : nr;
+            w[1] = ng;
+            w[2] = bgr ? nr : nb;
+            w[3] = na;
+        }
+        wide_store(p, i, n, half, w);
+        return;
+    }
     switch (f->target) {
     case PFF_TARGET_RGB8: p[3 * i] = r; p[3 * i + 1] = g; p[3 * i + 2] = b; return;          /* pixel.h:243-249 */
     case PFF_TARGET_BGR8: p[3 * i] = b; p[3 * i + 1] = g; p[3 * i + 2] = r; return;          /* pixel.h:251-257 */

@@ -185,7 +185,9 @@ def api(path=LIB_PATH):
 
 
 # Bytes per pixel of the render-target (format, type) pairs.
-TARGET_BYTES = {(7, 0): 4, (6, 0): 3, (8, 0): 3, (6, 2): 2, (8, 2): 2, (7, 3): 2, (9, 3): 2, (7, 4): 2, (9, 4): 2}
+TARGET_BYTES = {(7, 0): 4, (6, 0): 3, (8, 0): 3, (6, 2): 2, (8, 2): 2, (7, 3): 2, (9, 3): 2, (7, 4): 2, (9, 4): 2,
+                (5, 9): 4, (5, 10): 8, (6, 9): 6, (8, 9): 6, (6, 10): 12, (8, 10): 12,
+                (7, 9): 8, (9, 9): 8, (7, 10): 16, (9, 10): 16}
 
 
 def render(params, path=LIB_PATH):
@@ -193,11 +195,11 @@ def render(params, path=LIB_PATH):
     (color, depth float32[H, W], error code): color is uint32[H, W] for an
     RGBA8 target, else the target's bytes as uint8[H, W, bytes per pixel]."""
     h, w = params.height, params.width
-    color = np.zeros((h * w + 16,), np.uint32)
+    bpp = TARGET_BYTES[(params.fbFormat, params.fbType)] if params.fbFormat > 0 else 4
+    color = np.zeros((h * w + 16,), np.uint32) if bpp <= 4 else np.zeros(((h * w + 16) * bpp,), np.uint8)
     depth = np.zeros((h * w + 16,), np.float32)
     err = scenes().pfs_render(api(path), C.byref(params), color.ctypes.data, depth.ctypes.data)
     if params.fbFormat > 0:
-        bpp = TARGET_BYTES[(params.fbFormat, params.fbType)]
         return color.view(np.uint8)[: h * w * bpp].reshape(h, w, bpp).copy(), depth[: h * w].reshape(h, w), err
     return color[: h * w].reshape(h, w), depth[: h * w].reshape(h, w), err
 

@@ -60,6 +60,7 @@ struct DFrame {
     float*    depth;
     uint32_t  W, H;
     uint32_t  tilesX, tilesY, tileBase, kind;   // kind: PFF_TARGET_*
+    uint32_t  bpp, pad[3];                       // bytes per colour pixel
 };
 
 // ---------------------------------------------------------------------------
@@ -789,7 +790,8 @@ struct SetupArgs {
     uint4*          clipList;       // triangles to clip {gid, t, info, frame} (PHASE 0 appends)
     uint32_t*       clipCount;
     uint32_t        clipCap;        // entries clipList holds
-    uint32_t*       clearSnap;      // per frame: colour and depth bits of pixel 0 when the pass began
+    uint32_t*       clearSnap;      // per frame, 8 words: colour word and depth bits of pixel 0 when the
+                                    // pass began, then (wide targets) its bytes
 };
 
 // Tiles [tx0..tx1] x [ty0..ty1] overlapped by the covered pixels of a
@@ -1162,8 +1164,11 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
                 const DSeg sg = A.segs[s];
                 if (!sg.clear) continue;
                 const DFrame F = A.frames[sg.frame];
-                A.clearSnap[2 * sg.frame] = tgt_load(F.color, F.kind, 0);
-                A.clearSnap[2 * sg.frame + 1] = __float_as_uint(F.depth[0]);
+                A.clearSnap[8 * sg.frame] = F.kind < PFF_TARGET_FIRST_WIDE ? tgt_load(F.color, F.kind, 0) : 0u;
+                A.clearSnap[8 * sg.frame + 1] = __float_as_uint(F.depth[0]);
+                uint8_t* snap = (uint8_t*)&A.clearSnap[8 * sg.frame + 2];
+                if (F.kind >= PFF_TARGET_FIRST_WIDE)
+                    for (uint32_t b = 0; b < F.bpp; b++) snap[b] = F.color[b];
             }
     }
     // The triangle, its frame and positions are requested first, so their
@@ -1528,7 +1533,7 @@ struct RasterArgs {
                                     // the raster itself)
     uint32_t*       winner;         // per pixel of frame 0: 1 + raster triangle of the last
                                     // fragment written, 0 for none; null unless differentiable
-    const uint32_t* clearSnap;      // per frame: pixel 0 (colour, depth bits) when the pass began
+    const uint32_t* clearSnap;      // per frame, 8 words: pixel 0 (colour word, depth bits, bytes) when the pass began
 };
 
 // ---------------------------------------------------------------------------
@@ -1677,8 +1682,20 @@ __device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& 
     outZ = zq;
 }
 
+// Wide targets' SIMD accessors (defined with the texel decoders below).
+// A wide pixel's `win` is 0 while the frame has not written it, WIN_SNAP
+// when a clear's tail gave it pixel 0's bytes (kept in clearSnap), else
+// the writer's raster triangle + 1.
+constexpr uint32_t WIN_SNAP = 0xFFFFFFFEu;
+__device__ __forceinline__ uint32_t wide_get_simd(const uint8_t* p, uint32_t kind, size_t i);
+__device__ __forceinline__ void wide_set_simd(uint8_t* p, uint32_t kind, size_t i, uint32_t c);
+__device__ __forceinline__ uint32_t wide_rt_simd(uint32_t kind, uint32_t c);
+
 // Depth test and blend of one fragment against the lane's pixel
 // (depth.h:80-124, blend.h:137-274, triangles.c:434-438, 523-529).
+// WIDE: the kernel may meet wide targets (only its instances compile their
+// path, which costs registers).
+template <bool WIDE>
 __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t kind, uint32_t& col,
                                     float& dep, uint32_t& win) {
     if (meta & META_DEPTH) {
@@ -1695,6 +1712,13 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
     }
     if (kind == PFF_TARGET_RGBA8) {
         if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
+        col = c;
+    } else if (WIDE && kind >= PFF_TARGET_FIRST_WIDE) {
+        // col is the RGBA8 value of the pixel: the getter's reading of its
+        // bytes while unwritten, else the colour last written, which the
+        // target holds through the SIMD setter.
+        if (meta & META_BLEND)
+            c = blend_op((meta >> META_BMODE_SH) & 7u, c, (win != 0u && win != WIN_SNAP) ? wide_rt_simd(kind, col) : col);
         col = c;
     } else {
         // col holds the target's stored pixel: blending reads it through the
@@ -1713,7 +1737,7 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
 // Queued pixel p of a TW-wide block is (bx0 + p % TW, by0 + p / TW); lane
 // l owns pixels l + 64k, k < K, whose fragments sit in the queue slots set
 // in pm[k] (slots grow in submission order).
-template <bool PHONG, int K, bool ROT>
+template <bool PHONG, int K, bool ROT, bool WIDE>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
                                             int nf, uint64_t* pm, uint32_t kind, uint32_t* col, float* dep,
                                             uint32_t* win) {
@@ -1739,7 +1763,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
             const int s = __builtin_ctzll(m);
             m &= m - 1ull;
             const int j = Q.ftri[s];
-            rop(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], kind, col[k], dep[k], win[k]);
+            rop<WIDE>(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], kind, col[k], dep[k], win[k]);
         }
     }
     wave_sync();
@@ -1756,7 +1780,7 @@ __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
 // 8x8 with one pixel per lane (K = 1) or 16x16 with four pixels per lane
 // (K = 4: lane l owns column l % 16 of rows l / 16 + 4k), so that one walk
 // of the list serves 256 pixels.  Returns the covered samples it saw.
-template <int MODE, bool PHONG, int K>
+template <int MODE, bool PHONG, int K, bool WIDE>
 __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
                                                  uint32_t frame, int bx0, int by0, uint32_t e0, uint32_t e1,
                                                  WaveQueueT<PHONG>& Q) {
@@ -1782,14 +1806,18 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         win[k] = 0u;
         dep[k] = 0.0f;
         if (in[k]) {
+            const bool wide = WIDE && F.kind >= PFF_TARGET_FIRST_WIDE;
             if (sg.clear && pi[k] >= 8 && pi[k] < aligned) {       // context.c:693-713
                 col[k] = sg.clearColor;                              // as the SIMD setter stores it
                 dep[k] = sg.clearDepth;
+                if (wide) win[k] = 1u;                               // written by the clear
             } else if (sg.clear && pi[k] >= aligned) {              // the tail copies pixel 0
-                col[k] = A.clearSnap[2 * frame];
-                dep[k] = __uint_as_float(A.clearSnap[2 * frame + 1]);
+                col[k] = wide ? wide_get_simd((const uint8_t*)&A.clearSnap[8 * frame + 2], F.kind, 0)
+                              : A.clearSnap[8 * frame];
+                dep[k] = __uint_as_float(A.clearSnap[8 * frame + 1]);
+                if (wide) win[k] = WIN_SNAP;
             } else {
-                col[k] = tgt_load(F.color, F.kind, pi[k]);
+                col[k] = wide ? wide_get_simd(F.color, F.kind, pi[k]) : tgt_load(F.color, F.kind, pi[k]);
                 dep[k] = F.depth[pi[k]];
             }
         }
@@ -1902,12 +1930,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                         float fz;
                         shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), px, py[k],
                                               fc, fz);
-                        rop(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
+                        rop<WIDE>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
                     }
                     continue;
                 }
                 if (nf + c > 64) {
-                    flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
+                    flush_queue<PHONG, K, ROT, WIDE>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
                     nf = 0;
                 }
                 if (cov) {
@@ -1919,13 +1947,21 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 nf += c;
             }
         }
-        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
+        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT, WIDE>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
         wave_sync();
     }
 #pragma unroll
     for (int k = 0; k < K; k++)
         if (in[k]) {
-            tgt_store(F.color, F.kind, pi[k], col[k]);
+            if (!WIDE || F.kind < PFF_TARGET_FIRST_WIDE) {
+                tgt_store(F.color, F.kind, pi[k], col[k]);
+            } else if (win[k] == WIN_SNAP) {                         // pixel 0's bytes at the clear
+                const uint32_t bpp = F.bpp;
+                const uint8_t* snap = (const uint8_t*)&A.clearSnap[8 * frame + 2];
+                for (uint32_t b = 0; b < bpp; b++) F.color[pi[k] * bpp + b] = snap[b];
+            } else if (win[k] != 0u) {
+                wide_set_simd(F.color, F.kind, pi[k], col[k]);
+            }
             F.depth[pi[k]] = dep[k];
             if (A.winner) A.winner[pi[k]] = win[k];
         }
@@ -1953,7 +1989,7 @@ __device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
 // (BW = 8 for K = 1, 16 for K = 4).  When a tile is one block, each wave
 // takes its own tile (four tiles per workgroup); otherwise one tile per
 // workgroup, its blocks dealt round-robin to the four waves.
-template <int TS, int MODE, bool PHONG, int K>
+template <int TS, int MODE, bool PHONG, int K, bool WIDE>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     typedef WaveQueueT<PHONG> Queue;
     __shared__ Queue Qs[4];
@@ -1980,7 +2016,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
         const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        frags += raster_block<MODE, PHONG, K>(A, F, sg, frame, bx0, by0, e0, e1, Q);
+        frags += raster_block<MODE, PHONG, K, WIDE>(A, F, sg, frame, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -2529,6 +2565,161 @@ __device__ uint32_t decode_scalar_texel(const uint8_t* __restrict__ p, size_t i,
     return 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Wide render targets (PFF_TARGET_LA_HALF and after: luminance-alpha, RGB,
+// BGR, RGBA and BGRA in half or float components).  A kernel keeps such a
+// pixel as an RGBA8 value and whether the frame has written it: an
+// unwritten pixel keeps its bytes and its value is what the target's getter
+// reads from them; a written pixel will hold the setter's encoding of its
+// value, which blending reads back through the getter (the round trips
+// below).  The raster uses the SIMD accessors, the other kernels the scalar
+// ones, as the reference does.
+// ---------------------------------------------------------------------------
+
+// pfmFloatToHalfI (pfm.h:107-126): mantissa rounded half up, magnitudes
+// below 2^-14 flushed to signed zero, overflow to infinity, NaN to 0x7E00.
+__device__ __forceinline__ uint32_t float_to_half_ref(float x) {
+    const uint32_t ui = __float_as_uint(x);
+    const int32_t s = (int32_t)((ui >> 16) & 0x8000u);
+    const int32_t em = (int32_t)(ui & 0x7FFFFFFFu);
+    int32_t h = (em - (112 << 23) + (1 << 12)) >> 13;
+    if (em < (113 << 23)) h = 0;
+    if (em >= (143 << 23)) h = 0x7C00;
+    if (em > (255 << 23)) h = 0x7E00;
+    return (uint32_t)(s | h);
+}
+
+__device__ __forceinline__ bool wide_half(uint32_t kind) {
+    return kind == PFF_TARGET_LA_HALF || kind == PFF_TARGET_RGB_HALF || kind == PFF_TARGET_BGR_HALF
+        || kind == PFF_TARGET_RGBA_HALF || kind == PFF_TARGET_BGRA_HALF;
+}
+__device__ __forceinline__ uint32_t wide_comps(uint32_t kind) {
+    return kind <= PFF_TARGET_LA_FLOAT ? 2u : kind <= PFF_TARGET_BGR_FLOAT ? 3u : 4u;
+}
+__device__ __forceinline__ uint32_t wide_source(uint32_t kind) {
+    switch (kind) {
+    case PFF_TARGET_LA_HALF:    return PFF_SOURCE(5, 9);
+    case PFF_TARGET_LA_FLOAT:   return PFF_SOURCE(5, 10);
+    case PFF_TARGET_RGB_HALF:   return PFF_SOURCE(6, 9);
+    case PFF_TARGET_BGR_HALF:   return PFF_SOURCE(8, 9);
+    case PFF_TARGET_RGB_FLOAT:  return PFF_SOURCE(6, 10);
+    case PFF_TARGET_BGR_FLOAT:  return PFF_SOURCE(8, 10);
+    case PFF_TARGET_RGBA_HALF:  return PFF_SOURCE(7, 9);
+    case PFF_TARGET_BGRA_HALF:  return PFF_SOURCE(9, 9);
+    case PFF_TARGET_RGBA_FLOAT: return PFF_SOURCE(7, 10);
+    default:                    return PFF_SOURCE(9, 10);
+    }
+}
+
+// Stores component k of pixel i: a half or a float.
+__device__ __forceinline__ void wide_put(uint8_t* p, uint32_t kind, size_t i, uint32_t k, float v) {
+    const size_t n = wide_comps(kind);
+    if (wide_half(kind)) reinterpret_cast<uint16_t*>(p)[n * i + k] = (uint16_t)float_to_half_ref(v);
+    else reinterpret_cast<float*>(p)[n * i + k] = v;
+}
+
+// The components the SIMD setter stores for RGBA8 colour c, in memory
+// order: channels times 1.0f/255.0f (simd.h:88); LA takes the integer grey
+// (b*29 + g*150 + r*77) >> 8 (color.h:205-240); BGR stores red first
+// (pixel.h:1731-1848), BGRA blue first (pixel.h:2123-2250).
+__device__ __forceinline__ void wide_simd_comps(uint32_t kind, uint32_t c, float* v) {
+    const float inv = 1.0f / 255.0f;
+    const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
+    if (kind <= PFF_TARGET_LA_FLOAT) {                                        // pixel.h:1013-1090
+        const uint32_t grey = ((b * 29u + g * 150u) + r * 77u) >> 8;
+        v[0] = (float)(grey & 255u) * inv;
+        v[1] = (float)a * inv;
+        return;
+    }
+    const bool bgra = kind == PFF_TARGET_BGRA_HALF || kind == PFF_TARGET_BGRA_FLOAT;
+    v[0] = (float)(bgra ? b : r) * inv;
+    v[1] = (float)g * inv;
+    v[2] = (float)(bgra ? r : b) * inv;
+    v[3] = (float)a * inv;
+}
+
+__device__ __forceinline__ void wide_set_simd(uint8_t* p, uint32_t kind, size_t i, uint32_t c) {
+    float v[4];
+    wide_simd_comps(kind, c, v);
+    const uint32_t n = wide_comps(kind);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+        if (k < n) wide_put(p, kind, i, k, v[k]);
+}
+
+// The SIMD getter (pixel.h:2328-2383, 2666-2790, 2923-3040).
+__device__ __forceinline__ uint32_t wide_get_simd(const uint8_t* p, uint32_t kind, size_t i) {
+    return decode_source_texel(p, i, wide_source(kind));
+}
+
+// The SIMD getter's reading of the SIMD setter's encoding of c: each
+// stored component v read back as cvt_rne(v * 255), packed as the getter
+// packs it (BGR's first component and BGRA's third land in red).
+__device__ __forceinline__ uint32_t wide_rt_simd(uint32_t kind, uint32_t c) {
+    float v[4];
+    wide_simd_comps(kind, c, v);
+    const bool h = wide_half(kind);
+    uint32_t q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float s = h ? half_to_float_ref(float_to_half_ref(v[k])) : v[k];
+        q[k] = (uint32_t)cvt_rne_i32(s * 255.0f);
+    }
+    if (kind <= PFF_TARGET_LA_FLOAT) return ((q[1] << 24) | (q[0] << 16)) | ((q[0] << 8) | q[0]);
+    if (kind <= PFF_TARGET_BGR_FLOAT) return (0xFF000000u | (q[2] << 16)) | ((q[1] << 8) | q[0]);
+    const bool bgra = kind == PFF_TARGET_BGRA_HALF || kind == PFF_TARGET_BGRA_FLOAT;
+    return ((q[3] << 24) | ((bgra ? q[0] : q[2]) << 16)) | ((q[1] << 8) | (bgra ? q[2] : q[0]));
+}
+
+// The components the scalar setter stores (pixel.h:102-127, 220-290,
+// 347-400): channels times (float)PF_INV_255; LA the grey of
+// PF_COLOR_GARYSCALE in float (pixel.h:69-74); BGR swaps red and blue, the
+// BGRA normaliser does not (pixel.h:59-67).
+__device__ __forceinline__ void wide_scalar_comps(uint32_t kind, uint32_t c, float* v) {
+    const float inv = (float)(1.0 / 255.0);
+    const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u, a = c >> 24;
+    if (kind <= PFF_TARGET_LA_FLOAT) {
+        v[0] = (float)r * inv * 0.299f + (float)g * inv * 0.587f + (float)b * inv * 0.114f;
+        v[1] = (float)a * inv;
+        return;
+    }
+    const bool bgr = kind == PFF_TARGET_BGR_HALF || kind == PFF_TARGET_BGR_FLOAT;
+    v[0] = (float)(bgr ? b : r) * inv;
+    v[1] = (float)g * inv;
+    v[2] = (float)(bgr ? r : b) * inv;
+    v[3] = (float)a * inv;
+}
+
+__device__ __forceinline__ void wide_set_scalar(uint8_t* p, uint32_t kind, size_t i, uint32_t c) {
+    float v[4];
+    wide_scalar_comps(kind, c, v);
+    const uint32_t n = wide_comps(kind);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+        if (k < n) wide_put(p, kind, i, k, v[k]);
+}
+
+// The scalar getter (pixel.h:441-457, 589-639, 709-759).
+__device__ __forceinline__ uint32_t wide_get_scalar(const uint8_t* p, uint32_t kind, size_t i) {
+    return decode_scalar_texel(p, i, wide_source(kind));
+}
+
+// The scalar getter's reading of the scalar setter's encoding of c: each
+// component as (PFubyte)(v * 255), packed as the getter packs it.
+__device__ __forceinline__ uint32_t wide_rt_scalar(uint32_t kind, uint32_t c) {
+    float v[4];
+    wide_scalar_comps(kind, c, v);
+    const bool h = wide_half(kind);
+    uint32_t q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = f2u8((h ? half_to_float_ref(float_to_half_ref(v[k])) : v[k]) * 255.0f);
+    if (kind <= PFF_TARGET_LA_FLOAT) return pack4(q[0], q[0], q[0], q[1]);
+    const bool bgr = kind == PFF_TARGET_BGR_HALF || kind == PFF_TARGET_BGR_FLOAT;
+    const bool bgra = kind == PFF_TARGET_BGRA_HALF || kind == PFF_TARGET_BGRA_FLOAT;
+    if (kind <= PFF_TARGET_BGR_FLOAT) return bgr ? pack4(q[2], q[1], q[0], 255) : pack4(q[0], q[1], q[2], 255);
+    return bgra ? pack4(q[2], q[1], q[0], q[3]) : pack4(q[0], q[1], q[2], q[3]);
+}
+
 __global__ __launch_bounds__(256) void k_decode_image(const uint8_t* __restrict__ src, size_t n, uint32_t source,
                                                       uint32_t* __restrict__ out) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -2596,6 +2787,16 @@ __device__ __forceinline__ uint32_t tgt_set_scalar(uint32_t kind, uint32_t c) {
              | (uint32_t)(uint8_t)roundf((float)a * inv * 15.0f);
     default: return c;
     }
+}
+
+// One pixel of a target through the scalar getter / setter (points,
+// lines, pfDrawPixels, rects, fog).
+__device__ __forceinline__ uint32_t px_get_scalar(const uint8_t* p, uint32_t kind, size_t i) {
+    return kind >= PFF_TARGET_FIRST_WIDE ? wide_get_scalar(p, kind, i) : tgt_get_scalar(kind, tgt_load(p, kind, i));
+}
+__device__ __forceinline__ void px_set_scalar(uint8_t* p, uint32_t kind, size_t i, uint32_t c) {
+    if (kind >= PFF_TARGET_FIRST_WIDE) wide_set_scalar(p, kind, i, c);
+    else tgt_store(p, kind, i, tgt_set_scalar(kind, c));
 }
 
 struct DPixOp {
@@ -2677,8 +2878,7 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
         for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256) {
             const float z = o.depth[k];
             if (!(z >= o.fogEnd) && !(z > o.fogStart)) continue;      // untouched: no getter or setter call
-            const uint32_t c = fog_pixel(o, z, tgt_get_scalar(o.target, tgt_load(o.color, o.target, k)));
-            tgt_store(o.color, o.target, k, tgt_set_scalar(o.target, c));
+            px_set_scalar(o.color, o.target, k, fog_pixel(o, z, px_get_scalar(o.color, o.target, k)));
         }
         return;
     }
@@ -2697,8 +2897,8 @@ __global__ __launch_bounds__(256) void k_draw_pixels(const DPixOp* __restrict__ 
         if (src >= n) src = n - 1u;
         if (!(o.flags & PFF_PIXELS_RECT)) o.depth[dst] = o.zPos;      // pfRect writes no depth
         uint32_t c = o.texels[src];
-        if (o.flags & PFF_BLEND) c = blend_scalar(o.blendMode, c, tgt_get_scalar(o.target, tgt_load(o.color, o.target, dst)));
-        tgt_store(o.color, o.target, dst, tgt_set_scalar(o.target, c));
+        if (o.flags & PFF_BLEND) c = blend_scalar(o.blendMode, c, px_get_scalar(o.color, o.target, dst));
+        px_set_scalar(o.color, o.target, dst, c);
     }
 }
 
@@ -2796,7 +2996,11 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
     const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
     const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
     const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
-    uint32_t c = inside ? tgt_load(o.color, o.target, off) : 0u;      // stored form
+    // Narrow targets: c is the stored word.  Wide targets: c is the RGBA8
+    // value (the scalar getter's reading while unwritten, then the colour
+    // last written; dirty says written).
+    const bool wide = o.target >= PFF_TARGET_FIRST_WIDE;
+    uint32_t c = inside ? (wide ? wide_get_scalar(o.color, o.target, off) : tgt_load(o.color, o.target, off)) : 0u;
     float d = inside ? o.depth[off] : 0.0f;
     bool dirty = false;
     __shared__ uint32_t idx[256];
@@ -2819,15 +3023,19 @@ __global__ __launch_bounds__(256) void k_points(const DPtOp* __restrict__ ops, c
                        : (dx >= -s.ir && dx <= s.ir && dy >= -s.ir && dy <= s.ir && (float)(dy * dy + dx * dx) <= s.rSq);
         if (!inside || !cov) continue;
         if ((s.flags & PFF_DEPTH_TEST) && !depth_scalar(s.depthFunc, s.z, d)) continue;
-        c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, tgt_get_scalar(o.target, c))
-                                                            : s.color);
+        if (wide)
+            c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, dirty ? wide_rt_scalar(o.target, c) : c) : s.color;
+        else
+            c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, s.color, tgt_get_scalar(o.target, c))
+                                                                : s.color);
         d = s.z;
         dirty = true;
       }
       __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
-        tgt_store(o.color, o.target, off, c);
+        if (wide) wide_set_scalar(o.color, o.target, off, c);
+        else tgt_store(o.color, o.target, off, c);
         o.depth[off] = d;
     }
 }
@@ -3026,7 +3234,11 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
     const int32_t px = x0 + (int32_t)(threadIdx.x & 15u), py = y0 + (int32_t)(threadIdx.x >> 4);
     const bool inside = (uint32_t)px < o.W && (uint32_t)py < o.H;
     const size_t off = inside ? (size_t)py * o.W + (size_t)px : 0;
-    uint32_t c = inside ? tgt_load(o.color, o.target, off) : 0u;      // stored form
+    // Narrow targets: c is the stored word.  Wide targets: c is the RGBA8
+    // value (the scalar getter's reading while unwritten, then the colour
+    // last written; dirty says written).
+    const bool wide = o.target >= PFF_TARGET_FIRST_WIDE;
+    uint32_t c = inside ? (wide ? wide_get_scalar(o.color, o.target, off) : tgt_load(o.color, o.target, off)) : 0u;
     float d = inside ? o.depth[off] : 0.0f;
     bool dirty = false;
     // Line positions whose offset y * W + x is this pixel's.
@@ -3075,8 +3287,11 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
                     const int32_t ca = (int32_t)((s.c1 >> (8 * q)) & 255u), cb = (int32_t)((s.c2 >> (8 * q)) & 255u);
                     col |= (uint32_t)f2u8((float)ca + t * (float)(cb - ca)) << (8 * q);
                 }
-                c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, tgt_get_scalar(o.target, c))
-                                                                    : col);
+                if (wide)
+                    c = (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, dirty ? wide_rt_scalar(o.target, c) : c) : col;
+                else
+                    c = tgt_set_scalar(o.target, (s.flags & PFF_BLEND) ? blend_scalar(s.blendMode, col, tgt_get_scalar(o.target, c))
+                                                                        : col);
                 d = z;
                 dirty = true;
             }
@@ -3085,7 +3300,8 @@ __global__ __launch_bounds__(256) void k_lines(const DLnOp* __restrict__ ops, co
       __syncthreads();                                          // idx[] is rewritten by the next chunk
     }
     if (dirty) {
-        tgt_store(o.color, o.target, off, c);
+        if (wide) wide_set_scalar(o.color, o.target, off, c);
+        else tgt_store(o.color, o.target, off, c);
         o.depth[off] = d;
     }
 }
@@ -3316,6 +3532,7 @@ struct PFDjob {
     DevBuf   lines, lineScr;       // PFFline of all frames, their clipped screen forms
     uint32_t numIn = 0;
     bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
+    bool anyWide = false;                 // a frame's target is wide (PFF_TARGET_FIRST_WIDE on)
     std::vector<PassInfo> passes;
     PFDconfig cfg{};
     uint32_t tileShift = 3;
@@ -3547,21 +3764,28 @@ uint32_t grid_for(uint64_t n, uint32_t block) {
     return g == 0 ? 1u : (uint32_t)g;
 }
 
+// Jobs with a wide target run the K = 1 instances that compile its path.
 template <int TS, int MODE, int K>
-void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, hipStream_t s) {
+void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, bool wide, hipStream_t s) {
     constexpr int BW = (K == 1) ? 8 : 16;
     uint32_t grid = (TS == BW) ? (numTiles + 3) / 4 : numTiles;
-    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, K>), dim3(grid), dim3(256), 0, s, ra);
-    else       hipLaunchKernelGGL((k_raster<TS, MODE, false, K>), dim3(grid), dim3(256), 0, s, ra);
+    if (wide) {
+        const uint32_t g1 = (TS == 8) ? (numTiles + 3) / 4 : numTiles;
+        if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, 1, true>), dim3(g1), dim3(256), 0, s, ra);
+        else       hipLaunchKernelGGL((k_raster<TS, MODE, false, 1, true>), dim3(g1), dim3(256), 0, s, ra);
+        return;
+    }
+    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, K, false>), dim3(grid), dim3(256), 0, s, ra);
+    else       hipLaunchKernelGGL((k_raster<TS, MODE, false, K, false>), dim3(grid), dim3(256), 0, s, ra);
 }
 
 template <int MODE, int K>
-void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, bool phong, hipStream_t s) {
+void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, bool phong, bool wide, hipStream_t s) {
     switch (shift) {
-    case 3: launch_raster<8, MODE, 1>(ra, numTiles, phong, s); break;   // a 8x8 tile is one K = 1 block
-    case 4: launch_raster<16, MODE, K>(ra, numTiles, phong, s); break;
-    case 5: launch_raster<32, MODE, K>(ra, numTiles, phong, s); break;
-    default: launch_raster<64, MODE, K>(ra, numTiles, phong, s); break;
+    case 3: launch_raster<8, MODE, 1>(ra, numTiles, phong, wide, s); break;   // a 8x8 tile is one K = 1 block
+    case 4: launch_raster<16, MODE, K>(ra, numTiles, phong, wide, s); break;
+    case 5: launch_raster<32, MODE, K>(ra, numTiles, phong, wide, s); break;
+    default: launch_raster<64, MODE, K>(ra, numTiles, phong, wide, s); break;
     }
 }
 
@@ -3602,7 +3826,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if ((rc = ensure(J, J->blockRT, (size_t)nb * 4 + 4)) || (rc = ensure(J, J->blockPairs, (size_t)nb * 4 + 4))) return rc;
     if ((rc = ensure(J, J->scanTotal, 16))) return rc;
     if ((rc = ensure(J, J->clipList, (size_t)n * 16 + 16)) || (rc = ensure(J, J->clipCount, 16))) return rc;
-    if ((rc = ensure(J, J->clearSnap, J->frames.size() * 8))) return rc;
+    if ((rc = ensure(J, J->clearSnap, J->frames.size() * 32))) return rc;
     const size_t fragSlots = (size_t)J->numTiles * 4;
     const size_t passIdx = (size_t)(&P - J->passes.data());
     if (discover) {
@@ -3765,10 +3989,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.winner = J->diff ? (uint32_t*)J->winner.p : nullptr;
     ra.clearSnap = (const uint32_t*)J->clearSnap.p;
     switch (J->cfg.rasterMode) {
-    case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
-    case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
-    case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
-    default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->stream); break;
+    case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
+    case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
+    case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
+    default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
     }
     if (!P.pix.empty())
         hipLaunchKernelGGL(k_draw_pixels, dim3(P.pixGrid, (uint32_t)P.pix.size()), dim3(256), 0, J->stream,
@@ -3860,7 +4084,8 @@ int upload_frames_table(PFDjob* J) {
             const DSeg& sg = P.segs[si];
             t.active = 1;
             t.clear = sg.clear;
-            t.clearColor = tgt_set_simd(t.F.kind, sg.clearColor);        // stored form of the clear colour
+            t.clearColor = t.F.kind >= PFF_TARGET_FIRST_WIDE ? sg.clearColor      // wide: the RGBA8 value
+                                                             : tgt_set_simd(t.F.kind, sg.clearColor);   // stored form
             t.clearDepth = sg.clearDepth;
         }
     }
@@ -3988,6 +4213,7 @@ void job_reset(PFDjob* J) {
     J->passes.clear();
     J->numIn = 0;
     J->hasUV = J->hasNrm = J->anyPhong = J->anyLit = false;
+    J->anyWide = false;
     J->numTiles = 0;
     J->autoTiles = false;
     J->sortedInB = 0;
@@ -4101,7 +4327,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         anyUV |= fr.tris.texcoord != nullptr;
         anyNrm |= fr.tris.normal != nullptr;
         if (pff_target_bytes(fr.target) == 0u) PLAN_FAIL(PFD_E_INVALID, "frame %u: unknown render target format %u", f, fr.target);
-        targetBytes += align256((size_t)fr.width * fr.height * pff_target_bytes(fr.target))
+        targetBytes += align256((size_t)fr.width * fr.height * pff_target_bytes(fr.target) + 16)
                      + align256((size_t)fr.width * fr.height * 4);
         for (uint32_t k = 0; k < fr.numTextures; k++) {
             const PFFtexture& T = fr.textures[k];
@@ -4196,8 +4422,12 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         D.H = fr.height;
         const size_t npx = (size_t)fr.width * fr.height;
         D.kind = fr.target;
+        D.bpp = pff_target_bytes(fr.target);
+        J->anyWide = J->anyWide || fr.target >= PFF_TARGET_FIRST_WIDE;
         D.color = (uint8_t*)J->targetPool.p + targetOff;
-        targetOff += align256(npx * pff_target_bytes(fr.target));
+        // 16 spare bytes: the wide targets' getters read whole words past a
+        // pixel's last half component.
+        targetOff += align256(npx * D.bpp + 16);
         D.depth = (float*)((uint8_t*)J->targetPool.p + targetOff);
         targetOff += align256(npx * 4);
 

@@ -2636,12 +2636,45 @@ static void set_pixel(void* dst, size_t i, PFpixelformat f, PFdatatype t, PFcolo
     }
 }
 
+/* pfmHalfToFloatI (pfm.h:128-145): subnormals flush to signed zero,
+ * infinities and NaNs keep their payload. */
+static float half_to_float_ref(uint16_t h)
+{
+    const uint32_t s = (uint32_t)(h & 0x8000u) << 16;
+    const int32_t em = (int32_t)(h & 0x7FFFu);
+    int32_t r = (em + (112 << 10)) << 13;
+    if (em < (1 << 10)) r = 0;
+    if (em >= (31 << 10)) r += 112 << 23;
+    const uint32_t u = s | (uint32_t)r;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
 /* One pixel of a render target through the reference's scalar getter of
- * its pair (pixel.h:549-707): RGB/BGR bytes, and the 16-bit fields scaled
- * by 255.0f/31, /63 or /15 in float and truncated. */
+ * its pair (pixel.h:441-457, 549-759): RGB/BGR bytes; the 16-bit fields
+ * scaled by 255.0f/31, /63 or /15 in float and truncated; half and float
+ * components as (PFubyte)(v * 255). */
 static PFcolor get_pixel(const void* src, size_t i, PFpixelformat f, PFdatatype t)
 {
     const uint8_t* b = (const uint8_t*)src;
+    if (t == PF_HALF_FLOAT || t == PF_FLOAT) {
+        const size_t n = f == PF_LUMINANCE_ALPHA ? 2 : (f == PF_RGB || f == PF_BGR) ? 3 : 4;
+        uint8_t c[4] = { 0, 0, 0, 0 };
+        for (size_t k = 0; k < n; k++) {
+            float v;
+            if (t == PF_HALF_FLOAT) { uint16_t h; memcpy(&h, b + 2 * (n * i + k), 2); v = half_to_float_ref(h); }
+            else memcpy(&v, b + 4 * (n * i + k), 4);
+            c[k] = f2u8(v * 255.0f);
+        }
+        switch (f) {
+        case PF_LUMINANCE_ALPHA: return (PFcolor){ c[0], c[0], c[0], c[1] };   /* pixel.h:441-457 */
+        case PF_RGB:  return (PFcolor){ c[0], c[1], c[2], 255 };             /* pixel.h:589-639 */
+        case PF_BGR:  return (PFcolor){ c[2], c[1], c[0], 255 };
+        case PF_BGRA: return (PFcolor){ c[2], c[1], c[0], c[3] };            /* pixel.h:709-759 */
+        default:      return (PFcolor){ c[0], c[1], c[2], c[3] };
+        }
+    }
     if (t == PF_UNSIGNED_BYTE) {
         if (f == PF_RGB) return (PFcolor){ b[3 * i], b[3 * i + 1], b[3 * i + 2], 255 };
         if (f == PF_BGR) return (PFcolor){ b[3 * i + 2], b[3 * i + 1], b[3 * i], 255 };

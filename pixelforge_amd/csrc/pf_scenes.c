@@ -905,8 +905,11 @@ static void draw_scene(const PFSapi* a, Rng* r, const PFSparams* p, SceneTex* te
  * others). */
 static size_t target_bytes(PFpixelformat f, PFdatatype t)
 {
+    const size_t comps = f == PF_LUMINANCE_ALPHA ? 2 : (f == PF_RGB || f == PF_BGR) ? 3 : 4;
+    if (t == PF_HALF_FLOAT) return 2 * comps;
+    if (t == PF_FLOAT) return 4 * comps;
     if (t != PF_UNSIGNED_BYTE) return 2;
-    return (f == PF_RGB || f == PF_BGR) ? 3 : 4;
+    return comps;
 }
 
 /* Renders one scene into an offscreen framebuffer of the bound library (or,

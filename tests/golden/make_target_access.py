@@ -3,7 +3,8 @@ render targets other than RGBA8, as the CPU build of the reference
 (oracle/_ref, `make -C oracle ref`) performs them.
 
 For each render-target pair (RGB/BGR bytes, RGB/BGR 5_6_5, RGBA/BGRA
-5_5_5_1 and 4_4_4_4), on 16x16 targets:
+5_5_5_1 and 4_4_4_4, and luminance-alpha, RGB, BGR, RGBA and BGRA in half
+and float components), on 16x16 targets:
   - a context's main buffer of that pair filled with fixed random bytes,
     read back with pfReadPixels as RGBA / UNSIGNED_BYTE (the pair's scalar
     getter, pixel.h:549-707) and as RGB / UNSIGNED_SHORT_5_6_5;
@@ -26,19 +27,36 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpfref.so")
 
-TARGETS = [(6, 0), (8, 0), (6, 2), (8, 2), (7, 3), (9, 3), (7, 4), (9, 4)]
+TARGETS = [(6, 0), (8, 0), (6, 2), (8, 2), (7, 3), (9, 3), (7, 4), (9, 4),
+           (5, 9), (5, 10), (6, 9), (8, 9), (6, 10), (8, 10), (7, 9), (9, 9), (7, 10), (9, 10)]
 W = H = 16
 # (x, y, RGBA8 word, depth) of the pixel writes, in order.
 WRITES = [(0, 0, 0x31FF8040, 0.5), (5, 3, 0x32102030, 0.25), (15, 15, 0x00FFFFFF, 0.75),
           (7, 9, 0x33C0A080, 0.125), (8, 1, 0xFF0A141E, 0.9), (2, 14, 0x807F7F7F, 0.3)]
 
 
+def comps(f):
+    return 2 if f == 5 else 3 if f in (6, 8) else 4
+
+
 def bpp(f, t):
-    return 2 if t else 3
+    if t in (9, 10):
+        return comps(f) * (2 if t == 9 else 4)
+    return 2 if t else 3 if f in (6, 8) else 4
+
+
+# Component values of the half and float buffers besides uniform ones in
+# [-0.5, 1.5): exact ends, values just under a byte step, a half subnormal
+# (flushed to zero when read), infinities and a NaN (read as byte 0).
+SPECIAL = [0.0, -0.0, 1.0, 254.99 / 255, 1 / 255, 6e-8, 0.5, float("inf"), float("-inf"), float("nan")]
 
 
 def buffer_bytes(f, t, seed):
     rng = np.random.default_rng(seed)
+    if t in (9, 10):
+        v = rng.uniform(-0.5, 1.5, W * H * comps(f)).astype(np.float32)
+        v[rng.permutation(v.size)[: 4 * len(SPECIAL)]] = np.repeat(np.array(SPECIAL, np.float32), 4)
+        return v.astype(np.float16 if t == 9 else np.float32).view(np.uint8)
     return rng.integers(0, 256, W * H * bpp(f, t), dtype=np.uint8)
 
 

@@ -243,21 +243,23 @@ RECT_SCENES = [
 # unblended 3D scene in the context's main buffer created in that pair.
 _TARGETS = [("rgb", "ubyte"), ("bgr", "ubyte"), ("rgb", "u565"), ("bgr", "u565"),
             ("rgba", "u5551"), ("bgra", "u5551"), ("rgba", "u4444"), ("bgra", "u4444")]
+# Targets in half and float components.
+_WIDE_TARGETS = [(f, t) for f in ("lumalpha", "rgb", "bgr", "rgba", "bgra") for t in ("half", "float")]
 TARGET_SCENES = [
     (f"target_{f}_{t}_raster", dict(kind=pf.RANDOM_3D, width=61, height=45, ntris=150, seed=800 + k, texture=1,
                                     texW=17, texH=13, blend=1, blendMode=1 + k % 7,
                                     fbFormat=_FORMATS[f], fbType=_TYPES[t]))
-    for k, (f, t) in enumerate(_TARGETS)
+    for k, (f, t) in enumerate(_TARGETS + _WIDE_TARGETS)
 ] + [
     (f"target_{f}_{t}_prims", dict(kind=pf.RANDOM_NDC, width=64, height=48, ntris=40, seed=820 + k, pixelOps=3,
                                    pixFormat=7, pixType=0, points=120, pointSize=3.0, lines=40, lineWidth=2.0,
                                    rects=3, fog=1, fogStart=-0.5, fogEnd=0.8, fogAlpha=200, blend=1, blendMode=1,
                                    depthFunc=pf.PF_LEQUAL, fbFormat=_FORMATS[f], fbType=_TYPES[t]))
-    for k, (f, t) in enumerate(_TARGETS)
+    for k, (f, t) in enumerate(_TARGETS + _WIDE_TARGETS)
 ] + [
     (f"target_{f}_{t}_main", dict(kind=pf.RANDOM_3D, width=41, height=29, ntris=100, seed=840 + k,
                                   fbFormat=_FORMATS[f], fbType=_TYPES[t], fbMain=1))
-    for k, (f, t) in enumerate(_TARGETS + [("rgba", "ubyte")])
+    for k, (f, t) in enumerate(_TARGETS + [("rgba", "ubyte")] + _WIDE_TARGETS)
 ]
 
 # Golden frames rendered by the reference with extra scene parameters.

@@ -32,7 +32,8 @@ def test_gpu_matches_oracle(name, kw):
 
 
 CONFIG_SWEEP = ["c1_ndc_less", "c2_heightfield_nearest", "c3_spheres_phong", "blend_1", "clip_3d",
-                "odd_size_clear", "multi_frame", "bilinear_0", "target_rgb_ubyte_raster", "target_bgra_u4444_raster"]
+                "odd_size_clear", "multi_frame", "bilinear_0", "target_rgb_ubyte_raster", "target_bgra_u4444_raster",
+                "target_lumalpha_half_raster", "target_bgr_float_raster", "target_bgra_half_raster"]
 
 
 @pytest.mark.parametrize("tile_shift", [3, 4, 5, 6])

@@ -61,13 +61,15 @@ def test_error_codes_are_sticky_and_cleared():
 
 
 def test_unsupported_formats_are_rejected():
-    """Render targets: the nine pairs whose reference accessors touch only
-    their own pixel (RGBA, RGB and BGR bytes; RGB/BGR 5_6_5; RGBA/BGRA
-    5_5_5_1 and 4_4_4_4); every other pair is PF_INVALID_ENUM for
+    """Render targets: the nineteen pairs whose reference accessors touch
+    only their own pixel (RGBA, RGB and BGR bytes; RGB/BGR 5_6_5; RGBA/BGRA
+    5_5_5_1 and 4_4_4_4; luminance-alpha, RGB, BGR, RGBA and BGRA in half
+    and float components); every other pair is PF_INVALID_ENUM for
     pfCreateContext and pfGenFramebuffer."""
     L = pf.lib()
-    buf = np.zeros(64, np.uint32)
+    buf = np.zeros(8 * 8 * 4 + 64, np.uint32)
     targets = {(7, 0), (6, 0), (8, 0), (6, 2), (8, 2), (7, 3), (9, 3), (7, 4), (9, 4)}
+    targets |= {(f, t) for f in (5, 6, 7, 8, 9) for t in (9, 10)}
     accepted = set()
     for f in range(10):
         for t in range(12):
@@ -82,7 +84,7 @@ def test_unsupported_formats_are_rejected():
     L2, ctx, _ = _ctx()
     L2.pfGenFramebuffer.restype = FB
     try:
-        for f, t in [(9, 0), (4, 0), (7, 10), (6, 9), (0, 0)]:
+        for f, t in [(9, 0), (4, 0), (4, 9), (0, 10), (5, 0), (0, 0)]:
             fb = L2.pfGenFramebuffer(8, 8, f, t)
             assert not fb.texture and L2.pfGetError() == 1
         for f, t in sorted(targets):

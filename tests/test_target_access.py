@@ -42,8 +42,23 @@ def test_target_getters_closed_forms():
     for k, (f, t) in enumerate(mt.TARGETS):
         src = mt.buffer_bytes(f, t, 100 + k)
         rgba = ours[f"read_rgba_{f}_{t}"].reshape(-1, 4)
-        assert (rgba[:, 3] == 255).all() or t in (3, 4)
-        if t == 0:
+        assert (rgba[:, 3] == 255).all() or t in (3, 4) or f in (5, 7, 9)
+        if t in (9, 10):
+            # (PFubyte)(v * 255) as x86-64 converts: truncation toward zero
+            # to a 32-bit integer, low byte kept; infinities and NaN give the
+            # integer 0x80000000, so byte 0.  Half subnormals read as zero.
+            v = src.view(np.float16 if t == 9 else np.float32).astype(np.float32)
+            if t == 9:
+                v[np.abs(v) < np.float32(2.0 ** -14)] = 0.0
+            with np.errstate(invalid="ignore", over="ignore"):
+                p = v * np.float32(255)
+                byte = np.where(np.isfinite(p), np.trunc(np.nan_to_num(p)), 0).astype(np.int64) & 255
+            c = list(byte.reshape(-1, mt.comps(f)).T) + [255]
+            want = {5: (0, 0, 0, 1), 6: (0, 1, 2, -1), 8: (2, 1, 0, -1), 7: (0, 1, 2, 3), 9: (2, 1, 0, 3)}[f]
+            want = [c[j] for j in want]
+            for ch in range(4):
+                assert np.array_equal(rgba[:, ch], np.broadcast_to(want[ch], rgba[:, ch].shape)), (f, t, ch)
+        elif t == 0:
             px = src.reshape(-1, 3)
             want = px if f == 6 else px[:, ::-1]
             assert np.array_equal(rgba[:, :3], want)
@@ -78,6 +93,20 @@ def test_target_setters_closed_forms():
         (7, 4): (rnd(r, 15) << 12 | rnd(g, 15) << 8 | rnd(b, 15) << 4 | rnd(a, 15)).to_bytes(2, "little"),
     }
     want[(9, 3)], want[(9, 4)] = want[(7, 3)], want[(7, 4)]
+    # Half and float pairs: the channels times (float)(1/255); luminance
+    # 0.299 r + 0.587 g + 0.114 b of those in float; BGR reversed, BGRA in
+    # RGBA order.  Halves round to nearest (none of these values is a tie,
+    # where the reference's conversion rounds up).
+    inv = np.float32(1 / 255)
+    n = {ch: np.float32(v) * inv for ch, v in zip("rgba", (r, g, b, a))}
+    grey = n["r"] * np.float32(0.299) + n["g"] * np.float32(0.587) + n["b"] * np.float32(0.114)
+    order = {5: [grey, n["a"]], 6: [n["r"], n["g"], n["b"]], 8: [n["b"], n["g"], n["r"]],
+             7: [n["r"], n["g"], n["b"], n["a"]], 9: [n["r"], n["g"], n["b"], n["a"]]}
+    for f, vals in order.items():
+        v = np.array(vals, np.float32)
+        assert not ((v.view(np.uint32) & 0x1FFF) == 0x1000).any()
+        want[(f, 9)] = v.astype(np.float16).tobytes()
+        want[(f, 10)] = v.tobytes()
     for (f, t), w in want.items():
         bpp = mt.bpp(f, t)
         px = ours[f"fb_{f}_{t}"][3 * bpp:4 * bpp].tobytes()      # pixel (3, 0)
