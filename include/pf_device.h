@@ -62,6 +62,8 @@ typedef struct {
     uint64_t deviceBytes;      /* device memory held by the job               */
     uint32_t maxTileList;      /* longest per-tile pair list (last pass)      */
     uint32_t busyTiles;        /* tiles with a non-empty list (last pass)     */
+    uint32_t clippedTriangles; /* recorded triangles that needed clipping     */
+    uint32_t reserved;
 } PFDstats;
 
 /* Per-kernel timing of one run, in milliseconds (HIP events on the job's

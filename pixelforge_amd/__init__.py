@@ -78,7 +78,8 @@ class PFDstats(C.Structure):
     _fields_ = [("rasterTriangles", C.c_uint64), ("tilePairs", C.c_uint64),
                 ("fragments", C.c_uint64), ("inputTriangles", C.c_uint64),
                 ("numTiles", C.c_uint32), ("tileShift", C.c_uint32),
-                ("deviceBytes", C.c_uint64), ("maxTileList", C.c_uint32), ("busyTiles", C.c_uint32)]
+                ("deviceBytes", C.c_uint64), ("maxTileList", C.c_uint32), ("busyTiles", C.c_uint32),
+                ("clippedTriangles", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class PFDtimes(C.Structure):

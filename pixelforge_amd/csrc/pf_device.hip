@@ -3,19 +3,23 @@
 //
 // Pipeline for one pass of a job (a job holds one or more recorded frames;
 // a pass holds at most one clear+draw segment per frame):
-//   k_setup<false>   one thread per recorded triangle: lighting (Gouraud),
-//                    projection, clipping, fan split, cull; counts the raster
-//                    triangles it emits                   (triangles.c:62-116, 157-349)
-//   scan             exclusive scan of the counts -> output slots (submission order)
-//   k_setup<true>    same work, writes the raster-triangle records and the
-//                    number of screen tiles each one overlaps
-//   scan             -> pair slots
-//   k_pair_emit      (tile, raster triangle) pairs in submission order
-//   radix sort       stable LSD sort of the pairs by tile, 8 bits per pass,
-//                    so each tile's list stays in submission order
+//   k_setup<0>       one thread per recorded triangle: projection, the
+//                    clip tests, fan split, cull; counts the raster triangles
+//                    and tile pairs each emits and lists the triangles that
+//                    need clipping                    (triangles.c:62-116, 157-349)
+//   k_clip_*<0>      the listed triangles, clipped: their counts
+//   k_scan_local     exclusive scan of the counts -> record and pair slots
+//                    (submission order)
+//   k_setup<1>       same work with lighting; writes the raster-triangle
+//   (+ k_clip_*<1>)  records and the (tile, raster triangle) pairs; the
+//                    clipped triangles are written beside it on a second
+//                    stream
+//   radix sort       stable LSD sort of the pairs by tile, 8 bits per pass
+//                    (histogram, column scan, scatter), so each tile's list
+//                    stays in submission order
 //   k_tile_bounds    per-tile [begin, end) of the sorted pairs
-//   k_raster         one workgroup per tile; the tile's RGBA8 colour and f32
-//                    depth live in VGPRs (one pixel per lane per 8x8 block),
+//   k_raster         one workgroup per tile; the tile's colour and depth
+//                    live in VGPRs (one pixel per lane per 8x8 block),
 //                    initialised by the fused clear or loaded once, updated
 //                    triangle by triangle in list order, written back once
 //                                                          (triangles.c:287-558)
@@ -944,17 +948,65 @@ __device__ __forceinline__ void transform_pos(CState* st, float4 p, float* h) {
     h[3] = st->mvp[3] * p.x + st->mvp[7] * p.y + st->mvp[11] * p.z + st->mvp[15] * p.w;
 }
 
+// Corner k of recorded triangle t for one culling pass: position, clip
+// position and (PHASE 1) colour, texture coordinate and normal, lit when
+// lighting is on (Process_ProjectAndClipTriangle's per-vertex part,
+// triangles.c:62-104).
+template <int PHASE, bool LIT>
+__device__ __forceinline__ Vtx corner_vertex(const SetupArgs& A, CState* st, uint32_t si, int face, uint32_t t, int k,
+                                             float4 p) {
+    Vtx v;
+    const size_t vi = (size_t)t * 3 + k;
+    v.pos[0] = p.x; v.pos[1] = p.y; v.pos[2] = p.z; v.pos[3] = p.w;
+    transform_pos(st, p, v.hom);
+    v.col = (PHASE == 1) ? A.col[vi] : 0u;
+    if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; v.uv[0] = q.x; v.uv[1] = q.y; }
+    else { v.uv[0] = v.uv[1] = 0.0f; }
+    if (PHASE == 1 && A.nrm) { v.nrm[0] = A.nrm[3 * vi]; v.nrm[1] = A.nrm[3 * vi + 1]; v.nrm[2] = A.nrm[3 * vi + 2]; }
+    else { v.nrm[0] = v.nrm[1] = v.nrm[2] = 0.0f; }
+    v.scr[0] = v.scr[1] = 0.0f;
+    v.bw[0] = k == 0 ? 1.0f : 0.0f;
+    v.bw[1] = k == 1 ? 1.0f : 0.0f;
+    v.bw[2] = k == 2 ? 1.0f : 0.0f;
+    if (PHASE == 1 && LIT && (st->flags & PFF_LIGHTING)) {                // triangles.c:78-104
+        const PFFstate& stg = A.states[si];
+        const float m0 = st->matNormal[0], m1 = st->matNormal[1], m2 = st->matNormal[2];
+        const float m4 = st->matNormal[4], m5 = st->matNormal[5], m6 = st->matNormal[6];
+        const float m8 = st->matNormal[8], m9 = st->matNormal[9], m10 = st->matNormal[10];
+        const float m12 = st->matNormal[12], m13 = st->matNormal[13], m14 = st->matNormal[14];
+        float nx = m0 * v.nrm[0] + m4 * v.nrm[1] + m8 * v.nrm[2] + m12;
+        float ny = m1 * v.nrm[0] + m5 * v.nrm[1] + m9 * v.nrm[2] + m13;
+        float nz = m2 * v.nrm[0] + m6 * v.nrm[1] + m10 * v.nrm[2] + m14;
+        v.nrm[0] = nx; v.nrm[1] = ny; v.nrm[2] = nz;
+        float sq = nx * nx + ny * ny + nz * nz;
+        if (sq != 0.0f) {
+            float inv = __fdiv_rn(1.0f, sqrt_cr(sq));
+            for (int i = 0; i < 3; i++) v.nrm[i] = v.nrm[i] * inv;
+        }
+        uint32_t dcol = st->material[face].diffuse, cc = 0;
+        for (int i = 0; i < 4; i++) cc |= (uint32_t)((ch(v.col, i) * ch(dcol, i)) / 255) << (8 * i);
+        v.col = cc;
+        if (st->lightMode == 0) {
+            float vr[3] = { st->viewRow2[0], st->viewRow2[1], st->viewRow2[2] };
+            float ndv = dot3(v.nrm, vr);
+            v.col = light_vertex(stg, stg.material[ndv < 0.0f ? 0 : 1], v.col, stg.viewPos, v.pos, v.nrm);
+        }
+    }
+    return v;
+}
+
 // All culling passes of recorded triangle t under state st.  PHASE 0 counts
 // raster triangles and tile pairs (geometry only); PHASE 1 also lights the
 // vertices and writes records and pairs from the given slots.  With CLIP
-// false a triangle that needs clipping is left alone and true is returned:
-// the caller runs it again with CLIP true, one lane at a time, clipping in
-// the wave's LDS buffers (no per-lane private arrays, so no scratch).
+// false a triangle that needs clipping is left alone and true is returned
+// (a clip kernel sets it up); with CLIP true the lane clips it itself in
+// the LDS buffers it is given (no per-lane private arrays, so no scratch).
 template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL>
 __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
                                                uint32_t t, uint32_t info, const float4* P, uint32_t rtBase,
-                                               uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs, CVtx<VF>* cpoly,
-                                               CVtx<VF>* ctmp, PairRect* rects = nullptr) {
+                                               uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs,
+                                               CVtx<VF>* cpoly = nullptr, CVtx<VF>* ctmp = nullptr,
+                                               PairRect* rects = nullptr) {
     // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
     // tests from the positions alone.  The clip planes are taken in the
     // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
@@ -990,55 +1042,13 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
     }
     if (!is2D && reject) return false;
     if (!CLIP && !is2D && !inside) return true;
-    const uint32_t flags = st->flags;
     const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
     for (int face = 0; face < 2; face++) {
         if (!(info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT))) continue;
         // Vertex attributes are (re)loaded per pass rather than kept live
         // across both; the positions come in P.
         Vtx v[3];
-        for (int k = 0; k < 3; k++) {
-            size_t vi = (size_t)t * 3 + k;
-            const float4 p = P[k];
-            v[k].pos[0] = p.x; v[k].pos[1] = p.y; v[k].pos[2] = p.z; v[k].pos[3] = p.w;
-            transform_pos(st, p, v[k].hom);
-            v[k].col = (PHASE == 1) ? A.col[vi] : 0u;
-            if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; v[k].uv[0] = q.x; v[k].uv[1] = q.y; }
-            else { v[k].uv[0] = v[k].uv[1] = 0.0f; }
-            if (PHASE == 1 && A.nrm) { v[k].nrm[0] = A.nrm[3 * vi]; v[k].nrm[1] = A.nrm[3 * vi + 1]; v[k].nrm[2] = A.nrm[3 * vi + 2]; }
-            else { v[k].nrm[0] = v[k].nrm[1] = v[k].nrm[2] = 0.0f; }
-            v[k].scr[0] = v[k].scr[1] = 0.0f;
-            v[k].bw[0] = k == 0 ? 1.0f : 0.0f;
-            v[k].bw[1] = k == 1 ? 1.0f : 0.0f;
-            v[k].bw[2] = k == 2 ? 1.0f : 0.0f;
-        }
-        if (PHASE == 1 && LIT && (flags & PFF_LIGHTING)) {              // triangles.c:78-104
-            const PFFstate& stg = A.states[si];
-            for (int k = 0; k < 3; k++) {
-                const float m0 = st->matNormal[0], m1 = st->matNormal[1], m2 = st->matNormal[2];
-                const float m4 = st->matNormal[4], m5 = st->matNormal[5], m6 = st->matNormal[6];
-                const float m8 = st->matNormal[8], m9 = st->matNormal[9], m10 = st->matNormal[10];
-                const float m12 = st->matNormal[12], m13 = st->matNormal[13], m14 = st->matNormal[14];
-                float nx = m0 * v[k].nrm[0] + m4 * v[k].nrm[1] + m8 * v[k].nrm[2] + m12;
-                float ny = m1 * v[k].nrm[0] + m5 * v[k].nrm[1] + m9 * v[k].nrm[2] + m13;
-                float nz = m2 * v[k].nrm[0] + m6 * v[k].nrm[1] + m10 * v[k].nrm[2] + m14;
-                v[k].nrm[0] = nx; v[k].nrm[1] = ny; v[k].nrm[2] = nz;
-                float sq = nx * nx + ny * ny + nz * nz;
-                if (sq != 0.0f) {
-                    float inv = __fdiv_rn(1.0f, sqrt_cr(sq));
-                    for (int i = 0; i < 3; i++) v[k].nrm[i] = v[k].nrm[i] * inv;
-                }
-                uint32_t dcol = st->material[face].diffuse, cc = 0;
-                for (int i = 0; i < 4; i++) cc |= (uint32_t)((ch(v[k].col, i) * ch(dcol, i)) / 255) << (8 * i);
-                v[k].col = cc;
-                if (st->lightMode == 0) {
-                    float vr[3] = { st->viewRow2[0], st->viewRow2[1], st->viewRow2[2] };
-                    float ndv = dot3(v[k].nrm, vr);
-                    v[k].col = light_vertex(stg, stg.material[ndv < 0.0f ? 0 : 1], v[k].col,
-                                            stg.viewPos, v[k].pos, v[k].nrm);
-                }
-            }
-        }
+        for (int k = 0; k < 3; k++) v[k] = corner_vertex<PHASE, LIT>(A, st, si, face, t, k, P[k]);
         uint32_t np = 0;
         if (is2D) {                                                       // "2D" path
             for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
@@ -1145,7 +1155,7 @@ __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, 
 // per block; PHASE 1 turns the block sums (scanned in between) and a
 // block scan of the counts into output slots and writes.  Triangles that
 // need clipping are only listed here (PHASE 0) and handed their slots
-// (PHASE 1, through cntRT/cntPairs); k_clip does their work.
+// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
 template <int PHASE, bool LIT>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     __shared__ uint32_t lds[64];
@@ -1196,8 +1206,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         todo &= ~__builtin_amdgcn_ballot_w64(mine);
         if (mine)
             deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, P, rtBase,
-                                                         pairBase, nRT, nPairs, (CVtx<VF_FULL>*)nullptr,
-                                                         (CVtx<VF_FULL>*)nullptr);
+                                                         pairBase, nRT, nPairs);
     }
     if (PHASE == 0) {
         // List the triangles to clip: one counter update per wave.
@@ -1224,7 +1233,7 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
 
 // Per triangle of the pass, the first raster-triangle and pair slot: the
 // block's scanned sums plus a scan within the block, in place over the
-// counts.  Both k_setup<1> and k_clip<1> then find their slots there, so
+// counts.  Both k_setup<1> and the clip kernel then find their slots there, so
 // the two run side by side.
 // FUSED: the block sums are still raw (k_scan_top did not run); each block
 // adds up those of the blocks before it itself (few blocks).
@@ -1248,15 +1257,13 @@ __global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
     if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
 }
 
-// The listed triangles that need clipping, one per lane, each lane
-// clipping in its own LDS area (so no per-lane private arrays: no
-// scratch), in the smallest vertex form the pass needs.  One wave per
-// workgroup; the list is walked with a grid stride.  PHASE 0 stores the
-// counts and adds them to the block sums; PHASE 1 writes from the slots
-// k_setup<1> handed over.
-template <int VF>
-constexpr int clip_words() { return 2 * 12 * (int)(sizeof(CVtx<VF>) / 4) + 1; }   // odd: lanes in distinct banks
-constexpr int CLIP_GRID = 1024;
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS writes of a wavefront made visible to its other lanes.
 
 // The wavefront writes one pair rectangle per lane (w x h tiles from tile0,
 // rows tilesX apart, pairs from slot with raster triangle rt) together, 64
@@ -1297,8 +1304,17 @@ __device__ __forceinline__ void write_pair_rects(const SetupArgs& A, uint32_t sl
     }
 }
 
+// The listed triangles that need clipping, one per lane (k_clip_lane), each lane
+// clipping in its own LDS area (so no per-lane private arrays: no
+// scratch), in the smallest vertex form the pass needs.  One wave per
+// workgroup; the list is walked with a grid stride.  PHASE 0 stores the
+// counts and adds them to the block sums; PHASE 1 writes from the slots
+// k_setup<1> handed over.
+template <int VF>
+constexpr int clip_words() { return 2 * 12 * (int)(sizeof(CVtx<VF>) / 4) + 1; }   // odd: lanes in distinct banks
+
 template <int PHASE, bool LIT, int VF>
-__global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
+__global__ __launch_bounds__(64) void k_clip_lane(SetupArgs A) {
     constexpr int CLIP_WORDS = clip_words<VF>();
     __shared__ uint32_t mem[64 * CLIP_WORDS];
     __shared__ PairRect rectMem[PHASE == 1 ? 64 * MAX_FAN_RECTS : 1];
@@ -1308,7 +1324,7 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
     PairRect* rects = PHASE == 1 ? &rectMem[lane * MAX_FAN_RECTS] : nullptr;
     const uint32_t n = min(*A.clipCount, A.clipCap);
     // Wavefront-uniform loop: the pair writing below needs every lane.
-    for (uint32_t i0 = blockIdx.x * 64; i0 < n; i0 += CLIP_GRID * 64) {
+    for (uint32_t i0 = blockIdx.x * 64; i0 < n; i0 += gridDim.x * 64) {
         const uint32_t i = i0 + (uint32_t)lane;
         const bool valid = i < n;
         uint32_t nRT = 0, nPairs = 0, rtBase = 0, pairBase = 0, tilesX = 0;
@@ -1344,6 +1360,160 @@ __global__ __launch_bounds__(64) void k_clip(SetupArgs A) {
                 write_pair_rects(A, slot, r.tile0, w, h, tilesX, rtBase + k);
                 slot += w * h;
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+// Inclusive prefix sum over the 16 lanes of a lane group (lanes 16g ..
+// 16g+15 of the wavefront); every lane of the group must take part.
+__device__ __forceinline__ uint32_t group_incl_scan(uint32_t v) {
+    const int gl = threadIdx.x & 15;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)v, d, 16);
+        if (gl >= d) v += o;
+    }
+    return v;
+}
+
+// The listed triangles that need clipping, 16 lanes per triangle (four per
+// wavefront; k_clip_group): lower latency per triangle than k_clip_lane,
+// lower throughput, so it takes passes that clip few triangles.  Per culling pass the group's lanes 0-2 build (and light) the
+// corners; the polygon is clipped plane by plane against w >= eps, then
+// +-x, +-y, +-z <= w (Sutherland-Hodgman, triangles.c:157-244) with lane i
+// taking vertex i and the edge from its predecessor: where an edge crosses
+// the plane the lane emits the intersection (lerp of the two, ictx.c:67-90)
+// and then, when inside, its vertex, at places given by a prefix sum of the
+// emissions over the group, which is the serial clipper's output order
+// (and its cap of 12 vertices).  A plane with every vertex inside leaves
+// the polygon as it is and is skipped.  The fan triangles (0, i+1, i+2)
+// are then set up one per lane, their record and pair slots from prefix
+// sums of the survivors.  PHASE 0 stores the counts and adds them to the
+// block sums; PHASE 1 writes from the slots k_scan_local handed over, the
+// pairs of the whole wavefront's fan triangles together.
+template <int PHASE, bool LIT, int VF>
+__global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
+    __shared__ CVtx<VF> mem[4][2][12];
+    const int lane = threadIdx.x, grp = lane >> 4, gl = lane & 15;
+    const uint32_t n = min(*A.clipCount, A.clipCap);
+    // Wavefront-uniform loop: the pair writing below needs every lane.
+    for (uint32_t i0 = blockIdx.x * 4; i0 < n; i0 += gridDim.x * 4) {
+        const uint32_t item = i0 + (uint32_t)grp;
+        const bool valid = item < n;
+        uint32_t gid = 0, t = 0, info = 0, si = 0, tilesX = 0;
+        DFrame F;
+        if (valid) {
+            // The listed triangle with its recorded index, info and frame.
+            const uint4 cr = A.clipList[item];
+            gid = cr.x; t = cr.y; info = cr.z; si = info & PFF_STATE_MASK;
+            F = A.frames[cr.w];
+            tilesX = F.tilesX;
+        }
+        CState* st = as_const(A.states + si);
+        const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
+        const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
+        uint32_t nRT = 0, nPairs = 0;
+        for (int face = 0; face < 2; face++) {
+            const bool act = valid && (info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT));
+            CVtx<VF>* in = mem[grp][0];
+            CVtx<VF>* out = mem[grp][1];
+            if (act && gl < 3)
+                cv_store(in[gl], corner_vertex<PHASE, LIT>(A, st, si, face, t, gl, A.pos[(size_t)t * 3 + gl]));
+            wave_sync();
+            uint32_t cnt = act ? 3u : 0u;                      // uniform over the group
+            for (int pl = 0; pl < 7; pl++) {
+                const int ax = (pl - 1) >> 1, side = (pl - 1) & 1;
+                const bool has = (uint32_t)gl < cnt;
+                const int prev = gl == 0 ? (int)cnt - 1 : gl - 1;
+                float cw = 0.0f, ca = 0.0f, pw = 0.0f, pa = 0.0f;
+                if (has) {
+                    const float* hc = cv_hom(in[gl]);
+                    const float* hp = cv_hom(in[prev]);
+                    cw = hc[3]; pw = hp[3];
+                    if (pl > 0) { ca = side ? -hc[ax] : hc[ax]; pa = side ? -hp[ax] : hp[ax]; }
+                }
+                const bool insC = pl == 0 ? !(cw < 1e-5f) : (ca <= cw);
+                const bool insP = pl == 0 ? !(pw < 1e-5f) : (pa <= pw);
+                const uint32_t outside = (uint32_t)(__ballot(has && !insC) >> (16 * grp)) & 0xFFFFu;
+                if (outside == 0u) continue;                   // uniform over the group
+                const int cd = insC ? 1 : -1, pd = insP ? 1 : -1;
+                const bool cross = has && (pl == 0 ? pd * cd < 0 : pd * cd <= 0);
+                const bool keep = has && insC;
+                const uint32_t e = (cross ? 1u : 0u) + (keep ? 1u : 0u);
+                const uint32_t incl = group_incl_scan(e);
+                const uint32_t at = incl - e;
+                const uint32_t total = (uint32_t)__shfl((int)incl, 15, 16);
+                if (cross && at < 12u) {
+                    const float* hc = cv_hom(in[gl]);
+                    const float* hp = cv_hom(in[prev]);
+                    float tt;
+                    if (pl == 0) tt = __fdiv_rn(1e-5f - hp[3], hc[3] - hp[3]);
+                    else {
+                        float num, den;
+                        if (!side) { num = hp[3] - hp[ax]; den = num - (hc[3] - hc[ax]); }
+                        else       { num = hp[3] + hp[ax]; den = num - (hc[3] + hc[ax]); }
+                        tt = __fdiv_rn(num, den);
+                    }
+                    cv_store(out[at], lerp_vtx(cv_load(in[prev]), cv_load(in[gl]), tt));
+                }
+                const uint32_t atK = at + (cross ? 1u : 0u);
+                if (keep && atK < 12u) out[atK] = in[gl];
+                wave_sync();
+                cnt = min(total, 12u);
+                CVtx<VF>* sw = in; in = out; out = sw;
+            }
+            // Perspective divide (triangles.c:113-115 through
+            // Process_ProjectAndClipTriangle's tail), one vertex per lane.
+            if (cnt >= 3u && (uint32_t)gl < cnt) {
+                Vtx q = cv_load(in[gl]);
+                q.hom[2] = __fdiv_rn(1.0f, q.hom[2]);
+                q.uv[0] = q.uv[0] * q.hom[2];
+                q.uv[1] = q.uv[1] * q.hom[2];
+                const float invW = __fdiv_rn(1.0f, q.hom[3]);
+                q.hom[0] *= invW;
+                q.hom[1] *= invW;
+                cv_store(in[gl], q);
+            }
+            wave_sync();
+            // Fan triangle gl: (0, gl+1, gl+2), in screen space.
+            const bool fan = cnt >= 3u && (uint32_t)gl < cnt - 2u;
+            const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
+            auto corner = [&](int k) {
+                Vtx q = cv_load(in[k]);
+                to_screen(stl, q);
+                return q;
+            };
+            uint32_t np = 0;
+            bool ok = false;
+            if (fan) {
+                const Vtx a = corner(0), b = corner(gl + 1), c = corner(gl + 2);
+                ok = emit_raster<0>(A, st, F, si, t, face, true, a, b, c, 0u, 0u, np);
+            }
+            if (!ok) np = 0;
+            const uint32_t rInc = group_incl_scan(ok ? 1u : 0u);
+            const uint32_t pInc = group_incl_scan(np);
+            const uint32_t rAt = nRT + rInc - (ok ? 1u : 0u), pAt = nPairs + pInc - np;
+            nRT += (uint32_t)__shfl((int)rInc, 15, 16);
+            nPairs += (uint32_t)__shfl((int)pInc, 15, 16);
+            if (PHASE == 1) {
+                PairRect r{ 0u, 0u };
+                if (ok) {
+                    const Vtx a = corner(0), b = corner(gl + 1), c = corner(gl + 2);
+                    uint32_t np2 = 0;
+                    if (!emit_raster<1>(A, st, F, si, t, face, true, a, b, c, rtBase + rAt, pairBase + pAt, np2, &r))
+                        r = PairRect{ 0u, 0u };
+                }
+                write_pair_rects(A, pairBase + pAt, r.tile0, r.wh & 0xFFFFu, r.wh >> 16, tilesX, rtBase + rAt);
+            }
+            wave_sync();                                   // the next pass rebuilds the corners
+        }
+        if (PHASE == 0 && valid && gl == 0) {
+            A.cntRT[gid] = nRT;
+            A.cntPairs[gid] = nPairs;
+            if (nRT) atomicAdd(&A.blockRT[gid / 256], nRT);
+            if (nPairs) atomicAdd(&A.blockPairs[gid / 256], nPairs);
         }
     }
 }
@@ -1390,19 +1560,20 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
 
 
 constexpr int RADIX_THREADS = 256;
-constexpr int RADIX_ROUNDS = 8;      // 2048 items per block: more blocks, shorter serial rounds
-constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ROUNDS;
 
-// Digits of DB bits: 2^DB buckets (DB = 8 is used).
-template <int DB>
+// Digits of DB bits: 2^DB buckets (DB = 8 is used).  A block sorts ROUNDS
+// x RADIX_THREADS consecutive items: more, smaller blocks fill the chip
+// when the pairs are few; fewer, larger ones keep the per-block work of
+// the column scan and the scatter's set-up small when they are many.
+template <int DB, int ROUNDS>
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
                                                               uint32_t nb, uint32_t* __restrict__ hist) {
     constexpr uint32_t ND = 1u << DB;
     __shared__ uint32_t h[ND];
     for (uint32_t d = threadIdx.x; d < ND; d += RADIX_THREADS) h[d] = 0;
     __syncthreads();
-    size_t base = (size_t)blockIdx.x * RADIX_TILE;
-    for (int r = 0; r < RADIX_ROUNDS; r++) {
+    size_t base = (size_t)blockIdx.x * (RADIX_THREADS * ROUNDS);
+    for (int r = 0; r < ROUNDS; r++) {
         size_t i = base + (size_t)r * RADIX_THREADS + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & (ND - 1u)], 1u);
     }
@@ -1438,7 +1609,7 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_colscan(uint32_t* __res
 // an item's rank among equal digits is its rank within its wave (ballot
 // match on the DB digit bits) plus the counts of lower waves and of earlier
 // rounds.
-template <int DB>
+template <int DB, int ROUNDS>
 __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
                                                                  uint32_t n, uint32_t shift, uint32_t nb,
                                                                  const uint32_t* __restrict__ colPrefix,
@@ -1466,8 +1637,8 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
         }
     }
     const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    size_t blockBase = (size_t)blockIdx.x * RADIX_TILE;
-    for (int r = 0; r < RADIX_ROUNDS; r++) {
+    size_t blockBase = (size_t)blockIdx.x * (RADIX_THREADS * ROUNDS);
+    for (int r = 0; r < ROUNDS; r++) {
         for (int w = 0; w < RADIX_THREADS / 64; w++)
             for (uint32_t d = threadIdx.x; d < ND; d += RADIX_THREADS) cnt[w][d] = 0;
         __syncthreads();
@@ -1612,11 +1783,6 @@ __device__ __forceinline__ uint32_t staged_meta(const QT& Q, int j) {
     return Q.rec[j][(ROT ? ((5 + j) & 7) : 5) * 4];
 }
 
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Edge values at (px, py); small says every coefficient fits 24 bits
 // (META_SMALL), which allows 24-bit multiplies.
@@ -3474,6 +3640,7 @@ struct PassInfo {
     uint32_t numPassTris = 0;
     uint32_t numRT = 0;        // known after the first run
     uint32_t numPairs = 0;
+    uint32_t numClip = 0;      // triangles the pass clips (first run)
     bool     sized = false;
     std::vector<uint32_t> blockSeg;   // per 256-triangle block: segment of its first triangle
     std::vector<DPixOp> pix;   // pfDrawPixels calls run after this pass's raster
@@ -3509,7 +3676,7 @@ static __device__ __align__(256) uint8_t scratch[PFD_SCRATCH_BYTES];
 // while keeping its memory: buffers only grow.
 struct PFDjob {
     hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;           // k_clip<1> beside k_setup<1> (default priority); may be null
+    hipStream_t side = nullptr;           // the clip kernel beside k_setup<1> (default priority); may be null
     hipEvent_t fork = nullptr, join = nullptr;
     Arena* arena = nullptr;               // non-null: buffers come from the arena, never from hipMalloc
     bool autoTiles = false;               // choose tile size and raster mode from the first counts
@@ -3817,6 +3984,39 @@ uint32_t auto_config(PFDjob* J, uint32_t numRT, uint32_t numPairs) {
 // are read back (synchronising) and scratch buffers are sized; otherwise
 // the sizes of the previous run are used and nothing synchronises, so the
 // pass can be captured in a graph.
+// The clip kernel of a pass for `clipped` listed triangles (an estimate
+// before the pass is sized; either kernel walks the whole list): lane
+// groups up to 16k triangles (1080p height field: 2.3k), one lane per
+// triangle beyond (256-context batch: about 75k).
+template <int PHASE, bool LIT, int VF>
+void launch_clip(const SetupArgs& sa, uint32_t clipped, hipStream_t s) {
+    if (clipped <= 16384u)
+        hipLaunchKernelGGL((k_clip_group<PHASE, LIT, VF>), dim3(std::max(1u, (clipped + 3) / 4)), dim3(64), 0, s, sa);
+    else
+        hipLaunchKernelGGL((k_clip_lane<PHASE, LIT, VF>), dim3(std::min(4096u, (clipped + 63) / 64)), dim3(64), 0, s, sa);
+}
+
+// Radix blocks of ROUNDS x 256 pairs, by measurement (binning time of
+// 0.36 M to 5.1 M pairs): blocks of 1,024 up to a million pairs, of 2,048
+// up to four million, of 4,096 beyond.
+uint32_t radix_rounds(uint32_t pairs) {
+    if (pairs <= 1u << 20) return 4;
+    if (pairs <= 4u << 20) return 8;
+    return 16;
+}
+
+// One histogram + column scan + scatter pass of the pair sort.
+template <int ROUNDS>
+void launch_radix_pass(PFDjob* J, const uint32_t* kIn, const uint32_t* vIn, uint32_t* kOut, uint32_t* vOut,
+                       uint32_t n, uint32_t shift, uint32_t rb) {
+    hipLaunchKernelGGL((k_radix_hist<8, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, n, shift, rb,
+                       (uint32_t*)J->hist.p);
+    hipLaunchKernelGGL(k_radix_colscan, dim3(256), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
+                       (uint32_t*)J->histScan.p);
+    hipLaunchKernelGGL((k_radix_scatter<8, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, n, shift,
+                       rb, (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
+}
+
 int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     int rc = 0;
     if (P.numPassTris == 0 && P.segs.empty()) return 0;
@@ -3870,23 +4070,27 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
     // Replays with few blocks leave the block sums raw for k_scan_local<true>.
     const bool fusedScan = !discover && nb <= 1024;
+    // Triangles to clip: counted once the pass is sized, estimated before.
+    const uint32_t clip0 = discover ? n / 64 : P.numClip;
     for (;;) {
         // Every pass leaves the clip-list counter at zero (k_tile_bounds, or
         // the fill at its end); a discovering run clears it here as well.
         if (discover)
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
         hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-        hipLaunchKernelGGL((k_clip<0, false, VF_POS>), dim3(CLIP_GRID), dim3(64), 0, J->stream, sa);
+        launch_clip<0, false, VF_POS>(sa, clip0, J->stream);
         if (!fusedScan)
             hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
                                (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
         HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
         if (!discover) break;
-        uint32_t tot[2] = {0, 0};
+        uint32_t tot[2] = {0, 0}, clipped = 0;
         HIP_TRY(hipMemcpyAsync(tot, J->scanTotal.p, 8, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
+        HIP_TRY(hipMemcpyAsync(&clipped, J->clipCount.p, 4, hipMemcpyDeviceToHost, J->stream), PFD_E_LAUNCH);
         HIP_TRY(hipStreamSynchronize(J->stream), PFD_E_LAUNCH);
         P.numRT = tot[0];
         P.numPairs = tot[1];
+        P.numClip = std::min(clipped, n);
         // A job with an automatic configuration chooses its tiles from the
         // first pass's counts, before anything is drawn: the counts are
         // taken again at the chosen size (the clip list is refilled).
@@ -3926,11 +4130,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
             HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
         }
         if (J->anyLit) {
-            hipLaunchKernelGGL((k_clip<1, true, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
+            launch_clip<1, true, VF_FULL>(sa, P.numClip, clipStream);
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
         } else {
-            if (J->diff) hipLaunchKernelGGL((k_clip<1, false, VF_FULL>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
-            else hipLaunchKernelGGL((k_clip<1, false, VF_SLIM>), dim3(CLIP_GRID), dim3(64), 0, clipStream, sa);
+            if (J->diff) launch_clip<1, false, VF_FULL>(sa, P.numClip, clipStream);
+            else launch_clip<1, false, VF_SLIM>(sa, P.numClip, clipStream);
             hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
         }
         if (J->side) {
@@ -3947,7 +4151,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     if (numPairs) {
         uint32_t bits = 1;
         while (bits < 32 && (1u << bits) < J->numTiles) bits++;
-        const uint32_t rb = (numPairs + RADIX_TILE - 1) / RADIX_TILE;
+        const uint32_t rounds = radix_rounds(numPairs);
+        const uint32_t rb = (numPairs + RADIX_THREADS * rounds - 1) / (RADIX_THREADS * rounds);
         const uint32_t nh = rb * 256;
         if ((rc = ensure(J, J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J, J->histScan, (size_t)nh * 4 + 4))) return rc;
         uint32_t* kIn = keys; uint32_t* vIn = vals;
@@ -3955,12 +4160,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         // 8-bit digits (10-bit ones, which would sort 17..20-bit keys in two
         // passes, measured slower than a third 8-bit pass).
         for (uint32_t shift = 0; shift < bits; shift += 8) {
-            hipLaunchKernelGGL(k_radix_hist<8>, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, numPairs, shift, rb,
-                               (uint32_t*)J->hist.p);
-            hipLaunchKernelGGL(k_radix_colscan, dim3(256), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
-                               (uint32_t*)J->histScan.p);
-            hipLaunchKernelGGL(k_radix_scatter<8>, dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, numPairs, shift,
-                               rb, (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
+            switch (rounds) {
+            case 4:  launch_radix_pass<4>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            case 8:  launch_radix_pass<8>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            default: launch_radix_pass<16>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            }
             std::swap(kIn, kOut);
             std::swap(vIn, vOut);
         }
@@ -4953,7 +5157,11 @@ int pfdJobStats(PFDjob* J, PFDstats* st) {
         for (uint32_t v : tf) frags += v;
     }
     st->fragments = frags;
-    for (auto& P : J->passes) { st->rasterTriangles += P.numRT; st->tilePairs += P.numPairs; }
+    for (auto& P : J->passes) {
+        st->rasterTriangles += P.numRT;
+        st->tilePairs += P.numPairs;
+        st->clippedTriangles += P.numClip;
+    }
     st->inputTriangles = J->numIn;
     st->numTiles = J->numTiles;
     st->tileShift = J->tileShift;

@@ -299,7 +299,16 @@ SIZE_SCENES = [
 GOLDEN_SCENES = (CONFIG_SCENES + FEATURE_SCENES + BLEND_SCENES + DEPTH_SCENES + WRAP_SCENES + ARRAY_SCENES
                  + LIST_SCENES + FORMAT_SCENES + PIXEL_SCENES + FOG_SCENES + POINT_SCENES + LINE_SCENES
                  + RECT_SCENES + BILINEAR_SCENES + TARGET_SCENES)
-ALL_SCENES = GOLDEN_SCENES + EDGE_READ_SCENES + SIZE_SCENES
+# Many triangles crossing the clip planes: random 3D triangles around the
+# camera, about half of them clipped.  The library clips a pass's listed
+# triangles 16 lanes per triangle up to 16k of them and one lane per
+# triangle beyond; these scenes take one path each (about 11k and 34k
+# clipped triangles).
+CLIP_LOAD_SCENES = [
+    ("clip_load_20k", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=20000, seed=77)),
+    ("clip_load_60k", dict(kind=pf.RANDOM_3D, width=64, height=48, ntris=60000, seed=77)),
+]
+ALL_SCENES = GOLDEN_SCENES + EDGE_READ_SCENES + SIZE_SCENES + CLIP_LOAD_SCENES
 
 
 def params(kw):

@@ -5,6 +5,7 @@ kernels (the default executor) and by the oracle.  Colour and depth must be
 bit-identical.  Where a reference golden exists the GPU frame must also
 equal it.
 """
+import ctypes as C
 import os
 
 import numpy as np
@@ -34,6 +35,25 @@ def test_gpu_matches_oracle(name, kw):
 CONFIG_SWEEP = ["c1_ndc_less", "c2_heightfield_nearest", "c3_spheres_phong", "blend_1", "clip_3d",
                 "odd_size_clear", "multi_frame", "bilinear_0", "target_rgb_ubyte_raster", "target_bgra_u4444_raster",
                 "target_lumalpha_half_raster", "target_bgr_float_raster", "target_bgra_half_raster"]
+
+
+def test_clip_load_scenes_take_both_clip_kernels():
+    """The two clip-load scenes clip fewer and more than 16k triangles, so
+    the oracle comparisons above ran both clip kernels."""
+    counts = []
+    for name, kw in scenes.CLIP_LOAD_SCENES:
+        cap = pf.capture(scenes.params(kw))
+        try:
+            L = pf.lib()
+            assert L.pfdJobUploadTargets(cap.job) == 0
+            assert L.pfdJobLaunch(cap.job) == 0 and L.pfdJobSync(cap.job) == 0
+            st = pf.PFDstats()
+            assert L.pfdJobStats(cap.job, C.byref(st)) == 0
+            counts.append(int(st.clippedTriangles))
+        finally:
+            cap.close()
+    print("clipped", counts, flush=True)
+    assert 4096 < counts[0] <= 16384 < counts[1]
 
 
 @pytest.mark.parametrize("tile_shift", [3, 4, 5, 6])
