@@ -812,8 +812,11 @@ constexpr int MAX_FAN_RECTS = 10;       // fan triangles of a polygon of at most
 
 // Builds (or only counts) the raster triangle of screen vertices a, b, c for
 // one culling pass (Rasterize_Triangle setup, triangles.c:287-349) and its
-// tile pairs.  Returns false when it is culled or covers no pixel.
-template <int PHASE>
+// tile pairs.  Returns false when it is culled or covers no pixel.  EXTRA
+// false: the job has no lit state and no float-domain records (no Phong
+// record, no RTriD), so the corners' positions, normals and weights are
+// not needed.
+template <int PHASE, bool EXTRA = true>
 __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, const DFrame& F,
                                             uint32_t stateIdx, uint32_t src, int face, bool is3D,
                                             const Vtx& a, const Vtx& b, const Vtx& c,
@@ -891,7 +894,7 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st->depthFunc & 7u) << META_DFUNC_SH);
     if (flags & PFF_BLEND) meta |= META_BLEND | ((st->blendMode & 7u) << META_BMODE_SH);
     if ((flags & PFF_TEXTURE_2D) && st->texture >= 0) meta |= META_TEX;
-    const bool phong = (flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
+    const bool phong = EXTRA && (flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
     if (phong) meta |= META_PHONG;
     r.meta = meta;
     r.state = stateIdx;
@@ -914,7 +917,7 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
         l.pad[0] = l.pad[1] = 0.0f;
         A.rtl[rtSlot] = l;
     }
-    if (A.rtd) {
+    if (EXTRA && A.rtd) {
         RTriD d;
         for (int j = 0; j < 3; j++) { d.M[j] = a.bw[j]; d.M[3 + j] = b.bw[j]; d.M[6 + j] = c.bw[j]; }
         d.src = src;
@@ -952,7 +955,7 @@ __device__ __forceinline__ void transform_pos(CState* st, float4 p, float* h) {
 // position and (PHASE 1) colour, texture coordinate and normal, lit when
 // lighting is on (Process_ProjectAndClipTriangle's per-vertex part,
 // triangles.c:62-104).
-template <int PHASE, bool LIT>
+template <int PHASE, bool LIT, bool EXTRA = true>
 __device__ __forceinline__ Vtx corner_vertex(const SetupArgs& A, CState* st, uint32_t si, int face, uint32_t t, int k,
                                              float4 p) {
     Vtx v;
@@ -962,7 +965,7 @@ __device__ __forceinline__ Vtx corner_vertex(const SetupArgs& A, CState* st, uin
     v.col = (PHASE == 1) ? A.col[vi] : 0u;
     if (PHASE == 1 && A.uv) { float2 q = A.uv[vi]; v.uv[0] = q.x; v.uv[1] = q.y; }
     else { v.uv[0] = v.uv[1] = 0.0f; }
-    if (PHASE == 1 && A.nrm) { v.nrm[0] = A.nrm[3 * vi]; v.nrm[1] = A.nrm[3 * vi + 1]; v.nrm[2] = A.nrm[3 * vi + 2]; }
+    if (PHASE == 1 && (LIT || EXTRA) && A.nrm) { v.nrm[0] = A.nrm[3 * vi]; v.nrm[1] = A.nrm[3 * vi + 1]; v.nrm[2] = A.nrm[3 * vi + 2]; }
     else { v.nrm[0] = v.nrm[1] = v.nrm[2] = 0.0f; }
     v.scr[0] = v.scr[1] = 0.0f;
     v.bw[0] = k == 0 ? 1.0f : 0.0f;
@@ -1001,7 +1004,7 @@ __device__ __forceinline__ Vtx corner_vertex(const SetupArgs& A, CState* st, uin
 // false a triangle that needs clipping is left alone and true is returned
 // (a clip kernel sets it up); with CLIP true the lane clips it itself in
 // the LDS buffers it is given (no per-lane private arrays, so no scratch).
-template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL>
+template <int PHASE, bool LIT, bool CLIP, int VF = VF_FULL, bool EXTRA = true>
 __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, uint32_t si, const DFrame& F,
                                                uint32_t t, uint32_t info, const float4* P, uint32_t rtBase,
                                                uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs,
@@ -1048,11 +1051,11 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
         // Vertex attributes are (re)loaded per pass rather than kept live
         // across both; the positions come in P.
         Vtx v[3];
-        for (int k = 0; k < 3; k++) v[k] = corner_vertex<PHASE, LIT>(A, st, si, face, t, k, P[k]);
+        for (int k = 0; k < 3; k++) v[k] = corner_vertex<PHASE, LIT, EXTRA>(A, st, si, face, t, k, P[k]);
         uint32_t np = 0;
         if (is2D) {                                                       // "2D" path
             for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
-            if (emit_raster<PHASE>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
             }
@@ -1068,7 +1071,7 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
                 v[k].hom[1] *= invW;
                 to_screen(stl, v[k]);
             }
-            if (emit_raster<PHASE>(A, st, F, si, t, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
                 nRT++;
                 nPairs += np;
             }
@@ -1156,7 +1159,7 @@ __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, 
 // block scan of the counts into output slots and writes.  Triangles that
 // need clipping are only listed here (PHASE 0) and handed their slots
 // (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
-template <int PHASE, bool LIT>
+template <int PHASE, bool LIT, bool EXTRA = true>
 __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     __shared__ uint32_t lds[64];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1205,8 +1208,8 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         const bool mine = valid && si == s0;
         todo &= ~__builtin_amdgcn_ballot_w64(mine);
         if (mine)
-            deferred = setup_triangle<PHASE, LIT, false>(A, as_const(A.states + s0), s0, F, t, info, P, rtBase,
-                                                         pairBase, nRT, nPairs);
+            deferred = setup_triangle<PHASE, LIT, false, VF_FULL, EXTRA>(A, as_const(A.states + s0), s0, F, t, info,
+                                                                         P, rtBase, pairBase, nRT, nPairs);
     }
     if (PHASE == 0) {
         // List the triangles to clip: one counter update per wave.
@@ -4135,7 +4138,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         } else {
             if (J->diff) launch_clip<1, false, VF_FULL>(sa, P.numClip, clipStream);
             else launch_clip<1, false, VF_SLIM>(sa, P.numClip, clipStream);
-            hipLaunchKernelGGL((k_setup<1, false>), dim3(nb), dim3(256), 0, J->stream, sa);
+            if (J->diff) hipLaunchKernelGGL((k_setup<1, false, true>), dim3(nb), dim3(256), 0, J->stream, sa);
+            else hipLaunchKernelGGL((k_setup<1, false, false>), dim3(nb), dim3(256), 0, J->stream, sa);
         }
         if (J->side) {
             HIP_TRY(hipEventRecord(J->join, J->side), PFD_E_LAUNCH);
