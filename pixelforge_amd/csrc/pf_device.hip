@@ -4081,7 +4081,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         if (discover)
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, J->stream, (uint32_t*)J->clipCount.p, 1u, 0u);
         hipLaunchKernelGGL((k_setup<0, false>), dim3(nb), dim3(256), 0, J->stream, sa);
-        launch_clip<0, false, VF_POS>(sa, clip0, J->stream);
+        if (discover || P.numClip) launch_clip<0, false, VF_POS>(sa, clip0, J->stream);
         if (!fusedScan)
             hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, J->stream,
                                (uint32_t*)J->blockRT.p, (uint32_t*)J->blockPairs.p, nb, (uint32_t*)J->scanTotal.p);
@@ -4122,26 +4122,30 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.capRT = numRT;
     sa.capPairs = numPairs;
     if (numRT) {
-        // Slots per triangle, then the clipped triangles on the side stream
-        // while the others are set up (both read the slots only).
+        // Slots per triangle, then the clipped triangles and the others set
+        // up (both read the slots only).  Many clipped triangles (one lane
+        // each) are clipped on the side stream beside k_setup<1>; a few (lane
+        // groups, a few microseconds) on the main stream before it, where the
+        // fork and join would cost more than they overlap (1080p frame:
+        // setup 65 -> 53 us); none, not at all.
         if (fusedScan) hipLaunchKernelGGL(k_scan_local<true>, dim3(nb), dim3(256), 0, J->stream, sa);
         else hipLaunchKernelGGL(k_scan_local<false>, dim3(nb), dim3(256), 0, J->stream, sa);
-        // A job without a side stream runs the clipper after k_setup<1>.
-        hipStream_t clipStream = J->side ? J->side : J->stream;
-        if (J->side) {
+        const bool useSide = J->side && P.numClip > 16384u;
+        hipStream_t clipStream = useSide ? J->side : J->stream;
+        if (useSide) {
             HIP_TRY(hipEventRecord(J->fork, J->stream), PFD_E_LAUNCH);
             HIP_TRY(hipStreamWaitEvent(J->side, J->fork, 0), PFD_E_LAUNCH);
         }
         if (J->anyLit) {
-            launch_clip<1, true, VF_FULL>(sa, P.numClip, clipStream);
+            if (P.numClip) launch_clip<1, true, VF_FULL>(sa, P.numClip, clipStream);
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
         } else {
-            if (J->diff) launch_clip<1, false, VF_FULL>(sa, P.numClip, clipStream);
-            else launch_clip<1, false, VF_SLIM>(sa, P.numClip, clipStream);
+            if (P.numClip && J->diff) launch_clip<1, false, VF_FULL>(sa, P.numClip, clipStream);
+            else if (P.numClip) launch_clip<1, false, VF_SLIM>(sa, P.numClip, clipStream);
             if (J->diff) hipLaunchKernelGGL((k_setup<1, false, true>), dim3(nb), dim3(256), 0, J->stream, sa);
             else hipLaunchKernelGGL((k_setup<1, false, false>), dim3(nb), dim3(256), 0, J->stream, sa);
         }
-        if (J->side) {
+        if (useSide) {
             HIP_TRY(hipEventRecord(J->join, J->side), PFD_E_LAUNCH);
             HIP_TRY(hipStreamWaitEvent(J->stream, J->join, 0), PFD_E_LAUNCH);
         }
