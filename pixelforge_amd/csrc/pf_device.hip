@@ -809,6 +809,11 @@ struct PairRect {
     uint32_t tile0, wh;                 // wh = w | h << 16
 };
 constexpr int MAX_FAN_RECTS = 10;       // fan triangles of a polygon of at most 12 vertices
+// The pair rectangles of the (at most two) raster triangles one lane of the
+// setup emits for an unclipped triangle, in emission order.
+struct LaneRects {
+    PairRect a, b;
+};
 
 // Builds (or only counts) the raster triangle of screen vertices a, b, c for
 // one culling pass (Rasterize_Triangle setup, triangles.c:287-349) and its
@@ -1009,7 +1014,7 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
                                                uint32_t t, uint32_t info, const float4* P, uint32_t rtBase,
                                                uint32_t pairBase, uint32_t& nRT, uint32_t& nPairs,
                                                CVtx<VF>* cpoly = nullptr, CVtx<VF>* ctmp = nullptr,
-                                               PairRect* rects = nullptr) {
+                                               PairRect* rects = nullptr, LaneRects* lr = nullptr) {
     // Process_ProjectAndClipTriangle (triangles.c:246-280): the 2D and clip
     // tests from the positions alone.  The clip planes are taken in the
     // clipper's order (w >= eps, x <= w, -x <= w, y.., z..); a plane that has
@@ -1053,9 +1058,14 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
         Vtx v[3];
         for (int k = 0; k < 3; k++) v[k] = corner_vertex<PHASE, LIT, EXTRA>(A, st, si, face, t, k, P[k]);
         uint32_t np = 0;
+        // With lr (PHASE 1) the pairs are handed back as rectangles for the
+        // caller to write.
+        PairRect rr{ 0u, 0u };
         if (is2D) {                                                       // "2D" path
             for (int k = 0; k < 3; k++) to_screen(stl, v[k]);
-            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, false, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np,
+                                          (PHASE == 1 && lr) ? &rr : nullptr)) {
+                if (PHASE == 1 && lr) { if (nRT == 0) lr->a = rr; else lr->b = rr; }
                 nRT++;
                 nPairs += np;
             }
@@ -1071,7 +1081,9 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
                 v[k].hom[1] *= invW;
                 to_screen(stl, v[k]);
             }
-            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np)) {
+            if (emit_raster<PHASE, EXTRA>(A, st, F, si, t, face, true, v[0], v[1], v[2], rtBase + nRT, pairBase + nPairs, np,
+                                          (PHASE == 1 && lr) ? &rr : nullptr)) {
+                if (PHASE == 1 && lr) { if (nRT == 0) lr->a = rr; else lr->b = rr; }
                 nRT++;
                 nPairs += np;
             }
@@ -1137,9 +1149,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
     return r;
 }
 
-// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
-// per block; PHASE 1 turns the block sums (scanned in between) and a
-// block-local scan into output slots and writes records and pairs.
 // Recorded triangle, state and frame of pass-triangle gid: the last
 // segment whose passTriBase <= gid, found forward from its block's first.
 __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, uint32_t& t, uint32_t& info,
@@ -1153,120 +1162,6 @@ __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, 
     info = A.info[t];
     si = info & PFF_STATE_MASK;
 }
-
-// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
-// per block; PHASE 1 turns the block sums (scanned in between) and a
-// block scan of the counts into output slots and writes.  Triangles that
-// need clipping are only listed here (PHASE 0) and handed their slots
-// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
-template <int PHASE, bool LIT, bool EXTRA = true>
-__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
-    __shared__ uint32_t lds[64];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = gid < A.numPassTris;
-    if (PHASE == 0) {
-        // Reset the tile ranges of this pass (the raster reads [begin, end)).
-        for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
-        // pfClear copies pixel 0 into the pixels past the last multiple of 8
-        // before any later draw (context.c:707-710).  The raster kernel of
-        // this pass may already be writing pixel 0 while other tiles clear
-        // their tail, so the tail takes pixel 0 as it was when the pass
-        // began: every earlier pass has finished by the time this runs.
-        if (blockIdx.x == 0)
-            for (uint32_t s = threadIdx.x; s < A.numSegs; s += blockDim.x) {
-                const DSeg sg = A.segs[s];
-                if (!sg.clear) continue;
-                const DFrame F = A.frames[sg.frame];
-                A.clearSnap[8 * sg.frame] = F.kind < PFF_TARGET_FIRST_WIDE ? tgt_load(F.color, F.kind, 0) : 0u;
-                A.clearSnap[8 * sg.frame + 1] = __float_as_uint(F.depth[0]);
-                uint8_t* snap = (uint8_t*)&A.clearSnap[8 * sg.frame + 2];
-                if (F.kind >= PFF_TARGET_FIRST_WIDE)
-                    for (uint32_t b = 0; b < F.bpp; b++) snap[b] = F.color[b];
-            }
-    }
-    // The triangle, its frame and positions are requested first, so their
-    // latency overlaps the block scans of PHASE 1.
-    uint32_t t = 0, info = 0, si = 0, frame = 0;
-    DFrame F;
-    float4 P[3] = { make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f) };
-    if (valid) {
-        pass_triangle(A, gid, t, info, si, F, frame);
-        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
-    }
-    // PHASE 1: the slots of this triangle's records and pairs (k_scan_local
-    // turned the counts into offsets).
-    const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
-    const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
-    uint32_t nRT = 0, nPairs = 0;
-    // One iteration per distinct state in the wavefront: the state is then
-    // uniform and read with scalar loads.
-    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
-    bool deferred = false;
-    while (todo) {
-        const int leader = __builtin_ctzll(todo);
-        const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
-        const bool mine = valid && si == s0;
-        todo &= ~__builtin_amdgcn_ballot_w64(mine);
-        if (mine)
-            deferred = setup_triangle<PHASE, LIT, false, VF_FULL, EXTRA>(A, as_const(A.states + s0), s0, F, t, info,
-                                                                         P, rtBase, pairBase, nRT, nPairs);
-    }
-    if (PHASE == 0) {
-        // List the triangles to clip: one counter update per wave.
-        const int lane = threadIdx.x & 63;
-        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
-        if (D) {
-            uint32_t base = 0;
-            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
-            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
-            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            // The list holds every triangle of the pass once the counter
-            // starts at zero; the bound keeps a counter left over by a failed
-            // run from writing past it.
-            const uint32_t at = base + __popcll(D & ltMask);
-            if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
-        }
-        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
-        uint32_t totR, totP;
-        block_excl_scan(nRT, lds, totR);
-        block_excl_scan(nPairs, lds, totP);
-        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
-    }
-}
-
-// Per triangle of the pass, the first raster-triangle and pair slot: the
-// block's scanned sums plus a scan within the block, in place over the
-// counts.  Both k_setup<1> and the clip kernel then find their slots there, so
-// the two run side by side.
-// FUSED: the block sums are still raw (k_scan_top did not run); each block
-// adds up those of the blocks before it itself (few blocks).
-template <bool FUSED>
-__global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
-    __shared__ uint32_t lds[64];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = gid < A.numPassTris;
-    uint32_t tot, baseR, baseP;
-    if (FUSED) {
-        uint32_t sr = 0, sp = 0;
-        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
-        (void)block_excl_scan(sr, lds, baseR);
-        (void)block_excl_scan(sp, lds, baseP);
-    } else {
-        baseR = A.blockRT[blockIdx.x];
-        baseP = A.blockPairs[blockIdx.x];
-    }
-    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
-    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
-    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
-}
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// LDS writes of a wavefront made visible to its other lanes.
 
 // The wavefront writes one pair rectangle per lane (w x h tiles from tile0,
 // rows tilesX apart, pairs from slot with raster triangle rt) together, 64
@@ -1368,6 +1263,150 @@ __global__ __launch_bounds__(64) void k_clip_lane(SetupArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+
+// The pairs of one pair rectangle written by its lane alone, in the same
+// order (rows of w tiles from tile0, tilesX apart).
+__device__ __forceinline__ void write_pairs_lane(const SetupArgs& A, uint32_t slot, const PairRect& r, uint32_t tilesX,
+                                                 uint32_t rt) {
+    const uint32_t w = r.wh & 0xFFFFu, h = r.wh >> 16;
+    uint32_t p = slot;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++, p++) {
+            if (p >= A.capPairs) return;
+            A.keys[p] = r.tile0 + y * tilesX + x;
+            A.vals[p] = rt;
+        }
+}
+
+// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
+// per block; PHASE 1 turns the block sums (scanned in between) and a
+// block scan of the counts into output slots and writes.  Triangles that
+// need clipping are only listed here (PHASE 0) and handed their slots
+// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
+template <int PHASE, bool LIT, bool EXTRA = true>
+__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    if (PHASE == 0) {
+        // Reset the tile ranges of this pass (the raster reads [begin, end)).
+        for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
+        // pfClear copies pixel 0 into the pixels past the last multiple of 8
+        // before any later draw (context.c:707-710).  The raster kernel of
+        // this pass may already be writing pixel 0 while other tiles clear
+        // their tail, so the tail takes pixel 0 as it was when the pass
+        // began: every earlier pass has finished by the time this runs.
+        if (blockIdx.x == 0)
+            for (uint32_t s = threadIdx.x; s < A.numSegs; s += blockDim.x) {
+                const DSeg sg = A.segs[s];
+                if (!sg.clear) continue;
+                const DFrame F = A.frames[sg.frame];
+                A.clearSnap[8 * sg.frame] = F.kind < PFF_TARGET_FIRST_WIDE ? tgt_load(F.color, F.kind, 0) : 0u;
+                A.clearSnap[8 * sg.frame + 1] = __float_as_uint(F.depth[0]);
+                uint8_t* snap = (uint8_t*)&A.clearSnap[8 * sg.frame + 2];
+                if (F.kind >= PFF_TARGET_FIRST_WIDE)
+                    for (uint32_t b = 0; b < F.bpp; b++) snap[b] = F.color[b];
+            }
+    }
+    // The triangle, its frame and positions are requested first, so their
+    // latency overlaps the block scans of PHASE 1.
+    uint32_t t = 0, info = 0, si = 0, frame = 0;
+    DFrame F;
+    float4 P[3] = { make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f) };
+    if (valid) {
+        pass_triangle(A, gid, t, info, si, F, frame);
+        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
+    }
+    // PHASE 1: the slots of this triangle's records and pairs (k_scan_local
+    // turned the counts into offsets).
+    const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
+    const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
+    uint32_t nRT = 0, nPairs = 0;
+    // One iteration per distinct state in the wavefront: the state is then
+    // uniform and read with scalar loads.
+    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+    bool deferred = false;
+    LaneRects lr{ { 0u, 0u }, { 0u, 0u } };
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
+        const bool mine = valid && si == s0;
+        todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        if (mine)
+            deferred = setup_triangle<PHASE, LIT, false, VF_FULL, EXTRA>(A, as_const(A.states + s0), s0, F, t, info,
+                                                                         P, rtBase, pairBase, nRT, nPairs, nullptr,
+                                                                         nullptr, nullptr, &lr);
+    }
+    if (PHASE == 1) {
+        // The tile pairs: each lane its own when all are few, else the
+        // wavefront together, 64 per step (a lane looping over a large
+        // triangle's tiles would hold the others: 800x600 frame of 10k
+        // large triangles, 510 pairs each).
+        const uint32_t na = (lr.a.wh & 0xFFFFu) * (lr.a.wh >> 16), nb = (lr.b.wh & 0xFFFFu) * (lr.b.wh >> 16);
+        if (__builtin_amdgcn_ballot_w64(na + nb > 16u)) {
+            write_pair_rects(A, pairBase, lr.a.tile0, lr.a.wh & 0xFFFFu, lr.a.wh >> 16, F.tilesX, rtBase);
+            write_pair_rects(A, pairBase + na, lr.b.tile0, lr.b.wh & 0xFFFFu, lr.b.wh >> 16, F.tilesX, rtBase + 1u);
+        } else {
+            write_pairs_lane(A, pairBase, lr.a, F.tilesX, rtBase);
+            write_pairs_lane(A, pairBase + na, lr.b, F.tilesX, rtBase + 1u);
+        }
+    }
+    if (PHASE == 0) {
+        // List the triangles to clip: one counter update per wave.
+        const int lane = threadIdx.x & 63;
+        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
+        if (D) {
+            uint32_t base = 0;
+            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
+            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
+            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            // The list holds every triangle of the pass once the counter
+            // starts at zero; the bound keeps a counter left over by a failed
+            // run from writing past it.
+            const uint32_t at = base + __popcll(D & ltMask);
+            if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
+        }
+        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
+        uint32_t totR, totP;
+        block_excl_scan(nRT, lds, totR);
+        block_excl_scan(nPairs, lds, totP);
+        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
+    }
+}
+
+// Per triangle of the pass, the first raster-triangle and pair slot: the
+// block's scanned sums plus a scan within the block, in place over the
+// counts.  Both k_setup<1> and the clip kernel then find their slots there, so
+// the two run side by side.
+// FUSED: the block sums are still raw (k_scan_top did not run); each block
+// adds up those of the blocks before it itself (few blocks).
+template <bool FUSED>
+__global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    uint32_t tot, baseR, baseP;
+    if (FUSED) {
+        uint32_t sr = 0, sp = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
+        (void)block_excl_scan(sr, lds, baseR);
+        (void)block_excl_scan(sp, lds, baseP);
+    } else {
+        baseR = A.blockRT[blockIdx.x];
+        baseP = A.blockPairs[blockIdx.x];
+    }
+    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
+    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
+    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS writes of a wavefront made visible to its other lanes.
 
 // Inclusive prefix sum over the 16 lanes of a lane group (lanes 16g ..
 // 16g+15 of the wavefront); every lane of the group must take part.
