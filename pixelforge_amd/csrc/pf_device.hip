@@ -796,6 +796,9 @@ struct SetupArgs {
     uint32_t        clipCap;        // entries clipList holds
     uint32_t*       clearSnap;      // per frame, 8 words: colour word and depth bits of pixel 0 when the
                                     // pass began, then (wide targets) its bytes
+    uint32_t*       pairOff;        // non-null: PHASE 1 writes, per raster triangle, its first pair
+    uint4*          pairRect;       // slot and {tile0, w | h << 16, tilesX}; k_expand_pairs writes
+                                    // the pairs
 };
 
 // Tiles [tx0..tx1] x [ty0..ty1] overlapped by the covered pixels of a
@@ -933,6 +936,14 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     // Tile pairs of this raster triangle, in submission order: handed back
     // as a rectangle (clipped triangles: large, few) or written here (the
     // rest: mostly a few pairs, where the per-lane loop is cheaper).
+    if (A.pairOff) {
+        // Many pairs per raster triangle: k_expand_pairs writes them.
+        A.pairOff[rtSlot] = pairSlot;
+        A.pairRect[rtSlot] = make_uint4(F.tileBase + ty0 * F.tilesX + tx0, (tx1 - tx0 + 1) | ((ty1 - ty0 + 1) << 16),
+                                        F.tilesX, 0u);
+        if (rect) *rect = PairRect{ 0u, 0u };
+        return true;
+    }
     if (rect) {
         *rect = PairRect{ F.tileBase + ty0 * F.tilesX + tx0, (tx1 - tx0 + 1) | ((ty1 - ty0 + 1) << 16) };
         return true;
@@ -1557,6 +1568,40 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
             if (nRT) atomicAdd(&A.blockRT[gid / 256], nRT);
             if (nPairs) atomicAdd(&A.blockPairs[gid / 256], nPairs);
         }
+    }
+}
+
+// The tile pairs of passes whose raster triangles average many pairs, from
+// the rectangles the setup recorded: pair i belongs to the last raster
+// triangle whose first pair is at or before i (first pairs ascend with the
+// raster triangles, each of which has at least one pair), and is the
+// (i - first)-th tile of its rectangle, rows of w from tile0.  Four
+// consecutive pairs per thread share one search.
+__global__ __launch_bounds__(256) void k_expand_pairs(const uint32_t* __restrict__ pairOff,
+                                                      const uint4* __restrict__ pairRect, uint32_t numRT,
+                                                      uint32_t numPairs, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+    const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
+    if (i0 >= numPairs) return;
+    uint32_t lo = 0, hi = numRT;                      // pairOff[lo] <= i0 < pairOff[hi] (hi: numPairs)
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pairOff[mid] <= i0) lo = mid; else hi = mid;
+    }
+    uint32_t rt = lo;
+    uint32_t first = pairOff[rt], next = rt + 1u < numRT ? pairOff[rt + 1u] : numPairs;
+    uint4 r = pairRect[rt];
+    for (uint32_t i = i0; i < min(i0 + 4u, numPairs); i++) {
+        while (i >= next) {                           // into the next raster triangle
+            rt++;
+            first = next;
+            next = rt + 1u < numRT ? pairOff[rt + 1u] : numPairs;
+            r = pairRect[rt];
+        }
+        const uint32_t q = i - first, w = r.y & 0xFFFFu;
+        const uint32_t qy = q / w;
+        keys[i] = r.x + qy * r.z + (q - qy * w);
+        vals[i] = rt;
     }
 }
 
@@ -3749,6 +3794,7 @@ struct PFDjob {
     // scratch
     DevBuf cnt, off, blockRT, blockPairs, rts, rtl, keysA, valsA, keysB, valsB;
     DevBuf scanBlocks, scanTotal, hist, histScan, tileBegin, tileEnd, tileFrags, dTileFrame, clipList, clipCount, clearSnap;
+    DevBuf pairOff, pairRect;             // per raster triangle, when the pass expands its pairs
     uint32_t sortedInB = 0;               // per-pass result buffer selection (last pass)
     // graph
     hipGraph_t graph = nullptr;
@@ -3778,7 +3824,7 @@ void for_each_buf(PFDjob* J, F f) {
                        &J->histScan, &J->tileBegin, &J->tileEnd, &J->tileFrags, &J->dTileFrame, &J->clipList,
                        &J->clipCount, &J->clearSnap, &J->rtd, &J->winner, &J->vid, &J->texParams, &J->colParams,
                        &J->lightParams, &J->image, &J->dImage, &J->gTex, &J->gCol, &J->gLightPart, &J->gLight,
-                       &J->arrPool, &J->arrTable };
+                       &J->arrPool, &J->arrTable, &J->pairOff, &J->pairRect };
     for (DevBuf* b : bufs) f(*b);
 }
 
@@ -4160,6 +4206,17 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.vals = (uint32_t*)J->valsA.p;
     sa.capRT = numRT;
     sa.capPairs = numPairs;
+    // Raster triangles of more than 32 tile pairs on average (large
+    // triangles: 800x600 frame of 10k, 510 each): the setup records one
+    // rectangle per raster triangle and k_expand_pairs writes the pairs,
+    // one thread per four, instead of the setup's lanes looping over them.
+    const bool expand = numRT && numPairs / numRT > 32u;
+    if (expand) {
+        if ((rc = ensure(J, J->pairOff, (size_t)numRT * 4 + 4)) || (rc = ensure(J, J->pairRect, (size_t)numRT * 16 + 16)))
+            return rc;
+        sa.pairOff = (uint32_t*)J->pairOff.p;
+        sa.pairRect = (uint4*)J->pairRect.p;
+    }
     if (numRT) {
         // Slots per triangle, then the clipped triangles and the others set
         // up (both read the slots only).  Many clipped triangles (one lane
@@ -4189,6 +4246,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
             HIP_TRY(hipStreamWaitEvent(J->stream, J->join, 0), PFD_E_LAUNCH);
         }
     }
+    if (expand)
+        hipLaunchKernelGGL(k_expand_pairs, dim3(grid_for((numPairs + 3) / 4, 256)), dim3(256), 0, J->stream,
+                           (const uint32_t*)J->pairOff.p, (const uint4*)J->pairRect.p, numRT, numPairs,
+                           (uint32_t*)J->keysA.p, (uint32_t*)J->valsA.p);
     HIP_TRY(hipGetLastError(), PFD_E_LAUNCH);
     if (ev) HIP_TRY(hipEventRecord(ev[1], J->stream), PFD_E_LAUNCH);
 
