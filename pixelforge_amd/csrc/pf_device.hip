@@ -1897,15 +1897,33 @@ __device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_
 
 // Colour and depth of one fragment of raster triangle t at pixel (px, py)
 // (triangles.c:422-432, 500-521).
-template <bool PHONG>
-__device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& t, const float* lrec,
-                                               int px, int py, uint32_t& outCol, float& outZ) {
+// The depth test of a fragment at depth z against the pixel's depth
+// (depth.h:80-124: NOTEQUAL behaves as EQUAL); true when there is none.
+__device__ __forceinline__ bool depth_ok(uint32_t meta, float z, float dep) {
+    if (!(meta & META_DEPTH)) return true;
+    switch ((meta >> META_DFUNC_SH) & 7u) {
+    case 0: case 1: return z == dep;
+    case 2: return z < dep;
+    case 3: return z <= dep;
+    case 4: return z > dep;
+    default: return z >= dep;
+    }
+}
+
+// EARLY: the depth test against `dep` first; a fragment that fails it is
+// discarded by the ROP, so its colour is not computed and false is
+// returned (the pixel's result is the same).
+template <bool PHONG, bool EARLY = false>
+__device__ __forceinline__ bool shade_fragment(const RasterArgs& A, const RTri& t, const float* lrec,
+                                               int px, int py, uint32_t& outCol, float& outZ, float dep = 0.0f) {
     int32_t w1, w2, w3;
     edge_values(t, px, py, w1, w2, w3);
     const float inv = t.wInvSum;
     float b1 = (float)w1 * inv, b2 = (float)w2 * inv, b3 = (float)w3 * inv;
     float zq = rcp_x86((t.z[0] * b1 + t.z[1] * b2) + t.z[2] * b3);
     const uint32_t meta = t.meta;
+    outZ = zq;
+    if (EARLY && !depth_ok(meta, zq, dep)) return false;
     uint32_t frag = (meta & META_SMOOTH) ? bary_smooth(t.c[0], t.c[1], t.c[2], b1, b2, b3)
                                          : bary_flat(t.c[0], t.c[1], t.c[2], b1, b2, b3);
     if (meta & META_TEX) {
@@ -1932,7 +1950,7 @@ __device__ __forceinline__ void shade_fragment(const RasterArgs& A, const RTri& 
         frag = light_fragment(st, st->material[(meta & META_BACK) ? 1 : 0], frag, P, N);
     }
     outCol = frag;
-    outZ = zq;
+    return true;
 }
 
 // Wide targets' SIMD accessors (defined with the texel decoders below).
@@ -1951,18 +1969,7 @@ __device__ __forceinline__ uint32_t wide_rt_simd(uint32_t kind, uint32_t c);
 template <bool WIDE>
 __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t kind, uint32_t& col,
                                     float& dep, uint32_t& win) {
-    if (meta & META_DEPTH) {
-        uint32_t fn = (meta >> META_DFUNC_SH) & 7u;
-        bool ok;
-        switch (fn) {
-        case 0: case 1: ok = z == dep; break;
-        case 2: ok = z < dep; break;
-        case 3: ok = z <= dep; break;
-        case 4: ok = z > dep; break;
-        default: ok = z >= dep; break;
-        }
-        if (!ok) return;
-    }
+    if (!depth_ok(meta, z, dep)) return;
     if (kind == PFF_TARGET_RGBA8) {
         if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
         col = c;
@@ -2181,9 +2188,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     if (cov) {
                         uint32_t fc;
                         float fz;
-                        shade_fragment<PHONG>(A, t, staged_light<PHONG>(Q, j), px, py[k],
-                                              fc, fz);
-                        rop<WIDE>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
+                        if (shade_fragment<PHONG, true>(A, t, staged_light<PHONG>(Q, j), px, py[k], fc, fz, dep[k]))
+                            rop<WIDE>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
                     }
                     continue;
                 }
