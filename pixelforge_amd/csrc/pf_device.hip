@@ -1912,12 +1912,13 @@ __device__ __forceinline__ bool depth_ok(uint32_t meta, float z, float dep) {
 
 // EARLY: the depth test against `dep` first; a fragment that fails it is
 // discarded by the ROP, so its colour is not computed and false is
-// returned (the pixel's result is the same).
-template <bool PHONG, bool EARLY = false>
+// returned (the pixel's result is the same).  W: the edge values at the
+// pixel are given (w1, w2, w3) rather than computed.
+template <bool PHONG, bool EARLY = false, bool W = false>
 __device__ __forceinline__ bool shade_fragment(const RasterArgs& A, const RTri& t, const float* lrec,
-                                               int px, int py, uint32_t& outCol, float& outZ, float dep = 0.0f) {
-    int32_t w1, w2, w3;
-    edge_values(t, px, py, w1, w2, w3);
+                                               int px, int py, uint32_t& outCol, float& outZ, float dep = 0.0f,
+                                               int32_t w1 = 0, int32_t w2 = 0, int32_t w3 = 0) {
+    if (!W) edge_values(t, px, py, w1, w2, w3);
     const float inv = t.wInvSum;
     float b1 = (float)w1 * inv, b2 = (float)w2 * inv, b3 = (float)w3 * inv;
     float zq = rcp_x86((t.z[0] * b1 + t.z[1] * b2) + t.z[2] * b3);
@@ -2188,7 +2189,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     if (cov) {
                         uint32_t fc;
                         float fz;
-                        if (shade_fragment<PHONG, true>(A, t, staged_light<PHONG>(Q, j), px, py[k], fc, fz, dep[k]))
+                        if (shade_fragment<PHONG, true, true>(A, t, staged_light<PHONG>(Q, j), px, py[k], fc, fz, dep[k],
+                                                              w1, w2, w3))
                             rop<WIDE>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
                     }
                     continue;
