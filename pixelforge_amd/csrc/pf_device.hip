@@ -1965,16 +1965,17 @@ __device__ __forceinline__ uint32_t wide_rt_simd(uint32_t kind, uint32_t c);
 
 // Depth test and blend of one fragment against the lane's pixel
 // (depth.h:80-124, blend.h:137-274, triangles.c:434-438, 523-529).
-// WIDE: the kernel may meet wide targets (only its instances compile their
-// path, which costs registers).
-template <bool WIDE>
+// TK, the targets a kernel instance meets: 0 RGBA8 only, 1 any kind of at
+// most 32 bits, 2 wide ones too (each instance compiles only the paths it
+// needs: the others cost registers and instructions).
+template <int TK>
 __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t id, uint32_t kind, uint32_t& col,
                                     float& dep, uint32_t& win) {
     if (!depth_ok(meta, z, dep)) return;
-    if (kind == PFF_TARGET_RGBA8) {
+    if (TK == 0 || kind == PFF_TARGET_RGBA8) {
         if (meta & META_BLEND) c = blend_op((meta >> META_BMODE_SH) & 7u, c, col);
         col = c;
-    } else if (WIDE && kind >= PFF_TARGET_FIRST_WIDE) {
+    } else if (TK == 2 && kind >= PFF_TARGET_FIRST_WIDE) {
         // col is the RGBA8 value of the pixel: the getter's reading of its
         // bytes while unwritten, else the colour last written, which the
         // target holds through the SIMD setter.
@@ -1998,7 +1999,7 @@ __device__ __forceinline__ void rop(uint32_t meta, uint32_t c, float z, uint32_t
 // Queued pixel p of a TW-wide block is (bx0 + p % TW, by0 + p / TW); lane
 // l owns pixels l + 64k, k < K, whose fragments sit in the queue slots set
 // in pm[k] (slots grow in submission order).
-template <bool PHONG, int K, bool ROT, bool WIDE>
+template <bool PHONG, int K, bool ROT, int TK>
 __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHONG>& Q, int bx0, int by0, int lane,
                                             int nf, uint64_t* pm, uint32_t kind, uint32_t* col, float* dep,
                                             uint32_t* win) {
@@ -2024,7 +2025,7 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
             const int s = __builtin_ctzll(m);
             m &= m - 1ull;
             const int j = Q.ftri[s];
-            rop<WIDE>(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], kind, col[k], dep[k], win[k]);
+            rop<TK>(staged_meta<ROT>(Q, j), Q.fcol[s], Q.fz[s], Q.recId[j], kind, col[k], dep[k], win[k]);
         }
     }
     wave_sync();
@@ -2041,7 +2042,7 @@ __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
 // 8x8 with one pixel per lane (K = 1) or 16x16 with four pixels per lane
 // (K = 4: lane l owns column l % 16 of rows l / 16 + 4k), so that one walk
 // of the list serves 256 pixels.  Returns the covered samples it saw.
-template <int MODE, bool PHONG, int K, bool WIDE>
+template <int MODE, bool PHONG, int K, int TK>
 __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFrame& F, const DTileFrame& sg,
                                                  uint32_t frame, int bx0, int by0, uint32_t e0, uint32_t e1,
                                                  WaveQueueT<PHONG>& Q) {
@@ -2067,7 +2068,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         win[k] = 0u;
         dep[k] = 0.0f;
         if (in[k]) {
-            const bool wide = WIDE && F.kind >= PFF_TARGET_FIRST_WIDE;
+            const bool wide = TK == 2 && F.kind >= PFF_TARGET_FIRST_WIDE;
             if (sg.clear && pi[k] >= 8 && pi[k] < aligned) {       // context.c:693-713
                 col[k] = sg.clearColor;                              // as the SIMD setter stores it
                 dep[k] = sg.clearDepth;
@@ -2078,7 +2079,8 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 dep[k] = __uint_as_float(A.clearSnap[8 * frame + 1]);
                 if (wide) win[k] = WIN_SNAP;
             } else {
-                col[k] = wide ? wide_get_simd(F.color, F.kind, pi[k]) : tgt_load(F.color, F.kind, pi[k]);
+                col[k] = wide ? wide_get_simd(F.color, F.kind, pi[k])
+                              : (TK == 0 ? reinterpret_cast<const uint32_t*>(F.color)[pi[k]] : tgt_load(F.color, F.kind, pi[k]));
                 dep[k] = F.depth[pi[k]];
             }
         }
@@ -2191,12 +2193,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                         float fz;
                         if (shade_fragment<PHONG, true, true>(A, t, staged_light<PHONG>(Q, j), px, py[k], fc, fz, dep[k],
                                                               w1, w2, w3))
-                            rop<WIDE>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
+                            rop<TK>(t.meta, fc, fz, Q.recId[j], F.kind, col[k], dep[k], win[k]);
                     }
                     continue;
                 }
                 if (nf + c > 64) {
-                    flush_queue<PHONG, K, ROT, WIDE>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
+                    flush_queue<PHONG, K, ROT, TK>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
                     nf = 0;
                 }
                 if (cov) {
@@ -2208,13 +2210,15 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 nf += c;
             }
         }
-        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT, WIDE>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
+        if (PM == 0 && nf > 0) flush_queue<PHONG, K, ROT, TK>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
         wave_sync();
     }
 #pragma unroll
     for (int k = 0; k < K; k++)
         if (in[k]) {
-            if (!WIDE || F.kind < PFF_TARGET_FIRST_WIDE) {
+            if (TK == 0) {
+                reinterpret_cast<uint32_t*>(F.color)[pi[k]] = col[k];
+            } else if (TK == 1 || F.kind < PFF_TARGET_FIRST_WIDE) {
                 tgt_store(F.color, F.kind, pi[k], col[k]);
             } else if (win[k] == WIN_SNAP) {                         // pixel 0's bytes at the clear
                 const uint32_t bpp = F.bpp;
@@ -2250,7 +2254,7 @@ __device__ __forceinline__ uint32_t xcd_chunked(uint32_t i, uint32_t n) {
 // (BW = 8 for K = 1, 16 for K = 4).  When a tile is one block, each wave
 // takes its own tile (four tiles per workgroup); otherwise one tile per
 // workgroup, its blocks dealt round-robin to the four waves.
-template <int TS, int MODE, bool PHONG, int K, bool WIDE>
+template <int TS, int MODE, bool PHONG, int K, int TK>
 __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     typedef WaveQueueT<PHONG> Queue;
     __shared__ Queue Qs[4];
@@ -2277,7 +2281,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterArgs A) {
     for (int b = (NB == 1 ? 0 : wave); b < NB; b += (NB == 1 ? 1 : 4)) {
         const int bx0 = tx * TS + (b % BPR) * BW, by0 = ty * TS + (b / BPR) * BW;
         if (bx0 >= (int)F.W || by0 >= (int)F.H) continue;
-        frags += raster_block<MODE, PHONG, K, WIDE>(A, F, sg, frame, bx0, by0, e0, e1, Q);
+        frags += raster_block<MODE, PHONG, K, TK>(A, F, sg, frame, bx0, by0, e0, e1, Q);
     }
     if (lead) *fragOut = frags;
 }
@@ -3795,6 +3799,7 @@ struct PFDjob {
     uint32_t numIn = 0;
     bool hasUV = false, hasNrm = false, anyPhong = false, anyLit = false;
     bool anyWide = false;                 // a frame's target is wide (PFF_TARGET_FIRST_WIDE on)
+    bool anyNarrow = false;               // a frame's target is of at most 32 bits but not RGBA8
     std::vector<PassInfo> passes;
     PFDconfig cfg{};
     uint32_t tileShift = 3;
@@ -4027,28 +4032,37 @@ uint32_t grid_for(uint64_t n, uint32_t block) {
     return g == 0 ? 1u : (uint32_t)g;
 }
 
-// Jobs with a wide target run the K = 1 instances that compile its path.
+// The raster kernel's TK for the job's targets.
+int target_class(const PFDjob* J) { return J->anyWide ? 2 : (J->anyNarrow ? 1 : 0); }
+
+// tk: the job's targets (the TK of the raster kernel); jobs with a target
+// other than RGBA8 run the K = 1 instances that compile its path.
 template <int TS, int MODE, int K>
-void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, bool wide, hipStream_t s) {
+void launch_raster(const RasterArgs& ra, uint32_t numTiles, bool phong, int tk, hipStream_t s) {
     constexpr int BW = (K == 1) ? 8 : 16;
     uint32_t grid = (TS == BW) ? (numTiles + 3) / 4 : numTiles;
-    if (wide) {
+    if (tk != 0) {
         const uint32_t g1 = (TS == 8) ? (numTiles + 3) / 4 : numTiles;
-        if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, 1, true>), dim3(g1), dim3(256), 0, s, ra);
-        else       hipLaunchKernelGGL((k_raster<TS, MODE, false, 1, true>), dim3(g1), dim3(256), 0, s, ra);
+        if (tk == 2) {
+            if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, 1, 2>), dim3(g1), dim3(256), 0, s, ra);
+            else       hipLaunchKernelGGL((k_raster<TS, MODE, false, 1, 2>), dim3(g1), dim3(256), 0, s, ra);
+        } else {
+            if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, 1, 1>), dim3(g1), dim3(256), 0, s, ra);
+            else       hipLaunchKernelGGL((k_raster<TS, MODE, false, 1, 1>), dim3(g1), dim3(256), 0, s, ra);
+        }
         return;
     }
-    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, K, false>), dim3(grid), dim3(256), 0, s, ra);
-    else       hipLaunchKernelGGL((k_raster<TS, MODE, false, K, false>), dim3(grid), dim3(256), 0, s, ra);
+    if (phong) hipLaunchKernelGGL((k_raster<TS, MODE, true, K, 0>), dim3(grid), dim3(256), 0, s, ra);
+    else       hipLaunchKernelGGL((k_raster<TS, MODE, false, K, 0>), dim3(grid), dim3(256), 0, s, ra);
 }
 
 template <int MODE, int K>
-void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, bool phong, bool wide, hipStream_t s) {
+void launch_raster_ts(const RasterArgs& ra, uint32_t shift, uint32_t numTiles, bool phong, int tk, hipStream_t s) {
     switch (shift) {
-    case 3: launch_raster<8, MODE, 1>(ra, numTiles, phong, wide, s); break;   // a 8x8 tile is one K = 1 block
-    case 4: launch_raster<16, MODE, K>(ra, numTiles, phong, wide, s); break;
-    case 5: launch_raster<32, MODE, K>(ra, numTiles, phong, wide, s); break;
-    default: launch_raster<64, MODE, K>(ra, numTiles, phong, wide, s); break;
+    case 3: launch_raster<8, MODE, 1>(ra, numTiles, phong, tk, s); break;   // a 8x8 tile is one K = 1 block
+    case 4: launch_raster<16, MODE, K>(ra, numTiles, phong, tk, s); break;
+    case 5: launch_raster<32, MODE, K>(ra, numTiles, phong, tk, s); break;
+    default: launch_raster<64, MODE, K>(ra, numTiles, phong, tk, s); break;
     }
 }
 
@@ -4309,10 +4323,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     ra.winner = J->diff ? (uint32_t*)J->winner.p : nullptr;
     ra.clearSnap = (const uint32_t*)J->clearSnap.p;
     switch (J->cfg.rasterMode) {
-    case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
-    case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
-    case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
-    default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, J->anyWide, J->stream); break;
+    case 1: launch_raster_ts<1, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, target_class(J), J->stream); break;
+    case 2: launch_raster_ts<0, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, target_class(J), J->stream); break;
+    case 3: launch_raster_ts<1, 4>(ra, J->tileShift, J->numTiles, J->anyPhong, target_class(J), J->stream); break;
+    default: launch_raster_ts<0, 1>(ra, J->tileShift, J->numTiles, J->anyPhong, target_class(J), J->stream); break;
     }
     if (!P.pix.empty())
         hipLaunchKernelGGL(k_draw_pixels, dim3(P.pixGrid, (uint32_t)P.pix.size()), dim3(256), 0, J->stream,
@@ -4533,7 +4547,7 @@ void job_reset(PFDjob* J) {
     J->passes.clear();
     J->numIn = 0;
     J->hasUV = J->hasNrm = J->anyPhong = J->anyLit = false;
-    J->anyWide = false;
+    J->anyWide = J->anyNarrow = false;
     J->numTiles = 0;
     J->autoTiles = false;
     J->sortedInB = 0;
@@ -4744,6 +4758,7 @@ int job_plan(PFDjob* J, const PFFframe* frames, uint32_t count, bool wantVid) {
         D.kind = fr.target;
         D.bpp = pff_target_bytes(fr.target);
         J->anyWide = J->anyWide || fr.target >= PFF_TARGET_FIRST_WIDE;
+        J->anyNarrow = J->anyNarrow || (fr.target != PFF_TARGET_RGBA8 && fr.target < PFF_TARGET_FIRST_WIDE);
         D.color = (uint8_t*)J->targetPool.p + targetOff;
         // 16 spare bytes: the wide targets' getters read whole words past a
         // pixel's last half component.
