@@ -4136,16 +4136,35 @@ uint32_t radix_rounds(uint32_t pairs) {
     return 16;
 }
 
-// One histogram + column scan + scatter pass of the pair sort.
-template <int ROUNDS>
+// One histogram + column scan + scatter pass of the pair sort, DB-bit
+// digits from bit `shift`.
+template <int DB, int ROUNDS>
 void launch_radix_pass(PFDjob* J, const uint32_t* kIn, const uint32_t* vIn, uint32_t* kOut, uint32_t* vOut,
                        uint32_t n, uint32_t shift, uint32_t rb) {
-    hipLaunchKernelGGL((k_radix_hist<8, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, n, shift, rb,
+    hipLaunchKernelGGL((k_radix_hist<DB, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, n, shift, rb,
                        (uint32_t*)J->hist.p);
-    hipLaunchKernelGGL(k_radix_colscan, dim3(256), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
+    hipLaunchKernelGGL(k_radix_colscan, dim3(1u << DB), dim3(RADIX_THREADS), 0, J->stream, (uint32_t*)J->hist.p, rb,
                        (uint32_t*)J->histScan.p);
-    hipLaunchKernelGGL((k_radix_scatter<8, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, n, shift,
+    hipLaunchKernelGGL((k_radix_scatter<DB, ROUNDS>), dim3(rb), dim3(RADIX_THREADS), 0, J->stream, kIn, vIn, n, shift,
                        rb, (const uint32_t*)J->hist.p, (const uint32_t*)J->histScan.p, kOut, vOut);
+}
+
+template <int ROUNDS>
+void launch_radix_pass_db(uint32_t db, PFDjob* J, const uint32_t* kIn, const uint32_t* vIn, uint32_t* kOut,
+                          uint32_t* vOut, uint32_t n, uint32_t shift, uint32_t rb) {
+    if (db == 9) launch_radix_pass<9, ROUNDS>(J, kIn, vIn, kOut, vOut, n, shift, rb);
+    else launch_radix_pass<8, ROUNDS>(J, kIn, vIn, kOut, vOut, n, shift, rb);
+}
+
+// Digit width of the sort's passes for keys of `bits` bits: as few passes
+// as digits of at most 9 bits allow, the bits split evenly, never below 8
+// bits.  Measured: 4K frame at 8x8 tiles (17-bit keys, 1.4 M pairs) two
+// 9-bit passes 67 us against three 8-bit ones 81 us; one 11-bit pass for
+// 800x600's 11-bit keys (5.1 M pairs) 198 us against two 8-bit ones 173;
+// 10 + 9 bits for the batch's 19-bit keys no faster than three passes.
+uint32_t radix_digit_bits(uint32_t bits) {
+    const uint32_t passes = (bits + 8) / 9;
+    return std::max(8u, (bits + passes - 1) / passes);
 }
 
 int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
@@ -4304,17 +4323,16 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         while (bits < 32 && (1u << bits) < J->numTiles) bits++;
         const uint32_t rounds = radix_rounds(numPairs);
         const uint32_t rb = (numPairs + RADIX_THREADS * rounds - 1) / (RADIX_THREADS * rounds);
-        const uint32_t nh = rb * 256;
-        if ((rc = ensure(J, J->hist, (size_t)nh * 4 + 4)) || (rc = ensure(J, J->histScan, (size_t)nh * 4 + 4))) return rc;
+        const uint32_t db = radix_digit_bits(bits);
+        const size_t nh = (size_t)rb << db;
+        if ((rc = ensure(J, J->hist, nh * 4 + 4)) || (rc = ensure(J, J->histScan, ((size_t)4 << db) + 4))) return rc;
         uint32_t* kIn = keys; uint32_t* vIn = vals;
         uint32_t* kOut = (uint32_t*)J->keysB.p; uint32_t* vOut = (uint32_t*)J->valsB.p;
-        // 8-bit digits (10-bit ones, which would sort 17..20-bit keys in two
-        // passes, measured slower than a third 8-bit pass).
-        for (uint32_t shift = 0; shift < bits; shift += 8) {
+        for (uint32_t shift = 0; shift < bits; shift += db) {
             switch (rounds) {
-            case 4:  launch_radix_pass<4>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
-            case 8:  launch_radix_pass<8>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
-            default: launch_radix_pass<16>(J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            case 4:  launch_radix_pass_db<4>(db, J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            case 8:  launch_radix_pass_db<8>(db, J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
+            default: launch_radix_pass_db<16>(db, J, kIn, vIn, kOut, vOut, numPairs, shift, rb); break;
             }
             std::swap(kIn, kOut);
             std::swap(vIn, vOut);
