@@ -97,3 +97,31 @@ def test_config3_reduced_size_equals_oracle():
     assert oe == 0 and ge == 0
     assert frags > 1_000_000
     assert_same(oc, od, gc, gd, "config 3 at 1920x1080, 96 spheres")
+
+
+def test_config1_pairs_expanded_on_replay():
+    """Config 1's raster triangles average hundreds of tile pairs, so its
+    passes record one tile rectangle per raster triangle and a separate
+    kernel writes the pairs; replayed as a graph (three times, targets
+    restored in between) at 16x16 tiles, in place, the frame equals the
+    reference's digest."""
+    name, kw = [(n, k) for n, k in scenes.FULL_SIZE_SCENES if n.startswith("c1")][0]
+    cap = pf.capture(scenes.params(kw))
+    try:
+        lib = pf.lib()
+        snap = cap.snapshot()
+        cfg = pf.PFDconfig()
+        cfg.tileShift, cfg.rasterMode = 4, 1
+        assert lib.pfdJobSetConfig(cap.job, C.byref(cfg)) == 0
+        for rep in range(3):
+            cap.restore(snap)
+            assert lib.pfdJobLaunch(cap.job) == 0 and lib.pfdJobSync(cap.job) == 0
+            assert lib.pfdJobDownload(cap.job) == 0
+            (c, d), = cap.targets()
+            assert scenes.digest(c, d) == (FULL[name]["color_sha256"], FULL[name]["depth_sha256"]), f"replay {rep}"
+        st = pf.PFDstats()
+        assert lib.pfdJobStats(cap.job, C.byref(st)) == 0
+        print("pairs per raster triangle", st.tilePairs / st.rasterTriangles, flush=True)
+        assert st.tilePairs > 32 * st.rasterTriangles
+    finally:
+        cap.close()
