@@ -2,7 +2,8 @@
 work.  Each replay starts from the targets as they were at capture time
 (restored on the host and uploaded), so each one must equal the oracle,
 not merely the previous replay.  Covers the config-2 feature set at fixture
-size, the full 1080p benchmark frame and the 256-context benchmark batch,
+size, scenes that take either clip kernel, the full 1080p benchmark frame
+and the 256-context benchmark batch,
 plus replays that run back to back without a restore, as the benchmark's
 timed loop does."""
 import numpy as np
@@ -32,7 +33,11 @@ def _replays(cap, n=REPLAYS):
     return out
 
 
-@pytest.mark.parametrize("name", ["c2_heightfield_bilinear", "c3_spheres_phong", "clip_3d", "odd_clear_pixel0_3d"])
+# clip_load_20k / clip_load_60k: the lane-group clipper on the main stream
+# and the per-lane clipper on the side stream (fork and join inside the
+# captured graph).
+@pytest.mark.parametrize("name", ["c2_heightfield_bilinear", "c3_spheres_phong", "clip_3d", "odd_clear_pixel0_3d",
+                                  "clip_load_20k", "clip_load_60k"])
 def test_fixture_scene_replays(name):
     p = scenes.params(dict(scenes.ALL_SCENES)[name])
     ref = render_oracle(p)[0]
