@@ -4177,22 +4177,29 @@ int upload_frames_table(PFDjob* J);
 uint32_t auto_config(PFDjob* J, uint32_t numRT, uint32_t numPairs) {
     uint32_t s = J->tileShift;
     double per = (double)numPairs / (double)(numRT ? numRT : 1);
-    while (s < 6 && per > 32.0) {
+    auto tilesAt = [&](uint32_t shift) {
         uint64_t tiles = 0;
         for (const DFrame& F : J->frames)
-            tiles += (uint64_t)((F.W + (2u << s) - 1) >> (s + 1)) * ((F.H + (2u << s) - 1) >> (s + 1));
-        if (tiles < 1024) break;
+            tiles += (uint64_t)((F.W + (1u << shift) - 1) >> shift) * ((F.H + (1u << shift) - 1) >> shift);
+        return tiles;
+    };
+    while (s < 6 && per > 32.0) {
+        if (tilesAt(s + 1) < 1024) break;
         s++;
+        per *= 0.25;
+    }
+    // The packed raster at 16x16 tiles stages each 8x8 quadrant's own list
+    // entries, which beats 8x8 tiles for small triangles too (1080p height
+    // field 0.177 against 0.189 ms, 4K spheres 0.625 against 0.660 ms),
+    // while the frames keep 1024 tiles.
+    if (s == 3 && tilesAt(4) >= 1024) {
+        s = 4;
         per *= 0.25;
     }
     J->cfg.rasterMode = per > 16.0 ? 1u : 0u;
     return s;
 }
 
-// Issues one pass.  With discover set, the raster-triangle and pair counts
-// are read back (synchronising) and scratch buffers are sized; otherwise
-// the sizes of the previous run are used and nothing synchronises, so the
-// pass can be captured in a graph.
 // The clip kernel of a pass for `clipped` listed triangles (an estimate
 // before the pass is sized; either kernel walks the whole list): lane
 // groups up to 16k triangles (1080p height field: 2.3k), one lane per
@@ -4245,6 +4252,10 @@ uint32_t radix_digit_bits(uint32_t bits) {
     return std::max(8u, (bits + passes - 1) / passes);
 }
 
+// Issues one pass.  With discover set, the raster-triangle and pair counts
+// are read back (synchronising) and scratch buffers are sized; otherwise
+// the sizes of the previous run are used and nothing synchronises, so the
+// pass can be captured in a graph.
 int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     int rc = 0;
     if (P.numPassTris == 0 && P.segs.empty()) return 0;
