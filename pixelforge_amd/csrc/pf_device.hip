@@ -762,6 +762,14 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
     return (const __attribute__((address_space(4))) T*)p;
 }
 typedef const __attribute__((address_space(4))) PFFstate CState;
+// emit_raster reads flags .. numLights as two aligned 16-byte vectors.
+static_assert(sizeof(PFFstate) % 16 == 0 && offsetof(PFFstate, flags) % 16 == 0, "state raster fields alignment");
+static_assert(offsetof(PFFstate, depthFunc) == offsetof(PFFstate, flags) + 4 &&
+              offsetof(PFFstate, blendMode) == offsetof(PFFstate, flags) + 8 &&
+              offsetof(PFFstate, shadeModel) == offsetof(PFFstate, flags) + 12 &&
+              offsetof(PFFstate, lightMode) == offsetof(PFFstate, flags) + 16 &&
+              offsetof(PFFstate, texture) == offsetof(PFFstate, flags) + 20 &&
+              offsetof(PFFstate, numLights) == offsetof(PFFstate, flags) + 24, "state raster fields order");
 
 struct SetupArgs {
     const float4*   pos;
@@ -863,7 +871,7 @@ struct LaneRects {
 // false: the job has no lit state and no float-domain records (no Phong
 // record, no RTriD), so the corners' positions, normals and weights are
 // not needed.
-template <int PHASE, bool EXTRA = true>
+template <int PHASE, bool EXTRA = true, bool PRELOAD = false>
 __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, const DFrame& F,
                                             uint32_t stateIdx, uint32_t src, int face, bool is3D,
                                             const Vtx& a, const Vtx& b, const Vtx& c,
@@ -904,6 +912,18 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     pairsOut = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
     if (PHASE == 0) return true;
     if (rtSlot >= A.capRT) return false;
+    // PRELOAD (the clip kernels, whose lanes hold different states): the
+    // state's raster fields, flags .. numLights, in two 16-byte loads issued
+    // here; read one by one where they are tested, they made a chain of
+    // dependent vector loads.  With a state uniform over the wavefront they
+    // are scalar loads and are read where used (fewer live registers).
+    typedef uint32_t Words4 __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(4))) Words4 CW4;
+    Words4 sf0 = { 0u, 0u, 0u, 0u }, sf1 = { 0u, 0u, 0u, 0u };
+    if (PRELOAD) {
+        sf0 = reinterpret_cast<CW4*>(&st->flags)[0];                 // flags, depthFunc, blendMode, shadeModel
+        sf1 = reinterpret_cast<CW4*>(&st->flags)[1];                 // lightMode, texture, numLights
+    }
     uint32_t Cx[3];
     for (int k = 0; k < 3; k++) Cx[k] = 0u - (ox[k] * Ax[k] + oy[k] * By[k]);
     // No int32 wrap of any edge within 64 px of the bbox (then a block of a
@@ -936,19 +956,21 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     }
     if (small) meta |= META_SMALL;
     if (nowrap) meta |= META_NOWRAP | (is3D ? META_EXACT : 0u);
-    const uint32_t flags = st->flags;
-    if (st->shadeModel == 1) meta |= META_SMOOTH;
-    if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | ((st->depthFunc & 7u) << META_DFUNC_SH);
-    if (flags & PFF_BLEND) meta |= META_BLEND | ((st->blendMode & 7u) << META_BMODE_SH);
-    if ((flags & PFF_TEXTURE_2D) && st->texture >= 0) meta |= META_TEX;
-    const bool phong = EXTRA && (flags & PFF_LIGHTING) && st->lightMode == 1 && st->numLights > 0;
+    const uint32_t flags = PRELOAD ? sf0.x : st->flags;
+    const int32_t texIdx = PRELOAD ? (int32_t)sf1.y : st->texture;
+    if ((PRELOAD ? sf0.w : st->shadeModel) == 1) meta |= META_SMOOTH;
+    if (flags & PFF_DEPTH_TEST) meta |= META_DEPTH | (((PRELOAD ? sf0.y : st->depthFunc) & 7u) << META_DFUNC_SH);
+    if (flags & PFF_BLEND) meta |= META_BLEND | (((PRELOAD ? sf0.z : st->blendMode) & 7u) << META_BMODE_SH);
+    if ((flags & PFF_TEXTURE_2D) && texIdx >= 0) meta |= META_TEX;
+    const bool phong = EXTRA && (flags & PFF_LIGHTING) && (PRELOAD ? sf1.x : st->lightMode) == 1 &&
+                       (PRELOAD ? sf1.z : st->numLights) > 0;
     if (phong) meta |= META_PHONG;
     r.meta = meta;
     r.state = stateIdx;
     r.uv[0] = a.uv[0]; r.uv[1] = a.uv[1]; r.uv[2] = b.uv[0]; r.uv[3] = b.uv[1]; r.uv[4] = c.uv[0]; r.uv[5] = c.uv[1];
     r.tex[0] = r.tex[1] = r.tex[2] = 0;
     if (meta & META_TEX) {
-        const DTex T = A.texs[st->texture];
+        const DTex T = A.texs[texIdx];
         const uint64_t addr = (uint64_t)(uintptr_t)T.pixels;
         r.tex[0] = (uint32_t)addr;
         r.tex[1] = ((uint32_t)(addr >> 32) & 0xFFFFu) | ((T.format & 1u) << 16) | ((T.wrap & 3u) << 17) | ((T.filter & 1u) << 19);
@@ -1511,6 +1533,9 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
         const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
         const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
         uint32_t nRT = 0, nPairs = 0;
+        // Read before the clipping, whose LDS fences would keep the loads
+        // behind them.
+        const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
         for (int face = 0; face < 2; face++) {
             const bool act = valid && (info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT));
             CVtx<VF>* in = mem[grp][0];
@@ -1575,7 +1600,6 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
             wave_sync();
             // Fan triangle gl: (0, gl+1, gl+2), in screen space.
             const bool fan = cnt >= 3u && (uint32_t)gl < cnt - 2u;
-            const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
             auto corner = [&](int k) {
                 Vtx q = cv_load(in[k]);
                 to_screen(stl, q);
@@ -1598,7 +1622,7 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
                 if (ok) {
                     const Vtx a = corner(0), b = corner(gl + 1), c = corner(gl + 2);
                     uint32_t np2 = 0;
-                    if (!emit_raster<1>(A, st, F, si, t, face, true, a, b, c, rtBase + rAt, pairBase + pAt, np2, &r))
+                    if (!emit_raster<1, true, true>(A, st, F, si, t, face, true, a, b, c, rtBase + rAt, pairBase + pAt, np2, &r))
                         r = PairRect{ 0u, 0u };
                 }
                 write_pair_rects(A, pairBase + pAt, r.tile0, r.wh & 0xFFFFu, r.wh >> 16, tilesX, rtBase + rAt, r.q);
