@@ -762,6 +762,9 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* as_const(c
     return (const __attribute__((address_space(4))) T*)p;
 }
 typedef const __attribute__((address_space(4))) PFFstate CState;
+// Four words loaded as one vector (a uint4 struct may be split into
+// narrower loads).
+typedef uint32_t Words4 __attribute__((ext_vector_type(4)));
 // emit_raster reads flags .. numLights as two aligned 16-byte vectors.
 static_assert(sizeof(PFFstate) % 16 == 0 && offsetof(PFFstate, flags) % 16 == 0, "state raster fields alignment");
 static_assert(offsetof(PFFstate, depthFunc) == offsetof(PFFstate, flags) + 4 &&
@@ -917,7 +920,6 @@ __device__ __forceinline__ bool emit_raster(const SetupArgs& A, CState* st, cons
     // here; read one by one where they are tested, they made a chain of
     // dependent vector loads.  With a state uniform over the wavefront they
     // are scalar loads and are read where used (fewer live registers).
-    typedef uint32_t Words4 __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(4))) Words4 CW4;
     Words4 sf0 = { 0u, 0u, 0u, 0u }, sf1 = { 0u, 0u, 0u, 0u };
     if (PRELOAD) {
@@ -1520,15 +1522,14 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
     for (uint32_t i0 = blockIdx.x * 4; i0 < n; i0 += gridDim.x * 4) {
         const uint32_t item = i0 + (uint32_t)grp;
         const bool valid = item < n;
-        uint32_t gid = 0, t = 0, info = 0, si = 0, tilesX = 0;
-        DFrame F;
-        if (valid) {
-            // The listed triangle with its recorded index, info and frame.
-            const uint4 cr = A.clipList[item];
-            gid = cr.x; t = cr.y; info = cr.z; si = info & PFF_STATE_MASK;
-            F = A.frames[cr.w];
-            tilesX = F.tilesX;
-        }
+        // The listed triangle with its recorded index, info and frame.  An
+        // idle group reads the wavefront's first item (i0 < n) and does
+        // nothing with it: loads outside a branch join no values, so the
+        // loads that depend on them are issued without waiting.
+        const Words4 cr = reinterpret_cast<const Words4*>(A.clipList)[valid ? item : i0];
+        const uint32_t gid = cr.x, t = cr.y, info = cr.z, si = info & PFF_STATE_MASK;
+        const DFrame F = A.frames[cr.w];
+        const uint32_t tilesX = valid ? F.tilesX : 0u;
         CState* st = as_const(A.states + si);
         const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
         const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
@@ -1536,12 +1537,16 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
         // Read before the clipping, whose LDS fences would keep the loads
         // behind them.
         const float stl[4] = { (float)st->vpPos[0], (float)st->vpPos[1], (float)st->vpDim[0], (float)st->vpDim[1] };
+        // Unlit corners are the same in both culling passes: built once, with
+        // their loads issued beside the state's.
+        Vtx unlit;
+        if (!LIT && valid && gl < 3) unlit = corner_vertex<PHASE, false>(A, st, si, 0, t, gl, A.pos[(size_t)t * 3 + gl]);
         for (int face = 0; face < 2; face++) {
             const bool act = valid && (info & (face ? PFF_PASS_BACK : PFF_PASS_FRONT));
             CVtx<VF>* in = mem[grp][0];
             CVtx<VF>* out = mem[grp][1];
             if (act && gl < 3)
-                cv_store(in[gl], corner_vertex<PHASE, LIT>(A, st, si, face, t, gl, A.pos[(size_t)t * 3 + gl]));
+                cv_store(in[gl], LIT ? corner_vertex<PHASE, LIT>(A, st, si, face, t, gl, A.pos[(size_t)t * 3 + gl]) : unlit);
             wave_sync();
             uint32_t cnt = act ? 3u : 0u;                      // uniform over the group
             for (int pl = 0; pl < 7; pl++) {
