@@ -1486,15 +1486,32 @@ __device__ __forceinline__ void wave_sync() {
 
 // LDS writes of a wavefront made visible to its other lanes.
 
-// Inclusive prefix sum over the 16 lanes of a lane group (lanes 16g ..
-// 16g+15 of the wavefront); every lane of the group must take part.
+// Lane groups are the DPP unit's rows (lanes 16g .. 16g+15 of the
+// wavefront): the scans below move values within a row by DPP, in the VALU,
+// instead of through LDS.  Every lane of the group must take part.
+// DPP controls: row_shr:n = 0x110 + n (lane i takes lane i-n of its row;
+// lanes i < n take `old`, here 0), row_ror:n = 0x120 + n (rotation within
+// the row).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_row(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// Inclusive prefix sum over the lane group.
 __device__ __forceinline__ uint32_t group_incl_scan(uint32_t v) {
-    const int gl = threadIdx.x & 15;
-#pragma unroll
-    for (int d = 1; d < 16; d <<= 1) {
-        const uint32_t o = (uint32_t)__shfl_up((int)v, d, 16);
-        if (gl >= d) v += o;
-    }
+    v += dpp_row<0x111>(v);
+    v += dpp_row<0x112>(v);
+    v += dpp_row<0x114>(v);
+    v += dpp_row<0x118>(v);
+    return v;
+}
+
+// Sum over the lane group, in every lane of it.
+__device__ __forceinline__ uint32_t group_sum(uint32_t v) {
+    v += dpp_row<0x128>(v);
+    v += dpp_row<0x124>(v);
+    v += dpp_row<0x122>(v);
+    v += dpp_row<0x121>(v);
     return v;
 }
 
@@ -1570,7 +1587,7 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
                 const uint32_t e = (cross ? 1u : 0u) + (keep ? 1u : 0u);
                 const uint32_t incl = group_incl_scan(e);
                 const uint32_t at = incl - e;
-                const uint32_t total = (uint32_t)__shfl((int)incl, 15, 16);
+                const uint32_t total = group_sum(e);
                 if (cross && at < 12u) {
                     const float* hc = cv_hom(in[gl]);
                     const float* hp = cv_hom(in[prev]);
@@ -1620,8 +1637,8 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
             const uint32_t rInc = group_incl_scan(ok ? 1u : 0u);
             const uint32_t pInc = group_incl_scan(np);
             const uint32_t rAt = nRT + rInc - (ok ? 1u : 0u), pAt = nPairs + pInc - np;
-            nRT += (uint32_t)__shfl((int)rInc, 15, 16);
-            nPairs += (uint32_t)__shfl((int)pInc, 15, 16);
+            nRT += group_sum(ok ? 1u : 0u);
+            nPairs += group_sum(np);
             if (PHASE == 1) {
                 PairRect r{ 0u, 0u };
                 if (ok) {
