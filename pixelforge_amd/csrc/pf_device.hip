@@ -1195,13 +1195,19 @@ __device__ __forceinline__ bool setup_triangle(const SetupArgs& A, CState* st, u
     return false;
 }
 
+// Inclusive prefix sum over the wavefront, by DPP in the VALU: row shifts
+// 1, 2, 4, 8 scan each row of 16 lanes (row_shr:n = 0x110 + n; lanes
+// shifted in from outside the row add 0), then row_bcast:15 (0x142) adds
+// the last lane of rows 0 and 2 to rows 1 and 3 (row mask 0xA) and
+// row_bcast:31 (0x143) lane 31 to rows 2 and 3 (row mask 0xC); rows a mask
+// leaves out add 0.  Every lane of the wavefront must take part.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
     return v;
 }
 
@@ -1849,13 +1855,15 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
 }
 
 // Also clears the clip-list counter for the next pass (both clip kernels of
-// this pass have read it by now).
-__global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n,
+// this pass have read it by now).  Keys are tiles of the pass; the bound
+// only keeps a key that is not from writing outside the ranges.
+__global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numTiles,
                               uint32_t* __restrict__ tileBegin, uint32_t* __restrict__ tileEnd,
                               uint32_t* __restrict__ clipCount) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *clipCount = 0u;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t k = keys[i];
+        if (k >= numTiles) continue;
         if (i == 0 || keys[i - 1] != k) tileBegin[k] = (uint32_t)i;
         if (i == n - 1 || keys[i + 1] != k) tileEnd[k] = (uint32_t)i + 1;
     }
@@ -4479,7 +4487,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         keys = kIn;
         vals = vIn;
         hipLaunchKernelGGL(k_tile_bounds, dim3(grid_for(numPairs, 256 * 4)), dim3(256), 0, J->stream,
-                           keys, numPairs, (uint32_t*)J->tileBegin.p, (uint32_t*)J->tileEnd.p,
+                           keys, numPairs, J->numTiles, (uint32_t*)J->tileBegin.p, (uint32_t*)J->tileEnd.p,
                            (uint32_t*)J->clipCount.p);
     }
     if (!numPairs)
