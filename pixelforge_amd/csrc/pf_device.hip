@@ -805,6 +805,8 @@ struct SetupArgs {
     uint4*          clipList;       // triangles to clip {gid, t, info, frame} (PHASE 0 appends)
     uint32_t*       clipCount;
     uint32_t        clipCap;        // entries clipList holds
+    uint32_t        clipKnown;      // listed triangles when the pass is sized (the clip
+                                    // kernels then skip reading clipCount), else ~0u
     uint32_t*       clearSnap;      // per frame, 8 words: colour word and depth bits of pixel 0 when the
                                     // pass began, then (wide targets) its bytes
     uint32_t*       pairOff;        // non-null: PHASE 1 writes, per raster triangle, its first pair
@@ -1303,7 +1305,7 @@ __global__ __launch_bounds__(64) void k_clip_lane(SetupArgs A) {
     CVtx<VF>* poly = reinterpret_cast<CVtx<VF>*>(&mem[lane * CLIP_WORDS]);
     CVtx<VF>* tmp = poly + 12;
     PairRect* rects = PHASE == 1 ? &rectMem[lane * MAX_FAN_RECTS] : nullptr;
-    const uint32_t n = min(*A.clipCount, A.clipCap);
+    const uint32_t n = min(A.clipKnown != ~0u ? A.clipKnown : *A.clipCount, A.clipCap);
     // Wavefront-uniform loop: the pair writing below needs every lane.
     for (uint32_t i0 = blockIdx.x * 64; i0 < n; i0 += gridDim.x * 64) {
         const uint32_t i = i0 + (uint32_t)lane;
@@ -1540,7 +1542,7 @@ template <int PHASE, bool LIT, int VF>
 __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
     __shared__ CVtx<VF> mem[4][2][12];
     const int lane = threadIdx.x, grp = lane >> 4, gl = lane & 15;
-    const uint32_t n = min(*A.clipCount, A.clipCap);
+    const uint32_t n = min(A.clipKnown != ~0u ? A.clipKnown : *A.clipCount, A.clipCap);
     // Wavefront-uniform loop: the pair writing below needs every lane.
     for (uint32_t i0 = blockIdx.x * 4; i0 < n; i0 += gridDim.x * 4) {
         const uint32_t item = i0 + (uint32_t)grp;
@@ -4358,6 +4360,8 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
     sa.clipList = (uint4*)J->clipList.p;
     sa.clipCount = (uint32_t*)J->clipCount.p;
     sa.clipCap = n;
+    // A sized pass lists the same triangles on every run (the same inputs).
+    sa.clipKnown = discover ? ~0u : P.numClip;
     sa.clearSnap = (uint32_t*)J->clearSnap.p;
 
     if (ev) HIP_TRY(hipEventRecord(ev[0], J->stream), PFD_E_LAUNCH);
