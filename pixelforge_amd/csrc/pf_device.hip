@@ -807,6 +807,7 @@ struct SetupArgs {
     uint32_t        clipCap;        // entries clipList holds
     uint32_t        clipKnown;      // listed triangles when the pass is sized (the clip
                                     // kernels then skip reading clipCount), else ~0u
+    uint32_t        clipBlocks;     // k_setup<1, .., CLIPW>: leading workgroups that clip
     uint32_t*       clearSnap;      // per frame, 8 words: colour word and depth bits of pixel 0 when the
                                     // pass began, then (wide targets) its bytes
     uint32_t*       pairOff;        // non-null: PHASE 1 writes, per raster triangle, its first pair
@@ -1363,136 +1364,12 @@ __device__ __forceinline__ void write_pairs_lane(const SetupArgs& A, uint32_t sl
         }
 }
 
-// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
-// per block; PHASE 1 turns the block sums (scanned in between) and a
-// block scan of the counts into output slots and writes.  Triangles that
-// need clipping are only listed here (PHASE 0) and handed their slots
-// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
-template <int PHASE, bool LIT, bool EXTRA = true>
-__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
-    __shared__ uint32_t lds[64];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = gid < A.numPassTris;
-    if (PHASE == 0) {
-        // Reset the tile ranges of this pass (the raster reads [begin, end)).
-        for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
-        // pfClear copies pixel 0 into the pixels past the last multiple of 8
-        // before any later draw (context.c:707-710).  The raster kernel of
-        // this pass may already be writing pixel 0 while other tiles clear
-        // their tail, so the tail takes pixel 0 as it was when the pass
-        // began: every earlier pass has finished by the time this runs.
-        if (blockIdx.x == 0)
-            for (uint32_t s = threadIdx.x; s < A.numSegs; s += blockDim.x) {
-                const DSeg sg = A.segs[s];
-                if (!sg.clear) continue;
-                const DFrame F = A.frames[sg.frame];
-                A.clearSnap[8 * sg.frame] = F.kind < PFF_TARGET_FIRST_WIDE ? tgt_load(F.color, F.kind, 0) : 0u;
-                A.clearSnap[8 * sg.frame + 1] = __float_as_uint(F.depth[0]);
-                uint8_t* snap = (uint8_t*)&A.clearSnap[8 * sg.frame + 2];
-                if (F.kind >= PFF_TARGET_FIRST_WIDE)
-                    for (uint32_t b = 0; b < F.bpp; b++) snap[b] = F.color[b];
-            }
-    }
-    // The triangle, its frame and positions are requested first, so their
-    // latency overlaps the block scans of PHASE 1.
-    uint32_t t = 0, info = 0, si = 0, frame = 0;
-    DFrame F;
-    float4 P[3] = { make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f) };
-    if (valid) {
-        pass_triangle(A, gid, t, info, si, F, frame);
-        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
-    }
-    // PHASE 1: the slots of this triangle's records and pairs (k_scan_local
-    // turned the counts into offsets).
-    const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
-    const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
-    uint32_t nRT = 0, nPairs = 0;
-    // One iteration per distinct state in the wavefront: the state is then
-    // uniform and read with scalar loads.
-    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
-    bool deferred = false;
-    LaneRects lr{ { 0u, 0u }, { 0u, 0u } };
-    while (todo) {
-        const int leader = __builtin_ctzll(todo);
-        const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
-        const bool mine = valid && si == s0;
-        todo &= ~__builtin_amdgcn_ballot_w64(mine);
-        if (mine)
-            deferred = setup_triangle<PHASE, LIT, false, VF_FULL, EXTRA>(A, as_const(A.states + s0), s0, F, t, info,
-                                                                         P, rtBase, pairBase, nRT, nPairs, nullptr,
-                                                                         nullptr, nullptr, &lr);
-    }
-    if (PHASE == 1) {
-        // The tile pairs: each lane its own when all are few, else the
-        // wavefront together, 64 per step (a lane looping over a large
-        // triangle's tiles would hold the others: 800x600 frame of 10k
-        // large triangles, 510 pairs each).
-        const uint32_t na = (lr.a.wh & 0xFFFFu) * (lr.a.wh >> 16), nb = (lr.b.wh & 0xFFFFu) * (lr.b.wh >> 16);
-        if (__builtin_amdgcn_ballot_w64(na + nb > 16u)) {
-            write_pair_rects(A, pairBase, lr.a.tile0, lr.a.wh & 0xFFFFu, lr.a.wh >> 16, F.tilesX, rtBase, lr.a.q);
-            write_pair_rects(A, pairBase + na, lr.b.tile0, lr.b.wh & 0xFFFFu, lr.b.wh >> 16, F.tilesX, rtBase + 1u,
-                             lr.b.q);
-        } else {
-            write_pairs_lane(A, pairBase, lr.a, F.tilesX, rtBase);
-            write_pairs_lane(A, pairBase + na, lr.b, F.tilesX, rtBase + 1u);
-        }
-    }
-    if (PHASE == 0) {
-        // List the triangles to clip: one counter update per wave.
-        const int lane = threadIdx.x & 63;
-        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
-        if (D) {
-            uint32_t base = 0;
-            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
-            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
-            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            // The list holds every triangle of the pass once the counter
-            // starts at zero; the bound keeps a counter left over by a failed
-            // run from writing past it.
-            const uint32_t at = base + __popcll(D & ltMask);
-            if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
-        }
-        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
-        uint32_t totR, totP;
-        block_excl_scan(nRT, lds, totR);
-        block_excl_scan(nPairs, lds, totP);
-        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
-    }
-}
-
-// Per triangle of the pass, the first raster-triangle and pair slot: the
-// block's scanned sums plus a scan within the block, in place over the
-// counts.  Both k_setup<1> and the clip kernel then find their slots there, so
-// the two run side by side.
-// FUSED: the block sums are still raw (k_scan_top did not run); each block
-// adds up those of the blocks before it itself (few blocks).
-template <bool FUSED>
-__global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
-    __shared__ uint32_t lds[64];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = gid < A.numPassTris;
-    uint32_t tot, baseR, baseP;
-    if (FUSED) {
-        uint32_t sr = 0, sp = 0;
-        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
-        (void)block_excl_scan(sr, lds, baseR);
-        (void)block_excl_scan(sp, lds, baseP);
-    } else {
-        baseR = A.blockRT[blockIdx.x];
-        baseP = A.blockPairs[blockIdx.x];
-    }
-    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
-    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
-    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
-}
-
+// LDS writes of a wavefront made visible to its other lanes.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-
-// LDS writes of a wavefront made visible to its other lanes.
 
 // Lane groups are the DPP unit's rows (lanes 16g .. 16g+15 of the
 // wavefront): the scans below move values within a row by DPP, in the VALU,
@@ -1538,13 +1415,12 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v) {
 // sums of the survivors.  PHASE 0 stores the counts and adds them to the
 // block sums; PHASE 1 writes from the slots k_scan_local handed over, the
 // pairs of the whole wavefront's fan triangles together.
+// One step of a wavefront (every lane must call it): listed items i0 ..
+// i0+3 below n, one per lane group, in the wavefront's LDS buffers mem.
 template <int PHASE, bool LIT, int VF>
-__global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
-    __shared__ CVtx<VF> mem[4][2][12];
-    const int lane = threadIdx.x, grp = lane >> 4, gl = lane & 15;
-    const uint32_t n = min(A.clipKnown != ~0u ? A.clipKnown : *A.clipCount, A.clipCap);
-    // Wavefront-uniform loop: the pair writing below needs every lane.
-    for (uint32_t i0 = blockIdx.x * 4; i0 < n; i0 += gridDim.x * 4) {
+__device__ __forceinline__ void clip_group_items(const SetupArgs& A, CVtx<VF> (*mem)[2][12], uint32_t i0, uint32_t n) {
+    const int lane = threadIdx.x & 63, grp = lane >> 4, gl = lane & 15;
+    {
         const uint32_t item = i0 + (uint32_t)grp;
         const bool valid = item < n;
         // The listed triangle with its recorded index, info and frame.  An
@@ -1667,6 +1543,152 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
         }
     }
 }
+
+template <int PHASE, bool LIT, int VF>
+__global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
+    __shared__ CVtx<VF> mem[4][2][12];
+    const uint32_t n = min(A.clipKnown != ~0u ? A.clipKnown : *A.clipCount, A.clipCap);
+    // Wavefront-uniform loop: the pair writing needs every lane.
+    for (uint32_t i0 = blockIdx.x * 4; i0 < n; i0 += gridDim.x * 4) clip_group_items<PHASE, LIT, VF>(A, mem, i0, n);
+}
+
+// One thread per recorded triangle of the pass.  PHASE 0 counts and sums
+// per block; PHASE 1 turns the block sums (scanned in between) and a
+// block scan of the counts into output slots and writes.  Triangles that
+// need clipping are only listed here (PHASE 0) and handed their slots
+// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
+// CLIPW (PHASE 1 of unlit passes without float records): the first
+// A.clipBlocks workgroups clip the listed triangles instead, 16 per
+// workgroup and step (clip_group_items), beside the others' setup: one
+// launch, and the clip waves, dispatched first, overlap it.
+template <int PHASE, bool LIT, bool EXTRA = true, bool CLIPW = false>
+__global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    if constexpr (CLIPW) {
+        if (blockIdx.x < A.clipBlocks) {
+            __shared__ CVtx<VF_SLIM> cmem[4][4][2][12];
+            const uint32_t n = min(A.clipKnown, A.clipCap);
+            const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            for (uint32_t i0 = (blockIdx.x * 4 + wave) * 4; i0 < n; i0 += A.clipBlocks * 16)
+                clip_group_items<1, false, VF_SLIM>(A, cmem[wave], i0, n);
+            return;
+        }
+    }
+    const uint32_t gid = (blockIdx.x - (CLIPW ? A.clipBlocks : 0u)) * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    if (PHASE == 0) {
+        // Reset the tile ranges of this pass (the raster reads [begin, end)).
+        for (uint32_t i = gid; i < A.numTiles; i += gridDim.x * blockDim.x) { A.tileBegin[i] = 0; A.tileEnd[i] = 0; }
+        // pfClear copies pixel 0 into the pixels past the last multiple of 8
+        // before any later draw (context.c:707-710).  The raster kernel of
+        // this pass may already be writing pixel 0 while other tiles clear
+        // their tail, so the tail takes pixel 0 as it was when the pass
+        // began: every earlier pass has finished by the time this runs.
+        if (blockIdx.x == 0)
+            for (uint32_t s = threadIdx.x; s < A.numSegs; s += blockDim.x) {
+                const DSeg sg = A.segs[s];
+                if (!sg.clear) continue;
+                const DFrame F = A.frames[sg.frame];
+                A.clearSnap[8 * sg.frame] = F.kind < PFF_TARGET_FIRST_WIDE ? tgt_load(F.color, F.kind, 0) : 0u;
+                A.clearSnap[8 * sg.frame + 1] = __float_as_uint(F.depth[0]);
+                uint8_t* snap = (uint8_t*)&A.clearSnap[8 * sg.frame + 2];
+                if (F.kind >= PFF_TARGET_FIRST_WIDE)
+                    for (uint32_t b = 0; b < F.bpp; b++) snap[b] = F.color[b];
+            }
+    }
+    // The triangle, its frame and positions are requested first, so their
+    // latency overlaps the block scans of PHASE 1.
+    uint32_t t = 0, info = 0, si = 0, frame = 0;
+    DFrame F;
+    float4 P[3] = { make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f) };
+    if (valid) {
+        pass_triangle(A, gid, t, info, si, F, frame);
+        for (int k = 0; k < 3; k++) P[k] = A.pos[(size_t)t * 3 + k];
+    }
+    // PHASE 1: the slots of this triangle's records and pairs (k_scan_local
+    // turned the counts into offsets).
+    const uint32_t rtBase = (PHASE == 1 && valid) ? A.cntRT[gid] : 0u;
+    const uint32_t pairBase = (PHASE == 1 && valid) ? A.cntPairs[gid] : 0u;
+    uint32_t nRT = 0, nPairs = 0;
+    // One iteration per distinct state in the wavefront: the state is then
+    // uniform and read with scalar loads.
+    uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+    bool deferred = false;
+    LaneRects lr{ { 0u, 0u }, { 0u, 0u } };
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t s0 = __builtin_amdgcn_readlane(si, leader);
+        const bool mine = valid && si == s0;
+        todo &= ~__builtin_amdgcn_ballot_w64(mine);
+        if (mine)
+            deferred = setup_triangle<PHASE, LIT, false, VF_FULL, EXTRA>(A, as_const(A.states + s0), s0, F, t, info,
+                                                                         P, rtBase, pairBase, nRT, nPairs, nullptr,
+                                                                         nullptr, nullptr, &lr);
+    }
+    if (PHASE == 1) {
+        // The tile pairs: each lane its own when all are few, else the
+        // wavefront together, 64 per step (a lane looping over a large
+        // triangle's tiles would hold the others: 800x600 frame of 10k
+        // large triangles, 510 pairs each).
+        const uint32_t na = (lr.a.wh & 0xFFFFu) * (lr.a.wh >> 16), nb = (lr.b.wh & 0xFFFFu) * (lr.b.wh >> 16);
+        if (__builtin_amdgcn_ballot_w64(na + nb > 16u)) {
+            write_pair_rects(A, pairBase, lr.a.tile0, lr.a.wh & 0xFFFFu, lr.a.wh >> 16, F.tilesX, rtBase, lr.a.q);
+            write_pair_rects(A, pairBase + na, lr.b.tile0, lr.b.wh & 0xFFFFu, lr.b.wh >> 16, F.tilesX, rtBase + 1u,
+                             lr.b.q);
+        } else {
+            write_pairs_lane(A, pairBase, lr.a, F.tilesX, rtBase);
+            write_pairs_lane(A, pairBase + na, lr.b, F.tilesX, rtBase + 1u);
+        }
+    }
+    if (PHASE == 0) {
+        // List the triangles to clip: one counter update per wave.
+        const int lane = threadIdx.x & 63;
+        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
+        if (D) {
+            uint32_t base = 0;
+            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
+            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
+            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            // The list holds every triangle of the pass once the counter
+            // starts at zero; the bound keeps a counter left over by a failed
+            // run from writing past it.
+            const uint32_t at = base + __popcll(D & ltMask);
+            if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
+        }
+        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
+        uint32_t totR, totP;
+        block_excl_scan(nRT, lds, totR);
+        block_excl_scan(nPairs, lds, totP);
+        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
+    }
+}
+
+// Per triangle of the pass, the first raster-triangle and pair slot: the
+// block's scanned sums plus a scan within the block, in place over the
+// counts.  Both k_setup<1> and the clip kernel then find their slots there, so
+// the two run side by side.
+// FUSED: the block sums are still raw (k_scan_top did not run); each block
+// adds up those of the blocks before it itself (few blocks).
+template <bool FUSED>
+__global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
+    __shared__ uint32_t lds[64];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = gid < A.numPassTris;
+    uint32_t tot, baseR, baseP;
+    if (FUSED) {
+        uint32_t sr = 0, sp = 0;
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
+        (void)block_excl_scan(sr, lds, baseR);
+        (void)block_excl_scan(sp, lds, baseP);
+    } else {
+        baseR = A.blockRT[blockIdx.x];
+        baseP = A.blockPairs[blockIdx.x];
+    }
+    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
+    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
+    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
+}
+
 
 // The tile pairs of passes whose raster triangles average many pairs, from
 // the rectangles the setup recorded: pair i belongs to the last raster
@@ -4401,6 +4423,7 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         sa.tileShift = J->tileShift;
         sa.numTiles = J->numTiles;
     }
+    sa.clipKnown = P.numClip;                            // counted by now
     const uint32_t numRT = P.numRT, numPairs = P.numPairs;
     if ((rc = ensure(J, J->rts, (size_t)numRT * sizeof(RTri) + sizeof(RTri)))) return rc;
     if (J->anyPhong && (rc = ensure(J, J->rtl, (size_t)numRT * sizeof(RTriL) + sizeof(RTriL)))) return rc;
@@ -4434,9 +4457,10 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         // Slots per triangle, then the clipped triangles and the others set
         // up (both read the slots only).  Many clipped triangles (one lane
         // each) are clipped on the side stream beside k_setup<1>; a few (lane
-        // groups, a few microseconds) on the main stream before it, where the
-        // fork and join would cost more than they overlap (1080p frame:
-        // setup 65 -> 53 us); none, not at all.
+        // groups) by k_setup<1>'s own leading workgroups in unlit passes
+        // without float records, else by a kernel on the main stream before
+        // it, where the fork and join would cost more than they overlap
+        // (1080p frame: setup 65 -> 53 us); none, not at all.
         if (fusedScan) hipLaunchKernelGGL(k_scan_local<true>, dim3(nb), dim3(256), 0, J->stream, sa);
         else hipLaunchKernelGGL(k_scan_local<false>, dim3(nb), dim3(256), 0, J->stream, sa);
         const bool useSide = J->side && P.numClip > 16384u;
@@ -4448,6 +4472,11 @@ int run_pass(PFDjob* J, PassInfo& P, bool discover, hipEvent_t* ev) {
         if (J->anyLit) {
             if (P.numClip) launch_clip<1, true, VF_FULL>(sa, P.numClip, clipStream);
             hipLaunchKernelGGL((k_setup<1, true>), dim3(nb), dim3(256), 0, J->stream, sa);
+        } else if (!J->diff && P.numClip && P.numClip <= 16384u) {
+            // A few clipped triangles: clipped by k_setup<1>'s leading
+            // workgroups (1080p frame, 2,468 clipped: setup 50 -> 44 us).
+            sa.clipBlocks = (P.numClip + 15) / 16;
+            hipLaunchKernelGGL((k_setup<1, false, false, true>), dim3(nb + sa.clipBlocks), dim3(256), 0, J->stream, sa);
         } else {
             if (P.numClip && J->diff) launch_clip<1, false, VF_FULL>(sa, P.numClip, clipStream);
             else if (P.numClip) launch_clip<1, false, VF_SLIM>(sa, P.numClip, clipStream);
