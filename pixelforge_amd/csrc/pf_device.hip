@@ -1641,25 +1641,26 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
         }
     }
     if (PHASE == 0) {
-        // List the triangles to clip: one counter update per wave.
-        const int lane = threadIdx.x & 63;
-        const uint64_t D = __builtin_amdgcn_ballot_w64(deferred);
-        if (D) {
-            uint32_t base = 0;
-            if (lane == __builtin_ctzll(D)) base = atomicAdd(A.clipCount, (uint32_t)__popcll(D));
-            base = __builtin_amdgcn_readlane(base, __builtin_ctzll(D));
-            const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
+        uint32_t totR, totP, totD;
+        block_excl_scan(nRT, lds, totR);
+        block_excl_scan(nPairs, lds, totP);
+        // List the triangles to clip, in order within the block: one counter
+        // update per block (one per wavefront contended on the counter).
+        const uint32_t dAt = block_excl_scan(deferred ? 1u : 0u, lds, totD);
+        if (threadIdx.x == 0) {
+            A.blockRT[blockIdx.x] = totR;
+            A.blockPairs[blockIdx.x] = totP;
+            if (totD) lds[33] = atomicAdd(A.clipCount, totD);
+        }
+        if (totD) {                                            // uniform over the block
+            __syncthreads();
             // The list holds every triangle of the pass once the counter
             // starts at zero; the bound keeps a counter left over by a failed
             // run from writing past it.
-            const uint32_t at = base + __popcll(D & ltMask);
+            const uint32_t at = lds[33] + dAt;
             if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
         }
-        if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
-        uint32_t totR, totP;
-        block_excl_scan(nRT, lds, totR);
-        block_excl_scan(nPairs, lds, totP);
-        if (threadIdx.x == 0) { A.blockRT[blockIdx.x] = totR; A.blockPairs[blockIdx.x] = totP; }
     }
 }
 
