@@ -1556,7 +1556,8 @@ __global__ __launch_bounds__(64) void k_clip_group(SetupArgs A) {
 // per block; PHASE 1 turns the block sums (scanned in between) and a
 // block scan of the counts into output slots and writes.  Triangles that
 // need clipping are only listed here (PHASE 0) and handed their slots
-// (PHASE 1, through cntRT/cntPairs); a clip kernel does their work.
+// (PHASE 1, through cntRT/cntPairs); a clip kernel, or with CLIPW this
+// kernel's leading workgroups, does their work.
 // CLIPW (PHASE 1 of unlit passes without float records): the first
 // A.clipBlocks workgroups clip the listed triangles instead, 16 per
 // workgroup and step (clip_group_items), beside the others' setup: one
@@ -1880,8 +1881,9 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
 }
 
 // Also clears the clip-list counter for the next pass (both clip kernels of
-// this pass have read it by now).  Keys are tiles of the pass; the bound
-// only keeps a key that is not from writing outside the ranges.
+// this pass have read it by now).  Every key is a tile of the pass; one that
+// is not (none, while the counts are right) is skipped rather than written
+// past the ranges.
 __global__ void k_tile_bounds(const uint32_t* __restrict__ keys, uint32_t n, uint32_t numTiles,
                               uint32_t* __restrict__ tileBegin, uint32_t* __restrict__ tileEnd,
                               uint32_t* __restrict__ clipCount) {
