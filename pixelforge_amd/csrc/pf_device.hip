@@ -1233,6 +1233,38 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
     return r;
 }
 
+// Exclusive block scans of N values per thread at once (one pass of
+// barriers for all of them); total[i] is the block total of value i.
+// Blocks of at most 16 wavefronts; lds holds at least 16 N + N words.
+template <int N>
+__device__ __forceinline__ void block_excl_scan_n(const uint32_t (&v)[N], uint32_t* lds, uint32_t (&r)[N],
+                                                  uint32_t (&total)[N]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) inc[i] = wave_incl_scan(v[i]);
+    if (lane == 63)
+#pragma unroll
+        for (int i = 0; i < N; i++) lds[16 * i + wave] = inc[i];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const uint32_t w = lane < nw ? lds[16 * i + lane] : 0u;
+            const uint32_t wi = wave_incl_scan(w);
+            if (lane < nw) lds[16 * i + lane] = wi - w;
+            if (lane == nw - 1) lds[16 * N + i] = wi;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        r[i] = inc[i] - v[i] + lds[16 * i + wave];
+        total[i] = lds[16 * N + i];
+    }
+    __syncthreads();
+}
+
 // Recorded triangle, state and frame of pass-triangle gid: the last
 // segment whose passTriBase <= gid, found forward from its block's first.
 __device__ __forceinline__ void pass_triangle(const SetupArgs& A, uint32_t gid, uint32_t& t, uint32_t& info,
@@ -1643,23 +1675,24 @@ __global__ __launch_bounds__(256) void k_setup(SetupArgs A) {
     }
     if (PHASE == 0) {
         if (valid) { A.cntRT[gid] = nRT; A.cntPairs[gid] = nPairs; }
-        uint32_t totR, totP, totD;
-        block_excl_scan(nRT, lds, totR);
-        block_excl_scan(nPairs, lds, totP);
-        // List the triangles to clip, in order within the block: one counter
-        // update per block (one per wavefront contended on the counter).
-        const uint32_t dAt = block_excl_scan(deferred ? 1u : 0u, lds, totD);
+        // Block sums, and the triangles to clip listed in order within the
+        // block: one counter update per block (one per wavefront contended
+        // on the counter).
+        const uint32_t sv[3] = { nRT, nPairs, deferred ? 1u : 0u };
+        uint32_t ex[3], tot[3];
+        block_excl_scan_n<3>(sv, lds, ex, tot);
+        const uint32_t totD = tot[2], dAt = ex[2];
         if (threadIdx.x == 0) {
-            A.blockRT[blockIdx.x] = totR;
-            A.blockPairs[blockIdx.x] = totP;
-            if (totD) lds[33] = atomicAdd(A.clipCount, totD);
+            A.blockRT[blockIdx.x] = tot[0];
+            A.blockPairs[blockIdx.x] = tot[1];
+            if (totD) lds[63] = atomicAdd(A.clipCount, totD);
         }
         if (totD) {                                            // uniform over the block
             __syncthreads();
             // The list holds every triangle of the pass once the counter
             // starts at zero; the bound keeps a counter left over by a failed
             // run from writing past it.
-            const uint32_t at = lds[33] + dAt;
+            const uint32_t at = lds[63] + dAt;
             if (deferred && at < A.clipCap) A.clipList[at] = make_uint4(gid, t, info, frame);
         }
     }
@@ -1676,19 +1709,20 @@ __global__ __launch_bounds__(256) void k_scan_local(SetupArgs A) {
     __shared__ uint32_t lds[64];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = gid < A.numPassTris;
-    uint32_t tot, baseR, baseP;
+    uint32_t baseR, baseP, ex[2], tot[2];
     if (FUSED) {
-        uint32_t sr = 0, sp = 0;
-        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sr += A.blockRT[b]; sp += A.blockPairs[b]; }
-        (void)block_excl_scan(sr, lds, baseR);
-        (void)block_excl_scan(sp, lds, baseP);
+        uint32_t sb[2] = { 0u, 0u };
+        for (uint32_t b = threadIdx.x; b < blockIdx.x; b += 256) { sb[0] += A.blockRT[b]; sb[1] += A.blockPairs[b]; }
+        block_excl_scan_n<2>(sb, lds, ex, tot);
+        baseR = tot[0];
+        baseP = tot[1];
     } else {
         baseR = A.blockRT[blockIdx.x];
         baseP = A.blockPairs[blockIdx.x];
     }
-    const uint32_t r = block_excl_scan(valid ? A.cntRT[gid] : 0u, lds, tot) + baseR;
-    const uint32_t p = block_excl_scan(valid ? A.cntPairs[gid] : 0u, lds, tot) + baseP;
-    if (valid) { A.cntRT[gid] = r; A.cntPairs[gid] = p; }
+    const uint32_t sc[2] = { valid ? A.cntRT[gid] : 0u, valid ? A.cntPairs[gid] : 0u };
+    block_excl_scan_n<2>(sc, lds, ex, tot);
+    if (valid) { A.cntRT[gid] = ex[0] + baseR; A.cntPairs[gid] = ex[1] + baseP; }
 }
 
 
