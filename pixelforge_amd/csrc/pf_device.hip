@@ -2189,6 +2189,12 @@ __device__ __forceinline__ void flush_queue(const RasterArgs& A, WaveQueueT<PHON
     wave_sync();
 }
 
+// Set bits of the wavefront mask m below the calling lane (v_mbcnt; the
+// same as popcount(m & lanes-below-mask)).
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Pixels x0..x1 of rows y0..y1 of an 8x8 block as a 64-bit mask, bit
 // 8 y + x (the lane of pixel (x, y) when a wave holds one pixel per lane).
 __device__ __forceinline__ uint64_t rect_mask8(int x0, int x1, int y0, int y1) {
@@ -2245,7 +2251,6 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
             }
         }
     }
-    const uint64_t ltMask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // Last column and row of the block inside the target.
     const int xLim = min(bx0 + TW - 1, (int)F.W - 1), yLim = min(by0 + TW - 1, (int)F.H - 1);
     uint32_t frags = 0;
@@ -2262,7 +2267,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 const bool mine = has && ((v >> (PAIR_QUAD_SH + (uint32_t)qbit)) & 1u);
                 const uint64_t M = __builtin_amdgcn_ballot_w64(mine);
                 const uint32_t room = (uint32_t)BATCH - nb, found = (uint32_t)__popcll(M);
-                const uint32_t rank = (uint32_t)__popcll(M & ltMask);
+                const uint32_t rank = (uint32_t)lane_rank(M);
                 if (mine && rank < room) Q.recId[nb + rank] = v & PAIR_RT_MASK;
                 if (found > room) {
                     // The batch fills here: continue after its last entry.
@@ -2326,7 +2331,7 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
         // record allows them (decided before the loop: no per-record branch).
         const bool allSmall = __builtin_amdgcn_ballot_w64(ov && !(q5.x & META_SMALL)) == 0ull;
         if (ov) {
-            const int slot = (int)__popcll(OV & ltMask);
+            const int slot = lane_rank(OV);
             const uint4 q[8] = { q0, q1, q2, q3, q4, q5, q6, q7 };
             stage_rec<ROT>(Q, slot, q);
             Q.recId[slot] = r;
@@ -2371,8 +2376,12 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                 }
                 const bool inBox = (K == 1) ? (((rmj >> lane) & 1ull) != 0)
                                             : (in[k] & (px < xMax) & (exact | ((px >= xMin) & (py[k] >= yMin) & (py[k] <= yMax))));
-                const bool cov = inBox & ((w1 | w2 | w3) > 0);
-                const uint64_t M = __builtin_amdgcn_ballot_w64(cov);
+                const bool inEdges = (w1 | w2 | w3) > 0;
+                const bool cov = inBox & inEdges;
+                // The two ballots, each of a comparison, and their AND in
+                // scalar registers: a ballot of the combined flag rebuilds
+                // its mask through two vector instructions.
+                const uint64_t M = __builtin_amdgcn_ballot_w64(inEdges) & __builtin_amdgcn_ballot_w64(inBox);
                 if (!M) continue;
                 const int c = __popcll(M);
                 frags += (uint32_t)c;
@@ -2386,15 +2395,22 @@ __device__ __forceinline__ uint32_t raster_block(const RasterArgs& A, const DFra
                     }
                     continue;
                 }
-                if (nf + c > 64) {
-                    flush_queue<PHONG, K, ROT, TK>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
-                    nf = 0;
-                }
-                if (cov) {
-                    const int slot = nf + (int)__popcll(M & ltMask);
+                // Queued after the fragments before them; a full queue is
+                // flushed first.  That rare branch takes the lane's coverage
+                // from the ballot, so `cov` need not be kept across the flush
+                // (which cost instructions on every record).
+                const auto enqueue = [&](int base) {
+                    const int slot = base + lane_rank(M);
                     Q.ftri[slot] = (uint8_t)j;
                     Q.fpix[slot] = (uint8_t)(lane + 64 * k);
                     pm[k] |= 1ull << slot;
+                };
+                if (nf + c > 64) {
+                    flush_queue<PHONG, K, ROT, TK>(A, Q, bx0, by0, lane, nf, pm, F.kind, col, dep, win);
+                    nf = 0;
+                    if ((M >> lane) & 1ull) enqueue(0);
+                } else if (cov) {
+                    enqueue(nf);
                 }
                 nf += c;
             }
