@@ -2057,15 +2057,17 @@ __device__ __forceinline__ void edge_values(const TRI& t, int px, int py, int32_
 // (triangles.c:422-432, 500-521).
 // The depth test of a fragment at depth z against the pixel's depth
 // (depth.h:80-124: NOTEQUAL behaves as EQUAL); true when there is none.
+// Branch-free: the outcome of the comparison as one bit (1 less, 2 equal,
+// 4 greater; none when either value is NaN, where every comparison is
+// false) tested against the outcomes each function passes, three bits per
+// function: 0 and 1 equal, 2 less, 3 less or equal, 4 greater, 5-7
+// greater or equal, and entry 8 (no depth test) all.
 __device__ __forceinline__ bool depth_ok(uint32_t meta, float z, float dep) {
-    if (!(meta & META_DEPTH)) return true;
-    switch ((meta >> META_DFUNC_SH) & 7u) {
-    case 0: case 1: return z == dep;
-    case 2: return z < dep;
-    case 3: return z <= dep;
-    case 4: return z > dep;
-    default: return z >= dep;
-    }
+    constexpr uint32_t PASS = (2u << 0) | (2u << 3) | (1u << 6) | (3u << 9) | (4u << 12) | (6u << 15) | (6u << 18) |
+                              (6u << 21) | (7u << 24);
+    const uint32_t f = (meta & META_DEPTH) ? ((meta >> META_DFUNC_SH) & 7u) : 8u;
+    const uint32_t o = (z < dep ? 1u : 0u) | (z == dep ? 2u : 0u) | (z > dep ? 4u : 0u);
+    return ((PASS >> (3u * f)) & o) != 0u || f == 8u;
 }
 
 // EARLY: the depth test against `dep` first; a fragment that fails it is
