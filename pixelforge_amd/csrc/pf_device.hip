@@ -371,25 +371,28 @@ __device__ __forceinline__ uint32_t bary_flat(uint32_t c1, uint32_t c2, uint32_t
     return ((m == b1) ? c1 : 0u) | (((m == b2) ? c2 : 0u) | ((m == b3) ? c3 : 0u));
 }
 
-__device__ __forceinline__ uint32_t blend_op(uint32_t mode, uint32_t s, uint32_t d) {  // blend.h:137-274
+// The mode is chosen once per fragment, outside the channel loop (inside
+// it, it was a branch tree per channel), alpha blending first.
+template <typename F>
+__device__ __forceinline__ uint32_t blend_channels(uint32_t s, uint32_t d, F f) {
     uint32_t p = 0;
-    int sa = ch(s, 3) + 1;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        int sc = ch(s, i), dc = ch(d, i), o;
-        switch (mode) {
-        case 0: o = (sc + dc) >> 1; break;
-        case 1: o = ((i == 3 ? 255 : sc) * sa + dc * (256 - sa)) >> 8; break;
-        case 2: o = min(sc + dc, 255); break;
-        case 3: o = max(sc + dc, 0); break;
-        case 4: o = (sc * dc) >> 8; break;
-        case 5: o = min(((dc * (255 - sc)) >> 8) + sc, 255); break;
-        case 6: o = max(sc, dc); break;
-        default: o = min(sc, dc); break;
-        }
-        p |= (uint32_t)o << (8 * i);
-    }
+    for (int i = 0; i < 4; i++) p |= (uint32_t)f(i, ch(s, i), ch(d, i)) << (8 * i);
     return p;
+}
+__device__ __forceinline__ uint32_t blend_op(uint32_t mode, uint32_t s, uint32_t d) {  // blend.h:137-274
+    const int sa = ch(s, 3) + 1;
+    if (mode == 1)
+        return blend_channels(s, d, [&](int i, int sc, int dc) { return ((i == 3 ? 255 : sc) * sa + dc * (256 - sa)) >> 8; });
+    switch (mode) {
+    case 0: return blend_channels(s, d, [](int, int sc, int dc) { return (sc + dc) >> 1; });
+    case 2: return blend_channels(s, d, [](int, int sc, int dc) { return min(sc + dc, 255); });
+    case 3: return blend_channels(s, d, [](int, int sc, int dc) { return max(sc + dc, 0); });
+    case 4: return blend_channels(s, d, [](int, int sc, int dc) { return (sc * dc) >> 8; });
+    case 5: return blend_channels(s, d, [](int, int sc, int dc) { return min(((dc * (255 - sc)) >> 8) + sc, 255); });
+    case 6: return blend_channels(s, d, [](int, int sc, int dc) { return max(sc, dc); });
+    default: return blend_channels(s, d, [](int, int sc, int dc) { return min(sc, dc); });
+    }
 }
 
 __device__ __forceinline__ uint32_t modulate(uint32_t t, uint32_t f) {       // blend.h:199-212
