@@ -36,6 +36,7 @@
 #include <math.h>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include <mutex>
 #include <atomic>
 #include <condition_variable>
@@ -406,24 +407,33 @@ __device__ __forceinline__ uint32_t modulate(uint32_t t, uint32_t f) {       // 
 // Texture sampling (sampler.h:202-410; pixel.h:2909-3008)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t texel_at(const DTex& t, int32_t off) {
+// Texel at offset off (clamped into the texture) of an RGBA8 (RGBA8 true)
+// or RGBA float texture, as RGBA8.
+template <bool RGBA8>
+__device__ __forceinline__ uint32_t texel_at_f(const DTex& t, int32_t off) {
     int64_t n = (int64_t)t.w * t.h;
     int64_t o = off;
     o = o < 0 ? 0 : (o >= n ? n - 1 : o);
-    if (t.format == PFF_TEX_RGBA8) return ((const uint32_t*)t.pixels)[o];
+    if (RGBA8) return ((const uint32_t*)t.pixels)[o];
     const float4 p = ((const float4*)t.pixels)[o];
     int32_t r = cvt_rne_i32(p.x * 255.0f), g = cvt_rne_i32(p.y * 255.0f);
     int32_t b = cvt_rne_i32(p.z * 255.0f), a = cvt_rne_i32(p.w * 255.0f);
     return ((uint32_t)a << 24) | ((uint32_t)b << 16) | ((uint32_t)g << 8) | (uint32_t)r;
 }
+__device__ __forceinline__ uint32_t texel_at(const DTex& t, int32_t off) {
+    return t.format == PFF_TEX_RGBA8 ? texel_at_f<true>(t, off) : texel_at_f<false>(t, off);
+}
 
-__device__ __forceinline__ int32_t wrap_coord(uint32_t wrap, float u, uint32_t size) {
+// Texel coordinate of u in [0, size) for wrap mode WRAP (0 repeat, 1
+// mirrored repeat, else clamp).
+template <int WRAP>
+__device__ __forceinline__ int32_t wrap_coord_m(float u, uint32_t size) {
     float tw = (float)(size - 1);
-    if (wrap == 0) {
+    if (WRAP == 0) {
         int32_t x = cvt_rne_i32((u - truncf(u)) * tw);
         return x < 0 ? (int32_t)(0u - (uint32_t)x) : x;
     }
-    if (wrap == 1) {
+    if (WRAP == 1) {
         float a = fabsf(u);
         float m = a - floorf(__fdiv_rn(a, 2.0f)) * 2.0f;
         float mirror = 1.0f - (m - 1.0f);
@@ -431,6 +441,9 @@ __device__ __forceinline__ int32_t wrap_coord(uint32_t wrap, float u, uint32_t s
         return cvt_rne_i32(m * tw + 0.5f);
     }
     return cvt_rne_inrange(clamp_ps(u, 0.0f, 1.0f) * tw + 0.5f);
+}
+__device__ __forceinline__ int32_t wrap_coord(uint32_t wrap, float u, uint32_t size) {
+    return wrap == 0 ? wrap_coord_m<0>(u, size) : wrap == 1 ? wrap_coord_m<1>(u, size) : wrap_coord_m<2>(u, size);
 }
 
 __device__ __forceinline__ uint32_t lerp_color(uint32_t a, uint32_t b, float t) {   // color.h:137-144
@@ -461,21 +474,38 @@ __device__ __forceinline__ float rcp_int_rn(uint32_t w) {
 __device__ uint32_t sample_tex(const DTex& t, float u, float v) {
     uint32_t w = t.w;
     if (t.filter == 0) {
-        int32_t x = wrap_coord(t.wrap, u, t.w);
-        int32_t y = wrap_coord(t.wrap, v, t.h);
+        int32_t x, y;
+        if (t.wrap == 0) { x = wrap_coord_m<0>(u, t.w); y = wrap_coord_m<0>(v, t.h); }
+        else if (t.wrap == 1) { x = wrap_coord_m<1>(u, t.w); y = wrap_coord_m<1>(v, t.h); }
+        else { x = wrap_coord_m<2>(u, t.w); y = wrap_coord_m<2>(v, t.h); }
         return texel_at(t, (int32_t)((uint32_t)y * w + (uint32_t)x));
     }
     float tx = rcp_int_rn(t.w), ty = rcp_int_rn(t.h);   // texture sides are at most 65535
-    int32_t x0 = wrap_coord(t.wrap, u, t.w);
-    int32_t y0 = wrap_coord(t.wrap, v, t.h);
-    int32_t x1 = wrap_coord(t.wrap, u + tx, t.w);
-    int32_t y1 = wrap_coord(t.wrap, v + ty, t.h);
+    // The wrap mode and the texel format are chosen once for the four
+    // taps (per tap they were a branch tree each).
+    int32_t x0, y0, x1, y1;
+    const auto coords = [&](auto m) {
+        constexpr int M = decltype(m)::value;
+        x0 = wrap_coord_m<M>(u, t.w);
+        y0 = wrap_coord_m<M>(v, t.h);
+        x1 = wrap_coord_m<M>(u + tx, t.w);
+        y1 = wrap_coord_m<M>(v + ty, t.h);
+    };
+    if (t.wrap == 0) coords(std::integral_constant<int, 0>());
+    else if (t.wrap == 1) coords(std::integral_constant<int, 1>());
+    else coords(std::integral_constant<int, 2>());
     float fx = clamp_ps(u * (float)t.w - (float)x0, 0.0f, 1.0f);
     float fy = clamp_ps(v * (float)t.h - (float)y0, 0.0f, 1.0f);
-    uint32_t c00 = texel_at(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x0));
-    uint32_t c10 = texel_at(t, (int32_t)((uint32_t)y0 * w + (uint32_t)x1));
-    uint32_t c01 = texel_at(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x0));
-    uint32_t c11 = texel_at(t, (int32_t)((uint32_t)y1 * w + (uint32_t)x1));
+    const int32_t o00 = (int32_t)((uint32_t)y0 * w + (uint32_t)x0), o10 = (int32_t)((uint32_t)y0 * w + (uint32_t)x1);
+    const int32_t o01 = (int32_t)((uint32_t)y1 * w + (uint32_t)x0), o11 = (int32_t)((uint32_t)y1 * w + (uint32_t)x1);
+    uint32_t c00, c10, c01, c11;
+    if (t.format == PFF_TEX_RGBA8) {
+        c00 = texel_at_f<true>(t, o00); c10 = texel_at_f<true>(t, o10);
+        c01 = texel_at_f<true>(t, o01); c11 = texel_at_f<true>(t, o11);
+    } else {
+        c00 = texel_at_f<false>(t, o00); c10 = texel_at_f<false>(t, o10);
+        c01 = texel_at_f<false>(t, o01); c11 = texel_at_f<false>(t, o11);
+    }
     uint32_t c0 = lerp_color(c00, c10, fx);
     uint32_t c1 = lerp_color(c01, c11, fx);
     return lerp_color(c0, c1, fy);
