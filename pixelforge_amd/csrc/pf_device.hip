@@ -252,22 +252,28 @@ __device__ __forceinline__ float sqrt_cr(float x) { return __fsqrt_rn(x); }
 
 // _mm256_rcp_ps / _mm256_rsqrt_ps of the reference build (simd.h:1217-1245):
 // table approximations measured on Intel Xeon (include/pf_x86_approx.h).
+// Branch-free: the table result is formed for every input (its index is
+// always in range) and the special cases replace it by selects, in
+// reverse order of precedence (NaN / infinity, then zero and denormals,
+// then the rest).
 __device__ __forceinline__ float rcp_x86(float xf) {
     uint32_t x = __float_as_uint(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
-    if (E == 0xffu) return __uint_as_float(m ? (x | 0x00400000u) : s);
-    if (E == 0) return __uint_as_float(s | 0x7f800000u);
-    int oe = 253 - (int)E;
-    if (oe <= 0) return __uint_as_float(s);
-    return __uint_as_float(s | ((uint32_t)oe << 23) | ((uint32_t)pf_rcp_table[m >> 12] << 11));
+    const int oe = 253 - (int)E;
+    uint32_t r = s | ((uint32_t)oe << 23) | ((uint32_t)pf_rcp_table[m >> 12] << 11);
+    r = oe <= 0 ? s : r;                                        // results below the normal range
+    r = E == 0 ? (s | 0x7f800000u) : r;
+    r = E == 0xffu ? (m ? (x | 0x00400000u) : s) : r;
+    return __uint_as_float(r);
 }
 __device__ __forceinline__ float rsqrt_x86(float xf) {
     uint32_t x = __float_as_uint(xf), s = x & 0x80000000u, E = (x >> 23) & 0xffu, m = x & 0x7fffffu;
-    if (E == 0xffu) return __uint_as_float(m ? (x | 0x00400000u) : (s ? 0xffc00000u : 0u));
-    if (E == 0) return __uint_as_float(s | 0x7f800000u);
-    if (s) return __uint_as_float(0xffc00000u);
-    int odd = (int)(E & 1u);
-    int oe = odd ? 126 - ((int)E - 127) / 2 : 127 - ((int)E - 126) / 2;
-    return __uint_as_float(((uint32_t)oe << 23) | ((uint32_t)pf_rsqrt_table[odd][m >> 13] << 11));
+    const int odd = (int)(E & 1u);
+    const int oe = odd ? 126 - ((int)E - 127) / 2 : 127 - ((int)E - 126) / 2;
+    uint32_t r = ((uint32_t)oe << 23) | ((uint32_t)pf_rsqrt_table[odd][m >> 13] << 11);
+    r = s ? 0xffc00000u : r;
+    r = E == 0 ? (s | 0x7f800000u) : r;
+    r = E == 0xffu ? (m ? (x | 0x00400000u) : (s ? 0xffc00000u : 0u)) : r;
+    return __uint_as_float(r);
 }
 __device__ __forceinline__ int ch(uint32_t c, int i) { return (int)((c >> (8 * i)) & 0xFFu); }
 __device__ __forceinline__ uint8_t min255(int n) { return (uint8_t)(n | ((255 - n) >> 31)); }
